@@ -1,0 +1,166 @@
+/*
+ * parthenon_amd — C ABI of the MI355X-native per-stage meshblock data path.
+ *
+ * Plain pointers, sizes and opaque handles only. Every entry point returns a pth_status
+ * mirroring the reference's TaskStatus (src/basic_types.hpp:59); on PTH_FAIL the reason
+ * is available from pth_last_error(). Device work runs on the stream set with
+ * pth_mesh_set_stream (default: the null stream) and is asynchronous unless stated.
+ *
+ * Reference interfaces each entry replaces are cited as file:line relative to the
+ * reference repository root.
+ */
+#ifndef PARTHENON_AMD_H_
+#define PARTHENON_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* TaskStatus {complete, incomplete, iterate, fail}: src/basic_types.hpp:59 */
+typedef enum {
+  PTH_COMPLETE = 0,
+  PTH_INCOMPLETE = 1,
+  PTH_ITERATE = 2,
+  PTH_FAIL = 3
+} pth_status;
+
+/* BoundaryType: src/basic_types.hpp:156 (any, local, nonlocal, flxcor_send, flxcor_recv,
+ * gmg_same, gmg_restrict_send, gmg_restrict_recv, gmg_prolongate_send, gmg_prolongate_recv).
+ * On one GPU every neighbour is local, so any == local and nonlocal has no buffers. */
+typedef enum {
+  PTH_BT_ANY = 0,
+  PTH_BT_LOCAL = 1,
+  PTH_BT_NONLOCAL = 2,
+  PTH_BT_FLXCOR_SEND = 3,
+  PTH_BT_FLXCOR_RECV = 4,
+  PTH_BT_GMG_SAME = 5,
+  PTH_BT_GMG_RESTRICT_SEND = 6,
+  PTH_BT_GMG_RESTRICT_RECV = 7,
+  PTH_BT_GMG_PROLONGATE_SEND = 8,
+  PTH_BT_GMG_PROLONGATE_RECV = 9
+} pth_boundary_type;
+
+/* Metadata flags used by the path: src/interface/metadata.hpp (FillGhost, Independent,
+ * WithFluxes, OneCopy). Only cell-centred fields are supported. */
+enum {
+  PTH_FILL_GHOST = 1u << 0,
+  PTH_INDEPENDENT = 1u << 1,
+  PTH_WITH_FLUXES = 1u << 2,
+  PTH_ONE_COPY = 1u << 3
+};
+
+/* How SendBoundBufs/SetBounds launch: one persistent table-walking kernel per call (default),
+ * or the plain form with one kernel launch per boundary buffer. */
+typedef enum { PTH_PACK_PERSISTENT = 0, PTH_PACK_PER_BUFFER = 1 } pth_pack_mode;
+
+typedef struct pth_mesh pth_mesh;         /* Mesh: blocks, neighbours, fields, comm buffers */
+typedef struct pth_meshdata pth_meshdata; /* MeshData: one named container over all blocks */
+
+/* Mesh construction. Replaces Mesh::Mesh + static refinement (src/mesh/mesh.cpp:1054-1116)
+ * and Mesh::SetMeshBlockNeighbors (src/mesh/mesh-gmg.cpp:48-94). nx: mesh cells,
+ * bnx: meshblock cells, periodic: per-dimension flag (non-periodic faces have no neighbour). */
+pth_mesh *pth_mesh_create(const int nx[3], const int bnx[3], int nghost, const double xmin[3],
+                          const double xmax[3], const int periodic[3]);
+/* <parthenon/static_refinementN> box {x1min,x1max,x2min,x2max,x3min,x3max} at level. */
+pth_status pth_mesh_refine_box(pth_mesh *m, const double box[6], int level);
+/* Union of refinement boxes, one per level-`level` block intersecting the sphere. */
+pth_status pth_mesh_refine_sphere(pth_mesh *m, const double center[3], double radius, int level);
+/* StateDescriptor::AddField (src/interface/state_descriptor.cpp) for a cell-centred field of
+ * ncomp components. Must precede pth_mesh_finalize. */
+pth_status pth_mesh_add_field(pth_mesh *m, const char *name, int ncomp, unsigned flags);
+/* Builds blocks and neighbour links on the host only (no device work); idempotent.
+ * pth_mesh_finalize calls it first. */
+pth_status pth_mesh_plan(pth_mesh *m);
+/* Builds blocks, neighbours, the device buffer tables and communication buffers. */
+pth_status pth_mesh_finalize(pth_mesh *m);
+void pth_mesh_destroy(pth_mesh *m);
+pth_status pth_mesh_set_stream(pth_mesh *m, void *hip_stream);
+pth_status pth_mesh_synchronize(pth_mesh *m);
+pth_status pth_mesh_set_pack_mode(pth_mesh *m, int mode);
+
+int pth_mesh_nblocks(const pth_mesh *m);
+int pth_mesh_multilevel(const pth_mesh *m);
+/* out: level, lx1, lx2, lx3 ; x: xmin[3], xmax[3] of block gid. */
+pth_status pth_mesh_block_info(const pth_mesh *m, int gid, int64_t out[4], double x[6]);
+/* cellbounds n[3], ng[3] then c_cellbounds n[3], ng[3] (src/mesh/meshblock.cpp:201-218) */
+pth_status pth_mesh_index_shapes(const pth_mesh *m, int out[12]);
+/* Neighbour links of block gid (reference NeighborBlock): up to max records of 8 int64
+ * {gid, level, lx1, lx2, lx3 (as seen from gid, not wrapped), ox1, ox2, ox3}; returns count. */
+int pth_mesh_neighbors(const pth_mesh *m, int gid, int64_t *out, int max);
+/* Index boxes of every boundary region of block gid, one record of 7 ints per neighbour link
+ * and kind {kind, i_lo, i_hi, j_lo, j_hi, k_lo, k_hi}. kind: 0 send, 1 receive, 2 restriction
+ * before send, 3 prolongation after set, 4 restriction after set (kinds 2-4 in coarse index
+ * space). Returns the number of records written (kinds absent for a link are skipped). */
+int pth_mesh_region_boxes(const pth_mesh *m, int gid, int *out, int max);
+/* Boundary table sizes for bound type bt: number of buffers and of buffer elements used. */
+pth_status pth_mesh_exchange_counts(const pth_mesh *m, int bt, int64_t *nbuffers,
+                                    int64_t *nelements);
+
+/* MeshData<Real> lookup/creation by name (src/interface/mesh_data.hpp:191; the
+ * mesh_data.GetOrAdd calls of burgers_driver.cpp:76-79). Storage for non-OneCopy fields is
+ * allocated on first use; OneCopy fields are shared by every container. */
+pth_meshdata *pth_meshdata_get(pth_mesh *m, const char *name);
+
+/* Device pointers of a field: data [nblocks][ncomp][Ek][Ej][Ei]; coarse (multilevel) in the
+ * c_cellbounds shape; flux [3][nblocks][ncomp][Ek][Ej][Ei] (x-face i at index i). */
+pth_status pth_field_pointers(pth_meshdata *md, const char *field, double **data,
+                              double **coarse, double **flux);
+/* Synchronous host<->device copies of a whole field (n = number of doubles). which: 0 data,
+ * 1 coarse, 2 flux. */
+pth_status pth_field_upload(pth_meshdata *md, const char *field, int which, const double *host,
+                            size_t n);
+pth_status pth_field_download(pth_meshdata *md, const char *field, int which, double *host,
+                              size_t n);
+
+/* ---- boundary communication (src/bvals/comms/bvals_in_one.hpp:41-93) ---- */
+/* SendBoundBufs<bt>: boundary_communication.cpp:49-159 (restrict + pack). */
+pth_status pth_send_bound_bufs(pth_meshdata *md, int bt);
+/* StartReceiveBoundBufs<bt>: boundary_communication.cpp:173-186. */
+pth_status pth_start_receive_bound_bufs(pth_meshdata *md, int bt);
+/* ReceiveBoundBufs<bt>: boundary_communication.cpp:201-236 (state flip on one GPU). */
+pth_status pth_receive_bound_bufs(pth_meshdata *md, int bt);
+/* SetBounds<bt>: boundary_communication.cpp:251-348 (unpack + restrict). */
+pth_status pth_set_bounds(pth_meshdata *md, int bt);
+/* ProlongateBounds<bt>: boundary_communication.cpp:361-393 (minmod shared prolongation). */
+pth_status pth_prolongate_bounds(pth_meshdata *md, int bt);
+
+/* ---- Burgers benchmark package and Update:: tasks ---- */
+/* burgers_package::CalculateFluxes (benchmarks/burgers/burgers_package.cpp:202-404) on the
+ * field "U" of md; recon 0 = WENO5Z, 1 = MC-limited linear. The reconstructed face states
+ * (the reference's OneCopy Ulx..Urz) live in a library-owned scratch arena. */
+pth_status pth_burgers_calculate_fluxes(pth_meshdata *md, int recon);
+/* Update::FluxDivergence<MeshData> (src/interface/update.cpp:63-84). */
+pth_status pth_flux_divergence(pth_meshdata *in, pth_meshdata *dudt);
+/* Update::WeightedSumData over Independent fields (src/interface/update.hpp:71-90):
+ * out = w1 * in1 + w2 * in2 over entire arrays. */
+pth_status pth_weighted_sum_independent(pth_meshdata *in1, pth_meshdata *in2, double w1,
+                                        double w2, pth_meshdata *out);
+/* burgers_package::CalculateDerived (burgers_package.cpp:143-167). */
+pth_status pth_burgers_calculate_derived(pth_meshdata *md);
+/* burgers_package::EstimateTimestepMesh (burgers_package.cpp:170-200); synchronous. */
+pth_status pth_burgers_estimate_timestep(pth_meshdata *md, double cfl, double *dt);
+
+/* Problem generators for "U" of md (host-evaluated, uploaded): the seeded sum of sinusoids
+ * and the reference's tanh/exp profile (benchmarks/burgers/parthenon_app_inputs.cpp:34-79). */
+pth_status pth_burgers_pgen_sinusoid(pth_meshdata *md, uint64_t seed);
+pth_status pth_burgers_pgen_vibe(pth_meshdata *md);
+
+/* Instrumentation (the reference's PARTHENON_INSTRUMENT regions, src/utils/instrument.hpp):
+ * when enabled, every kernel launch is bracketed by HIP events on its stream.
+ * pth_profile_report waits for the recorded events, writes one line per kernel name,
+ * "name launches total_ms\n", into buf (NUL-terminated, at most len bytes), clears the
+ * records and returns the number of lines. No counterpart in the reference's FFI. */
+pth_status pth_profile_enable(int enable);
+int pth_profile_report(char *buf, int len);
+
+const char *pth_last_error(void);
+const char *pth_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PARTHENON_AMD_H_ */

@@ -1,0 +1,362 @@
+// ORACLE — TEST INFRASTRUCTURE ONLY.
+// C entry points over the CPU restatement (orc_*.hpp) so pytest can drive it through ctypes.
+// Only tests/, the smoke check and the CPU-baseline leg of bench.py may load liboracle.so;
+// the product library never links or calls it.
+#include <cstdint>
+#include <cstring>
+#include <memory>
+
+#include "orc_burgers.hpp"
+
+using namespace orc;
+
+namespace {
+
+// SplitMix64 (Steele, Lea, Flood 2014): the portable seeded generator every input uses.
+struct SplitMix64 {
+  std::uint64_t s;
+  std::uint64_t next() {
+    std::uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+  }
+  double uniform() { return (double)(next() >> 11) * 0x1.0p-53; }
+};
+
+struct OMesh {
+  std::unique_ptr<Mesh> m;
+};
+
+}  // namespace
+
+extern "C" {
+
+void *orc_mesh_new(const int *nx, const int *bnx, int nghost, const double *xmin,
+                   const double *xmax, const int *periodic) {
+  MeshDesc d;
+  for (int k = 0; k < 3; ++k) {
+    d.nx[k] = nx[k];
+    d.bnx[k] = bnx[k];
+    d.xmin[k] = xmin[k];
+    d.xmax[k] = xmax[k];
+    d.periodic[k] = periodic[k];
+  }
+  d.nghost = nghost;
+  auto *o = new OMesh;
+  o->m.reset(new Mesh(d));
+  return o;
+}
+
+void orc_mesh_refine_box(void *h, const double *box, int level) {
+  static_cast<OMesh *>(h)->m->AddRefinementBox(box, level);
+}
+
+void orc_mesh_refine_sphere(void *h, const double *c, double r, int level) {
+  static_cast<OMesh *>(h)->m->AddRefinementSphere(c, r, level);
+}
+
+void orc_mesh_finalize(void *h) { static_cast<OMesh *>(h)->m->Finalize(); }
+
+void orc_mesh_free(void *h) { delete static_cast<OMesh *>(h); }
+
+int orc_mesh_nblocks(void *h) { return (int)static_cast<OMesh *>(h)->m->blocks.size(); }
+
+int orc_mesh_multilevel(void *h) { return static_cast<OMesh *>(h)->m->multilevel ? 1 : 0; }
+
+// out: level, l1, l2, l3 ; x: xmin1..3, xmax1..3
+void orc_mesh_block(void *h, int gid, std::int64_t *out, double *x) {
+  const Block &b = static_cast<OMesh *>(h)->m->blocks.at(gid);
+  out[0] = b.loc.level;
+  for (int k = 0; k < 3; ++k) {
+    out[1 + k] = b.loc.l[k];
+    x[k] = b.xmin[k];
+    x[3 + k] = b.xmax[k];
+  }
+}
+
+int orc_mesh_nneighbors(void *h, int gid) {
+  return (int)static_cast<OMesh *>(h)->m->blocks.at(gid).neighbors.size();
+}
+
+// out: gid, level, origin l1..3, offsets 1..3
+void orc_mesh_neighbor(void *h, int gid, int n, std::int64_t *out) {
+  const NeighborBlock &nb = static_cast<OMesh *>(h)->m->blocks.at(gid).neighbors.at(n);
+  out[0] = nb.gid;
+  out[1] = nb.origin_loc.level;
+  for (int k = 0; k < 3; ++k) {
+    out[2 + k] = nb.origin_loc.l[k];
+    out[5 + k] = nb.off[k];
+  }
+}
+
+// out: cellbounds n[3], ng[3], c_cellbounds n[3], ng[3]
+void orc_mesh_shapes(void *h, int *out) {
+  const Mesh &m = *static_cast<OMesh *>(h)->m;
+  for (int k = 0; k < 3; ++k) {
+    out[k] = m.cellbounds.n[k];
+    out[3 + k] = m.cellbounds.ng[k];
+    out[6 + k] = m.c_cellbounds.n[k];
+    out[9 + k] = m.c_cellbounds.ng[k];
+  }
+}
+
+// Index boxes of every boundary region of block gid: records of 7 ints {kind, i_lo, i_hi,
+// j_lo, j_hi, k_lo, k_hi}; kind 0 send, 1 receive, 2 restriction before send, 3 prolongation
+// after set, 4 restriction after set (in the order the reference builds its caches).
+int orc_mesh_region_boxes(void *h, int gid, int *out, int max) {
+  const Mesh &m = *static_cast<OMesh *>(h)->m;
+  const Block &b = m.blocks.at(gid);
+  int n = 0;
+  auto put = [&](int kind, const Idx6 &x) {
+    if (n >= max) return;
+    int *o = out + 7 * n;
+    o[0] = kind;
+    o[1] = x.s[5];
+    o[2] = x.e[5];
+    o[3] = x.s[4];
+    o[4] = x.e[4];
+    o[5] = x.s[3];
+    o[6] = x.e[3];
+    ++n;
+  };
+  for (const auto &nb : b.neighbors) {
+    put(0, CalcIndices(m, b, nb, 1, nb.IsCell() ? IRT::InteriorSend : IRT::BoundaryInteriorSend,
+                       false));
+    put(1, CalcIndices(m, b, nb, 1, nb.IsCell() ? IRT::InteriorRecv : IRT::BoundaryExteriorRecv,
+                       false));
+    if (!m.multilevel) continue;
+    const bool coarser = nb.origin_loc.level < b.loc.level;
+    if (coarser) {
+      put(2, CalcIndices(m, b, nb, 1, IRT::BoundaryInteriorSend, true));
+      put(3, CalcIndices(m, b, nb, 1, IRT::BoundaryExteriorRecv, true));
+    } else if (b.loc.level > 0 && b.has_coarser_neighbor) {
+      put(4, CalcIndices(m, b, nb, 1, IRT::BoundaryExteriorRecv, true));
+    }
+  }
+  return n;
+}
+
+// Number of buffer elements actually used by one Send over nfields variables, plus the
+// allocated (GetBufferSize) total in *allocated.
+std::int64_t orc_exchange_counts(void *h, int nfields, const int *ncomp, std::int64_t *allocated) {
+  const Mesh &m = *static_cast<OMesh *>(h)->m;
+  std::int64_t used = 0, alloc = 0;
+  for (const auto &b : m.blocks)
+    for (int f = 0; f < nfields; ++f)
+      for (const auto &nb : b.neighbors) {
+        IRT st = nb.IsCell() ? IRT::InteriorSend : IRT::BoundaryInteriorSend;
+        used += CalcIndices(m, b, nb, ncomp[f], st, false).size();
+        alloc += GetBufferSize(m, nb, ncomp[f]);
+      }
+  if (allocated) *allocated = alloc;
+  return used;
+}
+
+// Send + Set (+ prolongation: 0 none, 1 minmod, 2 linear) over nfields variables, repeated nrep
+// times. data[f]: [nblocks][ncomp][E]; coarse[f]: [nblocks][ncomp][cE] or null for single level.
+void orc_exchange(void *h, int nfields, const int *ncomp, double **data, double **coarse,
+                  int prolong, int nrep) {
+  const Mesh &m = *static_cast<OMesh *>(h)->m;
+  std::vector<Field> fields(nfields);
+  const std::int64_t E = m.cellbounds.size(), cE = m.c_cellbounds.size();
+  const std::int64_t nb = m.blocks.size();
+  for (int f = 0; f < nfields; ++f) {
+    fields[f].ncomp = ncomp[f];
+    fields[f].data.assign(data[f], data[f] + nb * ncomp[f] * E);
+    if (m.multilevel) fields[f].coarse.assign(coarse[f], coarse[f] + nb * ncomp[f] * cE);
+  }
+  Exchange ex;
+  ex.Build(m, fields);
+  for (int r = 0; r < nrep; ++r) {
+    ex.Send(fields);
+    ex.Set(fields);
+    if (prolong) ex.Prolongate(fields, prolong == 1);
+  }
+  for (int f = 0; f < nfields; ++f) {
+    std::memcpy(data[f], fields[f].data.data(), sizeof(double) * nb * ncomp[f] * E);
+    if (m.multilevel && coarse[f])
+      std::memcpy(coarse[f], fields[f].coarse.data(), sizeof(double) * nb * ncomp[f] * cE);
+  }
+}
+
+// Restrict block interiors: coarse(c,k,j,i) over the whole coarse interior of every block.
+void orc_restrict_interior(void *h, int ncomp, const double *data, double *coarse) {
+  const Mesh &m = *static_cast<OMesh *>(h)->m;
+  Field f;
+  f.ncomp = ncomp;
+  const std::int64_t nb = m.blocks.size();
+  f.data.assign(data, data + nb * ncomp * m.cellbounds.size());
+  f.coarse.assign(coarse, coarse + nb * ncomp * m.c_cellbounds.size());
+  const Shape &cs = m.c_cellbounds;
+  for (const auto &b : m.blocks)
+    for (int c = 0; c < ncomp; ++c)
+      for (int k = cs.s(2); k <= cs.e(2); ++k)
+        for (int j = cs.s(1); j <= cs.e(1); ++j)
+          for (int i = cs.s(0); i <= cs.e(0); ++i) RestrictCell(m, b, f, c, k, j, i);
+  std::memcpy(coarse, f.coarse.data(), sizeof(double) * f.coarse.size());
+}
+
+// Prolongate one coarse cell per listed index (gid, c, k, j, i) from coarse into data.
+void orc_prolongate_cells(void *h, int ncomp, double *data, const double *coarse, int ncells,
+                          const int *cells, int minmod) {
+  const Mesh &m = *static_cast<OMesh *>(h)->m;
+  Field f;
+  f.ncomp = ncomp;
+  const std::int64_t nb = m.blocks.size();
+  f.data.assign(data, data + nb * ncomp * m.cellbounds.size());
+  f.coarse.assign(coarse, coarse + nb * ncomp * m.c_cellbounds.size());
+  for (int q = 0; q < ncells; ++q) {
+    const int *c = cells + 5 * q;
+    ProlongateCell(m, m.blocks.at(c[0]), f, c[1], c[2], c[3], c[4], minmod != 0);
+  }
+  std::memcpy(data, f.data.data(), sizeof(double) * f.data.size());
+}
+
+void orc_weno5z(int n, const double *q, double *ql, double *qr) {
+  for (int i = 0; i < n; ++i)
+    WENO5Z(q[5 * i], q[5 * i + 1], q[5 * i + 2], q[5 * i + 3], q[5 * i + 4], ql[i], qr[i]);
+}
+
+void orc_linear(int n, const double *q, double *ql, double *qr) {
+  for (int i = 0; i < n; ++i) Linear(q[3 * i], q[3 * i + 1], q[3 * i + 2], ql[i], qr[i]);
+}
+
+// in: 8 per face (uxl uxr uyl uyr uzl uzr upl upr); out: 5 per face (sl sr fux fuy fuz)
+void orc_lr_to_flux(int n, const double *in, double *out) {
+  for (int i = 0; i < n; ++i) {
+    const double *a = in + 8 * i;
+    double *o = out + 5 * i;
+    lr_to_flux(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], o[0], o[1], o[2], o[3], o[4]);
+  }
+}
+
+// ---- Burgers benchmark ----
+struct OBurgers {
+  Burgers b;
+};
+
+void *orc_burgers_new(void *mesh, int num_scalars, double cfl, int recon) {
+  auto *o = new OBurgers;
+  o->b.Init(*static_cast<OMesh *>(mesh)->m, num_scalars, cfl, recon);
+  return o;
+}
+
+void orc_burgers_free(void *h) { delete static_cast<OBurgers *>(h); }
+
+// Seeded sum-of-sinusoids initial condition: for each component c and mode m = 0..3,
+// amplitude A ~ U(0.1, 1), phase phi ~ U(0, 2 pi) drawn in (c, m, A, phi) order from
+// SplitMix64(seed); mode m varies along direction (c + m) % 3 with wavenumber m + 1:
+//   U_c(x) = sum_m A sin(2 pi (m + 1) x_{(c+m)%3} + phi)   over block interiors.
+void orc_burgers_pgen_sinusoid(void *h, std::uint64_t seed) {
+  Burgers &B = static_cast<OBurgers *>(h)->b;
+  SplitMix64 rng{seed};
+  std::vector<double> A(B.V * 4), P(B.V * 4);
+  for (int c = 0; c < B.V; ++c)
+    for (int m = 0; m < 4; ++m) {
+      A[4 * c + m] = 0.1 + 0.9 * rng.uniform();
+      P[4 * c + m] = 2.0 * M_PI * rng.uniform();
+    }
+  const Shape &s = B.m->cellbounds;
+  for (int b = 0; b < (int)B.m->blocks.size(); ++b) {
+    const Coords &co = B.m->blocks[b].coords;
+    for (int c = 0; c < B.V; ++c)
+      for (int k = s.s(2); k <= s.e(2); ++k)
+        for (int j = s.s(1); j <= s.e(1); ++j)
+          for (int i = s.s(0); i <= s.e(0); ++i) {
+            const double x[3] = {co.Xc(0, i), co.Xc(1, j), co.Xc(2, k)};
+            double v = 0.0;
+            for (int m = 0; m < 4; ++m)
+              v += A[4 * c + m] * std::sin(2.0 * M_PI * (m + 1) * x[(c + m) % 3] + P[4 * c + m]);
+            B.u(B.base.U.data, b, c, k, j, i) = v;
+          }
+  }
+}
+
+// The reference's own tanh/exp problem generator (benchmarks/burgers/parthenon_app_inputs.cpp:34-79)
+void orc_burgers_pgen_vibe(void *h) {
+  Burgers &B = static_cast<OBurgers *>(h)->b;
+  const Shape &s = B.m->cellbounds;
+  for (int b = 0; b < (int)B.m->blocks.size(); ++b) {
+    const Coords &co = B.m->blocks[b].coords;
+    for (int k = s.s(2); k <= s.e(2); ++k)
+      for (int j = s.s(1); j <= s.e(1); ++j)
+        for (int i = s.s(0); i <= s.e(0); ++i) {
+          const double x = co.Xc(0, i), y = co.Xc(1, j), z = co.Xc(2, k);
+          const double qx = (std::tanh(-20.0 * x) * std::cos(M_PI * x) + 1.0) *
+                            std::exp(-30.0 * y * y) * std::exp(-30.0 * z * z);
+          const double qy = (std::sin(M_PI * y) + 0.2) * std::exp(-30.0 * x * x) *
+                            std::exp(-30.0 * z * z);
+          const double qz = (std::tanh(-20. * z) * std::cos(M_PI * z) + 0.5) *
+                            std::exp(-30.0 * x * x) * std::exp(-30.0 * y * y);
+          B.u(B.base.U.data, b, 0, k, j, i) = qx;
+          B.u(B.base.U.data, b, 1, k, j, i) = qy;
+          B.u(B.base.U.data, b, 2, k, j, i) = qz;
+          for (int n = 3; n < B.V; ++n) {
+            double q = 1;
+            if (std::abs(x) < 0.025 && std::abs(y) < 0.15 && std::abs(z) < 0.025) q += 10.0;
+            B.u(B.base.U.data, b, n, k, j, i) = q;
+          }
+        }
+  }
+}
+
+void orc_burgers_initialize(void *h) { static_cast<OBurgers *>(h)->b.Initialize(); }
+
+void orc_burgers_cycles(void *h, int n) {
+  for (int q = 0; q < n; ++q) static_cast<OBurgers *>(h)->b.Cycle();
+}
+
+// Stage-level entry points for kernel parity tests on the "base" container.
+void orc_burgers_fluxes(void *h) {
+  Burgers &B = static_cast<OBurgers *>(h)->b;
+  B.CalculateFluxes(B.base);
+}
+void orc_burgers_fluxdiv(void *h) {
+  Burgers &B = static_cast<OBurgers *>(h)->b;
+  B.FluxDivergence(B.base, B.dudt);
+}
+void orc_burgers_derived(void *h) {
+  Burgers &B = static_cast<OBurgers *>(h)->b;
+  B.CalculateDerived(B.base);
+}
+double orc_burgers_estimate_dt(void *h) {
+  Burgers &B = static_cast<OBurgers *>(h)->b;
+  return B.EstimateTimestep(B.base);
+}
+
+double orc_burgers_dt(void *h) { return static_cast<OBurgers *>(h)->b.dt; }
+double orc_burgers_time(void *h) { return static_cast<OBurgers *>(h)->b.time; }
+
+static std::vector<double> *Select(Burgers &B, int which) {
+  switch (which) {
+    case 0: return &B.base.U.data;
+    case 1: return &B.one.U.data;
+    case 2: return &B.derived;
+    case 3: return &B.dudt.U.data;
+    case 4: return &B.base.flux[0];
+    case 5: return &B.base.flux[1];
+    case 6: return &B.base.flux[2];
+    default: return nullptr;
+  }
+}
+
+// which: 0 base U, 1 stage U, 2 derived, 3 dudt, 4..6 base fluxes x/y/z
+std::int64_t orc_burgers_size(void *h, int which) {
+  auto *v = Select(static_cast<OBurgers *>(h)->b, which);
+  return v ? (std::int64_t)v->size() : -1;
+}
+void orc_burgers_get(void *h, int which, double *out) {
+  auto *v = Select(static_cast<OBurgers *>(h)->b, which);
+  std::memcpy(out, v->data(), sizeof(double) * v->size());
+}
+void orc_burgers_set(void *h, int which, const double *in) {
+  auto *v = Select(static_cast<OBurgers *>(h)->b, which);
+  std::memcpy(v->data(), in, sizeof(double) * v->size());
+}
+
+double orc_burgers_mass_history(void *h, const double *xmn, const double *xmx, double vol) {
+  return static_cast<OBurgers *>(h)->b.MassHistory(xmn, xmx, vol);
+}
+
+}  // extern "C"

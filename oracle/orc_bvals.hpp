@@ -1,0 +1,391 @@
+// ORACLE — TEST INFRASTRUCTURE ONLY.
+// CPU restatement of the reference's boundary-buffer path for cell-centred variables:
+// BndInfo / ProResInfo index ranges, buffer sizes, pack (SendBoundBufs), unpack
+// (SetBounds), cell-average restriction and minmod prolongation (ProlongateBounds).
+//
+// Restated from (paths relative to the reference root):
+//   src/bvals/comms/bnd_info.cpp:105-252   CalcIndices
+//   src/bvals/comms/bnd_info.cpp:254-271   GetBufferSize
+//   src/bvals/comms/bnd_info.cpp:273-448   BndInfo / ProResInfo constructors
+//   src/bvals/comms/bvals_utils.hpp:38-64  Send/Receive buffer keys
+//   src/bvals/comms/boundary_communication.cpp:49-159,251-348,361-393  Send/Set/Prolongate
+//   src/mesh/forest/block_ownership.cpp:89-141  index-range masks (CC reduces to all-true)
+//   src/utils/indexer.hpp:70-190           6-D indexer and IsActive
+//   src/prolong_restrict/pr_ops.hpp:78-164 GetGridSpacings, GradMinMod, RestrictAverage
+//   src/prolong_restrict/pr_ops.hpp:166-289  ProlongateSharedGeneral (minmod / linear)
+//   src/prolong_restrict/pr_loops.hpp:111-159 loop over buffer subsets
+//   src/config.hpp.in:86                   SIGN(x) = (x < 0 ? -1 : 1)
+#pragma once
+#include <cstring>
+#include <map>
+#include <tuple>
+
+#include "orc_mesh.hpp"
+
+namespace orc {
+
+// A cell-centred variable over all blocks: [block][comp][k][j][i], plus its coarse buffer
+// (reference coarse_s) in the c_cellbounds shape when the mesh is multilevel.
+struct Field {
+  std::string name;
+  int ncomp = 1;
+  bool fill_ghost = true;
+  std::vector<double> data;
+  std::vector<double> coarse;
+};
+
+struct Idx6 {  // reference SpatiallyMaskedIndexer6D restricted to what CC fields need
+  int s[6], e[6];
+  bool mask[27];
+  std::int64_t size() const {
+    std::int64_t n = 1;
+    for (int q = 0; q < 6; ++q) n *= (e[q] - s[q] + 1);
+    return n;
+  }
+  void Decode(std::int64_t idx, int out[6]) const {
+    std::int64_t N[6];
+    std::int64_t cur = 1;
+    for (int q = 5; q >= 0; --q) {
+      N[q] = cur;
+      cur *= (e[q] - s[q] + 1);
+    }
+    for (int q = 0; q < 6; ++q) {
+      out[q] = (int)(idx / N[q]);
+      idx -= out[q] * N[q];
+      out[q] += s[q];
+    }
+  }
+  bool IsActive(int k, int j, int i) const {
+    int ii = (i == e[5]) - (i == s[5]);
+    int jj = (j == e[4]) - (j == s[4]);
+    int kk = (k == e[3]) - (k == s[3]);
+    return mask[(ii + 1) * 9 + (jj + 1) * 3 + (kk + 1)];
+  }
+};
+
+enum class IRT { BoundaryInteriorSend, BoundaryExteriorRecv, InteriorSend, InteriorRecv };
+
+// reference: bnd_info.cpp:105-252 for a cell-centred, non-flux, non-fine variable.
+inline Idx6 CalcIndices(const Mesh &m, const Block &b, const NeighborBlock &nb, int ncomp,
+                        IRT type, bool prores) {
+  const int g = m.d.nghost;
+  Shape shape = (prores || nb.loc.level < b.loc.level) ? m.c_cellbounds : m.cellbounds;
+  const int coarse_fac = nb.loc.level > b.loc.level ? 2 : 1;
+  int nbn[3];
+  for (int k = 0; k < 3; ++k) nbn[k] = m.active[k] ? m.d.bnx[k] / coarse_fac : 1;
+  int interior_offset = type == IRT::BoundaryInteriorSend ? g : 0;
+  int exterior_offset = type == IRT::BoundaryExteriorRecv ? g : 0;
+  if (prores) exterior_offset /= 2;
+  int s[3], e[3];
+  for (int dir = 0; dir < 3; ++dir) {
+    const int bs = shape.s(dir), be = shape.e(dir);
+    if (nb.off[dir] == 0) {
+      s[dir] = bs;
+      e[dir] = be;
+      if (b.loc.level < nb.origin_loc.level && m.active[dir]) {
+        const int extra = (be - bs + 1) - nbn[dir];
+        const std::int64_t par = nb.origin_loc.l[dir] % 2;
+        s[dir] += par == 1 ? extra - interior_offset : 0;
+        e[dir] -= par == 0 ? extra - interior_offset : 0;
+        if (type == IRT::InteriorSend && !prores) {
+          s[dir] -= g;
+          e[dir] += g;
+        }
+      }
+      if (b.loc.level > nb.origin_loc.level && m.active[dir]) {
+        const std::int64_t par = b.loc.l[dir] % 2;
+        s[dir] -= par == 1 ? exterior_offset : 0;
+        e[dir] += par == 0 ? exterior_offset : 0;
+        if (type == IRT::InteriorRecv && !prores) {
+          s[dir] -= g;
+          e[dir] += g;
+        }
+      }
+      if (prores && m.active[dir] && type == IRT::InteriorRecv) {
+        s[dir] -= g / 2;
+        e[dir] += g / 2;
+      }
+    } else if (nb.off[dir] > 0) {
+      s[dir] = be - interior_offset + 1;
+      e[dir] = be + exterior_offset;
+    } else {
+      s[dir] = bs - exterior_offset;
+      e[dir] = bs + interior_offset - 1;
+    }
+  }
+  Idx6 out;
+  out.s[0] = 0;
+  out.e[0] = 0;
+  out.s[1] = 0;
+  out.e[1] = 0;
+  out.s[2] = 0;
+  out.e[2] = ncomp - 1;
+  out.s[3] = s[2];
+  out.e[3] = e[2];
+  out.s[4] = s[1];
+  out.e[4] = e[1];
+  out.s[5] = s[0];
+  out.e[5] = e[0];
+  // For a cell-centred element, GetIndexRangeMaskFromOwnership copies the neighbour's
+  // ownership of its own volume (always true) into every entry, so the mask is all-true.
+  for (int q = 0; q < 27; ++q) out.mask[q] = true;
+  return out;
+}
+
+// reference: bnd_info.cpp:254-271
+inline std::int64_t GetBufferSize(const Mesh &m, const NeighborBlock &nb, int ncomp) {
+  const Shape &cb = m.cellbounds;
+  const int g = m.d.nghost;
+  std::int64_t sz = 1;
+  for (int k = 0; k < 3; ++k) sz *= (nb.off[k] == 0 ? cb.ent(k) + 1 : g + 1);
+  return sz * ncomp;
+}
+
+// Channel key: sender gid, receiver gid, variable index, location index (bvals_utils.hpp:43-64)
+using Key = std::tuple<int, int, int, int>;
+
+struct BndInfo {
+  Idx6 idx;
+  double *var = nullptr;  // block slab of data or coarse
+  Shape shape;
+  bool same_to_same = false;
+  Key key;
+};
+
+struct ProRes {
+  bool restrict_op = false, prolong_op = false;
+  Idx6 idx;
+  const Block *block = nullptr;
+  int field = 0;
+};
+
+inline double *BlockSlab(Field &f, const Shape &sh, int gid, bool coarse) {
+  std::vector<double> &v = coarse ? f.coarse : f.data;
+  return v.data() + (std::int64_t)gid * f.ncomp * sh.size();
+}
+
+inline double &At(double *slab, const Shape &sh, int c, int k, int j, int i) {
+  return slab[(((std::int64_t)c * sh.ent(2) + k) * sh.ent(1) + j) * sh.ent(0) + i];
+}
+
+inline int SIGN(double x) { return x < 0 ? -1 : 1; }
+
+// reference: pr_ops.hpp:105-164 (3-D, cell-centred, uniform volumes)
+inline void RestrictCell(const Mesh &m, const Block &b, Field &f, int c, int ck, int cj, int ci) {
+  const Shape &fs = m.cellbounds, &cs = m.c_cellbounds;
+  double *fine = BlockSlab(f, fs, b.gid, false);
+  double *coarse = BlockSlab(f, cs, b.gid, true);
+  const bool X1 = m.active[0], X2 = m.active[1], X3 = m.active[2];
+  const int i = X1 ? (ci - cs.s(0)) * 2 + fs.s(0) : fs.s(0);
+  const int j = X2 ? (cj - cs.s(1)) * 2 + fs.s(1) : fs.s(1);
+  const int k = X3 ? (ck - cs.s(2)) * 2 + fs.s(2) : fs.s(2);
+  double vol[2][2][2] = {}, terms[2][2][2] = {};
+  for (int ok = 0; ok < 1 + X3; ++ok)
+    for (int oj = 0; oj < 1 + X2; ++oj)
+      for (int oi = 0; oi < 1 + X1; ++oi) {
+        vol[ok][oj][oi] = b.coords.vol;
+        terms[ok][oj][oi] = vol[ok][oj][oi] * At(fine, fs, c, k + ok, j + oj, i + oi);
+      }
+  const double tvol = ((vol[0][0][0] + vol[0][1][0]) + (vol[0][0][1] + vol[0][1][1])) +
+                      ((vol[1][0][0] + vol[1][1][0]) + (vol[1][0][1] + vol[1][1][1]));
+  At(coarse, cs, c, ck, cj, ci) =
+      (((terms[0][0][0] + terms[0][1][0]) + (terms[0][0][1] + terms[0][1][1])) +
+       ((terms[1][0][0] + terms[1][1][0]) + (terms[1][0][1] + terms[1][1][1]))) /
+      tvol;
+}
+
+// reference: pr_ops.hpp:90-101
+inline double GradMinMod(double fc, double fm, double fp, double dxm, double dxp, double &gxm,
+                         double &gxp) {
+  gxm = (fc - fm) / dxm;
+  gxp = (fp - fc) / dxp;
+  return 0.5 * (SIGN(gxm) + SIGN(gxp)) * std::min(std::abs(gxm), std::abs(gxp));
+}
+
+// reference: pr_ops.hpp:166-283 (cell-centred); minmod=true is ProlongateSharedMinMod,
+// minmod=false is ProlongateSharedLinear.
+inline void ProlongateCell(const Mesh &m, const Block &b, Field &f, int c, int k, int j, int i,
+                           bool minmod) {
+  const Shape &fs = m.cellbounds, &cs = m.c_cellbounds;
+  double *fine = BlockSlab(f, fs, b.gid, false);
+  double *coarse = BlockSlab(f, cs, b.gid, true);
+  const bool X1 = m.active[0], X2 = m.active[1], X3 = m.active[2];
+  const int fi = X1 ? (i - cs.s(0)) * 2 + fs.s(0) : fs.s(0);
+  const int fj = X2 ? (j - cs.s(1)) * 2 + fs.s(1) : fs.s(1);
+  const int fk = X3 ? (k - cs.s(2)) * 2 + fs.s(2) : fs.s(2);
+  const Coords &cc = b.coarse_coords, &fc_ = b.coords;
+  const double fc = At(coarse, cs, c, k, j, i);
+  double dxfm[3] = {0, 0, 0}, dxfp[3] = {0, 0, 0}, gm[3] = {0, 0, 0}, gp[3] = {0, 0, 0};
+  const int ci[3] = {i, j, k}, fidx[3] = {fi, fj, fk};
+  for (int d = 0; d < 3; ++d) {
+    if (!m.active[d]) continue;
+    const double xm = cc.Xc(d, ci[d] - 1), xc = cc.Xc(d, ci[d]), xp = cc.Xc(d, ci[d] + 1);
+    const double dxm = xc - xm, dxp = xp - xc;
+    const double fxm = fc_.Xc(d, fidx[d]), fxp = fc_.Xc(d, fidx[d] + 1);
+    dxfm[d] = xc - fxm;
+    dxfp[d] = fxp - xc;
+    int km = k, jm = j, im = i, kp = k, jp = j, ip = i;
+    if (d == 0) { im = i - 1; ip = i + 1; }
+    if (d == 1) { jm = j - 1; jp = j + 1; }
+    if (d == 2) { km = k - 1; kp = k + 1; }
+    double gxm, gxp;
+    const double gc = GradMinMod(fc, At(coarse, cs, c, km, jm, im), At(coarse, cs, c, kp, jp, ip),
+                                 dxm, dxp, gxm, gxp);
+    if (minmod) {
+      gm[d] = gc;
+      gp[d] = gc;
+    } else {
+      gm[d] = gxm;
+      gp[d] = gxp;
+    }
+  }
+  const double g1m = gm[0], g1p = gp[0], g2m = gm[1], g2p = gp[1], g3m = gm[2], g3p = gp[2];
+  const double d1m = dxfm[0], d1p = dxfp[0], d2m = dxfm[1], d2p = dxfp[1], d3m = dxfm[2],
+               d3p = dxfp[2];
+  At(fine, fs, c, fk, fj, fi) = fc - (g1m * d1m + g2m * d2m + g3m * d3m);
+  if (X1) At(fine, fs, c, fk, fj, fi + 1) = fc + (g1p * d1p - g2m * d2m - g3m * d3m);
+  if (X2) At(fine, fs, c, fk, fj + 1, fi) = fc - (g1m * d1m - g2p * d2p + g3m * d3m);
+  if (X2 && X1) At(fine, fs, c, fk, fj + 1, fi + 1) = fc + (g1p * d1p + g2p * d2p - g3m * d3m);
+  if (X3) At(fine, fs, c, fk + 1, fj, fi) = fc - (g1m * d1m + g2m * d2m - g3p * d3p);
+  if (X3 && X1) At(fine, fs, c, fk + 1, fj, fi + 1) = fc + (g1p * d1p - g2m * d2m + g3p * d3p);
+  if (X3 && X2) At(fine, fs, c, fk + 1, fj + 1, fi) = fc - (g1m * d1m - g2p * d2p - g3p * d3p);
+  if (X3 && X2 && X1)
+    At(fine, fs, c, fk + 1, fj + 1, fi + 1) = fc + (g1p * d1p + g2p * d2p + g3p * d3p);
+}
+
+inline void RunRestrict(const Mesh &m, std::vector<Field> &fields, const std::vector<ProRes> &pr) {
+  for (const auto &p : pr) {
+    if (!p.restrict_op) continue;
+    for (std::int64_t n = 0; n < p.idx.size(); ++n) {
+      int t[6];
+      p.idx.Decode(n, t);
+      if (p.idx.IsActive(t[3], t[4], t[5]))
+        RestrictCell(m, *p.block, fields[p.field], t[2], t[3], t[4], t[5]);
+    }
+  }
+}
+
+inline void RunProlongate(const Mesh &m, std::vector<Field> &fields, const std::vector<ProRes> &pr,
+                          bool minmod) {
+  for (const auto &p : pr) {
+    if (!p.prolong_op) continue;
+    for (std::int64_t n = 0; n < p.idx.size(); ++n) {
+      int t[6];
+      p.idx.Decode(n, t);
+      if (p.idx.IsActive(t[3], t[4], t[5]))
+        ProlongateCell(m, *p.block, fields[p.field], t[2], t[3], t[4], t[5], minmod);
+    }
+  }
+}
+
+// Boundary exchange state for one set of fields on one mesh (reference MeshData caches +
+// Mesh::boundary_comm_map). Buffers live here, keyed like the reference channels.
+struct Exchange {
+  const Mesh *m;
+  std::map<Key, std::vector<double>> buffers;
+  std::vector<BndInfo> send, recv;
+  std::vector<ProRes> send_pr, set_pr;
+
+  void Build(const Mesh &mesh, std::vector<Field> &fields) {
+    m = &mesh;
+    send.clear();
+    recv.clear();
+    send_pr.clear();
+    set_pr.clear();
+    buffers.clear();
+    for (const auto &b : mesh.blocks) {
+      for (int fi = 0; fi < (int)fields.size(); ++fi) {
+        Field &f = fields[fi];
+        if (!f.fill_ghost) continue;
+        for (const auto &nb : b.neighbors) {
+          // send side (GetSendBndInfo + ProResInfo::GetSend)
+          IRT st = nb.IsCell() ? IRT::InteriorSend : IRT::BoundaryInteriorSend;
+          BndInfo s;
+          bool coarser = nb.origin_loc.level < b.loc.level;
+          s.shape = coarser ? mesh.c_cellbounds : mesh.cellbounds;
+          s.var = BlockSlab(f, s.shape, b.gid, coarser);
+          s.idx = CalcIndices(mesh, b, nb, f.ncomp, st, false);
+          s.same_to_same = (b.gid == nb.gid) && nb.IsCell();
+          s.key = Key(b.gid, nb.gid, fi, nb.GetIdx());
+          buffers[s.key].assign(GetBufferSize(mesh, nb, f.ncomp), 0.0);
+          send.push_back(s);
+          ProRes sp;
+          sp.block = &b;
+          sp.field = fi;
+          if (coarser) {
+            sp.restrict_op = true;
+            sp.idx = CalcIndices(mesh, b, nb, f.ncomp, IRT::BoundaryInteriorSend, true);
+          }
+          send_pr.push_back(sp);
+          // receive side (GetSetBndInfo + ProResInfo::GetSet)
+          IRT rt = nb.IsCell() ? IRT::InteriorRecv : IRT::BoundaryExteriorRecv;
+          BndInfo r;
+          r.shape = coarser ? mesh.c_cellbounds : mesh.cellbounds;
+          r.var = BlockSlab(f, r.shape, b.gid, coarser);
+          r.idx = CalcIndices(mesh, b, nb, f.ncomp, rt, false);
+          r.same_to_same = (b.gid == nb.gid) && nb.IsCell();
+          r.key = Key(nb.gid, b.gid, fi, nb.GetReverseIdx());
+          recv.push_back(r);
+          ProRes rp;
+          rp.block = &b;
+          rp.field = fi;
+          const bool restricted = b.loc.level > 0 && b.has_coarser_neighbor;
+          if (coarser) {
+            rp.prolong_op = true;
+            rp.idx = CalcIndices(mesh, b, nb, f.ncomp, IRT::BoundaryExteriorRecv, true);
+          } else if (restricted) {
+            rp.restrict_op = true;
+            rp.idx = CalcIndices(mesh, b, nb, f.ncomp, IRT::BoundaryExteriorRecv, true);
+          }
+          set_pr.push_back(rp);
+        }
+      }
+    }
+  }
+
+  // SendBoundBufs: restrict into coarse buffers, then pack (boundary_communication.cpp:49-159)
+  void Send(std::vector<Field> &fields) {
+    if (m->multilevel) RunRestrict(*m, fields, send_pr);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (size_t q = 0; q < send.size(); ++q) {
+      const BndInfo &s = send[q];
+      if (s.same_to_same) continue;
+      std::vector<double> &buf = buffers.at(s.key);
+      for (std::int64_t n = 0; n < s.idx.size(); ++n) {
+        int t[6];
+        s.idx.Decode(n, t);
+        buf[n] = At(s.var, s.shape, t[2], t[3], t[4], t[5]);
+      }
+    }
+  }
+
+  // SetBounds: unpack, then restrict ghost regions needed by prolongation (:251-348)
+  void Set(std::vector<Field> &fields) {
+    // regions of one Set write disjoint cells (or the same value from the same source)
+#pragma omp parallel for schedule(dynamic, 64)
+    for (size_t q = 0; q < recv.size(); ++q) {
+      const BndInfo &r = recv[q];
+      if (r.same_to_same) continue;
+      const std::vector<double> &buf = buffers.at(r.key);
+      for (std::int64_t n = 0; n < r.idx.size(); ++n) {
+        int t[6];
+        r.idx.Decode(n, t);
+        if (r.idx.IsActive(t[3], t[4], t[5])) At(r.var, r.shape, t[2], t[3], t[4], t[5]) = buf[n];
+      }
+    }
+    if (m->multilevel) RunRestrict(*m, fields, set_pr);
+  }
+
+  // ProlongateBounds (:361-393); ProlongateInternal is the identity for cell-centred fields.
+  void Prolongate(std::vector<Field> &fields, bool minmod = true) {
+    if (m->multilevel) RunProlongate(*m, fields, set_pr, minmod);
+  }
+
+  void All(std::vector<Field> &fields) {
+    Send(fields);
+    Set(fields);
+    Prolongate(fields);
+  }
+};
+
+}  // namespace orc

@@ -1,0 +1,365 @@
+"""parthenon_amd — MI355X-native per-stage meshblock data path behind Parthenon's task API.
+
+Host-side mirror of the reference interface for this path, over the C ABI in
+include/parthenon_amd.h (libparthenon_amd.so, built in-tree). Names follow the reference:
+
+  Mesh / MeshData                         src/mesh/mesh.hpp, src/interface/mesh_data.hpp
+  SendBoundBufs, StartReceiveBoundBufs,   src/bvals/comms/bvals_in_one.hpp:41-93
+  ReceiveBoundBufs, SetBounds,
+  ProlongateBounds
+  burgers_package.CalculateFluxes, ...    benchmarks/burgers/burgers_package.hpp:33-43
+  Update.FluxDivergence, ...              src/interface/update.hpp:59-300
+
+Every task returns a TaskStatus like the reference; a failure raises RuntimeError carrying
+the library's message. There is no CPU fallback: if the HIP library is missing, importing
+the compute API fails loudly.
+"""
+import ctypes as C
+import enum
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libparthenon_amd.so")
+_LIB = None
+
+
+class TaskStatus(enum.IntEnum):  # src/basic_types.hpp:59
+    complete = 0
+    incomplete = 1
+    iterate = 2
+    fail = 3
+
+
+class BoundaryType(enum.IntEnum):  # src/basic_types.hpp:156
+    any = 0
+    local = 1
+    nonlocal_ = 2
+    flxcor_send = 3
+    flxcor_recv = 4
+    gmg_same = 5
+    gmg_restrict_send = 6
+    gmg_restrict_recv = 7
+    gmg_prolongate_send = 8
+    gmg_prolongate_recv = 9
+
+
+class Metadata(enum.IntFlag):  # the flags of src/interface/metadata.hpp used on this path
+    FillGhost = 1
+    Independent = 2
+    WithFluxes = 4
+    OneCopy = 8
+
+
+class PackMode(enum.IntEnum):
+    persistent = 0
+    per_buffer = 1
+
+
+_D = C.c_double
+_I = C.c_int
+_I64 = C.c_int64
+_P = C.c_void_p
+_DP = C.POINTER(C.c_double)
+
+_SIGNATURES = {
+    "pth_mesh_create": (_P, [C.POINTER(_I), C.POINTER(_I), _I, _DP, _DP, C.POINTER(_I)]),
+    "pth_mesh_refine_box": (_I, [_P, _DP, _I]),
+    "pth_mesh_refine_sphere": (_I, [_P, _DP, _D, _I]),
+    "pth_mesh_add_field": (_I, [_P, C.c_char_p, _I, C.c_uint]),
+    "pth_mesh_plan": (_I, [_P]),
+    "pth_mesh_finalize": (_I, [_P]),
+    "pth_mesh_destroy": (None, [_P]),
+    "pth_mesh_set_stream": (_I, [_P, _P]),
+    "pth_mesh_synchronize": (_I, [_P]),
+    "pth_mesh_set_pack_mode": (_I, [_P, _I]),
+    "pth_mesh_nblocks": (_I, [_P]),
+    "pth_mesh_multilevel": (_I, [_P]),
+    "pth_mesh_block_info": (_I, [_P, _I, C.POINTER(_I64), _DP]),
+    "pth_mesh_index_shapes": (_I, [_P, C.POINTER(_I)]),
+    "pth_mesh_neighbors": (_I, [_P, _I, C.POINTER(_I64), _I]),
+    "pth_mesh_region_boxes": (_I, [_P, _I, C.POINTER(_I), _I]),
+    "pth_mesh_exchange_counts": (_I, [_P, _I, C.POINTER(_I64), C.POINTER(_I64)]),
+    "pth_meshdata_get": (_P, [_P, C.c_char_p]),
+    "pth_field_pointers": (_I, [_P, C.c_char_p, C.POINTER(_DP), C.POINTER(_DP), C.POINTER(_DP)]),
+    "pth_field_upload": (_I, [_P, C.c_char_p, _I, _DP, C.c_size_t]),
+    "pth_field_download": (_I, [_P, C.c_char_p, _I, _DP, C.c_size_t]),
+    "pth_send_bound_bufs": (_I, [_P, _I]),
+    "pth_start_receive_bound_bufs": (_I, [_P, _I]),
+    "pth_receive_bound_bufs": (_I, [_P, _I]),
+    "pth_set_bounds": (_I, [_P, _I]),
+    "pth_prolongate_bounds": (_I, [_P, _I]),
+    "pth_burgers_calculate_fluxes": (_I, [_P, _I]),
+    "pth_flux_divergence": (_I, [_P, _P]),
+    "pth_weighted_sum_independent": (_I, [_P, _P, _D, _D, _P]),
+    "pth_burgers_calculate_derived": (_I, [_P]),
+    "pth_burgers_estimate_timestep": (_I, [_P, _D, _DP]),
+    "pth_burgers_pgen_sinusoid": (_I, [_P, C.c_uint64]),
+    "pth_burgers_pgen_vibe": (_I, [_P]),
+    "pth_profile_enable": (_I, [_I]),
+    "pth_profile_report": (_I, [C.c_char_p, _I]),
+    "pth_last_error": (C.c_char_p, []),
+    "pth_version": (C.c_char_p, []),
+}
+
+
+def lib():
+    """Load libparthenon_amd.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with __graft_entry__.build() (no fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def exported_symbols():
+    return list(_SIGNATURES)
+
+
+def _check(status, what):
+    if status == TaskStatus.fail:
+        msg = lib().pth_last_error()
+        raise RuntimeError(f"{what} failed: {msg.decode() if msg else 'unknown error'}")
+    return TaskStatus(status)
+
+
+def _dptr(a):
+    if a.dtype != np.float64 or not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("arrays crossing the C ABI must be contiguous float64")
+    return a.ctypes.data_as(_DP)
+
+
+class Mesh:
+    """A uniform or statically refined Cartesian mesh of equal-size meshblocks."""
+
+    def __init__(self, nx, block, nghost, xmin=(-0.5, -0.5, -0.5), xmax=(0.5, 0.5, 0.5),
+                 periodic=(True, True, True), refine_boxes=(), refine_spheres=()):
+        L = lib()
+        arr3 = lambda v, t: (t * 3)(*v)
+        self._h = L.pth_mesh_create(arr3(nx, _I), arr3(block, _I), nghost, arr3(xmin, _D),
+                                    arr3(xmax, _D), arr3([int(bool(p)) for p in periodic], _I))
+        if not self._h:
+            raise RuntimeError("pth_mesh_create failed: " + L.pth_last_error().decode())
+        for box, level in refine_boxes:
+            _check(L.pth_mesh_refine_box(self._h, (_D * 6)(*box), level), "refine_box")
+        for center, radius, level in refine_spheres:
+            _check(L.pth_mesh_refine_sphere(self._h, arr3(center, _D), radius, level),
+                   "refine_sphere")
+        self.fields = {}
+        self._data = {}
+        self.finalized = False
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _LIB is not None:
+            self._data.clear()
+            _LIB.pth_mesh_destroy(self._h)
+            self._h = None
+
+    # -- setup --
+    def add_field(self, name, ncomp=1, flags=Metadata.FillGhost | Metadata.Independent):
+        _check(lib().pth_mesh_add_field(self._h, name.encode(), ncomp, int(flags)), "add_field")
+        self.fields[name] = (ncomp, int(flags))
+
+    def plan(self):
+        _check(lib().pth_mesh_plan(self._h), "plan")
+        self._shapes()
+
+    def finalize(self):
+        _check(lib().pth_mesh_finalize(self._h), "finalize")
+        self._shapes()
+        self.finalized = True
+
+    def _shapes(self):
+        s = (_I * 12)()
+        _check(lib().pth_mesh_index_shapes(self._h, s), "index_shapes")
+        self.n, self.ng = tuple(s[0:3]), tuple(s[3:6])
+        self.cn, self.cng = tuple(s[6:9]), tuple(s[9:12])
+        self.ent = tuple(self.n[d] + 2 * self.ng[d] for d in range(3))
+        self.cent = tuple(self.cn[d] + 2 * self.cng[d] for d in range(3))
+        self.nblocks = lib().pth_mesh_nblocks(self._h)
+        self.multilevel = bool(lib().pth_mesh_multilevel(self._h))
+
+    # -- queries --
+    def block(self, gid):
+        out = (_I64 * 4)()
+        x = (_D * 6)()
+        _check(lib().pth_mesh_block_info(self._h, gid, out, x), "block_info")
+        return dict(level=out[0], l=tuple(out[1:4]), xmin=tuple(x[0:3]), xmax=tuple(x[3:6]))
+
+    def neighbors(self, gid):
+        buf = (_I64 * (8 * 64))()
+        n = lib().pth_mesh_neighbors(self._h, gid, buf, 64)
+        return [dict(gid=buf[8 * q], level=buf[8 * q + 1], origin=tuple(buf[8 * q + 2:8 * q + 5]),
+                     off=tuple(buf[8 * q + 5:8 * q + 8])) for q in range(n)]
+
+    def region_boxes(self, gid):
+        buf = (_I * (7 * 512))()
+        n = lib().pth_mesh_region_boxes(self._h, gid, buf, 512)
+        return np.array(buf[:7 * n], dtype=np.int64).reshape(n, 7)
+
+    def exchange_counts(self, bt=BoundaryType.any):
+        nb, ne = _I64(), _I64()
+        _check(lib().pth_mesh_exchange_counts(self._h, int(bt), C.byref(nb), C.byref(ne)),
+               "exchange_counts")
+        return nb.value, ne.value
+
+    def field_shape(self, name):
+        ncomp = self.fields[name][0]
+        return (self.nblocks, ncomp, self.ent[2], self.ent[1], self.ent[0])
+
+    def coarse_shape(self, name):
+        ncomp = self.fields[name][0]
+        return (self.nblocks, ncomp, self.cent[2], self.cent[1], self.cent[0])
+
+    # -- runtime --
+    def mesh_data(self, name="base"):
+        """MeshData<Real> for a named container (mesh_data.GetOrAdd)."""
+        if name not in self._data:
+            h = lib().pth_meshdata_get(self._h, name.encode())
+            if not h:
+                raise RuntimeError("pth_meshdata_get failed: " + lib().pth_last_error().decode())
+            self._data[name] = MeshData(self, name, h)
+        return self._data[name]
+
+    def set_stream(self, stream_ptr):
+        _check(lib().pth_mesh_set_stream(self._h, C.c_void_p(stream_ptr)), "set_stream")
+
+    def set_pack_mode(self, mode):
+        _check(lib().pth_mesh_set_pack_mode(self._h, int(mode)), "set_pack_mode")
+
+    def synchronize(self):
+        _check(lib().pth_mesh_synchronize(self._h), "synchronize")
+
+
+class MeshData:
+    def __init__(self, mesh, name, handle):
+        self.mesh = mesh
+        self.name = name
+        self._h = handle
+
+    def _shape(self, field, which):
+        if which == 1:
+            return self.mesh.coarse_shape(field)
+        s = self.mesh.field_shape(field)
+        return (3,) + s if which == 2 else s
+
+    def upload(self, field, array, which=0):
+        a = np.ascontiguousarray(array, dtype=np.float64)
+        if a.shape != self._shape(field, which):
+            raise ValueError(f"shape {a.shape} != {self._shape(field, which)}")
+        _check(lib().pth_field_upload(self._h, field.encode(), which, _dptr(a), a.size), "upload")
+
+    def download(self, field, which=0):
+        out = np.empty(self._shape(field, which), dtype=np.float64)
+        _check(lib().pth_field_download(self._h, field.encode(), which, _dptr(out), out.size),
+               "download")
+        return out
+
+    def pointers(self, field):
+        d, c, f = _DP(), _DP(), _DP()
+        _check(lib().pth_field_pointers(self._h, field.encode(), C.byref(d), C.byref(c),
+                                        C.byref(f)), "pointers")
+        addr = lambda p: C.cast(p, C.c_void_p).value
+        return addr(d), addr(c), addr(f)
+
+
+# ---- boundary communication tasks (src/bvals/comms/bvals_in_one.hpp) ----
+def SendBoundBufs(md, bt=BoundaryType.any):
+    return _check(lib().pth_send_bound_bufs(md._h, int(bt)), "SendBoundBufs")
+
+
+def StartReceiveBoundBufs(md, bt=BoundaryType.any):
+    return _check(lib().pth_start_receive_bound_bufs(md._h, int(bt)), "StartReceiveBoundBufs")
+
+
+def ReceiveBoundBufs(md, bt=BoundaryType.any):
+    return _check(lib().pth_receive_bound_bufs(md._h, int(bt)), "ReceiveBoundBufs")
+
+
+def SetBounds(md, bt=BoundaryType.any):
+    return _check(lib().pth_set_bounds(md._h, int(bt)), "SetBounds")
+
+
+def ProlongateBounds(md, bt=BoundaryType.any):
+    return _check(lib().pth_prolongate_bounds(md._h, int(bt)), "ProlongateBounds")
+
+
+class burgers_package:  # benchmarks/burgers/burgers_package.hpp
+    WENO5 = 0
+    Linear = 1
+
+    @staticmethod
+    def CalculateFluxes(md, recon=0):
+        return _check(lib().pth_burgers_calculate_fluxes(md._h, recon), "CalculateFluxes")
+
+    @staticmethod
+    def CalculateDerived(md):
+        return _check(lib().pth_burgers_calculate_derived(md._h), "CalculateDerived")
+
+    @staticmethod
+    def EstimateTimestepMesh(md, cfl):
+        dt = _D()
+        _check(lib().pth_burgers_estimate_timestep(md._h, cfl, C.byref(dt)), "EstimateTimestep")
+        return dt.value
+
+    @staticmethod
+    def ProblemGeneratorSinusoid(md, seed):
+        return _check(lib().pth_burgers_pgen_sinusoid(md._h, seed), "pgen_sinusoid")
+
+    @staticmethod
+    def ProblemGenerator(md):
+        return _check(lib().pth_burgers_pgen_vibe(md._h), "pgen_vibe")
+
+
+class Update:  # src/interface/update.hpp
+    @staticmethod
+    def FluxDivergence(md_in, md_dudt):
+        return _check(lib().pth_flux_divergence(md_in._h, md_dudt._h), "FluxDivergence")
+
+    @staticmethod
+    def WeightedSumData(in1, in2, w1, w2, out):
+        return _check(lib().pth_weighted_sum_independent(in1._h, in2._h, w1, w2, out._h),
+                      "WeightedSumData")
+
+    @staticmethod
+    def AverageIndependentData(c1, c2, wgt1):
+        return Update.WeightedSumData(c1, c2, wgt1, 1.0 - wgt1, c1)
+
+    @staticmethod
+    def UpdateIndependentData(md_in, dudt, dt, out):
+        return Update.WeightedSumData(md_in, dudt, 1.0, dt, out)
+
+
+def profile_enable(on=True):
+    """Bracket every kernel launch with HIP events on its stream."""
+    _check(lib().pth_profile_enable(1 if on else 0), "profile_enable")
+
+
+def profile_report():
+    """{kernel name: (launches, total_ms)} for the launches recorded since the last report."""
+    buf = C.create_string_buffer(1 << 16)
+    lib().pth_profile_report(buf, len(buf))
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, count, ms = line.split()
+        out[name] = (int(count), float(ms))
+    return out
+
+
+def exchange(md, bt=BoundaryType.any):
+    """Send, receive, set (+ prolongate on multilevel meshes) in the library task order
+    (src/bvals/comms/boundary_communication.cpp:406-445)."""
+    SendBoundBufs(md, bt)
+    if ReceiveBoundBufs(md, bt) != TaskStatus.complete:
+        raise RuntimeError("ReceiveBoundBufs did not complete after SendBoundBufs")
+    SetBounds(md, bt)
+    if md.mesh.multilevel:
+        ProlongateBounds(md, bt)

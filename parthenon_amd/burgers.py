@@ -1,0 +1,85 @@
+"""Burgers benchmark driver over the parthenon_amd task API.
+
+Mirrors BurgersDriver::MakeTaskCollection (benchmarks/burgers/burgers_driver.cpp:53-148) with
+the low-storage RK2 integrator (src/time_integration/low_storage_integrator.cpp:67-83, stage
+names "base", "1", "base") and EvolutionDriver's time-step control
+(src/driver/driver.cpp:194-235): the first dt is the mesh estimate, later ones are
+min(2 * dt_prev, estimate).
+"""
+import math
+
+from . import (BoundaryType, Mesh, Metadata, SendBoundBufs, ReceiveBoundBufs, SetBounds,
+               ProlongateBounds, StartReceiveBoundBufs, TaskStatus, Update, burgers_package)
+
+RK2_BETA = (1.0, 0.5)
+
+
+def make_burgers_mesh(nx=(64, 64, 64), block=(16, 16, 16), nghost=4, num_scalars=4,
+                      refine_spheres=(), refine_boxes=()):
+    """Mesh with the fields burgers_package::Initialize registers
+    (benchmarks/burgers/burgers_package.cpp:76-94): U with FillGhost/Independent/WithFluxes and
+    the OneCopy derived field."""
+    m = Mesh(nx, block, nghost, refine_spheres=refine_spheres, refine_boxes=refine_boxes)
+    m.add_field("U", 3 + num_scalars,
+                Metadata.FillGhost | Metadata.Independent | Metadata.WithFluxes)
+    m.add_field("derived", 1, Metadata.OneCopy)
+    m.finalize()
+    return m
+
+
+class BurgersDriver:
+    def __init__(self, mesh, cfl=0.4, recon=burgers_package.WENO5):
+        self.mesh = mesh
+        self.cfl = cfl
+        self.recon = recon
+        self.base = mesh.mesh_data("base")
+        self.one = mesh.mesh_data("1")
+        self.dudt = mesh.mesh_data("dUdt")
+        self.dt = math.inf
+        self.time = 0.0
+        self.ncycle = 0
+
+    def _exchange(self, md):
+        any_ = BoundaryType.any
+        StartReceiveBoundBufs(md, any_)
+        SendBoundBufs(md, BoundaryType.nonlocal_)
+        SendBoundBufs(md, BoundaryType.local)
+        if ReceiveBoundBufs(md, BoundaryType.local) != TaskStatus.complete:
+            raise RuntimeError("local boundary buffers were not received")
+        SetBounds(md, BoundaryType.local)
+        ReceiveBoundBufs(md, BoundaryType.nonlocal_)
+        SetBounds(md, BoundaryType.nonlocal_)
+        if self.mesh.multilevel:
+            ProlongateBounds(md, any_)
+
+    def initialize(self, pgen="sinusoid", seed=0):
+        """Problem generator, boundary exchange, FillDerived, first time step
+        (src/mesh/mesh.cpp:745-800, src/driver/driver.cpp:67-70)."""
+        if pgen == "sinusoid":
+            burgers_package.ProblemGeneratorSinusoid(self.base, seed)
+        elif pgen == "vibe":
+            burgers_package.ProblemGenerator(self.base)
+        self._exchange(self.base)
+        burgers_package.CalculateDerived(self.base)
+        self.dt = burgers_package.EstimateTimestepMesh(self.base, self.cfl)
+
+    def stage(self, stage):
+        beta = RK2_BETA[stage - 1]
+        mc0 = self.base if stage == 1 else self.one
+        mc1 = self.one if stage == 1 else self.base
+        burgers_package.CalculateFluxes(mc0, self.recon)
+        Update.FluxDivergence(mc0, self.dudt)
+        Update.AverageIndependentData(mc0, self.base, beta)
+        Update.UpdateIndependentData(mc0, self.dudt, beta * self.dt, mc1)
+        self._exchange(mc1)
+        burgers_package.CalculateDerived(mc1)
+        if stage == 2:
+            return burgers_package.EstimateTimestepMesh(mc1, self.cfl)
+        return None
+
+    def cycle(self):
+        self.stage(1)
+        new_dt = self.stage(2)
+        self.ncycle += 1
+        self.time += self.dt
+        self.dt = min(2.0 * self.dt, new_dt)

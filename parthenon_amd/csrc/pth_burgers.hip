@@ -1,0 +1,405 @@
+// Burgers benchmark (Parthenon-VIBE) stage kernels for gfx950, FP64.
+//
+// Task-level kernels, one launch per reference task over every block of the MeshData:
+//   ReconKernel      burgers_package::CalculateFluxes, reconstruction part
+//                    (benchmarks/burgers/burgers_package.cpp:236-303, recon.hpp:28-99)
+//   FluxKernel       the HLL part (burgers_package.cpp:307-401, burgers_package.hpp:33-43)
+//   FluxDivKernel    Update::FluxDivergence (src/interface/update.cpp:63-84, update.hpp:43-57)
+//   WsumKernel       Update::WeightedSumData (src/interface/update.hpp:71-90)
+//   DerivedKernel    burgers_package::CalculateDerived (burgers_package.cpp:143-167)
+//   DtKernel         burgers_package::EstimateTimestepMesh (burgers_package.cpp:170-200):
+//                    per-workgroup min via 64-lane wavefront shuffles, then one final pass
+// Expression order matches the reference operator by operator and the library is built with
+// -ffp-contract=off, so results are reproducible bit for bit by an IEEE CPU evaluation.
+// Arrays are [block][comp][k][j][i] bricks of (n+2g)^3 with i fastest; consecutive lanes
+// take consecutive i, so every load and store is a coalesced run along x.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+
+#include "pth_burgers.hpp"
+
+namespace pth {
+
+namespace {
+
+constexpr int kWG = 256;
+
+struct Geo {   // brick shape
+  int e0, e1, e2;       // entire extents
+  int lo0, lo1, lo2;    // interior starts
+  int hi0, hi1, hi2;    // interior ends
+  int V;                // components
+  int ndim;
+  std::int64_t cells;   // e0*e1*e2
+};
+
+__device__ __forceinline__ double DMin(double a, double b) { return b < a ? b : a; }
+__device__ __forceinline__ double DMax(double a, double b) { return a < b ? b : a; }
+
+__device__ __forceinline__ double Mc(double dm, double dp) {
+  const double dc = (dm * dp > 0.0) * 0.5 * (dm + dp);
+  return copysign(DMin(fabs(dc), 2.0 * DMin(fabs(dm), fabs(dp))), dc);
+}
+
+__device__ __forceinline__ void LinearRecon(double qm, double q0, double qp, double &ql,
+                                            double &qr) {
+  double dq = qp - q0;
+  dq = 0.5 * Mc(q0 - qm, dq);
+  ql = q0 + dq;
+  qr = q0 - dq;
+}
+
+__device__ __forceinline__ void Weno5Z(double q0, double q1, double q2, double q3, double q4,
+                                       double &ql, double &qr) {
+  const double a00 = 1.0 / 3.0, a01 = -7.0 / 6.0, a02 = 11.0 / 6.0;
+  const double a10 = -1.0 / 6.0, a11 = 5.0 / 6.0, a12 = 1.0 / 3.0;
+  const double a20 = 1.0 / 3.0, a21 = 5.0 / 6.0, a22 = -1.0 / 6.0;
+  const double g0 = 0.1, g1 = 0.6, g2 = 0.3;
+  const double eps = 10.0 * DBL_MIN;
+  const double t13 = 13.0 / 3.0;
+  double a = q0 - 2 * q1 + q2;
+  double b = q0 - 4.0 * q1 + 3.0 * q2;
+  double be0 = t13 * a * a + b * b + eps;
+  a = q1 - 2.0 * q2 + q3;
+  b = q3 - q1;
+  double be1 = t13 * a * a + b * b + eps;
+  a = q2 - 2.0 * q3 + q4;
+  b = q4 - 4.0 * q3 + 3.0 * q2;
+  double be2 = t13 * a * a + b * b + eps;
+  const double tau5 = fabs(be2 - be0);
+  be0 = (be0 + tau5) / be0;
+  be1 = (be1 + tau5) / be1;
+  be2 = (be2 + tau5) / be2;
+  double w0 = g0 * be0 + eps;
+  double w1 = g1 * be1 + eps;
+  double w2 = g2 * be2 + eps;
+  double ws = 1.0 / (w0 + w1 + w2);
+  ql = w0 * (a00 * q0 + a01 * q1 + a02 * q2);
+  ql += w1 * (a10 * q1 + a11 * q2 + a12 * q3);
+  ql += w2 * (a20 * q2 + a21 * q3 + a22 * q4);
+  ql *= ws;
+  const double al = 3.0 * ws * w0 * w1 * w2 / (g2 * w0 * w1 + g1 * w0 * w2 + g0 * w1 * w2) + eps;
+  w0 = g0 * be2 + eps;
+  w1 = g1 * be1 + eps;
+  w2 = g2 * be0 + eps;
+  ws = 1.0 / (w0 + w1 + w2);
+  qr = w0 * (a00 * q4 + a01 * q3 + a02 * q2);
+  qr += w1 * (a10 * q3 + a11 * q2 + a12 * q1);
+  qr += w2 * (a20 * q2 + a21 * q1 + a22 * q0);
+  qr *= ws;
+  const double ar = 3.0 * ws * w0 * w1 * w2 / (g2 * w0 * w1 + g1 * w0 * w2 + g0 * w1 * w2) + eps;
+  double dq = q3 - q2;
+  dq = 0.5 * Mc(q2 - q1, dq);
+  const double alin = 2.0 * al * ar / (al + ar);
+  ql = alin * ql + (1.0 - alin) * (q2 + dq);
+  qr = alin * qr + (1.0 - alin) * (q2 - dq);
+}
+
+// One thread per (block, comp, k, j, i) of the box extended by one cell around the interior.
+// face layout: [6][block][comp][k][j][i] for Ulx, Urx, Uly, Ury, Ulz, Urz.
+__global__ void __launch_bounds__(kWG)
+    ReconKernel(const double *__restrict__ U, double *__restrict__ face, Geo G, int nblocks,
+                int recon) {
+  const int dk = G.ndim > 2 ? 1 : 0, dj = G.ndim > 1 ? 1 : 0;
+  const int b0 = G.hi0 - G.lo0 + 3, b1 = G.hi1 - G.lo1 + 1 + 2 * dj,
+            b2 = G.hi2 - G.lo2 + 1 + 2 * dk;
+  const std::int64_t per = static_cast<std::int64_t>(b0) * b1 * b2;
+  const std::int64_t total = per * G.V * nblocks;
+  const std::int64_t slab = G.cells * G.V * nblocks;
+  for (std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * kWG + threadIdx.x; t < total;
+       t += static_cast<std::int64_t>(gridDim.x) * kWG) {
+    std::int64_t r = t;
+    const int i = static_cast<int>(r % b0) + G.lo0 - 1;
+    r /= b0;
+    const int j = static_cast<int>(r % b1) + G.lo1 - dj;
+    r /= b1;
+    const int k = static_cast<int>(r % b2) + G.lo2 - dk;
+    r /= b2;  // r = block * V + comp
+    const double *q = U + r * G.cells;
+    const std::int64_t c = (static_cast<std::int64_t>(k) * G.e1 + j) * G.e0 + i;
+    double *f = face + r * G.cells;
+    const bool kin = k >= G.lo2 && k <= G.hi2, jin = j >= G.lo1 && j <= G.hi1,
+               iin = i >= G.lo0 && i <= G.hi0;
+    const bool xrec = kin && jin;
+    const bool yrec = kin && (G.ndim > 1) && iin;
+    const bool zrec = jin && (G.ndim > 2) && iin;
+    const std::int64_t sj = G.e0, sk = static_cast<std::int64_t>(G.e0) * G.e1;
+    double ql, qr;
+    if (xrec) {
+      if (recon == 0)
+        Weno5Z(q[c - 2], q[c - 1], q[c], q[c + 1], q[c + 2], ql, qr);
+      else
+        LinearRecon(q[c - 1], q[c], q[c + 1], ql, qr);
+      f[0 * slab + c + 1] = ql;
+      f[1 * slab + c] = qr;
+    }
+    if (yrec) {
+      if (recon == 0)
+        Weno5Z(q[c - 2 * sj], q[c - sj], q[c], q[c + sj], q[c + 2 * sj], ql, qr);
+      else
+        LinearRecon(q[c - sj], q[c], q[c + sj], ql, qr);
+      f[2 * slab + c + sj] = ql;
+      f[3 * slab + c] = qr;
+    }
+    if (zrec) {
+      if (recon == 0)
+        Weno5Z(q[c - 2 * sk], q[c - sk], q[c], q[c + sk], q[c + 2 * sk], ql, qr);
+      else
+        LinearRecon(q[c - sk], q[c], q[c + sk], ql, qr);
+      f[4 * slab + c + sk] = ql;
+      f[5 * slab + c] = qr;
+    }
+  }
+}
+
+// One thread per (block, k, j, i) face position; HLL velocity flux then scalar fluxes that
+// reuse the face's signal speeds. flux layout: [3][block][comp][k][j][i].
+__global__ void __launch_bounds__(kWG)
+    FluxKernel(const double *__restrict__ face, double *__restrict__ flux, Geo G, int nblocks) {
+  const int dk = G.ndim > 2 ? 1 : 0, dj = G.ndim > 1 ? 1 : 0;
+  const int b0 = G.hi0 - G.lo0 + 2, b1 = G.hi1 - G.lo1 + 1 + dj, b2 = G.hi2 - G.lo2 + 1 + dk;
+  const std::int64_t per = static_cast<std::int64_t>(b0) * b1 * b2;
+  const std::int64_t total = per * nblocks;
+  const std::int64_t slab = G.cells * G.V * nblocks;
+  for (std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * kWG + threadIdx.x; t < total;
+       t += static_cast<std::int64_t>(gridDim.x) * kWG) {
+    std::int64_t r = t;
+    const int i = static_cast<int>(r % b0) + G.lo0;
+    r /= b0;
+    const int j = static_cast<int>(r % b1) + G.lo1;
+    r /= b1;
+    const int k = static_cast<int>(r % b2) + G.lo2;
+    r /= b2;  // block
+    const std::int64_t c = (static_cast<std::int64_t>(k) * G.e1 + j) * G.e0 + i;
+    const std::int64_t base = r * G.V * G.cells + c;
+    const bool xflux = k <= G.hi2 && j <= G.hi1;
+    const bool yflux = G.ndim > 1 && k <= G.hi2 && i <= G.hi0;
+    const bool zflux = G.ndim > 2 && j <= G.hi1 && i <= G.hi0;
+    for (int d = 0; d < 3; ++d) {
+      const bool doit = d == 0 ? xflux : (d == 1 ? yflux : zflux);
+      if (!doit) continue;
+      const double *L = face + (2 * d) * slab + base;
+      const double *R = face + (2 * d + 1) * slab + base;
+      double *F = flux + d * slab + base;
+      const double upl = L[d * G.cells], upr = R[d * G.cells];
+      const double sl = DMin(DMin(upl, upr), 0.0);
+      const double sr = DMax(DMax(upl, upr), 0.0);
+      const double islsr = 1.0 / (sr - sl + (sl * sr == 0.0));
+#pragma unroll
+      for (int n = 0; n < 3; ++n) {
+        const double ul = L[n * G.cells], ur = R[n * G.cells];
+        F[n * G.cells] = 0.5 * (sr * ul * upl - sl * ur * upr + sl * sr * (ur - ul)) * islsr;
+      }
+      for (int n = 3; n < G.V; ++n) {
+        const double ql = L[n * G.cells], qr = R[n * G.cells];
+        F[n * G.cells] =
+            (sr * upl * ql - sl * upr * qr + sl * sr * (qr - ql)) / (sr - sl + (sl * sr == 0.0));
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kWG)
+    FluxDivKernel(const double *__restrict__ flux, double *__restrict__ dudt,
+                  const BlockCoefs *__restrict__ coef, Geo G, int nblocks) {
+  const int n0 = G.hi0 - G.lo0 + 1, n1 = G.hi1 - G.lo1 + 1, n2 = G.hi2 - G.lo2 + 1;
+  const std::int64_t total = static_cast<std::int64_t>(n0) * n1 * n2 * G.V * nblocks;
+  const std::int64_t slab = G.cells * G.V * nblocks;
+  const std::int64_t sj = G.e0, sk = static_cast<std::int64_t>(G.e0) * G.e1;
+  for (std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * kWG + threadIdx.x; t < total;
+       t += static_cast<std::int64_t>(gridDim.x) * kWG) {
+    std::int64_t r = t;
+    const int i = static_cast<int>(r % n0) + G.lo0;
+    r /= n0;
+    const int j = static_cast<int>(r % n1) + G.lo1;
+    r /= n1;
+    const int k = static_cast<int>(r % n2) + G.lo2;
+    r /= n2;  // block * V + comp
+    const int b = static_cast<int>(r / G.V);
+    const BlockCoefs &C = coef[b];
+    const std::int64_t c = r * G.cells + (static_cast<std::int64_t>(k) * G.e1 + j) * G.e0 + i;
+    const double *fx = flux + c, *fy = flux + slab + c, *fz = flux + 2 * slab + c;
+    double du = (C.area[0] * fx[1] - C.area[0] * fx[0]);
+    if (G.ndim >= 2) du += (C.area[1] * fy[sj] - C.area[1] * fy[0]);
+    if (G.ndim == 3) du += (C.area[2] * fz[sk] - C.area[2] * fz[0]);
+    dudt[c] = -du / C.vol;
+  }
+}
+
+__global__ void __launch_bounds__(kWG)
+    WsumKernel(const double *__restrict__ x, const double *__restrict__ y, double w1, double w2,
+               double *__restrict__ z, std::int64_t n) {
+  for (std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * kWG + threadIdx.x; t < n;
+       t += static_cast<std::int64_t>(gridDim.x) * kWG)
+    z[t] = w1 * x[t] + w2 * y[t];
+}
+
+__global__ void __launch_bounds__(kWG)
+    DerivedKernel(const double *__restrict__ U, double *__restrict__ out, Geo G, int nblocks) {
+  const int n0 = G.hi0 - G.lo0 + 1, n1 = G.hi1 - G.lo1 + 1, n2 = G.hi2 - G.lo2 + 1;
+  const std::int64_t total = static_cast<std::int64_t>(n0) * n1 * n2 * nblocks;
+  for (std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * kWG + threadIdx.x; t < total;
+       t += static_cast<std::int64_t>(gridDim.x) * kWG) {
+    std::int64_t r = t;
+    const int i = static_cast<int>(r % n0) + G.lo0;
+    r /= n0;
+    const int j = static_cast<int>(r % n1) + G.lo1;
+    r /= n1;
+    const int k = static_cast<int>(r % n2) + G.lo2;
+    r /= n2;
+    const std::int64_t c = (static_cast<std::int64_t>(k) * G.e1 + j) * G.e0 + i;
+    const double *u = U + r * G.V * G.cells + c;
+    const double u1 = u[0], u2 = u[G.cells], u3 = u[2 * G.cells], d = u[3 * G.cells];
+    out[r * G.cells + c] = 0.5 * d * (u1 * u1 + u2 * u2 + u3 * u3);
+  }
+}
+
+__device__ __forceinline__ double WaveMin(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = DMin(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Per-workgroup minimum of the CFL time step over interior cells; partial[blockIdx.x].
+__global__ void __launch_bounds__(kWG)
+    DtKernel(const double *__restrict__ U, const BlockCoefs *__restrict__ coef, Geo G,
+             int nblocks, double *__restrict__ partial) {
+  __shared__ double wmin[kWG / 64];
+  const int n0 = G.hi0 - G.lo0 + 1, n1 = G.hi1 - G.lo1 + 1, n2 = G.hi2 - G.lo2 + 1;
+  const std::int64_t total = static_cast<std::int64_t>(n0) * n1 * n2 * nblocks;
+  double m = DBL_MAX;
+  for (std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * kWG + threadIdx.x; t < total;
+       t += static_cast<std::int64_t>(gridDim.x) * kWG) {
+    std::int64_t r = t;
+    const int i = static_cast<int>(r % n0) + G.lo0;
+    r /= n0;
+    const int j = static_cast<int>(r % n1) + G.lo1;
+    r /= n1;
+    const int k = static_cast<int>(r % n2) + G.lo2;
+    r /= n2;
+    const BlockCoefs &C = coef[r];
+    const double *u = U + r * G.V * G.cells + (static_cast<std::int64_t>(k) * G.e1 + j) * G.e0 + i;
+    double s = fabs(u[0]) / C.dx[0];
+    if (G.ndim > 1) s = s + fabs(u[G.cells]) / C.dx[1];
+    if (G.ndim > 2) s = s + fabs(u[2 * G.cells]) / C.dx[2];
+    m = DMin(m, 1.0 / s);
+  }
+  m = WaveMin(m);
+  if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v = wmin[0];
+#pragma unroll
+    for (int w = 1; w < kWG / 64; ++w) v = DMin(v, wmin[w]);
+    partial[blockIdx.x] = v;
+  }
+}
+
+__global__ void __launch_bounds__(64) MinFinalKernel(const double *__restrict__ partial, int n,
+                                                     double *__restrict__ out) {
+  double m = DBL_MAX;
+  for (int t = threadIdx.x; t < n; t += 64) m = DMin(m, partial[t]);
+  m = WaveMin(m);
+  if (threadIdx.x == 0) out[0] = m;
+}
+
+int Grid(std::int64_t total) {
+  const std::int64_t want = (total + kWG - 1) / kWG;
+  return static_cast<int>(want < 8192 ? want : 8192);
+}
+
+Geo MakeGeo(const Topology &t, int V) {
+  Geo g;
+  g.e0 = t.fine.ent(0);
+  g.e1 = t.fine.ent(1);
+  g.e2 = t.fine.ent(2);
+  g.lo0 = t.fine.lo(0);
+  g.lo1 = t.fine.lo(1);
+  g.lo2 = t.fine.lo(2);
+  g.hi0 = t.fine.hi(0);
+  g.hi1 = t.fine.hi(1);
+  g.hi2 = t.fine.hi(2);
+  g.V = V;
+  g.ndim = t.ndim;
+  g.cells = t.fine.cells();
+  return g;
+}
+
+}  // namespace
+
+pth_status BurgersRecon(const Topology &t, int V, const double *U, double *face, int recon,
+                        hipStream_t s) {
+  ProfScope prof("recon", s);
+  const Geo G = MakeGeo(t, V);
+  const int nb = static_cast<int>(t.blocks.size());
+  const int dk = G.ndim > 2 ? 1 : 0, dj = G.ndim > 1 ? 1 : 0;
+  const std::int64_t total = static_cast<std::int64_t>(G.hi0 - G.lo0 + 3) *
+                             (G.hi1 - G.lo1 + 1 + 2 * dj) * (G.hi2 - G.lo2 + 1 + 2 * dk) * V * nb;
+  hipLaunchKernelGGL(ReconKernel, dim3(Grid(total)), dim3(kWG), 0, s, U, face, G, nb, recon);
+  PTH_HIP(hipGetLastError());
+  return PTH_COMPLETE;
+}
+
+pth_status BurgersFlux(const Topology &t, int V, const double *face, double *flux,
+                       hipStream_t s) {
+  ProfScope prof("hll_flux", s);
+  const Geo G = MakeGeo(t, V);
+  const int nb = static_cast<int>(t.blocks.size());
+  const int dk = G.ndim > 2 ? 1 : 0, dj = G.ndim > 1 ? 1 : 0;
+  const std::int64_t total = static_cast<std::int64_t>(G.hi0 - G.lo0 + 2) *
+                             (G.hi1 - G.lo1 + 1 + dj) * (G.hi2 - G.lo2 + 1 + dk) * nb;
+  hipLaunchKernelGGL(FluxKernel, dim3(Grid(total)), dim3(kWG), 0, s, face, flux, G, nb);
+  PTH_HIP(hipGetLastError());
+  return PTH_COMPLETE;
+}
+
+pth_status FluxDivergence(const Topology &t, int V, const double *flux, double *dudt,
+                          const BlockCoefs *coef, hipStream_t s) {
+  ProfScope prof("flux_div", s);
+  const Geo G = MakeGeo(t, V);
+  const int nb = static_cast<int>(t.blocks.size());
+  const std::int64_t total =
+      static_cast<std::int64_t>(t.fine.n[0]) * t.fine.n[1] * t.fine.n[2] * V * nb;
+  hipLaunchKernelGGL(FluxDivKernel, dim3(Grid(total)), dim3(kWG), 0, s, flux, dudt, coef, G, nb);
+  PTH_HIP(hipGetLastError());
+  return PTH_COMPLETE;
+}
+
+pth_status WeightedSum(const double *x, const double *y, double w1, double w2, double *z,
+                       std::int64_t n, hipStream_t s) {
+  if (n == 0) return PTH_COMPLETE;
+  ProfScope prof("weighted_sum", s);
+  hipLaunchKernelGGL(WsumKernel, dim3(Grid(n)), dim3(kWG), 0, s, x, y, w1, w2, z, n);
+  PTH_HIP(hipGetLastError());
+  return PTH_COMPLETE;
+}
+
+pth_status BurgersDerived(const Topology &t, int V, const double *U, double *out,
+                          hipStream_t s) {
+  ProfScope prof("derived", s);
+  const Geo G = MakeGeo(t, V);
+  const int nb = static_cast<int>(t.blocks.size());
+  const std::int64_t total =
+      static_cast<std::int64_t>(t.fine.n[0]) * t.fine.n[1] * t.fine.n[2] * nb;
+  hipLaunchKernelGGL(DerivedKernel, dim3(Grid(total)), dim3(kWG), 0, s, U, out, G, nb);
+  PTH_HIP(hipGetLastError());
+  return PTH_COMPLETE;
+}
+
+pth_status BurgersDtMin(const Topology &t, int V, const double *U, const BlockCoefs *coef,
+                        double *partial, int npartial, double *out, hipStream_t s) {
+  ProfScope prof("dt_reduce", s);
+  const Geo G = MakeGeo(t, V);
+  const int nb = static_cast<int>(t.blocks.size());
+  const std::int64_t total =
+      static_cast<std::int64_t>(t.fine.n[0]) * t.fine.n[1] * t.fine.n[2] * nb;
+  int grid = Grid(total);
+  if (grid > npartial) grid = npartial;
+  hipLaunchKernelGGL(DtKernel, dim3(grid), dim3(kWG), 0, s, U, coef, G, nb, partial);
+  hipLaunchKernelGGL(MinFinalKernel, dim3(1), dim3(64), 0, s, partial, grid, out);
+  PTH_HIP(hipGetLastError());
+  return PTH_COMPLETE;
+}
+
+}  // namespace pth

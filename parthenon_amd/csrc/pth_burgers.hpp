@@ -1,0 +1,25 @@
+// Host launchers of the Burgers stage kernels (pth_burgers.hip).
+#pragma once
+#include "pth_internal.hpp"
+
+namespace pth {
+
+pth_status BurgersRecon(const Topology &t, int V, const double *U, double *face, int recon,
+                        hipStream_t s);
+pth_status BurgersFlux(const Topology &t, int V, const double *face, double *flux,
+                       hipStream_t s);
+pth_status FluxDivergence(const Topology &t, int V, const double *flux, double *dudt,
+                          const BlockCoefs *coef, hipStream_t s);
+pth_status WeightedSum(const double *x, const double *y, double w1, double w2, double *z,
+                       std::int64_t n, hipStream_t s);
+pth_status BurgersDerived(const Topology &t, int V, const double *U, double *out,
+                          hipStream_t s);
+pth_status BurgersDtMin(const Topology &t, int V, const double *U, const BlockCoefs *coef,
+                        double *partial, int npartial, double *out, hipStream_t s);
+
+pth_status LaunchRestrictT(const Table &t, const Topology &topo, const DeviceBlockGeom *geom,
+                           hipStream_t s);
+pth_status LaunchProlongT(const Table &t, const Topology &topo, const DeviceBlockGeom *geom,
+                          hipStream_t s, bool minmod);
+
+}  // namespace pth

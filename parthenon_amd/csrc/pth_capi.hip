@@ -1,0 +1,880 @@
+// C ABI of parthenon_amd: mesh and MeshData management, boundary communication tasks and
+// the Burgers package tasks. Host-side decisions (table builds, buffer-state flips) happen
+// here; every data operation is a device kernel on the mesh's stream.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "pth_burgers.hpp"
+
+namespace pth {
+
+namespace {
+thread_local std::string g_error;
+}
+
+void SetError(const std::string &msg) { g_error = msg; }
+const char *ErrorText() { return g_error.c_str(); }
+
+Profiler &Prof() {
+  static Profiler p;
+  return p;
+}
+
+hipEvent_t Profiler::Take() {
+  if (pool.empty()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+  }
+  hipEvent_t e = pool.back();
+  pool.pop_back();
+  return e;
+}
+
+void Profiler::Clear() {
+  for (auto &r : records) {
+    pool.push_back(r.start);
+    pool.push_back(r.stop);
+  }
+  records.clear();
+}
+
+ProfScope::ProfScope(const char *name, hipStream_t s) : name_(name), s_(s) {
+  if (!Prof().on) return;
+  a_ = Prof().Take();
+  if (a_) (void)hipEventRecord(a_, s_);
+}
+
+ProfScope::~ProfScope() {
+  if (!a_) return;
+  hipEvent_t b = Prof().Take();
+  if (!b) return;
+  (void)hipEventRecord(b, s_);
+  Prof().records.push_back(ProfRecord{name_, a_, b});
+}
+
+MeshData::~MeshData() {
+  for (auto &s : store) {
+    if (!s.owned) continue;
+    if (s.data) (void)hipFree(s.data);
+    if (s.coarse) (void)hipFree(s.coarse);
+    if (s.flux) (void)hipFree(s.flux);
+  }
+  for (auto &c : cache) {
+    c.pack.Free();
+    c.unpack.Free();
+    c.restrict_send.Free();
+    c.restrict_set.Free();
+    c.prolong.Free();
+  }
+}
+
+Mesh::~Mesh() {
+  data.clear();
+  for (auto &s : one_copy) {
+    if (s.data) (void)hipFree(s.data);
+    if (s.coarse) (void)hipFree(s.coarse);
+    if (s.flux) (void)hipFree(s.flux);
+  }
+  for (auto *a : arena)
+    if (a) (void)hipFree(a);
+  if (geom) (void)hipFree(geom);
+  if (coef) (void)hipFree(coef);
+  if (face) (void)hipFree(face);
+  if (scratch) (void)hipFree(scratch);
+  if (pinned) (void)hipHostFree(pinned);
+}
+
+int Mesh::FieldIndex(const std::string &n) const {
+  for (size_t q = 0; q < fields.size(); ++q)
+    if (fields[q].name == n) return static_cast<int>(q);
+  return -1;
+}
+
+}  // namespace pth
+
+using namespace pth;
+
+namespace {
+
+pth_status Alloc(double **p, std::int64_t n) {
+  *p = nullptr;
+  if (n <= 0) return PTH_COMPLETE;
+  PTH_HIP(hipMalloc(p, sizeof(double) * n));
+  PTH_HIP(hipMemset(*p, 0, sizeof(double) * n));
+  return PTH_COMPLETE;
+}
+
+pth_status AllocStore(const Mesh &m, const FieldSpec &f, FieldStore &s) {
+  const std::int64_t nb = static_cast<std::int64_t>(m.topo.blocks.size());
+  s.ndata = nb * f.ncomp * m.topo.fine.cells();
+  s.ncoarse = m.topo.multilevel ? nb * f.ncomp * m.topo.coarse.cells() : 0;
+  s.nflux = (f.flags & PTH_WITH_FLUXES) ? 3 * s.ndata : 0;
+  if (Alloc(&s.data, s.ndata) != PTH_COMPLETE) return PTH_FAIL;
+  if (Alloc(&s.coarse, s.ncoarse) != PTH_COMPLETE) return PTH_FAIL;
+  if (Alloc(&s.flux, s.nflux) != PTH_COMPLETE) return PTH_FAIL;
+  return PTH_COMPLETE;
+}
+
+Region MakeRegion(double *a, double *b, const Box &box, int nc, int gid, const Extent &ea,
+                  const Extent *eb) {
+  Region r;
+  std::memset(&r, 0, sizeof(r));
+  r.a = a;
+  r.b = b;
+  r.nc = nc;
+  r.gid = gid;
+  for (int d = 0; d < 3; ++d) {
+    r.lo[d] = box.lo[d];
+    r.n[d] = box.hi[d] - box.lo[d] + 1;
+    r.magic[d] = Magic(r.n[d]);
+  }
+  r.astride[0] = ea.ent(0);
+  r.astride[1] = ea.ent(0) * ea.ent(1);
+  r.astride[2] = static_cast<int>(ea.cells());
+  if (eb) {
+    r.bstride[0] = eb->ent(0);
+    r.bstride[1] = eb->ent(0) * eb->ent(1);
+    r.bstride[2] = static_cast<int>(eb->cells());
+  }
+  return r;
+}
+
+bool IsExchangeType(int bt) { return bt == PTH_BT_ANY || bt == PTH_BT_LOCAL || bt == PTH_BT_NONLOCAL; }
+
+}  // namespace
+
+namespace pth {
+
+pth_status Mesh::Plan() {
+  if (planned) return PTH_COMPLETE;
+  const std::string err = topo.Finalize();
+  if (!err.empty()) {
+    SetError(err);
+    return PTH_FAIL;
+  }
+  planned = true;
+  return PTH_COMPLETE;
+}
+
+pth_status Mesh::Finalize() {
+  if (finalized) {
+    SetError("mesh already finalized");
+    return PTH_FAIL;
+  }
+  if (Plan() != PTH_COMPLETE) return PTH_FAIL;
+  const int nb = static_cast<int>(topo.blocks.size());
+  // per-block geometry for prolongation and metrics for the stencils
+  std::vector<DeviceBlockGeom> g(nb);
+  std::vector<BlockCoefs> c(nb);
+  for (int b = 0; b < nb; ++b) {
+    const Block &B = topo.blocks[b];
+    for (int d = 0; d < 3; ++d) {
+      g[b].x0[d] = B.x0[d];
+      g[b].dx[d] = B.dx[d];
+      g[b].cx0[d] = B.cx0[d];
+      g[b].cdx[d] = B.cdx[d];
+      c[b].dx[d] = B.dx[d];
+      c[b].area[d] = B.area[d];
+    }
+    c[b].vol = B.vol;
+    c[b].pad = 0.0;
+  }
+  PTH_HIP(hipMalloc(&geom, sizeof(DeviceBlockGeom) * nb));
+  PTH_HIP(hipMemcpy(geom, g.data(), sizeof(DeviceBlockGeom) * nb, hipMemcpyHostToDevice));
+  PTH_HIP(hipMalloc(&coef, sizeof(BlockCoefs) * nb));
+  PTH_HIP(hipMemcpy(coef, c.data(), sizeof(BlockCoefs) * nb, hipMemcpyHostToDevice));
+  // OneCopy fields: one storage shared by every MeshData
+  one_copy.assign(fields.size(), FieldStore());
+  for (size_t f = 0; f < fields.size(); ++f)
+    if (fields[f].flags & PTH_ONE_COPY)
+      if (AllocStore(*this, fields[f], one_copy[f]) != PTH_COMPLETE) return PTH_FAIL;
+  // communication channels: one contiguous slice per (sender, receiver, field, location)
+  arena.assign(fields.size(), nullptr);
+  arena_size.assign(fields.size(), 0);
+  channel_offset.clear();
+  for (size_t f = 0; f < fields.size(); ++f) {
+    if (!(fields[f].flags & PTH_FILL_GHOST)) continue;
+    std::int64_t off = 0;
+    std::map<Channel, std::int64_t> slice;
+    for (int b = 0; b < nb; ++b) {
+      const Block &B = topo.blocks[b];
+      for (const Link &l : B.links) {
+        const Side side = l.is_cell() ? Side::kSendInterior : Side::kSendBoundary;
+        const Box box = topo.Indices(B, l, side, false);
+        const Channel key(b, l.gid, static_cast<int>(f), l.loc_index());
+        channel_offset[key] = off;
+        slice[key] = box.count() * fields[f].ncomp;
+        off += slice[key];
+      }
+    }
+    // every receiving region must match the size of the slice its sender packs
+    for (int b = 0; b < nb; ++b) {
+      const Block &B = topo.blocks[b];
+      for (const Link &l : B.links) {
+        const Side side = l.is_cell() ? Side::kRecvInterior : Side::kRecvBoundary;
+        const Box box = topo.Indices(B, l, side, false);
+        const Channel key(l.gid, b, static_cast<int>(f), l.reverse_index());
+        auto it = slice.find(key);
+        if (it == slice.end() || it->second != box.count() * fields[f].ncomp) {
+          SetError("receiving region does not match the slice its sender packs");
+          return PTH_FAIL;
+        }
+      }
+    }
+    arena_size[f] = off;
+    if (Alloc(&arena[f], off) != PTH_COMPLETE) return PTH_FAIL;
+  }
+  scratch_n = 8192;
+  PTH_HIP(hipMalloc(&scratch, sizeof(double) * (scratch_n + 8)));
+  PTH_HIP(hipHostMalloc(&pinned, sizeof(double) * 8, hipHostMallocDefault));
+  finalized = true;
+  return PTH_COMPLETE;
+}
+
+MeshData *Mesh::Get(const std::string &name) {
+  auto it = data.find(name);
+  if (it != data.end()) return it->second.get();
+  std::unique_ptr<MeshData> md(new MeshData);
+  md->mesh = this;
+  md->name = name;
+  md->store.assign(fields.size(), FieldStore());
+  for (size_t f = 0; f < fields.size(); ++f) {
+    if (fields[f].flags & PTH_ONE_COPY) {
+      md->store[f] = one_copy[f];
+      md->store[f].owned = false;
+    } else if (AllocStore(*this, fields[f], md->store[f]) != PTH_COMPLETE) {
+      return nullptr;
+    }
+  }
+  MeshData *raw = md.get();
+  data[name] = std::move(md);
+  return raw;
+}
+
+// Builds the pack / unpack / restriction / prolongation region tables of md for bound type bt.
+pth_status Mesh::BuildCache(MeshData &md, int bt) {
+  BoundCache &c = md.cache[bt];
+  if (c.built) return PTH_COMPLETE;
+  c.pack.host.clear();
+  c.unpack.host.clear();
+  c.restrict_send.host.clear();
+  c.restrict_set.host.clear();
+  c.prolong.host.clear();
+  c.nbuffers = 0;
+  if (bt == PTH_BT_NONLOCAL) {  // one rank: no neighbour is remote
+    c.built = true;
+    return PTH_COMPLETE;
+  }
+  if (!IsExchangeType(bt)) {
+    SetError("bound type not available for cell-variable exchange in this build");
+    return PTH_FAIL;
+  }
+  const Extent &fe = topo.fine, &ce = topo.coarse;
+  for (int b = 0; b < static_cast<int>(topo.blocks.size()); ++b) {
+    const Block &B = topo.blocks[b];
+    for (size_t f = 0; f < fields.size(); ++f) {
+      if (!(fields[f].flags & PTH_FILL_GHOST)) continue;
+      const int nc = fields[f].ncomp;
+      FieldStore &S = md.store[f];
+      double *fine = S.data + static_cast<std::int64_t>(b) * nc * fe.cells();
+      double *crs = S.coarse ? S.coarse + static_cast<std::int64_t>(b) * nc * ce.cells() : nullptr;
+      for (const Link &l : B.links) {
+        const bool coarser = l.level < B.at.level;
+        const Extent &ve = coarser ? ce : fe;
+        double *var = coarser ? crs : fine;
+        // send region
+        const Side ss = l.is_cell() ? Side::kSendInterior : Side::kSendBoundary;
+        const Box sb = topo.Indices(B, l, ss, false);
+        const std::int64_t so =
+            channel_offset.at(Channel(b, l.gid, static_cast<int>(f), l.loc_index()));
+        c.pack.host.push_back(MakeRegion(var, arena[f] + so, sb, nc, b, ve, nullptr));
+        // receive region
+        const Side rs = l.is_cell() ? Side::kRecvInterior : Side::kRecvBoundary;
+        const Box rb = topo.Indices(B, l, rs, false);
+        const std::int64_t ro =
+            channel_offset.at(Channel(l.gid, b, static_cast<int>(f), l.reverse_index()));
+        c.unpack.host.push_back(MakeRegion(var, arena[f] + ro, rb, nc, b, ve, nullptr));
+        ++c.nbuffers;
+        if (!topo.multilevel) continue;
+        if (coarser) {
+          c.restrict_send.host.push_back(MakeRegion(
+              fine, crs, topo.Indices(B, l, Side::kSendBoundary, true), nc, b, fe, &ce));
+          c.prolong.host.push_back(MakeRegion(
+              fine, crs, topo.Indices(B, l, Side::kRecvBoundary, true), nc, b, fe, &ce));
+        } else if (B.coarser_neighbor && B.at.level > 0) {
+          c.restrict_set.host.push_back(MakeRegion(
+              fine, crs, topo.Indices(B, l, Side::kRecvBoundary, true), nc, b, fe, &ce));
+        }
+      }
+    }
+  }
+  for (Table *t : {&c.pack, &c.unpack, &c.restrict_send, &c.restrict_set, &c.prolong})
+    if (t->Upload() != PTH_COMPLETE) return PTH_FAIL;
+  c.built = true;
+  return PTH_COMPLETE;
+}
+
+}  // namespace pth
+
+namespace {
+
+int Bt(int bt) { return bt == PTH_BT_LOCAL ? PTH_BT_ANY : bt; }  // same channel group
+
+pth_status NeedFinal(const Mesh *m) {
+  if (!m || !m->finalized) {
+    SetError("mesh is not finalized");
+    return PTH_FAIL;
+  }
+  return PTH_COMPLETE;
+}
+
+// SplitMix64 (Steele, Lea, Flood 2014): the seeded generator of the synthetic inputs
+struct SplitMix64 {
+  std::uint64_t s;
+  std::uint64_t next() {
+    std::uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+  }
+  double uniform() { return static_cast<double>(next() >> 11) * 0x1.0p-53; }
+};
+
+}  // namespace
+
+
+extern "C" {
+
+const char *pth_last_error(void) { return ErrorText(); }
+const char *pth_version(void) { return "parthenon_amd 0.1 (gfx950)"; }
+
+pth_mesh *pth_mesh_create(const int nx[3], const int bnx[3], int nghost, const double xmin[3],
+                          const double xmax[3], const int periodic[3]) {
+  Mesh *m = new Mesh;
+  const std::string err = m->topo.Init(nx, bnx, nghost, xmin, xmax, periodic);
+  if (!err.empty()) {
+    SetError(err);
+    delete m;
+    return nullptr;
+  }
+  return reinterpret_cast<pth_mesh *>(m);
+}
+
+void pth_mesh_destroy(pth_mesh *m) { delete Impl(m); }
+
+pth_status pth_mesh_refine_box(pth_mesh *mh, const double box[6], int level) {
+  Mesh *m = Impl(mh);
+  if (m->planned || level < 1) {
+    SetError("refinement must precede finalize and have level >= 1");
+    return PTH_FAIL;
+  }
+  m->topo.RefineBox(box, level);
+  return PTH_COMPLETE;
+}
+
+pth_status pth_mesh_refine_sphere(pth_mesh *mh, const double center[3], double radius,
+                                  int level) {
+  Mesh *m = Impl(mh);
+  if (m->planned || level < 1) {
+    SetError("refinement must precede finalize and have level >= 1");
+    return PTH_FAIL;
+  }
+  m->topo.RefineSphere(center, radius, level);
+  return PTH_COMPLETE;
+}
+
+pth_status pth_mesh_add_field(pth_mesh *mh, const char *name, int ncomp, unsigned flags) {
+  Mesh *m = Impl(mh);
+  if (m->finalized) {
+    SetError("fields must be added before finalize");
+    return PTH_FAIL;
+  }
+  if (ncomp < 1 || m->FieldIndex(name) >= 0) {
+    SetError(std::string("bad or duplicate field ") + name);
+    return PTH_FAIL;
+  }
+  m->fields.push_back(FieldSpec{name, ncomp, flags});
+  return PTH_COMPLETE;
+}
+
+pth_status pth_mesh_plan(pth_mesh *mh) { return Impl(mh)->Plan(); }
+
+pth_status pth_mesh_finalize(pth_mesh *mh) { return Impl(mh)->Finalize(); }
+
+int pth_mesh_neighbors(const pth_mesh *mh, int gid, int64_t *out, int max) {
+  const Mesh *m = Impl(mh);
+  if (!m->planned || gid < 0 || gid >= static_cast<int>(m->topo.blocks.size())) return -1;
+  int n = 0;
+  for (const Link &l : m->topo.blocks[gid].links) {
+    if (n >= max) break;
+    int64_t *o = out + 8 * n;
+    o[0] = l.gid;
+    o[1] = l.level;
+    for (int d = 0; d < 3; ++d) {
+      o[2 + d] = l.seen[d];
+      o[5 + d] = l.off[d];
+    }
+    ++n;
+  }
+  return n;
+}
+
+int pth_mesh_region_boxes(const pth_mesh *mh, int gid, int *out, int max) {
+  const Mesh *m = Impl(mh);
+  if (!m->planned || gid < 0 || gid >= static_cast<int>(m->topo.blocks.size())) return -1;
+  const Block &B = m->topo.blocks[gid];
+  int n = 0;
+  auto put = [&](int kind, const Box &b) {
+    if (n >= max) return;
+    int *o = out + 7 * n;
+    o[0] = kind;
+    for (int d = 0; d < 3; ++d) {
+      o[1 + 2 * d] = b.lo[d];
+      o[2 + 2 * d] = b.hi[d];
+    }
+    ++n;
+  };
+  for (const Link &l : B.links) {
+    const bool coarser = l.level < B.at.level;
+    put(0, m->topo.Indices(B, l, l.is_cell() ? Side::kSendInterior : Side::kSendBoundary, false));
+    put(1, m->topo.Indices(B, l, l.is_cell() ? Side::kRecvInterior : Side::kRecvBoundary, false));
+    if (!m->topo.multilevel) continue;
+    if (coarser) {
+      put(2, m->topo.Indices(B, l, Side::kSendBoundary, true));
+      put(3, m->topo.Indices(B, l, Side::kRecvBoundary, true));
+    } else if (B.coarser_neighbor && B.at.level > 0) {
+      put(4, m->topo.Indices(B, l, Side::kRecvBoundary, true));
+    }
+  }
+  return n;
+}
+
+pth_status pth_mesh_set_stream(pth_mesh *mh, void *s) {
+  Impl(mh)->stream = static_cast<hipStream_t>(s);
+  return PTH_COMPLETE;
+}
+
+pth_status pth_mesh_synchronize(pth_mesh *mh) {
+  PTH_HIP(hipStreamSynchronize(Impl(mh)->stream));
+  return PTH_COMPLETE;
+}
+
+pth_status pth_mesh_set_pack_mode(pth_mesh *mh, int mode) {
+  if (mode != PTH_PACK_PERSISTENT && mode != PTH_PACK_PER_BUFFER) {
+    SetError("unknown pack mode");
+    return PTH_FAIL;
+  }
+  Impl(mh)->pack_mode = mode;
+  return PTH_COMPLETE;
+}
+
+int pth_mesh_nblocks(const pth_mesh *mh) {
+  if (!Impl(mh)->planned) return -1;
+  return static_cast<int>(Impl(mh)->topo.blocks.size());
+}
+
+int pth_mesh_multilevel(const pth_mesh *mh) { return Impl(mh)->topo.multilevel ? 1 : 0; }
+
+pth_status pth_mesh_block_info(const pth_mesh *mh, int gid, int64_t out[4], double x[6]) {
+  const Mesh *m = Impl(mh);
+  if (!m->planned) {
+    SetError("mesh is not planned");
+    return PTH_FAIL;
+  }
+  if (gid < 0 || gid >= static_cast<int>(m->topo.blocks.size())) {
+    SetError("block id out of range");
+    return PTH_FAIL;
+  }
+  const Block &b = m->topo.blocks[gid];
+  out[0] = b.at.level;
+  for (int d = 0; d < 3; ++d) {
+    out[1 + d] = b.at.x[d];
+    x[d] = b.xmin[d];
+    x[3 + d] = b.xmax[d];
+  }
+  return PTH_COMPLETE;
+}
+
+pth_status pth_mesh_index_shapes(const pth_mesh *mh, int out[12]) {
+  const Mesh *m = Impl(mh);
+  if (!m->planned) {
+    SetError("mesh is not planned");
+    return PTH_FAIL;
+  }
+  for (int d = 0; d < 3; ++d) {
+    out[d] = m->topo.fine.n[d];
+    out[3 + d] = m->topo.fine.g[d];
+    out[6 + d] = m->topo.coarse.n[d];
+    out[9 + d] = m->topo.coarse.g[d];
+  }
+  return PTH_COMPLETE;
+}
+
+pth_status pth_mesh_exchange_counts(const pth_mesh *mh, int bt, int64_t *nbuffers,
+                                    int64_t *nelements) {
+  const Mesh *m = Impl(mh);
+  if (!m->planned) {
+    SetError("mesh is not planned");
+    return PTH_FAIL;
+  }
+  std::int64_t nbuf = 0, nel = 0;
+  if (Bt(bt) == PTH_BT_ANY) {
+    for (const Block &B : m->topo.blocks)
+      for (const FieldSpec &f : m->fields) {
+        if (!(f.flags & PTH_FILL_GHOST)) continue;
+        for (const Link &l : B.links) {
+          const Side s = l.is_cell() ? Side::kSendInterior : Side::kSendBoundary;
+          nel += m->topo.Indices(B, l, s, false).count() * f.ncomp;
+          ++nbuf;
+        }
+      }
+  }
+  *nbuffers = nbuf;
+  *nelements = nel;
+  return PTH_COMPLETE;
+}
+
+pth_meshdata *pth_meshdata_get(pth_mesh *mh, const char *name) {
+  Mesh *m = Impl(mh);
+  if (NeedFinal(m) != PTH_COMPLETE) return nullptr;
+  return reinterpret_cast<pth_meshdata *>(m->Get(name));
+}
+
+pth_status pth_field_pointers(pth_meshdata *mdh, const char *field, double **data,
+                              double **coarse, double **flux) {
+  MeshData *md = Impl(mdh);
+  const int f = md->mesh->FieldIndex(field);
+  if (f < 0) {
+    SetError(std::string("no field ") + field);
+    return PTH_FAIL;
+  }
+  if (data) *data = md->store[f].data;
+  if (coarse) *coarse = md->store[f].coarse;
+  if (flux) *flux = md->store[f].flux;
+  return PTH_COMPLETE;
+}
+
+static pth_status FieldBuffer(MeshData *md, const char *field, int which, double **p,
+                              std::int64_t *n) {
+  const int f = md->mesh->FieldIndex(field);
+  if (f < 0) {
+    SetError(std::string("no field ") + field);
+    return PTH_FAIL;
+  }
+  FieldStore &s = md->store[f];
+  if (which == 0) {
+    *p = s.data;
+    *n = s.ndata;
+  } else if (which == 1) {
+    *p = s.coarse;
+    *n = s.ncoarse;
+  } else if (which == 2) {
+    *p = s.flux;
+    *n = s.nflux;
+  } else {
+    SetError("which must be 0 (data), 1 (coarse) or 2 (flux)");
+    return PTH_FAIL;
+  }
+  return PTH_COMPLETE;
+}
+
+pth_status pth_field_upload(pth_meshdata *mdh, const char *field, int which, const double *host,
+                            size_t n) {
+  MeshData *md = Impl(mdh);
+  double *p;
+  std::int64_t cap;
+  if (FieldBuffer(md, field, which, &p, &cap) != PTH_COMPLETE) return PTH_FAIL;
+  if (static_cast<std::int64_t>(n) != cap) {
+    SetError("size mismatch in pth_field_upload");
+    return PTH_FAIL;
+  }
+  PTH_HIP(hipMemcpyAsync(p, host, sizeof(double) * n, hipMemcpyHostToDevice, md->mesh->stream));
+  PTH_HIP(hipStreamSynchronize(md->mesh->stream));
+  return PTH_COMPLETE;
+}
+
+pth_status pth_field_download(pth_meshdata *mdh, const char *field, int which, double *host,
+                              size_t n) {
+  MeshData *md = Impl(mdh);
+  double *p;
+  std::int64_t cap;
+  if (FieldBuffer(md, field, which, &p, &cap) != PTH_COMPLETE) return PTH_FAIL;
+  if (static_cast<std::int64_t>(n) != cap) {
+    SetError("size mismatch in pth_field_download");
+    return PTH_FAIL;
+  }
+  PTH_HIP(hipMemcpyAsync(host, p, sizeof(double) * n, hipMemcpyDeviceToHost, md->mesh->stream));
+  PTH_HIP(hipStreamSynchronize(md->mesh->stream));
+  return PTH_COMPLETE;
+}
+
+// ---- boundary communication ----
+
+pth_status pth_send_bound_bufs(pth_meshdata *mdh, int bt) {
+  MeshData *md = Impl(mdh);
+  Mesh *m = md->mesh;
+  bt = Bt(bt);
+  if (m->BuildCache(*md, bt) != PTH_COMPLETE) return PTH_FAIL;
+  BoundCache &c = md->cache[bt];
+  if (c.nbuffers == 0) return PTH_COMPLETE;
+  if (m->state[bt] != BufState::kStale) return PTH_INCOMPLETE;  // previous exchange unfinished
+  if (m->topo.multilevel &&
+      LaunchRestrictT(c.restrict_send, m->topo, m->geom, m->stream) != PTH_COMPLETE)
+    return PTH_FAIL;
+  if (LaunchPack(c.pack, nullptr, m->stream, true, m->pack_mode) != PTH_COMPLETE)
+    return PTH_FAIL;
+  m->state[bt] = BufState::kSending;
+  return PTH_COMPLETE;
+}
+
+pth_status pth_start_receive_bound_bufs(pth_meshdata *mdh, int bt) {
+  MeshData *md = Impl(mdh);
+  return md->mesh->BuildCache(*md, Bt(bt));
+}
+
+pth_status pth_receive_bound_bufs(pth_meshdata *mdh, int bt) {
+  MeshData *md = Impl(mdh);
+  Mesh *m = md->mesh;
+  bt = Bt(bt);
+  if (m->BuildCache(*md, bt) != PTH_COMPLETE) return PTH_FAIL;
+  if (md->cache[bt].nbuffers == 0) return PTH_COMPLETE;
+  // one GPU: a sent buffer is received as soon as its pack is ordered before the unpack on
+  // the same stream
+  if (m->state[bt] == BufState::kSending) m->state[bt] = BufState::kReceived;
+  return m->state[bt] == BufState::kReceived ? PTH_COMPLETE : PTH_INCOMPLETE;
+}
+
+pth_status pth_set_bounds(pth_meshdata *mdh, int bt) {
+  MeshData *md = Impl(mdh);
+  Mesh *m = md->mesh;
+  bt = Bt(bt);
+  if (m->BuildCache(*md, bt) != PTH_COMPLETE) return PTH_FAIL;
+  BoundCache &c = md->cache[bt];
+  if (c.nbuffers == 0) return PTH_COMPLETE;
+  if (m->state[bt] != BufState::kReceived) {
+    SetError("SetBounds: buffers are not in a received state");
+    return PTH_FAIL;
+  }
+  if (LaunchPack(c.unpack, nullptr, m->stream, false, m->pack_mode) != PTH_COMPLETE)
+    return PTH_FAIL;
+  m->state[bt] = BufState::kStale;
+  if (m->topo.multilevel &&
+      LaunchRestrictT(c.restrict_set, m->topo, m->geom, m->stream) != PTH_COMPLETE)
+    return PTH_FAIL;
+  return PTH_COMPLETE;
+}
+
+pth_status pth_prolongate_bounds(pth_meshdata *mdh, int bt) {
+  MeshData *md = Impl(mdh);
+  Mesh *m = md->mesh;
+  bt = Bt(bt);
+  if (m->BuildCache(*md, bt) != PTH_COMPLETE) return PTH_FAIL;
+  BoundCache &c = md->cache[bt];
+  if (c.nbuffers == 0 || !m->topo.multilevel) return PTH_COMPLETE;
+  return LaunchProlongT(c.prolong, m->topo, m->geom, m->stream, true);
+}
+
+// ---- Burgers package and Update:: tasks ----
+
+static int NeedField(MeshData *md, const char *name, int min_comp) {
+  const int f = md->mesh->FieldIndex(name);
+  if (f < 0 || md->mesh->fields[f].ncomp < min_comp) {
+    SetError(std::string("Burgers tasks need field ") + name);
+    return -1;
+  }
+  return f;
+}
+
+pth_status pth_burgers_calculate_fluxes(pth_meshdata *mdh, int recon) {
+  MeshData *md = Impl(mdh);
+  Mesh *m = md->mesh;
+  const int f = NeedField(md, "U", 4);
+  if (f < 0) return PTH_FAIL;
+  if (!(m->fields[f].flags & PTH_WITH_FLUXES)) {
+    SetError("field U needs WithFluxes");
+    return PTH_FAIL;
+  }
+  const int V = m->fields[f].ncomp;
+  if (!m->face || m->face_V != V) {
+    if (m->face) (void)hipFree(m->face);
+    m->face = nullptr;
+    if (Alloc(&m->face, 6 * md->store[f].ndata) != PTH_COMPLETE) return PTH_FAIL;
+    m->face_V = V;
+  }
+  if (BurgersRecon(m->topo, V, md->store[f].data, m->face, recon, m->stream) != PTH_COMPLETE)
+    return PTH_FAIL;
+  return BurgersFlux(m->topo, V, m->face, md->store[f].flux, m->stream);
+}
+
+pth_status pth_flux_divergence(pth_meshdata *inh, pth_meshdata *dudth) {
+  MeshData *in = Impl(inh), *out = Impl(dudth);
+  Mesh *m = in->mesh;
+  for (size_t f = 0; f < m->fields.size(); ++f) {
+    if (!(m->fields[f].flags & PTH_WITH_FLUXES)) continue;
+    if (FluxDivergence(m->topo, m->fields[f].ncomp, in->store[f].flux, out->store[f].data,
+                       m->coef, m->stream) != PTH_COMPLETE)
+      return PTH_FAIL;
+  }
+  return PTH_COMPLETE;
+}
+
+pth_status pth_weighted_sum_independent(pth_meshdata *in1h, pth_meshdata *in2h, double w1,
+                                        double w2, pth_meshdata *outh) {
+  MeshData *x = Impl(in1h), *y = Impl(in2h), *z = Impl(outh);
+  Mesh *m = x->mesh;
+  for (size_t f = 0; f < m->fields.size(); ++f) {
+    if (!(m->fields[f].flags & PTH_INDEPENDENT)) continue;
+    if (WeightedSum(x->store[f].data, y->store[f].data, w1, w2, z->store[f].data,
+                    z->store[f].ndata, m->stream) != PTH_COMPLETE)
+      return PTH_FAIL;
+  }
+  return PTH_COMPLETE;
+}
+
+pth_status pth_burgers_calculate_derived(pth_meshdata *mdh) {
+  MeshData *md = Impl(mdh);
+  Mesh *m = md->mesh;
+  const int fu = NeedField(md, "U", 4), fd = NeedField(md, "derived", 1);
+  if (fu < 0 || fd < 0) return PTH_FAIL;
+  return BurgersDerived(m->topo, m->fields[fu].ncomp, md->store[fu].data, md->store[fd].data,
+                        m->stream);
+}
+
+pth_status pth_burgers_estimate_timestep(pth_meshdata *mdh, double cfl, double *dt) {
+  MeshData *md = Impl(mdh);
+  Mesh *m = md->mesh;
+  const int fu = NeedField(md, "U", 3);
+  if (fu < 0) return PTH_FAIL;
+  if (BurgersDtMin(m->topo, m->fields[fu].ncomp, md->store[fu].data, m->coef, m->scratch,
+                   static_cast<int>(m->scratch_n), m->scratch + m->scratch_n,
+                   m->stream) != PTH_COMPLETE)
+    return PTH_FAIL;
+  PTH_HIP(hipMemcpyAsync(m->pinned, m->scratch + m->scratch_n, sizeof(double),
+                         hipMemcpyDeviceToHost, m->stream));
+  PTH_HIP(hipStreamSynchronize(m->stream));
+  *dt = cfl * m->pinned[0];
+  return PTH_COMPLETE;
+}
+
+static pth_status PgenUpload(MeshData *md, int f, const std::vector<double> &h) {
+  Mesh *m = md->mesh;
+  PTH_HIP(hipMemcpy(md->store[f].data, h.data(), sizeof(double) * h.size(),
+                    hipMemcpyHostToDevice));
+  (void)m;
+  return PTH_COMPLETE;
+}
+
+// Seeded sum-of-sinusoids initial condition for U over block interiors: for component c and
+// mode m = 0..3 draw A ~ U(0.1, 1) then phi ~ U(0, 2 pi) from SplitMix64(seed) in (c, m)
+// order; mode m varies along direction (c + m) % 3 with wavenumber m + 1.
+pth_status pth_burgers_pgen_sinusoid(pth_meshdata *mdh, uint64_t seed) {
+  MeshData *md = Impl(mdh);
+  Mesh *m = md->mesh;
+  const int f = NeedField(md, "U", 3);
+  if (f < 0) return PTH_FAIL;
+  const int V = m->fields[f].ncomp;
+  SplitMix64 rng{seed};
+  std::vector<double> A(4 * V), P(4 * V);
+  for (int c = 0; c < V; ++c)
+    for (int q = 0; q < 4; ++q) {
+      A[4 * c + q] = 0.1 + 0.9 * rng.uniform();
+      P[4 * c + q] = 2.0 * M_PI * rng.uniform();
+    }
+  const Extent &e = m->topo.fine;
+  std::vector<double> h(md->store[f].ndata, 0.0);
+  for (size_t b = 0; b < m->topo.blocks.size(); ++b) {
+    const Block &B = m->topo.blocks[b];
+    for (int c = 0; c < V; ++c)
+      for (int k = e.lo(2); k <= e.hi(2); ++k)
+        for (int j = e.lo(1); j <= e.hi(1); ++j)
+          for (int i = e.lo(0); i <= e.hi(0); ++i) {
+            const double x[3] = {B.x0[0] + (i + 0.5) * B.dx[0], B.x0[1] + (j + 0.5) * B.dx[1],
+                                 B.x0[2] + (k + 0.5) * B.dx[2]};
+            double v = 0.0;
+            for (int q = 0; q < 4; ++q)
+              v += A[4 * c + q] * std::sin(2.0 * M_PI * (q + 1) * x[(c + q) % 3] + P[4 * c + q]);
+            h[((static_cast<std::int64_t>(b) * V + c) * e.ent(2) + k) * e.ent(1) * e.ent(0) +
+              static_cast<std::int64_t>(j) * e.ent(0) + i] = v;
+          }
+  }
+  return PgenUpload(md, f, h);
+}
+
+// The reference problem generator (benchmarks/burgers/parthenon_app_inputs.cpp:34-79).
+pth_status pth_burgers_pgen_vibe(pth_meshdata *mdh) {
+  MeshData *md = Impl(mdh);
+  Mesh *m = md->mesh;
+  const int f = NeedField(md, "U", 3);
+  if (f < 0) return PTH_FAIL;
+  const int V = m->fields[f].ncomp;
+  const Extent &e = m->topo.fine;
+  std::vector<double> h(md->store[f].ndata, 0.0);
+  for (size_t b = 0; b < m->topo.blocks.size(); ++b) {
+    const Block &B = m->topo.blocks[b];
+    for (int k = e.lo(2); k <= e.hi(2); ++k)
+      for (int j = e.lo(1); j <= e.hi(1); ++j)
+        for (int i = e.lo(0); i <= e.hi(0); ++i) {
+          const double x = B.x0[0] + (i + 0.5) * B.dx[0], y = B.x0[1] + (j + 0.5) * B.dx[1],
+                       z = B.x0[2] + (k + 0.5) * B.dx[2];
+          const double qv[3] = {(std::tanh(-20.0 * x) * std::cos(M_PI * x) + 1.0) *
+                                    std::exp(-30.0 * y * y) * std::exp(-30.0 * z * z),
+                                (std::sin(M_PI * y) + 0.2) * std::exp(-30.0 * x * x) *
+                                    std::exp(-30.0 * z * z),
+                                (std::tanh(-20. * z) * std::cos(M_PI * z) + 0.5) *
+                                    std::exp(-30.0 * x * x) * std::exp(-30.0 * y * y)};
+          for (int c = 0; c < V; ++c) {
+            double v;
+            if (c < 3) {
+              v = qv[c];
+            } else {
+              v = 1;
+              if (std::abs(x) < 0.025 && std::abs(y) < 0.15 && std::abs(z) < 0.025) v += 10.0;
+            }
+            h[((static_cast<std::int64_t>(b) * V + c) * e.ent(2) + k) * e.ent(1) * e.ent(0) +
+              static_cast<std::int64_t>(j) * e.ent(0) + i] = v;
+          }
+        }
+  }
+  return PgenUpload(md, f, h);
+}
+
+}  // extern "C"
+
+extern "C" {
+
+pth_status pth_profile_enable(int enable) {
+  Prof().on = enable != 0;
+  if (!Prof().on) Prof().Clear();
+  return PTH_COMPLETE;
+}
+
+int pth_profile_report(char *buf, int len) {
+  std::map<std::string, std::pair<int, double>> acc;
+  for (auto &r : Prof().records) {
+    if (hipEventSynchronize(r.stop) != hipSuccess) continue;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, r.start, r.stop) != hipSuccess) continue;
+    auto &a = acc[r.name];
+    a.first += 1;
+    a.second += ms;
+  }
+  Prof().Clear();
+  std::string out;
+  for (auto &kv : acc) {
+    char line[256];
+    std::snprintf(line, sizeof(line), "%s %d %.6f\n", kv.first.c_str(), kv.second.first,
+                  kv.second.second);
+    out += line;
+  }
+  if (buf && len > 0) {
+    std::strncpy(buf, out.c_str(), static_cast<size_t>(len) - 1);
+    buf[len - 1] = '\0';
+  }
+  return static_cast<int>(acc.size());
+}
+
+}  // extern "C"

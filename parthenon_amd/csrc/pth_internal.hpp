@@ -1,0 +1,262 @@
+// Internal host-side structures of the parthenon_amd library.
+//
+// Layout in HBM (one GPU, all blocks in one MeshData):
+//   field data   [block][comp][k][j][i]         i fastest, (n+2g)^3 brick per component
+//   coarse data  [block][comp][ck][cj][ci]      reference coarse_s, (n/2+2g)^3 per component
+//   face fluxes  [dir][block][comp][k][j][i]    x-face i stored at index i
+//   comm buffers one flat arena per field name, each channel a contiguous slice laid out
+//                [comp][k][j][i] exactly like the reference buffer order
+//   region table one 96-byte record per (block, neighbour, field) boundary region, walked
+//                by a single persistent kernel per pack / unpack / restrict / prolong call
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <set>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/parthenon_amd.h"
+
+namespace pth {
+
+void SetError(const std::string &msg);
+const char *ErrorText();
+
+#define PTH_HIP(call)                                                              \
+  do {                                                                             \
+    hipError_t err_ = (call);                                                      \
+    if (err_ != hipSuccess) {                                                      \
+      ::pth::SetError(std::string(#call) + ": " + hipGetErrorString(err_));        \
+      return PTH_FAIL;                                                             \
+    }                                                                              \
+  } while (0)
+
+// ---------------------------------------------------------------- topology ---------------
+struct Lattice {  // a block position: level and integer coordinates on that level's lattice
+  int level = 0;
+  std::int64_t x[3] = {0, 0, 0};
+  bool operator<(const Lattice &o) const {
+    return std::tie(level, x[2], x[1], x[0]) < std::tie(o.level, o.x[2], o.x[1], o.x[0]);
+  }
+  bool operator==(const Lattice &o) const {
+    return level == o.level && x[0] == o.x[0] && x[1] == o.x[1] && x[2] == o.x[2];
+  }
+};
+
+struct Link {             // one neighbour of a block (reference NeighborBlock, CC subset)
+  int gid;                // neighbour block id
+  int level;              // neighbour level
+  std::int64_t seen[3];   // neighbour coordinates as seen from this block (not wrapped)
+  int off[3];             // offset direction, each in {-1, 0, 1}
+  int loc_index() const { return (off[0] + 1) + 3 * (off[1] + 1) + 9 * (off[2] + 1); }
+  int reverse_index() const { return (1 - off[0]) + 3 * (1 - off[1]) + 9 * (1 - off[2]); }
+  bool is_cell() const { return off[0] == 0 && off[1] == 0 && off[2] == 0; }
+};
+
+struct Block {
+  Lattice at;
+  double xmin[3], xmax[3];
+  double dx[3];      // fine cell widths
+  double x0[3];      // fine coordinate origin: xmin - g dx  (cell centre = x0 + (i+.5) dx)
+  double cdx[3];     // coarse cell widths
+  double cx0[3];     // coarse coordinate origin
+  double vol;        // fine cell volume
+  double area[3];    // fine face areas
+  bool coarser_neighbor = false;
+  std::vector<Link> links;
+};
+
+struct Extent {  // index ranges of a block array per dimension (reference IndexShape)
+  int n[3], g[3];
+  int ent(int d) const { return n[d] + 2 * g[d]; }
+  int lo(int d) const { return g[d]; }
+  int hi(int d) const { return g[d] + n[d] - 1; }
+  std::int64_t cells() const { return (std::int64_t)ent(0) * ent(1) * ent(2); }
+};
+
+struct Box {  // inclusive index box [lo, hi] per dimension
+  int lo[3], hi[3];
+  std::int64_t count() const {
+    return (std::int64_t)(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1);
+  }
+};
+
+enum class Side { kSendBoundary, kRecvBoundary, kSendInterior, kRecvInterior };
+
+class Topology {
+ public:
+  int nx[3], bnx[3], ng;
+  double xmin[3], xmax[3];
+  bool periodic[3], active[3];
+  int ndim = 3;
+  std::int64_t nbase[3];
+  bool multilevel = false;
+  std::set<Lattice> leaf, inner;
+  std::vector<Block> blocks;
+  std::map<Lattice, int> id;
+  Extent fine, coarse;
+
+  std::string Init(const int *nx_, const int *bnx_, int g, const double *lo, const double *hi,
+                   const int *per);
+  void RefineBox(const double box[6], int level);
+  void RefineSphere(const double c[3], double r, int level);
+  std::string Finalize();
+  // Index box of one boundary region (reference CalcIndices, cell-centred fields).
+  Box Indices(const Block &b, const Link &l, Side side, bool prores) const;
+
+ private:
+  std::int64_t Span(int d, int level) const { return nbase[d] << level; }
+  bool Fold(Lattice &a) const;
+  std::vector<Lattice> Children(const Lattice &a) const;
+  void Split(const Lattice &a);
+  void Ensure(const Lattice &a);
+  double Position(int d, std::int64_t index, int edge, int level) const;
+  void Neighbors(int gid);
+};
+
+// ---------------------------------------------------------------- device tables ----------
+// One boundary region processed by the persistent kernels. For pack/unpack, `a` is the field
+// slab (data or coarse) of the block and `b` the start of its buffer slice. For restriction
+// and prolongation, `a` is the fine slab and `b` the coarse slab of the same block.
+struct Region {
+  double *a;
+  double *b;
+  std::int64_t start;     // first element in the flattened element space of the table
+  int lo[3];              // index box origin (i, j, k) in the index space of the box
+  int n[3];               // box extents (i, j, k)
+  int nc;                 // components
+  int gid;                // block id (coordinates for prolongation)
+  int astride[3];         // element strides of `a` for (j, k, comp)
+  int bstride[3];         // element strides of `b` for (j, k, comp) (restrict/prolong)
+  std::uint64_t magic[3]; // ceil(2^40 / n[d]): q = (x * magic) >> 40 for x < 2^30 / n[d]
+};
+static_assert(sizeof(Region) % 8 == 0, "region record alignment");
+
+struct DeviceBlockGeom {  // per-block coordinates for prolongation
+  double x0[3], dx[3], cx0[3], cdx[3];
+};
+
+struct BlockCoefs {  // per-block uniform-Cartesian metrics used by the stencils
+  double dx[3];
+  double area[3];
+  double vol;
+  double pad;
+};
+
+struct Table {
+  std::vector<Region> host;
+  Region *dev = nullptr;
+  std::int64_t *dev_start = nullptr;  // region start offsets (for the binary search)
+  std::int64_t total = 0;             // total elements
+  void Free();
+  pth_status Upload();
+};
+
+struct FieldSpec {
+  std::string name;
+  int ncomp;
+  unsigned flags;
+};
+
+struct FieldStore {
+  double *data = nullptr, *coarse = nullptr, *flux = nullptr;
+  std::int64_t ndata = 0, ncoarse = 0, nflux = 0;
+  bool owned = true;
+};
+
+class Mesh;
+
+struct BoundCache {  // per (MeshData, bound type): the pack / unpack / pro-res tables
+  bool built = false;
+  Table pack, unpack, restrict_send, restrict_set, prolong;
+  std::int64_t nbuffers = 0;
+};
+
+struct MeshData {
+  Mesh *mesh;
+  std::string name;
+  std::vector<FieldStore> store;  // indexed like Mesh::fields
+  BoundCache cache[10];
+  ~MeshData();
+};
+
+// channel key: sender gid, receiver gid, field index, location index
+using Channel = std::tuple<int, int, int, int>;
+
+enum class BufState { kStale, kSending, kReceived };
+
+class Mesh {
+ public:
+  Topology topo;
+  std::vector<FieldSpec> fields;
+  std::map<std::string, std::unique_ptr<MeshData>> data;
+  std::vector<FieldStore> one_copy;             // shared storage of OneCopy fields
+  std::vector<double *> arena;                  // comm buffer arena per field
+  std::vector<std::int64_t> arena_size;
+  std::map<Channel, std::int64_t> channel_offset;  // element offset within the field arena
+  BufState state[10] = {};
+  DeviceBlockGeom *geom = nullptr;
+  BlockCoefs *coef = nullptr;
+  double *face = nullptr;                       // Burgers face-state scratch
+  int face_V = 0;
+  hipStream_t stream = nullptr;
+  int pack_mode = PTH_PACK_PERSISTENT;
+  bool planned = false;    // host topology built
+  bool finalized = false;  // device storage, tables and buffers built
+  double *scratch = nullptr;                    // reduction partials
+  double *pinned = nullptr;                     // host-pinned readback slot
+  std::int64_t scratch_n = 0;
+
+  ~Mesh();
+  int FieldIndex(const std::string &n) const;
+  pth_status Plan();
+  pth_status Finalize();
+  MeshData *Get(const std::string &name);
+  pth_status Allocate(MeshData &md);
+  pth_status BuildCache(MeshData &md, int bt);
+};
+
+// Optional per-launch timing with HIP events on the launch stream (instrumentation regions
+// like the reference's PARTHENON_INSTRUMENT, src/utils/instrument.hpp).
+struct ProfRecord {
+  const char *name;
+  hipEvent_t start, stop;
+};
+struct Profiler {
+  bool on = false;
+  std::vector<ProfRecord> records;
+  std::vector<hipEvent_t> pool;
+  hipEvent_t Take();
+  void Clear();
+};
+Profiler &Prof();
+
+class ProfScope {  // records start/stop events around the launches in its lifetime
+ public:
+  ProfScope(const char *name, hipStream_t s);
+  ~ProfScope();
+
+ private:
+  const char *name_;
+  hipStream_t s_;
+  hipEvent_t a_ = nullptr;
+};
+
+// kernels (pth_bvals.hip)
+pth_status LaunchPack(const Table &t, double *arena_unused, hipStream_t s, bool to_buffer,
+                      int mode);
+pth_status LaunchRestrict(const Table &t, const DeviceBlockGeom *geom, hipStream_t s);
+pth_status LaunchProlong(const Table &t, const DeviceBlockGeom *geom, hipStream_t s);
+
+std::uint64_t Magic(int d);
+
+}  // namespace pth
+
+inline pth::Mesh *Impl(pth_mesh *m) { return reinterpret_cast<pth::Mesh *>(m); }
+inline const pth::Mesh *Impl(const pth_mesh *m) { return reinterpret_cast<const pth::Mesh *>(m); }
+inline pth::MeshData *Impl(pth_meshdata *m) { return reinterpret_cast<pth::MeshData *>(m); }

@@ -1,0 +1,179 @@
+"""ctypes wrapper over oracle/liboracle.so (the CPU restatement used as the parity checker).
+
+Test infrastructure only: imported by tests/, the smoke check and bench.py's CPU-baseline
+leg, never by the product package.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_LIB = None
+
+D = C.c_double
+I = C.c_int
+I64 = C.c_int64
+P = C.c_void_p
+DP = C.POINTER(C.c_double)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(ROOT, "oracle", "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built; run __graft_entry__.build()")
+        L = C.CDLL(path)
+        sig = {
+            "orc_mesh_new": (P, [C.POINTER(I), C.POINTER(I), I, DP, DP, C.POINTER(I)]),
+            "orc_mesh_refine_box": (None, [P, DP, I]),
+            "orc_mesh_refine_sphere": (None, [P, DP, D, I]),
+            "orc_mesh_finalize": (None, [P]),
+            "orc_mesh_free": (None, [P]),
+            "orc_mesh_nblocks": (I, [P]),
+            "orc_mesh_multilevel": (I, [P]),
+            "orc_mesh_block": (None, [P, I, C.POINTER(I64), DP]),
+            "orc_mesh_nneighbors": (I, [P, I]),
+            "orc_mesh_neighbor": (None, [P, I, I, C.POINTER(I64)]),
+            "orc_mesh_shapes": (None, [P, C.POINTER(I)]),
+            "orc_mesh_region_boxes": (I, [P, I, C.POINTER(I), I]),
+            "orc_exchange_counts": (I64, [P, I, C.POINTER(I), C.POINTER(I64)]),
+            "orc_exchange": (None, [P, I, C.POINTER(I), C.POINTER(DP), C.POINTER(DP), I, I]),
+            "orc_restrict_interior": (None, [P, I, DP, DP]),
+            "orc_prolongate_cells": (None, [P, I, DP, DP, I, C.POINTER(I), I]),
+            "orc_weno5z": (None, [I, DP, DP, DP]),
+            "orc_linear": (None, [I, DP, DP, DP]),
+            "orc_lr_to_flux": (None, [I, DP, DP]),
+            "orc_burgers_new": (P, [P, I, D, I]),
+            "orc_burgers_free": (None, [P]),
+            "orc_burgers_pgen_sinusoid": (None, [P, C.c_uint64]),
+            "orc_burgers_pgen_vibe": (None, [P]),
+            "orc_burgers_initialize": (None, [P]),
+            "orc_burgers_cycles": (None, [P, I]),
+            "orc_burgers_fluxes": (None, [P]),
+            "orc_burgers_fluxdiv": (None, [P]),
+            "orc_burgers_derived": (None, [P]),
+            "orc_burgers_estimate_dt": (D, [P]),
+            "orc_burgers_dt": (D, [P]),
+            "orc_burgers_time": (D, [P]),
+            "orc_burgers_size": (I64, [P, I]),
+            "orc_burgers_get": (None, [P, I, DP]),
+            "orc_burgers_set": (None, [P, I, DP]),
+            "orc_burgers_mass_history": (D, [P, DP, DP, D]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def dptr(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(DP)
+
+
+def iarr(vals, t=C.c_int):
+    return (t * len(vals))(*vals)
+
+
+class OracleMesh:
+    """Mesh restatement: uniform base grid, optional static refinement boxes/spheres."""
+
+    def __init__(self, nx, bnx, nghost, xmin=(-0.5, -0.5, -0.5), xmax=(0.5, 0.5, 0.5),
+                 periodic=(1, 1, 1), boxes=(), spheres=()):
+        L = lib()
+        self.h = L.orc_mesh_new(iarr(nx), iarr(bnx), nghost, (D * 3)(*xmin), (D * 3)(*xmax),
+                                iarr(periodic))
+        for box, level in boxes:
+            L.orc_mesh_refine_box(self.h, (D * 6)(*box), level)
+        for center, r, level in spheres:
+            L.orc_mesh_refine_sphere(self.h, (D * 3)(*center), r, level)
+        L.orc_mesh_finalize(self.h)
+        self.nblocks = L.orc_mesh_nblocks(self.h)
+        self.multilevel = bool(L.orc_mesh_multilevel(self.h))
+        s = (I * 12)()
+        L.orc_mesh_shapes(self.h, s)
+        self.n = tuple(s[0:3])
+        self.ng = tuple(s[3:6])
+        self.cn = tuple(s[6:9])
+        self.cng = tuple(s[9:12])
+        self.ent = tuple(self.n[d] + 2 * self.ng[d] for d in range(3))
+        self.cent = tuple(self.cn[d] + 2 * self.cng[d] for d in range(3))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.orc_mesh_free(self.h)
+            self.h = None
+
+    def block(self, gid):
+        out = (I64 * 4)()
+        x = (D * 6)()
+        lib().orc_mesh_block(self.h, gid, out, x)
+        return dict(level=out[0], l=tuple(out[1:4]), xmin=tuple(x[0:3]), xmax=tuple(x[3:6]))
+
+    def neighbors(self, gid):
+        res = []
+        for n in range(lib().orc_mesh_nneighbors(self.h, gid)):
+            out = (I64 * 8)()
+            lib().orc_mesh_neighbor(self.h, gid, n, out)
+            res.append(dict(gid=out[0], level=out[1], origin=tuple(out[2:5]),
+                            off=tuple(out[5:8])))
+        return res
+
+    def region_boxes(self, gid):
+        buf = (I * (7 * 512))()
+        n = lib().orc_mesh_region_boxes(self.h, gid, buf, 512)
+        return np.array(buf[:7 * n], dtype=np.int64).reshape(n, 7)
+
+    def field_shape(self, ncomp):
+        e = self.ent
+        return (self.nblocks, ncomp, e[2], e[1], e[0])
+
+    def coarse_shape(self, ncomp):
+        e = self.cent
+        return (self.nblocks, ncomp, e[2], e[1], e[0])
+
+    def exchange_counts(self, ncomps):
+        alloc = I64()
+        used = lib().orc_exchange_counts(self.h, len(ncomps), iarr(ncomps), C.byref(alloc))
+        return used, alloc.value
+
+    def exchange(self, fields, coarse=None, prolong=0, nrep=1):
+        """In-place Send+Set(+prolongation) over a list of float64 arrays."""
+        ncomps = [f.shape[1] for f in fields]
+        dps = (DP * len(fields))(*[dptr(f) for f in fields])
+        if coarse is None:
+            cps = (DP * len(fields))(*[None] * len(fields))
+        else:
+            cps = (DP * len(fields))(*[dptr(c) for c in coarse])
+        lib().orc_exchange(self.h, len(fields), iarr(ncomps), dps, cps, prolong, nrep)
+
+
+class OracleBurgers:
+    def __init__(self, mesh, num_scalars=4, cfl=0.4, recon=0):
+        self.mesh = mesh
+        self.V = 3 + num_scalars
+        self.h = lib().orc_burgers_new(mesh.h, num_scalars, cfl, recon)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.orc_burgers_free(self.h)
+            self.h = None
+
+    def get(self, which):
+        n = lib().orc_burgers_size(self.h, which)
+        out = np.empty(n, dtype=np.float64)
+        lib().orc_burgers_get(self.h, which, dptr(out))
+        return out
+
+    def set(self, which, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+        assert a.size == lib().orc_burgers_size(self.h, which)
+        lib().orc_burgers_set(self.h, which, dptr(a))
+
+    @property
+    def dt(self):
+        return lib().orc_burgers_dt(self.h)

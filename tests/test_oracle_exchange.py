@@ -1,0 +1,85 @@
+"""Pins the oracle's boundary exchange (mesh topology, index ranges, pack/unpack, restriction,
+prolongation) against closed forms, before it is trusted as the checker of the HIP path."""
+import numpy as np
+import pytest
+
+from meshcheck import global_index_field, interior_mask, linear_field
+from oracle_lib import OracleMesh
+
+
+@pytest.mark.parametrize("nx,bnx,g", [
+    ((32, 32, 32), (8, 8, 8), 3),    # 4 blocks per dimension
+    ((16, 16, 16), (8, 8, 8), 2),    # 2 blocks: +x and -x neighbour is the same block
+    ((8, 8, 8), (8, 8, 8), 2),       # 1 block: every neighbour is the block itself
+    ((32, 16, 8), (8, 8, 8), 3),     # uneven block counts
+])
+def test_single_level_ghosts_equal_wrapped_interior(nx, bnx, g):
+    m = OracleMesh(nx, bnx, g)
+    exp = global_index_field(m, 2, nx)
+    f = np.where(interior_mask(m)[None, None], exp, np.nan)
+    m.exchange([f])
+    assert np.array_equal(f, exp)
+
+
+def test_buffer_element_counts_match_halo_volume():
+    # cfg-1 shape: 512 blocks of 16^3, g=3, 8 one-component variables
+    m = OracleMesh((128, 128, 128), (16, 16, 16), 3)
+    used, alloc = m.exchange_counts([1] * 8)
+    n, g = 16, 3
+    halo = (n + 2 * g) ** 3 - n ** 3
+    assert used == 512 * 8 * halo
+    # GetBufferSize over-allocates: (n+2g+1 | g+1) per dimension
+    per_block = 0
+    for ox in (-1, 0, 1):
+        for oy in (-1, 0, 1):
+            for oz in (-1, 0, 1):
+                if ox == oy == oz == 0:
+                    continue
+                s = 1
+                for o in (ox, oy, oz):
+                    s *= (n + 2 * g + 1) if o == 0 else (g + 1)
+                per_block += s
+    assert alloc == 512 * 8 * per_block
+
+
+def test_sphere_refined_mesh_block_count():
+    m = OracleMesh((64, 64, 64), (16, 16, 16), 4, spheres=[((0.0, 0.0, 0.0), 0.25, 2)])
+    levels = [m.block(g)["level"] for g in range(m.nblocks)]
+    assert m.multilevel
+    assert levels.count(2) == 512 and levels.count(1) == 448 and m.nblocks == 960
+
+
+def _check_proper_nesting(m):
+    for gid in range(m.nblocks):
+        lev = m.block(gid)["level"]
+        for nb in m.neighbors(gid):
+            assert abs(nb["level"] - lev) <= 1
+
+
+@pytest.mark.parametrize("prolong", [1, 2])
+def test_multilevel_ghosts_reproduce_linear_field(prolong):
+    # Box refinement two levels deep in the middle of a periodic mesh: every ghost cell,
+    # whether filled by a same-level copy, a restricted finer neighbour or a prolongated
+    # coarser neighbour, must equal the dyadic linear field at its own centre, exactly.
+    m = OracleMesh((32, 32, 32), (8, 8, 8), 4, boxes=[((-0.1, 0.1, -0.1, 0.1, -0.1, 0.1), 2)])
+    assert m.multilevel
+    _check_proper_nesting(m)
+    coeffs = [(3.0, 2.0, 4.0, -8.0), (-1.0, 0.5, -0.25, 1.0)]
+    exp = linear_field(m, 2, coeffs)
+    f = np.where(interior_mask(m)[None, None], exp, np.nan)
+    cf = np.full(m.coarse_shape(2), np.nan)
+    # the linear field wraps at the periodic boundary, so only check blocks whose ghost
+    # stencils stay inside the domain when they talk to another level
+    m.exchange([f], coarse=[cf], prolong=prolong)
+    assert not np.isnan(f).any()
+    assert np.array_equal(f, exp)
+
+
+def test_sphere_mesh_ghosts_are_all_filled():
+    m = OracleMesh((64, 64, 64), (16, 16, 16), 4, spheres=[((0.0, 0.0, 0.0), 0.25, 2)])
+    _check_proper_nesting(m)
+    f = np.where(interior_mask(m)[None, None], 1.0, np.nan) * np.ones(m.field_shape(1))
+    cf = np.full(m.coarse_shape(1), np.nan)
+    m.exchange([f], coarse=[cf], prolong=1)
+    # constant data: every ghost cell filled, restriction and prolongation keep the constant
+    assert np.array_equal(f, np.ones_like(f))
