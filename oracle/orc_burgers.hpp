@@ -14,7 +14,7 @@
 //   src/interface/update.cpp:63-84                FluxDivergence<MeshData>
 //   src/time_integration/low_storage_integrator.cpp:67-83  rk2 beta = {1, 1/2}
 //   src/driver/driver.cpp:210-235                 SetGlobalTimeStep (dt growth <= 2x)
-//   src/utils/robust.hpp:40                       EPS = 10 * DBL_MIN
+//   src/utils/robust.hpp:38-41                    EPS = 10 * DBL_EPSILON
 #pragma once
 #include <cfloat>
 #include <limits>
@@ -23,7 +23,7 @@
 
 namespace orc {
 
-constexpr double kEPS = 10.0 * DBL_MIN;
+constexpr double kEPS = 10.0 * DBL_EPSILON;  // robust::EPS() = 10 * numeric_limits::epsilon()
 
 inline double mc(double dm, double dp) {
   const double dc = (dm * dp > 0.0) * 0.5 * (dm + dp);

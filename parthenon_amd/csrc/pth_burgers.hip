@@ -56,7 +56,7 @@ __device__ __forceinline__ void Weno5Z(double q0, double q1, double q2, double q
   const double a10 = -1.0 / 6.0, a11 = 5.0 / 6.0, a12 = 1.0 / 3.0;
   const double a20 = 1.0 / 3.0, a21 = 5.0 / 6.0, a22 = -1.0 / 6.0;
   const double g0 = 0.1, g1 = 0.6, g2 = 0.3;
-  const double eps = 10.0 * DBL_MIN;
+  const double eps = 10.0 * DBL_EPSILON;  // robust::EPS (src/utils/robust.hpp:38-41)
   const double t13 = 13.0 / 3.0;
   double a = q0 - 2 * q1 + q2;
   double b = q0 - 4.0 * q1 + 3.0 * q2;

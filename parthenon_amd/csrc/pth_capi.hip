@@ -274,6 +274,7 @@ pth_status Mesh::BuildCache(MeshData &md, int bt) {
     return PTH_FAIL;
   }
   const Extent &fe = topo.fine, &ce = topo.coarse;
+  std::vector<char> on_coarse;  // pack/unpack region q addresses the coarse buffer
   for (int b = 0; b < static_cast<int>(topo.blocks.size()); ++b) {
     const Block &B = topo.blocks[b];
     for (size_t f = 0; f < fields.size(); ++f) {
@@ -298,6 +299,7 @@ pth_status Mesh::BuildCache(MeshData &md, int bt) {
         const std::int64_t ro =
             channel_offset.at(Channel(l.gid, b, static_cast<int>(f), l.reverse_index()));
         c.unpack.host.push_back(MakeRegion(var, arena[f] + ro, rb, nc, b, ve, nullptr));
+        on_coarse.push_back(coarser ? 1 : 0);
         ++c.nbuffers;
         if (!topo.multilevel) continue;
         if (coarser) {
@@ -311,6 +313,36 @@ pth_status Mesh::BuildCache(MeshData &md, int bt) {
         }
       }
     }
+  }
+  // Every element a kernel will touch must lie inside its array: pack/unpack boxes inside the
+  // field brick, restriction children and prolongation stencils inside fine and coarse bricks.
+  auto inside = [](const Region &r, const Extent &e, int halo_lo, int halo_hi, int scale,
+                   int base_lo, int base_fine) {
+    for (int d = 0; d < 3; ++d) {
+      if (e.ent(d) == 1) {
+        if (r.lo[d] != 0 || r.n[d] != 1) return false;
+        continue;
+      }
+      const int lo = scale == 1 ? r.lo[d] - halo_lo : (r.lo[d] - base_lo) * 2 + base_fine;
+      const int hi = scale == 1 ? r.lo[d] + r.n[d] - 1 + halo_hi
+                                : (r.lo[d] + r.n[d] - 1 - base_lo) * 2 + base_fine + 1;
+      if (lo < 0 || hi >= e.ent(d)) return false;
+    }
+    return true;
+  };
+  bool ok = true;
+  for (size_t q = 0; q < c.pack.host.size(); ++q) {
+    const Extent &e = on_coarse[q] ? ce : fe;
+    ok = ok && inside(c.pack.host[q], e, 0, 0, 1, 0, 0) && inside(c.unpack.host[q], e, 0, 0, 1, 0, 0);
+  }
+  for (const Table *t : {&c.restrict_send, &c.restrict_set})
+    for (const Region &r : t->host)
+      ok = ok && inside(r, ce, 0, 0, 1, 0, 0) && inside(r, fe, 0, 0, 2, ce.lo(0), fe.lo(0));
+  for (const Region &r : c.prolong.host)
+    ok = ok && inside(r, ce, 1, 1, 1, 0, 0) && inside(r, fe, 0, 0, 2, ce.lo(0), fe.lo(0));
+  if (!ok) {
+    SetError("internal: a boundary region exceeds its array extents");
+    return PTH_FAIL;
   }
   for (Table *t : {&c.pack, &c.unpack, &c.restrict_send, &c.restrict_set, &c.prolong})
     if (t->Upload() != PTH_COMPLETE) return PTH_FAIL;
@@ -698,6 +730,19 @@ pth_status pth_burgers_calculate_fluxes(pth_meshdata *mdh, int recon) {
     SetError("field U needs WithFluxes");
     return PTH_FAIL;
   }
+  // burgers_package::Initialize (burgers_package.cpp:51-61): WENO5 needs nghost >= 4; the
+  // linear reconstruction reads one cell beyond the extended box and needs nghost >= 2
+  const int need = recon == 0 ? 4 : 2;
+  if (recon != 0 && recon != 1) {
+    SetError("recon must be 0 (weno5) or 1 (linear)");
+    return PTH_FAIL;
+  }
+  for (int d = 0; d < m->topo.ndim; ++d)
+    if (m->topo.fine.g[d] < need) {
+      SetError(recon == 0 ? "weno5 reconstruction requires 4 or more ghost cells"
+                          : "linear reconstruction requires 2 or more ghost cells");
+      return PTH_FAIL;
+    }
   const int V = m->fields[f].ncomp;
   if (!m->face || m->face_V != V) {
     if (m->face) (void)hipFree(m->face);

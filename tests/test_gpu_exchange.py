@@ -76,9 +76,10 @@ def test_task_state_machine():
     assert pa.ReceiveBoundBufs(md, pa.BoundaryType.nonlocal_) == pa.TaskStatus.complete
 
 
+# refinement kept away from the periodic boundary, where a linear field is not periodic
 REFINED = [
     dict(nx=(32, 32, 32), block=(8, 8, 8), g=4, boxes=[((-0.1, 0.1, -0.1, 0.1, -0.1, 0.1), 2)]),
-    dict(nx=(32, 32, 32), block=(8, 8, 8), g=2,
+    dict(nx=(64, 64, 64), block=(8, 8, 8), g=2,
          boxes=[((0.0, 0.2, -0.1, 0.1, 0.1, 0.3), 1), ((-0.2, -0.1, -0.2, -0.1, -0.2, -0.1), 2)]),
 ]
 

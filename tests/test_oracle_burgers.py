@@ -1,0 +1,151 @@
+"""Pins the oracle's Burgers stencils against what the mathematics fixes, before it is trusted as
+the checker of the HIP path: polynomial exactness of the WENO5Z and MC-linear reconstructions,
+boundedness next to a discontinuity, mirror symmetries of the reconstruction and of the HLL
+flux, exact steady states, the closed-form CFL time step and discrete conservation."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle_lib import OracleBurgers, OracleMesh, dptr, lib
+
+
+def weno(q):
+    q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 5)
+    ql = np.empty(len(q))
+    qr = np.empty(len(q))
+    lib().orc_weno5z(len(q), dptr(q), dptr(ql), dptr(qr))
+    return ql, qr
+
+
+def linear(q):
+    q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 3)
+    ql = np.empty(len(q))
+    qr = np.empty(len(q))
+    lib().orc_linear(len(q), dptr(q), dptr(ql), dptr(qr))
+    return ql, qr
+
+
+def hll(a):
+    a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1, 8)
+    out = np.empty((len(a), 5))
+    lib().orc_lr_to_flux(len(a), dptr(a), dptr(out))
+    return out
+
+
+rng = np.random.default_rng(3)
+
+
+def test_weno_constant_data():
+    c = rng.uniform(-5, 5, 200)
+    ql, qr = weno(np.repeat(c, 5))
+    np.testing.assert_allclose(ql, c, rtol=2e-15)
+    np.testing.assert_allclose(qr, c, rtol=2e-15)
+
+
+def test_weno_linear_data():
+    a, b = rng.uniform(-3, 3, 200), rng.uniform(-2, 2, 200)
+    q = a[:, None] + b[:, None] * np.arange(5)[None]
+    ql, qr = weno(q)
+    np.testing.assert_allclose(ql, q[:, 2] + b / 2, rtol=1e-13, atol=1e-14)
+    np.testing.assert_allclose(qr, q[:, 2] - b / 2, rtol=1e-13, atol=1e-14)
+
+
+def test_weno_quadratic_cell_averages_give_exact_interface_values():
+    # cell averages of x^2 over [i-1/2, i+1/2] are i^2 + 1/12; with equal smoothness
+    # indicators the weights are ideal and the 5-point reconstruction is exact for degree 2
+    for s in (-2.0, 0.0, 3.0):
+        x = np.arange(5) + s
+        ql, qr = weno(x ** 2 + 1.0 / 12.0)
+        assert abs(ql[0] - (x[2] + 0.5) ** 2) < 1e-12
+        assert abs(qr[0] - (x[2] - 0.5) ** 2) < 1e-12
+
+
+def test_weno_bounded_next_to_a_jump():
+    # a flat stencil next to a jump: smoothness indicators of 0 and O(100) must not overflow
+    cases = np.array([[1, 1, 1, 1, 11], [11, 1, 1, 1, 1], [1, 1, 1, 11, 11], [0, 0, 0, 0, 1e-3],
+                      [5, 5, 5, 5, 5 + 1e-12]], dtype=np.float64)
+    ql, qr = weno(cases)
+    assert np.all(np.isfinite(ql)) and np.all(np.isfinite(qr))
+    lo, hi = cases.min(axis=1), cases.max(axis=1)
+    assert np.all(ql >= lo - 1e-12) and np.all(ql <= hi + 1e-12)
+    assert np.all(qr >= lo - 1e-12) and np.all(qr <= hi + 1e-12)
+
+
+def test_weno_mirror_symmetry():
+    q = rng.normal(size=(500, 5))
+    ql, qr = weno(q)
+    ql2, qr2 = weno(q[:, ::-1].copy())
+    np.testing.assert_allclose(ql2, qr, rtol=1e-14, atol=1e-15)
+    np.testing.assert_allclose(qr2, ql, rtol=1e-14, atol=1e-15)
+
+
+def test_linear_reconstruction():
+    a, b = rng.uniform(-3, 3, 100), rng.uniform(-2, 2, 100)
+    q = a[:, None] + b[:, None] * np.arange(3)[None]
+    ql, qr = linear(q)
+    np.testing.assert_allclose(ql, q[:, 1] + b / 2, rtol=1e-14, atol=1e-15)
+    np.testing.assert_allclose(qr, q[:, 1] - b / 2, rtol=1e-14, atol=1e-15)
+    # a local extremum keeps the cell value on both faces
+    ql, qr = linear(np.array([[0.0, 1.0, 0.5], [2.0, -1.0, 3.0]]))
+    assert np.array_equal(ql, [1.0, -1.0]) and np.array_equal(qr, [1.0, -1.0])
+
+
+def test_hll_mirror_symmetry_and_rest_state():
+    a = rng.normal(size=(400, 8))
+    f = hll(a)
+    # reflect the face: swap left/right and negate every velocity; momentum fluxes are even
+    m = np.empty_like(a)
+    m[:, 0], m[:, 1] = -a[:, 1], -a[:, 0]
+    m[:, 2], m[:, 3] = -a[:, 3], -a[:, 2]
+    m[:, 4], m[:, 5] = -a[:, 5], -a[:, 4]
+    m[:, 6], m[:, 7] = -a[:, 7], -a[:, 6]
+    g = hll(m)
+    assert np.array_equal(g[:, 0], -f[:, 1]) and np.array_equal(g[:, 1], -f[:, 0])
+    np.testing.assert_array_equal(g[:, 2:], f[:, 2:])
+    z = hll(np.zeros((1, 8)))
+    assert np.array_equal(z, np.zeros((1, 5)))
+    # signal speeds bracket zero and the normal velocities
+    assert np.all(f[:, 0] <= 0) and np.all(f[:, 1] >= 0)
+    assert np.all(f[:, 0] <= np.minimum(a[:, 6], a[:, 7]))
+    assert np.all(f[:, 1] >= np.maximum(a[:, 6], a[:, 7]))
+
+
+def burgers(nx=(32, 32, 32), block=(16, 16, 16), g=4, ns=2, cfl=0.4):
+    m = OracleMesh(nx, block, g)
+    return m, OracleBurgers(m, ns, cfl, 0)
+
+
+def test_uniform_state_is_steady_and_dt_is_closed_form():
+    m, b = burgers()
+    U = b.get(0).reshape(m.field_shape(b.V))
+    vel = (0.75, -0.5, 0.25)
+    for c in range(b.V):
+        U[:, c] = vel[c] if c < 3 else 2.0
+    b.set(0, U)
+    lib().orc_burgers_initialize(b.h)
+    dx = 1.0 / 32
+    assert b.dt == 0.4 * (1.0 / (0.75 / dx + 0.5 / dx + 0.25 / dx))
+    lib().orc_burgers_cycles(b.h, 2)
+    assert np.array_equal(b.get(0).reshape(U.shape), U)
+
+
+def test_sinusoid_run_conserves_totals():
+    m, b = burgers(nx=(32, 32, 32), block=(8, 8, 8))
+    lib().orc_burgers_pgen_sinusoid(b.h, 9)
+    lib().orc_burgers_initialize(b.h)
+    g = m.ng[0]
+    tot = lambda: b.get(0).reshape(m.field_shape(b.V))[:, :, g:-g, g:-g, g:-g].sum(axis=(0, 2, 3, 4))
+    t0 = tot()
+    lib().orc_burgers_cycles(b.h, 5)
+    t1 = tot()
+    assert np.all(np.isfinite(t1))
+    assert np.max(np.abs(t1 - t0)) < 1e-10 * 32 ** 3
+
+
+def test_vibe_problem_stays_finite():
+    m, b = burgers(nx=(64, 64, 64), ns=4)
+    lib().orc_burgers_pgen_vibe(b.h)
+    lib().orc_burgers_initialize(b.h)
+    lib().orc_burgers_cycles(b.h, 3)
+    assert np.all(np.isfinite(b.get(0)))

@@ -56,12 +56,18 @@ def _check_proper_nesting(m):
             assert abs(nb["level"] - lev) <= 1
 
 
-@pytest.mark.parametrize("prolong", [1, 2])
-def test_multilevel_ghosts_reproduce_linear_field(prolong):
-    # Box refinement two levels deep in the middle of a periodic mesh: every ghost cell,
+@pytest.mark.parametrize("prolong,case", [(1, 0), (2, 0), (1, 1), (2, 1)])
+def test_multilevel_ghosts_reproduce_linear_field(prolong, case):
+    # Box refinement up to two levels deep in the middle of a periodic mesh: every ghost cell,
     # whether filled by a same-level copy, a restricted finer neighbour or a prolongated
     # coarser neighbour, must equal the dyadic linear field at its own centre, exactly.
-    m = OracleMesh((32, 32, 32), (8, 8, 8), 4, boxes=[((-0.1, 0.1, -0.1, 0.1, -0.1, 0.1), 2)])
+    if case == 0:
+        m = OracleMesh((32, 32, 32), (8, 8, 8), 4,
+                       boxes=[((-0.1, 0.1, -0.1, 0.1, -0.1, 0.1), 2)])
+    else:
+        m = OracleMesh((64, 64, 64), (8, 8, 8), 2,
+                       boxes=[((0.0, 0.2, -0.1, 0.1, 0.1, 0.3), 1),
+                              ((-0.2, -0.1, -0.2, -0.1, -0.2, -0.1), 2)])
     assert m.multilevel
     _check_proper_nesting(m)
     coeffs = [(3.0, 2.0, 4.0, -8.0), (-1.0, 0.5, -0.25, 1.0)]
