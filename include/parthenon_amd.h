@@ -143,6 +143,22 @@ pth_status pth_burgers_calculate_derived(pth_meshdata *md);
 /* burgers_package::EstimateTimestepMesh (burgers_package.cpp:170-200); synchronous. */
 pth_status pth_burgers_estimate_timestep(pth_meshdata *md, double cfl, double *dt);
 
+/* ---- device-resident Burgers driver (the task order of BurgersDriver::MakeTaskCollection,
+ * benchmarks/burgers/burgers_driver.cpp:53-148, RK2 of low_storage_integrator.cpp:67-83 and
+ * EvolutionDriver::SetGlobalTimeStep, src/driver/driver.cpp:210-235) on the containers "base"
+ * and "1". Each stage is one reconstruction launch, one fused flux/divergence/update launch
+ * and the boundary exchange; dt and time stay in device memory, so a cycle needs no host
+ * synchronization and can be captured once in a hipGraph and replayed. Single-level meshes. */
+/* Mesh::Initialize after the problem generator on "base": exchange, FillDerived, first dt. */
+pth_status pth_burgers_init(pth_mesh *m, double cfl, int recon);
+/* ncycles RK2 cycles; use_graph != 0 captures one cycle into a hipGraph on first use and
+ * replays it. Asynchronous on the driver's own stream. */
+pth_status pth_burgers_run_cycles(pth_mesh *m, int ncycles, int use_graph);
+/* Synchronous read of the device clock: current dt and simulated time. */
+pth_status pth_burgers_clock(pth_mesh *m, double *dt, double *time);
+/* Waits for the driver's stream. */
+pth_status pth_burgers_synchronize(pth_mesh *m);
+
 /* Problem generators for "U" of md (host-evaluated, uploaded): the seeded sum of sinusoids
  * and the reference's tanh/exp profile (benchmarks/burgers/parthenon_app_inputs.cpp:34-79). */
 pth_status pth_burgers_pgen_sinusoid(pth_meshdata *md, uint64_t seed);

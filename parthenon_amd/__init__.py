@@ -97,6 +97,10 @@ _SIGNATURES = {
     "pth_burgers_estimate_timestep": (_I, [_P, _D, _DP]),
     "pth_burgers_pgen_sinusoid": (_I, [_P, C.c_uint64]),
     "pth_burgers_pgen_vibe": (_I, [_P]),
+    "pth_burgers_init": (_I, [_P, _D, _I]),
+    "pth_burgers_run_cycles": (_I, [_P, _I, _I]),
+    "pth_burgers_clock": (_I, [_P, _DP, _DP]),
+    "pth_burgers_synchronize": (_I, [_P]),
     "pth_profile_enable": (_I, [_I]),
     "pth_profile_report": (_I, [C.c_char_p, _I]),
     "pth_last_error": (C.c_char_p, []),

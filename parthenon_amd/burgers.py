@@ -27,6 +27,43 @@ def make_burgers_mesh(nx=(64, 64, 64), block=(16, 16, 16), nghost=4, num_scalars
     return m
 
 
+class DeviceBurgersDriver:
+    """The same cycle with every host decision hoisted out: one reconstruction launch, one fused
+    flux/divergence/update launch and the exchange per stage, dt and time in device memory, and
+    (use_graph) one captured hipGraph replayed per cycle."""
+
+    def __init__(self, mesh, cfl=0.4, recon=burgers_package.WENO5, use_graph=True):
+        import ctypes as C
+        from . import _check, lib
+        self._C, self._check, self._lib = C, _check, lib
+        self.mesh = mesh
+        self.cfl = cfl
+        self.recon = recon
+        self.use_graph = use_graph
+        self.base = mesh.mesh_data("base")
+        self.one = mesh.mesh_data("1")
+
+    def initialize(self, pgen="sinusoid", seed=0):
+        if pgen == "sinusoid":
+            burgers_package.ProblemGeneratorSinusoid(self.base, seed)
+        elif pgen == "vibe":
+            burgers_package.ProblemGenerator(self.base)
+        self._check(self._lib().pth_burgers_init(self.mesh._h, self.cfl, self.recon), "init")
+
+    def cycles(self, n):
+        self._check(self._lib().pth_burgers_run_cycles(self.mesh._h, n, int(self.use_graph)),
+                    "run_cycles")
+
+    def synchronize(self):
+        self._check(self._lib().pth_burgers_synchronize(self.mesh._h), "synchronize")
+
+    def clock(self):
+        C = self._C
+        dt, t = C.c_double(), C.c_double()
+        self._check(self._lib().pth_burgers_clock(self.mesh._h, C.byref(dt), C.byref(t)), "clock")
+        return dt.value, t.value
+
+
 class BurgersDriver:
     def __init__(self, mesh, cfl=0.4, recon=burgers_package.WENO5):
         self.mesh = mesh

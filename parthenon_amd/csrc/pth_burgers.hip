@@ -304,6 +304,173 @@ __global__ void __launch_bounds__(64) MinFinalKernel(const double *__restrict__ 
   if (threadIdx.x == 0) out[0] = m;
 }
 
+// ---- fused stage path ----------------------------------------------------------------
+//
+// ReconTaskKernel: one thread = one WENO5Z (or linear) reconstruction of one cell in one
+// direction; blockIdx.y = block * V + comp. Per direction the task box is the interior extended
+// by one cell along that direction only, so every lane does the same amount of work.
+__global__ void __launch_bounds__(kWG)
+    ReconTaskKernel(const double *__restrict__ U, double *__restrict__ face, Geo G, int nblocks,
+                    int recon) {
+  const int n0 = G.hi0 - G.lo0 + 1, n1 = G.hi1 - G.lo1 + 1, n2 = G.hi2 - G.lo2 + 1;
+  const int per0 = (n0 + 2) * n1 * n2, per1 = n0 * (n1 + 2) * n2, per2 = n0 * n1 * (n2 + 2);
+  const int ntask = per0 + (G.ndim > 1 ? per1 : 0) + (G.ndim > 2 ? per2 : 0);
+  const int t = blockIdx.x * kWG + threadIdx.x;
+  if (t >= ntask) return;
+  const std::int64_t bc = blockIdx.y;  // block * V + comp
+  const std::int64_t slab = G.cells * G.V * nblocks;
+  const double *q = U + bc * G.cells;
+  double *f = face + bc * G.cells;
+  const int sj = G.e0, sk = G.e0 * G.e1;
+  int i, j, k, d, st;
+  if (t < per0) {
+    d = 0;
+    st = 1;
+    i = t % (n0 + 2) + G.lo0 - 1;
+    const int r = t / (n0 + 2);
+    j = r % n1 + G.lo1;
+    k = r / n1 + G.lo2;
+  } else if (t < per0 + per1) {
+    d = 1;
+    st = sj;
+    const int u = t - per0;
+    i = u % n0 + G.lo0;
+    const int r = u / n0;
+    j = r % (n1 + 2) + G.lo1 - 1;
+    k = r / (n1 + 2) + G.lo2;
+  } else {
+    d = 2;
+    st = sk;
+    const int u = t - per0 - per1;
+    i = u % n0 + G.lo0;
+    const int r = u / n0;
+    j = r % n1 + G.lo1;
+    k = r / n1 + G.lo2 - 1;
+  }
+  const int c = (k * G.e1 + j) * G.e0 + i;
+  double ql, qr;
+  if (recon == 0)
+    Weno5Z(q[c - 2 * st], q[c - st], q[c], q[c + st], q[c + 2 * st], ql, qr);
+  else
+    LinearRecon(q[c - st], q[c], q[c + st], ql, qr);
+  f[(2 * d) * slab + c + st] = ql;
+  f[(2 * d + 1) * slab + c] = qr;
+}
+
+// HLL flux of every component at one face from the face states (burgers_package.cpp:307-401
+// with lr_to_flux, burgers_package.hpp:33-43). L, R point at component 0 of the face in the
+// left/right state arrays; comp stride cs; d = normal direction.
+__device__ __forceinline__ void HllFace(const double *__restrict__ L, const double *__restrict__ R,
+                                        std::int64_t cs, int d, int V, double *F) {
+  const double upl = L[d * cs], upr = R[d * cs];
+  const double sl = DMin(DMin(upl, upr), 0.0);
+  const double sr = DMax(DMax(upl, upr), 0.0);
+  const double islsr = 1.0 / (sr - sl + (sl * sr == 0.0));
+#pragma unroll
+  for (int n = 0; n < 3; ++n) {
+    const double ul = L[n * cs], ur = R[n * cs];
+    F[n] = 0.5 * (sr * ul * upl - sl * ur * upr + sl * sr * (ur - ul)) * islsr;
+  }
+  for (int n = 3; n < V; ++n) {
+    const double ql = L[n * cs], qr = R[n * cs];
+    F[n] = (sr * upl * ql - sl * upr * qr + sl * sr * (qr - ql)) / (sr - sl + (sl * sr == 0.0));
+  }
+}
+
+constexpr int kMaxV = 16;  // components handled by the fused update kernel
+
+// FusedUpdateKernel: one thread = one interior cell. HLL fluxes on both faces of the cell in
+// every direction, flux divergence (update.hpp:43-57), the RK averaging and update in the
+// WeightedSumData form (update.hpp:71-90, 122-137), the derived field and the per-workgroup
+// CFL minimum of the updated state; optionally writes the low-face fluxes (and the last high
+// face) for flux correction. Reads face states written by ReconTaskKernel.
+__global__ void __launch_bounds__(kWG)
+    FusedUpdateKernel(const double *__restrict__ face, const double *U0, const double *base,
+                      double *out, double *__restrict__ derived, double *__restrict__ flux,
+                      double *__restrict__ partial, const BlockCoefs *__restrict__ coef, Geo G,
+                      int nblocks, double beta, const double *__restrict__ dtvar) {
+  __shared__ double wmin[kWG / 64];
+  const int n0 = G.hi0 - G.lo0 + 1, n1 = G.hi1 - G.lo1 + 1, n2 = G.hi2 - G.lo2 + 1;
+  const int b = blockIdx.y;
+  const int t = blockIdx.x * kWG + threadIdx.x;
+  double m = DBL_MAX;
+  if (t < n0 * n1 * n2) {
+    const int i = t % n0 + G.lo0;
+    const int r = t / n0;
+    const int j = r % n1 + G.lo1;
+    const int k = r / n1 + G.lo2;
+    const int V = G.V;
+    const std::int64_t slab = G.cells * V * nblocks;
+    const std::int64_t cs = G.cells;
+    const std::int64_t c0 = static_cast<std::int64_t>(b) * V * cs + (k * G.e1 + j) * G.e0 + i;
+    const BlockCoefs &C = coef[b];
+    const std::int64_t st[3] = {1, G.e0, static_cast<std::int64_t>(G.e0) * G.e1};
+    double du[kMaxV];
+    for (int d = 0; d < G.ndim; ++d) {
+      double Flo[kMaxV], Fhi[kMaxV];
+      const double *L = face + (2 * d) * slab + c0, *R = face + (2 * d + 1) * slab + c0;
+      HllFace(L, R, cs, d, V, Flo);
+      HllFace(L + st[d], R + st[d], cs, d, V, Fhi);
+      const double A = C.area[d];
+      for (int n = 0; n < V; ++n) {
+        const double term = (A * Fhi[n] - A * Flo[n]);
+        du[n] = d == 0 ? term : du[n] + term;
+      }
+      if (flux) {
+        double *Fo = flux + d * slab + c0;
+        for (int n = 0; n < V; ++n) Fo[n * cs] = Flo[n];
+        const bool last = (d == 0 && i == G.hi0) || (d == 1 && j == G.hi1) || (d == 2 && k == G.hi2);
+        if (last)
+          for (int n = 0; n < V; ++n) Fo[n * cs + st[d]] = Fhi[n];
+      }
+    }
+    const double bdt = beta * dtvar[0];
+    const double w2 = 1.0 - beta;
+    double u1[kMaxV];
+    for (int n = 0; n < V; ++n) {
+      const double dudt = -du[n] / C.vol;
+      const double avg = beta * U0[c0 + n * cs] + w2 * base[c0 + n * cs];
+      u1[n] = 1.0 * avg + bdt * dudt;
+      out[c0 + n * cs] = u1[n];
+    }
+    if (derived)
+      derived[static_cast<std::int64_t>(b) * cs + (k * G.e1 + j) * G.e0 + i] =
+          0.5 * u1[3] * (u1[0] * u1[0] + u1[1] * u1[1] + u1[2] * u1[2]);
+    double s = fabs(u1[0]) / C.dx[0];
+    if (G.ndim > 1) s = s + fabs(u1[1]) / C.dx[1];
+    if (G.ndim > 2) s = s + fabs(u1[2]) / C.dx[2];
+    m = 1.0 / s;
+  }
+  if (!partial) return;
+  m = WaveMin(m);
+  if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v = wmin[0];
+#pragma unroll
+    for (int w = 1; w < kWG / 64; ++w) v = DMin(v, wmin[w]);
+    partial[blockIdx.y * gridDim.x + blockIdx.x] = v;
+  }
+}
+
+// Final CFL minimum and EvolutionDriver::SetGlobalTimeStep on device (driver.cpp:210-235):
+// dtvar = {dt, time, estimate}; time += dt (when advance), dt = min(2 dt, cfl * min).
+__global__ void __launch_bounds__(64)
+    ClockKernel(const double *__restrict__ partial, int n, double cfl, double *dtvar,
+                int advance) {
+  double m = DBL_MAX;
+  for (int t = threadIdx.x; t < n; t += 64) m = DMin(m, partial[t]);
+  m = WaveMin(m);
+  if (threadIdx.x == 0) {
+    const double est = cfl * m;
+    double dt = dtvar[0];
+    if (advance) dtvar[1] = dtvar[1] + dt;
+    if (dt < 0.1 * DBL_MAX) dt *= 2.0;
+    dtvar[0] = DMin(dt, est);
+    dtvar[2] = est;
+  }
+}
+
 int Grid(std::int64_t total) {
   const std::int64_t want = (total + kWG - 1) / kWG;
   return static_cast<int>(want < 8192 ? want : 8192);
@@ -383,6 +550,52 @@ pth_status BurgersDerived(const Topology &t, int V, const double *U, double *out
   const std::int64_t total =
       static_cast<std::int64_t>(t.fine.n[0]) * t.fine.n[1] * t.fine.n[2] * nb;
   hipLaunchKernelGGL(DerivedKernel, dim3(Grid(total)), dim3(kWG), 0, s, U, out, G, nb);
+  PTH_HIP(hipGetLastError());
+  return PTH_COMPLETE;
+}
+
+pth_status BurgersReconTasks(const Topology &t, int V, const double *U, double *face, int recon,
+                             hipStream_t s) {
+  ProfScope prof("recon_tasks", s);
+  const Geo G = MakeGeo(t, V);
+  const int nb = static_cast<int>(t.blocks.size());
+  const int n0 = t.fine.n[0], n1 = t.fine.n[1], n2 = t.fine.n[2];
+  int ntask = (n0 + 2) * n1 * n2;
+  if (G.ndim > 1) ntask += n0 * (n1 + 2) * n2;
+  if (G.ndim > 2) ntask += n0 * n1 * (n2 + 2);
+  hipLaunchKernelGGL(ReconTaskKernel, dim3((ntask + kWG - 1) / kWG, nb * V), dim3(kWG), 0, s, U,
+                     face, G, nb, recon);
+  PTH_HIP(hipGetLastError());
+  return PTH_COMPLETE;
+}
+
+int FusedUpdatePartials(const Topology &t) {
+  const int cells = t.fine.n[0] * t.fine.n[1] * t.fine.n[2];
+  return ((cells + kWG - 1) / kWG) * static_cast<int>(t.blocks.size());
+}
+
+pth_status BurgersFusedUpdate(const Topology &t, int V, const double *face, const double *U0,
+                              const double *base, double *out, double *derived, double *flux,
+                              double *partial, const BlockCoefs *coef, double beta,
+                              const double *dtvar, hipStream_t s) {
+  if (V > kMaxV) {
+    SetError("fused Burgers update supports at most 16 components");
+    return PTH_FAIL;
+  }
+  ProfScope prof("fused_update", s);
+  const Geo G = MakeGeo(t, V);
+  const int nb = static_cast<int>(t.blocks.size());
+  const int cells = t.fine.n[0] * t.fine.n[1] * t.fine.n[2];
+  hipLaunchKernelGGL(FusedUpdateKernel, dim3((cells + kWG - 1) / kWG, nb), dim3(kWG), 0, s, face,
+                     U0, base, out, derived, flux, partial, coef, G, nb, beta, dtvar);
+  PTH_HIP(hipGetLastError());
+  return PTH_COMPLETE;
+}
+
+pth_status BurgersClock(const double *partial, int n, double cfl, double *dtvar, int advance,
+                        hipStream_t s) {
+  ProfScope prof("clock", s);
+  hipLaunchKernelGGL(ClockKernel, dim3(1), dim3(64), 0, s, partial, n, cfl, dtvar, advance);
   PTH_HIP(hipGetLastError());
   return PTH_COMPLETE;
 }

@@ -17,6 +17,17 @@ pth_status BurgersDerived(const Topology &t, int V, const double *U, double *out
 pth_status BurgersDtMin(const Topology &t, int V, const double *U, const BlockCoefs *coef,
                         double *partial, int npartial, double *out, hipStream_t s);
 
+// fused stage path
+pth_status BurgersReconTasks(const Topology &t, int V, const double *U, double *face, int recon,
+                             hipStream_t s);
+int FusedUpdatePartials(const Topology &t);
+pth_status BurgersFusedUpdate(const Topology &t, int V, const double *face, const double *U0,
+                              const double *base, double *out, double *derived, double *flux,
+                              double *partial, const BlockCoefs *coef, double beta,
+                              const double *dtvar, hipStream_t s);
+pth_status BurgersClock(const double *partial, int n, double cfl, double *dtvar, int advance,
+                        hipStream_t s);
+
 pth_status LaunchRestrictT(const Table &t, const Topology &topo, const DeviceBlockGeom *geom,
                            hipStream_t s);
 pth_status LaunchProlongT(const Table &t, const Topology &topo, const DeviceBlockGeom *geom,

@@ -84,6 +84,10 @@ Mesh::~Mesh() {
   if (geom) (void)hipFree(geom);
   if (coef) (void)hipFree(coef);
   if (face) (void)hipFree(face);
+  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (run_stream) (void)hipStreamDestroy(run_stream);
+  if (dtvar) (void)hipFree(dtvar);
   if (scratch) (void)hipFree(scratch);
   if (pinned) (void)hipHostFree(pinned);
 }

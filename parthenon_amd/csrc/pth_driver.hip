@@ -1,0 +1,214 @@
+// Device-resident Burgers driver: the task order of BurgersDriver::MakeTaskCollection
+// (benchmarks/burgers/burgers_driver.cpp:53-148) with the low-storage RK2 integrator
+// (src/time_integration/low_storage_integrator.cpp:67-83) and EvolutionDriver's time-step
+// control (src/driver/driver.cpp:194-235), with every host decision hoisted out of the stage:
+//
+//   stage 1: recon(base) -> fused update base -> "1" -> exchange("1")
+//   stage 2: recon("1")  -> fused update ("1", base) -> base + CFL partials -> exchange(base)
+//   clock:   time += dt; dt = min(2 dt, cfl * min)   (all on device)
+//
+// The simulation clock lives in device memory, so a whole cycle has no host synchronization
+// and is captured once into a hipGraph and replayed. The exchange inside a cycle uses the same
+// persistent pack / unpack / restrict / prolong tables as SendBoundBufs / SetBounds /
+// ProlongateBounds.
+//
+// Containers: the fused update writes only interior cells of the stage's output. Ghost cells of
+// that output are refilled by the exchange that follows, and the reference's in-place average
+// of "1" in stage 2 is never read again (stage 1 of the next cycle overwrites it), so the
+// trajectory of "base" is identical to the task-by-task form.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstring>
+
+#include "pth_burgers.hpp"
+
+namespace pth {
+namespace {
+
+pth_status Exchange(Mesh &m, MeshData &md, hipStream_t s) {
+  if (m.BuildCache(md, PTH_BT_ANY) != PTH_COMPLETE) return PTH_FAIL;
+  BoundCache &c = md.cache[PTH_BT_ANY];
+  if (c.nbuffers == 0) return PTH_COMPLETE;
+  if (m.topo.multilevel && LaunchRestrictT(c.restrict_send, m.topo, m.geom, s) != PTH_COMPLETE)
+    return PTH_FAIL;
+  if (LaunchPack(c.pack, nullptr, s, true, m.pack_mode) != PTH_COMPLETE) return PTH_FAIL;
+  if (LaunchPack(c.unpack, nullptr, s, false, m.pack_mode) != PTH_COMPLETE) return PTH_FAIL;
+  if (m.topo.multilevel) {
+    if (LaunchRestrictT(c.restrict_set, m.topo, m.geom, s) != PTH_COMPLETE) return PTH_FAIL;
+    if (LaunchProlongT(c.prolong, m.topo, m.geom, s, true) != PTH_COMPLETE) return PTH_FAIL;
+  }
+  return PTH_COMPLETE;
+}
+
+struct Ctx {
+  Mesh *m;
+  MeshData *base, *one;
+  int fu, fd, V;
+};
+
+pth_status Setup(Mesh *m, Ctx &x) {
+  if (!m->finalized) {
+    SetError("mesh is not finalized");
+    return PTH_FAIL;
+  }
+  if (m->topo.multilevel) {
+    SetError("the device Burgers driver needs flux correction on multilevel meshes (not built)");
+    return PTH_FAIL;
+  }
+  x.m = m;
+  x.fu = m->FieldIndex("U");
+  x.fd = m->FieldIndex("derived");
+  if (x.fu < 0 || m->fields[x.fu].ncomp < 4) {
+    SetError("Burgers driver needs field U with >= 4 components");
+    return PTH_FAIL;
+  }
+  x.V = m->fields[x.fu].ncomp;
+  x.base = m->Get("base");
+  x.one = m->Get("1");
+  if (!x.base || !x.one) return PTH_FAIL;
+  if (!m->run_stream) PTH_HIP(hipStreamCreateWithFlags(&m->run_stream, hipStreamNonBlocking));
+  if (!m->dtvar) PTH_HIP(hipMalloc(&m->dtvar, sizeof(double) * 4));
+  const std::int64_t nd = x.base->store[x.fu].ndata;
+  if (!m->face || m->face_V != x.V) {
+    if (m->face) (void)hipFree(m->face);
+    m->face = nullptr;
+    PTH_HIP(hipMalloc(&m->face, sizeof(double) * 6 * nd));
+    PTH_HIP(hipMemset(m->face, 0, sizeof(double) * 6 * nd));
+    m->face_V = x.V;
+  }
+  if (FusedUpdatePartials(m->topo) > m->scratch_n) {
+    SetError("too many CFL partials for the scratch area");
+    return PTH_FAIL;
+  }
+  return PTH_COMPLETE;
+}
+
+pth_status Stage(Ctx &x, int stage, hipStream_t s) {
+  Mesh &m = *x.m;
+  MeshData &mc0 = stage == 1 ? *x.base : *x.one;
+  MeshData &mc1 = stage == 1 ? *x.one : *x.base;
+  const double beta = stage == 1 ? 1.0 : 0.5;
+  double *derived = x.fd >= 0 ? x.base->store[x.fd].data : nullptr;
+  if (BurgersReconTasks(m.topo, x.V, mc0.store[x.fu].data, m.face, m.run_recon, s) !=
+      PTH_COMPLETE)
+    return PTH_FAIL;
+  if (BurgersFusedUpdate(m.topo, x.V, m.face, mc0.store[x.fu].data, x.base->store[x.fu].data,
+                         mc1.store[x.fu].data, derived, nullptr,
+                         stage == 2 ? m.scratch : nullptr, m.coef, beta, m.dtvar,
+                         s) != PTH_COMPLETE)
+    return PTH_FAIL;
+  if (Exchange(m, mc1, s) != PTH_COMPLETE) return PTH_FAIL;
+  if (stage == 2)
+    return BurgersClock(m.scratch, FusedUpdatePartials(m.topo), m.run_cfl, m.dtvar, 1, s);
+  return PTH_COMPLETE;
+}
+
+pth_status Cycle(Ctx &x, hipStream_t s) {
+  if (Stage(x, 1, s) != PTH_COMPLETE) return PTH_FAIL;
+  return Stage(x, 2, s);
+}
+
+}  // namespace
+}  // namespace pth
+
+using namespace pth;
+
+extern "C" {
+
+pth_status pth_burgers_init(pth_mesh *mh, double cfl, int recon) {
+  Mesh *m = Impl(mh);
+  Ctx x;
+  if (Setup(m, x) != PTH_COMPLETE) return PTH_FAIL;
+  for (int d = 0; d < m->topo.ndim; ++d)
+    if (m->topo.fine.g[d] < (recon == 0 ? 4 : 2)) {
+      SetError("weno5 reconstruction requires 4 or more ghost cells");
+      return PTH_FAIL;
+    }
+  m->run_cfl = cfl;
+  m->run_recon = recon;
+  if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
+  if (m->graph) (void)hipGraphDestroy(m->graph);
+  m->graph_exec = nullptr;
+  m->graph = nullptr;
+  hipStream_t s = m->run_stream;
+  // order after any work the caller queued on the mesh stream
+  PTH_HIP(hipStreamSynchronize(m->stream));
+  const double init[4] = {DBL_MAX, 0.0, 0.0, 0.0};
+  PTH_HIP(hipMemcpyAsync(m->dtvar, init, sizeof(init), hipMemcpyHostToDevice, s));
+  // Mesh::Initialize after the problem generator: exchange, FillDerived, first time step
+  if (Exchange(*m, *x.base, s) != PTH_COMPLETE) return PTH_FAIL;
+  if (x.fd >= 0 && BurgersDerived(m->topo, x.V, x.base->store[x.fu].data,
+                                  x.base->store[x.fd].data, s) != PTH_COMPLETE)
+    return PTH_FAIL;
+  const int np = static_cast<int>(m->scratch_n);
+  if (BurgersDtMin(m->topo, x.V, x.base->store[x.fu].data, m->coef, m->scratch, np,
+                   m->scratch + m->scratch_n, s) != PTH_COMPLETE)
+    return PTH_FAIL;
+  // the estimate of BurgersDtMin is the un-scaled minimum; apply the CFL and the clock rule
+  if (BurgersClock(m->scratch + m->scratch_n, 1, cfl, m->dtvar, 0, s) != PTH_COMPLETE)
+    return PTH_FAIL;
+  PTH_HIP(hipStreamSynchronize(s));
+  m->run_ready = true;
+  return PTH_COMPLETE;
+}
+
+pth_status pth_burgers_run_cycles(pth_mesh *mh, int ncycles, int use_graph) {
+  Mesh *m = Impl(mh);
+  if (!m->run_ready) {
+    SetError("call pth_burgers_init first");
+    return PTH_FAIL;
+  }
+  Ctx x;
+  if (Setup(m, x) != PTH_COMPLETE) return PTH_FAIL;
+  hipStream_t s = m->run_stream;
+  if (!use_graph) {
+    for (int c = 0; c < ncycles; ++c)
+      if (Cycle(x, s) != PTH_COMPLETE) return PTH_FAIL;
+    return PTH_COMPLETE;
+  }
+  if (!m->graph_exec) {
+    // tables must exist before capture (their uploads are synchronous copies)
+    if (m->BuildCache(*x.base, PTH_BT_ANY) != PTH_COMPLETE) return PTH_FAIL;
+    if (m->BuildCache(*x.one, PTH_BT_ANY) != PTH_COMPLETE) return PTH_FAIL;
+    const bool prof = Prof().on;
+    Prof().on = false;  // no event records inside the captured cycle
+    PTH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const pth_status st = Cycle(x, s);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    Prof().on = prof;
+    if (st != PTH_COMPLETE) {
+      if (g) (void)hipGraphDestroy(g);
+      return PTH_FAIL;
+    }
+    PTH_HIP(e);
+    m->graph = g;
+    PTH_HIP(hipGraphInstantiate(&m->graph_exec, g, nullptr, nullptr, 0));
+  }
+  ProfScope prof("graph_cycle", s);
+  for (int c = 0; c < ncycles; ++c) PTH_HIP(hipGraphLaunch(m->graph_exec, s));
+  return PTH_COMPLETE;
+}
+
+pth_status pth_burgers_clock(pth_mesh *mh, double *dt, double *time) {
+  Mesh *m = Impl(mh);
+  if (!m->run_ready) {
+    SetError("call pth_burgers_init first");
+    return PTH_FAIL;
+  }
+  double v[4];
+  PTH_HIP(hipStreamSynchronize(m->run_stream));
+  PTH_HIP(hipMemcpy(v, m->dtvar, sizeof(v), hipMemcpyDeviceToHost));
+  if (dt) *dt = v[0];
+  if (time) *time = v[1];
+  return PTH_COMPLETE;
+}
+
+pth_status pth_burgers_synchronize(pth_mesh *mh) {
+  Mesh *m = Impl(mh);
+  if (m->run_stream) PTH_HIP(hipStreamSynchronize(m->run_stream));
+  return PTH_COMPLETE;
+}
+
+}  // extern "C"

@@ -204,6 +204,14 @@ class Mesh {
   BlockCoefs *coef = nullptr;
   double *face = nullptr;                       // Burgers face-state scratch
   int face_V = 0;
+  // device-resident Burgers driver state: clock {dt, time, estimate} and the captured cycle
+  double *dtvar = nullptr;
+  hipStream_t run_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  double run_cfl = 0.0;
+  int run_recon = 0;
+  bool run_ready = false;
   hipStream_t stream = nullptr;
   int pack_mode = PTH_PACK_PERSISTENT;
   bool planned = false;    // host topology built

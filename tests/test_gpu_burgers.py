@@ -77,6 +77,42 @@ def test_ten_cycles_match_oracle(pgen):
     assert abs(d.time - lib().orc_burgers_time(ob.h)) <= RTOL * d.time
 
 
+@pytest.mark.parametrize("use_graph", [False, True], ids=["eager", "graph"])
+def test_device_driver_matches_oracle_every_cycle(use_graph):
+    # fused stage launches, device-resident clock; with the graph, one captured cycle replayed,
+    # with host downloads (eager work) between the replays, each replay checked
+    print(f"case device driver graph={use_graph}", flush=True)
+    from parthenon_amd.burgers import DeviceBurgersDriver
+    from oracle_lib import lib
+    nx, block, g, ns = (64, 64, 64), (16, 16, 16), 4, 4
+    m = make_burgers_mesh(nx, block, g, ns)
+    d = DeviceBurgersDriver(m, cfl=0.4, use_graph=use_graph)
+    d.initialize("sinusoid", seed=3)
+    ob = OracleBurgers(OracleMesh(nx, block, g), ns, 0.4, 0)
+    lib().orc_burgers_pgen_sinusoid(ob.h, 3)
+    lib().orc_burgers_initialize(ob.h)
+    dt, t = d.clock()
+    assert dt == ob.dt and t == 0.0
+    for c in range(4):
+        d.cycles(1)
+        lib().orc_burgers_cycles(ob.h, 1)
+        assert_close(d.base.download("U"), ob.get(0), f"U after cycle {c + 1}")
+        assert_close(d.base.download("derived"), ob.get(2), f"derived after cycle {c + 1}")
+        dt, t = d.clock()
+        assert dt == ob.dt and t == lib().orc_burgers_time(ob.h)
+    d.cycles(6)
+    lib().orc_burgers_cycles(ob.h, 6)
+    assert_close(d.base.download("U"), ob.get(0), "U after 10 cycles")
+
+
+def test_device_driver_rejects_small_nghost():
+    from parthenon_amd.burgers import DeviceBurgersDriver
+    m = make_burgers_mesh((32, 32, 32), (16, 16, 16), 2, 1)
+    d = DeviceBurgersDriver(m)
+    with pytest.raises(RuntimeError, match="ghost"):
+        d.initialize("sinusoid", seed=0)
+
+
 def test_conservation_to_roundoff():
     print("case conservation", flush=True)
     m, d, om, ob = pair(nx=(32, 32, 32), block=(8, 8, 8))
