@@ -87,9 +87,62 @@ print(json.dumps({{"wall": dt}}))
     return json.loads(r.stdout.strip().splitlines()[-1])["wall"]
 
 
+def exchange_bench(args):
+    """Ghost-exchange microbenchmark: 128^3 mesh in 512 blocks of 16^3, nghost 3, 8 FP64
+    one-component fields, U(-1,1) interiors; one step = SendBoundBufs + ReceiveBoundBufs +
+    SetBounds over all 26 neighbour buffers of every block."""
+    import parthenon_amd as pa
+    from parthenon_amd.inputs import interior_uniform
+
+    m = pa.Mesh((128, 128, 128), (16, 16, 16), 3)
+    for f in range(8):
+        m.add_field(f"q{f}", 1)
+    m.finalize()
+    m.set_pack_mode(pa.PackMode.per_buffer if args.pack == "per_buffer" else pa.PackMode.persistent)
+    md = m.mesh_data("base")
+    for f in range(8):
+        md.upload(f"q{f}", interior_uniform(m.field_shape(f"q{f}"), m.ng, seed=1000 + f, fill=0.0))
+    xchg = pa.exchange_fused if args.pack == "fused" else pa.exchange
+    for _ in range(args.warmup):
+        xchg(md)
+    m.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        xchg(md)
+    m.synchronize()
+    wall = time.perf_counter() - t0
+    pa.profile_enable(True)
+    pa.profile_report()
+    for _ in range(min(args.steps, 50)):
+        xchg(md)
+    m.synchronize()
+    prof = pa.profile_report()
+    pa.profile_enable(False)
+    nbuf, nel = m.exchange_counts()
+    alg = 4 * 8 * nel  # pack R+W and unpack R+W, FP64
+    us = {k: ms / c * 1e3 for k, (c, ms) in prof.items()}
+    dom = max(prof, key=lambda k: prof[k][1])
+    achieved = 16 * nel / (us[dom] * 1e-6) / 1e9
+    print(json.dumps({
+        "metric": "boundary exchanges/s (Send+Recv+Set, 26 buffers per block)",
+        "value": args.steps / wall, "unit": "exchanges/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic U(-1,1)",
+        "config": {"workload": "exchange 128^3 mesh, 512 blocks of 16^3, nghost 3, 8 fields",
+                   "pack": args.pack, "buffers": nbuf, "elements": nel},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": 16 * nel, "avg_launch_us": us[dom]},
+        "exchange_roofline": {"algorithmic_bytes": alg,
+                              "achieved_GBps": alg / (wall / args.steps) / 1e9},
+        "kernels_avg_us": us}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=["burgers", "exchange"], default="burgers")
+    ap.add_argument("--pack", choices=["persistent", "per_buffer", "fused"], default="persistent")
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--driver", choices=["graph", "eager", "tasks"], default="graph")
@@ -100,6 +153,8 @@ def main():
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is a design, not a measured path)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "exchange":
+        return exchange_bench(args)
 
     import parthenon_amd as pa
     from parthenon_amd.burgers import BurgersDriver, DeviceBurgersDriver, make_burgers_mesh

@@ -127,6 +127,14 @@ pth_status pth_set_bounds(pth_meshdata *md, int bt);
 /* ProlongateBounds<bt>: boundary_communication.cpp:361-393 (minmod shared prolongation). */
 pth_status pth_prolongate_bounds(pth_meshdata *md, int bt);
 
+/* Send + Receive + Set (+ ProlongateBounds on multilevel meshes) of bound type any in one
+ * call. On one GPU the buffer stage is skipped: every element moves from its sending cell to
+ * its receiving cell through the same region tables in one launch (send regions read only
+ * interior cells, receive regions write only ghost cells). Ghost data is identical to the
+ * buffered sequence. No counterpart in the reference, whose exchange is split into tasks so
+ * that MPI messages can overlap; it is for single-GPU drivers that do not need the buffers. */
+pth_status pth_exchange_fused(pth_meshdata *md, int bt);
+
 /* ---- Burgers benchmark package and Update:: tasks ---- */
 /* burgers_package::CalculateFluxes (benchmarks/burgers/burgers_package.cpp:202-404) on the
  * field "U" of md; recon 0 = WENO5Z, 1 = MC-limited linear. The reconstructed face states

@@ -97,6 +97,7 @@ _SIGNATURES = {
     "pth_burgers_estimate_timestep": (_I, [_P, _D, _DP]),
     "pth_burgers_pgen_sinusoid": (_I, [_P, C.c_uint64]),
     "pth_burgers_pgen_vibe": (_I, [_P]),
+    "pth_exchange_fused": (_I, [_P, _I]),
     "pth_burgers_init": (_I, [_P, _D, _I]),
     "pth_burgers_run_cycles": (_I, [_P, _I, _I]),
     "pth_burgers_clock": (_I, [_P, _DP, _DP]),
@@ -356,6 +357,11 @@ def profile_report():
         name, count, ms = line.split()
         out[name] = (int(count), float(ms))
     return out
+
+
+def exchange_fused(md, bt=BoundaryType.any):
+    """Send+Receive+Set(+Prolongate) in one call with the buffer stage skipped (one GPU)."""
+    return _check(lib().pth_exchange_fused(md._h, int(bt)), "exchange_fused")
 
 
 def exchange(md, bt=BoundaryType.any):

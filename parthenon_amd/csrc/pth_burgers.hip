@@ -360,8 +360,9 @@ __global__ void __launch_bounds__(kWG)
 // HLL flux of every component at one face from the face states (burgers_package.cpp:307-401
 // with lr_to_flux, burgers_package.hpp:33-43). L, R point at component 0 of the face in the
 // left/right state arrays; comp stride cs; d = normal direction.
+template <int V>
 __device__ __forceinline__ void HllFace(const double *__restrict__ L, const double *__restrict__ R,
-                                        std::int64_t cs, int d, int V, double *F) {
+                                        std::int64_t cs, int d, double *F) {
   const double upl = L[d * cs], upr = R[d * cs];
   const double sl = DMin(DMin(upl, upr), 0.0);
   const double sr = DMax(DMax(upl, upr), 0.0);
@@ -371,6 +372,7 @@ __device__ __forceinline__ void HllFace(const double *__restrict__ L, const doub
     const double ul = L[n * cs], ur = R[n * cs];
     F[n] = 0.5 * (sr * ul * upl - sl * ur * upr + sl * sr * (ur - ul)) * islsr;
   }
+#pragma unroll
   for (int n = 3; n < V; ++n) {
     const double ql = L[n * cs], qr = R[n * cs];
     F[n] = (sr * upl * ql - sl * upr * qr + sl * sr * (qr - ql)) / (sr - sl + (sl * sr == 0.0));
@@ -384,6 +386,7 @@ constexpr int kMaxV = 16;  // components handled by the fused update kernel
 // WeightedSumData form (update.hpp:71-90, 122-137), the derived field and the per-workgroup
 // CFL minimum of the updated state; optionally writes the low-face fluxes (and the last high
 // face) for flux correction. Reads face states written by ReconTaskKernel.
+template <int V>
 __global__ void __launch_bounds__(kWG)
     FusedUpdateKernel(const double *__restrict__ face, const double *U0, const double *base,
                       double *out, double *__restrict__ derived, double *__restrict__ flux,
@@ -399,34 +402,37 @@ __global__ void __launch_bounds__(kWG)
     const int r = t / n0;
     const int j = r % n1 + G.lo1;
     const int k = r / n1 + G.lo2;
-    const int V = G.V;
     const std::int64_t slab = G.cells * V * nblocks;
     const std::int64_t cs = G.cells;
     const std::int64_t c0 = static_cast<std::int64_t>(b) * V * cs + (k * G.e1 + j) * G.e0 + i;
     const BlockCoefs &C = coef[b];
     const std::int64_t st[3] = {1, G.e0, static_cast<std::int64_t>(G.e0) * G.e1};
-    double du[kMaxV];
+    double du[V];
     for (int d = 0; d < G.ndim; ++d) {
-      double Flo[kMaxV], Fhi[kMaxV];
+      double Flo[V], Fhi[V];
       const double *L = face + (2 * d) * slab + c0, *R = face + (2 * d + 1) * slab + c0;
-      HllFace(L, R, cs, d, V, Flo);
-      HllFace(L + st[d], R + st[d], cs, d, V, Fhi);
+      HllFace<V>(L, R, cs, d, Flo);
+      HllFace<V>(L + st[d], R + st[d], cs, d, Fhi);
       const double A = C.area[d];
+#pragma unroll
       for (int n = 0; n < V; ++n) {
         const double term = (A * Fhi[n] - A * Flo[n]);
         du[n] = d == 0 ? term : du[n] + term;
       }
       if (flux) {
         double *Fo = flux + d * slab + c0;
+#pragma unroll
         for (int n = 0; n < V; ++n) Fo[n * cs] = Flo[n];
         const bool last = (d == 0 && i == G.hi0) || (d == 1 && j == G.hi1) || (d == 2 && k == G.hi2);
         if (last)
+#pragma unroll
           for (int n = 0; n < V; ++n) Fo[n * cs + st[d]] = Fhi[n];
       }
     }
     const double bdt = beta * dtvar[0];
     const double w2 = 1.0 - beta;
-    double u1[kMaxV];
+    double u1[V];
+#pragma unroll
     for (int n = 0; n < V; ++n) {
       const double dudt = -du[n] / C.vol;
       const double avg = beta * U0[c0 + n * cs] + w2 * base[c0 + n * cs];
@@ -582,12 +588,36 @@ pth_status BurgersFusedUpdate(const Topology &t, int V, const double *face, cons
     SetError("fused Burgers update supports at most 16 components");
     return PTH_FAIL;
   }
+  if (V < 4) {
+    SetError("fused Burgers update needs at least 4 components");
+    return PTH_FAIL;
+  }
   ProfScope prof("fused_update", s);
   const Geo G = MakeGeo(t, V);
   const int nb = static_cast<int>(t.blocks.size());
   const int cells = t.fine.n[0] * t.fine.n[1] * t.fine.n[2];
-  hipLaunchKernelGGL(FusedUpdateKernel, dim3((cells + kWG - 1) / kWG, nb), dim3(kWG), 0, s, face,
-                     U0, base, out, derived, flux, partial, coef, G, nb, beta, dtvar);
+  const dim3 grid((cells + kWG - 1) / kWG, nb);
+#define PTH_FUSED_CASE(NV)                                                                     \
+  case NV:                                                                                     \
+    hipLaunchKernelGGL(FusedUpdateKernel<NV>, grid, dim3(kWG), 0, s, face, U0, base, out,      \
+                       derived, flux, partial, coef, G, nb, beta, dtvar);                      \
+    break;
+  switch (V) {
+    PTH_FUSED_CASE(4)
+    PTH_FUSED_CASE(5)
+    PTH_FUSED_CASE(6)
+    PTH_FUSED_CASE(7)
+    PTH_FUSED_CASE(8)
+    PTH_FUSED_CASE(9)
+    PTH_FUSED_CASE(10)
+    PTH_FUSED_CASE(11)
+    PTH_FUSED_CASE(12)
+    PTH_FUSED_CASE(13)
+    PTH_FUSED_CASE(14)
+    PTH_FUSED_CASE(15)
+    PTH_FUSED_CASE(16)
+  }
+#undef PTH_FUSED_CASE
   PTH_HIP(hipGetLastError());
   return PTH_COMPLETE;
 }

@@ -3,7 +3,9 @@
 // here; every data operation is a device kernel on the mesh's stream.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <string>
 
@@ -79,8 +81,7 @@ Mesh::~Mesh() {
     if (s.coarse) (void)hipFree(s.coarse);
     if (s.flux) (void)hipFree(s.flux);
   }
-  for (auto *a : arena)
-    if (a) (void)hipFree(a);
+  if (arena_base) (void)hipFree(arena_base);
   if (geom) (void)hipFree(geom);
   if (coef) (void)hipFree(coef);
   if (face) (void)hipFree(face);
@@ -196,41 +197,57 @@ pth_status Mesh::Finalize() {
   for (size_t f = 0; f < fields.size(); ++f)
     if (fields[f].flags & PTH_ONE_COPY)
       if (AllocStore(*this, fields[f], one_copy[f]) != PTH_COMPLETE) return PTH_FAIL;
-  // communication channels: one contiguous slice per (sender, receiver, field, location)
+  // communication channels: one arena for every FillGhost field, one contiguous slice per
+  // (sender, receiver, field, location), laid out receiver-block-major: the unpack then writes
+  // each receiving brick's ghost shell in one pass (partial cache lines merge in L2), and the
+  // pack gathers from the senders
   arena.assign(fields.size(), nullptr);
   arena_size.assign(fields.size(), 0);
   channel_offset.clear();
-  for (size_t f = 0; f < fields.size(); ++f) {
-    if (!(fields[f].flags & PTH_FILL_GHOST)) continue;
-    std::int64_t off = 0;
-    std::map<Channel, std::int64_t> slice;
-    for (int b = 0; b < nb; ++b) {
-      const Block &B = topo.blocks[b];
+  std::map<Channel, std::int64_t> slice;
+  for (int b = 0; b < nb; ++b) {
+    const Block &B = topo.blocks[b];
+    for (size_t f = 0; f < fields.size(); ++f) {
+      if (!(fields[f].flags & PTH_FILL_GHOST)) continue;
       for (const Link &l : B.links) {
         const Side side = l.is_cell() ? Side::kSendInterior : Side::kSendBoundary;
-        const Box box = topo.Indices(B, l, side, false);
-        const Channel key(b, l.gid, static_cast<int>(f), l.loc_index());
-        channel_offset[key] = off;
-        slice[key] = box.count() * fields[f].ncomp;
-        off += slice[key];
+        slice[Channel(b, l.gid, static_cast<int>(f), l.loc_index())] =
+            topo.Indices(B, l, side, false).count() * fields[f].ncomp;
       }
     }
-    // every receiving region must match the size of the slice its sender packs
-    for (int b = 0; b < nb; ++b) {
-      const Block &B = topo.blocks[b];
+  }
+  // every receiving region must match the size of the slice its sender packs
+  std::int64_t off = 0;
+  for (int b = 0; b < nb; ++b) {
+    const Block &B = topo.blocks[b];
+    for (size_t f = 0; f < fields.size(); ++f) {
+      if (!(fields[f].flags & PTH_FILL_GHOST)) continue;
       for (const Link &l : B.links) {
         const Side side = l.is_cell() ? Side::kRecvInterior : Side::kRecvBoundary;
         const Box box = topo.Indices(B, l, side, false);
         const Channel key(l.gid, b, static_cast<int>(f), l.reverse_index());
         auto it = slice.find(key);
-        if (it == slice.end() || it->second != box.count() * fields[f].ncomp) {
+        if (it == slice.end() || it->second != box.count() * fields[f].ncomp ||
+            channel_offset.count(key)) {
           SetError("receiving region does not match the slice its sender packs");
           return PTH_FAIL;
         }
+        channel_offset[key] = off;
+        off += it->second;
       }
     }
+  }
+  if (channel_offset.size() != slice.size()) {
+    SetError("a sending region has no receiver");
+    return PTH_FAIL;
+  }
+  double *all = nullptr;
+  if (Alloc(&all, off) != PTH_COMPLETE) return PTH_FAIL;
+  arena_base = all;
+  arena_total = off;
+  for (size_t f = 0; f < fields.size(); ++f) {
+    arena[f] = all;
     arena_size[f] = off;
-    if (Alloc(&arena[f], off) != PTH_COMPLETE) return PTH_FAIL;
   }
   scratch_n = 8192;
   PTH_HIP(hipMalloc(&scratch, sizeof(double) * (scratch_n + 8)));
@@ -347,6 +364,54 @@ pth_status Mesh::BuildCache(MeshData &md, int bt) {
   if (!ok) {
     SetError("internal: a boundary region exceeds its array extents");
     return PTH_FAIL;
+  }
+  // The order of regions within a pack or an unpack is free (their writes are disjoint), so
+  // order both like the arena; then both tables walk the arena front to back and get a
+  // per-element gather index.
+  for (Table *t : {&c.pack, &c.unpack})
+    std::sort(t->host.begin(), t->host.end(),
+              [](const Region &x, const Region &y) { return x.b < y.b; });
+  std::vector<std::pair<double *, std::int64_t>> arrays;  // candidate gather bases
+  for (size_t f = 0; f < fields.size(); ++f) {
+    if (!(fields[f].flags & PTH_FILL_GHOST)) continue;
+    arrays.emplace_back(md.store[f].data, md.store[f].ndata);
+    if (md.store[f].coarse) arrays.emplace_back(md.store[f].coarse, md.store[f].ncoarse);
+  }
+  for (Table *t : {&c.pack, &c.unpack}) {
+    t->idx.clear();
+    t->bases.clear();
+    t->arena = nullptr;
+    bool gather = static_cast<int>(arrays.size()) <= kIdxSlots && arena_base != nullptr;
+    for (const auto &a : arrays) gather = gather && a.second <= static_cast<std::int64_t>(kIdxMask);
+    std::int64_t pos = 0;
+    for (const Region &r : t->host) {
+      gather = gather && r.b == arena_base + pos;
+      pos += static_cast<std::int64_t>(r.n[0]) * r.n[1] * r.n[2] * r.nc;
+    }
+    gather = gather && pos == arena_total;
+    if (!gather) continue;  // the table keeps its region walk
+    for (const auto &a : arrays) t->bases.push_back(a.first);
+    t->idx.reserve(pos);
+    for (const Region &r : t->host) {
+      std::uint32_t slot = 0;
+      std::int64_t rel = 0;
+      for (size_t s = 0; s < arrays.size(); ++s)
+        if (r.a >= arrays[s].first && r.a < arrays[s].first + arrays[s].second) {
+          slot = static_cast<std::uint32_t>(s);
+          rel = r.a - arrays[s].first;
+        }
+      for (int cc = 0; cc < r.nc; ++cc)
+        for (int k = 0; k < r.n[2]; ++k)
+          for (int j = 0; j < r.n[1]; ++j)
+            for (int i = 0; i < r.n[0]; ++i) {
+              const std::int64_t o = rel + static_cast<std::int64_t>(cc) * r.astride[2] +
+                                     static_cast<std::int64_t>(k + r.lo[2]) * r.astride[1] +
+                                     static_cast<std::int64_t>(j + r.lo[1]) * r.astride[0] +
+                                     (i + r.lo[0]);
+              t->idx.push_back((slot << kIdxShift) | static_cast<std::uint32_t>(o));
+            }
+    }
+    t->arena = arena_base;
   }
   for (Table *t : {&c.pack, &c.unpack, &c.restrict_send, &c.restrict_set, &c.prolong})
     if (t->Upload() != PTH_COMPLETE) return PTH_FAIL;

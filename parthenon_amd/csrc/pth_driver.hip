@@ -32,8 +32,12 @@ pth_status Exchange(Mesh &m, MeshData &md, hipStream_t s) {
   if (c.nbuffers == 0) return PTH_COMPLETE;
   if (m.topo.multilevel && LaunchRestrictT(c.restrict_send, m.topo, m.geom, s) != PTH_COMPLETE)
     return PTH_FAIL;
-  if (LaunchPack(c.pack, nullptr, s, true, m.pack_mode) != PTH_COMPLETE) return PTH_FAIL;
-  if (LaunchPack(c.unpack, nullptr, s, false, m.pack_mode) != PTH_COMPLETE) return PTH_FAIL;
+  if (m.pack_mode == PTH_PACK_PERSISTENT && CanExchangeDirect(c.pack, c.unpack)) {
+    if (LaunchExchangeDirect(c.pack, c.unpack, s) != PTH_COMPLETE) return PTH_FAIL;
+  } else {
+    if (LaunchPack(c.pack, nullptr, s, true, m.pack_mode) != PTH_COMPLETE) return PTH_FAIL;
+    if (LaunchPack(c.unpack, nullptr, s, false, m.pack_mode) != PTH_COMPLETE) return PTH_FAIL;
+  }
   if (m.topo.multilevel) {
     if (LaunchRestrictT(c.restrict_set, m.topo, m.geom, s) != PTH_COMPLETE) return PTH_FAIL;
     if (LaunchProlongT(c.prolong, m.topo, m.geom, s, true) != PTH_COMPLETE) return PTH_FAIL;
@@ -115,6 +119,22 @@ pth_status Cycle(Ctx &x, hipStream_t s) {
 using namespace pth;
 
 extern "C" {
+
+pth_status pth_exchange_fused(pth_meshdata *mdh, int bt) {
+  MeshData *md = Impl(mdh);
+  Mesh *m = md->mesh;
+  if (bt == PTH_BT_LOCAL) bt = PTH_BT_ANY;
+  if (bt == PTH_BT_NONLOCAL) return PTH_COMPLETE;
+  if (bt != PTH_BT_ANY) {
+    SetError("fused exchange serves the cell-variable bound types any/local/nonlocal");
+    return PTH_FAIL;
+  }
+  if (!m->finalized) {
+    SetError("mesh is not finalized");
+    return PTH_FAIL;
+  }
+  return Exchange(*m, *md, m->stream);
+}
 
 pth_status pth_burgers_init(pth_mesh *mh, double cfl, int recon) {
   Mesh *m = Impl(mh);

@@ -148,14 +148,34 @@ struct BlockCoefs {  // per-block uniform-Cartesian metrics used by the stencils
   double pad;
 };
 
+struct Item {  // a run of consecutive elements over up to kItemRegions consecutive regions
+  int region;    // first region
+  int nreg;      // regions touched
+  int begin;     // element offset within the first region
+  int count;     // elements
+};
+constexpr int kItemRegions = 8;
+constexpr int kItemElems = 4096;
+
 struct Table {
   std::vector<Region> host;
+  std::vector<Item> items;
   Region *dev = nullptr;
-  std::int64_t *dev_start = nullptr;  // region start offsets (for the binary search)
-  std::int64_t total = 0;             // total elements
+  Item *dev_items = nullptr;
+  std::int64_t total = 0;  // total elements
+  // Gather form of a pack/unpack table whose elements are in arena order: element e moves
+  // between arena[e] and bases[idx[e] >> kIdxShift][idx[e] & kIdxMask].
+  std::vector<std::uint32_t> idx;
+  std::vector<double *> bases;
+  std::uint32_t *dev_idx = nullptr;
+  double **dev_bases = nullptr;
+  double *arena = nullptr;
   void Free();
   pth_status Upload();
 };
+constexpr int kIdxShift = 27;
+constexpr std::uint32_t kIdxMask = (1u << kIdxShift) - 1;
+constexpr int kIdxSlots = 32;
 
 struct FieldSpec {
   std::string name;
@@ -196,8 +216,10 @@ class Mesh {
   std::vector<FieldSpec> fields;
   std::map<std::string, std::unique_ptr<MeshData>> data;
   std::vector<FieldStore> one_copy;             // shared storage of OneCopy fields
-  std::vector<double *> arena;                  // comm buffer arena per field
+  std::vector<double *> arena;                  // comm buffer arena seen by each field
   std::vector<std::int64_t> arena_size;
+  double *arena_base = nullptr;                 // the single arena of every FillGhost field
+  std::int64_t arena_total = 0;
   std::map<Channel, std::int64_t> channel_offset;  // element offset within the field arena
   BufState state[10] = {};
   DeviceBlockGeom *geom = nullptr;
@@ -258,6 +280,8 @@ class ProfScope {  // records start/stop events around the launches in its lifet
 // kernels (pth_bvals.hip)
 pth_status LaunchPack(const Table &t, double *arena_unused, hipStream_t s, bool to_buffer,
                       int mode);
+bool CanExchangeDirect(const Table &pack, const Table &unpack);
+pth_status LaunchExchangeDirect(const Table &pack, const Table &unpack, hipStream_t s);
 pth_status LaunchRestrict(const Table &t, const DeviceBlockGeom *geom, hipStream_t s);
 pth_status LaunchProlong(const Table &t, const DeviceBlockGeom *geom, hipStream_t s);
 

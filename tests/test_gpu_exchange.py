@@ -37,20 +37,23 @@ def test_ghosts_equal_wrapped_interior(nx, block, g):
     assert np.array_equal(md.download("q0"), exp)
 
 
-@pytest.mark.parametrize("mode", [pa.PackMode.persistent, pa.PackMode.per_buffer])
+@pytest.mark.parametrize("mode", ["persistent", "per_buffer", "fused"])
 def test_exchange_microbenchmark_shape_matches_oracle(mode):
     # 128^3 mesh in 512 blocks of 16^3, nghost 3, 8 one-component fields, U(-1,1) interiors
-    print(f"case 128^3/16^3 g=3 x8 mode={mode.name}", flush=True)
+    print(f"case 128^3/16^3 g=3 x8 mode={mode}", flush=True)
     nx, block, g = (128, 128, 128), (16, 16, 16), 3
     m = product_mesh(nx, block, g, nfields=8)
-    m.set_pack_mode(mode)
+    m.set_pack_mode(pa.PackMode.per_buffer if mode == "per_buffer" else pa.PackMode.persistent)
     md = m.mesh_data("base")
     o = OracleMesh(nx, block, g)
     fields = [interior_uniform(m.field_shape(f"q{f}"), m.ng, seed=1000 + f, fill=0.0)
               for f in range(8)]
     for f, a in enumerate(fields):
         md.upload(f"q{f}", a)
-    pa.exchange(md)
+    if mode == "fused":
+        pa.exchange_fused(md)
+    else:
+        pa.exchange(md)
     o.exchange(fields)
     exp = global_index_field(m, 1, nx)
     for f in range(8):
@@ -102,8 +105,9 @@ def test_multilevel_linear_field_closed_form(spec):
 @pytest.mark.parametrize("spec", REFINED + [
     dict(nx=(64, 64, 64), block=(16, 16, 16), g=4, spheres=[((0.0, 0.0, 0.0), 0.25, 2)])],
     ids=["box2", "two-boxes", "sphere960"])
-def test_multilevel_random_data_matches_oracle(spec):
-    print(f"case refined random {spec}", flush=True)
+@pytest.mark.parametrize("fused", [False, True], ids=["tasks", "fused"])
+def test_multilevel_random_data_matches_oracle(spec, fused):
+    print(f"case refined random {spec} fused={fused}", flush=True)
     boxes, spheres = spec.get("boxes", ()), spec.get("spheres", ())
     m = product_mesh(spec["nx"], spec["block"], spec["g"], ncomp=3, refine_boxes=boxes,
                      refine_spheres=spheres)
@@ -113,7 +117,10 @@ def test_multilevel_random_data_matches_oracle(spec):
     c = np.zeros(m.coarse_shape("q0"))
     md.upload("q0", f)
     md.upload("q0", c, which=1)
-    pa.exchange(md)
+    if fused:
+        pa.exchange_fused(md)
+    else:
+        pa.exchange(md)
     o.exchange([f], coarse=[c], prolong=1)
     got, gotc = md.download("q0"), md.download("q0", which=1)
     assert np.array_equal(gotc, c)           # restriction into coarse buffers: bit-exact
