@@ -1,0 +1,227 @@
+// parthenon_amd.hpp — C++ mirror of the reference's task-level interface for the per-stage
+// meshblock data path, implemented over the C ABI of parthenon_amd.h (header-only).
+//
+// A driver written against the reference (e.g. benchmarks/burgers/burgers_driver.cpp) calls
+// the same names with the same argument meaning:
+//   parthenon::SendBoundBufs<BT>, StartReceiveBoundBufs<BT>, ReceiveBoundBufs<BT>,
+//   SetBounds<BT>, ProlongateBounds<BT>        src/bvals/comms/bvals_in_one.hpp:41-67
+//   parthenon::Update::FluxDivergence, AverageIndependentData, UpdateIndependentData,
+//   WeightedSumData                           src/interface/update.hpp:59-137
+//   burgers_package::CalculateFluxes, CalculateDerived, EstimateTimestepMesh
+//                                             benchmarks/burgers/burgers_package.hpp:24-30
+//   MeshData<Real> from Mesh::mesh_data.GetOrAdd(name, partition)
+//                                             src/interface/mesh_data.hpp:191, burgers_driver.cpp:76
+// Error behaviour: TaskStatus::fail from the library becomes a std::runtime_error carrying the
+// library's message (the reference's PARTHENON_REQUIRE_THROWS / PARTHENON_FAIL).
+#ifndef PARTHENON_AMD_HPP_
+#define PARTHENON_AMD_HPP_
+
+#include <array>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "parthenon_amd.h"
+
+namespace parthenon {
+
+using Real = double;
+
+enum class TaskStatus { complete, incomplete, iterate, fail };  // src/basic_types.hpp:59
+
+enum class BoundaryType : int {  // src/basic_types.hpp:156
+  any = PTH_BT_ANY,
+  local = PTH_BT_LOCAL,
+  nonlocal = PTH_BT_NONLOCAL,
+  flxcor_send = PTH_BT_FLXCOR_SEND,
+  flxcor_recv = PTH_BT_FLXCOR_RECV,
+  gmg_same = PTH_BT_GMG_SAME,
+  gmg_restrict_send = PTH_BT_GMG_RESTRICT_SEND,
+  gmg_restrict_recv = PTH_BT_GMG_RESTRICT_RECV,
+  gmg_prolongate_send = PTH_BT_GMG_PROLONGATE_SEND,
+  gmg_prolongate_recv = PTH_BT_GMG_PROLONGATE_RECV
+};
+
+struct Metadata {  // the cell-variable flags used on this path (src/interface/metadata.hpp)
+  enum Flag : unsigned {
+    FillGhost = PTH_FILL_GHOST,
+    Independent = PTH_INDEPENDENT,
+    WithFluxes = PTH_WITH_FLUXES,
+    OneCopy = PTH_ONE_COPY
+  };
+};
+
+namespace detail {
+inline TaskStatus Check(pth_status s, const char *what) {
+  if (s == PTH_FAIL) {
+    const char *msg = pth_last_error();
+    throw std::runtime_error(std::string(what) + ": " + (msg ? msg : "unknown error"));
+  }
+  return static_cast<TaskStatus>(s);
+}
+}  // namespace detail
+
+class Mesh;
+
+template <typename T>
+class MeshData {
+ public:
+  MeshData(Mesh *mesh, pth_meshdata *h, std::string name)
+      : mesh_(mesh), h_(h), name_(std::move(name)) {}
+  Mesh *GetMeshPointer() const { return mesh_; }
+  Mesh *GetParentPointer() const { return mesh_; }
+  pth_meshdata *handle() const { return h_; }
+  const std::string &StageName() const { return name_; }
+  // host copies of a whole field (which: 0 data, 1 coarse buffer, 2 face fluxes)
+  void Upload(const std::string &field, const std::vector<T> &host, int which = 0) {
+    detail::Check(pth_field_upload(h_, field.c_str(), which, host.data(), host.size()), "Upload");
+  }
+  void Download(const std::string &field, std::vector<T> &host, int which = 0) const {
+    detail::Check(pth_field_download(h_, field.c_str(), which, host.data(), host.size()),
+                  "Download");
+  }
+
+ private:
+  Mesh *mesh_;
+  pth_meshdata *h_;
+  std::string name_;
+};
+
+// Mesh: a uniform or statically refined Cartesian mesh of equal meshblocks on one GPU.
+class Mesh {
+ public:
+  struct Field {
+    std::string name;
+    int ncomp;
+    unsigned flags;
+  };
+  Mesh(std::array<int, 3> nx, std::array<int, 3> block, int nghost,
+       std::array<Real, 3> xmin = {-0.5, -0.5, -0.5}, std::array<Real, 3> xmax = {0.5, 0.5, 0.5},
+       std::array<int, 3> periodic = {1, 1, 1}) {
+    h_ = pth_mesh_create(nx.data(), block.data(), nghost, xmin.data(), xmax.data(),
+                         periodic.data());
+    if (!h_) throw std::runtime_error(std::string("Mesh: ") + pth_last_error());
+  }
+  ~Mesh() {
+    data_.clear();
+    if (h_) pth_mesh_destroy(h_);
+  }
+  Mesh(const Mesh &) = delete;
+  Mesh &operator=(const Mesh &) = delete;
+
+  // <parthenon/static_refinementN> (src/mesh/mesh.cpp:1085-1116)
+  void AddStaticRefinement(std::array<Real, 6> box, int level) {
+    detail::Check(pth_mesh_refine_box(h_, box.data(), level), "AddStaticRefinement");
+  }
+  // StateDescriptor::AddField for a cell-centred field
+  void AddField(const std::string &name, int ncomp, unsigned flags) {
+    detail::Check(pth_mesh_add_field(h_, name.c_str(), ncomp, flags), "AddField");
+  }
+  void Initialize() { detail::Check(pth_mesh_finalize(h_), "Initialize"); }
+
+  int nbtotal() const { return pth_mesh_nblocks(h_); }
+  bool multilevel() const { return pth_mesh_multilevel(h_) != 0; }
+  pth_mesh *handle() const { return h_; }
+
+  // mesh_data.GetOrAdd(name, partition): one partition (every block) on one GPU
+  std::shared_ptr<MeshData<Real>> &GetOrAdd(const std::string &name, int partition = 0) {
+    if (partition != 0) throw std::runtime_error("GetOrAdd: one partition per GPU");
+    auto it = data_.find(name);
+    if (it == data_.end()) {
+      pth_meshdata *h = pth_meshdata_get(h_, name.c_str());
+      if (!h) throw std::runtime_error(std::string("GetOrAdd: ") + pth_last_error());
+      it = data_.emplace(name, std::make_shared<MeshData<Real>>(this, h, name)).first;
+    }
+    return it->second;
+  }
+  int DefaultNumPartitions() const { return 1; }
+
+ private:
+  pth_mesh *h_ = nullptr;
+  std::map<std::string, std::shared_ptr<MeshData<Real>>> data_;
+};
+
+// ---- boundary communication tasks (src/bvals/comms/bvals_in_one.hpp:41-67) ----
+template <BoundaryType BT>
+TaskStatus SendBoundBufs(std::shared_ptr<MeshData<Real>> &md) {
+  return detail::Check(pth_send_bound_bufs(md->handle(), static_cast<int>(BT)), "SendBoundBufs");
+}
+template <BoundaryType BT>
+TaskStatus StartReceiveBoundBufs(std::shared_ptr<MeshData<Real>> &md) {
+  return detail::Check(pth_start_receive_bound_bufs(md->handle(), static_cast<int>(BT)),
+                       "StartReceiveBoundBufs");
+}
+template <BoundaryType BT>
+TaskStatus ReceiveBoundBufs(std::shared_ptr<MeshData<Real>> &md) {
+  return detail::Check(pth_receive_bound_bufs(md->handle(), static_cast<int>(BT)),
+                       "ReceiveBoundBufs");
+}
+template <BoundaryType BT>
+TaskStatus SetBounds(std::shared_ptr<MeshData<Real>> &md) {
+  return detail::Check(pth_set_bounds(md->handle(), static_cast<int>(BT)), "SetBounds");
+}
+template <BoundaryType BT>
+TaskStatus ProlongateBounds(std::shared_ptr<MeshData<Real>> &md) {
+  return detail::Check(pth_prolongate_bounds(md->handle(), static_cast<int>(BT)),
+                       "ProlongateBounds");
+}
+// aliases of bvals_in_one.hpp:50-61
+inline TaskStatus SendBoundaryBuffers(std::shared_ptr<MeshData<Real>> &md) {
+  return SendBoundBufs<BoundaryType::any>(md);
+}
+inline TaskStatus ReceiveBoundaryBuffers(std::shared_ptr<MeshData<Real>> &md) {
+  return ReceiveBoundBufs<BoundaryType::any>(md);
+}
+inline TaskStatus SetBoundaries(std::shared_ptr<MeshData<Real>> &md) {
+  return SetBounds<BoundaryType::any>(md);
+}
+
+// ---- Update:: tasks (src/interface/update.hpp) ----
+namespace Update {
+template <typename T>
+TaskStatus FluxDivergence(T *in, T *dudt) {
+  return detail::Check(pth_flux_divergence(in->handle(), dudt->handle()), "FluxDivergence");
+}
+template <typename T>
+TaskStatus WeightedSumDataIndependent(T *in1, T *in2, const Real w1, const Real w2, T *out) {
+  return detail::Check(
+      pth_weighted_sum_independent(in1->handle(), in2->handle(), w1, w2, out->handle()),
+      "WeightedSumData");
+}
+template <typename T>
+TaskStatus UpdateIndependentData(T *in, T *dudt, const Real dt, T *out) {
+  return WeightedSumDataIndependent(in, dudt, 1.0, dt, out);
+}
+template <typename T>
+TaskStatus AverageIndependentData(T *c1, T *c2, const Real wgt1) {
+  return WeightedSumDataIndependent(c1, c2, wgt1, (1.0 - wgt1), c1);
+}
+}  // namespace Update
+
+}  // namespace parthenon
+
+// ---- the Burgers benchmark package (benchmarks/burgers/burgers_package.hpp) ----
+namespace burgers_package {
+using parthenon::MeshData;
+using parthenon::Real;
+using parthenon::TaskStatus;
+enum class ReconType { WENO5 = 0, Linear = 1 };
+
+inline TaskStatus CalculateFluxes(MeshData<Real> *md, ReconType recon = ReconType::WENO5) {
+  return parthenon::detail::Check(
+      pth_burgers_calculate_fluxes(md->handle(), static_cast<int>(recon)), "CalculateFluxes");
+}
+inline void CalculateDerived(MeshData<Real> *md) {
+  parthenon::detail::Check(pth_burgers_calculate_derived(md->handle()), "CalculateDerived");
+}
+inline Real EstimateTimestepMesh(MeshData<Real> *md, Real cfl) {
+  Real dt = 0;
+  parthenon::detail::Check(pth_burgers_estimate_timestep(md->handle(), cfl, &dt),
+                           "EstimateTimestepMesh");
+  return dt;
+}
+}  // namespace burgers_package
+
+#endif  // PARTHENON_AMD_HPP_

@@ -355,6 +355,58 @@ void orc_burgers_set(void *h, int which, const double *in) {
   std::memcpy(v->data(), in, sizeof(double) * v->size());
 }
 
+void orc_burgers_set_flux_correction(void *h, int on) {
+  static_cast<OBurgers *>(h)->b.flux_correction = on != 0;
+}
+
+// Sum over blocks and interior cells of U * cell volume, per component (out[V]), in block /
+// k / j / i order: the discrete conserved totals.
+void orc_burgers_totals(void *h, double *out) {
+  Burgers &B = static_cast<OBurgers *>(h)->b;
+  const Shape &s = B.m->cellbounds;
+  for (int n = 0; n < B.V; ++n) {
+    double acc = 0.0;
+    for (int b = 0; b < (int)B.m->blocks.size(); ++b) {
+      const double vol = B.m->blocks[b].coords.vol;
+      for (int k = s.s(2); k <= s.e(2); ++k)
+        for (int j = s.s(1); j <= s.e(1); ++j)
+          for (int i = s.s(0); i <= s.e(0); ++i) acc += B.u(B.base.U.data, b, n, k, j, i) * vol;
+    }
+    out[n] = acc;
+  }
+}
+
+// Flux-correction index boxes of block gid: records {kind, d, i_lo, i_hi, j_lo, j_hi, k_lo,
+// k_hi}; kind 5 = restrict+send face box (coarse index space), 6 = receive face box.
+int orc_mesh_flxcor_boxes(void *h, int gid, int V, int *out, int max) {
+  const Mesh &m = *static_cast<OMesh *>(h)->m;
+  const Block &b = m.blocks.at(gid);
+  int n = 0;
+  if (!m.multilevel) return 0;
+  for (const auto &nb : b.neighbors) {
+    const int d = FluxCorrection::FaceDir(nb);
+    if (d < 0) continue;
+    auto put = [&](int kind, const Idx6 &x) {
+      if (n >= max) return;
+      int *o = out + 8 * n;
+      o[0] = kind;
+      o[1] = d;
+      o[2] = x.s[5];
+      o[3] = x.e[5];
+      o[4] = x.s[4];
+      o[5] = x.e[4];
+      o[6] = x.s[3];
+      o[7] = x.e[3];
+      ++n;
+    };
+    if (nb.loc.level == b.loc.level - 1)
+      put(5, CalcIndices(m, b, nb, V, IRT::BoundaryInteriorSend, false, d + 1, true));
+    if (nb.loc.level - 1 == b.loc.level)
+      put(6, CalcIndices(m, b, nb, V, IRT::BoundaryExteriorRecv, false, d + 1, true));
+  }
+  return n;
+}
+
 double orc_burgers_mass_history(void *h, const double *xmn, const double *xmx, double vol) {
   return static_cast<OBurgers *>(h)->b.MassHistory(xmn, xmx, vol);
 }

@@ -98,7 +98,8 @@ inline void lr_to_flux(double uxl, double uxr, double uyl, double uyr, double uz
 // State of one Burgers container ("base", "1") : U and its three face-flux arrays.
 struct Container {
   Field U;
-  std::vector<double> flux[3];  // same brick shape as U, face i stored at index i
+  std::vector<double> flux[3];   // same brick shape as U, face i stored at index i
+  std::vector<double> cflux[3];  // coarse buffer of the flux variable (multilevel)
 };
 
 struct Burgers {
@@ -110,6 +111,8 @@ struct Burgers {
   std::vector<double> face[6];  // Ulx, Urx, Uly, Ury, Ulz, Urz (OneCopy scratch)
   std::vector<double> derived;  // OneCopy
   Exchange ex_base, ex_one;
+  FluxCorrection flxcor;
+  bool flux_correction = true;  // the flxcor tasks of burgers_driver.cpp:84-96 (multilevel)
   std::vector<Field> fb, fo;    // field lists for the exchanges
   double dt = std::numeric_limits<double>::max(), time = 0.0;
   int ncycle = 0;
@@ -127,7 +130,10 @@ struct Burgers {
       c->U.ncomp = V;
       c->U.data.assign(nb * V * slab(), 0.0);
       if (mesh.multilevel) c->U.coarse.assign(nb * V * mesh.c_cellbounds.size(), 0.0);
-      for (int d = 0; d < 3; ++d) c->flux[d].assign(nb * V * slab(), 0.0);
+      for (int d = 0; d < 3; ++d) {
+        c->flux[d].assign(nb * V * slab(), 0.0);
+        if (mesh.multilevel) c->cflux[d].assign(nb * V * mesh.c_cellbounds.size(), 0.0);
+      }
     }
     for (int q = 0; q < 6; ++q) face[q].assign(nb * V * slab(), 0.0);
     derived.assign(nb * slab(), 0.0);
@@ -307,6 +313,7 @@ struct Burgers {
     std::swap(fo[0], one.U);
     ex_one.Build(*m, fo);
     std::swap(fo[0], one.U);
+    flxcor.Build(*m, V);
   }
 
   // Mesh::Initialize after the problem generator: exchange, FillDerived, first dt
@@ -323,8 +330,9 @@ struct Burgers {
     dt = std::min(dt, block_dt);
   }
 
-  // One RK2 cycle in the task order of burgers_driver.cpp:53-148 (single level: no flux
-  // correction tasks).
+  // One RK2 cycle in the task order of burgers_driver.cpp:53-148: fluxes, flux correction
+  // (multilevel), divergence, averaging, update, exchange (+ restriction / prolongation),
+  // derived field, time-step estimate on the last stage.
   void Cycle() {
     double new_dt = 0;
     for (int stage = 1; stage <= 2; ++stage) {
@@ -334,6 +342,7 @@ struct Burgers {
       Exchange &ex1 = stage == 1 ? ex_one : ex_base;
       std::vector<Field> &fl1 = stage == 1 ? fo : fb;
       CalculateFluxes(mc0);
+      if (m->multilevel && flux_correction) flxcor.Run(mc0.flux, mc0.cflux, V);
       FluxDivergence(mc0, dudt);
       WeightedSum(mc0.U.data, base.U.data, beta, 1.0 - beta, mc0.U.data);
       WeightedSum(mc0.U.data, dudt.U.data, 1.0, beta * dt, mc1.U.data);

@@ -65,20 +65,25 @@ struct Idx6 {  // reference SpatiallyMaskedIndexer6D restricted to what CC field
 
 enum class IRT { BoundaryInteriorSend, BoundaryExteriorRecv, InteriorSend, InteriorRecv };
 
-// reference: bnd_info.cpp:105-252 for a cell-centred, non-flux, non-fine variable.
+// reference: bnd_info.cpp:105-252 for a non-fine variable. el: 0 = cell centre (CC),
+// 1..3 = face F1..F3 (the x-, y-, z-face element); flux marks a flux variable, whose regions
+// are the shared face itself.
 inline Idx6 CalcIndices(const Mesh &m, const Block &b, const NeighborBlock &nb, int ncomp,
-                        IRT type, bool prores) {
+                        IRT type, bool prores, int el = 0, bool flux = false) {
   const int g = m.d.nghost;
   Shape shape = (prores || nb.loc.level < b.loc.level) ? m.c_cellbounds : m.cellbounds;
   const int coarse_fac = nb.loc.level > b.loc.level ? 2 : 1;
+  int top[3];  // TopologicalOffsetI/J/K (basic_types.hpp:195-203)
+  for (int k = 0; k < 3; ++k) top[k] = (el == k + 1) ? 1 : 0;
   int nbn[3];
-  for (int k = 0; k < 3; ++k) nbn[k] = m.active[k] ? m.d.bnx[k] / coarse_fac : 1;
+  for (int k = 0; k < 3; ++k) nbn[k] = m.active[k] ? m.d.bnx[k] / coarse_fac + top[k] : 1;
   int interior_offset = type == IRT::BoundaryInteriorSend ? g : 0;
   int exterior_offset = type == IRT::BoundaryExteriorRecv ? g : 0;
   if (prores) exterior_offset /= 2;
   int s[3], e[3];
   for (int dir = 0; dir < 3; ++dir) {
-    const int bs = shape.s(dir), be = shape.e(dir);
+    // interior bounds of the element (domain.hpp:183-229): faces add one index on the normal
+    const int bs = shape.s(dir), be = shape.e(dir) + (m.active[dir] ? top[dir] : 0);
     if (nb.off[dir] == 0) {
       s[dir] = bs;
       e[dir] = be;
@@ -106,11 +111,11 @@ inline Idx6 CalcIndices(const Mesh &m, const Block &b, const NeighborBlock &nb, 
         e[dir] += g / 2;
       }
     } else if (nb.off[dir] > 0) {
-      s[dir] = be - interior_offset + 1;
-      e[dir] = be + exterior_offset;
+      s[dir] = be + (flux ? 0 : -interior_offset + 1 - top[dir]);
+      e[dir] = be + (flux ? 0 : exterior_offset);
     } else {
-      s[dir] = bs - exterior_offset;
-      e[dir] = bs + interior_offset - 1;
+      s[dir] = bs + (flux ? 0 : -exterior_offset);
+      e[dir] = bs + (flux ? 0 : interior_offset - 1 + top[dir]);
     }
   }
   Idx6 out;
@@ -126,11 +131,135 @@ inline Idx6 CalcIndices(const Mesh &m, const Block &b, const NeighborBlock &nb, 
   out.e[4] = e[1];
   out.s[5] = s[0];
   out.e[5] = e[0];
-  // For a cell-centred element, GetIndexRangeMaskFromOwnership copies the neighbour's
-  // ownership of its own volume (always true) into every entry, so the mask is all-true.
+  // For a cell-centred element GetIndexRangeMaskFromOwnership copies the neighbour's ownership
+  // of its own volume (always true) into every entry; for the face element of a face-neighbour
+  // flux region the box is one index thick along the normal, so only entries copied from the
+  // same volume ownership are read. Either way the mask is all-true.
   for (int q = 0; q < 27; ++q) out.mask[q] = true;
   return out;
 }
+
+// RestrictAverage::Do (pr_ops.hpp:105-164) for a face element F(d+1) of a flux variable:
+// area-weighted average of the 2x2 fine faces sharing the coarse face, fixed summation order.
+inline void RestrictFace(const Mesh &m, const Block &b, const std::vector<double> &fine,
+                         std::vector<double> &coarse, int V, int d, int c, int ck, int cj,
+                         int ci) {
+  const Shape &fs = m.cellbounds, &cs = m.c_cellbounds;
+  const bool X[3] = {m.active[0] && d != 0, m.active[1] && d != 1, m.active[2] && d != 2};
+  const int i = m.active[0] ? (ci - cs.s(0)) * 2 + fs.s(0) : fs.s(0);
+  const int j = m.active[1] ? (cj - cs.s(1)) * 2 + fs.s(1) : fs.s(1);
+  const int k = m.active[2] ? (ck - cs.s(2)) * 2 + fs.s(2) : fs.s(2);
+  const double area = b.coords.area[d];
+  const std::int64_t fslab = (std::int64_t)b.gid * V * fs.size(), cslab = (std::int64_t)b.gid * V * cs.size();
+  auto F = [&](int kk, int jj, int ii) {
+    return fine[fslab + (((std::int64_t)c * fs.ent(2) + kk) * fs.ent(1) + jj) * fs.ent(0) + ii];
+  };
+  double vol[2][2][2] = {}, terms[2][2][2] = {};
+  for (int ok = 0; ok < 1 + X[2]; ++ok)
+    for (int oj = 0; oj < 1 + X[1]; ++oj)
+      for (int oi = 0; oi < 1 + X[0]; ++oi) {
+        vol[ok][oj][oi] = area;
+        terms[ok][oj][oi] = vol[ok][oj][oi] * F(k + ok, j + oj, i + oi);
+      }
+  const double tvol = ((vol[0][0][0] + vol[0][1][0]) + (vol[0][0][1] + vol[0][1][1])) +
+                      ((vol[1][0][0] + vol[1][1][0]) + (vol[1][0][1] + vol[1][1][1]));
+  coarse[cslab + (((std::int64_t)c * cs.ent(2) + ck) * cs.ent(1) + cj) * cs.ent(0) + ci] =
+      (((terms[0][0][0] + terms[0][1][0]) + (terms[0][0][1] + terms[0][1][1])) +
+       ((terms[1][0][0] + terms[1][1][0]) + (terms[1][0][1] + terms[1][1][1]))) /
+      tvol;
+}
+
+// Channel key: sender gid, receiver gid, variable index, location index (bvals_utils.hpp:43-64)
+using Key = std::tuple<int, int, int, int>;
+
+// Flux correction of a cell variable with fluxes (SendBoundBufs<flxcor_send>, ReceiveBoundBufs /
+// SetBounds<flxcor_recv>, boundary_communication.cpp:454-461; ForEachBoundary selection
+// loop_utils.hpp:139-162): a block with a coarser face neighbour restricts its face fluxes on
+// that face into the flux variable's coarse buffer and sends them; the coarser block overwrites
+// its own fluxes on the matching half of the face.
+struct FluxCorrection {
+  struct Send {
+    const Block *b;
+    int d;
+    Idx6 idx;
+    Key key;
+  };
+  struct Recv {
+    int gid, d;
+    Idx6 idx;
+    Key key;
+  };
+  const Mesh *m = nullptr;
+  std::vector<Send> send;
+  std::vector<Recv> recv;
+  std::map<Key, std::vector<double>> buffers;
+
+  static int FaceDir(const NeighborBlock &nb) {
+    int n = 0, d = -1;
+    for (int k = 0; k < 3; ++k)
+      if (nb.off[k] != 0) {
+        ++n;
+        d = k;
+      }
+    return n == 1 ? d : -1;
+  }
+
+  void Build(const Mesh &mesh, int V) {
+    m = &mesh;
+    send.clear();
+    recv.clear();
+    buffers.clear();
+    if (!mesh.multilevel) return;
+    for (const auto &b : mesh.blocks)
+      for (const auto &nb : b.neighbors) {
+        const int d = FaceDir(nb);
+        if (d < 0) continue;
+        if (nb.loc.level == b.loc.level - 1) {  // flxcor_send
+          Send s;
+          s.b = &b;
+          s.d = d;
+          s.idx = CalcIndices(mesh, b, nb, V, IRT::BoundaryInteriorSend, false, d + 1, true);
+          s.key = Key(b.gid, nb.gid, 0, nb.GetIdx());
+          buffers[s.key].assign(s.idx.size(), 0.0);
+          send.push_back(s);
+        }
+        if (nb.loc.level - 1 == b.loc.level) {  // flxcor_recv
+          Recv r;
+          r.gid = b.gid;
+          r.d = d;
+          r.idx = CalcIndices(mesh, b, nb, V, IRT::BoundaryExteriorRecv, false, d + 1, true);
+          r.key = Key(nb.gid, b.gid, 0, nb.GetReverseIdx());
+          recv.push_back(r);
+        }
+      }
+  }
+
+  // flux[d]: fine fluxes [block][V][brick]; cflux[d]: coarse buffers [block][V][coarse brick]
+  void Run(std::vector<double> *flux, std::vector<double> *cflux, int V) {
+    const Shape &fs = m->cellbounds, &cs = m->c_cellbounds;
+    for (const auto &s : send) {  // restrict (ProResInfo::GetSend) then pack from coarse_s
+      std::vector<double> &buf = buffers.at(s.key);
+      for (std::int64_t n = 0; n < s.idx.size(); ++n) {
+        int t[6];
+        s.idx.Decode(n, t);
+        RestrictFace(*m, *s.b, flux[s.d], cflux[s.d], V, s.d, t[2], t[3], t[4], t[5]);
+        buf[n] = cflux[s.d][(std::int64_t)s.b->gid * V * cs.size() +
+                            (((std::int64_t)t[2] * cs.ent(2) + t[3]) * cs.ent(1) + t[4]) *
+                                cs.ent(0) + t[5]];
+      }
+    }
+    for (const auto &r : recv) {  // SetBounds<flxcor_recv>: overwrite the coarse block's fluxes
+      const std::vector<double> &buf = buffers.at(r.key);
+      for (std::int64_t n = 0; n < r.idx.size(); ++n) {
+        int t[6];
+        r.idx.Decode(n, t);
+        flux[r.d][(std::int64_t)r.gid * V * fs.size() +
+                  (((std::int64_t)t[2] * fs.ent(2) + t[3]) * fs.ent(1) + t[4]) * fs.ent(0) +
+                  t[5]] = buf[n];
+      }
+    }
+  }
+};
 
 // reference: bnd_info.cpp:254-271
 inline std::int64_t GetBufferSize(const Mesh &m, const NeighborBlock &nb, int ncomp) {
@@ -140,9 +269,6 @@ inline std::int64_t GetBufferSize(const Mesh &m, const NeighborBlock &nb, int nc
   for (int k = 0; k < 3; ++k) sz *= (nb.off[k] == 0 ? cb.ent(k) + 1 : g + 1);
   return sz * ncomp;
 }
-
-// Channel key: sender gid, receiver gid, variable index, location index (bvals_utils.hpp:43-64)
-using Key = std::tuple<int, int, int, int>;
 
 struct BndInfo {
   Idx6 idx;

@@ -1,0 +1,122 @@
+// A C++ driver written against the reference's task-level names, run through
+// include/parthenon_amd.hpp: the task order of BurgersDriver::MakeTaskCollection
+// (benchmarks/burgers/burgers_driver.cpp:53-148) with a sequential task list, the RK2
+// integrator (low_storage_integrator.cpp:67-83) and the driver's dt rule (driver.cpp:210-235).
+//
+// usage: burgers_task_driver <ncycles> <out.bin>
+// writes the final "base" U (float64, [block][comp][k][j][i]) followed by dt and time.
+#include <cstdio>
+#include <cstdlib>
+#include <functional>
+#include <limits>
+#include <vector>
+
+#include "parthenon_amd.hpp"
+
+using namespace parthenon;
+
+namespace {
+
+// Minimal sequential stand-in for the reference's TaskList: each task is re-polled while it
+// returns incomplete, like Task::operator() re-enqueueing itself (src/tasks/tasks.hpp:93-118).
+struct TaskList {
+  std::vector<std::function<TaskStatus()>> tasks;
+  void AddTask(std::function<TaskStatus()> f) { tasks.push_back(std::move(f)); }
+  void Execute() {
+    for (auto &t : tasks) {
+      int polls = 0;
+      while (t() == TaskStatus::incomplete)
+        if (++polls > 1000) throw std::runtime_error("task never completed");
+    }
+  }
+};
+
+struct Integrator {
+  int nstages = 2;
+  Real beta[2] = {1.0, 0.5};
+  const char *stage_name[3] = {"base", "1", "base"};
+  Real dt = std::numeric_limits<Real>::max();
+};
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  if (argc < 3) {
+    std::fprintf(stderr, "usage: %s ncycles out.bin\n", argv[0]);
+    return 2;
+  }
+  const int ncycles = std::atoi(argv[1]);
+  const Real cfl = 0.4;
+  try {
+    // burgers_package::Initialize: U (3 + 4 scalars) and the OneCopy derived field
+    Mesh mesh({64, 64, 64}, {16, 16, 16}, 4);
+    mesh.AddField("U", 7, Metadata::FillGhost | Metadata::Independent | Metadata::WithFluxes);
+    mesh.AddField("derived", 1, Metadata::OneCopy);
+    mesh.Initialize();
+    auto &base = mesh.GetOrAdd("base", 0);
+    if (pth_burgers_pgen_sinusoid(base->handle(), 0) != PTH_COMPLETE)
+      throw std::runtime_error(pth_last_error());
+    // Mesh::Initialize after the problem generator: CommunicateBoundaries, FillDerived
+    SendBoundaryBuffers(base);
+    ReceiveBoundaryBuffers(base);
+    SetBoundaries(base);
+    burgers_package::CalculateDerived(base.get());
+    Integrator integ;
+    Real time = 0.0;
+    integ.dt = std::min(integ.dt, burgers_package::EstimateTimestepMesh(base.get(), cfl));
+
+    for (int cycle = 0; cycle < ncycles; ++cycle) {
+      Real new_dt = std::numeric_limits<Real>::max();
+      for (int stage = 1; stage <= integ.nstages; ++stage) {
+        const Real beta = integ.beta[stage - 1];
+        const Real dt = integ.dt;
+        auto &mbase = mesh.GetOrAdd("base", 0);
+        auto &mc0 = mesh.GetOrAdd(integ.stage_name[stage - 1], 0);
+        auto &mc1 = mesh.GetOrAdd(integ.stage_name[stage], 0);
+        auto &mdudt = mesh.GetOrAdd("dUdt", 0);
+        constexpr auto any = BoundaryType::any;
+        constexpr auto local = BoundaryType::local;
+        constexpr auto nonlocal = BoundaryType::nonlocal;
+        TaskList tl;
+        tl.AddTask([&] { return StartReceiveBoundBufs<any>(mc1); });
+        tl.AddTask([&] { return burgers_package::CalculateFluxes(mc0.get()); });
+        tl.AddTask([&] { return Update::FluxDivergence(mc0.get(), mdudt.get()); });
+        tl.AddTask([&] { return Update::AverageIndependentData(mc0.get(), mbase.get(), beta); });
+        tl.AddTask([&] {
+          return Update::UpdateIndependentData(mc0.get(), mdudt.get(), beta * dt, mc1.get());
+        });
+        tl.AddTask([&] { return SendBoundBufs<nonlocal>(mc1); });
+        tl.AddTask([&] { return SendBoundBufs<local>(mc1); });
+        tl.AddTask([&] { return ReceiveBoundBufs<local>(mc1); });
+        tl.AddTask([&] { return SetBounds<local>(mc1); });
+        tl.AddTask([&] { return ReceiveBoundBufs<nonlocal>(mc1); });
+        tl.AddTask([&] { return SetBounds<nonlocal>(mc1); });
+        tl.AddTask([&] {
+          burgers_package::CalculateDerived(mc1.get());
+          return TaskStatus::complete;
+        });
+        if (stage == integ.nstages)
+          tl.AddTask([&] {
+            new_dt = burgers_package::EstimateTimestepMesh(mc1.get(), cfl);
+            return TaskStatus::complete;
+          });
+        tl.Execute();
+      }
+      time += integ.dt;
+      // SetGlobalTimeStep: growth limited to 2x
+      integ.dt = std::min(2.0 * integ.dt, new_dt);
+    }
+    std::vector<Real> U(static_cast<size_t>(mesh.nbtotal()) * 7 * 24 * 24 * 24);
+    base->Download("U", U);
+    FILE *f = std::fopen(argv[2], "wb");
+    if (!f) throw std::runtime_error("cannot open output");
+    std::fwrite(U.data(), sizeof(Real), U.size(), f);
+    std::fwrite(&integ.dt, sizeof(Real), 1, f);
+    std::fwrite(&time, sizeof(Real), 1, f);
+    std::fclose(f);
+  } catch (const std::exception &e) {
+    std::fprintf(stderr, "burgers_task_driver: %s\n", e.what());
+    return 1;
+  }
+  return 0;
+}

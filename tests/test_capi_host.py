@@ -77,6 +77,16 @@ def test_exchange_counts_match_oracle():
     assert nbuf == 512 * 26
 
 
+def test_cpp_interface_header_compiles_and_driver_links():
+    import subprocess
+    src = os.path.join(ROOT, "tests", "cpp", "burgers_task_driver.cpp")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror",
+                        "-I" + os.path.join(ROOT, "include"), src],
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    assert r.returncode == 0, r.stderr
+    assert os.path.exists(os.path.join(ROOT, "tests", "cpp", "burgers_task_driver"))
+
+
 def test_refinement_rejects_odd_nghost():
     p = pa.Mesh((32, 32, 32), (8, 8, 8), 3, refine_boxes=[((-0.1, 0.1) * 3, 1)])
     with pytest.raises(RuntimeError, match="even nghost"):

@@ -105,6 +105,27 @@ def test_device_driver_matches_oracle_every_cycle(use_graph):
     assert_close(d.base.download("U"), ob.get(0), "U after 10 cycles")
 
 
+def test_cpp_task_driver_matches_oracle(tmp_path):
+    # the reference's task order written in C++ against include/parthenon_amd.hpp
+    import os
+    import subprocess
+    from oracle_lib import lib
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "burgers_task_driver")
+    out = str(tmp_path / "u.bin")
+    print("case cpp task driver 5 cycles", flush=True)
+    r = subprocess.run([exe, "5", out], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = np.fromfile(out, dtype=np.float64)
+    U, dt, t = raw[:-2], raw[-2], raw[-1]
+    ob = OracleBurgers(OracleMesh((64, 64, 64), (16, 16, 16), 4), 4, 0.4, 0)
+    lib().orc_burgers_pgen_sinusoid(ob.h, 0)
+    lib().orc_burgers_initialize(ob.h)
+    lib().orc_burgers_cycles(ob.h, 5)
+    assert_close(U, ob.get(0), "C++ driver U after 5 cycles")
+    assert dt == ob.dt and t == lib().orc_burgers_time(ob.h)
+
+
 def test_device_driver_rejects_small_nghost():
     from parthenon_amd.burgers import DeviceBurgersDriver
     m = make_burgers_mesh((32, 32, 32), (16, 16, 16), 2, 1)

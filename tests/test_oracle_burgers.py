@@ -149,3 +149,73 @@ def test_vibe_problem_stays_finite():
     lib().orc_burgers_initialize(b.h)
     lib().orc_burgers_cycles(b.h, 3)
     assert np.all(np.isfinite(b.get(0)))
+
+
+def refined_burgers(flux_correction, cycles):
+    """Two-level mesh: 32^3 cells in 8^3 blocks with the central 2^3 base blocks refined once."""
+    m = OracleMesh((32, 32, 32), (8, 8, 8), 4, boxes=[((-0.1, 0.1, -0.1, 0.1, -0.1, 0.1), 1)])
+    b = OracleBurgers(m, 1, 0.4, 0)
+    lib().orc_burgers_set_flux_correction(b.h, int(flux_correction))
+    lib().orc_burgers_pgen_sinusoid(b.h, 21)
+    lib().orc_burgers_initialize(b.h)
+    t0 = b.totals()
+    lib().orc_burgers_cycles(b.h, cycles)
+    return m, b, t0, b.totals()
+
+
+def test_flux_correction_makes_two_level_run_conservative():
+    # Sum of U * cell volume over the leaf blocks changes only through unmatched fluxes across
+    # coarse/fine faces: with flux correction it is conserved to round-off, without it is not.
+    m, b, t0, t1 = refined_burgers(True, 3)
+    assert m.multilevel and m.nblocks == 120
+    assert np.max(np.abs(t1 - t0)) < 1e-14
+    _, _, u0, u1 = refined_burgers(False, 3)
+    assert np.array_equal(t0, u0)
+    assert np.max(np.abs(u1 - u0)) > 1e-8
+
+
+def coarse_fine_face_errors(m, b):
+    """For every coarse face cell on a coarse/fine face, |coarse flux - mean of the 4 fine fluxes
+    on the same face area|, matched by block geometry (not by the oracle's index boxes)."""
+    g, n = m.ng[0], m.n[0]
+    E = m.ent
+    F = [b.get(4 + d).reshape(m.nblocks, b.V, E[2], E[1], E[0]) for d in range(3)]
+    blocks = [m.block(i) for i in range(m.nblocks)]
+    errs = []
+    for gid, bk in enumerate(blocks):
+        for nb in m.neighbors(gid):
+            off = nb["off"]
+            if nb["level"] != bk["level"] + 1 or sum(o != 0 for o in off) != 1:
+                continue
+            d = [t for t in range(3) if off[t]][0]
+            fb = blocks[nb["gid"]]
+            tr = [t for t in range(3) if t != d]
+            dxc = [(bk["xmax"][t] - bk["xmin"][t]) / n for t in range(3)]
+            dxf = [(fb["xmax"][t] - fb["xmin"][t]) / n for t in range(3)]
+            for a in range(n):
+                for c in range(n):
+                    cen = {tr[0]: bk["xmin"][tr[0]] + (a + 0.5) * dxc[tr[0]],
+                           tr[1]: bk["xmin"][tr[1]] + (c + 0.5) * dxc[tr[1]]}
+                    if not all(fb["xmin"][t] < cen[t] < fb["xmax"][t] for t in tr):
+                        continue
+                    ci = {d: g if off[d] < 0 else g + n, tr[0]: g + a, tr[1]: g + c}
+                    lo = {t: g + int(round((cen[t] - fb["xmin"][t]) / dxf[t])) - 1 for t in tr}
+                    fine = []
+                    for p in range(2):
+                        for q in range(2):
+                            fi = {d: g + n if off[d] < 0 else g, tr[0]: lo[tr[0]] + p,
+                                  tr[1]: lo[tr[1]] + q}
+                            fine.append(F[d][nb["gid"], :, fi[2], fi[1], fi[0]])
+                    coarse = F[d][gid, :, ci[2], ci[1], ci[0]]
+                    errs.append(np.max(np.abs(coarse - np.mean(fine, axis=0))))
+    return np.array(errs)
+
+
+def test_flux_correction_overwrites_each_coarse_face_with_the_fine_face_average():
+    m, b, _, _ = refined_burgers(True, 1)
+    e = coarse_fine_face_errors(m, b)
+    # 8 refined base blocks expose 24 coarse/fine block faces of 8x8 coarse face cells
+    assert len(e) == 24 * 64
+    assert e.max() < 1e-14
+    m, b, _, _ = refined_burgers(False, 1)
+    assert coarse_fine_face_errors(m, b).max() > 1e-6
