@@ -1,13 +1,18 @@
 #!/usr/bin/env python3
 """Benchmark: Burgers (Parthenon-VIBE) RK2 cycle throughput on one MI355X.
 
-Workload: 64^3 mesh in 64 meshblocks of 16^3, nghost 4, 3 velocities + 4 passive scalars,
-FP64, WENO5 + HLL, RK2, CFL 0.4, periodic, seeded sum-of-sinusoids initial state
-(SplitMix64 seed 0). One step = one full RK2 cycle: per stage the reconstruction, the fused
-HLL/flux-divergence/RK-update/derived launch and the boundary exchange (persistent pack and
-unpack over the buffer table); the CFL estimate and SetGlobalTimeStep on device at the end.
+Default workload (--workload burgers-amr), the static-refined Burgers step: a 64^3 base mesh on
+[-0.5, 0.5]^3 in 16^3 meshblocks, refined two levels on every block that meets the sphere of
+radius 0.25 at the centre (960 blocks with 2:1 nesting), nghost 4, 3 velocities + 4 passive
+scalars, FP64, WENO5 + HLL, RK2, CFL 0.4, periodic, seeded sum-of-sinusoids initial state
+(SplitMix64 seed 2). --workload burgers-uniform is the single-level 64^3 mesh in 64 blocks
+(seed 0). One step = one full RK2 cycle: per stage the reconstruction, the flux-correction
+launch (multilevel), the fused HLL/flux-divergence/RK-update/derived launch and the boundary
+exchange (restriction, direct same-GPU copy over the buffer table, restriction of received
+regions, minmod prolongation); the CFL estimate and SetGlobalTimeStep on device at the end.
 Inputs are resident in HBM before timing starts. Throughput is zone-cycles/s
-= nblocks * 16^3 * steps / wall time (the reference's definition, src/driver/driver.cpp:56-63).
+= (sum of block cells) * steps / wall time (the reference's definition,
+src/driver/driver.cpp:56-63).
 
 Timed region 1 (the reported value): `steps` cycles, each one replay of the captured hipGraph
 (--driver graph, default), bracketed by stream synchronizations. Timed region 2 (roofline):
@@ -49,29 +54,31 @@ def kernel_bytes(nb, n, g, V, exch_elems):
         "dt_reduce": 8 * nb * 3 * I,
         "pack": 16 * exch_elems,
         "unpack": 16 * exch_elems,
+        "exchange_direct": 16 * exch_elems,
     }
 
 
-def step_bytes(nb, n, g, V):
-    """Algorithmic bytes of one RK2 cycle in the reference task decomposition."""
+def step_bytes(nb, n, g, V, exchange_bytes):
+    """Algorithmic bytes of one RK2 cycle in the reference task decomposition: per block the
+    reconstruction/flux, divergence, averaging, update, derived and dt tasks, plus the
+    exchange's buffer traffic."""
     E = (n + 2 * g) ** 3
     I = n ** 3
     F = (n + 1) * n * n
     RU = I + 18 * n * n
-    H = E - I
-    return 8 * nb * (V * (2 * RU + 12 * F + 2 * I + 11 * E + 8 * H) + 13 * I)
+    return 8 * nb * (V * (2 * RU + 12 * F + 2 * I + 11 * E) + 13 * I) + exchange_bytes
 
 
-def cpu_baseline(nx, block, g, ns, cycles, threads):
+def cpu_baseline(nx, block, g, ns, spheres, seed, cycles, threads):
     """Oracle (C++ restatement, OpenMP over meshblocks) on `cycles` cycles of the workload,
     run in a child process so its OpenMP thread count is set before the library loads."""
     code = f"""
 import sys, time, json
 sys.path.insert(0, {os.path.join(ROOT, 'tests')!r})
 from oracle_lib import OracleMesh, OracleBurgers, lib
-m = OracleMesh({nx}, {block}, {g})
+m = OracleMesh({nx}, {block}, {g}, spheres={list(spheres)!r})
 b = OracleBurgers(m, {ns}, 0.4, 0)
-lib().orc_burgers_pgen_sinusoid(b.h, 0)
+lib().orc_burgers_pgen_sinusoid(b.h, {seed})
 lib().orc_burgers_initialize(b.h)
 lib().orc_burgers_cycles(b.h, 1)
 t = time.perf_counter()
@@ -85,6 +92,29 @@ print(json.dumps({{"wall": dt}}))
     if r.returncode != 0:
         raise RuntimeError("cpu baseline failed: " + r.stderr[-2000:])
     return json.loads(r.stdout.strip().splitlines()[-1])["wall"]
+
+
+def copy_bench(args):
+    """Plain device copy of 1 GiB (one read and one write per byte): the reference kernel for
+    scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counts to bytes."""
+    import torch
+    n = 1 << 27
+    x = torch.rand(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(args.warmup):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    nbytes = 2 * 8 * n
+    print(json.dumps({"metric": "device copy bandwidth (counter calibration)",
+                      "value": nbytes * args.steps / wall / 1e9, "unit": "GB/s",
+                      "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                      "bytes_read_per_copy": 8 * n, "bytes_written_per_copy": 8 * n,
+                      "higher_is_better": True, "dtype": "f64", "data": "synthetic"}))
 
 
 def exchange_bench(args):
@@ -141,13 +171,15 @@ def exchange_bench(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["burgers", "exchange"], default="burgers")
+    ap.add_argument("--workload",
+                    choices=["burgers-amr", "burgers-uniform", "exchange", "copy"],
+                    default="burgers-amr")
     ap.add_argument("--pack", choices=["persistent", "per_buffer", "fused"], default="persistent")
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--driver", choices=["graph", "eager", "tasks"], default="graph")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-cycles", type=int, default=60)
+    ap.add_argument("--cpu-cycles", type=int, default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -155,21 +187,27 @@ def main():
         sys.exit(2)
     if args.workload == "exchange":
         return exchange_bench(args)
+    if args.workload == "copy":
+        return copy_bench(args)
 
     import parthenon_amd as pa
     from parthenon_amd.burgers import BurgersDriver, DeviceBurgersDriver, make_burgers_mesh
 
     nx, block, g, ns = (64, 64, 64), (16, 16, 16), 4, 4
+    amr = args.workload == "burgers-amr"
+    spheres = [((0.0, 0.0, 0.0), 0.25, 2)] if amr else []
+    seed = 2 if amr else 0
+    cpu_cycles = args.cpu_cycles or (10 if amr else 60)
     V = 3 + ns
-    mesh = make_burgers_mesh(nx, block, g, ns)
+    mesh = make_burgers_mesh(nx, block, g, ns, refine_spheres=spheres)
     if args.driver == "tasks":
         drv = BurgersDriver(mesh, cfl=0.4)
-        drv.initialize("sinusoid", seed=0)
+        drv.initialize("sinusoid", seed=seed)
         run = lambda k: [drv.cycle() for _ in range(k)]
         sync = mesh.synchronize
     else:
         drv = DeviceBurgersDriver(mesh, cfl=0.4, use_graph=(args.driver == "graph"))
-        drv.initialize("sinusoid", seed=0)
+        drv.initialize("sinusoid", seed=seed)
         run = drv.cycles
         sync = drv.synchronize
 
@@ -181,7 +219,7 @@ def main():
     wall = time.perf_counter() - t0
 
     # roofline pass: the same cycles launched eagerly, HIP events around every kernel
-    prof_steps = min(args.steps, 50)
+    prof_steps = min(args.steps, 50 if not amr else 10)
     if args.driver == "graph":
         drv.use_graph = False
     pa.profile_enable(True)
@@ -204,7 +242,17 @@ def main():
     kernels = {k: {"launches_per_step": c / prof_steps, "avg_us": ms / c * 1e3,
                    "GBps": (kb[k] / (ms / c * 1e-3) / 1e9) if k in kb else None}
                for k, (c, ms) in sorted(prof.items())}
-    sb = step_bytes(nb, n, g, V)
+    # interior part of the reference task decomposition plus the exchange's buffer traffic
+    # counted from the live table (send R+W and set R+W per element, two stages); restriction,
+    # prolongation and flux-correction traffic is not counted (a lower bound)
+    sb = step_bytes(nb, n, g, V, 2 * 4 * 8 * nel)
+    if amr:
+        workload = ("burgers-amr 64^3 base mesh in 16^3 blocks, two static levels on the blocks "
+                    "meeting the centred sphere r=0.25 (960 blocks), nghost 4, 3 velocities + "
+                    "4 scalars, WENO5/HLL/RK2 with flux correction, CFL 0.4, periodic")
+    else:
+        workload = ("burgers-uniform 64^3 mesh, 64 blocks of 16^3, nghost 4, 3 velocities + "
+                    "4 scalars, WENO5/HLL/RK2, CFL 0.4, periodic")
     out = {
         "metric": "zone-cycles/s (Burgers RK2 cycle, FP64)",
         "value": value,
@@ -217,11 +265,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded sum of sinusoids, SplitMix64 seed 0)",
-        "config": {"workload": "burgers-vibe 64^3 mesh, 64 blocks of 16^3, nghost 4, "
-                               "3 velocities + 4 scalars, WENO5/HLL/RK2, CFL 0.4, periodic",
-                   "driver": args.driver, "nblocks": nb, "block": list(block), "nghost": g,
-                   "ncomp": V},
+        "data": f"synthetic (seeded sum of sinusoids, SplitMix64 seed {seed})",
+        "config": {"workload": workload, "driver": args.driver, "nblocks": nb,
+                   "block": list(block), "nghost": g, "ncomp": V},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": None, "algorithmic_bytes_per_launch": kb.get(dom),
@@ -233,11 +279,11 @@ def main():
     }
     if not args.no_cpu_baseline:
         threads = min(len(os.sched_getaffinity(0)), 16)
-        cw = cpu_baseline(nx, block, g, ns, args.cpu_cycles, threads)
+        cw = cpu_baseline(nx, block, g, ns, spheres, seed, cpu_cycles, threads)
         out["cpu_baseline"] = {
-            "value": nb * n ** 3 * args.cpu_cycles / cw, "unit": "zone-cycles/s",
+            "value": nb * n ** 3 * cpu_cycles / cw, "unit": "zone-cycles/s",
             "cores": threads, "kind": "port",
-            "sample": f"{args.cpu_cycles} RK2 cycles of the same workload after 1 warm-up cycle, "
+            "sample": f"{cpu_cycles} RK2 cycles of the same workload after 1 warm-up cycle, "
                       f"oracle C++ restatement with OpenMP over meshblocks ({cw:.2f} s)"}
     print(json.dumps(out))
 

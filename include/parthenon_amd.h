@@ -93,7 +93,9 @@ int pth_mesh_neighbors(const pth_mesh *m, int gid, int64_t *out, int max);
 /* Index boxes of every boundary region of block gid, one record of 7 ints per neighbour link
  * and kind {kind, i_lo, i_hi, j_lo, j_hi, k_lo, k_hi}. kind: 0 send, 1 receive, 2 restriction
  * before send, 3 prolongation after set, 4 restriction after set (kinds 2-4 in coarse index
- * space). Returns the number of records written (kinds absent for a link are skipped). */
+ * space), 5 flux-correction restriction before send (face fluxes, coarse index space), 6
+ * flux-correction receive (loop_utils.hpp:146-160, bnd_info.cpp:387-403). Returns the number
+ * of records written (kinds absent for a link are skipped). */
 int pth_mesh_region_boxes(const pth_mesh *m, int gid, int *out, int max);
 /* Boundary table sizes for bound type bt: number of buffers and of buffer elements used. */
 pth_status pth_mesh_exchange_counts(const pth_mesh *m, int bt, int64_t *nbuffers,

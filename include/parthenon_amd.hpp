@@ -115,6 +115,11 @@ class Mesh {
   void AddStaticRefinement(std::array<Real, 6> box, int level) {
     detail::Check(pth_mesh_refine_box(h_, box.data(), level), "AddStaticRefinement");
   }
+  // static refinement of every block that intersects a sphere, as a union of per-block boxes
+  void AddRefinementSphere(std::array<Real, 3> center, Real radius, int level) {
+    detail::Check(pth_mesh_refine_sphere(h_, center.data(), radius, level),
+                  "AddRefinementSphere");
+  }
   // StateDescriptor::AddField for a cell-centred field
   void AddField(const std::string &name, int ncomp, unsigned flags) {
     detail::Check(pth_mesh_add_field(h_, name.c_str(), ncomp, flags), "AddField");
@@ -176,6 +181,22 @@ inline TaskStatus ReceiveBoundaryBuffers(std::shared_ptr<MeshData<Real>> &md) {
 }
 inline TaskStatus SetBoundaries(std::shared_ptr<MeshData<Real>> &md) {
   return SetBounds<BoundaryType::any>(md);
+}
+inline TaskStatus ProlongateBoundaries(std::shared_ptr<MeshData<Real>> &md) {
+  return ProlongateBounds<BoundaryType::any>(md);
+}
+// flux-correction tasks of bvals_in_one.hpp:69-80
+inline TaskStatus StartReceiveFluxCorrections(std::shared_ptr<MeshData<Real>> &md) {
+  return StartReceiveBoundBufs<BoundaryType::flxcor_recv>(md);
+}
+inline TaskStatus LoadAndSendFluxCorrections(std::shared_ptr<MeshData<Real>> &md) {
+  return SendBoundBufs<BoundaryType::flxcor_send>(md);
+}
+inline TaskStatus ReceiveFluxCorrections(std::shared_ptr<MeshData<Real>> &md) {
+  return ReceiveBoundBufs<BoundaryType::flxcor_recv>(md);
+}
+inline TaskStatus SetFluxCorrections(std::shared_ptr<MeshData<Real>> &md) {
+  return SetBounds<BoundaryType::flxcor_recv>(md);
 }
 
 // ---- Update:: tasks (src/interface/update.hpp) ----

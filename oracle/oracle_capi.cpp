@@ -133,6 +133,14 @@ int orc_mesh_region_boxes(void *h, int gid, int *out, int max) {
     } else if (b.loc.level > 0 && b.has_coarser_neighbor) {
       put(4, CalcIndices(m, b, nb, 1, IRT::BoundaryExteriorRecv, true));
     }
+    // flux correction of a face flux variable (element F(d+1)): 5 restrict+send box in coarse
+    // index space, 6 receive box
+    const int d = FluxCorrection::FaceDir(nb);
+    if (d < 0) continue;
+    if (nb.loc.level == b.loc.level - 1)
+      put(5, CalcIndices(m, b, nb, 1, IRT::BoundaryInteriorSend, false, d + 1, true));
+    if (nb.loc.level - 1 == b.loc.level)
+      put(6, CalcIndices(m, b, nb, 1, IRT::BoundaryExteriorRecv, false, d + 1, true));
   }
   return n;
 }
@@ -374,37 +382,6 @@ void orc_burgers_totals(void *h, double *out) {
     }
     out[n] = acc;
   }
-}
-
-// Flux-correction index boxes of block gid: records {kind, d, i_lo, i_hi, j_lo, j_hi, k_lo,
-// k_hi}; kind 5 = restrict+send face box (coarse index space), 6 = receive face box.
-int orc_mesh_flxcor_boxes(void *h, int gid, int V, int *out, int max) {
-  const Mesh &m = *static_cast<OMesh *>(h)->m;
-  const Block &b = m.blocks.at(gid);
-  int n = 0;
-  if (!m.multilevel) return 0;
-  for (const auto &nb : b.neighbors) {
-    const int d = FluxCorrection::FaceDir(nb);
-    if (d < 0) continue;
-    auto put = [&](int kind, const Idx6 &x) {
-      if (n >= max) return;
-      int *o = out + 8 * n;
-      o[0] = kind;
-      o[1] = d;
-      o[2] = x.s[5];
-      o[3] = x.e[5];
-      o[4] = x.s[4];
-      o[5] = x.e[4];
-      o[6] = x.s[3];
-      o[7] = x.e[3];
-      ++n;
-    };
-    if (nb.loc.level == b.loc.level - 1)
-      put(5, CalcIndices(m, b, nb, V, IRT::BoundaryInteriorSend, false, d + 1, true));
-    if (nb.loc.level - 1 == b.loc.level)
-      put(6, CalcIndices(m, b, nb, V, IRT::BoundaryExteriorRecv, false, d + 1, true));
-  }
-  return n;
 }
 
 double orc_burgers_mass_history(void *h, const double *xmn, const double *xmx, double vol) {

@@ -297,6 +297,23 @@ def ProlongateBounds(md, bt=BoundaryType.any):
     return _check(lib().pth_prolongate_bounds(md._h, int(bt)), "ProlongateBounds")
 
 
+# flux-correction tasks (src/bvals/comms/bvals_in_one.hpp:69-80)
+def StartReceiveFluxCorrections(md):
+    return StartReceiveBoundBufs(md, BoundaryType.flxcor_recv)
+
+
+def LoadAndSendFluxCorrections(md):
+    return SendBoundBufs(md, BoundaryType.flxcor_send)
+
+
+def ReceiveFluxCorrections(md):
+    return ReceiveBoundBufs(md, BoundaryType.flxcor_recv)
+
+
+def SetFluxCorrections(md):
+    return SetBounds(md, BoundaryType.flxcor_recv)
+
+
 class burgers_package:  # benchmarks/burgers/burgers_package.hpp
     WENO5 = 0
     Linear = 1

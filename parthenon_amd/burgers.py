@@ -1,6 +1,9 @@
 """Burgers benchmark driver over the parthenon_amd task API.
 
-Mirrors BurgersDriver::MakeTaskCollection (benchmarks/burgers/burgers_driver.cpp:53-148) with
+Mirrors BurgersDriver::MakeTaskCollection (benchmarks/burgers/burgers_driver.cpp:53-148),
+including its flux-correction tasks (lines 89-96), with prolongation after the sets in the
+library exchange order (AddBoundaryExchangeTasks, src/bvals/comms/boundary_communication.cpp:
+406-445; the shipped benchmark never prolongates inside a stage) and
 the low-storage RK2 integrator (src/time_integration/low_storage_integrator.cpp:67-83, stage
 names "base", "1", "base") and EvolutionDriver's time-step control
 (src/driver/driver.cpp:194-235): the first dt is the mesh estimate, later ones are
@@ -8,8 +11,10 @@ min(2 * dt_prev, estimate).
 """
 import math
 
-from . import (BoundaryType, Mesh, Metadata, SendBoundBufs, ReceiveBoundBufs, SetBounds,
-               ProlongateBounds, StartReceiveBoundBufs, TaskStatus, Update, burgers_package)
+from . import (BoundaryType, LoadAndSendFluxCorrections, Mesh, Metadata, ProlongateBounds,
+               ReceiveBoundBufs, ReceiveFluxCorrections, SendBoundBufs, SetBounds,
+               SetFluxCorrections, StartReceiveBoundBufs, StartReceiveFluxCorrections,
+               TaskStatus, Update, burgers_package)
 
 RK2_BETA = (1.0, 0.5)
 
@@ -78,7 +83,6 @@ class BurgersDriver:
 
     def _exchange(self, md):
         any_ = BoundaryType.any
-        StartReceiveBoundBufs(md, any_)
         SendBoundBufs(md, BoundaryType.nonlocal_)
         SendBoundBufs(md, BoundaryType.local)
         if ReceiveBoundBufs(md, BoundaryType.local) != TaskStatus.complete:
@@ -96,6 +100,7 @@ class BurgersDriver:
             burgers_package.ProblemGeneratorSinusoid(self.base, seed)
         elif pgen == "vibe":
             burgers_package.ProblemGenerator(self.base)
+        StartReceiveBoundBufs(self.base, BoundaryType.any)
         self._exchange(self.base)
         burgers_package.CalculateDerived(self.base)
         self.dt = burgers_package.EstimateTimestepMesh(self.base, self.cfl)
@@ -104,7 +109,13 @@ class BurgersDriver:
         beta = RK2_BETA[stage - 1]
         mc0 = self.base if stage == 1 else self.one
         mc1 = self.one if stage == 1 else self.base
+        StartReceiveBoundBufs(mc1, BoundaryType.any)
+        StartReceiveFluxCorrections(mc0)
         burgers_package.CalculateFluxes(mc0, self.recon)
+        LoadAndSendFluxCorrections(mc0)
+        if ReceiveFluxCorrections(mc0) != TaskStatus.complete:
+            raise RuntimeError("flux corrections were not received")
+        SetFluxCorrections(mc0)
         Update.FluxDivergence(mc0, self.dudt)
         Update.AverageIndependentData(mc0, self.base, beta)
         Update.UpdateIndependentData(mc0, self.dudt, beta * self.dt, mc1)

@@ -385,13 +385,16 @@ constexpr int kMaxV = 16;  // components handled by the fused update kernel
 // every direction, flux divergence (update.hpp:43-57), the RK averaging and update in the
 // WeightedSumData form (update.hpp:71-90, 122-137), the derived field and the per-workgroup
 // CFL minimum of the updated state; optionally writes the low-face fluxes (and the last high
-// face) for flux correction. Reads face states written by ReconTaskKernel.
+// face) for flux correction. Reads face states written by ReconTaskKernel. On multilevel
+// meshes `fover` gives per block face the correction-store image whose values replace the HLL
+// fluxes of that face (SetBounds<flxcor_recv> before FluxDivergence in the reference order).
 template <int V>
 __global__ void __launch_bounds__(kWG)
     FusedUpdateKernel(const double *__restrict__ face, const double *U0, const double *base,
                       double *out, double *__restrict__ derived, double *__restrict__ flux,
                       double *__restrict__ partial, const BlockCoefs *__restrict__ coef, Geo G,
-                      int nblocks, double beta, const double *__restrict__ dtvar) {
+                      int nblocks, double beta, const double *__restrict__ dtvar,
+                      const int *__restrict__ fover, const double *__restrict__ fcor) {
   __shared__ double wmin[kWG / 64];
   const int n0 = G.hi0 - G.lo0 + 1, n1 = G.hi1 - G.lo1 + 1, n2 = G.hi2 - G.lo2 + 1;
   const int b = blockIdx.y;
@@ -413,6 +416,24 @@ __global__ void __launch_bounds__(kWG)
       const double *L = face + (2 * d) * slab + c0, *R = face + (2 * d + 1) * slab + c0;
       HllFace<V>(L, R, cs, d, Flo);
       HllFace<V>(L + st[d], R + st[d], cs, d, Fhi);
+      if (fover) {  // flux correction: a finer neighbour's restricted fluxes replace ours
+        const int at = d == 0 ? i - G.lo0 : d == 1 ? j - G.lo1 : k - G.lo2;
+        const int top = d == 0 ? G.hi0 - i : d == 1 ? G.hi1 - j : G.hi2 - k;
+        if (at == 0 || top == 0) {
+          const int a = d == 0 ? j - G.lo1 : i - G.lo0;
+          const int q = d == 2 ? j - G.lo1 : k - G.lo2;
+          const int rows = d == 0 ? n1 : n0;
+          const int comp = rows * (d == 2 ? n1 : n2);
+          const int olo = at == 0 ? fover[6 * b + 2 * d] : -1;
+          const int ohi = top == 0 ? fover[6 * b + 2 * d + 1] : -1;
+          if (olo >= 0)
+#pragma unroll
+            for (int n = 0; n < V; ++n) Flo[n] = fcor[olo + a + q * rows + n * comp];
+          if (ohi >= 0)
+#pragma unroll
+            for (int n = 0; n < V; ++n) Fhi[n] = fcor[ohi + a + q * rows + n * comp];
+        }
+      }
       const double A = C.area[d];
 #pragma unroll
       for (int n = 0; n < V; ++n) {
@@ -457,6 +478,139 @@ __global__ void __launch_bounds__(kWG)
     for (int w = 1; w < kWG / 64; ++w) v = DMin(v, wmin[w]);
     partial[blockIdx.y * gridDim.x + blockIdx.x] = v;
   }
+}
+
+// RestrictAverage::Do for face element F(D+1) (pr_ops.hpp:105-164): area-weighted average of
+// the fine face fluxes fqp (p along the lower transverse dimension t1, q along t2) sharing one
+// coarse face, in the reference's fixed summation order; faces beyond the active dimensions
+// carry zero weight and a zero term, as in the reference loop.
+template <int D>
+__device__ __forceinline__ double RestrictFaceValue(double area, double f00, double f01,
+                                                    double f10, double f11, int s1, int s2) {
+  double T[2][2][2], W[2][2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) T[a][b][c] = W[a][b][c] = 0.0;
+  const double f[2][2] = {{f00, f01}, {f10, f11}};
+#pragma unroll
+  for (int q = 0; q < 2; ++q)
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      if (p >= s1 || q >= s2) continue;
+      // [ok][oj][oi]: D=0 -> (t2 = z, t1 = y); D=1 -> (t2 = z, t1 = x); D=2 -> (t2 = y, t1 = x)
+      double &t = D == 0 ? T[q][p][0] : D == 1 ? T[q][0][p] : T[0][q][p];
+      double &w = D == 0 ? W[q][p][0] : D == 1 ? W[q][0][p] : W[0][q][p];
+      w = area;
+      t = w * f[q][p];
+    }
+  const double tvol = ((W[0][0][0] + W[0][1][0]) + (W[0][0][1] + W[0][1][1])) +
+                      ((W[1][0][0] + W[1][1][0]) + (W[1][0][1] + W[1][1][1]));
+  return (((T[0][0][0] + T[0][1][0]) + (T[0][0][1] + T[0][1][1])) +
+          ((T[1][0][0] + T[1][1][0]) + (T[1][0][1] + T[1][1][1]))) /
+         tvol;
+}
+
+__device__ __forceinline__ double RestrictFace(int d, double area, double f00, double f01,
+                                               double f10, double f11, int s1, int s2) {
+  if (d == 0) return RestrictFaceValue<0>(area, f00, f01, f10, f11, s1, s2);
+  if (d == 1) return RestrictFaceValue<1>(area, f00, f01, f10, f11, s1, s2);
+  return RestrictFaceValue<2>(area, f00, f01, f10, f11, s1, s2);
+}
+
+// Brick offset of fine face (p, q) of patch cell (a, c): the face index along d, and transverse
+// fine cells step * (a, c) + (p, q) from the patch start.
+__device__ __forceinline__ std::int64_t PatchFineCell(const FluxPatch &P, const Geo &G, int a,
+                                                      int c, int p, int q) {
+  const int u = P.ft[0] + P.step[0] * a + p, v = P.ft[1] + P.step[1] * c + q;
+  const int i = P.d == 0 ? P.fface : u;
+  const int j = P.d == 1 ? P.fface : (P.d == 0 ? u : v);
+  const int k = P.d == 2 ? P.fface : v;
+  return (static_cast<std::int64_t>(k) * G.e1 + j) * G.e0 + i;
+}
+
+__device__ __forceinline__ std::int64_t PatchCoarseCell(const FluxPatch &P, const Geo &G, int a,
+                                                        int c) {
+  const int u = P.ct[0] + a, v = P.ct[1] + c;
+  const int i = P.d == 0 ? P.cface : u;
+  const int j = P.d == 1 ? P.cface : (P.d == 0 ? u : v);
+  const int k = P.d == 2 ? P.cface : v;
+  return (static_cast<std::int64_t>(k) * G.e1 + j) * G.e0 + i;
+}
+
+// flxcor_send of the fused stage: one thread per coarse face cell of a patch evaluates the HLL
+// fluxes of its 2x2 fine faces from the fine block's face states (the values CalculateFluxes
+// would store there) and writes their restriction to the correction store.
+template <int V>
+__global__ void __launch_bounds__(64)
+    FlxcorFaceKernel(const FluxPatch *__restrict__ patches, const double *__restrict__ face,
+                     double *__restrict__ store, Geo G, int nblocks) {
+  const FluxPatch &P = patches[blockIdx.x];
+  const int t = blockIdx.y * 64 + threadIdx.x;
+  if (t >= P.n[0] * P.n[1]) return;
+  const int a = t % P.n[0], c = t / P.n[0];
+  const std::int64_t cs = G.cells, slab = G.cells * V * nblocks;
+  const std::int64_t base = static_cast<std::int64_t>(P.fine) * V * cs;
+  double F[2][2][V];
+#pragma unroll
+  for (int q = 0; q < 2; ++q)
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      if (p >= P.step[0] || q >= P.step[1]) {
+#pragma unroll
+        for (int n = 0; n < V; ++n) F[q][p][n] = 0.0;
+        continue;
+      }
+      const std::int64_t o = base + PatchFineCell(P, G, a, c, p, q);
+      HllFace<V>(face + (2 * P.d) * slab + o, face + (2 * P.d + 1) * slab + o, cs, P.d,
+                 F[q][p]);
+    }
+  const std::int64_t dst = P.dst + a + static_cast<std::int64_t>(c) * P.rows;
+#pragma unroll
+  for (int n = 0; n < V; ++n)
+    store[dst + static_cast<std::int64_t>(n) * P.comp] =
+        RestrictFace(P.d, P.area, F[0][0][n], F[0][1][n], F[1][0][n], F[1][1][n], P.step[0],
+                     P.step[1]);
+}
+
+// SendBoundBufs<flxcor_send> over stored face fluxes [dir][block][comp][brick].
+__global__ void __launch_bounds__(64)
+    FlxcorSendKernel(const FluxPatch *__restrict__ patches, const double *__restrict__ flux,
+                     double *__restrict__ store, Geo G, int nblocks, int nc) {
+  const FluxPatch &P = patches[blockIdx.x];
+  const int t = blockIdx.y * 64 + threadIdx.x;
+  if (t >= P.n[0] * P.n[1]) return;
+  const int a = t % P.n[0], c = t / P.n[0];
+  const std::int64_t cs = G.cells, slab = G.cells * nc * nblocks;
+  const double *F = flux + P.d * slab + static_cast<std::int64_t>(P.fine) * nc * cs;
+  const std::int64_t o00 = PatchFineCell(P, G, a, c, 0, 0);
+  const std::int64_t o01 = P.step[0] > 1 ? PatchFineCell(P, G, a, c, 1, 0) : o00;
+  const std::int64_t o10 = P.step[1] > 1 ? PatchFineCell(P, G, a, c, 0, 1) : o00;
+  const std::int64_t o11 =
+      P.step[0] > 1 && P.step[1] > 1 ? PatchFineCell(P, G, a, c, 1, 1) : o00;
+  const std::int64_t dst = P.dst + a + static_cast<std::int64_t>(c) * P.rows;
+  for (int n = 0; n < nc; ++n) {
+    const double *Fn = F + n * cs;
+    store[dst + static_cast<std::int64_t>(n) * P.comp] =
+        RestrictFace(P.d, P.area, Fn[o00], Fn[o01], Fn[o10], Fn[o11], P.step[0], P.step[1]);
+  }
+}
+
+// SetBounds<flxcor_recv>: the coarse block's stored fluxes on the patch's part of its face.
+__global__ void __launch_bounds__(64)
+    FlxcorSetKernel(const FluxPatch *__restrict__ patches, const double *__restrict__ store,
+                    double *__restrict__ flux, Geo G, int nblocks, int nc) {
+  const FluxPatch &P = patches[blockIdx.x];
+  const int t = blockIdx.y * 64 + threadIdx.x;
+  if (t >= P.n[0] * P.n[1]) return;
+  const int a = t % P.n[0], c = t / P.n[0];
+  const std::int64_t cs = G.cells, slab = G.cells * nc * nblocks;
+  double *F = flux + P.d * slab + static_cast<std::int64_t>(P.coarse) * nc * cs +
+              PatchCoarseCell(P, G, a, c);
+  const std::int64_t src = P.dst + a + static_cast<std::int64_t>(c) * P.rows;
+  for (int n = 0; n < nc; ++n) F[n * cs] = store[src + static_cast<std::int64_t>(n) * P.comp];
 }
 
 // Final CFL minimum and EvolutionDriver::SetGlobalTimeStep on device (driver.cpp:210-235):
@@ -583,7 +737,7 @@ int FusedUpdatePartials(const Topology &t) {
 pth_status BurgersFusedUpdate(const Topology &t, int V, const double *face, const double *U0,
                               const double *base, double *out, double *derived, double *flux,
                               double *partial, const BlockCoefs *coef, double beta,
-                              const double *dtvar, hipStream_t s) {
+                              const double *dtvar, const FluxCorrection *fc, hipStream_t s) {
   if (V > kMaxV) {
     SetError("fused Burgers update supports at most 16 components");
     return PTH_FAIL;
@@ -597,10 +751,13 @@ pth_status BurgersFusedUpdate(const Topology &t, int V, const double *face, cons
   const int nb = static_cast<int>(t.blocks.size());
   const int cells = t.fine.n[0] * t.fine.n[1] * t.fine.n[2];
   const dim3 grid((cells + kWG - 1) / kWG, nb);
+  const bool over = fc && !fc->host.empty();
+  const int *fover = over ? fc->dev_face : nullptr;
+  const double *fcor = over ? fc->store : nullptr;
 #define PTH_FUSED_CASE(NV)                                                                     \
   case NV:                                                                                     \
     hipLaunchKernelGGL(FusedUpdateKernel<NV>, grid, dim3(kWG), 0, s, face, U0, base, out,      \
-                       derived, flux, partial, coef, G, nb, beta, dtvar);                      \
+                       derived, flux, partial, coef, G, nb, beta, dtvar, fover, fcor);         \
     break;
   switch (V) {
     PTH_FUSED_CASE(4)
@@ -618,6 +775,67 @@ pth_status BurgersFusedUpdate(const Topology &t, int V, const double *face, cons
     PTH_FUSED_CASE(16)
   }
 #undef PTH_FUSED_CASE
+  PTH_HIP(hipGetLastError());
+  return PTH_COMPLETE;
+}
+
+pth_status FlxcorFromFaces(const Topology &t, int V, const FluxCorrection &fc,
+                          const double *face, hipStream_t s) {
+  if (fc.host.empty()) return PTH_COMPLETE;
+  if (V > kMaxV || V < 4 || fc.ncomp != V) {
+    SetError("flux correction from face states: component count mismatch");
+    return PTH_FAIL;
+  }
+  ProfScope prof("flxcor_faces", s);
+  const Geo G = MakeGeo(t, V);
+  const int nb = static_cast<int>(t.blocks.size());
+  const dim3 grid(static_cast<unsigned>(fc.host.size()), (fc.max_cells + 63) / 64);
+#define PTH_FLX_CASE(NV)                                                                     \
+  case NV:                                                                                   \
+    hipLaunchKernelGGL(FlxcorFaceKernel<NV>, grid, dim3(64), 0, s, fc.dev, face, fc.store, G, \
+                       nb);                                                                  \
+    break;
+  switch (V) {
+    PTH_FLX_CASE(4)
+    PTH_FLX_CASE(5)
+    PTH_FLX_CASE(6)
+    PTH_FLX_CASE(7)
+    PTH_FLX_CASE(8)
+    PTH_FLX_CASE(9)
+    PTH_FLX_CASE(10)
+    PTH_FLX_CASE(11)
+    PTH_FLX_CASE(12)
+    PTH_FLX_CASE(13)
+    PTH_FLX_CASE(14)
+    PTH_FLX_CASE(15)
+    PTH_FLX_CASE(16)
+  }
+#undef PTH_FLX_CASE
+  PTH_HIP(hipGetLastError());
+  return PTH_COMPLETE;
+}
+
+pth_status FlxcorSend(const Topology &t, const FluxCorrection &fc, const double *flux,
+                      hipStream_t s) {
+  if (fc.host.empty()) return PTH_COMPLETE;
+  ProfScope prof("flxcor_send", s);
+  const Geo G = MakeGeo(t, fc.ncomp);
+  const int nb = static_cast<int>(t.blocks.size());
+  const dim3 grid(static_cast<unsigned>(fc.host.size()), (fc.max_cells + 63) / 64);
+  hipLaunchKernelGGL(FlxcorSendKernel, grid, dim3(64), 0, s, fc.dev, flux, fc.store, G, nb,
+                     fc.ncomp);
+  PTH_HIP(hipGetLastError());
+  return PTH_COMPLETE;
+}
+
+pth_status FlxcorSet(const Topology &t, const FluxCorrection &fc, double *flux, hipStream_t s) {
+  if (fc.host.empty()) return PTH_COMPLETE;
+  ProfScope prof("flxcor_set", s);
+  const Geo G = MakeGeo(t, fc.ncomp);
+  const int nb = static_cast<int>(t.blocks.size());
+  const dim3 grid(static_cast<unsigned>(fc.host.size()), (fc.max_cells + 63) / 64);
+  hipLaunchKernelGGL(FlxcorSetKernel, grid, dim3(64), 0, s, fc.dev, fc.store, flux, G, nb,
+                     fc.ncomp);
   PTH_HIP(hipGetLastError());
   return PTH_COMPLETE;
 }

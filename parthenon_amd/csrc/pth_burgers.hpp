@@ -24,7 +24,14 @@ int FusedUpdatePartials(const Topology &t);
 pth_status BurgersFusedUpdate(const Topology &t, int V, const double *face, const double *U0,
                               const double *base, double *out, double *derived, double *flux,
                               double *partial, const BlockCoefs *coef, double beta,
-                              const double *dtvar, hipStream_t s);
+                              const double *dtvar, const FluxCorrection *fc, hipStream_t s);
+// flux correction: restricted fine fluxes into the correction store, from the face states of
+// the fused stage or from stored fluxes; and the store into the coarse blocks' stored fluxes
+pth_status FlxcorFromFaces(const Topology &t, int V, const FluxCorrection &fc,
+                           const double *face, hipStream_t s);
+pth_status FlxcorSend(const Topology &t, const FluxCorrection &fc, const double *flux,
+                      hipStream_t s);
+pth_status FlxcorSet(const Topology &t, const FluxCorrection &fc, double *flux, hipStream_t s);
 pth_status BurgersClock(const double *partial, int n, double cfl, double *dtvar, int advance,
                         hipStream_t s);
 

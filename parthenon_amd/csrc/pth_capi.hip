@@ -76,6 +76,7 @@ MeshData::~MeshData() {
 
 Mesh::~Mesh() {
   data.clear();
+  for (auto &kv : flxcor) kv.second.Free();
   for (auto &s : one_copy) {
     if (s.data) (void)hipFree(s.data);
     if (s.coarse) (void)hipFree(s.coarse);
@@ -88,6 +89,8 @@ Mesh::~Mesh() {
   if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
   if (graph) (void)hipGraphDestroy(graph);
   if (run_stream) (void)hipStreamDestroy(run_stream);
+  for (hipEvent_t e : order_ev)
+    if (e) (void)hipEventDestroy(e);
   if (dtvar) (void)hipFree(dtvar);
   if (scratch) (void)hipFree(scratch);
   if (pinned) (void)hipHostFree(pinned);
@@ -249,7 +252,7 @@ pth_status Mesh::Finalize() {
     arena[f] = all;
     arena_size[f] = off;
   }
-  scratch_n = 8192;
+  scratch_n = std::max<std::int64_t>(8192, FusedUpdatePartials(topo));
   PTH_HIP(hipMalloc(&scratch, sizeof(double) * (scratch_n + 8)));
   PTH_HIP(hipHostMalloc(&pinned, sizeof(double) * 8, hipHostMallocDefault));
   finalized = true;
@@ -419,6 +422,125 @@ pth_status Mesh::BuildCache(MeshData &md, int bt) {
   return PTH_COMPLETE;
 }
 
+void FluxCorrection::Free() {
+  if (dev) (void)hipFree(dev);
+  if (dev_face) (void)hipFree(dev_face);
+  if (store) (void)hipFree(store);
+  dev = nullptr;
+  dev_face = nullptr;
+  store = nullptr;
+  built = false;
+}
+
+// Geometry of the flux-correction patch that block `fine` sends to its coarser face neighbour
+// `coarse` lying across direction d on side sgn (+1 above, -1 below): the fine face index, the
+// receiver's face index, and the transverse ranges (the fine block covers the half of the
+// receiver's face given by the parity of its own lattice position).
+FluxPatch PatchGeometry(const Topology &topo, int fine, int coarse, int d, int sgn) {
+  const Extent &fe = topo.fine;
+  const Block &B = topo.blocks[fine];
+  const int t[2] = {d == 0 ? 1 : 0, d == 2 ? 1 : 2};
+  FluxPatch p;
+  std::memset(&p, 0, sizeof(p));
+  p.fine = fine;
+  p.coarse = coarse;
+  p.d = d;
+  p.fface = sgn > 0 ? fe.lo(d) + fe.n[d] : fe.lo(d);
+  p.cface = sgn > 0 ? fe.lo(d) : fe.lo(d) + fe.n[d];
+  for (int q = 0; q < 2; ++q) {
+    const int dim = t[q];
+    const bool act = topo.active[dim];
+    p.step[q] = act ? 2 : 1;
+    p.n[q] = act ? fe.n[dim] / 2 : 1;
+    p.ft[q] = fe.lo(dim);
+    p.ct[q] = fe.lo(dim) + (act ? static_cast<int>(B.at.x[dim] & 1) * (fe.n[dim] / 2) : 0);
+  }
+  return p;
+}
+
+// Patches of the flux-correction exchange of field f. A block sends to a face neighbour one
+// level coarser (loop_utils.hpp:147-151 with flxcor_send); the receiver's face is then covered
+// by 2^(ndim-1) such patches, one per child of its same-level neighbour, which the 2:1 level
+// balance of the mesh guarantees and the builder checks.
+FluxCorrection *Mesh::FluxPatches(int f) {
+  if (f < 0 || f >= static_cast<int>(fields.size()) || !(fields[f].flags & PTH_WITH_FLUXES)) {
+    SetError("flux correction needs a field with fluxes");
+    return nullptr;
+  }
+  FluxCorrection &c = flxcor[f];
+  if (c.built) return &c;
+  const Extent &fe = topo.fine;
+  const int nb = static_cast<int>(topo.blocks.size());
+  const int V = fields[f].ncomp;
+  c.ncomp = V;
+  c.host.clear();
+  c.face.assign(6 * static_cast<size_t>(nb), -1);
+  c.max_cells = 0;
+  std::vector<int> hits(6 * static_cast<size_t>(nb), 0);
+  std::int64_t total = 0;
+  for (int b = 0; b < nb; ++b) {
+    const Block &B = topo.blocks[b];
+    for (const Link &l : B.links) {
+      int nz = 0, d = -1;
+      for (int q = 0; q < 3; ++q)
+        if (l.off[q] != 0) {
+          ++nz;
+          d = q;
+        }
+      if (nz != 1 || l.level != B.at.level - 1) continue;
+      const int t[2] = {d == 0 ? 1 : 0, d == 2 ? 1 : 2};
+      FluxPatch p = PatchGeometry(topo, b, l.gid, d, l.off[d]);
+      const int side = l.off[d] > 0 ? 0 : 1;  // receiver's low face when it lies above
+      const size_t key = 6 * static_cast<size_t>(l.gid) + 2 * d + side;
+      if (c.face[key] < 0) {
+        c.face[key] = static_cast<int>(total);
+        total += static_cast<std::int64_t>(fe.n[t[0]]) * fe.n[t[1]] * V;
+        if (total >= (std::int64_t(1) << 31)) {
+          SetError("flux correction store exceeds 2^31 values");
+          return nullptr;
+        }
+      }
+      ++hits[key];
+      p.rows = fe.n[t[0]];
+      p.comp = fe.n[t[0]] * fe.n[t[1]];
+      p.dst = c.face[key] + (p.ct[0] - fe.lo(t[0])) + (p.ct[1] - fe.lo(t[1])) * p.rows;
+      p.area = B.area[d];
+      c.max_cells = std::max(c.max_cells, p.n[0] * p.n[1]);
+      c.host.push_back(p);
+    }
+  }
+  for (int b = 0; b < nb; ++b)
+    for (int d = 0; d < 3; ++d)
+      for (int s = 0; s < 2; ++s) {
+        const size_t key = 6 * static_cast<size_t>(b) + 2 * d + s;
+        if (c.face[key] < 0) continue;
+        int want = 1;
+        for (int q = 0; q < 3; ++q)
+          if (q != d && topo.active[q]) want *= 2;
+        if (hits[key] != want) {
+          SetError("flux correction: a coarse face is not covered by its finer neighbours");
+          return nullptr;
+        }
+      }
+  c.store_n = total;
+  if (!c.host.empty()) {
+    if (hipMalloc(&c.dev, sizeof(FluxPatch) * c.host.size()) != hipSuccess ||
+        hipMalloc(&c.dev_face, sizeof(int) * c.face.size()) != hipSuccess ||
+        hipMalloc(&c.store, sizeof(double) * total) != hipSuccess ||
+        hipMemset(c.store, 0, sizeof(double) * total) != hipSuccess ||
+        hipMemcpy(c.dev, c.host.data(), sizeof(FluxPatch) * c.host.size(),
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c.dev_face, c.face.data(), sizeof(int) * c.face.size(),
+                  hipMemcpyHostToDevice) != hipSuccess) {
+      SetError("flux correction: device allocation failed");
+      c.Free();
+      return nullptr;
+    }
+  }
+  c.built = true;
+  return &c;
+}
+
 }  // namespace pth
 
 namespace {
@@ -549,6 +671,34 @@ int pth_mesh_region_boxes(const pth_mesh *mh, int gid, int *out, int max) {
       put(3, m->topo.Indices(B, l, Side::kRecvBoundary, true));
     } else if (B.coarser_neighbor && B.at.level > 0) {
       put(4, m->topo.Indices(B, l, Side::kRecvBoundary, true));
+    }
+    // flux correction: 5 restrict+send box (coarse index space), 6 receive box
+    int nz = 0, d = -1;
+    for (int q = 0; q < 3; ++q)
+      if (l.off[q] != 0) {
+        ++nz;
+        d = q;
+      }
+    if (nz != 1 || (l.level != B.at.level - 1 && l.level != B.at.level + 1)) continue;
+    const int t[2] = {d == 0 ? 1 : 0, d == 2 ? 1 : 2};
+    Box box;
+    if (l.level == B.at.level - 1) {
+      const FluxPatch p = PatchGeometry(m->topo, gid, l.gid, d, l.off[d]);
+      const Extent &ce = m->topo.coarse;
+      box.lo[d] = box.hi[d] = ce.lo(d) + (p.fface == m->topo.fine.lo(d) ? 0 : ce.n[d]);
+      for (int q = 0; q < 2; ++q) {
+        box.lo[t[q]] = ce.lo(t[q]);
+        box.hi[t[q]] = ce.lo(t[q]) + p.n[q] - 1;
+      }
+      put(5, box);
+    } else {
+      const FluxPatch p = PatchGeometry(m->topo, l.gid, gid, d, -l.off[d]);
+      box.lo[d] = box.hi[d] = p.cface;
+      for (int q = 0; q < 2; ++q) {
+        box.lo[t[q]] = p.ct[q];
+        box.hi[t[q]] = p.ct[q] + p.n[q] - 1;
+      }
+      put(6, box);
     }
   }
   return n;
@@ -715,9 +865,40 @@ pth_status pth_field_download(pth_meshdata *mdh, const char *field, int which, d
 
 // ---- boundary communication ----
 
+static bool IsFlxcor(int bt) { return bt == PTH_BT_FLXCOR_SEND || bt == PTH_BT_FLXCOR_RECV; }
+
+// Flux correction over every field with fluxes (SendBoundBufs<flxcor_send>, ReceiveBoundBufs /
+// SetBounds<flxcor_recv>): the restricted fine fluxes go to the correction store, then into
+// the coarse blocks' stored fluxes. Nothing to do on a single-level mesh (no channels).
+static pth_status FlxcorStep(MeshData *md, int step) {
+  Mesh *m = md->mesh;
+  if (NeedFinal(m) != PTH_COMPLETE) return PTH_FAIL;
+  if (!m->topo.multilevel) return PTH_COMPLETE;
+  if (step == 1) {  // receive: one GPU, the store is ordered on the same stream
+    if (m->flxcor_state == BufState::kSending) m->flxcor_state = BufState::kReceived;
+    return m->flxcor_state == BufState::kReceived ? PTH_COMPLETE : PTH_INCOMPLETE;
+  }
+  if (step == 0 && m->flxcor_state != BufState::kStale) return PTH_INCOMPLETE;
+  if (step == 2 && m->flxcor_state != BufState::kReceived) {
+    SetError("SetBounds: flux corrections are not in a received state");
+    return PTH_FAIL;
+  }
+  for (size_t f = 0; f < m->fields.size(); ++f) {
+    if (!(m->fields[f].flags & PTH_WITH_FLUXES)) continue;
+    FluxCorrection *fc = m->FluxPatches(static_cast<int>(f));
+    if (!fc) return PTH_FAIL;
+    const pth_status st = step == 0 ? FlxcorSend(m->topo, *fc, md->store[f].flux, m->stream)
+                                    : FlxcorSet(m->topo, *fc, md->store[f].flux, m->stream);
+    if (st != PTH_COMPLETE) return PTH_FAIL;
+  }
+  m->flxcor_state = step == 0 ? BufState::kSending : BufState::kStale;
+  return PTH_COMPLETE;
+}
+
 pth_status pth_send_bound_bufs(pth_meshdata *mdh, int bt) {
   MeshData *md = Impl(mdh);
   Mesh *m = md->mesh;
+  if (IsFlxcor(bt)) return FlxcorStep(md, 0);
   bt = Bt(bt);
   if (m->BuildCache(*md, bt) != PTH_COMPLETE) return PTH_FAIL;
   BoundCache &c = md->cache[bt];
@@ -734,12 +915,14 @@ pth_status pth_send_bound_bufs(pth_meshdata *mdh, int bt) {
 
 pth_status pth_start_receive_bound_bufs(pth_meshdata *mdh, int bt) {
   MeshData *md = Impl(mdh);
+  if (IsFlxcor(bt)) return NeedFinal(md->mesh);
   return md->mesh->BuildCache(*md, Bt(bt));
 }
 
 pth_status pth_receive_bound_bufs(pth_meshdata *mdh, int bt) {
   MeshData *md = Impl(mdh);
   Mesh *m = md->mesh;
+  if (IsFlxcor(bt)) return FlxcorStep(md, 1);
   bt = Bt(bt);
   if (m->BuildCache(*md, bt) != PTH_COMPLETE) return PTH_FAIL;
   if (md->cache[bt].nbuffers == 0) return PTH_COMPLETE;
@@ -752,6 +935,7 @@ pth_status pth_receive_bound_bufs(pth_meshdata *mdh, int bt) {
 pth_status pth_set_bounds(pth_meshdata *mdh, int bt) {
   MeshData *md = Impl(mdh);
   Mesh *m = md->mesh;
+  if (IsFlxcor(bt)) return FlxcorStep(md, 2);
   bt = Bt(bt);
   if (m->BuildCache(*md, bt) != PTH_COMPLETE) return PTH_FAIL;
   BoundCache &c = md->cache[bt];

@@ -3,8 +3,9 @@
 // (src/time_integration/low_storage_integrator.cpp:67-83) and EvolutionDriver's time-step
 // control (src/driver/driver.cpp:194-235), with every host decision hoisted out of the stage:
 //
-//   stage 1: recon(base) -> fused update base -> "1" -> exchange("1")
-//   stage 2: recon("1")  -> fused update ("1", base) -> base + CFL partials -> exchange(base)
+//   stage 1: recon(base) -> [flux correction] -> fused update base -> "1" -> exchange("1")
+//   stage 2: recon("1")  -> [flux correction] -> fused update ("1", base) -> base + CFL
+//            partials -> exchange(base)
 //   clock:   time += dt; dt = min(2 dt, cfl * min)   (all on device)
 //
 // The simulation clock lives in device memory, so a whole cycle has no host synchronization
@@ -49,15 +50,12 @@ struct Ctx {
   Mesh *m;
   MeshData *base, *one;
   int fu, fd, V;
+  const FluxCorrection *fc = nullptr;  // multilevel: flux-correction patches of U
 };
 
 pth_status Setup(Mesh *m, Ctx &x) {
   if (!m->finalized) {
     SetError("mesh is not finalized");
-    return PTH_FAIL;
-  }
-  if (m->topo.multilevel) {
-    SetError("the device Burgers driver needs flux correction on multilevel meshes (not built)");
     return PTH_FAIL;
   }
   x.m = m;
@@ -68,6 +66,14 @@ pth_status Setup(Mesh *m, Ctx &x) {
     return PTH_FAIL;
   }
   x.V = m->fields[x.fu].ncomp;
+  if (m->topo.multilevel) {
+    if (!(m->fields[x.fu].flags & PTH_WITH_FLUXES)) {
+      SetError("multilevel Burgers needs U declared WithFluxes (flux correction)");
+      return PTH_FAIL;
+    }
+    x.fc = m->FluxPatches(x.fu);
+    if (!x.fc) return PTH_FAIL;
+  }
   x.base = m->Get("base");
   x.one = m->Get("1");
   if (!x.base || !x.one) return PTH_FAIL;
@@ -97,9 +103,12 @@ pth_status Stage(Ctx &x, int stage, hipStream_t s) {
   if (BurgersReconTasks(m.topo, x.V, mc0.store[x.fu].data, m.face, m.run_recon, s) !=
       PTH_COMPLETE)
     return PTH_FAIL;
+  // LoadAndSendFluxCorrections / SetFluxCorrections: the restricted fine fluxes land in the
+  // correction store that the fused update reads in place of its own coarse-face fluxes
+  if (x.fc && FlxcorFromFaces(m.topo, x.V, *x.fc, m.face, s) != PTH_COMPLETE) return PTH_FAIL;
   if (BurgersFusedUpdate(m.topo, x.V, m.face, mc0.store[x.fu].data, x.base->store[x.fu].data,
                          mc1.store[x.fu].data, derived, nullptr,
-                         stage == 2 ? m.scratch : nullptr, m.coef, beta, m.dtvar,
+                         stage == 2 ? m.scratch : nullptr, m.coef, beta, m.dtvar, x.fc,
                          s) != PTH_COMPLETE)
     return PTH_FAIL;
   if (Exchange(m, mc1, s) != PTH_COMPLETE) return PTH_FAIL;
@@ -111,6 +120,37 @@ pth_status Stage(Ctx &x, int stage, hipStream_t s) {
 pth_status Cycle(Ctx &x, hipStream_t s) {
   if (Stage(x, 1, s) != PTH_COMPLETE) return PTH_FAIL;
   return Stage(x, 2, s);
+}
+
+// ncycles eager cycles, or replays of the cycle captured on first use
+pth_status RunCycles(Mesh &m, Ctx &x, hipStream_t s, int ncycles, bool use_graph) {
+  if (!use_graph) {
+    for (int c = 0; c < ncycles; ++c)
+      if (Cycle(x, s) != PTH_COMPLETE) return PTH_FAIL;
+    return PTH_COMPLETE;
+  }
+  if (!m.graph_exec) {
+    // tables must exist before capture (their uploads are synchronous copies)
+    if (m.BuildCache(*x.base, PTH_BT_ANY) != PTH_COMPLETE) return PTH_FAIL;
+    if (m.BuildCache(*x.one, PTH_BT_ANY) != PTH_COMPLETE) return PTH_FAIL;
+    const bool prof = Prof().on;
+    Prof().on = false;  // no event records inside the captured cycle
+    PTH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const pth_status st = Cycle(x, s);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    Prof().on = prof;
+    if (st != PTH_COMPLETE) {
+      if (g) (void)hipGraphDestroy(g);
+      return PTH_FAIL;
+    }
+    PTH_HIP(e);
+    m.graph = g;
+    PTH_HIP(hipGraphInstantiate(&m.graph_exec, g, nullptr, nullptr, 0));
+  }
+  ProfScope prof("graph_cycle", s);
+  for (int c = 0; c < ncycles; ++c) PTH_HIP(hipGraphLaunch(m.graph_exec, s));
+  return PTH_COMPLETE;
 }
 
 }  // namespace
@@ -182,32 +222,15 @@ pth_status pth_burgers_run_cycles(pth_mesh *mh, int ncycles, int use_graph) {
   Ctx x;
   if (Setup(m, x) != PTH_COMPLETE) return PTH_FAIL;
   hipStream_t s = m->run_stream;
-  if (!use_graph) {
-    for (int c = 0; c < ncycles; ++c)
-      if (Cycle(x, s) != PTH_COMPLETE) return PTH_FAIL;
-    return PTH_COMPLETE;
-  }
-  if (!m->graph_exec) {
-    // tables must exist before capture (their uploads are synchronous copies)
-    if (m->BuildCache(*x.base, PTH_BT_ANY) != PTH_COMPLETE) return PTH_FAIL;
-    if (m->BuildCache(*x.one, PTH_BT_ANY) != PTH_COMPLETE) return PTH_FAIL;
-    const bool prof = Prof().on;
-    Prof().on = false;  // no event records inside the captured cycle
-    PTH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    const pth_status st = Cycle(x, s);
-    hipGraph_t g = nullptr;
-    const hipError_t e = hipStreamEndCapture(s, &g);
-    Prof().on = prof;
-    if (st != PTH_COMPLETE) {
-      if (g) (void)hipGraphDestroy(g);
-      return PTH_FAIL;
-    }
-    PTH_HIP(e);
-    m->graph = g;
-    PTH_HIP(hipGraphInstantiate(&m->graph_exec, g, nullptr, nullptr, 0));
-  }
-  ProfScope prof("graph_cycle", s);
-  for (int c = 0; c < ncycles; ++c) PTH_HIP(hipGraphLaunch(m->graph_exec, s));
+  for (hipEvent_t &e : m->order_ev)
+    if (!e) PTH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  // the cycles start after work already queued on the mesh stream (uploads, task calls), and
+  // work queued on the mesh stream afterwards (downloads, task calls) starts after the cycles
+  PTH_HIP(hipEventRecord(m->order_ev[0], m->stream));
+  PTH_HIP(hipStreamWaitEvent(s, m->order_ev[0], 0));
+  if (RunCycles(*m, x, s, ncycles, use_graph != 0) != PTH_COMPLETE) return PTH_FAIL;
+  PTH_HIP(hipEventRecord(m->order_ev[1], s));
+  PTH_HIP(hipStreamWaitEvent(m->stream, m->order_ev[1], 0));
   return PTH_COMPLETE;
 }
 

@@ -177,6 +177,43 @@ constexpr int kIdxShift = 27;
 constexpr std::uint32_t kIdxMask = (1u << kIdxShift) - 1;
 constexpr int kIdxSlots = 32;
 
+// Flux correction (reference flxcor_send / flxcor_recv, boundary_communication.cpp:454-461; the
+// face-element index boxes of bnd_info.cpp:69-101, 106-212, 273-305 and 387-403; the neighbour
+// selection of loop_utils.hpp:133-162): one patch per (fine block, coarser face neighbour).
+// Its coarse face cells are the 2x2 (2-D: 2, 1-D: 1) area-weighted averages of the fine
+// block's face fluxes; they replace the coarse neighbour's fluxes on the matching part of its
+// face. Values live in a correction store holding one image per corrected block face,
+// laid out [comp][t2][t1] over that face's interior cells (t1 < t2 the transverse dimensions).
+struct FluxPatch {
+  int fine, coarse, d;
+  int fface;          // fine face index along d (g or g + n)
+  int cface;          // receiver face index along d
+  int ft[2];          // fine transverse start (t1, t2)
+  int ct[2];          // receiver transverse start (t1, t2)
+  int n[2];           // coarse face cells along t1, t2
+  int step[2];        // fine cells per coarse cell along t1, t2 (2 if active, else 1)
+  int rows, comp;     // store strides between t2 rows and between components
+  std::int64_t dst;   // store offset of coarse face cell (0, 0), component 0
+  double area;        // fine face area (reference Coordinates::Volume<F(d+1)>)
+  double pad;
+};
+static_assert(sizeof(FluxPatch) % 8 == 0, "patch record alignment");
+
+FluxPatch PatchGeometry(const Topology &topo, int fine, int coarse, int d, int sgn);
+
+struct FluxCorrection {  // per field with fluxes
+  bool built = false;
+  int ncomp = 0;
+  std::vector<FluxPatch> host;
+  std::vector<int> face;   // [block][6] store offset of face (d, side) image, or -1
+  FluxPatch *dev = nullptr;
+  int *dev_face = nullptr;
+  double *store = nullptr;
+  std::int64_t store_n = 0;
+  int max_cells = 0;       // largest patch (coarse face cells)
+  void Free();
+};
+
 struct FieldSpec {
   std::string name;
   int ncomp;
@@ -229,6 +266,7 @@ class Mesh {
   // device-resident Burgers driver state: clock {dt, time, estimate} and the captured cycle
   double *dtvar = nullptr;
   hipStream_t run_stream = nullptr;
+  hipEvent_t order_ev[2] = {nullptr, nullptr};  // mesh stream <-> driver stream ordering
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   double run_cfl = 0.0;
@@ -241,6 +279,8 @@ class Mesh {
   double *scratch = nullptr;                    // reduction partials
   double *pinned = nullptr;                     // host-pinned readback slot
   std::int64_t scratch_n = 0;
+  std::map<int, FluxCorrection> flxcor;         // by field index (fields with fluxes)
+  BufState flxcor_state = BufState::kStale;
 
   ~Mesh();
   int FieldIndex(const std::string &n) const;
@@ -249,6 +289,8 @@ class Mesh {
   MeshData *Get(const std::string &name);
   pth_status Allocate(MeshData &md);
   pth_status BuildCache(MeshData &md, int bt);
+  // patches of field f (built on first use); null with an error if f has no fluxes
+  FluxCorrection *FluxPatches(int f);
 };
 
 // Optional per-launch timing with HIP events on the launch stream (instrumentation regions

@@ -3,8 +3,11 @@
 // (benchmarks/burgers/burgers_driver.cpp:53-148) with a sequential task list, the RK2
 // integrator (low_storage_integrator.cpp:67-83) and the driver's dt rule (driver.cpp:210-235).
 //
-// usage: burgers_task_driver <ncycles> <out.bin>
-// writes the final "base" U (float64, [block][comp][k][j][i]) followed by dt and time.
+// usage: burgers_task_driver <ncycles> <out.bin> [sphere_radius]
+// writes the final "base" U (float64, [block][comp][k][j][i]) followed by dt and time. With a
+// sphere radius the mesh is statically refined and the stage adds the flux-correction tasks
+// (burgers_driver.cpp:89-96) and ProlongateBounds after the sets (the library exchange order
+// of AddBoundaryExchangeTasks, boundary_communication.cpp:406-445).
 #include <cstdio>
 #include <cstdlib>
 #include <functional>
@@ -42,14 +45,17 @@ struct Integrator {
 
 int main(int argc, char **argv) {
   if (argc < 3) {
-    std::fprintf(stderr, "usage: %s ncycles out.bin\n", argv[0]);
+    std::fprintf(stderr, "usage: %s ncycles out.bin [sphere_radius]\n", argv[0]);
     return 2;
   }
   const int ncycles = std::atoi(argv[1]);
+  const Real radius = argc > 3 ? std::atof(argv[3]) : 0.0;
   const Real cfl = 0.4;
   try {
-    // burgers_package::Initialize: U (3 + 4 scalars) and the OneCopy derived field
+    // burgers_package::Initialize: U (3 + 4 scalars) and the OneCopy derived field; with a
+    // radius, two levels of static refinement on the blocks meeting a centred sphere
     Mesh mesh({64, 64, 64}, {16, 16, 16}, 4);
+    if (radius > 0.0) mesh.AddRefinementSphere({0.0, 0.0, 0.0}, radius, 2);
     mesh.AddField("U", 7, Metadata::FillGhost | Metadata::Independent | Metadata::WithFluxes);
     mesh.AddField("derived", 1, Metadata::OneCopy);
     mesh.Initialize();
@@ -60,6 +66,7 @@ int main(int argc, char **argv) {
     SendBoundaryBuffers(base);
     ReceiveBoundaryBuffers(base);
     SetBoundaries(base);
+    if (mesh.multilevel()) ProlongateBoundaries(base);
     burgers_package::CalculateDerived(base.get());
     Integrator integ;
     Real time = 0.0;
@@ -79,7 +86,11 @@ int main(int argc, char **argv) {
         constexpr auto nonlocal = BoundaryType::nonlocal;
         TaskList tl;
         tl.AddTask([&] { return StartReceiveBoundBufs<any>(mc1); });
+        tl.AddTask([&] { return StartReceiveFluxCorrections(mc0); });
         tl.AddTask([&] { return burgers_package::CalculateFluxes(mc0.get()); });
+        tl.AddTask([&] { return LoadAndSendFluxCorrections(mc0); });
+        tl.AddTask([&] { return ReceiveFluxCorrections(mc0); });
+        tl.AddTask([&] { return SetFluxCorrections(mc0); });
         tl.AddTask([&] { return Update::FluxDivergence(mc0.get(), mdudt.get()); });
         tl.AddTask([&] { return Update::AverageIndependentData(mc0.get(), mbase.get(), beta); });
         tl.AddTask([&] {
@@ -91,6 +102,8 @@ int main(int argc, char **argv) {
         tl.AddTask([&] { return SetBounds<local>(mc1); });
         tl.AddTask([&] { return ReceiveBoundBufs<nonlocal>(mc1); });
         tl.AddTask([&] { return SetBounds<nonlocal>(mc1); });
+        // the library exchange order (AddBoundaryExchangeTasks): prolongation after the sets
+        if (mesh.multilevel()) tl.AddTask([&] { return ProlongateBounds<any>(mc1); });
         tl.AddTask([&] {
           burgers_package::CalculateDerived(mc1.get());
           return TaskStatus::complete;

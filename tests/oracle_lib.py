@@ -63,7 +63,6 @@ def lib():
             "orc_burgers_mass_history": (D, [P, DP, DP, D]),
             "orc_burgers_set_flux_correction": (None, [P, I]),
             "orc_burgers_totals": (None, [P, DP]),
-            "orc_mesh_flxcor_boxes": (I, [P, I, I, C.POINTER(I), I]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -130,11 +129,6 @@ class OracleMesh:
         buf = (I * (7 * 512))()
         n = lib().orc_mesh_region_boxes(self.h, gid, buf, 512)
         return np.array(buf[:7 * n], dtype=np.int64).reshape(n, 7)
-
-    def flxcor_boxes(self, gid, V=1):
-        buf = (I * (8 * 64))()
-        n = lib().orc_mesh_flxcor_boxes(self.h, gid, V, buf, 64)
-        return np.array(buf[:8 * n], dtype=np.int64).reshape(n, 8)
 
     def field_shape(self, ncomp):
         e = self.ent

@@ -151,3 +151,111 @@ def test_conservation_to_roundoff():
     n = 32 ** 3
     # periodic flux form: sum of U over the mesh changes only by round-off
     assert np.all(np.abs(t1 - t0) <= 1e-12 * n * np.maximum(1.0, np.abs(t0) / n)), (t0, t1)
+
+
+# Static refinement: a box refined once on a 32^3 mesh of 8^3 blocks (120 blocks), and the
+# static-refined benchmark mesh: 64^3 in 16^3 blocks, two levels on the blocks meeting the
+# sphere of radius 0.25 at the centre (960 blocks), nghost 4.
+REFINED = {
+    "box120": dict(nx=(32, 32, 32), block=(8, 8, 8),
+                   refine_boxes=[((-0.1, 0.1, -0.1, 0.1, -0.1, 0.1), 1)]),
+    "sphere960": dict(nx=(64, 64, 64), block=(16, 16, 16),
+                      refine_spheres=[((0.0, 0.0, 0.0), 0.25, 2)]),
+}
+
+
+@pytest.mark.parametrize("name", ["box120", "sphere960"])
+def test_multilevel_task_driver_matches_oracle(name):
+    # task-by-task stage: fluxes, flux correction, divergence, update, exchange with
+    # restriction and prolongation
+    spec = REFINED[name]
+    print(f"case multilevel task driver {name}", flush=True)
+    from oracle_lib import lib
+    m, d, om, ob = pair(nx=spec["nx"], block=spec["block"],
+                        refine_boxes=spec.get("refine_boxes", ()),
+                        refine_spheres=spec.get("refine_spheres", ()))
+    assert m.multilevel and m.nblocks == om.nblocks
+    lib().orc_burgers_pgen_sinusoid(ob.h, 2)
+    lib().orc_burgers_initialize(ob.h)
+    d.initialize("sinusoid", seed=2)
+    assert np.array_equal(d.base.download("U").ravel(), ob.get(0)), "initial state"
+    ncyc = 3 if name == "box120" else 2
+    for c in range(ncyc):
+        d.cycle()
+        lib().orc_burgers_cycles(ob.h, 1)
+        assert_close(d.base.download("U"), ob.get(0), f"U after cycle {c + 1}")
+        assert d.dt == ob.dt
+
+
+@pytest.mark.parametrize("use_graph", [False, True], ids=["eager", "graph"])
+@pytest.mark.parametrize("name", ["box120", "sphere960"])
+def test_multilevel_device_driver_matches_oracle(name, use_graph):
+    # fused stage with the flux-correction launch between reconstruction and update
+    spec = REFINED[name]
+    print(f"case multilevel device driver {name} graph={use_graph}", flush=True)
+    from parthenon_amd.burgers import DeviceBurgersDriver
+    from oracle_lib import lib
+    m = make_burgers_mesh(spec["nx"], spec["block"], 4, 4,
+                          refine_boxes=spec.get("refine_boxes", ()),
+                          refine_spheres=spec.get("refine_spheres", ()))
+    d = DeviceBurgersDriver(m, cfl=0.4, use_graph=use_graph)
+    d.initialize("sinusoid", seed=2)
+    om = OracleMesh(spec["nx"], spec["block"], 4, boxes=spec.get("refine_boxes", ()),
+                    spheres=spec.get("refine_spheres", ()))
+    ob = OracleBurgers(om, 4, 0.4, 0)
+    lib().orc_burgers_pgen_sinusoid(ob.h, 2)
+    lib().orc_burgers_initialize(ob.h)
+    dt, _ = d.clock()
+    assert dt == ob.dt
+    ncyc = 3 if name == "box120" else 2
+    for c in range(ncyc):
+        d.cycles(1)
+        lib().orc_burgers_cycles(ob.h, 1)
+        assert_close(d.base.download("U"), ob.get(0), f"U after cycle {c + 1}")
+        dt, t = d.clock()
+        assert dt == ob.dt and t == lib().orc_burgers_time(ob.h)
+
+
+def test_cpp_task_driver_multilevel_matches_oracle(tmp_path):
+    import os
+    import subprocess
+    from oracle_lib import lib
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "burgers_task_driver")
+    out = str(tmp_path / "u.bin")
+    print("case cpp task driver sphere960 2 cycles", flush=True)
+    r = subprocess.run([exe, "2", out, "0.25"], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = np.fromfile(out, dtype=np.float64)
+    U, dt, t = raw[:-2], raw[-2], raw[-1]
+    om = OracleMesh((64, 64, 64), (16, 16, 16), 4, spheres=[((0.0, 0.0, 0.0), 0.25, 2)])
+    ob = OracleBurgers(om, 4, 0.4, 0)
+    lib().orc_burgers_pgen_sinusoid(ob.h, 0)
+    lib().orc_burgers_initialize(ob.h)
+    lib().orc_burgers_cycles(ob.h, 2)
+    assert_close(U, ob.get(0), "C++ driver U after 2 cycles (sphere960)")
+    assert dt == ob.dt and t == lib().orc_burgers_time(ob.h)
+
+
+def test_multilevel_conservation_to_roundoff():
+    # with flux correction the two-level scheme conserves sum(U * cell volume) over the leaf
+    # blocks to round-off
+    print("case multilevel conservation", flush=True)
+    from parthenon_amd.burgers import DeviceBurgersDriver
+    spec = REFINED["box120"]
+    m = make_burgers_mesh(spec["nx"], spec["block"], 4, 1, refine_boxes=spec["refine_boxes"])
+    d = DeviceBurgersDriver(m, cfl=0.4, use_graph=True)
+    d.initialize("sinusoid", seed=7)
+    g = m.ng[0]
+    vol = np.array([np.prod(np.subtract(m.block(b)["xmax"], m.block(b)["xmin"])) / 8 ** 3
+                    for b in range(m.nblocks)])
+
+    def totals():
+        U = d.base.download("U")[:, :, g:-g, g:-g, g:-g]
+        return (U.sum(axis=(2, 3, 4)) * vol[:, None]).sum(axis=0)
+
+    t0 = totals()
+    d.cycles(4)
+    t1 = totals()
+    assert np.all(np.isfinite(t1))
+    assert np.max(np.abs(t1 - t0)) < 1e-14, (t0, t1)
