@@ -15,9 +15,11 @@
 // take consecutive i, so every load and store is a coalesced run along x.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 
 #include "pth_burgers.hpp"
+#include "pth_device.hpp"
 
 namespace pth {
 
@@ -307,54 +309,68 @@ __global__ void __launch_bounds__(64) MinFinalKernel(const double *__restrict__ 
 // ---- fused stage path ----------------------------------------------------------------
 //
 // ReconTaskKernel: one thread = one WENO5Z (or linear) reconstruction of one cell in one
-// direction; blockIdx.y = block * V + comp. Per direction the task box is the interior extended
-// by one cell along that direction only, so every lane does the same amount of work.
-__global__ void __launch_bounds__(kWG)
-    ReconTaskKernel(const double *__restrict__ U, double *__restrict__ face, Geo G, int nblocks,
-                    int recon) {
-  const int n0 = G.hi0 - G.lo0 + 1, n1 = G.hi1 - G.lo1 + 1, n2 = G.hi2 - G.lo2 + 1;
-  const int per0 = (n0 + 2) * n1 * n2, per1 = n0 * (n1 + 2) * n2, per2 = n0 * n1 * (n2 + 2);
-  const int ntask = per0 + (G.ndim > 1 ? per1 : 0) + (G.ndim > 2 ? per2 : 0);
-  const int t = blockIdx.x * kWG + threadIdx.x;
-  if (t >= ntask) return;
-  const std::int64_t bc = blockIdx.y;  // block * V + comp
-  const std::int64_t slab = G.cells * G.V * nblocks;
-  const double *q = U + bc * G.cells;
-  double *f = face + bc * G.cells;
-  const int sj = G.e0, sk = G.e0 * G.e1;
-  int i, j, k, d, st;
-  if (t < per0) {
-    d = 0;
-    st = 1;
-    i = t % (n0 + 2) + G.lo0 - 1;
-    const int r = t / (n0 + 2);
-    j = r % n1 + G.lo1;
-    k = r / n1 + G.lo2;
-  } else if (t < per0 + per1) {
-    d = 1;
-    st = sj;
-    const int u = t - per0;
-    i = u % n0 + G.lo0;
-    const int r = u / n0;
-    j = r % (n1 + 2) + G.lo1 - 1;
-    k = r / (n1 + 2) + G.lo2;
-  } else {
-    d = 2;
-    st = sk;
-    const int u = t - per0 - per1;
-    i = u % n0 + G.lo0;
-    const int r = u / n0;
-    j = r % n1 + G.lo1;
-    k = r / n1 + G.lo2 - 1;
-  }
-  const int c = (k * G.e1 + j) * G.e0 + i;
+// direction over one brick (block, comp). Per direction the task box
+// is the interior extended by one cell along that direction only, so every lane does the same
+// amount of work. The index work around the FP64
+// reconstruction is kept small: the brick shape is a compile-time constant for the common
+// cubic blocks (N interior cells, NG ghosts; N == 0 reads the shape at run time), the direction
+// is uniform per workgroup, and loads and stores use 32-bit byte offsets from per-workgroup
+// base pointers.
+__device__ __forceinline__ double LoadAt(const double *base, unsigned byte_off) {
+  return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+__device__ __forceinline__ void StoreAt(double *base, unsigned byte_off, double v) {
+  *reinterpret_cast<double *>(reinterpret_cast<char *>(base) + byte_off) = v;
+}
+
+template <int N, int NG, int D>
+__device__ __forceinline__ void ReconTask(const double *__restrict__ q, double *__restrict__ fl,
+                                          double *__restrict__ fr, const Geo &G, int t,
+                                          int recon) {
+  const int n0 = N ? N : G.hi0 - G.lo0 + 1, n1 = N ? N : G.hi1 - G.lo1 + 1,
+            n2 = N ? N : G.hi2 - G.lo2 + 1;
+  const int lo0 = N ? NG : G.lo0, lo1 = N ? NG : G.lo1, lo2 = N ? NG : G.lo2;
+  const int e0 = N ? N + 2 * NG : G.e0, e1 = N ? N + 2 * NG : G.e1;
+  const int m0 = n0 + (D == 0 ? 2 : 0), m1 = n1 + (D == 1 ? 2 : 0), m2 = n2 + (D == 2 ? 2 : 0);
+  if (t >= m0 * m1 * m2) return;
+  const int i = t % m0 + lo0 - (D == 0 ? 1 : 0);
+  const int r = t / m0;
+  const int j = r % m1 + lo1 - (D == 1 ? 1 : 0);
+  const int k = r / m1 + lo2 - (D == 2 ? 1 : 0);
+  const unsigned c = 8u * static_cast<unsigned>((k * e1 + j) * e0 + i);
+  const unsigned st = 8u * static_cast<unsigned>(D == 0 ? 1 : D == 1 ? e0 : e0 * e1);
   double ql, qr;
   if (recon == 0)
-    Weno5Z(q[c - 2 * st], q[c - st], q[c], q[c + st], q[c + 2 * st], ql, qr);
+    Weno5Z(LoadAt(q, c - 2 * st), LoadAt(q, c - st), LoadAt(q, c), LoadAt(q, c + st),
+           LoadAt(q, c + 2 * st), ql, qr);
   else
-    LinearRecon(q[c - st], q[c], q[c + st], ql, qr);
-  f[(2 * d) * slab + c + st] = ql;
-  f[(2 * d + 1) * slab + c] = qr;
+    LinearRecon(LoadAt(q, c - st), LoadAt(q, c), LoadAt(q, c + st), ql, qr);
+  StoreAt(fl, c + st, ql);
+  StoreAt(fr, c, qr);
+}
+
+template <int N, int NG>
+__global__ void __launch_bounds__(kWG)
+    ReconTaskKernel(const double *__restrict__ U, double *__restrict__ face, Geo G, int nblocks,
+                    int recon, unsigned per_brick) {
+  // logical workgroup = (block * V + comp) * per_brick + tile * ndim + direction; consecutive
+  // logical workgroups take the directions of one tile and run on the same XCD, so the three
+  // passes over the same cells of U meet in one L2
+  const unsigned L = XcdRemap(blockIdx.x, gridDim.x);
+  const std::int64_t bc = L / per_brick;
+  const unsigned r = L % per_brick;
+  const std::int64_t slab = G.cells * G.V * nblocks;
+  const double *q = U + bc * G.cells;
+  const int d = r % G.ndim;
+  double *fl = face + 2 * d * slab + bc * G.cells;
+  double *fr = fl + slab;
+  const int t = (r / G.ndim) * kWG + threadIdx.x;
+  if (d == 0)
+    ReconTask<N, NG, 0>(q, fl, fr, G, t, recon);
+  else if (d == 1)
+    ReconTask<N, NG, 1>(q, fl, fr, G, t, recon);
+  else
+    ReconTask<N, NG, 2>(q, fl, fr, G, t, recon);
 }
 
 // HLL flux of every component at one face from the face states (burgers_package.cpp:307-401
@@ -397,8 +413,10 @@ __global__ void __launch_bounds__(kWG)
                       const int *__restrict__ fover, const double *__restrict__ fcor) {
   __shared__ double wmin[kWG / 64];
   const int n0 = G.hi0 - G.lo0 + 1, n1 = G.hi1 - G.lo1 + 1, n2 = G.hi2 - G.lo2 + 1;
-  const int b = blockIdx.y;
-  const int t = blockIdx.x * kWG + threadIdx.x;
+  const unsigned L = XcdRemap(blockIdx.x, gridDim.x);  // = block * tiles + tile
+  const int tiles = (n0 * n1 * n2 + kWG - 1) / kWG;
+  const int b = static_cast<int>(L / tiles);
+  const int t = static_cast<int>(L % tiles) * kWG + threadIdx.x;
   double m = DBL_MAX;
   if (t < n0 * n1 * n2) {
     const int i = t % n0 + G.lo0;
@@ -476,7 +494,7 @@ __global__ void __launch_bounds__(kWG)
     double v = wmin[0];
 #pragma unroll
     for (int w = 1; w < kWG / 64; ++w) v = DMin(v, wmin[w]);
-    partial[blockIdx.y * gridDim.x + blockIdx.x] = v;
+    partial[L] = v;
   }
 }
 
@@ -615,13 +633,20 @@ __global__ void __launch_bounds__(64)
 
 // Final CFL minimum and EvolutionDriver::SetGlobalTimeStep on device (driver.cpp:210-235):
 // dtvar = {dt, time, estimate}; time += dt (when advance), dt = min(2 dt, cfl * min).
-__global__ void __launch_bounds__(64)
+// One workgroup of kClockWG threads: strided partial minima, wavefront shuffles, then LDS.
+constexpr int kClockWG = 1024;
+__global__ void __launch_bounds__(kClockWG)
     ClockKernel(const double *__restrict__ partial, int n, double cfl, double *dtvar,
                 int advance) {
+  __shared__ double wmin[kClockWG / 64];
   double m = DBL_MAX;
-  for (int t = threadIdx.x; t < n; t += 64) m = DMin(m, partial[t]);
+  for (int t = threadIdx.x; t < n; t += kClockWG) m = DMin(m, partial[t]);
   m = WaveMin(m);
+  if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = m;
+  __syncthreads();
   if (threadIdx.x == 0) {
+    m = wmin[0];
+    for (int w = 1; w < kClockWG / 64; ++w) m = DMin(m, wmin[w]);
     const double est = cfl * m;
     double dt = dtvar[0];
     if (advance) dtvar[1] = dtvar[1] + dt;
@@ -721,10 +746,26 @@ pth_status BurgersReconTasks(const Topology &t, int V, const double *U, double *
   const int nb = static_cast<int>(t.blocks.size());
   const int n0 = t.fine.n[0], n1 = t.fine.n[1], n2 = t.fine.n[2];
   int ntask = (n0 + 2) * n1 * n2;
-  if (G.ndim > 1) ntask += n0 * (n1 + 2) * n2;
-  if (G.ndim > 2) ntask += n0 * n1 * (n2 + 2);
-  hipLaunchKernelGGL(ReconTaskKernel, dim3((ntask + kWG - 1) / kWG, nb * V), dim3(kWG), 0, s, U,
-                     face, G, nb, recon);
+  if (G.ndim > 1) ntask = std::max(ntask, n0 * (n1 + 2) * n2);
+  if (G.ndim > 2) ntask = std::max(ntask, n0 * n1 * (n2 + 2));
+  const unsigned per_brick = static_cast<unsigned>(((ntask + kWG - 1) / kWG) * G.ndim);
+  const dim3 grid(per_brick * nb * V);
+  const bool cubic = G.ndim == 3 && n0 == n1 && n1 == n2 && t.fine.g[0] == t.fine.g[1] &&
+                     t.fine.g[1] == t.fine.g[2];
+  const int g = t.fine.g[0];
+  auto run = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, grid, dim3(kWG), 0, s, U, face, G, nb, recon, per_brick);
+  };
+  if (cubic && n0 == 16 && g == 4)
+    run(ReconTaskKernel<16, 4>);
+  else if (cubic && n0 == 16 && g == 3)
+    run(ReconTaskKernel<16, 3>);
+  else if (cubic && n0 == 32 && g == 4)
+    run(ReconTaskKernel<32, 4>);
+  else if (cubic && n0 == 8 && g == 4)
+    run(ReconTaskKernel<8, 4>);
+  else
+    run(ReconTaskKernel<0, 0>);
   PTH_HIP(hipGetLastError());
   return PTH_COMPLETE;
 }
@@ -750,7 +791,7 @@ pth_status BurgersFusedUpdate(const Topology &t, int V, const double *face, cons
   const Geo G = MakeGeo(t, V);
   const int nb = static_cast<int>(t.blocks.size());
   const int cells = t.fine.n[0] * t.fine.n[1] * t.fine.n[2];
-  const dim3 grid((cells + kWG - 1) / kWG, nb);
+  const dim3 grid(((cells + kWG - 1) / kWG) * nb);
   const bool over = fc && !fc->host.empty();
   const int *fover = over ? fc->dev_face : nullptr;
   const double *fcor = over ? fc->store : nullptr;
@@ -843,7 +884,8 @@ pth_status FlxcorSet(const Topology &t, const FluxCorrection &fc, double *flux, 
 pth_status BurgersClock(const double *partial, int n, double cfl, double *dtvar, int advance,
                         hipStream_t s) {
   ProfScope prof("clock", s);
-  hipLaunchKernelGGL(ClockKernel, dim3(1), dim3(64), 0, s, partial, n, cfl, dtvar, advance);
+  hipLaunchKernelGGL(ClockKernel, dim3(1), dim3(kClockWG), 0, s, partial, n, cfl, dtvar,
+                     advance);
   PTH_HIP(hipGetLastError());
   return PTH_COMPLETE;
 }
