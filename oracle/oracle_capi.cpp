@@ -7,6 +7,7 @@
 #include <memory>
 
 #include "orc_burgers.hpp"
+#include "orc_gmg.hpp"
 
 using namespace orc;
 
@@ -386,6 +387,97 @@ void orc_burgers_totals(void *h, double *out) {
 
 double orc_burgers_mass_history(void *h, const double *xmn, const double *xmx, double vol) {
   return static_cast<OBurgers *>(h)->b.MassHistory(xmn, xmx, vol);
+}
+
+// ---- geometric multigrid Poisson solver (orc_gmg.hpp) ----
+struct OGmg {
+  orc::gmg::Solver s;
+  std::vector<double> u, b;
+  explicit OGmg(int N) : s(N), u(s.lv[0].Size(), 0.0), b(s.lv[0].Size(), 0.0) {}
+  // interior [k][j][i] of a finest-level array
+  void Get(const std::vector<double> &a, double *out) const {
+    const auto &L = s.lv[0];
+    std::int64_t q = 0;
+    for (int k = 0; k < L.n; ++k)
+      for (int j = 0; j < L.n; ++j)
+        for (int i = 0; i < L.n; ++i) out[q++] = a[L.Idx(k, j, i)];
+  }
+  void Set(std::vector<double> &a, const double *in) {
+    const auto &L = s.lv[0];
+    std::int64_t q = 0;
+    for (int k = 0; k < L.n; ++k)
+      for (int j = 0; j < L.n; ++j)
+        for (int i = 0; i < L.n; ++i) a[L.Idx(k, j, i)] = in[q++];
+  }
+};
+
+void *orc_gmg_new(int N) { return new OGmg(N); }
+void orc_gmg_free(void *h) { delete static_cast<OGmg *>(h); }
+int orc_gmg_levels(void *h) { return (int)static_cast<OGmg *>(h)->s.lv.size(); }
+void orc_gmg_set_params(void *h, double omega, int pre, int post) {
+  auto &s = static_cast<OGmg *>(h)->s;
+  s.omega = omega;
+  s.pre = pre;
+  s.post = post;
+}
+void orc_gmg_rhs_blobs(void *h, std::uint64_t seed) {
+  OGmg &g = *static_cast<OGmg *>(h);
+  g.s.SetRhsBlobs(seed);
+  g.b = g.s.lv[0].b;
+}
+void orc_gmg_set_rhs(void *h, const double *in) { static_cast<OGmg *>(h)->Set(static_cast<OGmg *>(h)->b, in); }
+void orc_gmg_get_rhs(void *h, double *out) { static_cast<OGmg *>(h)->Get(static_cast<OGmg *>(h)->b, out); }
+void orc_gmg_set_u(void *h, const double *in) { static_cast<OGmg *>(h)->Set(static_cast<OGmg *>(h)->u, in); }
+void orc_gmg_get_u(void *h, double *out) { static_cast<OGmg *>(h)->Get(static_cast<OGmg *>(h)->u, out); }
+// A x on the finest level (FixedFace 0 ghosts), interior arrays in and out
+void orc_gmg_apply(void *h, const double *x, double *out) {
+  OGmg &g = *static_cast<OGmg *>(h);
+  std::vector<double> xa(g.s.lv[0].Size(), 0.0), oa(g.s.lv[0].Size(), 0.0);
+  g.Set(xa, x);
+  g.s.Apply(0, xa, oa);
+  g.Get(oa, out);
+}
+int orc_gmg_solve(void *h, double tol, int maxit) {
+  OGmg &g = *static_cast<OGmg *>(h);
+  return g.s.Solve(g.u, g.b, tol, maxit);
+}
+int orc_gmg_history(void *h, double *out, int max) {
+  const auto &hist = static_cast<OGmg *>(h)->s.history;
+  const int n = (int)hist.size() < max ? (int)hist.size() : max;
+  for (int q = 0; q < n; ++q) out[q] = hist[q];
+  return (int)hist.size();
+}
+
+// transfer operators between finest-level l and l+1 on interior arrays (for their pins)
+void orc_gmg_restrict(void *h, int l, const double *fine, double *coarse) {
+  OGmg &g = *static_cast<OGmg *>(h);
+  auto &F = g.s.lv[l], &C = g.s.lv[l + 1];
+  std::int64_t q = 0;
+  for (int k = 0; k < F.n; ++k)
+    for (int j = 0; j < F.n; ++j)
+      for (int i = 0; i < F.n; ++i) F.r[F.Idx(k, j, i)] = fine[q++];
+  g.s.Restrict(l);
+  q = 0;
+  for (int k = 0; k < C.n; ++k)
+    for (int j = 0; j < C.n; ++j)
+      for (int i = 0; i < C.n; ++i) coarse[q++] = C.b[C.Idx(k, j, i)];
+}
+void orc_gmg_prolong_add(void *h, int l, const double *coarse, double *fine) {
+  OGmg &g = *static_cast<OGmg *>(h);
+  auto &F = g.s.lv[l], &C = g.s.lv[l + 1];
+  std::int64_t q = 0;
+  for (int k = 0; k < C.n; ++k)
+    for (int j = 0; j < C.n; ++j)
+      for (int i = 0; i < C.n; ++i) C.x[C.Idx(k, j, i)] = coarse[q++];
+  q = 0;
+  for (int k = 0; k < F.n; ++k)
+    for (int j = 0; j < F.n; ++j)
+      for (int i = 0; i < F.n; ++i) F.x[F.Idx(k, j, i)] = fine[q++];
+  g.s.ProlongAdd(l);
+  q = 0;
+  for (int k = 0; k < F.n; ++k)
+    for (int j = 0; j < F.n; ++j)
+      for (int i = 0; i < F.n; ++i) fine[q++] = F.x[F.Idx(k, j, i)];
 }
 
 }  // extern "C"

@@ -63,6 +63,20 @@ def lib():
             "orc_burgers_mass_history": (D, [P, DP, DP, D]),
             "orc_burgers_set_flux_correction": (None, [P, I]),
             "orc_burgers_totals": (None, [P, DP]),
+            "orc_gmg_new": (P, [I]),
+            "orc_gmg_free": (None, [P]),
+            "orc_gmg_levels": (I, [P]),
+            "orc_gmg_set_params": (None, [P, D, I, I]),
+            "orc_gmg_rhs_blobs": (None, [P, C.c_uint64]),
+            "orc_gmg_set_rhs": (None, [P, DP]),
+            "orc_gmg_get_rhs": (None, [P, DP]),
+            "orc_gmg_set_u": (None, [P, DP]),
+            "orc_gmg_get_u": (None, [P, DP]),
+            "orc_gmg_apply": (None, [P, DP, DP]),
+            "orc_gmg_solve": (I, [P, D, I]),
+            "orc_gmg_history": (I, [P, DP, I]),
+            "orc_gmg_restrict": (None, [P, I, DP, DP]),
+            "orc_gmg_prolong_add": (None, [P, I, DP, DP]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -184,3 +198,62 @@ class OracleBurgers:
         out = np.empty(self.V)
         lib().orc_burgers_totals(self.h, dptr(out))
         return out
+
+
+class OracleGmg:
+    """FP64 geometric-multigrid Poisson solver on the uniform N^3 cell-centred grid of [0,1]^3
+    (FixedFace 0 boundaries); arrays are interior (N, N, N) in [k][j][i] order."""
+
+    def __init__(self, N, omega=2.0 / 3.0, pre=2, post=2):
+        self.N = N
+        self.h = lib().orc_gmg_new(N)
+        lib().orc_gmg_set_params(self.h, omega, pre, post)
+        self.levels = lib().orc_gmg_levels(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.orc_gmg_free(self.h)
+            self.h = None
+
+    def _arr(self, n=None):
+        n = self.N if n is None else n
+        return np.zeros((n, n, n))
+
+    def rhs_blobs(self, seed):
+        lib().orc_gmg_rhs_blobs(self.h, seed)
+        out = self._arr()
+        lib().orc_gmg_get_rhs(self.h, dptr(out))
+        return out
+
+    def set_rhs(self, b):
+        lib().orc_gmg_set_rhs(self.h, dptr(np.ascontiguousarray(b, dtype=np.float64)))
+
+    def set_u(self, u):
+        lib().orc_gmg_set_u(self.h, dptr(np.ascontiguousarray(u, dtype=np.float64)))
+
+    def u(self):
+        out = self._arr()
+        lib().orc_gmg_get_u(self.h, dptr(out))
+        return out
+
+    def apply(self, x):
+        out = self._arr()
+        lib().orc_gmg_apply(self.h, dptr(np.ascontiguousarray(x, dtype=np.float64)), dptr(out))
+        return out
+
+    def solve(self, tol=1e-10, maxit=100):
+        it = lib().orc_gmg_solve(self.h, tol, maxit)
+        hist = np.zeros(maxit + 2)
+        n = lib().orc_gmg_history(self.h, dptr(hist), len(hist))
+        return it, hist[:n]
+
+    def restrict(self, level, fine):
+        n = fine.shape[0] // 2
+        out = self._arr(n)
+        lib().orc_gmg_restrict(self.h, level, dptr(np.ascontiguousarray(fine)), dptr(out))
+        return out
+
+    def prolong_add(self, level, coarse, fine):
+        f = np.ascontiguousarray(fine, dtype=np.float64).copy()
+        lib().orc_gmg_prolong_add(self.h, level, dptr(np.ascontiguousarray(coarse)), dptr(f))
+        return f
