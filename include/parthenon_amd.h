@@ -174,6 +174,51 @@ pth_status pth_burgers_synchronize(pth_mesh *m);
 pth_status pth_burgers_pgen_sinusoid(pth_meshdata *md, uint64_t seed);
 pth_status pth_burgers_pgen_vibe(pth_meshdata *md);
 
+/* ---- geometric-multigrid Poisson solver ----
+ * One solver object per N^3 cell-centred problem on [0,1]^3 with FixedFace-0 boundaries:
+ * the reference's MGSolver<u, rhs, PoissonEquation> (src/solvers/mg_solver.hpp:37-521,
+ * example/poisson_gmg/poisson_equation.hpp:41-205, poisson_package.cpp:60-130) on the uniform
+ * GMG hierarchy of src/mesh/mesh-gmg.cpp:103-154 (levels halve while the count is even).
+ * precision 0: V-cycles in FP32 inside an FP64 iterative-refinement loop (residual and update
+ * in FP64); precision 1: the same solver in FP64. omega, pre and post: weighted-Jacobi weight
+ * and sweeps (the reference's MGParams smoother, mg_solver.hpp:312-345, fixes SRJ weights).
+ * Arrays are the N^3 interior cells in [k][j][i] order (host memory). */
+typedef struct pth_gmg pth_gmg;
+pth_gmg *pth_gmg_create(int n, int precision, double omega, int pre, int post);
+void pth_gmg_destroy(pth_gmg *g);
+int pth_gmg_levels(const pth_gmg *g);
+/* RHS: 16 Gaussian blobs s exp(-|x-c|^2/w^2), c ~ U(0.1,0.9)^3, w ~ U(0.02,0.08), s = +-1,
+ * from SplitMix64(seed) in blob order (cx, cy, cz, w, s); resets the solution to 0. */
+pth_status pth_gmg_set_rhs_blobs(pth_gmg *g, uint64_t seed);
+pth_status pth_gmg_set_rhs(pth_gmg *g, const double *host, size_t n);
+pth_status pth_gmg_get_rhs(pth_gmg *g, double *host, size_t n);
+pth_status pth_gmg_get_solution(pth_gmg *g, double *host, size_t n);
+pth_status pth_gmg_reset(pth_gmg *g);
+/* MGSolver::AddTasks iteration (mg_solver.hpp:119-161): V-cycles until the relative L2
+ * residual ||b - A u|| / ||b|| < tol or max_iters cycles; iters and final_rel reported. */
+pth_status pth_gmg_solve(pth_gmg *g, double tol, int max_iters, int *iters, double *final_rel);
+/* relative residual before every V-cycle and after the last; returns the count */
+int pth_gmg_history(const pth_gmg *g, double *out, int max);
+
+/* Serving loop of independent Poisson solves (no counterpart in the reference, which holds
+ * one Mesh per process, src/parthenon_manager.cpp:170-173). Requests arrive at given times;
+ * they are admitted while the device-memory budget and the active-slot limit allow; every
+ * outer iteration runs the same V-cycle stage of all active solves as one launch. */
+typedef struct pth_gmg_server pth_gmg_server;
+pth_gmg_server *pth_gmg_server_create(int precision, double omega, int pre, int post, double tol,
+                                      int max_iters, int64_t budget_bytes, int max_active,
+                                      int keep_solutions);
+void pth_gmg_server_destroy(pth_gmg_server *s);
+pth_status pth_gmg_server_submit(pth_gmg_server *s, int n, uint64_t seed, double arrival_s,
+                                 int *id);
+/* serve every submitted request; realtime: honour arrival times on the wall clock */
+pth_status pth_gmg_server_run(pth_gmg_server *s, int realtime);
+/* out: arrival, admission, completion (s, server clock), latency (s), final relative residual */
+pth_status pth_gmg_server_result(const pth_gmg_server *s, int id, double out[5], int *iters);
+pth_status pth_gmg_server_solution(const pth_gmg_server *s, int id, double *host, size_t n);
+/* out: outer iterations run, largest batch, device bytes held; busy_s: time in iterations */
+pth_status pth_gmg_server_stats(const pth_gmg_server *s, int64_t out[3], double *busy_s);
+
 /* Instrumentation (the reference's PARTHENON_INSTRUMENT regions, src/utils/instrument.hpp):
  * when enabled, every kernel launch is bracketed by HIP events on its stream.
  * pth_profile_report waits for the recorded events, writes one line per kernel name,

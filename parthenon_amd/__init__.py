@@ -102,6 +102,23 @@ _SIGNATURES = {
     "pth_burgers_run_cycles": (_I, [_P, _I, _I]),
     "pth_burgers_clock": (_I, [_P, _DP, _DP]),
     "pth_burgers_synchronize": (_I, [_P]),
+    "pth_gmg_create": (_P, [_I, _I, _D, _I, _I]),
+    "pth_gmg_destroy": (None, [_P]),
+    "pth_gmg_levels": (_I, [_P]),
+    "pth_gmg_set_rhs_blobs": (_I, [_P, C.c_uint64]),
+    "pth_gmg_set_rhs": (_I, [_P, _DP, C.c_size_t]),
+    "pth_gmg_get_rhs": (_I, [_P, _DP, C.c_size_t]),
+    "pth_gmg_get_solution": (_I, [_P, _DP, C.c_size_t]),
+    "pth_gmg_reset": (_I, [_P]),
+    "pth_gmg_solve": (_I, [_P, _D, _I, C.POINTER(_I), _DP]),
+    "pth_gmg_history": (_I, [_P, _DP, _I]),
+    "pth_gmg_server_create": (_P, [_I, _D, _I, _I, _D, _I, _I64, _I, _I]),
+    "pth_gmg_server_destroy": (None, [_P]),
+    "pth_gmg_server_submit": (_I, [_P, _I, C.c_uint64, _D, C.POINTER(_I)]),
+    "pth_gmg_server_run": (_I, [_P, _I]),
+    "pth_gmg_server_result": (_I, [_P, _I, _DP, C.POINTER(_I)]),
+    "pth_gmg_server_solution": (_I, [_P, _I, _DP, C.c_size_t]),
+    "pth_gmg_server_stats": (_I, [_P, C.POINTER(_I64), _DP]),
     "pth_profile_enable": (_I, [_I]),
     "pth_profile_report": (_I, [C.c_char_p, _I]),
     "pth_last_error": (C.c_char_p, []),

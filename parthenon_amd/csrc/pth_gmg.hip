@@ -1,0 +1,775 @@
+// Geometric-multigrid Poisson solver on the GPU (the reference's poisson_gmg MGSolver,
+// src/solvers/mg_solver.hpp:37-521, with the PoissonEquation operator of
+// example/poisson_gmg/poisson_equation.hpp:41-205), for uniform N^3 cell-centred grids on
+// [0, 1]^3 with FixedFace-0 boundaries (boundary_conditions_generic.hpp:174-255).
+//
+// Mixed precision by iterative refinement:
+//   outer (FP64):  r = b - A u  (with u <- u + e fused in),  ||r|| / ||b|| stop test
+//   inner (T):     one V-cycle on A e = r from e = 0: weighted Jacobi pre-smoothing,
+//                  residual + average restriction, coarser levels, linear prolongation of the
+//                  correction, post-smoothing; the coarsest level only smooths
+// T = float is the mixed solver, T = double the plain FP64 solver: same kernels.
+//
+// Levels are single arrays of n^3 cells (the blocks of the reference's GMG grids are a
+// distribution artefact: on one GPU every level fits whole), so no level needs a ghost
+// exchange; the Dirichlet ghost (2 * 0 - mirror) is formed on the fly at the domain faces.
+// Levels of at most kBottom cells per side run inside one workgroup in LDS.
+//
+// Every kernel is batched: its grid covers a table of entries, one per (problem, level), so
+// the same V-cycle stage of several concurrent solves is one launch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "pth_gmg.hpp"
+
+namespace pth {
+namespace {
+
+constexpr int kWG = 256;
+
+template <typename T>
+__device__ __forceinline__ T Ratio(T a, T b) {  // robust::ratio (utils/robust.hpp:59-62)
+  const T sgn = b >= T(0) ? T(1) : T(-1);
+  const T small = T(10) * (sizeof(T) == 8 ? T(DBL_MIN) : T(FLT_MIN));
+  return a / (b + sgn * small);
+}
+
+// 7-point operator at one cell in the reference's flux form (D = 1, alpha = 0); neighbours
+// across a domain face are the FixedFace-0 ghost -u.
+template <typename T>
+__device__ __forceinline__ T ApplyA(T u, T xm, T xp, T ym, T yp, T zm, T zp, T dx) {
+  const T D = T(1), alpha = T(0);
+  T out = -alpha * u;
+  const T f1l = D / dx * (xm - u);
+  const T f1r = D / dx * (u - xp);
+  out += (f1l - f1r) / dx;
+  const T f2l = D * (ym - u) / dx;
+  const T f2r = D * (u - yp) / dx;
+  out += (f2l - f2r) / dx;
+  const T f3l = D * (zm - u) / dx;
+  const T f3r = D * (u - zp) / dx;
+  out += (f3l - f3r) / dx;
+  return out;
+}
+
+template <typename T>
+__device__ __forceinline__ T Diagonal(T dx) {  // SetDiagonal (poisson_equation.hpp:86-125)
+  const T D = T(1), alpha = T(0);
+  T d = -(D + D) / (dx * dx) - alpha;
+  d -= (D + D) / (dx * dx);
+  d -= (D + D) / (dx * dx);
+  return d;
+}
+
+struct Cell {
+  int i, j, k;
+};
+__device__ __forceinline__ Cell Decode(int c, int n) {
+  Cell q;
+  q.i = c % n;
+  const int r = c / n;
+  q.j = r % n;
+  q.k = r / n;
+  return q;
+}
+
+// the 6 neighbours of cell c of an n^3 array with the Dirichlet ghost
+template <typename T>
+struct Nbrs {
+  T u, xm, xp, ym, yp, zm, zp;
+};
+template <typename T>
+__device__ __forceinline__ Nbrs<T> Load(const T *__restrict__ x, int c, const Cell &q, int n) {
+  Nbrs<T> v;
+  const int s1 = n, s2 = n * n;
+  v.u = x[c];
+  v.xm = q.i > 0 ? x[c - 1] : -v.u;
+  v.xp = q.i < n - 1 ? x[c + 1] : -v.u;
+  v.ym = q.j > 0 ? x[c - s1] : -v.u;
+  v.yp = q.j < n - 1 ? x[c + s1] : -v.u;
+  v.zm = q.k > 0 ? x[c - s2] : -v.u;
+  v.zp = q.k < n - 1 ? x[c + s2] : -v.u;
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T JacobiPoint(const Nbrs<T> &v, T b, T dx, T D, T w) {
+  const T off = ApplyA(v.u, v.xm, v.xp, v.ym, v.yp, v.zm, v.zp, dx) - D * v.u;
+  const T val = b - off;
+  return w * Ratio(val, D) + (T(1) - w) * v.u;
+}
+
+// entry lookup: entries sorted by wg_begin, the last one ends at gridDim.x
+template <typename E>
+__device__ __forceinline__ const E &Find(const E *__restrict__ e, int ne, int &local) {
+  int lo = 0;
+  for (int q = 1; q < ne; ++q)
+    if (e[q].wg_begin <= static_cast<int>(blockIdx.x)) lo = q;
+  local = blockIdx.x - e[lo].wg_begin;
+  return e[lo];
+}
+
+// ---- kernels --------------------------------------------------------------------------
+
+// FP64 outer step: u_new = u_old + e (e may be null), r = b - A u_new, r stored in T as the
+// inner right-hand side, per-workgroup partial sums of r^2.
+template <typename T>
+__global__ void __launch_bounds__(kWG) OuterKernel(const GmgEntry<T> *__restrict__ E, int ne) {
+  __shared__ double wsum[kWG / 64];
+  int local;
+  const GmgEntry<T> &e = Find(E, ne, local);
+  const int n = e.n, c = local * kWG + threadIdx.x;
+  double rr = 0.0;
+  if (c < n * n * n) {
+    const Cell q = Decode(c, n);
+    const double dx = 1.0 / n;
+    const double *uo = e.u_old;
+    Nbrs<double> v = Load(uo, c, q, n);
+    if (e.x) {  // add the correction at the 7 points (the ghost follows the updated value)
+      const Nbrs<T> d = Load(static_cast<const T *>(e.x), c, q, n);
+      v.u += static_cast<double>(d.u);
+      v.xm = q.i > 0 ? v.xm + static_cast<double>(d.xm) : -v.u;
+      v.xp = q.i < n - 1 ? v.xp + static_cast<double>(d.xp) : -v.u;
+      v.ym = q.j > 0 ? v.ym + static_cast<double>(d.ym) : -v.u;
+      v.yp = q.j < n - 1 ? v.yp + static_cast<double>(d.yp) : -v.u;
+      v.zm = q.k > 0 ? v.zm + static_cast<double>(d.zm) : -v.u;
+      v.zp = q.k < n - 1 ? v.zp + static_cast<double>(d.zp) : -v.u;
+      e.u_new[c] = v.u;
+    }
+    const double r = e.b64[c] - ApplyA(v.u, v.xm, v.xp, v.ym, v.yp, v.zm, v.zp, dx);
+    e.b[c] = static_cast<T>(r);
+    rr = r * r;
+  }
+  for (int off = 32; off > 0; off >>= 1) rr += __shfl_xor(rr, off, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = rr;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < kWG / 64; ++w) s += wsum[w];
+    e.partial[local] = s;
+  }
+}
+
+// per-entry sum of the partials of OuterKernel (or of the rhs norm): one workgroup per entry
+template <typename T>
+__global__ void __launch_bounds__(kWG)
+    NormKernel(const GmgEntry<T> *__restrict__ E, double *__restrict__ out) {
+  __shared__ double wsum[kWG / 64];
+  const GmgEntry<T> &e = E[blockIdx.x];
+  const int n = e.n, np = (n * n * n + kWG - 1) / kWG;
+  double s = 0.0;
+  for (int p = threadIdx.x; p < np; p += kWG) s += e.partial[p];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kWG / 64; ++w) t += wsum[w];
+    out[blockIdx.x] = t;
+  }
+}
+
+// partial sums of b^2 (the stop test's denominator)
+template <typename T>
+__global__ void __launch_bounds__(kWG) RhsNormKernel(const GmgEntry<T> *__restrict__ E, int ne) {
+  __shared__ double wsum[kWG / 64];
+  int local;
+  const GmgEntry<T> &e = Find(E, ne, local);
+  const int n = e.n, c = local * kWG + threadIdx.x;
+  double s = 0.0;
+  if (c < n * n * n) s = e.b64[c] * e.b64[c];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kWG / 64; ++w) t += wsum[w];
+    e.partial[local] = t;
+  }
+}
+
+// Pre-smoothing from x = 0: one sweep (PRE2 = false) or two sweeps fused, the first sweep's
+// values at the 7 points being formed from b where they are needed (the same operations as
+// two separate sweeps, so the same bits).
+template <typename T, bool PRE2>
+__global__ void __launch_bounds__(kWG)
+    PreZeroKernel(const GmgEntry<T> *__restrict__ E, int ne, T w) {
+  int local;
+  const GmgEntry<T> &e = Find(E, ne, local);
+  const int n = e.n, c = local * kWG + threadIdx.x;
+  if (c >= n * n * n) return;
+  const Cell q = Decode(c, n);
+  const T dx = T(1) / T(n), D = Diagonal(dx);
+  const Nbrs<T> zero = {T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
+  const T *b = e.b;
+  if (!PRE2) {
+    e.x[c] = JacobiPoint(zero, b[c], dx, D, w);
+    return;
+  }
+  const Nbrs<T> bb = Load(b, c, q, n);
+  Nbrs<T> x1;
+  x1.u = JacobiPoint(zero, bb.u, dx, D, w);
+  x1.xm = q.i > 0 ? JacobiPoint(zero, bb.xm, dx, D, w) : -x1.u;
+  x1.xp = q.i < n - 1 ? JacobiPoint(zero, bb.xp, dx, D, w) : -x1.u;
+  x1.ym = q.j > 0 ? JacobiPoint(zero, bb.ym, dx, D, w) : -x1.u;
+  x1.yp = q.j < n - 1 ? JacobiPoint(zero, bb.yp, dx, D, w) : -x1.u;
+  x1.zm = q.k > 0 ? JacobiPoint(zero, bb.zm, dx, D, w) : -x1.u;
+  x1.zp = q.k < n - 1 ? JacobiPoint(zero, bb.zp, dx, D, w) : -x1.u;
+  e.x[c] = JacobiPoint(x1, bb.u, dx, D, w);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kWG) ZeroKernel(const GmgEntry<T> *__restrict__ E, int ne) {
+  int local;
+  const GmgEntry<T> &e = Find(E, ne, local);
+  const int c = local * kWG + threadIdx.x;
+  if (c < e.n * e.n * e.n) e.x[c] = T(0);
+}
+
+// one Jacobi sweep x -> t (the caller swaps x and t)
+template <typename T>
+__global__ void __launch_bounds__(kWG)
+    JacobiKernel(const GmgEntry<T> *__restrict__ E, int ne, T w) {
+  int local;
+  const GmgEntry<T> &e = Find(E, ne, local);
+  const int n = e.n, c = local * kWG + threadIdx.x;
+  if (c >= n * n * n) return;
+  const Cell q = Decode(c, n);
+  const T dx = T(1) / T(n), D = Diagonal(dx);
+  e.t[c] = JacobiPoint(Load(static_cast<const T *>(e.x), c, q, n), e.b[c], dx, D, w);
+}
+
+template <typename T>
+__device__ __forceinline__ T Restrict8(const T t[2][2][2], T vol) {
+  const T tv = ((vol + vol) + (vol + vol)) + ((vol + vol) + (vol + vol));
+  return (((t[0][0][0] + t[0][1][0]) + (t[0][0][1] + t[0][1][1])) +
+          ((t[1][0][0] + t[1][1][0]) + (t[1][0][1] + t[1][1][1]))) /
+         tv;
+}
+
+// residual of the 8 children of one coarse cell and their average (RestrictAverage) into the
+// coarser level's rhs; one thread per coarse cell
+template <typename T>
+__global__ void __launch_bounds__(kWG)
+    ResidualRestrictKernel(const GmgEntry<T> *__restrict__ E, int ne) {
+  int local;
+  const GmgEntry<T> &e = Find(E, ne, local);
+  const int nc = e.nc, n = e.n, cc = local * kWG + threadIdx.x;
+  if (cc >= nc * nc * nc) return;
+  const Cell p = Decode(cc, nc);
+  const T dx = T(1) / T(n), vol = dx * dx * dx;
+  T t[2][2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+      for (int c2 = 0; c2 < 2; ++c2) {
+        const Cell q = {2 * p.i + c2, 2 * p.j + bb, 2 * p.k + a};
+        const int c = (q.k * n + q.j) * n + q.i;
+        const Nbrs<T> v = Load(static_cast<const T *>(e.x), c, q, n);
+        const T r = e.b[c] - ApplyA(v.u, v.xm, v.xp, v.ym, v.yp, v.zm, v.zp, dx);
+        t[a][bb][c2] = vol * r;
+      }
+  e.bc[cc] = Restrict8(t, vol);
+}
+
+// x += ProlongateSharedLinear(coarse x); one thread per fine cell, coarse ghosts from the
+// Dirichlet condition (pr_ops.hpp:166-289 term grouping)
+template <typename T>
+__global__ void __launch_bounds__(kWG) ProlongKernel(const GmgEntry<T> *__restrict__ E, int ne) {
+  int local;
+  const GmgEntry<T> &e = Find(E, ne, local);
+  const int n = e.n, nc = e.nc, c = local * kWG + threadIdx.x;
+  if (c >= n * n * n) return;
+  const Cell q = Decode(c, n);
+  const Cell p = {q.i >> 1, q.j >> 1, q.k >> 1};
+  const int oi = q.i & 1, oj = q.j & 1, ok = q.k & 1;
+  const T *xc = e.xc;
+  const int cc = (p.k * nc + p.j) * nc + p.i;
+  const T fc = xc[cc];
+  const T hc = T(1) / T(nc), df = T(0.25) * hc;
+  T g1 = 0, g2 = 0, g3 = 0;  // the one-sided gradient on this child's side
+  if (oi == 0) {
+    const T fm = p.i > 0 ? xc[cc - 1] : -fc;
+    g1 = (fc - fm) / hc;
+  } else {
+    const T fp = p.i < nc - 1 ? xc[cc + 1] : -fc;
+    g1 = (fp - fc) / hc;
+  }
+  if (oj == 0) {
+    const T fm = p.j > 0 ? xc[cc - nc] : -fc;
+    g2 = (fc - fm) / hc;
+  } else {
+    const T fp = p.j < nc - 1 ? xc[cc + nc] : -fc;
+    g2 = (fp - fc) / hc;
+  }
+  if (ok == 0) {
+    const T fm = p.k > 0 ? xc[cc - nc * nc] : -fc;
+    g3 = (fc - fm) / hc;
+  } else {
+    const T fp = p.k < nc - 1 ? xc[cc + nc * nc] : -fc;
+    g3 = (fp - fc) / hc;
+  }
+  T v;
+  switch ((ok << 2) | (oj << 1) | oi) {
+    case 0: v = fc - (g1 * df + g2 * df + g3 * df); break;
+    case 1: v = fc + (g1 * df - g2 * df - g3 * df); break;
+    case 2: v = fc - (g1 * df - g2 * df + g3 * df); break;
+    case 3: v = fc + (g1 * df + g2 * df - g3 * df); break;
+    case 4: v = fc - (g1 * df + g2 * df - g3 * df); break;
+    case 5: v = fc + (g1 * df - g2 * df + g3 * df); break;
+    case 6: v = fc - (g1 * df - g2 * df - g3 * df); break;
+    default: v = fc + (g1 * df + g2 * df + g3 * df); break;
+  }
+  e.x[c] += v;
+}
+
+// ---- bottom of the V-cycle in one workgroup (levels of at most NB cells per side) --------
+template <typename T>
+__device__ void BSweep(T *x, T *t, const T *b, int n, T w, bool zero) {
+  const int N3 = n * n * n;
+  const T dx = T(1) / T(n), D = Diagonal(dx);
+  const Nbrs<T> zn = {T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
+  for (int c = threadIdx.x; c < N3; c += blockDim.x) {
+    const Cell q = Decode(c, n);
+    t[c] = JacobiPoint(zero ? zn : Load(static_cast<const T *>(x), c, q, n), b[c], dx, D, w);
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < N3; c += blockDim.x) x[c] = t[c];
+  __syncthreads();
+}
+
+template <typename T>
+__device__ void BRestrict(const T *x, const T *b, T *bc, T *xc, int n) {
+  const int nc = n / 2, N3 = nc * nc * nc;
+  const T dx = T(1) / T(n), vol = dx * dx * dx;
+  for (int cc = threadIdx.x; cc < N3; cc += blockDim.x) {
+    const Cell p = Decode(cc, nc);
+    T t[2][2][2];
+    for (int a = 0; a < 2; ++a)
+      for (int bb = 0; bb < 2; ++bb)
+        for (int c2 = 0; c2 < 2; ++c2) {
+          const Cell q = {2 * p.i + c2, 2 * p.j + bb, 2 * p.k + a};
+          const int c = (q.k * n + q.j) * n + q.i;
+          const Nbrs<T> v = Load(x, c, q, n);
+          t[a][bb][c2] = vol * (b[c] - ApplyA(v.u, v.xm, v.xp, v.ym, v.yp, v.zm, v.zp, dx));
+        }
+    bc[cc] = Restrict8(t, vol);
+    xc[cc] = T(0);
+  }
+  __syncthreads();
+}
+
+template <typename T>
+__device__ void BProlong(T *x, const T *xc, int n) {
+  const int nc = n / 2, N3 = n * n * n;
+  const T hc = T(1) / T(nc), df = T(0.25) * hc;
+  for (int c = threadIdx.x; c < N3; c += blockDim.x) {
+    const Cell q = Decode(c, n);
+    const Cell p = {q.i >> 1, q.j >> 1, q.k >> 1};
+    const int oi = q.i & 1, oj = q.j & 1, ok = q.k & 1;
+    const int cc = (p.k * nc + p.j) * nc + p.i;
+    const T fc = xc[cc];
+    const T g1 = oi == 0 ? (fc - (p.i > 0 ? xc[cc - 1] : -fc)) / hc
+                         : ((p.i < nc - 1 ? xc[cc + 1] : -fc) - fc) / hc;
+    const T g2 = oj == 0 ? (fc - (p.j > 0 ? xc[cc - nc] : -fc)) / hc
+                         : ((p.j < nc - 1 ? xc[cc + nc] : -fc) - fc) / hc;
+    const T g3 = ok == 0 ? (fc - (p.k > 0 ? xc[cc - nc * nc] : -fc)) / hc
+                         : ((p.k < nc - 1 ? xc[cc + nc * nc] : -fc) - fc) / hc;
+    T v;
+    switch ((ok << 2) | (oj << 1) | oi) {
+      case 0: v = fc - (g1 * df + g2 * df + g3 * df); break;
+      case 1: v = fc + (g1 * df - g2 * df - g3 * df); break;
+      case 2: v = fc - (g1 * df - g2 * df + g3 * df); break;
+      case 3: v = fc + (g1 * df + g2 * df - g3 * df); break;
+      case 4: v = fc - (g1 * df + g2 * df - g3 * df); break;
+      case 5: v = fc + (g1 * df - g2 * df + g3 * df); break;
+      case 6: v = fc - (g1 * df - g2 * df - g3 * df); break;
+      default: v = fc + (g1 * df + g2 * df + g3 * df); break;
+    }
+    x[c] += v;
+  }
+  __syncthreads();
+}
+
+// One workgroup per entry: the V-cycle from this entry's level (rhs e.b, result in e.x) down
+// to the coarsest level, every level in LDS.
+template <typename T, int NB>
+__global__ void __launch_bounds__(1024)
+    BottomKernel(const GmgEntry<T> *__restrict__ E, T w, int pre, int post) {
+  constexpr int kCap = NB * NB * NB + (NB / 2) * (NB / 2) * (NB / 2) * 8 / 7 + 8;
+  __shared__ T sx[kCap], st[kCap], sb[kCap];
+  const GmgEntry<T> &e = E[blockIdx.x];
+  int nl = 0, off[8], nn[8];
+  for (int n = e.n, o = 0; nl < 8; n /= 2) {
+    off[nl] = o;
+    nn[nl] = n;
+    ++nl;
+    o += n * n * n;
+    if (n % 2 != 0 || n == 1) break;
+  }
+  const int N3 = e.n * e.n * e.n;
+  for (int c = threadIdx.x; c < N3; c += blockDim.x) sb[c] = e.b[c];
+  __syncthreads();
+  for (int l = 0; l < nl; ++l) {
+    T *x = sx + off[l], *t = st + off[l], *b = sb + off[l];
+    for (int s = 0; s < pre; ++s) BSweep(x, t, b, nn[l], w, s == 0);
+    if (pre == 0) {
+      for (int c = threadIdx.x; c < nn[l] * nn[l] * nn[l]; c += blockDim.x) x[c] = T(0);
+      __syncthreads();
+    }
+    if (l + 1 == nl) break;
+    BRestrict(static_cast<const T *>(x), static_cast<const T *>(b), sb + off[l + 1],
+              sx + off[l + 1], nn[l]);
+  }
+  for (int l = nl - 2; l >= 0; --l) {
+    T *x = sx + off[l], *t = st + off[l], *b = sb + off[l];
+    BProlong(x, static_cast<const T *>(sx + off[l + 1]), nn[l]);
+    for (int s = 0; s < post; ++s) BSweep(x, t, b, nn[l], w, false);
+  }
+  for (int c = threadIdx.x; c < N3; c += blockDim.x) e.x[c] = sx[c];
+}
+
+// Right-hand side: 16 seeded Gaussian blobs at cell centres (FP64)
+struct BlobParams {
+  double v[80];  // per blob: cx, cy, cz, width, sign
+};
+__global__ void __launch_bounds__(kWG) BlobKernel(double *__restrict__ b, int n, BlobParams P) {
+  const double *blob = P.v;
+  const int c = blockIdx.x * kWG + threadIdx.x;
+  if (c >= n * n * n) return;
+  const Cell q = Decode(c, n);
+  const double h = 1.0 / n;
+  const double x = (q.i + 0.5) * h, y = (q.j + 0.5) * h, z = (q.k + 0.5) * h;
+  double v = 0.0;
+  for (int m = 0; m < 16; ++m) {
+    const double *p = blob + 5 * m;
+    const double dx = x - p[0], dy = y - p[1], dz = z - p[2];
+    v += p[4] * exp(-(dx * dx + dy * dy + dz * dz) / (p[3] * p[3]));
+  }
+  b[c] = v;
+}
+
+__global__ void __launch_bounds__(kWG) FillKernel(double *__restrict__ a, std::int64_t n) {
+  const std::int64_t c = static_cast<std::int64_t>(blockIdx.x) * kWG + threadIdx.x;
+  if (c < n) a[c] = 0.0;
+}
+
+int Blocks(std::int64_t cells) { return static_cast<int>((cells + kWG - 1) / kWG); }
+
+}  // namespace
+
+// ---- host side -------------------------------------------------------------------------
+
+void GmgBlobs(std::uint64_t seed, double out[80]) {
+  std::uint64_t st = seed;
+  auto next = [&]() {
+    std::uint64_t z = (st += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return static_cast<double>((z ^ (z >> 31)) >> 11) * 0x1.0p-53;
+  };
+  for (int m = 0; m < 16; ++m) {
+    for (int d = 0; d < 3; ++d) out[5 * m + d] = 0.1 + 0.8 * next();
+    out[5 * m + 3] = 0.02 + 0.06 * next();
+    out[5 * m + 4] = next() < 0.5 ? -1.0 : 1.0;
+  }
+}
+
+std::vector<int> GmgLevelSizes(int N) {
+  std::vector<int> s;
+  for (int n = N;; n /= 2) {
+    s.push_back(n);
+    if (n % 2 != 0 || n == 1) break;
+  }
+  return s;
+}
+
+template <typename T>
+std::int64_t GmgProblem<T>::Bytes(int N) {
+  std::int64_t b = 0;
+  for (int n : GmgLevelSizes(N)) b += 3 * static_cast<std::int64_t>(n) * n * n * sizeof(T);
+  const std::int64_t c = static_cast<std::int64_t>(N) * N * N;
+  return b + 3 * c * 8 + ((c + kWG - 1) / kWG) * 8;
+}
+
+template <typename T>
+pth_status GmgProblem<T>::Allocate(int N_, int bottom) {
+  N = N_;
+  sizes = GmgLevelSizes(N);
+  nbottom = 0;
+  while (nbottom < static_cast<int>(sizes.size()) && sizes[nbottom] > bottom) ++nbottom;
+  if (nbottom == static_cast<int>(sizes.size())) {
+    SetError("GMG: the coarsest level is larger than the in-LDS bottom solver");
+    return PTH_FAIL;
+  }
+  const std::int64_t c = static_cast<std::int64_t>(N) * N * N;
+  if (c >= (std::int64_t(1) << 31)) {
+    SetError("GMG: grid too large");
+    return PTH_FAIL;
+  }
+  std::int64_t total = 0;
+  for (int n : sizes) total += 3 * static_cast<std::int64_t>(n) * n * n;
+  PTH_HIP(hipMalloc(&levels, sizeof(T) * total));
+  PTH_HIP(hipMemset(levels, 0, sizeof(T) * total));
+  PTH_HIP(hipMalloc(&u[0], sizeof(double) * c));
+  PTH_HIP(hipMalloc(&u[1], sizeof(double) * c));
+  PTH_HIP(hipMalloc(&b64, sizeof(double) * c));
+  PTH_HIP(hipMalloc(&partial, sizeof(double) * ((c + kWG - 1) / kWG)));
+  PTH_HIP(hipMemset(u[0], 0, sizeof(double) * c));
+  PTH_HIP(hipMemset(u[1], 0, sizeof(double) * c));
+  PTH_HIP(hipMemset(b64, 0, sizeof(double) * c));
+  PTH_HIP(hipDeviceSynchronize());  // the clears above are ordered before any stream's work
+  lv.clear();
+  std::int64_t o = 0;
+  for (int n : sizes) {
+    const std::int64_t s = static_cast<std::int64_t>(n) * n * n;
+    lv.push_back({levels + o, levels + o + s, levels + o + 2 * s});
+    o += 3 * s;
+  }
+  cur = 0;
+  return PTH_COMPLETE;
+}
+
+template <typename T>
+void GmgProblem<T>::Free() {
+  if (levels) (void)hipFree(levels);
+  for (double *&p : u)
+    if (p) (void)hipFree(p);
+  if (b64) (void)hipFree(b64);
+  if (partial) (void)hipFree(partial);
+  levels = nullptr;
+  u[0] = u[1] = b64 = partial = nullptr;
+}
+
+template <typename T>
+GmgEntry<T> GmgProblem<T>::Entry(int l) const {
+  GmgEntry<T> e;
+  std::memset(&e, 0, sizeof(e));
+  e.x = lv[l].x;
+  e.t = lv[l].t;
+  e.b = lv[l].b;
+  if (l + 1 < static_cast<int>(lv.size())) {
+    e.xc = lv[l + 1].x;
+    e.bc = lv[l + 1].b;
+    e.nc = sizes[l + 1];
+  }
+  e.n = sizes[l];
+  e.u_old = u[cur];
+  e.u_new = u[1 - cur];
+  e.b64 = b64;
+  e.partial = partial;
+  return e;
+}
+
+template <typename T>
+pth_status GmgBatch<T>::Stage(const std::vector<GmgEntry<T>> &es, bool coarse_cells,
+                              GmgEntry<T> **dev, int *ne, int *nwg) {
+  // entries are written to a host staging ring and copied once per stage
+  std::vector<GmgEntry<T>> v = es;
+  int wg = 0;
+  for (auto &e : v) {
+    e.wg_begin = wg;
+    const std::int64_t n = coarse_cells ? e.nc : e.n;
+    wg += Blocks(n * n * n);
+  }
+  if (static_cast<int>(v.size()) > kStageCap) {
+    SetError("GMG: too many batched entries in one launch");
+    return PTH_FAIL;
+  }
+  if (stage_used + static_cast<int>(v.size()) > kStageCap) {
+    // the ring is full: the copies queued from it must have run before it is rewritten
+    PTH_HIP(hipStreamSynchronize(stream));
+    stage_used = 0;
+  }
+  GmgEntry<T> *h = stage_host + stage_used, *d = stage_dev + stage_used;
+  std::copy(v.begin(), v.end(), h);
+  PTH_HIP(hipMemcpyAsync(d, h, sizeof(GmgEntry<T>) * v.size(), hipMemcpyHostToDevice, stream));
+  stage_used += static_cast<int>(v.size());
+  *dev = d;
+  *ne = static_cast<int>(v.size());
+  *nwg = wg;
+  return PTH_COMPLETE;
+}
+
+template <typename T>
+pth_status GmgBatch<T>::Init(hipStream_t s) {
+  stream = s;
+  if (!stage_host) {
+    PTH_HIP(hipHostMalloc(&stage_host, sizeof(GmgEntry<T>) * kStageCap));
+    PTH_HIP(hipMalloc(&stage_dev, sizeof(GmgEntry<T>) * kStageCap));
+    PTH_HIP(hipHostMalloc(&norms_host, sizeof(double) * kMaxProblems));
+    PTH_HIP(hipMalloc(&norms_dev, sizeof(double) * kMaxProblems));
+  }
+  return PTH_COMPLETE;
+}
+
+template <typename T>
+void GmgBatch<T>::Free() {
+  if (stage_host) (void)hipHostFree(stage_host);
+  if (stage_dev) (void)hipFree(stage_dev);
+  if (norms_host) (void)hipHostFree(norms_host);
+  if (norms_dev) (void)hipFree(norms_dev);
+  stage_host = nullptr;
+  stage_dev = nullptr;
+  norms_host = nullptr;
+  norms_dev = nullptr;
+}
+
+template <typename T>
+pth_status GmgBatch<T>::RhsNorms(const std::vector<GmgProblem<T> *> &ps, double *out) {
+  std::vector<GmgEntry<T>> es;
+  for (auto *p : ps) es.push_back(p->Entry(0));
+  GmgEntry<T> *d;
+  int ne, nwg;
+  if (Stage(es, false, &d, &ne, &nwg) != PTH_COMPLETE) return PTH_FAIL;
+  hipLaunchKernelGGL(RhsNormKernel<T>, dim3(nwg), dim3(kWG), 0, stream, d, ne);
+  hipLaunchKernelGGL(NormKernel<T>, dim3(ne), dim3(kWG), 0, stream, d, norms_dev);
+  PTH_HIP(hipGetLastError());
+  PTH_HIP(hipMemcpyAsync(norms_host, norms_dev, sizeof(double) * ne, hipMemcpyDeviceToHost,
+                         stream));
+  PTH_HIP(hipStreamSynchronize(stream));
+  stage_used = 0;  // every queued copy has run
+  for (int q = 0; q < ne; ++q) out[q] = std::sqrt(norms_host[q]);
+  return PTH_COMPLETE;
+}
+
+// Outer FP64 step for every problem in ps: apply the pending correction (when correct[q]),
+// form r = b - A u as the inner right-hand side, and return ||r|| in out[] (synchronizes).
+template <typename T>
+pth_status GmgBatch<T>::Residual(const std::vector<GmgProblem<T> *> &ps,
+                                 const std::vector<char> &correct, double *out) {
+  std::vector<GmgEntry<T>> es;
+  for (size_t q = 0; q < ps.size(); ++q) {
+    GmgEntry<T> e = ps[q]->Entry(0);
+    if (!correct[q]) e.x = nullptr;
+    es.push_back(e);
+  }
+  GmgEntry<T> *d;
+  int ne, nwg;
+  if (Stage(es, false, &d, &ne, &nwg) != PTH_COMPLETE) return PTH_FAIL;
+  {
+    ProfScope prof("gmg_outer", stream);
+    hipLaunchKernelGGL(OuterKernel<T>, dim3(nwg), dim3(kWG), 0, stream, d, ne);
+  }
+  hipLaunchKernelGGL(NormKernel<T>, dim3(ne), dim3(kWG), 0, stream, d, norms_dev);
+  PTH_HIP(hipGetLastError());
+  PTH_HIP(hipMemcpyAsync(norms_host, norms_dev, sizeof(double) * ne, hipMemcpyDeviceToHost,
+                         stream));
+  PTH_HIP(hipStreamSynchronize(stream));
+  stage_used = 0;  // every queued copy has run
+  for (size_t q = 0; q < ps.size(); ++q) {
+    if (correct[q]) ps[q]->cur = 1 - ps[q]->cur;
+    out[q] = std::sqrt(norms_host[q]);
+  }
+  return PTH_COMPLETE;
+}
+
+// One V-cycle on A e = r from e = 0 for every problem in ps, level by level: the same stage of
+// every problem is one launch. The level arrays' x / t roles follow the Jacobi ping-pong.
+template <typename T>
+pth_status GmgBatch<T>::Cycle(const std::vector<GmgProblem<T> *> &ps, double omega, int pre,
+                              int post) {
+  const T w = static_cast<T>(omega);
+  int L = 0;
+  for (auto *p : ps) L = std::max(L, p->nbottom);
+  auto entries = [&](int l) {
+    std::vector<GmgEntry<T>> es;
+    for (auto *p : ps)
+      if (l < p->nbottom) es.push_back(p->Entry(l));
+    return es;
+  };
+  auto sweep = [&](int l) -> pth_status {  // x -> t, then t becomes x
+    GmgEntry<T> *d;
+    int ne, nwg;
+    if (Stage(entries(l), false, &d, &ne, &nwg) != PTH_COMPLETE) return PTH_FAIL;
+    ProfScope prof("gmg_jacobi", stream);
+    hipLaunchKernelGGL(JacobiKernel<T>, dim3(nwg), dim3(kWG), 0, stream, d, ne, w);
+    PTH_HIP(hipGetLastError());
+    for (auto *p : ps)
+      if (l < p->nbottom) std::swap(p->lv[l].x, p->lv[l].t);
+    return PTH_COMPLETE;
+  };
+  for (int l = 0; l < L; ++l) {
+    GmgEntry<T> *d;
+    int ne, nwg;
+    if (pre > 0) {
+      if (Stage(entries(l), false, &d, &ne, &nwg) != PTH_COMPLETE) return PTH_FAIL;
+      ProfScope prof("gmg_pre", stream);
+      if (pre >= 2)
+        hipLaunchKernelGGL((PreZeroKernel<T, true>), dim3(nwg), dim3(kWG), 0, stream, d, ne,
+                           w);
+      else
+        hipLaunchKernelGGL((PreZeroKernel<T, false>), dim3(nwg), dim3(kWG), 0, stream, d, ne,
+                           w);
+      PTH_HIP(hipGetLastError());
+    } else {
+      if (Stage(entries(l), false, &d, &ne, &nwg) != PTH_COMPLETE) return PTH_FAIL;
+      hipLaunchKernelGGL(ZeroKernel<T>, dim3(nwg), dim3(kWG), 0, stream, d, ne);
+      PTH_HIP(hipGetLastError());
+    }
+    for (int s = 2; s < pre; ++s)
+      if (sweep(l) != PTH_COMPLETE) return PTH_FAIL;
+    if (Stage(entries(l), true, &d, &ne, &nwg) != PTH_COMPLETE) return PTH_FAIL;
+    ProfScope prof("gmg_restrict", stream);
+    hipLaunchKernelGGL(ResidualRestrictKernel<T>, dim3(nwg), dim3(kWG), 0, stream, d, ne);
+    PTH_HIP(hipGetLastError());
+  }
+  {
+    std::vector<GmgEntry<T>> es;
+    for (auto *p : ps) es.push_back(p->Entry(p->nbottom));
+    GmgEntry<T> *d;
+    int ne, nwg;
+    if (Stage(es, false, &d, &ne, &nwg) != PTH_COMPLETE) return PTH_FAIL;
+    ProfScope prof("gmg_bottom", stream);
+    hipLaunchKernelGGL((BottomKernel<T, GmgBottom<T>::kN>), dim3(ne), dim3(1024), 0, stream, d,
+                       w, pre, post);
+    PTH_HIP(hipGetLastError());
+  }
+  for (int l = L - 1; l >= 0; --l) {
+    GmgEntry<T> *d;
+    int ne, nwg;
+    if (Stage(entries(l), false, &d, &ne, &nwg) != PTH_COMPLETE) return PTH_FAIL;
+    {
+      ProfScope prof("gmg_prolong", stream);
+      hipLaunchKernelGGL(ProlongKernel<T>, dim3(nwg), dim3(kWG), 0, stream, d, ne);
+      PTH_HIP(hipGetLastError());
+    }
+    for (int s = 0; s < post; ++s)
+      if (sweep(l) != PTH_COMPLETE) return PTH_FAIL;
+  }
+  return PTH_COMPLETE;
+}
+
+template <typename T>
+pth_status GmgBatch<T>::SetBlobs(GmgProblem<T> &p, std::uint64_t seed) {
+  BlobParams P;
+  GmgBlobs(seed, P.v);
+  hipLaunchKernelGGL(BlobKernel, dim3(Blocks(static_cast<std::int64_t>(p.N) * p.N * p.N)),
+                     dim3(kWG), 0, stream, p.b64, p.N, P);
+  PTH_HIP(hipGetLastError());
+  return PTH_COMPLETE;
+}
+
+// u = 0 on the finest level (the solve's initial guess)
+template <typename T>
+pth_status GmgBatch<T>::Reset(GmgProblem<T> &p) {
+  const std::int64_t c = static_cast<std::int64_t>(p.N) * p.N * p.N;
+  hipLaunchKernelGGL(FillKernel, dim3(Blocks(c)), dim3(kWG), 0, stream, p.u[0], c);
+  PTH_HIP(hipGetLastError());
+  p.cur = 0;
+  return PTH_COMPLETE;
+}
+
+template struct GmgProblem<float>;
+template struct GmgProblem<double>;
+template struct GmgBatch<float>;
+template struct GmgBatch<double>;
+
+}  // namespace pth

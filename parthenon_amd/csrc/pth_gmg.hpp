@@ -1,0 +1,78 @@
+// Host-side structures of the geometric-multigrid Poisson solver (pth_gmg.hip).
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "pth_internal.hpp"
+
+namespace pth {
+
+// One (problem, level) entry of a batched launch. Level arrays are n^3 cells, [k][j][i].
+template <typename T>
+struct GmgEntry {
+  T *x, *t, *b;        // this level: solution (correction), Jacobi scratch, right-hand side
+  T *xc, *bc;          // next coarser level's solution and right-hand side (or null)
+  double *u_old, *u_new;  // finest level FP64 solution, ping-pong (outer step only)
+  const double *b64;      // finest level FP64 right-hand side
+  double *partial;        // per-workgroup partial sums of the outer step
+  int n, nc;              // cells per side on this and the next coarser level
+  int wg_begin;           // first workgroup of this entry in the batched grid
+  int pad;
+};
+
+// largest level handled inside one workgroup in LDS (three n^3 arrays of T plus the coarser
+// levels below the 64 KiB static LDS limit)
+template <typename T>
+struct GmgBottom;
+template <>
+struct GmgBottom<float> {
+  static constexpr int kN = 16;
+};
+template <>
+struct GmgBottom<double> {
+  static constexpr int kN = 8;
+};
+
+template <typename T>
+struct GmgProblem {
+  struct Arrays {
+    T *x, *t, *b;
+  };
+  int N = 0;
+  std::vector<int> sizes;  // cells per side of every level, finest first
+  int nbottom = 0;         // first level solved in LDS
+  std::vector<Arrays> lv;
+  T *levels = nullptr;     // one allocation for all levels
+  double *u[2] = {nullptr, nullptr};
+  double *b64 = nullptr, *partial = nullptr;
+  int cur = 0;             // u[cur] is the current solution
+  static std::int64_t Bytes(int N);  // device bytes one problem holds
+  pth_status Allocate(int N, int bottom);
+  void Free();
+  GmgEntry<T> Entry(int l) const;
+};
+
+template <typename T>
+struct GmgBatch {
+  static constexpr int kStageCap = 8192;
+  static constexpr int kMaxProblems = 1024;
+  hipStream_t stream = nullptr;
+  GmgEntry<T> *stage_host = nullptr, *stage_dev = nullptr;
+  int stage_used = 0;
+  double *norms_host = nullptr, *norms_dev = nullptr;
+  pth_status Init(hipStream_t s);
+  void Free();
+  pth_status Stage(const std::vector<GmgEntry<T>> &es, bool coarse_cells, GmgEntry<T> **dev,
+                   int *ne, int *nwg);
+  pth_status RhsNorms(const std::vector<GmgProblem<T> *> &ps, double *out);
+  pth_status Residual(const std::vector<GmgProblem<T> *> &ps, const std::vector<char> &correct,
+                      double *out);
+  pth_status Cycle(const std::vector<GmgProblem<T> *> &ps, double omega, int pre, int post);
+  pth_status SetBlobs(GmgProblem<T> &p, std::uint64_t seed);
+  pth_status Reset(GmgProblem<T> &p);
+};
+
+void GmgBlobs(std::uint64_t seed, double out[80]);
+std::vector<int> GmgLevelSizes(int N);
+
+}  // namespace pth
