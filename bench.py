@@ -117,6 +117,176 @@ def copy_bench(args):
                       "higher_is_better": True, "dtype": "f64", "data": "synthetic"}))
 
 
+def gmg_cpu_baseline(N, seed, threads):
+    """Oracle FP64 GMG solve of the same problem in a child process (OpenMP loops)."""
+    code = f"""
+import sys, time, json
+import numpy as np
+sys.path.insert(0, {os.path.join(ROOT, 'tests')!r})
+from oracle_lib import OracleGmg
+g = OracleGmg({N})
+g.rhs_blobs({seed})
+t = time.perf_counter()
+it, hist = g.solve(tol=1e-10, maxit=100)
+dt = time.perf_counter() - t
+np.save(sys.argv[1], g.u())
+print(json.dumps({{"wall": dt, "iters": int(it)}}))
+"""
+    import tempfile
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="spread")
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "u.npy")
+        r = subprocess.run([sys.executable, "-c", code, path], env=env, stdout=subprocess.PIPE,
+                           stderr=subprocess.PIPE, text=True, timeout=900)
+        if r.returncode != 0:
+            raise RuntimeError("gmg cpu baseline failed: " + r.stderr[-2000:])
+        import numpy as np
+        return json.loads(r.stdout.strip().splitlines()[-1]), np.load(path)
+
+
+def gmg_bench(args):
+    """Mixed-precision Poisson GMG: 256^3 cells on [0,1]^3 (512 blocks of 32^3 in the
+    reference's layout), 7-point operator, FixedFace-0 boundaries, 16 seeded Gaussian blobs
+    (SplitMix64 seed 3), V-cycles of 2+2 weighted-Jacobi sweeps (omega 2/3) in FP32 inside
+    FP64 iterative refinement, to 1e-10 relative residual. One step = one solve from u = 0.
+    The same solves with the FP64 solver are timed beside it."""
+    import numpy as np
+    import parthenon_amd as pa
+    from parthenon_amd.gmg import MGSolver
+
+    N, seed = 256, 3
+    res = {}
+    sols = {}
+    for prec in ("mixed", "fp64"):
+        s = MGSolver(N, prec)
+        s.set_rhs_blobs(seed)
+        for _ in range(args.warmup):
+            s.reset()
+            s.solve()
+        steps = args.steps
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.reset()
+            its, rel = s.solve()
+        wall = time.perf_counter() - t0
+        # roofline pass: one more solve with HIP events around every launch
+        pa.profile_enable(True)
+        pa.profile_report()
+        s.reset()
+        its, rel = s.solve()
+        prof = pa.profile_report()
+        pa.profile_enable(False)
+        res[prec] = dict(wall=wall, steps=steps, iters=its, rel=rel, prof=prof)
+        sols[prec] = s.solution()
+        del s
+    m = res["mixed"]
+    c = N ** 3
+    es = 4  # FP32 inner
+    # algorithmic bytes per launch of the finest-level kernels
+    kb = {"gmg_outer": 32 * c, "gmg_pre_l0": 2 * es * c, "gmg_jacobi_l0": 3 * es * c,
+          "gmg_restrict_l0": 2 * es * c + es * c // 8,
+          "gmg_prolong_l0": 2 * es * c + es * c // 8}
+    prof = m["prof"]
+    dom = max((k for k in prof if k in kb), key=lambda k: prof[k][1])
+    n_l, ms = prof[dom]
+    avg_s = ms / n_l * 1e-3
+    achieved = kb[dom] / avg_s / 1e9
+    # per outer iteration: the finest level's traffic plus 1/7 of it for the coarser levels
+    it_bytes = int(32 * c + (2 * es + 2 * 3 * es + 2 * (2 * es + es / 8)) * c * 8 / 7)
+    per_it = m["wall"] / m["steps"] / max(m["iters"], 1)
+    out = {
+        "metric": "Poisson GMG solves/s (256^3, 1e-10 relative residual)",
+        "value": m["steps"] / m["wall"], "unit": "solves/s", "n_gpus": 1,
+        "steps": m["steps"], "warmup": args.warmup,
+        "ms_per_step": m["wall"] / m["steps"] * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 V-cycle / f64 refinement",
+        "data": f"synthetic (16 Gaussian blobs, SplitMix64 seed {seed})",
+        "config": {"workload": "poisson-gmg 256^3 on [0,1]^3, 7-point, FixedFace 0, "
+                               "omega 2/3 Jacobi 2+2, V-cycle to 1e-10 relative",
+                   "levels": 9, "iterations": m["iters"], "final_rel_residual": m["rel"]},
+        "fp64": {"solves_per_s": res["fp64"]["steps"] / res["fp64"]["wall"],
+                 "iterations": res["fp64"]["iters"],
+                 "final_rel_residual": res["fp64"]["rel"]},
+        "speedup_vs_fp64": (m["steps"] / m["wall"]) /
+                           (res["fp64"]["steps"] / res["fp64"]["wall"]),
+        "error_vs_fp64_solver": float(np.sqrt(np.mean((sols["mixed"] - sols["fp64"]) ** 2)) /
+                                      np.sqrt(np.mean(sols["fp64"] ** 2))),
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": kb[dom], "avg_launch_us": avg_s * 1e6},
+        "iteration_roofline": {"algorithmic_bytes_per_iteration": it_bytes,
+                               "ms_per_iteration": per_it * 1e3,
+                               "achieved_GBps": it_bytes / per_it / 1e9,
+                               "frac": it_bytes / per_it / 1e9 / HBM_PEAK_GBS},
+        "kernels": {k: {"launches_per_solve": v[0], "avg_us": v[1] / v[0] * 1e3}
+                    for k, v in sorted(prof.items())},
+    }
+    if not args.no_cpu_baseline:
+        threads = min(len(os.sched_getaffinity(0)), 16)
+        info, u_cpu = gmg_cpu_baseline(N, seed, threads)
+        out["error_vs_oracle"] = float(np.sqrt(np.mean((sols["mixed"] - u_cpu) ** 2)) /
+                                       np.sqrt(np.mean(u_cpu ** 2)))
+        out["cpu_baseline"] = {"value": 1.0 / info["wall"], "unit": "solves/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"1 FP64 solve of the same problem by the oracle "
+                                         f"({info['iters']} V-cycles, {info['wall']:.1f} s)"}
+    print(json.dumps(out))
+
+
+def gmg_serve_bench(args):
+    """Served Poisson solves: seeded Poisson-process arrivals at 50 requests/s, grid sizes
+    {32, 64, 128, 192}^3 with weights {0.4, 0.3, 0.2, 0.1}, block size 16^3 or 32^3 (50/50,
+    recorded; the solver does not depend on it), blob RHS seeded per request, mixed-precision
+    GMG to 1e-10 relative residual. One step = one request; latency = completion - arrival."""
+    import numpy as np
+    from parthenon_amd.gmg import Server
+
+    nreq = args.steps
+    rng = np.random.default_rng(4)
+    arrivals = np.cumsum(rng.exponential(1 / 50.0, nreq))
+    sizes = rng.choice([32, 64, 128, 192], size=nreq, p=[0.4, 0.3, 0.2, 0.1])
+    blocks = rng.choice([16, 32], size=nreq)
+    srv = Server("mixed", budget_bytes=64 << 30, max_active=64)
+    # warm the memory pool with the request sizes outside the timed stream
+    warm = Server("mixed", budget_bytes=64 << 30, max_active=8)
+    for n in (32, 64, 128, 192):
+        warm.submit(int(n), 1)
+    warm.run(realtime=False)
+    del warm
+    ids = [srv.submit(int(n), 1000 + q, float(t)) for q, (n, t) in enumerate(zip(sizes, arrivals))]
+    t0 = time.perf_counter()
+    srv.run(realtime=True)
+    wall = time.perf_counter() - t0
+    lat = np.array([srv.result(r)["latency"] for r in ids])
+    rel = np.array([srv.result(r)["rel_residual"] for r in ids])
+    st = srv.stats()
+    # capacity: the same requests all present at t = 0
+    cap = Server("mixed", budget_bytes=64 << 30, max_active=64)
+    for q, n in enumerate(sizes[:min(nreq, 400)]):
+        cap.submit(int(n), 1000 + q, 0.0)
+    tc = time.perf_counter()
+    cap.run(realtime=False)
+    cap_wall = time.perf_counter() - tc
+    out = {
+        "metric": "served Poisson solves/s and latency (mixed-precision GMG, 1e-10)",
+        "value": nreq / wall, "unit": "solves/s", "n_gpus": 1, "steps": nreq, "warmup": 0,
+        "ms_per_step": wall / nreq * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 V-cycle / f64 refinement",
+        "data": "synthetic (Poisson arrivals 50/s, seed 4; blob RHS per request)",
+        "config": {"workload": "gmg-serve: sizes {32,64,128,192}^3 p={.4,.3,.2,.1}, "
+                               "blocks 16^3/32^3, 50 req/s",
+                   "requests": nreq, "block_sizes_drawn": {"16": int(np.sum(blocks == 16)),
+                                                           "32": int(np.sum(blocks == 32))}},
+        "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                       "p99": float(np.percentile(lat, 99) * 1e3),
+                       "max": float(lat.max() * 1e3)},
+        "max_rel_residual": float(rel.max()),
+        "server": st,
+        "capacity_solves_per_s": min(nreq, 400) / cap_wall,
+    }
+    print(json.dumps(out))
+
+
 def exchange_bench(args):
     """Ghost-exchange microbenchmark: 128^3 mesh in 512 blocks of 16^3, nghost 3, 8 FP64
     one-component fields, U(-1,1) interiors; one step = SendBoundBufs + ReceiveBoundBufs +
@@ -172,7 +342,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload",
-                    choices=["burgers-amr", "burgers-uniform", "exchange", "copy"],
+                    choices=["burgers-amr", "burgers-uniform", "exchange", "copy", "gmg",
+                             "gmg-serve"],
                     default="burgers-amr")
     ap.add_argument("--pack", choices=["persistent", "per_buffer", "fused"], default="persistent")
     ap.add_argument("--steps", type=int, default=200)
@@ -189,6 +360,16 @@ def main():
         return exchange_bench(args)
     if args.workload == "copy":
         return copy_bench(args)
+    if args.workload == "gmg":
+        if "--steps" not in sys.argv:
+            args.steps = 10
+        if "--warmup" not in sys.argv:
+            args.warmup = 1
+        return gmg_bench(args)
+    if args.workload == "gmg-serve":
+        if "--steps" not in sys.argv:
+            args.steps = 2000
+        return gmg_serve_bench(args)
 
     import parthenon_amd as pa
     from parthenon_amd.burgers import BurgersDriver, DeviceBurgersDriver, make_burgers_mesh

@@ -498,6 +498,151 @@ __global__ void __launch_bounds__(kWG)
   }
 }
 
+// ---- plane-tiled reconstruction + HLL (cubic blocks of N <= 16 cells per side) ----------
+//
+// ReconFluxKernel: one workgroup = one plane of one block for one direction. The WENO5Z (or
+// linear) states of the plane's cells, extended by one cell along the direction, go to LDS;
+// then the HLL flux of every face of the plane goes to the flux array [3][block][comp][brick]
+// (x-face i at index i). The operations are those of ReconTaskKernel followed by HllFace, so
+// the fluxes are the same bits, and the face states never reach HBM: the flux array is half
+// their size and the update reads it without recomputing HLL on shared faces.
+template <int N, int NG, int V>
+__global__ void __launch_bounds__(kWG)
+    ReconFluxKernel(const double *__restrict__ U, double *__restrict__ flux, int nblocks,
+                    int recon) {
+  constexpr int E = N + 2 * NG;  // brick extent
+  constexpr int P = (N + 2) * N;  // cells of one plane, extended along the direction
+  constexpr int F = (N + 1) * N;  // faces of one plane
+  __shared__ double sl[V * P], sr[V * P];
+  // logical workgroup = (block * 3 + direction) * N + plane; one block's planes are adjacent
+  // and stay on one XCD
+  const unsigned L = XcdRemap(blockIdx.x, gridDim.x);
+  const int b = static_cast<int>(L / (3 * N)), r = static_cast<int>(L % (3 * N));
+  const int d = r / N, plane = r % N;
+  constexpr std::int64_t cs = static_cast<std::int64_t>(E) * E * E;
+  const std::int64_t slab = cs * V * nblocks;
+  const double *Ub = U + static_cast<std::int64_t>(b) * V * cs;
+  const int W = d == 0 ? N + 2 : N;  // plane row length (i fastest)
+  const unsigned st = 8u * static_cast<unsigned>(d == 0 ? 1 : d == 1 ? E : E * E);
+  for (int t = threadIdx.x; t < V * P; t += kWG) {
+    const int n = t / P, p = t % P;
+    const int a = p % W, row = p / W;
+    int i, j, k;
+    if (d == 0) {
+      i = NG - 1 + a;
+      j = NG + row;
+      k = NG + plane;
+    } else if (d == 1) {
+      i = NG + a;
+      j = NG - 1 + row;
+      k = NG + plane;
+    } else {
+      i = NG + a;
+      j = NG + plane;
+      k = NG - 1 + row;
+    }
+    const unsigned c = 8u * static_cast<unsigned>((k * E + j) * E + i);
+    const double *q = Ub + n * cs;
+    double ql, qr;
+    if (recon == 0)
+      Weno5Z(LoadAt(q, c - 2 * st), LoadAt(q, c - st), LoadAt(q, c), LoadAt(q, c + st),
+             LoadAt(q, c + 2 * st), ql, qr);
+    else
+      LinearRecon(LoadAt(q, c - st), LoadAt(q, c), LoadAt(q, c + st), ql, qr);
+    sl[n * P + p] = ql;
+    sr[n * P + p] = qr;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < F; t += kWG) {
+    int pl, pr, i, j, k;
+    if (d == 0) {  // face between extended cells a and a + 1 of row j
+      const int a = t % (N + 1), row = t / (N + 1);
+      pl = row * W + a;
+      pr = pl + 1;
+      i = NG + a;
+      j = NG + row;
+      k = NG + plane;
+    } else {  // face between extended rows row and row + 1
+      const int a = t % N, row = t / N;
+      pl = row * W + a;
+      pr = pl + W;
+      i = NG + a;
+      j = d == 1 ? NG + row : NG + plane;
+      k = d == 1 ? NG + plane : NG + row;
+    }
+    double Fv[V];
+    HllFace<V>(sl + pl, sr + pr, P, d, Fv);
+    double *fo = flux + d * slab + static_cast<std::int64_t>(b) * V * cs + (k * E + j) * E + i;
+#pragma unroll
+    for (int n = 0; n < V; ++n) fo[n * cs] = Fv[n];
+  }
+}
+
+// FluxUpdateKernel: FusedUpdateKernel reading stored face fluxes instead of recomputing HLL
+// (flux divergence, RK averaging and update, derived field, CFL partial); same operations.
+template <int V>
+__global__ void __launch_bounds__(kWG)
+    FluxUpdateKernel(const double *__restrict__ flux, const double *U0, const double *base,
+                     double *out, double *__restrict__ derived, double *__restrict__ partial,
+                     const BlockCoefs *__restrict__ coef, Geo G, int nblocks, double beta,
+                     const double *__restrict__ dtvar) {
+  __shared__ double wmin[kWG / 64];
+  const int n0 = G.hi0 - G.lo0 + 1, n1 = G.hi1 - G.lo1 + 1, n2 = G.hi2 - G.lo2 + 1;
+  const unsigned L = XcdRemap(blockIdx.x, gridDim.x);  // = block * tiles + tile
+  const int tiles = (n0 * n1 * n2 + kWG - 1) / kWG;
+  const int b = static_cast<int>(L / tiles);
+  const int t = static_cast<int>(L % tiles) * kWG + threadIdx.x;
+  double m = DBL_MAX;
+  if (t < n0 * n1 * n2) {
+    const int i = t % n0 + G.lo0;
+    const int r = t / n0;
+    const int j = r % n1 + G.lo1;
+    const int k = r / n1 + G.lo2;
+    const std::int64_t slab = G.cells * V * nblocks;
+    const std::int64_t cs = G.cells;
+    const std::int64_t c0 = static_cast<std::int64_t>(b) * V * cs + (k * G.e1 + j) * G.e0 + i;
+    const BlockCoefs &C = coef[b];
+    const std::int64_t st[3] = {1, G.e0, static_cast<std::int64_t>(G.e0) * G.e1};
+    double du[V];
+    for (int d = 0; d < G.ndim; ++d) {
+      const double *Fd = flux + d * slab + c0;
+      const double A = C.area[d];
+#pragma unroll
+      for (int n = 0; n < V; ++n) {
+        const double term = (A * Fd[n * cs + st[d]] - A * Fd[n * cs]);
+        du[n] = d == 0 ? term : du[n] + term;
+      }
+    }
+    const double bdt = beta * dtvar[0];
+    const double w2 = 1.0 - beta;
+    double u1[V];
+#pragma unroll
+    for (int n = 0; n < V; ++n) {
+      const double dudt = -du[n] / C.vol;
+      const double avg = beta * U0[c0 + n * cs] + w2 * base[c0 + n * cs];
+      u1[n] = 1.0 * avg + bdt * dudt;
+      out[c0 + n * cs] = u1[n];
+    }
+    if (derived)
+      derived[static_cast<std::int64_t>(b) * cs + (k * G.e1 + j) * G.e0 + i] =
+          0.5 * u1[3] * (u1[0] * u1[0] + u1[1] * u1[1] + u1[2] * u1[2]);
+    double sp = fabs(u1[0]) / C.dx[0];
+    if (G.ndim > 1) sp = sp + fabs(u1[1]) / C.dx[1];
+    if (G.ndim > 2) sp = sp + fabs(u1[2]) / C.dx[2];
+    m = 1.0 / sp;
+  }
+  if (!partial) return;
+  m = WaveMin(m);
+  if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v = wmin[0];
+#pragma unroll
+    for (int w = 1; w < kWG / 64; ++w) v = DMin(v, wmin[w]);
+    partial[L] = v;
+  }
+}
+
 // RestrictAverage::Do for face element F(D+1) (pr_ops.hpp:105-164): area-weighted average of
 // the fine face fluxes fqp (p along the lower transverse dimension t1, q along t2) sharing one
 // coarse face, in the reference's fixed summation order; faces beyond the active dimensions
@@ -877,6 +1022,87 @@ pth_status FlxcorSet(const Topology &t, const FluxCorrection &fc, double *flux, 
   const dim3 grid(static_cast<unsigned>(fc.host.size()), (fc.max_cells + 63) / 64);
   hipLaunchKernelGGL(FlxcorSetKernel, grid, dim3(64), 0, s, fc.dev, fc.store, flux, G, nb,
                      fc.ncomp);
+  PTH_HIP(hipGetLastError());
+  return PTH_COMPLETE;
+}
+
+bool ReconFluxSupported(const Topology &t, int V) {
+  const Extent &f = t.fine;
+  const bool cubic = t.ndim == 3 && f.n[0] == f.n[1] && f.n[1] == f.n[2] &&
+                     f.g[0] == f.g[1] && f.g[1] == f.g[2];
+  return cubic && (f.n[0] == 16 || f.n[0] == 8) && f.g[0] == 4 && V >= 4 && V <= 13;
+}
+
+pth_status BurgersReconFlux(const Topology &t, int V, const double *U, double *flux, int recon,
+                            hipStream_t s) {
+  if (!ReconFluxSupported(t, V)) {
+    SetError("plane-tiled reconstruction: unsupported block shape or component count");
+    return PTH_FAIL;
+  }
+  ProfScope prof("recon_flux", s);
+  const int nb = static_cast<int>(t.blocks.size());
+  const int N = t.fine.n[0];
+  const dim3 grid(static_cast<unsigned>(3 * N * nb));
+#define PTH_RF_CASE(NV)                                                                        \
+  case NV:                                                                                     \
+    if (N == 16)                                                                               \
+      hipLaunchKernelGGL((ReconFluxKernel<16, 4, NV>), grid, dim3(kWG), 0, s, U, flux, nb,      \
+                         recon);                                                               \
+    else                                                                                       \
+      hipLaunchKernelGGL((ReconFluxKernel<8, 4, NV>), grid, dim3(kWG), 0, s, U, flux, nb,       \
+                         recon);                                                               \
+    break;
+  switch (V) {
+    PTH_RF_CASE(4)
+    PTH_RF_CASE(5)
+    PTH_RF_CASE(6)
+    PTH_RF_CASE(7)
+    PTH_RF_CASE(8)
+    PTH_RF_CASE(9)
+    PTH_RF_CASE(10)
+    PTH_RF_CASE(11)
+    PTH_RF_CASE(12)
+    PTH_RF_CASE(13)
+  }
+#undef PTH_RF_CASE
+  PTH_HIP(hipGetLastError());
+  return PTH_COMPLETE;
+}
+
+pth_status BurgersFluxUpdate(const Topology &t, int V, const double *flux, const double *U0,
+                             const double *base, double *out, double *derived, double *partial,
+                             const BlockCoefs *coef, double beta, const double *dtvar,
+                             hipStream_t s) {
+  if (V > kMaxV || V < 4) {
+    SetError("flux update supports 4 to 16 components");
+    return PTH_FAIL;
+  }
+  ProfScope prof("flux_update", s);
+  const Geo G = MakeGeo(t, V);
+  const int nb = static_cast<int>(t.blocks.size());
+  const int cells = t.fine.n[0] * t.fine.n[1] * t.fine.n[2];
+  const dim3 grid(((cells + kWG - 1) / kWG) * nb);
+#define PTH_FU_CASE(NV)                                                                       \
+  case NV:                                                                                    \
+    hipLaunchKernelGGL(FluxUpdateKernel<NV>, grid, dim3(kWG), 0, s, flux, U0, base, out,      \
+                       derived, partial, coef, G, nb, beta, dtvar);                           \
+    break;
+  switch (V) {
+    PTH_FU_CASE(4)
+    PTH_FU_CASE(5)
+    PTH_FU_CASE(6)
+    PTH_FU_CASE(7)
+    PTH_FU_CASE(8)
+    PTH_FU_CASE(9)
+    PTH_FU_CASE(10)
+    PTH_FU_CASE(11)
+    PTH_FU_CASE(12)
+    PTH_FU_CASE(13)
+    PTH_FU_CASE(14)
+    PTH_FU_CASE(15)
+    PTH_FU_CASE(16)
+  }
+#undef PTH_FU_CASE
   PTH_HIP(hipGetLastError());
   return PTH_COMPLETE;
 }

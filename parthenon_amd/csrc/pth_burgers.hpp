@@ -32,6 +32,14 @@ pth_status FlxcorFromFaces(const Topology &t, int V, const FluxCorrection &fc,
 pth_status FlxcorSend(const Topology &t, const FluxCorrection &fc, const double *flux,
                       hipStream_t s);
 pth_status FlxcorSet(const Topology &t, const FluxCorrection &fc, double *flux, hipStream_t s);
+// plane-tiled reconstruction + HLL into the flux array, and the update from stored fluxes
+bool ReconFluxSupported(const Topology &t, int V);
+pth_status BurgersReconFlux(const Topology &t, int V, const double *U, double *flux, int recon,
+                            hipStream_t s);
+pth_status BurgersFluxUpdate(const Topology &t, int V, const double *flux, const double *U0,
+                             const double *base, double *out, double *derived, double *partial,
+                             const BlockCoefs *coef, double beta, const double *dtvar,
+                             hipStream_t s);
 pth_status BurgersClock(const double *partial, int n, double cfl, double *dtvar, int advance,
                         hipStream_t s);
 

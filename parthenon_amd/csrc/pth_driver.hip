@@ -3,10 +3,14 @@
 // (src/time_integration/low_storage_integrator.cpp:67-83) and EvolutionDriver's time-step
 // control (src/driver/driver.cpp:194-235), with every host decision hoisted out of the stage:
 //
-//   stage 1: recon(base) -> [flux correction] -> fused update base -> "1" -> exchange("1")
-//   stage 2: recon("1")  -> [flux correction] -> fused update ("1", base) -> base + CFL
+//   stage 1: recon+HLL(base) -> [flux correction] -> update base -> "1" -> exchange("1")
+//   stage 2: recon+HLL("1")  -> [flux correction] -> update ("1", base) -> base + CFL
 //            partials -> exchange(base)
 //   clock:   time += dt; dt = min(2 dt, cfl * min)   (all on device)
+//
+// For cubic blocks of 8 or 16 cells with 4 ghosts (and 4-13 components) the reconstruction
+// and HLL run plane by plane in LDS and only the fluxes reach HBM; other shapes keep the face
+// states in HBM and evaluate HLL inside the update.
 //
 // The simulation clock lives in device memory, so a whole cycle has no host synchronization
 // and is captured once into a hipGraph and replayed. The exchange inside a cycle uses the same
@@ -80,7 +84,8 @@ pth_status Setup(Mesh *m, Ctx &x) {
   if (!m->run_stream) PTH_HIP(hipStreamCreateWithFlags(&m->run_stream, hipStreamNonBlocking));
   if (!m->dtvar) PTH_HIP(hipMalloc(&m->dtvar, sizeof(double) * 4));
   const std::int64_t nd = x.base->store[x.fu].ndata;
-  if (!m->face || m->face_V != x.V) {
+  const bool planes = x.base->store[x.fu].flux && ReconFluxSupported(m->topo, x.V);
+  if (!planes && (!m->face || m->face_V != x.V)) {  // face-state scratch of the other path
     if (m->face) (void)hipFree(m->face);
     m->face = nullptr;
     PTH_HIP(hipMalloc(&m->face, sizeof(double) * 6 * nd));
@@ -100,17 +105,34 @@ pth_status Stage(Ctx &x, int stage, hipStream_t s) {
   MeshData &mc1 = stage == 1 ? *x.one : *x.base;
   const double beta = stage == 1 ? 1.0 : 0.5;
   double *derived = x.fd >= 0 ? x.base->store[x.fd].data : nullptr;
-  if (BurgersReconTasks(m.topo, x.V, mc0.store[x.fu].data, m.face, m.run_recon, s) !=
-      PTH_COMPLETE)
-    return PTH_FAIL;
-  // LoadAndSendFluxCorrections / SetFluxCorrections: the restricted fine fluxes land in the
-  // correction store that the fused update reads in place of its own coarse-face fluxes
-  if (x.fc && FlxcorFromFaces(m.topo, x.V, *x.fc, m.face, s) != PTH_COMPLETE) return PTH_FAIL;
-  if (BurgersFusedUpdate(m.topo, x.V, m.face, mc0.store[x.fu].data, x.base->store[x.fu].data,
-                         mc1.store[x.fu].data, derived, nullptr,
-                         stage == 2 ? m.scratch : nullptr, m.coef, beta, m.dtvar, x.fc,
-                         s) != PTH_COMPLETE)
-    return PTH_FAIL;
+  double *flux = x.base->store[x.fu].flux;  // scratch of both stages
+  if (flux && ReconFluxSupported(m.topo, x.V)) {
+    // plane-tiled reconstruction + HLL into the flux array, the flux-correction tasks on the
+    // stored fluxes (LoadAndSendFluxCorrections, SetFluxCorrections), then the update
+    if (BurgersReconFlux(m.topo, x.V, mc0.store[x.fu].data, flux, m.run_recon, s) !=
+        PTH_COMPLETE)
+      return PTH_FAIL;
+    if (x.fc && (FlxcorSend(m.topo, *x.fc, flux, s) != PTH_COMPLETE ||
+                 FlxcorSet(m.topo, *x.fc, flux, s) != PTH_COMPLETE))
+      return PTH_FAIL;
+    if (BurgersFluxUpdate(m.topo, x.V, flux, mc0.store[x.fu].data, x.base->store[x.fu].data,
+                          mc1.store[x.fu].data, derived, stage == 2 ? m.scratch : nullptr,
+                          m.coef, beta, m.dtvar, s) != PTH_COMPLETE)
+      return PTH_FAIL;
+  } else {
+    if (BurgersReconTasks(m.topo, x.V, mc0.store[x.fu].data, m.face, m.run_recon, s) !=
+        PTH_COMPLETE)
+      return PTH_FAIL;
+    // LoadAndSendFluxCorrections / SetFluxCorrections: the restricted fine fluxes land in
+    // the correction store that the fused update reads in place of its coarse-face fluxes
+    if (x.fc && FlxcorFromFaces(m.topo, x.V, *x.fc, m.face, s) != PTH_COMPLETE)
+      return PTH_FAIL;
+    if (BurgersFusedUpdate(m.topo, x.V, m.face, mc0.store[x.fu].data,
+                           x.base->store[x.fu].data, mc1.store[x.fu].data, derived, nullptr,
+                           stage == 2 ? m.scratch : nullptr, m.coef, beta, m.dtvar, x.fc,
+                           s) != PTH_COMPLETE)
+      return PTH_FAIL;
+  }
   if (Exchange(m, mc1, s) != PTH_COMPLETE) return PTH_FAIL;
   if (stage == 2)
     return BurgersClock(m.scratch, FusedUpdatePartials(m.topo), m.run_cfl, m.dtvar, 1, s);

@@ -689,7 +689,7 @@ pth_status GmgBatch<T>::Cycle(const std::vector<GmgProblem<T> *> &ps, double ome
     GmgEntry<T> *d;
     int ne, nwg;
     if (Stage(entries(l), false, &d, &ne, &nwg) != PTH_COMPLETE) return PTH_FAIL;
-    ProfScope prof("gmg_jacobi", stream);
+    ProfScope prof(l == 0 ? "gmg_jacobi_l0" : "gmg_jacobi", stream);
     hipLaunchKernelGGL(JacobiKernel<T>, dim3(nwg), dim3(kWG), 0, stream, d, ne, w);
     PTH_HIP(hipGetLastError());
     for (auto *p : ps)
@@ -701,7 +701,7 @@ pth_status GmgBatch<T>::Cycle(const std::vector<GmgProblem<T> *> &ps, double ome
     int ne, nwg;
     if (pre > 0) {
       if (Stage(entries(l), false, &d, &ne, &nwg) != PTH_COMPLETE) return PTH_FAIL;
-      ProfScope prof("gmg_pre", stream);
+      ProfScope prof(l == 0 ? "gmg_pre_l0" : "gmg_pre", stream);
       if (pre >= 2)
         hipLaunchKernelGGL((PreZeroKernel<T, true>), dim3(nwg), dim3(kWG), 0, stream, d, ne,
                            w);
@@ -717,7 +717,7 @@ pth_status GmgBatch<T>::Cycle(const std::vector<GmgProblem<T> *> &ps, double ome
     for (int s = 2; s < pre; ++s)
       if (sweep(l) != PTH_COMPLETE) return PTH_FAIL;
     if (Stage(entries(l), true, &d, &ne, &nwg) != PTH_COMPLETE) return PTH_FAIL;
-    ProfScope prof("gmg_restrict", stream);
+    ProfScope prof(l == 0 ? "gmg_restrict_l0" : "gmg_restrict", stream);
     hipLaunchKernelGGL(ResidualRestrictKernel<T>, dim3(nwg), dim3(kWG), 0, stream, d, ne);
     PTH_HIP(hipGetLastError());
   }
@@ -737,7 +737,7 @@ pth_status GmgBatch<T>::Cycle(const std::vector<GmgProblem<T> *> &ps, double ome
     int ne, nwg;
     if (Stage(entries(l), false, &d, &ne, &nwg) != PTH_COMPLETE) return PTH_FAIL;
     {
-      ProfScope prof("gmg_prolong", stream);
+      ProfScope prof(l == 0 ? "gmg_prolong_l0" : "gmg_prolong", stream);
       hipLaunchKernelGGL(ProlongKernel<T>, dim3(nwg), dim3(kWG), 0, stream, d, ne);
       PTH_HIP(hipGetLastError());
     }

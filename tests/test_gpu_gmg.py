@@ -36,8 +36,13 @@ def test_fp64_solver_matches_oracle(N):
     g, b, it, hist = oracle_solve(N, seed=3)
     s = MGSolver(N, "fp64")
     assert s.levels == g.levels
+    # the device blob RHS agrees with the host one to round-off (the device exp and the host
+    # exp may differ in the last bit of each of the 16 terms)
     s.set_rhs_blobs(3)
-    assert np.array_equal(s.rhs(), b), "device blob RHS equals the oracle's"
+    db = s.rhs()
+    assert np.max(np.abs(db - b)) <= 1e-14 * np.max(np.abs(b))
+    # the solver itself is compared on the oracle's RHS
+    s.set_rhs(b)
     its, rel = s.solve()
     h = s.history()
     assert its == it, (its, it)
