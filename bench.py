@@ -154,11 +154,15 @@ def gmg_bench(args):
     import parthenon_amd as pa
     from parthenon_amd.gmg import MGSolver
 
+    from parthenon_amd.gmg import MGParams
     N, seed = 256, 3
     res = {}
     sols = {}
-    for prec in ("mixed", "fp64"):
-        s = MGSolver(N, prec)
+    configs = {"mixed": ("mixed", args.inner_cycles), "fp64": ("fp64", 1)}
+    if args.inner_cycles != 1:
+        configs["mixed_inner1"] = ("mixed", 1)
+    for name, (prec, inner) in configs.items():
+        s = MGSolver(N, prec, MGParams(inner_cycles=inner))
         s.set_rhs_blobs(seed)
         for _ in range(args.warmup):
             s.reset()
@@ -176,8 +180,8 @@ def gmg_bench(args):
         its, rel = s.solve()
         prof = pa.profile_report()
         pa.profile_enable(False)
-        res[prec] = dict(wall=wall, steps=steps, iters=its, rel=rel, prof=prof)
-        sols[prec] = s.solution()
+        res[name] = dict(wall=wall, steps=steps, iters=its, rel=rel, prof=prof)
+        sols[name] = s.solution()
         del s
     m = res["mixed"]
     c = N ** 3
@@ -193,7 +197,7 @@ def gmg_bench(args):
     achieved = kb[dom] / avg_s / 1e9
     # per outer iteration: the finest level's traffic plus 1/7 of it for the coarser levels
     it_bytes = int(32 * c + (2 * es + 2 * 3 * es + 2 * (2 * es + es / 8)) * c * 8 / 7)
-    per_it = m["wall"] / m["steps"] / max(m["iters"], 1)
+    per_it = m["wall"] / m["steps"] / max(m["iters"] * args.inner_cycles, 1)
     out = {
         "metric": "Poisson GMG solves/s (256^3, 1e-10 relative residual)",
         "value": m["steps"] / m["wall"], "unit": "solves/s", "n_gpus": 1,
@@ -203,12 +207,17 @@ def gmg_bench(args):
         "data": f"synthetic (16 Gaussian blobs, SplitMix64 seed {seed})",
         "config": {"workload": "poisson-gmg 256^3 on [0,1]^3, 7-point, FixedFace 0, "
                                "omega 2/3 Jacobi 2+2, V-cycle to 1e-10 relative",
-                   "levels": 9, "iterations": m["iters"], "final_rel_residual": m["rel"]},
+                   "levels": 9, "inner_cycles_per_fp64_step": args.inner_cycles,
+                   "fp64_steps": m["iters"], "final_rel_residual": m["rel"]},
         "fp64": {"solves_per_s": res["fp64"]["steps"] / res["fp64"]["wall"],
                  "iterations": res["fp64"]["iters"],
                  "final_rel_residual": res["fp64"]["rel"]},
         "speedup_vs_fp64": (m["steps"] / m["wall"]) /
                            (res["fp64"]["steps"] / res["fp64"]["wall"]),
+        "mixed_one_cycle_per_step": ({"solves_per_s": res["mixed_inner1"]["steps"] /
+                                      res["mixed_inner1"]["wall"],
+                                      "fp64_steps": res["mixed_inner1"]["iters"]}
+                                     if "mixed_inner1" in res else None),
         "error_vs_fp64_solver": float(np.sqrt(np.mean((sols["mixed"] - sols["fp64"]) ** 2)) /
                                       np.sqrt(np.mean(sols["fp64"] ** 2))),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
@@ -351,6 +360,8 @@ def main():
     ap.add_argument("--driver", choices=["graph", "eager", "tasks"], default="graph")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-cycles", type=int, default=None)
+    ap.add_argument("--inner-cycles", type=int, default=2,
+                    help="gmg: FP32 V-cycles per FP64 refinement step")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "

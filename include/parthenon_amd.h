@@ -165,6 +165,10 @@ pth_status pth_burgers_init(pth_mesh *m, double cfl, int recon);
  * replays it. Asynchronous on the driver's own stream. */
 pth_status pth_burgers_run_cycles(pth_mesh *m, int ncycles, int use_graph);
 /* Synchronous read of the device clock: current dt and simulated time. */
+/* Driver option (no counterpart in the reference): with on != 0 (the default) and cubic blocks
+ * of 8 or 16 cells with 4 ghosts, each stage's input exchange runs on a second stream beside
+ * the reconstruction of the faces whose stencils read no ghost cell; results are identical. */
+pth_status pth_burgers_set_overlap(pth_mesh *m, int on);
 pth_status pth_burgers_clock(pth_mesh *m, double *dt, double *time);
 /* Waits for the driver's stream. */
 pth_status pth_burgers_synchronize(pth_mesh *m);
@@ -194,6 +198,9 @@ pth_status pth_gmg_set_rhs(pth_gmg *g, const double *host, size_t n);
 pth_status pth_gmg_get_rhs(pth_gmg *g, double *host, size_t n);
 pth_status pth_gmg_get_solution(pth_gmg *g, double *host, size_t n);
 pth_status pth_gmg_reset(pth_gmg *g);
+/* V-cycles per FP64 refinement step (default 1); cycles after the first continue from the
+ * current correction on the same right-hand side. */
+pth_status pth_gmg_set_inner_cycles(pth_gmg *g, int k);
 /* MGSolver::AddTasks iteration (mg_solver.hpp:119-161): V-cycles until the relative L2
  * residual ||b - A u|| / ||b|| < tol or max_iters cycles; iters and final_rel reported. */
 pth_status pth_gmg_solve(pth_gmg *g, double tol, int max_iters, int *iters, double *final_rel);
@@ -211,6 +218,7 @@ pth_gmg_server *pth_gmg_server_create(int precision, double omega, int pre, int 
 void pth_gmg_server_destroy(pth_gmg_server *s);
 pth_status pth_gmg_server_submit(pth_gmg_server *s, int n, uint64_t seed, double arrival_s,
                                  int *id);
+pth_status pth_gmg_server_set_inner_cycles(pth_gmg_server *s, int k);
 /* serve every submitted request; realtime: honour arrival times on the wall clock */
 pth_status pth_gmg_server_run(pth_gmg_server *s, int realtime);
 /* out: arrival, admission, completion (s, server clock), latency (s), final relative residual */

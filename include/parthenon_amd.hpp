@@ -17,6 +17,7 @@
 #define PARTHENON_AMD_HPP_
 
 #include <array>
+#include <cstdint>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -220,6 +221,61 @@ TaskStatus AverageIndependentData(T *c1, T *c2, const Real wgt1) {
   return WeightedSumDataIndependent(c1, c2, wgt1, (1.0 - wgt1), c1);
 }
 }  // namespace Update
+
+// ---- multigrid Poisson solver (src/solvers/mg_solver.hpp:37-521) ----
+namespace solvers {
+// MGParams (mg_solver.hpp:37-58) with the weighted-Jacobi weight and sweep counts explicit;
+// residual_tolerance is relative: ||b - A u|| / ||b||.
+struct MGParams {
+  int max_iters = 1000;
+  Real residual_tolerance = 1.e-10;
+  Real omega = 2.0 / 3.0;
+  int pre_sweeps = 2, post_sweeps = 2;
+  bool mixed_precision = true;  // FP32 V-cycles inside FP64 iterative refinement
+};
+
+// MGSolver<u, rhs, PoissonEquation> on the uniform n^3 grid of example/poisson_gmg ([0,1]^3,
+// FixedFace 0): set the right-hand side, Solve(), read the solution; host arrays in [k][j][i].
+class MGSolver {
+ public:
+  MGSolver(int n, MGParams p = MGParams()) : n_(n), p_(p) {
+    h_ = pth_gmg_create(n, p.mixed_precision ? 0 : 1, p.omega, p.pre_sweeps, p.post_sweeps);
+    if (!h_) throw std::runtime_error(std::string("MGSolver: ") + pth_last_error());
+  }
+  ~MGSolver() {
+    if (h_) pth_gmg_destroy(h_);
+  }
+  MGSolver(const MGSolver &) = delete;
+  MGSolver &operator=(const MGSolver &) = delete;
+  void SetRhs(const std::vector<Real> &b) {
+    detail::Check(pth_gmg_set_rhs(h_, b.data(), b.size()), "SetRhs");
+  }
+  void SetRhsBlobs(std::uint64_t seed) {
+    detail::Check(pth_gmg_set_rhs_blobs(h_, seed), "SetRhsBlobs");
+  }
+  // the AddTasks iteration (mg_solver.hpp:119-161) run to completion
+  int Solve() {
+    int it = 0;
+    detail::Check(pth_gmg_solve(h_, p_.residual_tolerance, p_.max_iters, &it, &final_residual_),
+                  "Solve");
+    final_iteration_ = it;
+    return it;
+  }
+  void GetSolution(std::vector<Real> &u) const {
+    u.resize(static_cast<size_t>(n_) * n_ * n_);
+    detail::Check(pth_gmg_get_solution(h_, u.data(), u.size()), "GetSolution");
+  }
+  Real GetFinalResidual() const { return final_residual_; }
+  int GetFinalIterations() const { return final_iteration_; }
+
+ private:
+  int n_;
+  MGParams p_;
+  pth_gmg *h_ = nullptr;
+  Real final_residual_ = 0;
+  int final_iteration_ = 0;
+};
+}  // namespace solvers
 
 }  // namespace parthenon
 

@@ -101,6 +101,7 @@ _SIGNATURES = {
     "pth_burgers_init": (_I, [_P, _D, _I]),
     "pth_burgers_run_cycles": (_I, [_P, _I, _I]),
     "pth_burgers_clock": (_I, [_P, _DP, _DP]),
+    "pth_burgers_set_overlap": (_I, [_P, _I]),
     "pth_burgers_synchronize": (_I, [_P]),
     "pth_gmg_create": (_P, [_I, _I, _D, _I, _I]),
     "pth_gmg_destroy": (None, [_P]),
@@ -111,6 +112,8 @@ _SIGNATURES = {
     "pth_gmg_get_solution": (_I, [_P, _DP, C.c_size_t]),
     "pth_gmg_reset": (_I, [_P]),
     "pth_gmg_solve": (_I, [_P, _D, _I, C.POINTER(_I), _DP]),
+    "pth_gmg_set_inner_cycles": (_I, [_P, _I]),
+    "pth_gmg_server_set_inner_cycles": (_I, [_P, _I]),
     "pth_gmg_history": (_I, [_P, _DP, _I]),
     "pth_gmg_server_create": (_P, [_I, _D, _I, _I, _D, _I, _I64, _I, _I]),
     "pth_gmg_server_destroy": (None, [_P]),

@@ -37,7 +37,8 @@ class DeviceBurgersDriver:
     flux/divergence/update launch and the exchange per stage, dt and time in device memory, and
     (use_graph) one captured hipGraph replayed per cycle."""
 
-    def __init__(self, mesh, cfl=0.4, recon=burgers_package.WENO5, use_graph=True):
+    def __init__(self, mesh, cfl=0.4, recon=burgers_package.WENO5, use_graph=True,
+                 overlap=True):
         import ctypes as C
         from . import _check, lib
         self._C, self._check, self._lib = C, _check, lib
@@ -47,6 +48,8 @@ class DeviceBurgersDriver:
         self.use_graph = use_graph
         self.base = mesh.mesh_data("base")
         self.one = mesh.mesh_data("1")
+        # each stage's input exchange beside the ghost-free reconstruction (same results)
+        _check(lib().pth_burgers_set_overlap(mesh._h, int(overlap)), "set_overlap")
 
     def initialize(self, pgen="sinusoid", seed=0):
         if pgen == "sinusoid":

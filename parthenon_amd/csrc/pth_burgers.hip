@@ -506,13 +506,32 @@ __global__ void __launch_bounds__(kWG)
 // (x-face i at index i). The operations are those of ReconTaskKernel followed by HllFace, so
 // the fluxes are the same bits, and the face states never reach HBM: the flux array is half
 // their size and the update reads it without recomputing HLL on shared faces.
-template <int N, int NG, int V>
+// PART selects the cells and faces of each plane row along the direction (ext index a of the
+// N + 2 cells lo - 1 .. hi + 1; face f between ext cells f and f + 1):
+//   0: every cell and face;
+//   1: the cells whose WENO stencil reads no ghost cell (a in [3, N - 2]) and the faces
+//      between two of them (f in [3, N - 3]): no dependence on the boundary exchange;
+//   2: the rest (a in [0, 3] and [N - 2, N + 1], f in [0, 2] and [N - 2, N]); the cells 3 and
+//      N - 2 are evaluated in both parts.
+template <int N, int PART>
+struct PlaneSet {
+  static constexpr int kCells = PART == 0 ? N + 2 : PART == 1 ? N - 4 : 8;
+  static constexpr int kFaces = PART == 0 ? N + 1 : PART == 1 ? N - 5 : 6;
+  __device__ static int Cell(int q) {
+    return PART == 0 ? q : PART == 1 ? 3 + q : (q < 4 ? q : N - 6 + q);
+  }
+  __device__ static int Face(int q) {
+    return PART == 0 ? q : PART == 1 ? 3 + q : (q < 3 ? q : N - 5 + q);
+  }
+};
+
+template <int N, int NG, int V, int PART>
 __global__ void __launch_bounds__(kWG)
     ReconFluxKernel(const double *__restrict__ U, double *__restrict__ flux, int nblocks,
                     int recon) {
   constexpr int E = N + 2 * NG;  // brick extent
   constexpr int P = (N + 2) * N;  // cells of one plane, extended along the direction
-  constexpr int F = (N + 1) * N;  // faces of one plane
+  using S = PlaneSet<N, PART>;
   __shared__ double sl[V * P], sr[V * P];
   // logical workgroup = (block * 3 + direction) * N + plane; one block's planes are adjacent
   // and stay on one XCD
@@ -522,24 +541,33 @@ __global__ void __launch_bounds__(kWG)
   constexpr std::int64_t cs = static_cast<std::int64_t>(E) * E * E;
   const std::int64_t slab = cs * V * nblocks;
   const double *Ub = U + static_cast<std::int64_t>(b) * V * cs;
-  const int W = d == 0 ? N + 2 : N;  // plane row length (i fastest)
   const unsigned st = 8u * static_cast<unsigned>(d == 0 ? 1 : d == 1 ? E : E * E);
-  for (int t = threadIdx.x; t < V * P; t += kWG) {
-    const int n = t / P, p = t % P;
-    const int a = p % W, row = p / W;
+  // plane index of (ext cell a along the direction, transverse tv): rows of N + 2 along x for
+  // d = 0, else rows of N along x; consecutive threads take consecutive x
+  for (int t = threadIdx.x; t < V * S::kCells * N; t += kWG) {
+    const int n = t / (S::kCells * N), u = t % (S::kCells * N);
+    int a, tv;
+    if (d == 0) {
+      a = S::Cell(u % S::kCells);
+      tv = u / S::kCells;
+    } else {
+      tv = u % N;
+      a = S::Cell(u / N);
+    }
+    const int p = d == 0 ? tv * (N + 2) + a : a * N + tv;
     int i, j, k;
     if (d == 0) {
       i = NG - 1 + a;
-      j = NG + row;
+      j = NG + tv;
       k = NG + plane;
     } else if (d == 1) {
-      i = NG + a;
-      j = NG - 1 + row;
+      i = NG + tv;
+      j = NG - 1 + a;
       k = NG + plane;
     } else {
-      i = NG + a;
+      i = NG + tv;
       j = NG + plane;
-      k = NG - 1 + row;
+      k = NG - 1 + a;
     }
     const unsigned c = 8u * static_cast<unsigned>((k * E + j) * E + i);
     const double *q = Ub + n * cs;
@@ -553,22 +581,28 @@ __global__ void __launch_bounds__(kWG)
     sr[n * P + p] = qr;
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < F; t += kWG) {
+  for (int t = threadIdx.x; t < S::kFaces * N; t += kWG) {
+    int f, tv;
+    if (d == 0) {
+      f = S::Face(t % S::kFaces);
+      tv = t / S::kFaces;
+    } else {
+      tv = t % N;
+      f = S::Face(t / N);
+    }
     int pl, pr, i, j, k;
-    if (d == 0) {  // face between extended cells a and a + 1 of row j
-      const int a = t % (N + 1), row = t / (N + 1);
-      pl = row * W + a;
+    if (d == 0) {  // face between ext cells f and f + 1 of row tv
+      pl = tv * (N + 2) + f;
       pr = pl + 1;
-      i = NG + a;
-      j = NG + row;
+      i = NG + f;
+      j = NG + tv;
       k = NG + plane;
-    } else {  // face between extended rows row and row + 1
-      const int a = t % N, row = t / N;
-      pl = row * W + a;
-      pr = pl + W;
-      i = NG + a;
-      j = d == 1 ? NG + row : NG + plane;
-      k = d == 1 ? NG + plane : NG + row;
+    } else {  // face between ext rows f and f + 1
+      pl = f * N + tv;
+      pr = pl + N;
+      i = NG + tv;
+      j = d == 1 ? NG + f : NG + plane;
+      k = d == 1 ? NG + plane : NG + f;
     }
     double Fv[V];
     HllFace<V>(sl + pl, sr + pr, P, d, Fv);
@@ -1034,23 +1068,33 @@ bool ReconFluxSupported(const Topology &t, int V) {
 }
 
 pth_status BurgersReconFlux(const Topology &t, int V, const double *U, double *flux, int recon,
-                            hipStream_t s) {
-  if (!ReconFluxSupported(t, V)) {
+                            int part, hipStream_t s) {
+  if (!ReconFluxSupported(t, V) || part < 0 || part > 2) {
     SetError("plane-tiled reconstruction: unsupported block shape or component count");
     return PTH_FAIL;
   }
-  ProfScope prof("recon_flux", s);
+  ProfScope prof(part == 0 ? "recon_flux" : part == 1 ? "recon_flux_inner" : "recon_flux_outer",
+                 s);
   const int nb = static_cast<int>(t.blocks.size());
   const int N = t.fine.n[0];
   const dim3 grid(static_cast<unsigned>(3 * N * nb));
-#define PTH_RF_CASE(NV)                                                                        \
-  case NV:                                                                                     \
-    if (N == 16)                                                                               \
-      hipLaunchKernelGGL((ReconFluxKernel<16, 4, NV>), grid, dim3(kWG), 0, s, U, flux, nb,      \
-                         recon);                                                               \
-    else                                                                                       \
-      hipLaunchKernelGGL((ReconFluxKernel<8, 4, NV>), grid, dim3(kWG), 0, s, U, flux, nb,       \
-                         recon);                                                               \
+#define PTH_RF_LAUNCH(NN, NV, PT)                                                             \
+  hipLaunchKernelGGL((ReconFluxKernel<NN, 4, NV, PT>), grid, dim3(kWG), 0, s, U, flux, nb,     \
+                     recon)
+#define PTH_RF_PARTS(NN, NV)                                                                  \
+  if (part == 0)                                                                              \
+    PTH_RF_LAUNCH(NN, NV, 0);                                                                 \
+  else if (part == 1)                                                                         \
+    PTH_RF_LAUNCH(NN, NV, 1);                                                                 \
+  else                                                                                        \
+    PTH_RF_LAUNCH(NN, NV, 2);
+#define PTH_RF_CASE(NV)                                                                       \
+  case NV:                                                                                    \
+    if (N == 16) {                                                                            \
+      PTH_RF_PARTS(16, NV)                                                                    \
+    } else {                                                                                  \
+      PTH_RF_PARTS(8, NV)                                                                     \
+    }                                                                                         \
     break;
   switch (V) {
     PTH_RF_CASE(4)
@@ -1065,6 +1109,8 @@ pth_status BurgersReconFlux(const Topology &t, int V, const double *U, double *f
     PTH_RF_CASE(13)
   }
 #undef PTH_RF_CASE
+#undef PTH_RF_PARTS
+#undef PTH_RF_LAUNCH
   PTH_HIP(hipGetLastError());
   return PTH_COMPLETE;
 }

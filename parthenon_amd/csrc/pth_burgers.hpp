@@ -34,8 +34,9 @@ pth_status FlxcorSend(const Topology &t, const FluxCorrection &fc, const double 
 pth_status FlxcorSet(const Topology &t, const FluxCorrection &fc, double *flux, hipStream_t s);
 // plane-tiled reconstruction + HLL into the flux array, and the update from stored fluxes
 bool ReconFluxSupported(const Topology &t, int V);
+// part 0: every face; 1: faces independent of ghost cells; 2: the remaining faces
 pth_status BurgersReconFlux(const Topology &t, int V, const double *U, double *flux, int recon,
-                            hipStream_t s);
+                            int part, hipStream_t s);
 pth_status BurgersFluxUpdate(const Topology &t, int V, const double *flux, const double *U0,
                              const double *base, double *out, double *derived, double *partial,
                              const BlockCoefs *coef, double beta, const double *dtvar,

@@ -91,6 +91,9 @@ Mesh::~Mesh() {
   if (run_stream) (void)hipStreamDestroy(run_stream);
   for (hipEvent_t e : order_ev)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : fork_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (side_stream) (void)hipStreamDestroy(side_stream);
   if (dtvar) (void)hipFree(dtvar);
   if (scratch) (void)hipFree(scratch);
   if (pinned) (void)hipHostFree(pinned);

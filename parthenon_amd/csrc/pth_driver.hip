@@ -82,6 +82,10 @@ pth_status Setup(Mesh *m, Ctx &x) {
   x.one = m->Get("1");
   if (!x.base || !x.one) return PTH_FAIL;
   if (!m->run_stream) PTH_HIP(hipStreamCreateWithFlags(&m->run_stream, hipStreamNonBlocking));
+  if (!m->side_stream)
+    PTH_HIP(hipStreamCreateWithFlags(&m->side_stream, hipStreamNonBlocking));
+  for (hipEvent_t &e : m->fork_ev)
+    if (!e) PTH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   if (!m->dtvar) PTH_HIP(hipMalloc(&m->dtvar, sizeof(double) * 4));
   const std::int64_t nd = x.base->store[x.fu].ndata;
   const bool planes = x.base->store[x.fu].flux && ReconFluxSupported(m->topo, x.V);
@@ -109,7 +113,7 @@ pth_status Stage(Ctx &x, int stage, hipStream_t s) {
   if (flux && ReconFluxSupported(m.topo, x.V)) {
     // plane-tiled reconstruction + HLL into the flux array, the flux-correction tasks on the
     // stored fluxes (LoadAndSendFluxCorrections, SetFluxCorrections), then the update
-    if (BurgersReconFlux(m.topo, x.V, mc0.store[x.fu].data, flux, m.run_recon, s) !=
+    if (BurgersReconFlux(m.topo, x.V, mc0.store[x.fu].data, flux, m.run_recon, 0, s) !=
         PTH_COMPLETE)
       return PTH_FAIL;
     if (x.fc && (FlxcorSend(m.topo, *x.fc, flux, s) != PTH_COMPLETE ||
@@ -139,7 +143,50 @@ pth_status Stage(Ctx &x, int stage, hipStream_t s) {
   return PTH_COMPLETE;
 }
 
+bool Overlapped(const Ctx &x) {
+  return x.m->run_overlap && x.base->store[x.fu].flux && ReconFluxSupported(x.m->topo, x.V);
+}
+
+// The stage with its input's boundary exchange moved from the end of the previous stage to
+// the start of this one, beside the reconstruction of the cells whose stencils read no ghost:
+//   main stream: recon+HLL (ghost-free faces)  |  side stream: exchange(input)
+//   join -> recon+HLL (remaining faces) -> [flux correction] -> update -> [clock]
+// The two branches touch disjoint data: the exchange reads interior cells and writes ghost
+// cells and coarse buffers, the first reconstruction reads interior cells and writes fluxes.
+pth_status StageOverlap(Ctx &x, int stage, hipStream_t s) {
+  Mesh &m = *x.m;
+  MeshData &mc0 = stage == 1 ? *x.base : *x.one;
+  MeshData &mc1 = stage == 1 ? *x.one : *x.base;
+  const double beta = stage == 1 ? 1.0 : 0.5;
+  double *derived = x.fd >= 0 ? x.base->store[x.fd].data : nullptr;
+  double *flux = x.base->store[x.fu].flux;
+  const double *U = mc0.store[x.fu].data;
+  PTH_HIP(hipEventRecord(m.fork_ev[0], s));
+  PTH_HIP(hipStreamWaitEvent(m.side_stream, m.fork_ev[0], 0));
+  if (Exchange(m, mc0, m.side_stream) != PTH_COMPLETE) return PTH_FAIL;
+  if (BurgersReconFlux(m.topo, x.V, U, flux, m.run_recon, 1, s) != PTH_COMPLETE)
+    return PTH_FAIL;
+  PTH_HIP(hipEventRecord(m.fork_ev[1], m.side_stream));
+  PTH_HIP(hipStreamWaitEvent(s, m.fork_ev[1], 0));
+  if (BurgersReconFlux(m.topo, x.V, U, flux, m.run_recon, 2, s) != PTH_COMPLETE)
+    return PTH_FAIL;
+  if (x.fc && (FlxcorSend(m.topo, *x.fc, flux, s) != PTH_COMPLETE ||
+               FlxcorSet(m.topo, *x.fc, flux, s) != PTH_COMPLETE))
+    return PTH_FAIL;
+  if (BurgersFluxUpdate(m.topo, x.V, flux, U, x.base->store[x.fu].data, mc1.store[x.fu].data,
+                        derived, stage == 2 ? m.scratch : nullptr, m.coef, beta, m.dtvar,
+                        s) != PTH_COMPLETE)
+    return PTH_FAIL;
+  if (stage == 2)
+    return BurgersClock(m.scratch, FusedUpdatePartials(m.topo), m.run_cfl, m.dtvar, 1, s);
+  return PTH_COMPLETE;
+}
+
 pth_status Cycle(Ctx &x, hipStream_t s) {
+  if (Overlapped(x)) {
+    if (StageOverlap(x, 1, s) != PTH_COMPLETE) return PTH_FAIL;
+    return StageOverlap(x, 2, s);
+  }
   if (Stage(x, 1, s) != PTH_COMPLETE) return PTH_FAIL;
   return Stage(x, 2, s);
 }
@@ -170,8 +217,10 @@ pth_status RunCycles(Mesh &m, Ctx &x, hipStream_t s, int ncycles, bool use_graph
     m.graph = g;
     PTH_HIP(hipGraphInstantiate(&m.graph_exec, g, nullptr, nullptr, 0));
   }
-  ProfScope prof("graph_cycle", s);
-  for (int c = 0; c < ncycles; ++c) PTH_HIP(hipGraphLaunch(m.graph_exec, s));
+  {
+    ProfScope prof("graph_cycle", s);
+    for (int c = 0; c < ncycles; ++c) PTH_HIP(hipGraphLaunch(m.graph_exec, s));
+  }
   return PTH_COMPLETE;
 }
 
@@ -251,8 +300,20 @@ pth_status pth_burgers_run_cycles(pth_mesh *mh, int ncycles, int use_graph) {
   PTH_HIP(hipEventRecord(m->order_ev[0], m->stream));
   PTH_HIP(hipStreamWaitEvent(s, m->order_ev[0], 0));
   if (RunCycles(*m, x, s, ncycles, use_graph != 0) != PTH_COMPLETE) return PTH_FAIL;
+  // with the exchange at the start of each stage, the last output's ghosts are still stale
+  if (ncycles > 0 && Overlapped(x) && Exchange(*m, *x.base, s) != PTH_COMPLETE) return PTH_FAIL;
   PTH_HIP(hipEventRecord(m->order_ev[1], s));
   PTH_HIP(hipStreamWaitEvent(m->stream, m->order_ev[1], 0));
+  return PTH_COMPLETE;
+}
+
+pth_status pth_burgers_set_overlap(pth_mesh *mh, int on) {
+  Mesh *m = Impl(mh);
+  m->run_overlap = on != 0;
+  if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
+  if (m->graph) (void)hipGraphDestroy(m->graph);
+  m->graph_exec = nullptr;
+  m->graph = nullptr;
   return PTH_COMPLETE;
 }
 

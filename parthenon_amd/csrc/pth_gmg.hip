@@ -25,6 +25,7 @@
 #include <cstring>
 #include <vector>
 
+#include "pth_device.hpp"
 #include "pth_gmg.hpp"
 
 namespace pth {
@@ -39,12 +40,58 @@ __device__ __forceinline__ T Ratio(T a, T b) {  // robust::ratio (utils/robust.h
   return a / (b + sgn * small);
 }
 
+template <typename T>
+__device__ __forceinline__ T Diagonal(T dx) {  // SetDiagonal (poisson_equation.hpp:86-125)
+  const T D = T(1), alpha = T(0);
+  T d = -(D + D) / (dx * dx) - alpha;
+  d -= (D + D) / (dx * dx);
+  d -= (D + D) / (dx * dx);
+  return d;
+}
+
+// Per-level constants. Divisions by dx, by the restriction's total volume and by the coarse
+// spacing become multiplications by their inverses where that gives the same bits (n a power
+// of two: the divisors are powers of two), and always in FP32, where the inner cycle only has
+// to reduce the error; the FP64 Jacobi keeps the reference's division by the diagonal.
+template <typename T>
+struct Lv {
+  T dx, idx, D, rD, tv, itv;
+  bool mul;
+};
+template <typename T>
+__device__ __forceinline__ Lv<T> MakeLv(int n) {
+  Lv<T> o;
+  o.dx = T(1) / T(n);
+  o.idx = T(n);
+  o.mul = sizeof(T) == 4 || (n & (n - 1)) == 0;
+  o.D = Diagonal(o.dx);
+  const T small = T(10) * (sizeof(T) == 8 ? T(DBL_MIN) : T(FLT_MIN));
+  o.rD = T(1) / (o.D + (o.D >= T(0) ? T(1) : T(-1)) * small);
+  const T vol = o.dx * o.dx * o.dx;
+  o.tv = ((vol + vol) + (vol + vol)) + ((vol + vol) + (vol + vol));
+  o.itv = T(1) / o.tv;
+  return o;
+}
+
 // 7-point operator at one cell in the reference's flux form (D = 1, alpha = 0); neighbours
 // across a domain face are the FixedFace-0 ghost -u.
 template <typename T>
-__device__ __forceinline__ T ApplyA(T u, T xm, T xp, T ym, T yp, T zm, T zp, T dx) {
+__device__ __forceinline__ T ApplyA(T u, T xm, T xp, T ym, T yp, T zm, T zp, const Lv<T> &o) {
   const T D = T(1), alpha = T(0);
   T out = -alpha * u;
+  if (o.mul) {  // D / dx == idx and x / dx == x * idx
+    const T f1l = o.idx * (xm - u);
+    const T f1r = o.idx * (u - xp);
+    out += (f1l - f1r) * o.idx;
+    const T f2l = D * (ym - u) * o.idx;
+    const T f2r = D * (u - yp) * o.idx;
+    out += (f2l - f2r) * o.idx;
+    const T f3l = D * (zm - u) * o.idx;
+    const T f3r = D * (u - zp) * o.idx;
+    out += (f3l - f3r) * o.idx;
+    return out;
+  }
+  const T dx = o.dx;
   const T f1l = D / dx * (xm - u);
   const T f1r = D / dx * (u - xp);
   out += (f1l - f1r) / dx;
@@ -55,15 +102,6 @@ __device__ __forceinline__ T ApplyA(T u, T xm, T xp, T ym, T yp, T zm, T zp, T d
   const T f3r = D * (u - zp) / dx;
   out += (f3l - f3r) / dx;
   return out;
-}
-
-template <typename T>
-__device__ __forceinline__ T Diagonal(T dx) {  // SetDiagonal (poisson_equation.hpp:86-125)
-  const T D = T(1), alpha = T(0);
-  T d = -(D + D) / (dx * dx) - alpha;
-  d -= (D + D) / (dx * dx);
-  d -= (D + D) / (dx * dx);
-  return d;
 }
 
 struct Cell {
@@ -98,19 +136,23 @@ __device__ __forceinline__ Nbrs<T> Load(const T *__restrict__ x, int c, const Ce
 }
 
 template <typename T>
-__device__ __forceinline__ T JacobiPoint(const Nbrs<T> &v, T b, T dx, T D, T w) {
-  const T off = ApplyA(v.u, v.xm, v.xp, v.ym, v.yp, v.zm, v.zp, dx) - D * v.u;
+__device__ __forceinline__ T JacobiPoint(const Nbrs<T> &v, T b, const Lv<T> &o, T w) {
+  const T off = ApplyA(v.u, v.xm, v.xp, v.ym, v.yp, v.zm, v.zp, o) - o.D * v.u;
   const T val = b - off;
-  return w * Ratio(val, D) + (T(1) - w) * v.u;
+  const T q = sizeof(T) == 4 ? val * o.rD : Ratio(val, o.D);
+  return w * q + (T(1) - w) * v.u;
 }
 
-// entry lookup: entries sorted by wg_begin, the last one ends at gridDim.x
+// entry lookup: entries sorted by wg_begin, the last one ends at gridDim.x. The logical
+// workgroup id is XCD-remapped, so each XCD walks contiguous rows and planes of a level and
+// finds the y and z neighbours of its rows in its own L2.
 template <typename E>
 __device__ __forceinline__ const E &Find(const E *__restrict__ e, int ne, int &local) {
+  const int w = static_cast<int>(XcdRemap(blockIdx.x, gridDim.x));
   int lo = 0;
   for (int q = 1; q < ne; ++q)
-    if (e[q].wg_begin <= static_cast<int>(blockIdx.x)) lo = q;
-  local = blockIdx.x - e[lo].wg_begin;
+    if (e[q].wg_begin <= w) lo = q;
+  local = w - e[lo].wg_begin;
   return e[lo];
 }
 
@@ -127,7 +169,7 @@ __global__ void __launch_bounds__(kWG) OuterKernel(const GmgEntry<T> *__restrict
   double rr = 0.0;
   if (c < n * n * n) {
     const Cell q = Decode(c, n);
-    const double dx = 1.0 / n;
+    const Lv<double> o = MakeLv<double>(n);
     const double *uo = e.u_old;
     Nbrs<double> v = Load(uo, c, q, n);
     if (e.x) {  // add the correction at the 7 points (the ghost follows the updated value)
@@ -141,7 +183,7 @@ __global__ void __launch_bounds__(kWG) OuterKernel(const GmgEntry<T> *__restrict
       v.zp = q.k < n - 1 ? v.zp + static_cast<double>(d.zp) : -v.u;
       e.u_new[c] = v.u;
     }
-    const double r = e.b64[c] - ApplyA(v.u, v.xm, v.xp, v.ym, v.yp, v.zm, v.zp, dx);
+    const double r = e.b64[c] - ApplyA(v.u, v.xm, v.xp, v.ym, v.yp, v.zm, v.zp, o);
     e.b[c] = static_cast<T>(r);
     rr = r * r;
   }
@@ -204,23 +246,23 @@ __global__ void __launch_bounds__(kWG)
   const int n = e.n, c = local * kWG + threadIdx.x;
   if (c >= n * n * n) return;
   const Cell q = Decode(c, n);
-  const T dx = T(1) / T(n), D = Diagonal(dx);
+  const Lv<T> o = MakeLv<T>(n);
   const Nbrs<T> zero = {T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
   const T *b = e.b;
   if (!PRE2) {
-    e.x[c] = JacobiPoint(zero, b[c], dx, D, w);
+    e.x[c] = JacobiPoint(zero, b[c], o, w);
     return;
   }
   const Nbrs<T> bb = Load(b, c, q, n);
   Nbrs<T> x1;
-  x1.u = JacobiPoint(zero, bb.u, dx, D, w);
-  x1.xm = q.i > 0 ? JacobiPoint(zero, bb.xm, dx, D, w) : -x1.u;
-  x1.xp = q.i < n - 1 ? JacobiPoint(zero, bb.xp, dx, D, w) : -x1.u;
-  x1.ym = q.j > 0 ? JacobiPoint(zero, bb.ym, dx, D, w) : -x1.u;
-  x1.yp = q.j < n - 1 ? JacobiPoint(zero, bb.yp, dx, D, w) : -x1.u;
-  x1.zm = q.k > 0 ? JacobiPoint(zero, bb.zm, dx, D, w) : -x1.u;
-  x1.zp = q.k < n - 1 ? JacobiPoint(zero, bb.zp, dx, D, w) : -x1.u;
-  e.x[c] = JacobiPoint(x1, bb.u, dx, D, w);
+  x1.u = JacobiPoint(zero, bb.u, o, w);
+  x1.xm = q.i > 0 ? JacobiPoint(zero, bb.xm, o, w) : -x1.u;
+  x1.xp = q.i < n - 1 ? JacobiPoint(zero, bb.xp, o, w) : -x1.u;
+  x1.ym = q.j > 0 ? JacobiPoint(zero, bb.ym, o, w) : -x1.u;
+  x1.yp = q.j < n - 1 ? JacobiPoint(zero, bb.yp, o, w) : -x1.u;
+  x1.zm = q.k > 0 ? JacobiPoint(zero, bb.zm, o, w) : -x1.u;
+  x1.zp = q.k < n - 1 ? JacobiPoint(zero, bb.zp, o, w) : -x1.u;
+  e.x[c] = JacobiPoint(x1, bb.u, o, w);
 }
 
 template <typename T>
@@ -240,16 +282,15 @@ __global__ void __launch_bounds__(kWG)
   const int n = e.n, c = local * kWG + threadIdx.x;
   if (c >= n * n * n) return;
   const Cell q = Decode(c, n);
-  const T dx = T(1) / T(n), D = Diagonal(dx);
-  e.t[c] = JacobiPoint(Load(static_cast<const T *>(e.x), c, q, n), e.b[c], dx, D, w);
+  const Lv<T> o = MakeLv<T>(n);
+  e.t[c] = JacobiPoint(Load(static_cast<const T *>(e.x), c, q, n), e.b[c], o, w);
 }
 
 template <typename T>
-__device__ __forceinline__ T Restrict8(const T t[2][2][2], T vol) {
-  const T tv = ((vol + vol) + (vol + vol)) + ((vol + vol) + (vol + vol));
-  return (((t[0][0][0] + t[0][1][0]) + (t[0][0][1] + t[0][1][1])) +
-          ((t[1][0][0] + t[1][1][0]) + (t[1][0][1] + t[1][1][1]))) /
-         tv;
+__device__ __forceinline__ T Restrict8(const T t[2][2][2], const Lv<T> &o) {
+  const T sum = ((t[0][0][0] + t[0][1][0]) + (t[0][0][1] + t[0][1][1])) +
+                ((t[1][0][0] + t[1][1][0]) + (t[1][0][1] + t[1][1][1]));
+  return o.mul ? sum * o.itv : sum / o.tv;
 }
 
 // residual of the 8 children of one coarse cell and their average (RestrictAverage) into the
@@ -262,7 +303,8 @@ __global__ void __launch_bounds__(kWG)
   const int nc = e.nc, n = e.n, cc = local * kWG + threadIdx.x;
   if (cc >= nc * nc * nc) return;
   const Cell p = Decode(cc, nc);
-  const T dx = T(1) / T(n), vol = dx * dx * dx;
+  const Lv<T> o = MakeLv<T>(n);
+  const T vol = o.dx * o.dx * o.dx;
   T t[2][2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
@@ -273,10 +315,10 @@ __global__ void __launch_bounds__(kWG)
         const Cell q = {2 * p.i + c2, 2 * p.j + bb, 2 * p.k + a};
         const int c = (q.k * n + q.j) * n + q.i;
         const Nbrs<T> v = Load(static_cast<const T *>(e.x), c, q, n);
-        const T r = e.b[c] - ApplyA(v.u, v.xm, v.xp, v.ym, v.yp, v.zm, v.zp, dx);
+        const T r = e.b[c] - ApplyA(v.u, v.xm, v.xp, v.ym, v.yp, v.zm, v.zp, o);
         t[a][bb][c2] = vol * r;
       }
-  e.bc[cc] = Restrict8(t, vol);
+  e.bc[cc] = Restrict8(t, o);
 }
 
 // x += ProlongateSharedLinear(coarse x); one thread per fine cell, coarse ghosts from the
@@ -293,28 +335,30 @@ __global__ void __launch_bounds__(kWG) ProlongKernel(const GmgEntry<T> *__restri
   const T *xc = e.xc;
   const int cc = (p.k * nc + p.j) * nc + p.i;
   const T fc = xc[cc];
-  const T hc = T(1) / T(nc), df = T(0.25) * hc;
+  const Lv<T> oc = MakeLv<T>(nc);
+  const T hc = oc.dx, df = T(0.25) * hc;
+  auto grad = [&](T diff) { return oc.mul ? diff * oc.idx : diff / hc; };
   T g1 = 0, g2 = 0, g3 = 0;  // the one-sided gradient on this child's side
   if (oi == 0) {
     const T fm = p.i > 0 ? xc[cc - 1] : -fc;
-    g1 = (fc - fm) / hc;
+    g1 = grad(fc - fm);
   } else {
     const T fp = p.i < nc - 1 ? xc[cc + 1] : -fc;
-    g1 = (fp - fc) / hc;
+    g1 = grad(fp - fc);
   }
   if (oj == 0) {
     const T fm = p.j > 0 ? xc[cc - nc] : -fc;
-    g2 = (fc - fm) / hc;
+    g2 = grad(fc - fm);
   } else {
     const T fp = p.j < nc - 1 ? xc[cc + nc] : -fc;
-    g2 = (fp - fc) / hc;
+    g2 = grad(fp - fc);
   }
   if (ok == 0) {
     const T fm = p.k > 0 ? xc[cc - nc * nc] : -fc;
-    g3 = (fc - fm) / hc;
+    g3 = grad(fc - fm);
   } else {
     const T fp = p.k < nc - 1 ? xc[cc + nc * nc] : -fc;
-    g3 = (fp - fc) / hc;
+    g3 = grad(fp - fc);
   }
   T v;
   switch ((ok << 2) | (oj << 1) | oi) {
@@ -334,11 +378,11 @@ __global__ void __launch_bounds__(kWG) ProlongKernel(const GmgEntry<T> *__restri
 template <typename T>
 __device__ void BSweep(T *x, T *t, const T *b, int n, T w, bool zero) {
   const int N3 = n * n * n;
-  const T dx = T(1) / T(n), D = Diagonal(dx);
+  const Lv<T> o = MakeLv<T>(n);
   const Nbrs<T> zn = {T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
   for (int c = threadIdx.x; c < N3; c += blockDim.x) {
     const Cell q = Decode(c, n);
-    t[c] = JacobiPoint(zero ? zn : Load(static_cast<const T *>(x), c, q, n), b[c], dx, D, w);
+    t[c] = JacobiPoint(zero ? zn : Load(static_cast<const T *>(x), c, q, n), b[c], o, w);
   }
   __syncthreads();
   for (int c = threadIdx.x; c < N3; c += blockDim.x) x[c] = t[c];
@@ -348,7 +392,8 @@ __device__ void BSweep(T *x, T *t, const T *b, int n, T w, bool zero) {
 template <typename T>
 __device__ void BRestrict(const T *x, const T *b, T *bc, T *xc, int n) {
   const int nc = n / 2, N3 = nc * nc * nc;
-  const T dx = T(1) / T(n), vol = dx * dx * dx;
+  const Lv<T> o = MakeLv<T>(n);
+  const T vol = o.dx * o.dx * o.dx;
   for (int cc = threadIdx.x; cc < N3; cc += blockDim.x) {
     const Cell p = Decode(cc, nc);
     T t[2][2][2];
@@ -358,9 +403,9 @@ __device__ void BRestrict(const T *x, const T *b, T *bc, T *xc, int n) {
           const Cell q = {2 * p.i + c2, 2 * p.j + bb, 2 * p.k + a};
           const int c = (q.k * n + q.j) * n + q.i;
           const Nbrs<T> v = Load(x, c, q, n);
-          t[a][bb][c2] = vol * (b[c] - ApplyA(v.u, v.xm, v.xp, v.ym, v.yp, v.zm, v.zp, dx));
+          t[a][bb][c2] = vol * (b[c] - ApplyA(v.u, v.xm, v.xp, v.ym, v.yp, v.zm, v.zp, o));
         }
-    bc[cc] = Restrict8(t, vol);
+    bc[cc] = Restrict8(t, o);
     xc[cc] = T(0);
   }
   __syncthreads();
@@ -369,19 +414,21 @@ __device__ void BRestrict(const T *x, const T *b, T *bc, T *xc, int n) {
 template <typename T>
 __device__ void BProlong(T *x, const T *xc, int n) {
   const int nc = n / 2, N3 = n * n * n;
-  const T hc = T(1) / T(nc), df = T(0.25) * hc;
+  const Lv<T> oc = MakeLv<T>(nc);
+  const T hc = oc.dx, df = T(0.25) * hc;
+  auto grad = [&](T diff) { return oc.mul ? diff * oc.idx : diff / hc; };
   for (int c = threadIdx.x; c < N3; c += blockDim.x) {
     const Cell q = Decode(c, n);
     const Cell p = {q.i >> 1, q.j >> 1, q.k >> 1};
     const int oi = q.i & 1, oj = q.j & 1, ok = q.k & 1;
     const int cc = (p.k * nc + p.j) * nc + p.i;
     const T fc = xc[cc];
-    const T g1 = oi == 0 ? (fc - (p.i > 0 ? xc[cc - 1] : -fc)) / hc
-                         : ((p.i < nc - 1 ? xc[cc + 1] : -fc) - fc) / hc;
-    const T g2 = oj == 0 ? (fc - (p.j > 0 ? xc[cc - nc] : -fc)) / hc
-                         : ((p.j < nc - 1 ? xc[cc + nc] : -fc) - fc) / hc;
-    const T g3 = ok == 0 ? (fc - (p.k > 0 ? xc[cc - nc * nc] : -fc)) / hc
-                         : ((p.k < nc - 1 ? xc[cc + nc * nc] : -fc) - fc) / hc;
+    const T g1 = oi == 0 ? grad(fc - (p.i > 0 ? xc[cc - 1] : -fc))
+                         : grad((p.i < nc - 1 ? xc[cc + 1] : -fc) - fc);
+    const T g2 = oj == 0 ? grad(fc - (p.j > 0 ? xc[cc - nc] : -fc))
+                         : grad((p.j < nc - 1 ? xc[cc + nc] : -fc) - fc);
+    const T g3 = ok == 0 ? grad(fc - (p.k > 0 ? xc[cc - nc * nc] : -fc))
+                         : grad((p.k < nc - 1 ? xc[cc + nc * nc] : -fc) - fc);
     T v;
     switch ((ok << 2) | (oj << 1) | oi) {
       case 0: v = fc - (g1 * df + g2 * df + g3 * df); break;
@@ -415,12 +462,17 @@ __global__ void __launch_bounds__(1024)
     if (n % 2 != 0 || n == 1) break;
   }
   const int N3 = e.n * e.n * e.n;
-  for (int c = threadIdx.x; c < N3; c += blockDim.x) sb[c] = e.b[c];
+  const bool warm = e.warm != 0;
+  for (int c = threadIdx.x; c < N3; c += blockDim.x) {
+    sb[c] = e.b[c];
+    if (warm) sx[c] = e.x[c];
+  }
   __syncthreads();
   for (int l = 0; l < nl; ++l) {
     T *x = sx + off[l], *t = st + off[l], *b = sb + off[l];
-    for (int s = 0; s < pre; ++s) BSweep(x, t, b, nn[l], w, s == 0);
-    if (pre == 0) {
+    const bool from_x = warm && l == 0;
+    for (int s = 0; s < pre; ++s) BSweep(x, t, b, nn[l], w, s == 0 && !from_x);
+    if (pre == 0 && !from_x) {
       for (int c = threadIdx.x; c < nn[l] * nn[l] * nn[l]; c += blockDim.x) x[c] = T(0);
       __syncthreads();
     }
@@ -675,7 +727,7 @@ pth_status GmgBatch<T>::Residual(const std::vector<GmgProblem<T> *> &ps,
 // every problem is one launch. The level arrays' x / t roles follow the Jacobi ping-pong.
 template <typename T>
 pth_status GmgBatch<T>::Cycle(const std::vector<GmgProblem<T> *> &ps, double omega, int pre,
-                              int post) {
+                              int post, bool warm) {
   const T w = static_cast<T>(omega);
   int L = 0;
   for (auto *p : ps) L = std::max(L, p->nbottom);
@@ -699,7 +751,10 @@ pth_status GmgBatch<T>::Cycle(const std::vector<GmgProblem<T> *> &ps, double ome
   for (int l = 0; l < L; ++l) {
     GmgEntry<T> *d;
     int ne, nwg;
-    if (pre > 0) {
+    if (warm && l == 0) {
+      for (int s = 0; s < pre; ++s)
+        if (sweep(l) != PTH_COMPLETE) return PTH_FAIL;
+    } else if (pre > 0) {
       if (Stage(entries(l), false, &d, &ne, &nwg) != PTH_COMPLETE) return PTH_FAIL;
       ProfScope prof(l == 0 ? "gmg_pre_l0" : "gmg_pre", stream);
       if (pre >= 2)
@@ -714,8 +769,9 @@ pth_status GmgBatch<T>::Cycle(const std::vector<GmgProblem<T> *> &ps, double ome
       hipLaunchKernelGGL(ZeroKernel<T>, dim3(nwg), dim3(kWG), 0, stream, d, ne);
       PTH_HIP(hipGetLastError());
     }
-    for (int s = 2; s < pre; ++s)
-      if (sweep(l) != PTH_COMPLETE) return PTH_FAIL;
+    if (!(warm && l == 0))
+      for (int s = 2; s < pre; ++s)
+        if (sweep(l) != PTH_COMPLETE) return PTH_FAIL;
     if (Stage(entries(l), true, &d, &ne, &nwg) != PTH_COMPLETE) return PTH_FAIL;
     ProfScope prof(l == 0 ? "gmg_restrict_l0" : "gmg_restrict", stream);
     hipLaunchKernelGGL(ResidualRestrictKernel<T>, dim3(nwg), dim3(kWG), 0, stream, d, ne);
@@ -723,7 +779,10 @@ pth_status GmgBatch<T>::Cycle(const std::vector<GmgProblem<T> *> &ps, double ome
   }
   {
     std::vector<GmgEntry<T>> es;
-    for (auto *p : ps) es.push_back(p->Entry(p->nbottom));
+    for (auto *p : ps) {
+      es.push_back(p->Entry(p->nbottom));
+      es.back().warm = warm && p->nbottom == 0;
+    }
     GmgEntry<T> *d;
     int ne, nwg;
     if (Stage(es, false, &d, &ne, &nwg) != PTH_COMPLETE) return PTH_FAIL;

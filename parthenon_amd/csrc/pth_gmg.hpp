@@ -17,7 +17,7 @@ struct GmgEntry {
   double *partial;        // per-workgroup partial sums of the outer step
   int n, nc;              // cells per side on this and the next coarser level
   int wg_begin;           // first workgroup of this entry in the batched grid
-  int pad;
+  int warm;               // bottom solve: start from x instead of 0
 };
 
 // largest level handled inside one workgroup in LDS (three n^3 arrays of T plus the coarser
@@ -67,7 +67,10 @@ struct GmgBatch {
   pth_status RhsNorms(const std::vector<GmgProblem<T> *> &ps, double *out);
   pth_status Residual(const std::vector<GmgProblem<T> *> &ps, const std::vector<char> &correct,
                       double *out);
-  pth_status Cycle(const std::vector<GmgProblem<T> *> &ps, double omega, int pre, int post);
+  // warm: the finest level starts from its current x (a further inner cycle on the same
+  // right-hand side) instead of 0
+  pth_status Cycle(const std::vector<GmgProblem<T> *> &ps, double omega, int pre, int post,
+                   bool warm = false);
   pth_status SetBlobs(GmgProblem<T> &p, std::uint64_t seed);
   pth_status Reset(GmgProblem<T> &p);
 };

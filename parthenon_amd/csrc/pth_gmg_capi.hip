@@ -30,6 +30,7 @@ struct Params {
   int precision;  // 0: FP32 inner cycle (mixed), 1: FP64
   double omega;
   int pre, post;
+  int inner = 1;  // V-cycles per FP64 refinement step
 };
 
 template <typename T>
@@ -61,7 +62,8 @@ struct Solver1 {
       const double r = bnorm > 0 ? rn / bnorm : rn;
       history.push_back(r);
       if (r < tol || it >= maxit) break;
-      if (batch.Cycle(ps, P.omega, P.pre, P.post) != PTH_COMPLETE) return PTH_FAIL;
+      for (int c = 0; c < P.inner; ++c)
+        if (batch.Cycle(ps, P.omega, P.pre, P.post, c > 0) != PTH_COMPLETE) return PTH_FAIL;
       corr[0] = 1;
     }
     if (iters) *iters = it;
@@ -235,8 +237,8 @@ struct Server {
         keep_act.push_back(a);
         cyc.push_back(a.p);
       }
-      if (!cyc.empty() && batch.Cycle(cyc, P.omega, P.pre, P.post) != PTH_COMPLETE)
-        return PTH_FAIL;
+      for (int c = 0; c < P.inner && !cyc.empty(); ++c)
+        if (batch.Cycle(cyc, P.omega, P.pre, P.post, c > 0) != PTH_COMPLETE) return PTH_FAIL;
       max_batch = std::max(max_batch, static_cast<int>(act.size()));
       act.swap(keep_act);
       ++iterations;
@@ -349,6 +351,15 @@ pth_status pth_gmg_reset(pth_gmg *h) {
   });
 }
 
+pth_status pth_gmg_set_inner_cycles(pth_gmg *h, int k) {
+  if (k < 1) {
+    SetError("pth_gmg_set_inner_cycles: k must be >= 1");
+    return PTH_FAIL;
+  }
+  h->P.inner = k;
+  return PTH_COMPLETE;
+}
+
 pth_status pth_gmg_solve(pth_gmg *h, double tol, int max_iters, int *iters, double *final_rel) {
   return h->Do([&](auto &S) { return S.Solve(h->P, tol, max_iters, iters, final_rel); });
 }
@@ -408,6 +419,17 @@ pth_status pth_gmg_server_submit(pth_gmg_server *h, int n, uint64_t seed, double
     r.arrival = arrival_s;
     S.reqs.push_back(std::move(r));
     if (id) *id = S.reqs.back().id;
+    return PTH_COMPLETE;
+  });
+}
+
+pth_status pth_gmg_server_set_inner_cycles(pth_gmg_server *h, int k) {
+  if (k < 1) {
+    SetError("pth_gmg_server_set_inner_cycles: k must be >= 1");
+    return PTH_FAIL;
+  }
+  return h->Do([&](auto &S) {
+    S.P.inner = k;
     return PTH_COMPLETE;
   });
 }

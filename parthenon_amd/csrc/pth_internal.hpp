@@ -267,6 +267,9 @@ class Mesh {
   double *dtvar = nullptr;
   hipStream_t run_stream = nullptr;
   hipEvent_t order_ev[2] = {nullptr, nullptr};  // mesh stream <-> driver stream ordering
+  hipStream_t side_stream = nullptr;            // driver: the exchange beside the stencils
+  hipEvent_t fork_ev[2] = {nullptr, nullptr};
+  int run_overlap = 1;                          // driver: overlap exchange and stencils
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   double run_cfl = 0.0;

@@ -23,6 +23,7 @@ class MGParams:  # src/solvers/mg_solver.hpp:37-58, with the smoother weight exp
     omega: float = 2.0 / 3.0
     pre_sweeps: int = 2
     post_sweeps: int = 2
+    inner_cycles: int = 1               # V-cycles per FP64 refinement step
 
 
 class MGSolver:
@@ -35,6 +36,7 @@ class MGSolver:
             raise RuntimeError("pth_gmg_create: " + lib().pth_last_error().decode())
         self._h = h
         self.levels = lib().pth_gmg_levels(h)
+        _check(lib().pth_gmg_set_inner_cycles(h, p.inner_cycles), "set_inner_cycles")
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -85,6 +87,7 @@ class Server:
             raise RuntimeError("pth_gmg_server_create: " + lib().pth_last_error().decode())
         self._h = h
         self.sizes = {}
+        _check(lib().pth_gmg_server_set_inner_cycles(h, p.inner_cycles), "set_inner_cycles")
 
     def __del__(self):
         if getattr(self, "_h", None):

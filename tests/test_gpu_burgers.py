@@ -77,16 +77,18 @@ def test_ten_cycles_match_oracle(pgen):
     assert abs(d.time - lib().orc_burgers_time(ob.h)) <= RTOL * d.time
 
 
+@pytest.mark.parametrize("overlap", [True, False], ids=["overlap", "serial"])
 @pytest.mark.parametrize("use_graph", [False, True], ids=["eager", "graph"])
-def test_device_driver_matches_oracle_every_cycle(use_graph):
+def test_device_driver_matches_oracle_every_cycle(use_graph, overlap):
     # fused stage launches, device-resident clock; with the graph, one captured cycle replayed,
-    # with host downloads (eager work) between the replays, each replay checked
-    print(f"case device driver graph={use_graph}", flush=True)
+    # with host downloads (eager work) between the replays, each replay checked; with overlap,
+    # each stage's exchange runs on a second stream beside the ghost-free reconstruction
+    print(f"case device driver graph={use_graph} overlap={overlap}", flush=True)
     from parthenon_amd.burgers import DeviceBurgersDriver
     from oracle_lib import lib
     nx, block, g, ns = (64, 64, 64), (16, 16, 16), 4, 4
     m = make_burgers_mesh(nx, block, g, ns)
-    d = DeviceBurgersDriver(m, cfl=0.4, use_graph=use_graph)
+    d = DeviceBurgersDriver(m, cfl=0.4, use_graph=use_graph, overlap=overlap)
     d.initialize("sinusoid", seed=3)
     ob = OracleBurgers(OracleMesh(nx, block, g), ns, 0.4, 0)
     lib().orc_burgers_pgen_sinusoid(ob.h, 3)
@@ -187,18 +189,20 @@ def test_multilevel_task_driver_matches_oracle(name):
         assert d.dt == ob.dt
 
 
+@pytest.mark.parametrize("overlap", [True, False], ids=["overlap", "serial"])
 @pytest.mark.parametrize("use_graph", [False, True], ids=["eager", "graph"])
 @pytest.mark.parametrize("name", ["box120", "sphere960"])
-def test_multilevel_device_driver_matches_oracle(name, use_graph):
-    # fused stage with the flux-correction launch between reconstruction and update
+def test_multilevel_device_driver_matches_oracle(name, use_graph, overlap):
+    # fused stage with the flux-correction launches between reconstruction and update
     spec = REFINED[name]
-    print(f"case multilevel device driver {name} graph={use_graph}", flush=True)
+    print(f"case multilevel device driver {name} graph={use_graph} overlap={overlap}",
+          flush=True)
     from parthenon_amd.burgers import DeviceBurgersDriver
     from oracle_lib import lib
     m = make_burgers_mesh(spec["nx"], spec["block"], 4, 4,
                           refine_boxes=spec.get("refine_boxes", ()),
                           refine_spheres=spec.get("refine_spheres", ()))
-    d = DeviceBurgersDriver(m, cfl=0.4, use_graph=use_graph)
+    d = DeviceBurgersDriver(m, cfl=0.4, use_graph=use_graph, overlap=overlap)
     d.initialize("sinusoid", seed=2)
     om = OracleMesh(spec["nx"], spec["block"], 4, boxes=spec.get("refine_boxes", ()),
                     spheres=spec.get("refine_spheres", ()))
