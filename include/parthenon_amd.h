@@ -173,6 +173,11 @@ pth_status pth_burgers_clock(pth_mesh *m, double *dt, double *time);
 /* Waits for the driver's stream. */
 pth_status pth_burgers_synchronize(pth_mesh *m);
 
+/* History: the 8 octant masses sum(U^2 * cell volume / mesh volume) of the package's history
+ * variables "MS Mass 0..7" (burgers_package.cpp:96-135, MassHistory :406-439), octants in the
+ * reference order (x outer, then y, then z); synchronous. */
+pth_status pth_burgers_mass_history(pth_meshdata *md, double out[8]);
+
 /* Problem generators for "U" of md (host-evaluated, uploaded): the seeded sum of sinusoids
  * and the reference's tanh/exp profile (benchmarks/burgers/parthenon_app_inputs.cpp:34-79). */
 pth_status pth_burgers_pgen_sinusoid(pth_meshdata *md, uint64_t seed);

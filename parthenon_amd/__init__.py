@@ -97,6 +97,7 @@ _SIGNATURES = {
     "pth_burgers_estimate_timestep": (_I, [_P, _D, _DP]),
     "pth_burgers_pgen_sinusoid": (_I, [_P, C.c_uint64]),
     "pth_burgers_pgen_vibe": (_I, [_P]),
+    "pth_burgers_mass_history": (_I, [_P, _DP]),
     "pth_exchange_fused": (_I, [_P, _I]),
     "pth_burgers_init": (_I, [_P, _D, _I]),
     "pth_burgers_run_cycles": (_I, [_P, _I, _I]),
@@ -359,6 +360,13 @@ class burgers_package:  # benchmarks/burgers/burgers_package.hpp
     @staticmethod
     def ProblemGenerator(md):
         return _check(lib().pth_burgers_pgen_vibe(md._h), "pgen_vibe")
+
+    @staticmethod
+    def MassHistory(md):
+        """The 8 octant masses (history variables "MS Mass 0..7")."""
+        out = (_D * 8)()
+        _check(lib().pth_burgers_mass_history(md._h, out), "MassHistory")
+        return list(out)
 
 
 class Update:  # src/interface/update.hpp

@@ -810,6 +810,68 @@ __global__ void __launch_bounds__(64)
   for (int n = 0; n < nc; ++n) F[n * cs] = store[src + static_cast<std::int64_t>(n) * P.comp];
 }
 
+// burgers_package::MassHistory (benchmarks/burgers/burgers_package.cpp:406-439) for the 8
+// octants of the mesh at once: sum over interior cells and components of mask * q * q * w,
+// w = cell volume / (mesh volume + 1e-20), mask from the inclusive cell-centre test.
+// Per-workgroup partials [workgroup][8]; the order of the sum is this build's own (the
+// reference's Kokkos::Sum order is not fixed either).
+struct Octants {
+  double lo[8][3], hi[8][3];
+};
+__global__ void __launch_bounds__(kWG)
+    MassHistoryKernel(const double *__restrict__ U, const DeviceBlockGeom *__restrict__ geom,
+                      const BlockCoefs *__restrict__ coef, Geo G, int nblocks, Octants O,
+                      double mesh_vol, double *__restrict__ partial) {
+  __shared__ double acc[kWG / 64][8];
+  const int n0 = G.hi0 - G.lo0 + 1, n1 = G.hi1 - G.lo1 + 1, n2 = G.hi2 - G.lo2 + 1;
+  const std::int64_t per = static_cast<std::int64_t>(n0) * n1 * n2;
+  const std::int64_t total = per * G.V * nblocks;
+  double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * kWG + threadIdx.x; t < total;
+       t += static_cast<std::int64_t>(gridDim.x) * kWG) {
+    std::int64_t r = t;
+    const int i = static_cast<int>(r % n0) + G.lo0;
+    r /= n0;
+    const int j = static_cast<int>(r % n1) + G.lo1;
+    r /= n1;
+    const int k = static_cast<int>(r % n2) + G.lo2;
+    r /= n2;  // block * V + comp
+    const int b = static_cast<int>(r / G.V);
+    const DeviceBlockGeom &g = geom[b];
+    const double w = coef[b].vol / (mesh_vol + 1e-20);
+    const double x1 = g.x0[0] + (i + 0.5) * g.dx[0];
+    const double x2 = g.x0[1] + (j + 0.5) * g.dx[1];
+    const double x3 = g.x0[2] + (k + 0.5) * g.dx[2];
+    const double q = U[r * G.cells + (static_cast<std::int64_t>(k) * G.e1 + j) * G.e0 + i];
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      const double mask = (O.lo[o][0] <= x1) && (x1 <= O.hi[o][0]) && (O.lo[o][1] <= x2) &&
+                          (x2 <= O.hi[o][1]) && (O.lo[o][2] <= x3) && (x3 <= O.hi[o][2]);
+      s[o] += mask * q * q * w;
+    }
+  }
+#pragma unroll
+  for (int o = 0; o < 8; ++o) {
+    double v = s[o];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) acc[threadIdx.x >> 6][o] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    double v = 0.0;
+    for (int w2 = 0; w2 < kWG / 64; ++w2) v += acc[w2][threadIdx.x];
+    partial[8 * blockIdx.x + threadIdx.x] = v;
+  }
+}
+
+__global__ void __launch_bounds__(64)
+    SumPartials8Kernel(const double *__restrict__ partial, int n, double *__restrict__ out) {
+  if (threadIdx.x >= 8) return;
+  double v = 0.0;
+  for (int q = 0; q < n; ++q) v += partial[8 * q + threadIdx.x];
+  out[threadIdx.x] = v;
+}
+
 // Final CFL minimum and EvolutionDriver::SetGlobalTimeStep on device (driver.cpp:210-235):
 // dtvar = {dt, time, estimate}; time += dt (when advance), dt = min(2 dt, cfl * min).
 // One workgroup of kClockWG threads: strided partial minima, wavefront shuffles, then LDS.
@@ -1149,6 +1211,41 @@ pth_status BurgersFluxUpdate(const Topology &t, int V, const double *flux, const
     PTH_FU_CASE(16)
   }
 #undef PTH_FU_CASE
+  PTH_HIP(hipGetLastError());
+  return PTH_COMPLETE;
+}
+
+pth_status BurgersMassHistory(const Topology &t, int V, const double *U,
+                              const DeviceBlockGeom *geom, const BlockCoefs *coef,
+                              const double mesh_lo[3], const double mesh_hi[3], double *scratch,
+                              int nscratch, double *out, hipStream_t s) {
+  ProfScope prof("mass_history", s);
+  // octants in the reference order (burgers_package.cpp:104-119): x outer, then y, then z
+  Octants O;
+  double mid[3], vol = 1.0;
+  for (int d = 0; d < 3; ++d) {
+    vol *= (mesh_hi[d] - mesh_lo[d]);
+    mid[d] = 0.5 * (mesh_lo[d] + mesh_hi[d]);
+  }
+  int o = 0;
+  for (int s1 = 0; s1 < 2; ++s1)
+    for (int s2 = 0; s2 < 2; ++s2)
+      for (int s3 = 0; s3 < 2; ++s3, ++o) {
+        const int side[3] = {s1, s2, s3};
+        for (int d = 0; d < 3; ++d) {
+          O.lo[o][d] = side[d] ? mid[d] : mesh_lo[d];
+          O.hi[o][d] = side[d] ? mesh_hi[d] : mid[d];
+        }
+      }
+  const Geo G = MakeGeo(t, V);
+  const int nb = static_cast<int>(t.blocks.size());
+  const std::int64_t total =
+      static_cast<std::int64_t>(t.fine.n[0]) * t.fine.n[1] * t.fine.n[2] * V * nb;
+  int grid = Grid(total);
+  if (grid > nscratch / 8) grid = nscratch / 8;
+  hipLaunchKernelGGL(MassHistoryKernel, dim3(grid), dim3(kWG), 0, s, U, geom, coef, G, nb, O,
+                     vol, scratch);
+  hipLaunchKernelGGL(SumPartials8Kernel, dim3(1), dim3(64), 0, s, scratch, grid, out);
   PTH_HIP(hipGetLastError());
   return PTH_COMPLETE;
 }

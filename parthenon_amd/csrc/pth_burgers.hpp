@@ -43,6 +43,11 @@ pth_status BurgersFluxUpdate(const Topology &t, int V, const double *flux, const
                              hipStream_t s);
 pth_status BurgersClock(const double *partial, int n, double cfl, double *dtvar, int advance,
                         hipStream_t s);
+// octant sums of U^2 * volume / mesh volume (8 values into out, device memory)
+pth_status BurgersMassHistory(const Topology &t, int V, const double *U,
+                              const DeviceBlockGeom *geom, const BlockCoefs *coef,
+                              const double mesh_lo[3], const double mesh_hi[3], double *scratch,
+                              int nscratch, double *out, hipStream_t s);
 
 pth_status LaunchRestrictT(const Table &t, const Topology &topo, const DeviceBlockGeom *geom,
                            hipStream_t s);

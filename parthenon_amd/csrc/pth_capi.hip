@@ -1061,6 +1061,25 @@ pth_status pth_burgers_estimate_timestep(pth_meshdata *mdh, double cfl, double *
   return PTH_COMPLETE;
 }
 
+// History: the 8 octant masses of burgers_package (burgers_package.cpp:96-135, 406-439)
+pth_status pth_burgers_mass_history(pth_meshdata *mdh, double out[8]) {
+  MeshData *md = Impl(mdh);
+  Mesh *m = md->mesh;
+  const int fu = NeedField(md, "U", 3);
+  if (fu < 0) return PTH_FAIL;
+  // partials in the first scratch slots, the 8 results after them
+  const int nscr = static_cast<int>(std::min<std::int64_t>(m->scratch_n - 8, 8 * 2048));
+  if (BurgersMassHistory(m->topo, m->fields[fu].ncomp, md->store[fu].data, m->geom, m->coef,
+                         m->topo.xmin, m->topo.xmax, m->scratch, nscr, m->scratch + nscr,
+                         m->stream) != PTH_COMPLETE)
+    return PTH_FAIL;
+  PTH_HIP(hipMemcpyAsync(m->pinned, m->scratch + nscr, sizeof(double) * 8,
+                         hipMemcpyDeviceToHost, m->stream));
+  PTH_HIP(hipStreamSynchronize(m->stream));
+  for (int q = 0; q < 8; ++q) out[q] = m->pinned[q];
+  return PTH_COMPLETE;
+}
+
 static pth_status PgenUpload(MeshData *md, int f, const std::vector<double> &h) {
   Mesh *m = md->mesh;
   PTH_HIP(hipMemcpy(md->store[f].data, h.data(), sizeof(double) * h.size(),

@@ -263,3 +263,46 @@ def test_multilevel_conservation_to_roundoff():
     t1 = totals()
     assert np.all(np.isfinite(t1))
     assert np.max(np.abs(t1 - t0)) < 1e-14, (t0, t1)
+
+
+def test_history_octant_masses_match_oracle(tmp_path):
+    # MassHistory on the refined mesh after a few cycles, written in the .hst format and
+    # compared with the oracle's octant sums (Kokkos::Sum order is not fixed: 1e-12 relative)
+    print("case history octant masses", flush=True)
+    import os
+    from oracle_lib import lib
+    from parthenon_amd.burgers import DeviceBurgersDriver
+    from parthenon_amd.history import HistoryOutput, rel_diff
+    import ctypes as C
+    spec = REFINED["box120"]
+    m = make_burgers_mesh(spec["nx"], spec["block"], 4, 4, refine_boxes=spec["refine_boxes"])
+    d = DeviceBurgersDriver(m, cfl=0.4)
+    d.initialize("sinusoid", seed=6)
+    ob = OracleBurgers(OracleMesh(spec["nx"], spec["block"], 4, boxes=spec["refine_boxes"]),
+                       4, 0.4, 0)
+    lib().orc_burgers_pgen_sinusoid(ob.h, 6)
+    lib().orc_burgers_initialize(ob.h)
+    hst = HistoryOutput(basename=str(tmp_path / "burgers"))
+    mid, lo, hi = 0.0, -0.5, 0.5
+    rows_o = []
+    for c in range(3):
+        d.cycles(1)
+        lib().orc_burgers_cycles(ob.h, 1)
+        dt, t = d.clock()
+        got = hst.write(d.base, t, dt, c + 1, m.nblocks)
+        exp = []
+        for s1 in (0, 1):
+            for s2 in (0, 1):
+                for s3 in (0, 1):
+                    xmn = [mid if s else lo for s in (s1, s2, s3)]
+                    xmx = [hi if s else mid for s in (s1, s2, s3)]
+                    exp.append(lib().orc_burgers_mass_history(
+                        ob.h, (C.c_double * 3)(*xmn), (C.c_double * 3)(*xmx), 1.0))
+        rows_o.append([t, dt, c + 1, m.nblocks] + exp)
+        assert np.max(rel_diff(got, exp)) < 1e-12, (got, exp)
+    table = np.loadtxt(hst.fname)
+    assert table.shape == (3, 12)
+    # the written table agrees with the oracle's rows at the printed precision
+    assert np.max(rel_diff(table, np.array(rows_o))) < 1e-4
+    with open(hst.fname) as f:
+        assert f.readline().startswith("#  History data")
