@@ -43,7 +43,17 @@ def kernel_bytes(nb, n, g, V, exch_elems):
     F = (n + 1) * n * n
     RU = I + 18 * n * n
     recon = 8 * V * nb * (RU + 6 * (n + 2) * n * n)
+    # plane kernels: part 0 reads every cell its stencils touch (interior + 3 ghost layers)
+    # and writes the 3 face-flux arrays; the ghost-free part reads the interior only and writes
+    # the faces between cells 3 .. n - 2 of each row; the boundary part reads the cells within
+    # 5 of a block face (plus the ghosts) and writes the 6 faces per row next to the boundary
+    inner_faces = max(n - 5, 0) * n * n
+    outer_cells = I - max(n - 10, 0) ** 3 + 18 * n * n
     return {
+        "recon_flux": 8 * V * nb * (RU + 3 * F),
+        "recon_flux_inner": 8 * V * nb * (I + 3 * inner_faces),
+        "recon_flux_outer": 8 * V * nb * (outer_cells + 3 * 6 * n * n),
+        "flux_update": 8 * nb * (3 * V * F + 3 * V * I + I),
         "recon": recon,
         "recon_tasks": recon,
         "fused_update": 8 * nb * (6 * V * F + 3 * V * I + I),

@@ -42,11 +42,23 @@ void *orc_mesh_new(const int *nx, const int *bnx, int nghost, const double *xmin
     d.xmin[k] = xmin[k];
     d.xmax[k] = xmax[k];
     d.periodic[k] = periodic[k];
+    d.bc[2 * k] = d.bc[2 * k + 1] = periodic[k] ? kBcPeriodic : kBcOutflow;
   }
   d.nghost = nghost;
   auto *o = new OMesh;
   o->m.reset(new Mesh(d));
   return o;
+}
+
+// Boundary condition of a domain face (0 inner x1, 1 outer x1, ..., 5 outer x3): kBcOutflow or
+// kBcReflect on a non-periodic dimension (its faces default to outflow). Returns 0, or -1 if
+// the face belongs to a periodic dimension or the type is unknown.
+int orc_mesh_set_boundary(void *h, int face, int type) {
+  Mesh &m = *static_cast<OMesh *>(h)->m;
+  if (face < 0 || face >= 6 || m.d.periodic[face / 2]) return -1;
+  if (type != kBcOutflow && type != kBcReflect) return -1;
+  m.d.bc[face] = type;
+  return 0;
 }
 
 void orc_mesh_refine_box(void *h, const double *box, int level) {
@@ -186,6 +198,50 @@ void orc_exchange(void *h, int nfields, const int *ncomp, double **data, double 
     std::memcpy(data[f], fields[f].data.data(), sizeof(double) * nb * ncomp[f] * E);
     if (m.multilevel && coarse[f])
       std::memcpy(coarse[f], fields[f].coarse.data(), sizeof(double) * nb * ncomp[f] * cE);
+  }
+}
+
+// The boundary exchange with physical boundary conditions (Exchange::WithBoundaryConditions,
+// minmod prolongation), repeated nrep times; vector[f] marks Metadata::Vector fields.
+void orc_exchange_bc(void *h, int nfields, const int *ncomp, const int *vector, double **data,
+                     double **coarse, int nrep) {
+  const Mesh &m = *static_cast<OMesh *>(h)->m;
+  std::vector<Field> fields(nfields);
+  const std::int64_t E = m.cellbounds.size(), cE = m.c_cellbounds.size();
+  const std::int64_t nb = m.blocks.size();
+  for (int f = 0; f < nfields; ++f) {
+    fields[f].ncomp = ncomp[f];
+    fields[f].vector = vector[f] != 0;
+    fields[f].data.assign(data[f], data[f] + nb * ncomp[f] * E);
+    if (m.multilevel) fields[f].coarse.assign(coarse[f], coarse[f] + nb * ncomp[f] * cE);
+  }
+  Exchange ex;
+  ex.Build(m, fields);
+  for (int r = 0; r < nrep; ++r) ex.WithBoundaryConditions(fields);
+  for (int f = 0; f < nfields; ++f) {
+    std::memcpy(data[f], fields[f].data.data(), sizeof(double) * nb * ncomp[f] * E);
+    if (m.multilevel && coarse[f])
+      std::memcpy(coarse[f], fields[f].coarse.data(), sizeof(double) * nb * ncomp[f] * cE);
+  }
+}
+
+// Physical boundary conditions alone on data (coarse = 0) or coarse buffers (coarse = 1).
+void orc_apply_bc(void *h, int nfields, const int *ncomp, const int *vector, double **arrays,
+                  int coarse) {
+  const Mesh &m = *static_cast<OMesh *>(h)->m;
+  std::vector<Field> fields(nfields);
+  const std::int64_t E = coarse ? m.c_cellbounds.size() : m.cellbounds.size();
+  const std::int64_t nb = m.blocks.size();
+  for (int f = 0; f < nfields; ++f) {
+    fields[f].ncomp = ncomp[f];
+    fields[f].vector = vector[f] != 0;
+    std::vector<double> &v = coarse ? fields[f].coarse : fields[f].data;
+    v.assign(arrays[f], arrays[f] + nb * ncomp[f] * E);
+  }
+  ApplyBoundaryConditions(m, fields, coarse != 0);
+  for (int f = 0; f < nfields; ++f) {
+    const std::vector<double> &v = coarse ? fields[f].coarse : fields[f].data;
+    std::memcpy(arrays[f], v.data(), sizeof(double) * nb * ncomp[f] * E);
   }
 }
 

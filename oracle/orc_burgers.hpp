@@ -298,9 +298,7 @@ struct Burgers {
   void DoExchange(Container &c, Exchange &ex, std::vector<Field> &flist) {
     flist.resize(1);
     std::swap(flist[0], c.U);
-    ex.Send(flist);
-    ex.Set(flist);
-    ex.Prolongate(flist);
+    ex.WithBoundaryConditions(flist);
     std::swap(flist[0], c.U);
   }
 

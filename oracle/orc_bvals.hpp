@@ -30,6 +30,7 @@ struct Field {
   std::string name;
   int ncomp = 1;
   bool fill_ghost = true;
+  bool vector = false;  // Metadata::Vector: component v has vector_component v + 1
   std::vector<double> data;
   std::vector<double> coarse;
 };
@@ -404,6 +405,68 @@ inline void RunProlongate(const Mesh &m, std::vector<Field> &fields, const std::
   }
 }
 
+// ---- physical boundary conditions ----
+// reference DoPhysicalBoundary_ (boundary_conditions.cpp:197-211) on the block's boundary flag:
+// a face is physical when it lies on the domain boundary of an active, non-periodic dimension.
+inline bool PhysicalFace(const Mesh &m, const Block &b, int face) {
+  const int d = face / 2;
+  if (!m.active[d] || m.d.bc[face] == kBcPeriodic) return false;
+  const std::int64_t N = m.NAt(d, b.loc.level);
+  return face % 2 == 0 ? b.loc.l[d] == 0 : b.loc.l[d] == N - 1;
+}
+
+// reference GenericBC<DIR, SIDE, Outflow|Reflect> (boundary_conditions_generic.hpp:174-255)
+// for one cell-centred FillGhost field of one block: the loop of par_for_bndry over
+// IndexDomain::inner_xD / outer_xD (ghost layers along D, the entire range along the other
+// dimensions; domain.hpp:184-252), Outflow copying the first / last interior cell (ref),
+// Reflect copying the mirror cell (offset - index) and negating the vector component D.
+inline void GenericBC(const Mesh &m, const Block &b, Field &f, bool coarse, int dir, bool inner,
+                      int type) {
+  const Shape &sh = coarse ? m.c_cellbounds : m.cellbounds;
+  double *slab = BlockSlab(f, sh, b.gid, coarse);
+  const int ref = inner ? sh.s(dir) : sh.e(dir);
+  const int offset = 2 * ref + (inner ? -1 : 1);
+  int lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = 0;
+    hi[d] = sh.ent(d) - 1;
+  }
+  if (inner)
+    hi[dir] = sh.s(dir) - 1;
+  else
+    lo[dir] = sh.e(dir) + 1;
+  for (int l = 0; l < f.ncomp; ++l) {
+    // vector_component (v + 1 for component v of a Vector field, sparse_pack_base.cpp:250-251)
+    // equals the direction X1DIR = 1, X2DIR = 2, X3DIR = 3
+    const bool reflect = f.vector && l + 1 == dir + 1;
+    for (int k = lo[2]; k <= hi[2]; ++k)
+      for (int j = lo[1]; j <= hi[1]; ++j)
+        for (int i = lo[0]; i <= hi[0]; ++i) {
+          int src[3] = {i, j, k};
+          src[dir] = type == kBcReflect ? offset - src[dir] : ref;
+          const double q = At(slab, sh, l, src[2], src[1], src[0]);
+          At(slab, sh, l, k, j, i) = type == kBcReflect ? (reflect ? -1.0 : 1.0) * q : q;
+        }
+  }
+}
+
+// reference ApplyBoundaryConditionsOnCoarseOrFine (boundary_conditions.cpp:36-58), over the
+// blocks of the MeshData (ApplyBoundaryConditionsOnCoarseOrFineMD, :82-87): faces in
+// BoundaryFace order inner_x1, outer_x1, inner_x2, ..., outer_x3.
+inline void ApplyBoundaryConditions(const Mesh &m, std::vector<Field> &fields, bool coarse) {
+  for (const auto &b : m.blocks)
+    for (int face = 0; face < 6; ++face)
+      if (PhysicalFace(m, b, face))
+        for (auto &f : fields)
+          if (f.fill_ghost) GenericBC(m, b, f, coarse, face / 2, face % 2 == 0, m.d.bc[face]);
+}
+
+inline bool HasPhysicalBoundary(const Mesh &m) {
+  for (int face = 0; face < 6; ++face)
+    if (m.active[face / 2] && m.d.bc[face] != kBcPeriodic) return true;
+  return false;
+}
+
 // Boundary exchange state for one set of fields on one mesh (reference MeshData caches +
 // Mesh::boundary_comm_map). Buffers live here, keyed like the reference channels.
 struct Exchange {
@@ -511,6 +574,20 @@ struct Exchange {
     Send(fields);
     Set(fields);
     Prolongate(fields);
+  }
+
+  // AddBoundaryExchangeTasks (boundary_communication.cpp:432-445) and the boundary part of
+  // Mesh::CommunicateBoundaries (mesh.cpp:697-705): send, set (+ restriction), coarse-buffer
+  // boundary conditions, prolongation, boundary conditions.
+  void WithBoundaryConditions(std::vector<Field> &fields, bool minmod = true) {
+    Send(fields);
+    Set(fields);
+    const bool bc = HasPhysicalBoundary(*m);
+    if (m->multilevel) {
+      if (bc) ApplyBoundaryConditions(*m, fields, true);
+      RunProlongate(*m, fields, set_pr, minmod);
+    }
+    if (bc) ApplyBoundaryConditions(*m, fields, false);
   }
 };
 

@@ -97,12 +97,17 @@ struct Block {
   bool has_coarser_neighbor = false;
 };
 
+// Boundary condition of a domain face (reference <parthenon/mesh> ix1_bc ... ox3_bc,
+// mesh.cpp:1180-1185, default "outflow"; "periodic" on both faces of a periodic dimension).
+enum BcType { kBcPeriodic = 0, kBcOutflow = 1, kBcReflect = 2 };
+
 struct MeshDesc {
   int nx[3];
   int bnx[3];
   int nghost;
   double xmin[3], xmax[3];
   int periodic[3];
+  int bc[6] = {kBcPeriodic, kBcPeriodic, kBcPeriodic, kBcPeriodic, kBcPeriodic, kBcPeriodic};
 };
 
 struct Mesh {

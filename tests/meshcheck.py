@@ -71,3 +71,44 @@ def linear_field(mesh_like, ncomp, coeffs, domain=((-0.5, 0.5),) * 3):
             a, b1, b2, b3 = coeffs[c]
             out[gid, c] = a + b1 * x + b2 * y + b3 * z
     return out
+
+
+def bc_index_field(mesh_like, ncomp, nx, bcs, vector=False):
+    """Single-level mesh with physical boundaries: value = sign * (c*1e7 + global cell id) of
+    the source cell of every cell. Along each dimension a global index outside the mesh maps
+    back inside by its face's condition: periodic wraps, outflow copies the first / last cell,
+    reflect mirrors across the face. Component c < 3 of a vector field changes sign once per
+    reflecting face it crosses along dimension c."""
+    shape = (mesh_like.nblocks, ncomp) + tuple(reversed(mesh_like.ent))
+    expected = np.empty(shape)
+    for gid in range(mesh_like.nblocks):
+        b = mesh_like.block(gid)
+        assert b["level"] == 0
+        idx, refl = [], []
+        for d in range(3):
+            g = mesh_like.ng[d]
+            gi = b["l"][d] * mesh_like.n[d] + np.arange(mesh_like.ent[d]) - g
+            src = gi.copy()
+            r = np.zeros(gi.shape, dtype=bool)
+            lo, hi = gi < 0, gi >= nx[d]
+            for mask, face, edge, mirror in ((lo, bcs[2 * d], 0, -1 - gi),
+                                             (hi, bcs[2 * d + 1], nx[d] - 1,
+                                              2 * nx[d] - 1 - gi)):
+                if face == "periodic":
+                    src[mask] = np.mod(gi[mask], nx[d])
+                elif face == "outflow":
+                    src[mask] = edge
+                else:
+                    src[mask] = mirror[mask]
+                    r |= mask
+            idx.append(src)
+            refl.append(r)
+        gk, gj, gi = np.meshgrid(idx[2], idx[1], idx[0], indexing="ij")
+        rk, rj, ri = np.meshgrid(refl[2], refl[1], refl[0], indexing="ij")
+        cell = (gk * nx[1] + gj) * nx[0] + gi
+        for c in range(ncomp):
+            v = c * 1.0e7 + cell.astype(np.float64)
+            if vector and c < 3:
+                v = np.where((ri, rj, rk)[c], -v, v)
+            expected[gid, c] = v
+    return expected

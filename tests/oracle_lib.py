@@ -40,6 +40,10 @@ def lib():
             "orc_mesh_region_boxes": (I, [P, I, C.POINTER(I), I]),
             "orc_exchange_counts": (I64, [P, I, C.POINTER(I), C.POINTER(I64)]),
             "orc_exchange": (None, [P, I, C.POINTER(I), C.POINTER(DP), C.POINTER(DP), I, I]),
+            "orc_mesh_set_boundary": (I, [P, I, I]),
+            "orc_exchange_bc": (None, [P, I, C.POINTER(I), C.POINTER(I), C.POINTER(DP),
+                                       C.POINTER(DP), I]),
+            "orc_apply_bc": (None, [P, I, C.POINTER(I), C.POINTER(I), C.POINTER(DP), I]),
             "orc_restrict_interior": (None, [P, I, DP, DP]),
             "orc_prolongate_cells": (None, [P, I, DP, DP, I, C.POINTER(I), I]),
             "orc_weno5z": (None, [I, DP, DP, DP]),
@@ -95,14 +99,25 @@ def iarr(vals, t=C.c_int):
     return (t * len(vals))(*vals)
 
 
+BC_TYPES = {"periodic": 0, "outflow": 1, "reflect": 2}
+
+
 class OracleMesh:
     """Mesh restatement: uniform base grid, optional static refinement boxes/spheres."""
 
     def __init__(self, nx, bnx, nghost, xmin=(-0.5, -0.5, -0.5), xmax=(0.5, 0.5, 0.5),
-                 periodic=(1, 1, 1), boxes=(), spheres=()):
+                 periodic=(1, 1, 1), boxes=(), spheres=(), bcs=None):
+        """bcs: optional 6 face conditions (inner x1, outer x1, ..., outer x3), each
+        "periodic", "outflow" or "reflect"; faces of non-periodic dimensions default to
+        outflow."""
         L = lib()
         self.h = L.orc_mesh_new(iarr(nx), iarr(bnx), nghost, (D * 3)(*xmin), (D * 3)(*xmax),
                                 iarr(periodic))
+        for face, bc in enumerate(bcs or ()):
+            if bc == "periodic":
+                continue
+            if L.orc_mesh_set_boundary(self.h, face, BC_TYPES[bc]) != 0:
+                raise ValueError(f"face {face}: boundary {bc!r} not allowed")
         for box, level in boxes:
             L.orc_mesh_refine_box(self.h, (D * 6)(*box), level)
         for center, r, level in spheres:
@@ -166,6 +181,25 @@ class OracleMesh:
         else:
             cps = (DP * len(fields))(*[dptr(c) for c in coarse])
         lib().orc_exchange(self.h, len(fields), iarr(ncomps), dps, cps, prolong, nrep)
+
+    def exchange_bc(self, fields, coarse=None, vector=None, nrep=1):
+        """In-place exchange in library order with physical boundary conditions: send, set,
+        coarse-buffer conditions, minmod prolongation (multilevel), conditions."""
+        ncomps = [f.shape[1] for f in fields]
+        vec = iarr([int(v) for v in (vector or [0] * len(fields))])
+        dps = (DP * len(fields))(*[dptr(f) for f in fields])
+        if coarse is None:
+            cps = (DP * len(fields))(*[None] * len(fields))
+        else:
+            cps = (DP * len(fields))(*[dptr(c) for c in coarse])
+        lib().orc_exchange_bc(self.h, len(fields), iarr(ncomps), vec, dps, cps, nrep)
+
+    def apply_bc(self, arrays, vector=None, coarse=False):
+        """In-place physical boundary conditions on data (or coarse) arrays."""
+        ncomps = [f.shape[1] for f in arrays]
+        vec = iarr([int(v) for v in (vector or [0] * len(arrays))])
+        dps = (DP * len(arrays))(*[dptr(f) for f in arrays])
+        lib().orc_apply_bc(self.h, len(arrays), iarr(ncomps), vec, dps, int(coarse))
 
 
 class OracleBurgers:
