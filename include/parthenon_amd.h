@@ -49,12 +49,19 @@ enum {
   PTH_FILL_GHOST = 1u << 0,
   PTH_INDEPENDENT = 1u << 1,
   PTH_WITH_FLUXES = 1u << 2,
-  PTH_ONE_COPY = 1u << 3
+  PTH_ONE_COPY = 1u << 3,
+  PTH_VECTOR = 1u << 4 /* Metadata::Vector: component v is the x(v+1) vector component */
 };
 
 /* How SendBoundBufs/SetBounds launch: one persistent table-walking kernel per call (default),
  * or the plain form with one kernel launch per boundary buffer. */
 typedef enum { PTH_PACK_PERSISTENT = 0, PTH_PACK_PER_BUFFER = 1 } pth_pack_mode;
+
+/* Boundary condition of a domain face (<parthenon/mesh> ix1_bc ... ox3_bc, src/mesh/mesh.cpp:
+ * 1180-1185; the reference's "outflow" and "reflecting" functions, src/bvals/
+ * boundary_conditions.cpp:89-137). Faces are numbered like BoundaryFace: 0 inner x1,
+ * 1 outer x1, 2 inner x2, 3 outer x2, 4 inner x3, 5 outer x3. */
+typedef enum { PTH_BC_PERIODIC = 0, PTH_BC_OUTFLOW = 1, PTH_BC_REFLECT = 2 } pth_bc;
 
 typedef struct pth_mesh pth_mesh;         /* Mesh: blocks, neighbours, fields, comm buffers */
 typedef struct pth_meshdata pth_meshdata; /* MeshData: one named container over all blocks */
@@ -64,6 +71,9 @@ typedef struct pth_meshdata pth_meshdata; /* MeshData: one named container over 
  * bnx: meshblock cells, periodic: per-dimension flag (non-periodic faces have no neighbour). */
 pth_mesh *pth_mesh_create(const int nx[3], const int bnx[3], int nghost, const double xmin[3],
                           const double xmax[3], const int periodic[3]);
+/* Condition of one face of a non-periodic dimension (default PTH_BC_OUTFLOW; the faces of a
+ * periodic dimension stay periodic). Must precede pth_mesh_plan / pth_mesh_finalize. */
+pth_status pth_mesh_set_boundary(pth_mesh *m, int face, int type);
 /* <parthenon/static_refinementN> box {x1min,x1max,x2min,x2max,x3min,x3max} at level. */
 pth_status pth_mesh_refine_box(pth_mesh *m, const double box[6], int level);
 /* Union of refinement boxes, one per level-`level` block intersecting the sphere. */
@@ -128,9 +138,16 @@ pth_status pth_receive_bound_bufs(pth_meshdata *md, int bt);
 pth_status pth_set_bounds(pth_meshdata *md, int bt);
 /* ProlongateBounds<bt>: boundary_communication.cpp:361-393 (minmod shared prolongation). */
 pth_status pth_prolongate_bounds(pth_meshdata *md, int bt);
+/* ApplyBoundaryConditionsOnCoarseOrFineMD (src/bvals/boundary_conditions.cpp:82-87): the
+ * physical boundary conditions of every FillGhost field on the blocks' data (coarse = 0) or
+ * coarse buffers (coarse = 1), GenericBC Outflow / Reflect (boundary_conditions_generic.hpp:
+ * 174-255) on each face that lies on the domain boundary of a non-periodic dimension. The
+ * faces are applied in one launch whose result equals the reference's face-by-face order. */
+pth_status pth_apply_boundary_conditions(pth_meshdata *md, int coarse);
 
-/* Send + Receive + Set (+ ProlongateBounds on multilevel meshes) of bound type any in one
- * call. On one GPU the buffer stage is skipped: every element moves from its sending cell to
+/* Send + Receive + Set (+ coarse-buffer boundary conditions and ProlongateBounds on multilevel
+ * meshes, then the boundary conditions: the order of AddBoundaryExchangeTasks,
+ * boundary_communication.cpp:432-445) of bound type any in one call. On one GPU the buffer stage is skipped: every element moves from its sending cell to
  * its receiving cell through the same region tables in one launch (send regions read only
  * interior cells, receive regions write only ghost cells). Ghost data is identical to the
  * buffered sequence. No counterpart in the reference, whose exchange is split into tasks so

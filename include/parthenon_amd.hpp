@@ -50,9 +50,13 @@ struct Metadata {  // the cell-variable flags used on this path (src/interface/m
     FillGhost = PTH_FILL_GHOST,
     Independent = PTH_INDEPENDENT,
     WithFluxes = PTH_WITH_FLUXES,
-    OneCopy = PTH_ONE_COPY
+    OneCopy = PTH_ONE_COPY,
+    Vector = PTH_VECTOR
   };
 };
+
+// src/defs.hpp:127-137 BoundaryFace: the six faces of a block or of the domain
+enum class BoundaryFace { inner_x1 = 0, outer_x1, inner_x2, outer_x2, inner_x3, outer_x3 };
 
 namespace detail {
 inline TaskStatus Check(pth_status s, const char *what) {
@@ -112,6 +116,14 @@ class Mesh {
   Mesh(const Mesh &) = delete;
   Mesh &operator=(const Mesh &) = delete;
 
+  // <parthenon/mesh> ix1_bc ... ox3_bc (src/mesh/mesh.cpp:1180-1185): "outflow" or
+  // "reflecting" on a face of a non-periodic dimension (its default is outflow)
+  void SetBoundaryCondition(BoundaryFace face, const std::string &name) {
+    int type = name == "outflow" ? PTH_BC_OUTFLOW : name == "reflecting" ? PTH_BC_REFLECT : -1;
+    if (type < 0) throw std::runtime_error("SetBoundaryCondition: unknown condition " + name);
+    detail::Check(pth_mesh_set_boundary(h_, static_cast<int>(face), type),
+                  "SetBoundaryCondition");
+  }
   // <parthenon/static_refinementN> (src/mesh/mesh.cpp:1085-1116)
   void AddStaticRefinement(std::array<Real, 6> box, int level) {
     detail::Check(pth_mesh_refine_box(h_, box.data(), level), "AddStaticRefinement");
@@ -185,6 +197,15 @@ inline TaskStatus SetBoundaries(std::shared_ptr<MeshData<Real>> &md) {
 }
 inline TaskStatus ProlongateBoundaries(std::shared_ptr<MeshData<Real>> &md) {
   return ProlongateBounds<BoundaryType::any>(md);
+}
+// physical boundary conditions (src/bvals/boundary_conditions.hpp:40-50)
+inline TaskStatus ApplyBoundaryConditionsOnCoarseOrFineMD(std::shared_ptr<MeshData<Real>> &md,
+                                                         bool coarse) {
+  return detail::Check(pth_apply_boundary_conditions(md->handle(), coarse ? 1 : 0),
+                       "ApplyBoundaryConditionsOnCoarseOrFineMD");
+}
+inline TaskStatus ApplyBoundaryConditionsMD(std::shared_ptr<MeshData<Real>> &md) {
+  return ApplyBoundaryConditionsOnCoarseOrFineMD(md, false);
 }
 // flux-correction tasks of bvals_in_one.hpp:69-80
 inline TaskStatus StartReceiveFluxCorrections(std::shared_ptr<MeshData<Real>> &md) {

@@ -50,6 +50,13 @@ class Metadata(enum.IntFlag):  # the flags of src/interface/metadata.hpp used on
     Independent = 2
     WithFluxes = 4
     OneCopy = 8
+    Vector = 16
+
+
+class BoundaryCondition(enum.IntEnum):  # <parthenon/mesh> ix1_bc ... ox3_bc
+    periodic = 0
+    outflow = 1
+    reflect = 2
 
 
 class PackMode(enum.IntEnum):
@@ -66,6 +73,8 @@ _DP = C.POINTER(C.c_double)
 _SIGNATURES = {
     "pth_mesh_create": (_P, [C.POINTER(_I), C.POINTER(_I), _I, _DP, _DP, C.POINTER(_I)]),
     "pth_mesh_refine_box": (_I, [_P, _DP, _I]),
+    "pth_mesh_set_boundary": (_I, [_P, _I, _I]),
+    "pth_apply_boundary_conditions": (_I, [_P, _I]),
     "pth_mesh_refine_sphere": (_I, [_P, _DP, _D, _I]),
     "pth_mesh_add_field": (_I, [_P, C.c_char_p, _I, C.c_uint]),
     "pth_mesh_plan": (_I, [_P]),
@@ -167,13 +176,20 @@ class Mesh:
     """A uniform or statically refined Cartesian mesh of equal-size meshblocks."""
 
     def __init__(self, nx, block, nghost, xmin=(-0.5, -0.5, -0.5), xmax=(0.5, 0.5, 0.5),
-                 periodic=(True, True, True), refine_boxes=(), refine_spheres=()):
+                 periodic=(True, True, True), refine_boxes=(), refine_spheres=(), bcs=None):
+        """bcs: optional conditions of the 6 faces (inner x1, outer x1, ..., outer x3), each
+        a BoundaryCondition or its name; faces of non-periodic dimensions default to
+        outflow, faces of periodic dimensions must stay periodic."""
         L = lib()
         arr3 = lambda v, t: (t * 3)(*v)
         self._h = L.pth_mesh_create(arr3(nx, _I), arr3(block, _I), nghost, arr3(xmin, _D),
                                     arr3(xmax, _D), arr3([int(bool(p)) for p in periodic], _I))
         if not self._h:
             raise RuntimeError("pth_mesh_create failed: " + L.pth_last_error().decode())
+        for face, bc in enumerate(bcs or ()):
+            bc = BoundaryCondition[bc] if isinstance(bc, str) else BoundaryCondition(bc)
+            if bc != BoundaryCondition.periodic:
+                _check(L.pth_mesh_set_boundary(self._h, face, int(bc)), "set_boundary")
         for box, level in refine_boxes:
             _check(L.pth_mesh_refine_box(self._h, (_D * 6)(*box), level), "refine_box")
         for center, radius, level in refine_spheres:
@@ -318,6 +334,16 @@ def ProlongateBounds(md, bt=BoundaryType.any):
     return _check(lib().pth_prolongate_bounds(md._h, int(bt)), "ProlongateBounds")
 
 
+# physical boundary conditions (src/bvals/boundary_conditions.hpp:40-50)
+def ApplyBoundaryConditionsOnCoarseOrFineMD(md, coarse):
+    return _check(lib().pth_apply_boundary_conditions(md._h, int(bool(coarse))),
+                  "ApplyBoundaryConditionsOnCoarseOrFineMD")
+
+
+def ApplyBoundaryConditionsMD(md):
+    return ApplyBoundaryConditionsOnCoarseOrFineMD(md, False)
+
+
 # flux-correction tasks (src/bvals/comms/bvals_in_one.hpp:69-80)
 def StartReceiveFluxCorrections(md):
     return StartReceiveBoundBufs(md, BoundaryType.flxcor_recv)
@@ -405,16 +431,20 @@ def profile_report():
 
 
 def exchange_fused(md, bt=BoundaryType.any):
-    """Send+Receive+Set(+Prolongate) in one call with the buffer stage skipped (one GPU)."""
+    """Send+Receive+Set(+coarse conditions, Prolongate), conditions in one call with the buffer
+    stage skipped (one GPU)."""
     return _check(lib().pth_exchange_fused(md._h, int(bt)), "exchange_fused")
 
 
 def exchange(md, bt=BoundaryType.any):
-    """Send, receive, set (+ prolongate on multilevel meshes) in the library task order
+    """Send, receive, set (+ coarse-buffer conditions and prolongation on multilevel meshes),
+    boundary conditions: the library task order of AddBoundaryExchangeTasks
     (src/bvals/comms/boundary_communication.cpp:406-445)."""
     SendBoundBufs(md, bt)
     if ReceiveBoundBufs(md, bt) != TaskStatus.complete:
         raise RuntimeError("ReceiveBoundBufs did not complete after SendBoundBufs")
     SetBounds(md, bt)
     if md.mesh.multilevel:
+        ApplyBoundaryConditionsOnCoarseOrFineMD(md, True)
         ProlongateBounds(md, bt)
+    ApplyBoundaryConditionsOnCoarseOrFineMD(md, False)

@@ -11,7 +11,8 @@ min(2 * dt_prev, estimate).
 """
 import math
 
-from . import (BoundaryType, LoadAndSendFluxCorrections, Mesh, Metadata, ProlongateBounds,
+from . import (ApplyBoundaryConditionsOnCoarseOrFineMD, BoundaryType,
+               LoadAndSendFluxCorrections, Mesh, Metadata, ProlongateBounds,
                ReceiveBoundBufs, ReceiveFluxCorrections, SendBoundBufs, SetBounds,
                SetFluxCorrections, StartReceiveBoundBufs, StartReceiveFluxCorrections,
                TaskStatus, Update, burgers_package)
@@ -20,11 +21,17 @@ RK2_BETA = (1.0, 0.5)
 
 
 def make_burgers_mesh(nx=(64, 64, 64), block=(16, 16, 16), nghost=4, num_scalars=4,
-                      refine_spheres=(), refine_boxes=()):
+                      refine_spheres=(), refine_boxes=(), bcs=None):
     """Mesh with the fields burgers_package::Initialize registers
     (benchmarks/burgers/burgers_package.cpp:76-94): U with FillGhost/Independent/WithFluxes and
-    the OneCopy derived field."""
-    m = Mesh(nx, block, nghost, refine_spheres=refine_spheres, refine_boxes=refine_boxes)
+    the OneCopy derived field. bcs: optional face conditions (inner x1, outer x1, ...,
+    outer x3); a dimension whose faces are not periodic is not periodic."""
+    periodic = (True, True, True)
+    if bcs is not None:
+        periodic = tuple(bcs[2 * d] == "periodic" and bcs[2 * d + 1] == "periodic"
+                         for d in range(3))
+    m = Mesh(nx, block, nghost, periodic=periodic, refine_spheres=refine_spheres,
+             refine_boxes=refine_boxes, bcs=bcs)
     m.add_field("U", 3 + num_scalars,
                 Metadata.FillGhost | Metadata.Independent | Metadata.WithFluxes)
     m.add_field("derived", 1, Metadata.OneCopy)
@@ -93,8 +100,12 @@ class BurgersDriver:
         SetBounds(md, BoundaryType.local)
         ReceiveBoundBufs(md, BoundaryType.nonlocal_)
         SetBounds(md, BoundaryType.nonlocal_)
+        # AddBoundaryExchangeTasks order (boundary_communication.cpp:432-445); the Burgers task
+        # list applies the conditions to "sc1" after its exchange (burgers_driver.cpp:130-137)
         if self.mesh.multilevel:
+            ApplyBoundaryConditionsOnCoarseOrFineMD(md, True)
             ProlongateBounds(md, any_)
+        ApplyBoundaryConditionsOnCoarseOrFineMD(md, False)
 
     def initialize(self, pgen="sinusoid", seed=0):
         """Problem generator, boundary exchange, FillDerived, first time step

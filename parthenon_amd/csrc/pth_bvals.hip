@@ -249,7 +249,50 @@ __global__ void __launch_bounds__(kWG)
 
 int GridFor(int nitems) { return nitems; }  // one workgroup per work item of the table
 
+// Physical boundary conditions: one workgroup per box of ghost cells. Every cell copies its
+// source cell (negated for a reflected vector component). Sources lie outside every box of
+// the launch, so the order of the workgroups does not matter.
+__global__ void __launch_bounds__(kWG) BoundaryConditionKernel(const BcBox *__restrict__ boxes) {
+  const BcBox &B = boxes[blockIdx.x];
+  const int n0 = B.n[0], n01 = B.n[0] * B.n[1];
+  const int cells = n01 * B.n[2];
+  const std::int64_t cs = static_cast<std::int64_t>(B.e[0]) * B.e[1] * B.e[2];
+  for (int t = threadIdx.x; t < cells * B.ncomp; t += kWG) {
+    const int c = t / cells, r = t % cells;
+    const int i = B.lo[0] + r % n0;
+    const int j = B.lo[1] + (r % n01) / n0;
+    const int k = B.lo[2] + r / n01;
+    const int si = B.a[0] + B.b[0] * i, sj = B.a[1] + B.b[1] * j, sk = B.a[2] + B.b[2] * k;
+    double *q = B.slab + c * cs;
+    const double v = q[(static_cast<std::int64_t>(sk) * B.e[1] + sj) * B.e[0] + si];
+    const bool neg = B.vec && c < 3 && ((B.flip >> c) & 1);
+    q[(static_cast<std::int64_t>(k) * B.e[1] + j) * B.e[0] + i] = neg ? -v : v;
+  }
+}
+
 }  // namespace
+
+void BcTable::Free() {
+  if (dev) (void)hipFree(dev);
+  dev = nullptr;
+}
+
+pth_status BcTable::Upload() {
+  Free();
+  if (host.empty()) return PTH_COMPLETE;
+  PTH_HIP(hipMalloc(&dev, sizeof(BcBox) * host.size()));
+  PTH_HIP(hipMemcpy(dev, host.data(), sizeof(BcBox) * host.size(), hipMemcpyHostToDevice));
+  return PTH_COMPLETE;
+}
+
+pth_status LaunchBoundaryConditions(const BcTable &t, hipStream_t s) {
+  if (t.host.empty()) return PTH_COMPLETE;
+  ProfScope prof("boundary_conditions", s);
+  hipLaunchKernelGGL(BoundaryConditionKernel, dim3(static_cast<unsigned>(t.host.size())),
+                     dim3(kWG), 0, s, t.dev);
+  PTH_HIP(hipGetLastError());
+  return PTH_COMPLETE;
+}
 
 std::uint64_t Magic(int d) {
   const std::uint64_t one = 1ull << 40;

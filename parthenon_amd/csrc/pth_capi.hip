@@ -71,6 +71,8 @@ MeshData::~MeshData() {
     c.restrict_send.Free();
     c.restrict_set.Free();
     c.prolong.Free();
+    c.bc[0].Free();
+    c.bc[1].Free();
   }
 }
 
@@ -421,7 +423,95 @@ pth_status Mesh::BuildCache(MeshData &md, int bt) {
   }
   for (Table *t : {&c.pack, &c.unpack, &c.restrict_send, &c.restrict_set, &c.prolong})
     if (t->Upload() != PTH_COMPLETE) return PTH_FAIL;
+  if (BuildBoundaryConditions(md, c) != PTH_COMPLETE) return PTH_FAIL;
   c.built = true;
+  return PTH_COMPLETE;
+}
+
+// Boxes of the physical boundary conditions (reference ApplyBoundaryConditionsOnCoarseOrFine,
+// boundary_conditions.cpp:36-58, with GenericBC, boundary_conditions_generic.hpp:174-255):
+// per block, FillGhost field and array (data; coarse buffer on multilevel meshes), the brick
+// is cut into 3 x 3 x 3 classes (inner ghosts, interior, outer ghosts per dimension); each
+// class touching a physical face becomes one box whose cells all map to their source the same
+// way. The reference applies the faces one after another over the entire transverse range
+// (par_for_bndry over IndexDomain::inner_xD / outer_xD, domain.hpp:184-252); a cell in the
+// ghosts of several physical faces therefore ends with the value of the cell reached by
+// mapping each of those dimensions, which is what the box's source map gives directly.
+pth_status Mesh::BuildBoundaryConditions(MeshData &md, BoundCache &c) {
+  c.bc[0].host.clear();
+  c.bc[1].host.clear();
+  const int nb = static_cast<int>(topo.blocks.size());
+  for (int b = 0; b < nb; ++b) {
+    const Block &B = topo.blocks[b];
+    bool phys[6];
+    bool any = false;
+    for (int face = 0; face < 6; ++face) {
+      const int d = face / 2;
+      const std::int64_t last = (topo.nbase[d] << B.at.level) - 1;
+      phys[face] = topo.active[d] && topo.bc[face] != PTH_BC_PERIODIC &&
+                   (face % 2 == 0 ? B.at.x[d] == 0 : B.at.x[d] == last);
+      any = any || phys[face];
+    }
+    if (!any) continue;
+    for (size_t f = 0; f < fields.size(); ++f) {
+      if (!(fields[f].flags & PTH_FILL_GHOST)) continue;
+      const int nc = fields[f].ncomp;
+      for (int coarse = 0; coarse < (topo.multilevel ? 2 : 1); ++coarse) {
+        const Extent &e = coarse ? topo.coarse : topo.fine;
+        double *base = coarse ? md.store[f].coarse : md.store[f].data;
+        if (!base) continue;
+        double *slab = base + static_cast<std::int64_t>(b) * nc * e.cells();
+        for (int cz = 0; cz < 3; ++cz)
+          for (int cy = 0; cy < 3; ++cy)
+            for (int cx = 0; cx < 3; ++cx) {
+              const int cls[3] = {cx, cy, cz};
+              BcBox x;
+              std::memset(&x, 0, sizeof(x));
+              x.slab = slab;
+              x.ncomp = nc;
+              x.vec = (fields[f].flags & PTH_VECTOR) ? 1 : 0;
+              bool mapped = false, empty = false;
+              for (int d = 0; d < 3; ++d) {
+                const int lo = e.lo(d), hi = e.hi(d), ent = e.ent(d);
+                const int r0 = cls[d] == 0 ? 0 : cls[d] == 1 ? lo : hi + 1;
+                const int r1 = cls[d] == 0 ? lo - 1 : cls[d] == 1 ? hi : ent - 1;
+                if (r1 < r0) empty = true;
+                x.lo[d] = r0;
+                x.n[d] = r1 - r0 + 1;
+                x.e[d] = ent;
+                x.a[d] = 0;
+                x.b[d] = 1;
+                const int face = cls[d] == 0 ? 2 * d : cls[d] == 2 ? 2 * d + 1 : -1;
+                if (face < 0 || !phys[face]) continue;
+                mapped = true;
+                const int ref = cls[d] == 0 ? lo : hi;
+                if (topo.bc[face] == PTH_BC_REFLECT) {
+                  x.a[d] = 2 * ref + (cls[d] == 0 ? -1 : 1);
+                  x.b[d] = -1;
+                  x.flip |= 1 << d;
+                } else {
+                  x.a[d] = ref;
+                  x.b[d] = 0;
+                }
+              }
+              if (empty || !mapped) continue;
+              // every written and every read cell lies inside the brick
+              for (int d = 0; d < 3; ++d) {
+                const int w0 = x.lo[d], w1 = x.lo[d] + x.n[d] - 1;
+                const int s0 = x.a[d] + x.b[d] * w0, s1 = x.a[d] + x.b[d] * w1;
+                if (w0 < 0 || w1 >= x.e[d] || std::min(s0, s1) < 0 ||
+                    std::max(s0, s1) >= x.e[d]) {
+                  SetError("internal: a boundary-condition box exceeds its brick");
+                  return PTH_FAIL;
+                }
+              }
+              c.bc[coarse].host.push_back(x);
+            }
+      }
+    }
+  }
+  for (BcTable *t : {&c.bc[0], &c.bc[1]})
+    if (t->Upload() != PTH_COMPLETE) return PTH_FAIL;
   return PTH_COMPLETE;
 }
 
@@ -591,6 +681,28 @@ pth_mesh *pth_mesh_create(const int nx[3], const int bnx[3], int nghost, const d
 }
 
 void pth_mesh_destroy(pth_mesh *m) { delete Impl(m); }
+
+pth_status pth_mesh_set_boundary(pth_mesh *mh, int face, int type) {
+  Mesh *m = Impl(mh);
+  if (m->planned) {
+    SetError("pth_mesh_set_boundary after pth_mesh_plan / pth_mesh_finalize");
+    return PTH_FAIL;
+  }
+  if (face < 0 || face >= 6) {
+    SetError("boundary face must be 0..5 (inner x1, outer x1, ..., outer x3)");
+    return PTH_FAIL;
+  }
+  if (type != PTH_BC_OUTFLOW && type != PTH_BC_REFLECT) {
+    SetError("boundary condition must be PTH_BC_OUTFLOW or PTH_BC_REFLECT");
+    return PTH_FAIL;
+  }
+  if (m->topo.periodic[face / 2]) {
+    SetError("the faces of a periodic dimension stay periodic");
+    return PTH_FAIL;
+  }
+  m->topo.bc[face] = type;
+  return PTH_COMPLETE;
+}
 
 pth_status pth_mesh_refine_box(pth_mesh *mh, const double box[6], int level) {
   Mesh *m = Impl(mh);
@@ -954,6 +1066,14 @@ pth_status pth_set_bounds(pth_meshdata *mdh, int bt) {
       LaunchRestrictT(c.restrict_set, m->topo, m->geom, m->stream) != PTH_COMPLETE)
     return PTH_FAIL;
   return PTH_COMPLETE;
+}
+
+pth_status pth_apply_boundary_conditions(pth_meshdata *mdh, int coarse) {
+  MeshData *md = Impl(mdh);
+  Mesh *m = md->mesh;
+  if (NeedFinal(m) != PTH_COMPLETE) return PTH_FAIL;
+  if (m->BuildCache(*md, PTH_BT_ANY) != PTH_COMPLETE) return PTH_FAIL;
+  return LaunchBoundaryConditions(md->cache[PTH_BT_ANY].bc[coarse ? 1 : 0], m->stream);
 }
 
 pth_status pth_prolongate_bounds(pth_meshdata *mdh, int bt) {

@@ -31,23 +31,32 @@
 namespace pth {
 namespace {
 
+// The exchange of AddBoundaryExchangeTasks (boundary_communication.cpp:432-445): send + set
+// (restriction on multilevel meshes), coarse-buffer boundary conditions, prolongation,
+// boundary conditions.
 pth_status Exchange(Mesh &m, MeshData &md, hipStream_t s) {
   if (m.BuildCache(md, PTH_BT_ANY) != PTH_COMPLETE) return PTH_FAIL;
   BoundCache &c = md.cache[PTH_BT_ANY];
-  if (c.nbuffers == 0) return PTH_COMPLETE;
-  if (m.topo.multilevel && LaunchRestrictT(c.restrict_send, m.topo, m.geom, s) != PTH_COMPLETE)
-    return PTH_FAIL;
-  if (m.pack_mode == PTH_PACK_PERSISTENT && CanExchangeDirect(c.pack, c.unpack)) {
-    if (LaunchExchangeDirect(c.pack, c.unpack, s) != PTH_COMPLETE) return PTH_FAIL;
-  } else {
-    if (LaunchPack(c.pack, nullptr, s, true, m.pack_mode) != PTH_COMPLETE) return PTH_FAIL;
-    if (LaunchPack(c.unpack, nullptr, s, false, m.pack_mode) != PTH_COMPLETE) return PTH_FAIL;
+  if (c.nbuffers > 0) {
+    if (m.topo.multilevel &&
+        LaunchRestrictT(c.restrict_send, m.topo, m.geom, s) != PTH_COMPLETE)
+      return PTH_FAIL;
+    if (m.pack_mode == PTH_PACK_PERSISTENT && CanExchangeDirect(c.pack, c.unpack)) {
+      if (LaunchExchangeDirect(c.pack, c.unpack, s) != PTH_COMPLETE) return PTH_FAIL;
+    } else {
+      if (LaunchPack(c.pack, nullptr, s, true, m.pack_mode) != PTH_COMPLETE) return PTH_FAIL;
+      if (LaunchPack(c.unpack, nullptr, s, false, m.pack_mode) != PTH_COMPLETE) return PTH_FAIL;
+    }
+    if (m.topo.multilevel &&
+        LaunchRestrictT(c.restrict_set, m.topo, m.geom, s) != PTH_COMPLETE)
+      return PTH_FAIL;
   }
   if (m.topo.multilevel) {
-    if (LaunchRestrictT(c.restrict_set, m.topo, m.geom, s) != PTH_COMPLETE) return PTH_FAIL;
-    if (LaunchProlongT(c.prolong, m.topo, m.geom, s, true) != PTH_COMPLETE) return PTH_FAIL;
+    if (LaunchBoundaryConditions(c.bc[1], s) != PTH_COMPLETE) return PTH_FAIL;
+    if (c.nbuffers > 0 && LaunchProlongT(c.prolong, m.topo, m.geom, s, true) != PTH_COMPLETE)
+      return PTH_FAIL;
   }
-  return PTH_COMPLETE;
+  return LaunchBoundaryConditions(c.bc[0], s);
 }
 
 struct Ctx {

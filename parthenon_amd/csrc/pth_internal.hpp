@@ -93,6 +93,7 @@ class Topology {
   int nx[3], bnx[3], ng;
   double xmin[3], xmax[3];
   bool periodic[3], active[3];
+  int bc[6];  // pth_bc of each domain face (periodic dimensions: PTH_BC_PERIODIC)
   int ndim = 3;
   std::int64_t nbase[3];
   bool multilevel = false;
@@ -228,9 +229,36 @@ struct FieldStore {
 
 class Mesh;
 
+// One box of ghost cells of one block and field that the physical boundary conditions fill
+// (reference GenericBC, boundary_conditions_generic.hpp:174-255). Along each dimension every
+// cell of the box lies in the same class (inner ghosts, interior, outer ghosts), so each cell
+// has one source cell src[d] = a[d] + b[d] * idx[d]: b = 1, a = 0 off the condition's faces;
+// b = 0, a = first / last interior index for outflow; b = -1, a = 2 * ref -/+ 1 for
+// reflection. Sources lie in no condition's ghost region, so one launch equals the
+// reference's face-by-face passes; component c < 3 of a vector field is negated when
+// reflected along c (bit c of flip).
+struct BcBox {
+  double *slab;  // component 0 of the block's data or coarse brick
+  int lo[3], n[3];
+  int a[3], b[3];
+  int e[3];      // brick extents (i, j, k)
+  int ncomp;
+  int flip;      // reflecting dimensions
+  int vec;       // 1 for a Metadata::Vector field
+};
+static_assert(sizeof(BcBox) % 8 == 0, "bc box alignment");
+
+struct BcTable {
+  std::vector<BcBox> host;
+  BcBox *dev = nullptr;
+  void Free();
+  pth_status Upload();
+};
+
 struct BoundCache {  // per (MeshData, bound type): the pack / unpack / pro-res tables
   bool built = false;
   Table pack, unpack, restrict_send, restrict_set, prolong;
+  BcTable bc[2];  // physical boundary conditions on data (0) and coarse buffers (1)
   std::int64_t nbuffers = 0;
 };
 
@@ -292,6 +320,7 @@ class Mesh {
   MeshData *Get(const std::string &name);
   pth_status Allocate(MeshData &md);
   pth_status BuildCache(MeshData &md, int bt);
+  pth_status BuildBoundaryConditions(MeshData &md, BoundCache &c);
   // patches of field f (built on first use); null with an error if f has no fluxes
   FluxCorrection *FluxPatches(int f);
 };
@@ -329,6 +358,7 @@ bool CanExchangeDirect(const Table &pack, const Table &unpack);
 pth_status LaunchExchangeDirect(const Table &pack, const Table &unpack, hipStream_t s);
 pth_status LaunchRestrict(const Table &t, const DeviceBlockGeom *geom, hipStream_t s);
 pth_status LaunchProlong(const Table &t, const DeviceBlockGeom *geom, hipStream_t s);
+pth_status LaunchBoundaryConditions(const BcTable &t, hipStream_t s);
 
 std::uint64_t Magic(int d);
 

@@ -43,6 +43,7 @@ std::string Topology::Init(const int *nx_, const int *bnx_, int g, const double 
     xmin[d] = lo[d];
     xmax[d] = hi[d];
     periodic[d] = per[d] != 0;
+    bc[2 * d] = bc[2 * d + 1] = periodic[d] ? PTH_BC_PERIODIC : PTH_BC_OUTFLOW;
     active[d] = nx[d] > 1;
     if (active[d]) ndim = d + 1;
     if (bnx[d] < 1 || nx[d] % bnx[d] != 0) return "meshblock size must divide the mesh size";

@@ -3,15 +3,18 @@
 // (benchmarks/burgers/burgers_driver.cpp:53-148) with a sequential task list, the RK2
 // integrator (low_storage_integrator.cpp:67-83) and the driver's dt rule (driver.cpp:210-235).
 //
-// usage: burgers_task_driver <ncycles> <out.bin> [sphere_radius]
+// usage: burgers_task_driver <ncycles> <out.bin> [sphere_radius] [open]
 // writes the final "base" U (float64, [block][comp][k][j][i]) followed by dt and time. With a
 // sphere radius the mesh is statically refined and the stage adds the flux-correction tasks
 // (burgers_driver.cpp:89-96) and ProlongateBounds after the sets (the library exchange order
-// of AddBoundaryExchangeTasks, boundary_communication.cpp:406-445).
+// of AddBoundaryExchangeTasks, boundary_communication.cpp:406-445). "open" makes x1 and x3
+// non-periodic (inner x1 reflecting, the other three faces outflow) and adds the boundary
+// condition tasks (burgers_driver.cpp:130-137; coarse buffers before the prolongation).
 #include <cstdio>
 #include <cstdlib>
 #include <functional>
 #include <limits>
+#include <string>
 #include <vector>
 
 #include "parthenon_amd.hpp"
@@ -45,16 +48,19 @@ struct Integrator {
 
 int main(int argc, char **argv) {
   if (argc < 3) {
-    std::fprintf(stderr, "usage: %s ncycles out.bin [sphere_radius]\n", argv[0]);
+    std::fprintf(stderr, "usage: %s ncycles out.bin [sphere_radius] [open]\n", argv[0]);
     return 2;
   }
   const int ncycles = std::atoi(argv[1]);
   const Real radius = argc > 3 ? std::atof(argv[3]) : 0.0;
+  const bool open = argc > 4 && std::string(argv[4]) == "open";
   const Real cfl = 0.4;
   try {
     // burgers_package::Initialize: U (3 + 4 scalars) and the OneCopy derived field; with a
     // radius, two levels of static refinement on the blocks meeting a centred sphere
-    Mesh mesh({64, 64, 64}, {16, 16, 16}, 4);
+    Mesh mesh({64, 64, 64}, {16, 16, 16}, 4, {-0.5, -0.5, -0.5}, {0.5, 0.5, 0.5},
+              {open ? 0 : 1, 1, open ? 0 : 1});
+    if (open) mesh.SetBoundaryCondition(BoundaryFace::inner_x1, "reflecting");
     if (radius > 0.0) mesh.AddRefinementSphere({0.0, 0.0, 0.0}, radius, 2);
     mesh.AddField("U", 7, Metadata::FillGhost | Metadata::Independent | Metadata::WithFluxes);
     mesh.AddField("derived", 1, Metadata::OneCopy);
@@ -66,7 +72,11 @@ int main(int argc, char **argv) {
     SendBoundaryBuffers(base);
     ReceiveBoundaryBuffers(base);
     SetBoundaries(base);
-    if (mesh.multilevel()) ProlongateBoundaries(base);
+    if (mesh.multilevel()) {
+      ApplyBoundaryConditionsOnCoarseOrFineMD(base, true);
+      ProlongateBoundaries(base);
+    }
+    ApplyBoundaryConditionsMD(base);
     burgers_package::CalculateDerived(base.get());
     Integrator integ;
     Real time = 0.0;
@@ -103,7 +113,11 @@ int main(int argc, char **argv) {
         tl.AddTask([&] { return ReceiveBoundBufs<nonlocal>(mc1); });
         tl.AddTask([&] { return SetBounds<nonlocal>(mc1); });
         // the library exchange order (AddBoundaryExchangeTasks): prolongation after the sets
-        if (mesh.multilevel()) tl.AddTask([&] { return ProlongateBounds<any>(mc1); });
+        if (mesh.multilevel()) {
+          tl.AddTask([&] { return ApplyBoundaryConditionsOnCoarseOrFineMD(mc1, true); });
+          tl.AddTask([&] { return ProlongateBounds<any>(mc1); });
+        }
+        tl.AddTask([&] { return ApplyBoundaryConditionsMD(mc1); });
         tl.AddTask([&] {
           burgers_package::CalculateDerived(mc1.get());
           return TaskStatus::complete;

@@ -1,0 +1,201 @@
+"""GPU parity of the physical boundary conditions (Outflow / Reflect, reference GenericBC,
+boundary_conditions_generic.hpp:174-255) through the C ABI: the exchange with conditions
+against the closed-form source-cell field and the oracle (bit-exact: the conditions copy or
+negate values), on uniform and refined meshes (coarse-buffer conditions before the
+prolongation), and full Burgers runs on non-periodic meshes against the oracle (1e-13)."""
+import numpy as np
+import pytest
+
+import parthenon_amd as pa
+from meshcheck import bc_index_field, interior_mask
+from oracle_lib import OracleBurgers, OracleMesh
+from parthenon_amd.inputs import interior_uniform
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    ((8, 8, 8), (8, 8, 8), 3, ("outflow",) * 6),
+    ((8, 8, 8), (8, 8, 8), 4, ("reflect", "outflow", "outflow", "reflect", "reflect",
+                               "outflow")),
+    ((32, 16, 16), (8, 8, 8), 3, ("reflect", "outflow", "periodic", "periodic", "outflow",
+                                  "reflect")),
+    ((16, 16, 32), (8, 8, 8), 4, ("outflow", "outflow", "reflect", "reflect", "periodic",
+                                  "periodic")),
+]
+
+
+def periodic_flags(bcs):
+    return tuple(bcs[2 * d] == "periodic" for d in range(3))
+
+
+def bc_mesh(nx, block, g, bcs, vector, ncomp=4, **kw):
+    m = pa.Mesh(nx, block, g, periodic=periodic_flags(bcs), bcs=bcs, **kw)
+    flags = pa.Metadata.FillGhost | pa.Metadata.Independent
+    if vector:
+        flags |= pa.Metadata.Vector
+    m.add_field("q", ncomp, flags)
+    m.finalize()
+    return m
+
+
+@pytest.mark.parametrize("nx,block,g,bcs", CASES)
+@pytest.mark.parametrize("vector", [False, True], ids=["scalar", "vector"])
+@pytest.mark.parametrize("fused", [False, True], ids=["tasks", "fused"])
+def test_exchange_with_conditions_matches_closed_form(nx, block, g, bcs, vector, fused):
+    print(f"case bc nx={nx} g={g} bcs={bcs} vector={vector} fused={fused}", flush=True)
+    m = bc_mesh(nx, block, g, bcs, vector)
+    md = m.mesh_data("base")
+    exp = bc_index_field(m, 4, nx, bcs, vector=vector)
+    md.upload("q", np.where(interior_mask(m)[None, None], exp, np.nan))
+    if fused:
+        pa.exchange_fused(md)
+    else:
+        pa.exchange(md)
+    got = md.download("q")
+    assert np.array_equal(got, exp)
+    # and the same bits as the oracle's face-by-face passes (signed zeros included)
+    o = OracleMesh(nx, block, g, periodic=[int(p) for p in periodic_flags(bcs)], bcs=bcs)
+    f = np.where(interior_mask(m)[None, None], exp, np.nan)
+    o.exchange_bc([f], vector=[vector])
+    assert np.array_equal(got.view(np.uint64), f.view(np.uint64))
+
+
+def test_conditions_task_leaves_other_cells():
+    print("case bc task alone", flush=True)
+    bcs = ("outflow", "reflect", "periodic", "periodic", "reflect", "outflow")
+    nx = (32, 16, 16)
+    m = bc_mesh(nx, (8, 8, 8), 2, bcs, True, ncomp=3)
+    md = m.mesh_data("base")
+    rng = np.random.default_rng(1)
+    f = rng.standard_normal(m.field_shape("q"))
+    md.upload("q", f)
+    assert pa.ApplyBoundaryConditionsMD(md) == pa.TaskStatus.complete
+    o = OracleMesh(nx, (8, 8, 8), 2, periodic=(0, 1, 0), bcs=bcs)
+    o.apply_bc([f], vector=[True])
+    assert np.array_equal(md.download("q").view(np.uint64), f.view(np.uint64))
+
+
+REFINED_BC = [
+    dict(nx=(32, 32, 32), block=(8, 8, 8), g=4, bcs=("outflow",) * 6,
+         boxes=[((-0.5, -0.3, -0.1, 0.1, -0.5, -0.2), 1)]),
+    dict(nx=(64, 64, 64), block=(16, 16, 16), g=4,
+         bcs=("reflect", "outflow", "periodic", "periodic", "outflow", "reflect"),
+         spheres=[((-0.3, 0.0, 0.35), 0.2, 2)]),
+]
+
+
+@pytest.mark.parametrize("spec", REFINED_BC, ids=["box-corner", "sphere-at-faces"])
+@pytest.mark.parametrize("fused", [False, True], ids=["tasks", "fused"])
+@pytest.mark.parametrize("vector", [False, True], ids=["scalar", "vector"])
+def test_multilevel_conditions_match_oracle(spec, fused, vector):
+    print(f"case refined bc {spec} fused={fused} vector={vector}", flush=True)
+    boxes, spheres, bcs = spec.get("boxes", ()), spec.get("spheres", ()), spec["bcs"]
+    m = bc_mesh(spec["nx"], spec["block"], spec["g"], bcs, vector, ncomp=3,
+                refine_boxes=boxes, refine_spheres=spheres)
+    assert m.multilevel
+    o = OracleMesh(spec["nx"], spec["block"], spec["g"],
+                   periodic=[int(p) for p in periodic_flags(bcs)], bcs=bcs, boxes=boxes,
+                   spheres=spheres)
+    assert o.nblocks == m.nblocks
+    md = m.mesh_data("base")
+    f = interior_uniform(m.field_shape("q"), m.ng, seed=5, fill=0.0)
+    c = np.zeros(m.coarse_shape("q"))
+    md.upload("q", f)
+    md.upload("q", c, which=1)
+    if fused:
+        pa.exchange_fused(md)
+    else:
+        pa.exchange(md)
+    o.exchange_bc([f], coarse=[c], vector=[vector])
+    got, gotc = md.download("q"), md.download("q", which=1)
+    assert np.array_equal(gotc, c)
+    np.testing.assert_allclose(got, f, rtol=1e-13, atol=0)
+    print("bit-exact:", np.array_equal(got, f), flush=True)
+
+
+def burgers_pair(bcs, spheres=(), use_graph=True, overlap=True):
+    from parthenon_amd.burgers import DeviceBurgersDriver, make_burgers_mesh
+    from oracle_lib import lib
+    nx, block, g, ns = (64, 64, 64), (16, 16, 16), 4, 4
+    m = make_burgers_mesh(nx, block, g, ns, refine_spheres=spheres, bcs=bcs)
+    d = DeviceBurgersDriver(m, cfl=0.4, use_graph=use_graph, overlap=overlap)
+    d.initialize("sinusoid", seed=6)
+    om = OracleMesh(nx, block, g, periodic=[int(p) for p in periodic_flags(bcs)], bcs=bcs,
+                    spheres=spheres)
+    ob = OracleBurgers(om, ns, 0.4, 0)
+    lib().orc_burgers_pgen_sinusoid(ob.h, 6)
+    lib().orc_burgers_initialize(ob.h)
+    return m, d, ob
+
+
+def assert_close(got, exp, what, rtol=1e-13):
+    got, exp = np.asarray(got).ravel(), np.asarray(exp).ravel()
+    err = np.max(np.abs(got - exp)) / np.max(np.abs(exp))
+    print(f"{what}: max rel err {err:.3e}, bit-exact {np.array_equal(got, exp)}", flush=True)
+    assert err <= rtol, what
+
+
+OPEN = ("reflect", "outflow", "periodic", "periodic", "outflow", "outflow")
+
+
+@pytest.mark.parametrize("refined", [False, True], ids=["uniform", "sphere"])
+@pytest.mark.parametrize("use_graph,overlap", [(False, False), (True, True)],
+                         ids=["eager-serial", "graph-overlap"])
+def test_burgers_open_boundaries_match_oracle(refined, use_graph, overlap):
+    print(f"case burgers open refined={refined} graph={use_graph} overlap={overlap}",
+          flush=True)
+    from oracle_lib import lib
+    spheres = [((0.0, 0.0, 0.0), 0.25, 2)] if refined else []
+    m, d, ob = burgers_pair(OPEN, spheres, use_graph, overlap)
+    for c in range(3):
+        d.cycles(1)
+        lib().orc_burgers_cycles(ob.h, 1)
+        assert_close(d.base.download("U"), ob.get(0), f"U after cycle {c + 1}")
+        dt, t = d.clock()
+        assert dt == ob.dt and t == lib().orc_burgers_time(ob.h)
+    d.cycles(5)
+    lib().orc_burgers_cycles(ob.h, 5)
+    assert_close(d.base.download("U"), ob.get(0), "U after 8 cycles")
+
+
+def test_burgers_task_driver_open_boundaries():
+    print("case burgers task driver open", flush=True)
+    from oracle_lib import lib
+    from parthenon_amd.burgers import BurgersDriver, make_burgers_mesh
+    nx, block, g, ns = (32, 32, 32), (16, 16, 16), 4, 4
+    bcs = ("outflow", "reflect", "reflect", "outflow", "periodic", "periodic")
+    m = make_burgers_mesh(nx, block, g, ns, bcs=bcs)
+    d = BurgersDriver(m, cfl=0.4)
+    d.initialize("sinusoid", seed=2)
+    ob = OracleBurgers(OracleMesh(nx, block, g, periodic=(0, 0, 1), bcs=bcs), ns, 0.4, 0)
+    lib().orc_burgers_pgen_sinusoid(ob.h, 2)
+    lib().orc_burgers_initialize(ob.h)
+    assert np.array_equal(d.base.download("U").ravel(), ob.get(0)), "initial state incl. ghosts"
+    for _ in range(4):
+        d.cycle()
+    lib().orc_burgers_cycles(ob.h, 4)
+    assert_close(d.base.download("U"), ob.get(0), "U after 4 cycles")
+
+
+@pytest.mark.parametrize("radius", [0.0, 0.25], ids=["uniform", "sphere"])
+def test_cpp_task_driver_open_boundaries(tmp_path, radius):
+    import os
+    import subprocess
+    from oracle_lib import lib
+    print(f"case cpp task driver open radius={radius}", flush=True)
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "burgers_task_driver")
+    out = str(tmp_path / "u.bin")
+    r = subprocess.run([exe, "3", out, str(radius), "open"], stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = np.fromfile(out, dtype=np.float64)
+    bcs = ("reflect", "outflow", "periodic", "periodic", "outflow", "outflow")
+    spheres = [((0.0, 0.0, 0.0), radius, 2)] if radius > 0 else []
+    om = OracleMesh((64, 64, 64), (16, 16, 16), 4, periodic=(0, 1, 0), bcs=bcs,
+                    spheres=spheres)
+    ob = OracleBurgers(om, 4, 0.4, 0)
+    lib().orc_burgers_pgen_sinusoid(ob.h, 0)
+    lib().orc_burgers_initialize(ob.h)
+    lib().orc_burgers_cycles(ob.h, 3)
+    assert_close(raw[:-2], ob.get(0), "C++ driver U after 3 cycles")
+    assert raw[-2] == ob.dt
