@@ -50,6 +50,9 @@ def kernel_bytes(nb, n, g, V, exch_elems):
     inner_faces = max(n - 5, 0) * n * n
     outer_cells = I - max(n - 10, 0) ** 3 + 18 * n * n
     return {
+        # block sweep: U over the stencil union and base's interior read, the output and the
+        # derived field written (stage 2, where base and U are different arrays)
+        "block_sweep": 8 * nb * (V * (RU + 2 * I) + I),
         "recon_flux": 8 * V * nb * (RU + 3 * F),
         "recon_flux_inner": 8 * V * nb * (I + 3 * inner_faces),
         "recon_flux_outer": 8 * V * nb * (outer_cells + 3 * 6 * n * n),
@@ -369,6 +372,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--driver", choices=["graph", "eager", "tasks"], default="graph")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--form", choices=["auto", "sweep", "overlap", "serial"], default="auto",
+                    help="burgers device driver stage form: one block-sweep launch per stage; "
+                         "plane reconstruction with the exchange on a second stream; plane "
+                         "reconstruction, update, exchange on one stream; auto = timed at "
+                         "initialization")
     ap.add_argument("--cpu-cycles", type=int, default=None)
     ap.add_argument("--inner-cycles", type=int, default=2,
                     help="gmg: FP32 V-cycles per FP64 refinement step")
@@ -408,7 +416,9 @@ def main():
         run = lambda k: [drv.cycle() for _ in range(k)]
         sync = mesh.synchronize
     else:
-        drv = DeviceBurgersDriver(mesh, cfl=0.4, use_graph=(args.driver == "graph"))
+        form = {} if args.form == "auto" else dict(overlap=args.form == "overlap",
+                                                      sweep=args.form == "sweep")
+        drv = DeviceBurgersDriver(mesh, cfl=0.4, use_graph=(args.driver == "graph"), **form)
         drv.initialize("sinusoid", seed=seed)
         run = drv.cycles
         sync = drv.synchronize
@@ -469,7 +479,10 @@ def main():
         "dtype": "f64",
         "data": f"synthetic (seeded sum of sinusoids, SplitMix64 seed {seed})",
         "config": {"workload": workload, "driver": args.driver, "nblocks": nb,
-                   "block": list(block), "nghost": g, "ncomp": V},
+                   "block": list(block), "nghost": g, "ncomp": V,
+                   "stage_form": drv.form() if args.driver != "tasks" else "tasks",
+                   "tuning_planes_per_wg_and_sweep_waves":
+                       drv.tuning() if args.driver != "tasks" else None},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": None, "algorithmic_bytes_per_launch": kb.get(dom),

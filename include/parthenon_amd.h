@@ -181,11 +181,28 @@ pth_status pth_burgers_init(pth_mesh *m, double cfl, int recon);
 /* ncycles RK2 cycles; use_graph != 0 captures one cycle into a hipGraph on first use and
  * replays it. Asynchronous on the driver's own stream. */
 pth_status pth_burgers_run_cycles(pth_mesh *m, int ncycles, int use_graph);
-/* Synchronous read of the device clock: current dt and simulated time. */
-/* Driver option (no counterpart in the reference): with on != 0 (the default) and cubic blocks
- * of 8 or 16 cells with 4 ghosts, each stage's input exchange runs on a second stream beside
- * the reconstruction of the faces whose stencils read no ghost cell; results are identical. */
+/* Driver stage form (no counterpart in the reference; results are identical in every form).
+ * For cubic blocks of 8 or 16 cells with 4 ghosts a stage is either one block-sweep launch
+ * (reconstruction, HLL, flux divergence and RK update per block, no face state or flux in
+ * HBM) followed by the exchange, or the plane-tiled reconstruction into the flux array and
+ * the update, with the stage input's exchange either beside the reconstruction of the faces
+ * whose stencils read no ghost cell (on a second stream) or after the update. Unless one of
+ * the setters below is called first, pth_burgers_init times whole cycles of each form on the
+ * mesh's state (restored afterwards) and keeps the fastest. set_sweep(on != 0) selects the
+ * sweep; set_overlap chooses between the two plane forms when the sweep is off.
+ * pth_burgers_form reports the form (PTH_INCOMPLETE before the automatic choice). */
 pth_status pth_burgers_set_overlap(pth_mesh *m, int on);
+pth_status pth_burgers_set_sweep(pth_mesh *m, int on);
+pth_status pth_burgers_form(pth_mesh *m, int *sweep, int *overlap);
+/* Driver tuning (no counterpart in the reference): planes per workgroup (1, 2 or 4) of the
+ * plane-tiled reconstruction for its three forms (all faces, ghost-free faces, the rest), and
+ * the block sweep's register budget (3 or 4 waves per SIMD) as planes[3] of
+ * pth_burgers_tuning. pth_burgers_init picks them by timing each choice on the mesh's data
+ * unless set here first; pth_burgers_tuning returns PTH_INCOMPLETE while no choice has been
+ * made. Results do not depend on the choice. */
+pth_status pth_burgers_tuning(pth_mesh *m, int planes[4]);
+pth_status pth_burgers_set_tuning(pth_mesh *m, const int planes[4]);
+/* Synchronous read of the device clock: current dt and simulated time. */
 pth_status pth_burgers_clock(pth_mesh *m, double *dt, double *time);
 /* Waits for the driver's stream. */
 pth_status pth_burgers_synchronize(pth_mesh *m);

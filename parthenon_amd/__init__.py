@@ -112,6 +112,10 @@ _SIGNATURES = {
     "pth_burgers_run_cycles": (_I, [_P, _I, _I]),
     "pth_burgers_clock": (_I, [_P, _DP, _DP]),
     "pth_burgers_set_overlap": (_I, [_P, _I]),
+    "pth_burgers_set_sweep": (_I, [_P, _I]),
+    "pth_burgers_form": (_I, [_P, C.POINTER(_I), C.POINTER(_I)]),
+    "pth_burgers_tuning": (_I, [_P, C.POINTER(_I)]),
+    "pth_burgers_set_tuning": (_I, [_P, C.POINTER(_I)]),
     "pth_burgers_synchronize": (_I, [_P]),
     "pth_gmg_create": (_P, [_I, _I, _D, _I, _I]),
     "pth_gmg_destroy": (None, [_P]),

@@ -45,7 +45,7 @@ class DeviceBurgersDriver:
     (use_graph) one captured hipGraph replayed per cycle."""
 
     def __init__(self, mesh, cfl=0.4, recon=burgers_package.WENO5, use_graph=True,
-                 overlap=True):
+                 overlap=None, sweep=None):
         import ctypes as C
         from . import _check, lib
         self._C, self._check, self._lib = C, _check, lib
@@ -55,8 +55,14 @@ class DeviceBurgersDriver:
         self.use_graph = use_graph
         self.base = mesh.mesh_data("base")
         self.one = mesh.mesh_data("1")
-        # each stage's input exchange beside the ghost-free reconstruction (same results)
-        _check(lib().pth_burgers_set_overlap(mesh._h, int(overlap)), "set_overlap")
+        # stage form (same results): sweep = one block-sweep launch per stage; otherwise the
+        # plane reconstruction, with overlap = the stage input's exchange beside the ghost-free
+        # reconstruction on a second stream. Left as None, initialize() times the forms and
+        # keeps the fastest.
+        if overlap is not None:
+            _check(lib().pth_burgers_set_overlap(mesh._h, int(overlap)), "set_overlap")
+        if sweep is not None:
+            _check(lib().pth_burgers_set_sweep(mesh._h, int(sweep)), "set_sweep")
 
     def initialize(self, pgen="sinusoid", seed=0):
         if pgen == "sinusoid":
@@ -71,6 +77,26 @@ class DeviceBurgersDriver:
 
     def synchronize(self):
         self._check(self._lib().pth_burgers_synchronize(self.mesh._h), "synchronize")
+
+    def form(self):
+        """The stage form in use: "sweep", "overlap" or "serial"."""
+        C = self._C
+        sw, ov = C.c_int(), C.c_int()
+        self._lib().pth_burgers_form(self.mesh._h, C.byref(sw), C.byref(ov))
+        return "sweep" if sw.value else ("overlap" if ov.value else "serial")
+
+    def tuning(self):
+        """Planes per workgroup of the three plane-reconstruction forms and the block sweep's
+        waves per SIMD (picked at initialize)."""
+        C = self._C
+        out = (C.c_int * 4)()
+        self._lib().pth_burgers_tuning(self.mesh._h, out)
+        return list(out)
+
+    def set_tuning(self, planes):
+        C = self._C
+        self._check(self._lib().pth_burgers_set_tuning(self.mesh._h, (C.c_int * 4)(*planes)),
+                    "set_tuning")
 
     def clock(self):
         C = self._C

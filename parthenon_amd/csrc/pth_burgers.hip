@@ -16,6 +16,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
+
+#include <algorithm>
 #include <cfloat>
 
 #include "pth_burgers.hpp"
@@ -374,25 +377,31 @@ __global__ void __launch_bounds__(kWG)
 }
 
 // HLL flux of every component at one face from the face states (burgers_package.cpp:307-401
-// with lr_to_flux, burgers_package.hpp:33-43). L, R point at component 0 of the face in the
-// left/right state arrays; comp stride cs; d = normal direction.
-template <int V>
-__device__ __forceinline__ void HllFace(const double *__restrict__ L, const double *__restrict__ R,
-                                        std::int64_t cs, int d, double *F) {
-  const double upl = L[d * cs], upr = R[d * cs];
+// with lr_to_flux, burgers_package.hpp:33-43). l(n), r(n) give component n of the left and
+// right states; d = normal direction.
+template <int V, class LS, class RS>
+__device__ __forceinline__ void HllGen(LS l, RS r, int d, double *F) {
+  const double upl = l(d), upr = r(d);
   const double sl = DMin(DMin(upl, upr), 0.0);
   const double sr = DMax(DMax(upl, upr), 0.0);
   const double islsr = 1.0 / (sr - sl + (sl * sr == 0.0));
 #pragma unroll
   for (int n = 0; n < 3; ++n) {
-    const double ul = L[n * cs], ur = R[n * cs];
+    const double ul = l(n), ur = r(n);
     F[n] = 0.5 * (sr * ul * upl - sl * ur * upr + sl * sr * (ur - ul)) * islsr;
   }
 #pragma unroll
   for (int n = 3; n < V; ++n) {
-    const double ql = L[n * cs], qr = R[n * cs];
+    const double ql = l(n), qr = r(n);
     F[n] = (sr * upl * ql - sl * upr * qr + sl * sr * (qr - ql)) / (sr - sl + (sl * sr == 0.0));
   }
+}
+
+// HllGen over state arrays: L, R point at component 0 of the face; component stride cs.
+template <int V>
+__device__ __forceinline__ void HllFace(const double *__restrict__ L, const double *__restrict__ R,
+                                        std::int64_t cs, int d, double *F) {
+  HllGen<V>([&](int n) { return L[n * cs]; }, [&](int n) { return R[n * cs]; }, d, F);
 }
 
 constexpr int kMaxV = 16;  // components handled by the fused update kernel
@@ -500,12 +509,13 @@ __global__ void __launch_bounds__(kWG)
 
 // ---- plane-tiled reconstruction + HLL (cubic blocks of N <= 16 cells per side) ----------
 //
-// ReconFluxKernel: one workgroup = one plane of one block for one direction. The WENO5Z (or
-// linear) states of the plane's cells, extended by one cell along the direction, go to LDS;
-// then the HLL flux of every face of the plane goes to the flux array [3][block][comp][brick]
-// (x-face i at index i). The operations are those of ReconTaskKernel followed by HllFace, so
-// the fluxes are the same bits, and the face states never reach HBM: the flux array is half
-// their size and the update reads it without recomputing HLL on shared faces.
+// ReconFluxKernel: one workgroup = PPW consecutive planes of one block for one direction. For
+// each plane the WENO5Z (or linear) states of the plane's cells, extended by one cell along the
+// direction, go to LDS; then the HLL flux of every face of the plane goes to the flux array
+// [3][block][comp][brick] (x-face i at index i). The operations are those of ReconTaskKernel
+// followed by HllFace, so the fluxes are the same bits, and the face states never reach HBM:
+// the flux array is half their size and the update reads it without recomputing HLL on shared
+// faces. PPW (planes per workgroup) changes only the work split and is picked by timing.
 // PART selects the cells and faces of each plane row along the direction (ext index a of the
 // N + 2 cells lo - 1 .. hi + 1; face f between ext cells f and f + 1):
 //   0: every cell and face;
@@ -513,6 +523,8 @@ __global__ void __launch_bounds__(kWG)
 //      between two of them (f in [3, N - 3]): no dependence on the boundary exchange;
 //   2: the rest (a in [0, 3] and [N - 2, N + 1], f in [0, 2] and [N - 2, N]); the cells 3 and
 //      N - 2 are evaluated in both parts.
+// The LDS holds the part's cells only: local cell q of a row is ext cell Cell(q), and face q
+// lies between local cells Left(q) and Left(q) + 1.
 template <int N, int PART>
 struct PlaneSet {
   static constexpr int kCells = PART == 0 ? N + 2 : PART == 1 ? N - 4 : 8;
@@ -523,92 +535,99 @@ struct PlaneSet {
   __device__ static int Face(int q) {
     return PART == 0 ? q : PART == 1 ? 3 + q : (q < 3 ? q : N - 5 + q);
   }
+  __device__ static int Left(int q) { return PART == 2 && q >= 3 ? q + 1 : q; }
 };
 
-template <int N, int NG, int V, int PART>
+template <int N, int NG, int V, int PART, int PPW>
 __global__ void __launch_bounds__(kWG)
     ReconFluxKernel(const double *__restrict__ U, double *__restrict__ flux, int nblocks,
                     int recon) {
   constexpr int E = N + 2 * NG;  // brick extent
-  constexpr int P = (N + 2) * N;  // cells of one plane, extended along the direction
   using S = PlaneSet<N, PART>;
+  constexpr int P = S::kCells * N;  // states of one plane per component
+  constexpr int G = N / PPW;        // plane groups per (block, direction)
   __shared__ double sl[V * P], sr[V * P];
-  // logical workgroup = (block * 3 + direction) * N + plane; one block's planes are adjacent
+  // logical workgroup = (block * 3 + direction) * G + group; one block's planes are adjacent
   // and stay on one XCD
   const unsigned L = XcdRemap(blockIdx.x, gridDim.x);
-  const int b = static_cast<int>(L / (3 * N)), r = static_cast<int>(L % (3 * N));
-  const int d = r / N, plane = r % N;
+  const int b = static_cast<int>(L / (3 * G)), r = static_cast<int>(L % (3 * G));
+  const int d = r / G, plane0 = (r % G) * PPW;
   constexpr std::int64_t cs = static_cast<std::int64_t>(E) * E * E;
   const std::int64_t slab = cs * V * nblocks;
   const double *Ub = U + static_cast<std::int64_t>(b) * V * cs;
   const unsigned st = 8u * static_cast<unsigned>(d == 0 ? 1 : d == 1 ? E : E * E);
-  // plane index of (ext cell a along the direction, transverse tv): rows of N + 2 along x for
-  // d = 0, else rows of N along x; consecutive threads take consecutive x
-  for (int t = threadIdx.x; t < V * S::kCells * N; t += kWG) {
-    const int n = t / (S::kCells * N), u = t % (S::kCells * N);
-    int a, tv;
-    if (d == 0) {
-      a = S::Cell(u % S::kCells);
-      tv = u / S::kCells;
-    } else {
-      tv = u % N;
-      a = S::Cell(u / N);
+  for (int pp = 0; pp < PPW; ++pp) {
+    const int plane = plane0 + pp;
+    if (pp > 0) __syncthreads();  // the previous plane's faces have read the states
+    // plane index of (local cell q along the direction, transverse tv): rows of kCells along
+    // x for d = 0, else rows of N along x; consecutive threads take consecutive x
+    for (int t = threadIdx.x; t < V * P; t += kWG) {
+      const int n = t / P, u = t % P;
+      int q, tv;
+      if (d == 0) {
+        q = u % S::kCells;
+        tv = u / S::kCells;
+      } else {
+        tv = u % N;
+        q = u / N;
+      }
+      const int a = S::Cell(q);
+      int i, j, k;
+      if (d == 0) {
+        i = NG - 1 + a;
+        j = NG + tv;
+        k = NG + plane;
+      } else if (d == 1) {
+        i = NG + tv;
+        j = NG - 1 + a;
+        k = NG + plane;
+      } else {
+        i = NG + tv;
+        j = NG + plane;
+        k = NG - 1 + a;
+      }
+      const unsigned c = 8u * static_cast<unsigned>((k * E + j) * E + i);
+      const double *qv = Ub + n * cs;
+      double ql, qr;
+      if (recon == 0)
+        Weno5Z(LoadAt(qv, c - 2 * st), LoadAt(qv, c - st), LoadAt(qv, c), LoadAt(qv, c + st),
+               LoadAt(qv, c + 2 * st), ql, qr);
+      else
+        LinearRecon(LoadAt(qv, c - st), LoadAt(qv, c), LoadAt(qv, c + st), ql, qr);
+      sl[t] = ql;  // t = n * P + u and u is the plane index
+      sr[t] = qr;
     }
-    const int p = d == 0 ? tv * (N + 2) + a : a * N + tv;
-    int i, j, k;
-    if (d == 0) {
-      i = NG - 1 + a;
-      j = NG + tv;
-      k = NG + plane;
-    } else if (d == 1) {
-      i = NG + tv;
-      j = NG - 1 + a;
-      k = NG + plane;
-    } else {
-      i = NG + tv;
-      j = NG + plane;
-      k = NG - 1 + a;
-    }
-    const unsigned c = 8u * static_cast<unsigned>((k * E + j) * E + i);
-    const double *q = Ub + n * cs;
-    double ql, qr;
-    if (recon == 0)
-      Weno5Z(LoadAt(q, c - 2 * st), LoadAt(q, c - st), LoadAt(q, c), LoadAt(q, c + st),
-             LoadAt(q, c + 2 * st), ql, qr);
-    else
-      LinearRecon(LoadAt(q, c - st), LoadAt(q, c), LoadAt(q, c + st), ql, qr);
-    sl[n * P + p] = ql;
-    sr[n * P + p] = qr;
-  }
-  __syncthreads();
-  for (int t = threadIdx.x; t < S::kFaces * N; t += kWG) {
-    int f, tv;
-    if (d == 0) {
-      f = S::Face(t % S::kFaces);
-      tv = t / S::kFaces;
-    } else {
-      tv = t % N;
-      f = S::Face(t / N);
-    }
-    int pl, pr, i, j, k;
-    if (d == 0) {  // face between ext cells f and f + 1 of row tv
-      pl = tv * (N + 2) + f;
-      pr = pl + 1;
-      i = NG + f;
-      j = NG + tv;
-      k = NG + plane;
-    } else {  // face between ext rows f and f + 1
-      pl = f * N + tv;
-      pr = pl + N;
-      i = NG + tv;
-      j = d == 1 ? NG + f : NG + plane;
-      k = d == 1 ? NG + plane : NG + f;
-    }
-    double Fv[V];
-    HllFace<V>(sl + pl, sr + pr, P, d, Fv);
-    double *fo = flux + d * slab + static_cast<std::int64_t>(b) * V * cs + (k * E + j) * E + i;
+    __syncthreads();
+    for (int t = threadIdx.x; t < S::kFaces * N; t += kWG) {
+      int q, tv;
+      if (d == 0) {
+        q = t % S::kFaces;
+        tv = t / S::kFaces;
+      } else {
+        tv = t % N;
+        q = t / N;
+      }
+      const int f = S::Face(q), lq = S::Left(q);
+      int pl, pr, i, j, k;
+      if (d == 0) {  // face between ext cells f and f + 1 of row tv
+        pl = tv * S::kCells + lq;
+        pr = pl + 1;
+        i = NG + f;
+        j = NG + tv;
+        k = NG + plane;
+      } else {  // face between ext rows f and f + 1
+        pl = lq * N + tv;
+        pr = pl + N;
+        i = NG + tv;
+        j = d == 1 ? NG + f : NG + plane;
+        k = d == 1 ? NG + plane : NG + f;
+      }
+      double Fv[V];
+      HllFace<V>(sl + pl, sr + pr, P, d, Fv);
+      double *fo = flux + d * slab + static_cast<std::int64_t>(b) * V * cs + (k * E + j) * E + i;
 #pragma unroll
-    for (int n = 0; n < V; ++n) fo[n * cs] = Fv[n];
+      for (int n = 0; n < V; ++n) fo[n * cs] = Fv[n];
+    }
   }
 }
 
@@ -808,6 +827,245 @@ __global__ void __launch_bounds__(64)
               PatchCoarseCell(P, G, a, c);
   const std::int64_t src = P.dst + a + static_cast<std::int64_t>(c) * P.rows;
   for (int n = 0; n < nc; ++n) F[n * cs] = store[src + static_cast<std::int64_t>(n) * P.comp];
+}
+
+// Left and right states of one cell from its stencil along a direction (byte offsets c of the
+// cell and st of one step): WENO5Z or the MC-limited linear reconstruction.
+__device__ __forceinline__ void CellStates(const double *q, unsigned c, unsigned st, int recon,
+                                           double &ql, double &qr) {
+  if (recon == 0)
+    Weno5Z(LoadAt(q, c - 2 * st), LoadAt(q, c - st), LoadAt(q, c), LoadAt(q, c + st),
+           LoadAt(q, c + 2 * st), ql, qr);
+  else
+    LinearRecon(LoadAt(q, c - st), LoadAt(q, c), LoadAt(q, c + st), ql, qr);
+}
+
+// flxcor_send of the block-sweep stage: one thread per coarse face cell of a patch evaluates
+// the HLL fluxes of its 2x2 fine faces straight from the fine block's state (the left state of
+// face f is ql of cell f - 1, the right state qr of cell f, as CalculateFluxes stores them) and
+// writes their restriction to the correction store.
+template <int V>
+__global__ void __launch_bounds__(64)
+    FlxcorDirectKernel(const FluxPatch *__restrict__ patches, const double *__restrict__ U,
+                       double *__restrict__ store, Geo G, int recon) {
+  const FluxPatch &P = patches[blockIdx.x];
+  const int t = blockIdx.y * 64 + threadIdx.x;
+  if (t >= P.n[0] * P.n[1]) return;
+  const int a = t % P.n[0], c = t / P.n[0];
+  const std::int64_t cs = G.cells;
+  const double *Ub = U + static_cast<std::int64_t>(P.fine) * V * cs;
+  const unsigned st = 8u * static_cast<unsigned>(P.d == 0 ? 1 : P.d == 1 ? G.e0 : G.e0 * G.e1);
+  // flux of fine face (p, q) of this coarse face cell; zero where the patch has no such face
+  auto face = [&](int p, int q, double *F) {
+    if (p >= P.step[0] || q >= P.step[1]) {
+#pragma unroll
+      for (int n = 0; n < V; ++n) F[n] = 0.0;
+      return;
+    }
+    const unsigned o = 8u * static_cast<unsigned>(PatchFineCell(P, G, a, c, p, q));
+    double l[V], r[V];
+#pragma unroll
+    for (int n = 0; n < V; ++n) {
+      double x, y;
+      CellStates(Ub + n * cs, o - st, st, recon, l[n], x);
+      CellStates(Ub + n * cs, o, st, recon, y, r[n]);
+    }
+    HllGen<V>([&](int n) { return l[n]; }, [&](int n) { return r[n]; }, P.d, F);
+  };
+  double F00[V], F01[V], F10[V], F11[V];
+  face(0, 0, F00);
+  face(1, 0, F01);
+  face(0, 1, F10);
+  face(1, 1, F11);
+  const std::int64_t dst = P.dst + a + static_cast<std::int64_t>(c) * P.rows;
+#pragma unroll
+  for (int n = 0; n < V; ++n)
+    store[dst + static_cast<std::int64_t>(n) * P.comp] = RestrictFace(
+        P.d, P.area, F00[n], F01[n], F10[n], F11[n], P.step[0], P.step[1]);
+}
+
+// ---- block sweep: reconstruction, HLL, flux divergence and RK update in one launch ----------
+//
+// One thread per (i, j) column of a cubic block of N cells per side, marching k over interior
+// planes. Per plane: the x states of the plane's rows (extended by one cell each way) go to
+// LDS and each thread takes the HLL fluxes of its cell's two x faces; then the same for y;
+// then the z states of the plane above, whose qr closes the high z face (the low z face of
+// plane k is the high z face of plane k - 1, carried in registers). The flux divergence sums
+// x, y, z in the order of FluxDivHelper (update.hpp:43-57), then the RK averaging and update,
+// the derived field and the CFL minimum follow as in FluxUpdateKernel. Every face flux is the
+// same expression as in the other forms (shared faces are evaluated by both neighbouring
+// cells), so the stage output is the same bits, and no flux or face state reaches HBM. On
+// multilevel meshes, faces with a finer neighbour take the restricted fine fluxes from the
+// correction store (fover / fcor as in FusedUpdateKernel).
+// The grid is persistent: workgroup g takes planes [g T / G, (g + 1) T / G) of the
+// T = nblocks * N planes in block order, so every workgroup does the same number of planes
+// (a segment that starts inside a block first rebuilds the z state below its first plane).
+// partial[g] is the workgroup's CFL minimum.
+
+template <int N, int NG, int V, int MINW>
+__global__ void __launch_bounds__(N * N, MINW)
+    BlockSweepKernel(const double *__restrict__ U, const double *base, double *out,
+                     double *__restrict__ derived, double *__restrict__ partial,
+                     const BlockCoefs *__restrict__ coef, int nblocks, double beta,
+                     const double *__restrict__ dtvar, int recon, const int *__restrict__ fover,
+                     const double *__restrict__ fcor) {
+  constexpr int E = N + 2 * NG;
+  constexpr int NT = N * N;
+  constexpr int R = N + 2;  // states per row: ext cells lo - 1 .. hi + 1
+  constexpr int P = R * N;  // states of one plane per component
+  __shared__ double sl[V * P], sr[V * P];
+  __shared__ double wmin[(NT + 63) / 64];
+  const int tx = threadIdx.x % N, ty = threadIdx.x / N;
+  constexpr std::int64_t cs = static_cast<std::int64_t>(E) * E * E;
+  const double bdt = beta * dtvar[0];
+  const double w2 = 1.0 - beta;
+  constexpr int comp = N * N;  // correction-store stride between components
+  constexpr unsigned sx = 8u, sy = 8u * E, sz = 8u * E * E;
+  double m = DBL_MAX;
+  const std::int64_t T = static_cast<std::int64_t>(nblocks) * N;
+  std::int64_t pnext = T * blockIdx.x / gridDim.x;
+  const std::int64_t pend = T * (blockIdx.x + 1) / gridDim.x;
+  while (pnext < pend) {
+    const int b = static_cast<int>(pnext / N);
+    const int k0 = static_cast<int>(pnext % N);
+    const int k1 = static_cast<int>(pend - pnext < N - k0 ? k0 + (pend - pnext) : N);
+    pnext += k1 - k0;
+    const double *Ub = U + static_cast<std::int64_t>(b) * V * cs;
+    const BlockCoefs &C = coef[b];
+    const int ox0 = fover ? fover[6 * b + 0] : -1, ox1 = fover ? fover[6 * b + 1] : -1;
+    const int oy0 = fover ? fover[6 * b + 2] : -1, oy1 = fover ? fover[6 * b + 3] : -1;
+    const int oz0 = fover ? fover[6 * b + 4] : -1, oz1 = fover ? fover[6 * b + 5] : -1;
+    // z states of one plane of cells kc for every column, [n][j][i] (a flat loop like x and y)
+    auto zstates = [&](int kc) {
+      for (int t = threadIdx.x; t < V * NT; t += NT) {
+        const int n = t / NT, u = t % NT;
+        const unsigned c = 8u * static_cast<unsigned>((kc * E + NG + u / N) * E + NG + u % N);
+        CellStates(Ub + n * cs, c, sz, recon, sl[n * P + u], sr[n * P + u]);
+      }
+    };
+    // z faces: zl = ql of the cell below the current plane, zf = flux on the current low z face
+    double zl[V], zf[V];
+    __syncthreads();  // the previous segment's states have been read
+    zstates(NG + k0 - 1);
+    __syncthreads();
+#pragma unroll
+    for (int n = 0; n < V; ++n) zl[n] = sl[n * P + threadIdx.x];
+    __syncthreads();
+    zstates(NG + k0);
+    __syncthreads();
+    HllGen<V>([&](int n) { return zl[n]; }, [&](int n) { return sr[n * P + threadIdx.x]; }, 2, zf);
+    if (k0 == 0 && oz0 >= 0)
+#pragma unroll
+      for (int n = 0; n < V; ++n) zf[n] = fcor[oz0 + tx + ty * N + n * comp];
+#pragma unroll
+    for (int n = 0; n < V; ++n) zl[n] = sl[n * P + threadIdx.x];
+    for (int k = k0; k < k1; ++k) {
+      const int kk = NG + k;
+      double du[V];
+      // ---- x: states of ext cells lo - 1 .. hi + 1 of every row of the plane, [n][row][a]
+      __syncthreads();  // the previous states have been read
+      for (int t = threadIdx.x; t < V * P; t += NT) {
+        const int n = t / P, u = t % P;
+        const int a = u % R, row = u / R;
+        const unsigned c = 8u * static_cast<unsigned>((kk * E + NG + row) * E + NG - 1 + a);
+        CellStates(Ub + n * cs, c, sx, recon, sl[t], sr[t]);
+      }
+      __syncthreads();
+      {
+        // du = A F(hi) - A F(lo); A F(lo) is formed first so one flux array is live at a time
+        double F[V];
+        const double *L = sl + ty * R + tx, *Rr = sr + ty * R + tx;
+        const double A = C.area[0];
+        HllGen<V>([&](int n) { return L[n * P]; }, [&](int n) { return Rr[n * P + 1]; }, 0, F);
+        if (tx == 0 && ox0 >= 0)
+#pragma unroll
+          for (int n = 0; n < V; ++n) F[n] = fcor[ox0 + ty + k * N + n * comp];
+#pragma unroll
+        for (int n = 0; n < V; ++n) du[n] = A * F[n];
+        HllGen<V>([&](int n) { return L[n * P + 1]; }, [&](int n) { return Rr[n * P + 2]; }, 0,
+                  F);
+        if (tx == N - 1 && ox1 >= 0)
+#pragma unroll
+          for (int n = 0; n < V; ++n) F[n] = fcor[ox1 + ty + k * N + n * comp];
+#pragma unroll
+        for (int n = 0; n < V; ++n) du[n] = (A * F[n] - du[n]);
+      }
+      __syncthreads();  // the x states have been read
+      // ---- y: states of ext cells lo - 1 .. hi + 1 of every column, [n][a][i]
+      for (int t = threadIdx.x; t < V * P; t += NT) {
+        const int n = t / P, u = t % P;
+        const int i = u % N, a = u / N;
+        const unsigned c = 8u * static_cast<unsigned>((kk * E + NG - 1 + a) * E + NG + i);
+        CellStates(Ub + n * cs, c, sy, recon, sl[t], sr[t]);
+      }
+      __syncthreads();
+      {
+        double F[V], alo[V];
+        const double *L = sl + ty * N + tx, *Rr = sr + ty * N + tx;
+        const double A = C.area[1];
+        HllGen<V>([&](int n) { return L[n * P]; }, [&](int n) { return Rr[n * P + N]; }, 1, F);
+        if (ty == 0 && oy0 >= 0)
+#pragma unroll
+          for (int n = 0; n < V; ++n) F[n] = fcor[oy0 + tx + k * N + n * comp];
+#pragma unroll
+        for (int n = 0; n < V; ++n) alo[n] = A * F[n];
+        HllGen<V>([&](int n) { return L[n * P + N]; }, [&](int n) { return Rr[n * P + 2 * N]; },
+                  1, F);
+        if (ty == N - 1 && oy1 >= 0)
+#pragma unroll
+          for (int n = 0; n < V; ++n) F[n] = fcor[oy1 + tx + k * N + n * comp];
+#pragma unroll
+        for (int n = 0; n < V; ++n) du[n] = du[n] + (A * F[n] - alo[n]);
+      }
+      __syncthreads();  // the y states have been read
+      // ---- z: qr of the cell above this plane gives the high z face; its ql moves down
+      zstates(kk + 1);
+      __syncthreads();
+      {
+        double Fhi[V];
+        HllGen<V>([&](int n) { return zl[n]; }, [&](int n) { return sr[n * P + threadIdx.x]; }, 2,
+                  Fhi);
+        if (k == N - 1 && oz1 >= 0)
+#pragma unroll
+          for (int n = 0; n < V; ++n) Fhi[n] = fcor[oz1 + tx + ty * N + n * comp];
+        const double A = C.area[2];
+#pragma unroll
+        for (int n = 0; n < V; ++n) {
+          du[n] = du[n] + (A * Fhi[n] - A * zf[n]);
+          zf[n] = Fhi[n];
+          zl[n] = sl[n * P + threadIdx.x];
+        }
+      }
+      // ---- divergence, RK averaging and update, derived field, CFL
+      const std::int64_t cell = (static_cast<std::int64_t>(kk) * E + NG + ty) * E + NG + tx;
+      const std::int64_t c0 = static_cast<std::int64_t>(b) * V * cs + cell;
+      double u1[V];
+#pragma unroll
+      for (int n = 0; n < V; ++n) {
+        const double dudt = -du[n] / C.vol;
+        const double avg = beta * U[c0 + n * cs] + w2 * base[c0 + n * cs];
+        u1[n] = 1.0 * avg + bdt * dudt;
+        out[c0 + n * cs] = u1[n];
+      }
+      if (derived)
+        derived[static_cast<std::int64_t>(b) * cs + cell] =
+            0.5 * u1[3] * (u1[0] * u1[0] + u1[1] * u1[1] + u1[2] * u1[2]);
+      double sp = fabs(u1[0]) / C.dx[0];
+      sp = sp + fabs(u1[1]) / C.dx[1];
+      sp = sp + fabs(u1[2]) / C.dx[2];
+      m = DMin(m, 1.0 / sp);
+    }
+  }
+  if (!partial) return;
+  m = WaveMin(m);
+  if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v = wmin[0];
+#pragma unroll
+    for (int w = 1; w < (NT + 63) / 64; ++w) v = DMin(v, wmin[w]);
+    partial[blockIdx.x] = v;
+  }
 }
 
 // burgers_package::MassHistory (benchmarks/burgers/burgers_package.cpp:406-439) for the 8
@@ -1130,26 +1388,34 @@ bool ReconFluxSupported(const Topology &t, int V) {
 }
 
 pth_status BurgersReconFlux(const Topology &t, int V, const double *U, double *flux, int recon,
-                            int part, hipStream_t s) {
-  if (!ReconFluxSupported(t, V) || part < 0 || part > 2) {
-    SetError("plane-tiled reconstruction: unsupported block shape or component count");
+                            int part, int ppw, hipStream_t s) {
+  if (!ReconFluxSupported(t, V) || part < 0 || part > 2 || (ppw != 1 && ppw != 2 && ppw != 4)) {
+    SetError("plane-tiled reconstruction: unsupported block shape, component count or split");
     return PTH_FAIL;
   }
   ProfScope prof(part == 0 ? "recon_flux" : part == 1 ? "recon_flux_inner" : "recon_flux_outer",
                  s);
   const int nb = static_cast<int>(t.blocks.size());
   const int N = t.fine.n[0];
-  const dim3 grid(static_cast<unsigned>(3 * N * nb));
-#define PTH_RF_LAUNCH(NN, NV, PT)                                                             \
-  hipLaunchKernelGGL((ReconFluxKernel<NN, 4, NV, PT>), grid, dim3(kWG), 0, s, U, flux, nb,     \
+  const dim3 grid(static_cast<unsigned>(3 * (N / ppw) * nb));
+#define PTH_RF_LAUNCH(NN, NV, PT, PW)                                                         \
+  hipLaunchKernelGGL((ReconFluxKernel<NN, 4, NV, PT, PW>), grid, dim3(kWG), 0, s, U, flux, nb, \
                      recon)
-#define PTH_RF_PARTS(NN, NV)                                                                  \
-  if (part == 0)                                                                              \
-    PTH_RF_LAUNCH(NN, NV, 0);                                                                 \
-  else if (part == 1)                                                                         \
-    PTH_RF_LAUNCH(NN, NV, 1);                                                                 \
+#define PTH_RF_PPW(NN, NV, PT)                                                                \
+  if (ppw == 1)                                                                               \
+    PTH_RF_LAUNCH(NN, NV, PT, 1);                                                             \
+  else if (ppw == 2)                                                                          \
+    PTH_RF_LAUNCH(NN, NV, PT, 2);                                                             \
   else                                                                                        \
-    PTH_RF_LAUNCH(NN, NV, 2);
+    PTH_RF_LAUNCH(NN, NV, PT, 4);
+#define PTH_RF_PARTS(NN, NV)                                                                  \
+  if (part == 0) {                                                                            \
+    PTH_RF_PPW(NN, NV, 0)                                                                     \
+  } else if (part == 1) {                                                                     \
+    PTH_RF_PPW(NN, NV, 1)                                                                     \
+  } else {                                                                                    \
+    PTH_RF_PPW(NN, NV, 2)                                                                     \
+  }
 #define PTH_RF_CASE(NV)                                                                       \
   case NV:                                                                                    \
     if (N == 16) {                                                                            \
@@ -1172,7 +1438,119 @@ pth_status BurgersReconFlux(const Topology &t, int V, const double *U, double *f
   }
 #undef PTH_RF_CASE
 #undef PTH_RF_PARTS
+#undef PTH_RF_PPW
 #undef PTH_RF_LAUNCH
+  PTH_HIP(hipGetLastError());
+  return PTH_COMPLETE;
+}
+
+bool BurgersSweepSupported(const Topology &t, int V) { return ReconFluxSupported(t, V); }
+
+// Workgroups of a kernel resident on the whole device at once (occupancy x compute units).
+int ResidentGroups(const void *kernel, int threads) {
+  static std::map<const void *, int> cache;
+  auto it = cache.find(kernel);
+  if (it != cache.end()) return it->second;
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess ||
+      per < 1 || cus < 1)
+    return -1;
+  cache[kernel] = per * cus;
+  return per * cus;
+}
+
+pth_status BurgersSweep(const Topology &t, int V, const double *U, const double *base,
+                        double *out, double *derived, double *partial, int max_partials,
+                        const BlockCoefs *coef, double beta, const double *dtvar, int recon,
+                        const FluxCorrection *fc, int waves, int *groups, hipStream_t s) {
+  if (!BurgersSweepSupported(t, V) || (waves != 3 && waves != 4)) {
+    SetError("block sweep: unsupported block shape or component count");
+    return PTH_FAIL;
+  }
+  ProfScope prof("block_sweep", s);
+  const int nb = static_cast<int>(t.blocks.size());
+  const int N = t.fine.n[0];
+  const int *fover = fc ? fc->dev_face : nullptr;
+  const double *fcor = fc ? fc->store : nullptr;
+  int grid = 0;
+#define PTH_SW_LAUNCH_W(NN, NV, W)                                                            \
+  {                                                                                           \
+    const int r = ResidentGroups(reinterpret_cast<const void *>(&BlockSweepKernel<NN, 4, NV, W>), \
+                                 NN * NN);                                                    \
+    if (r < 1) {                                                                              \
+      SetError("block sweep: occupancy query failed");                                        \
+      return PTH_FAIL;                                                                        \
+    }                                                                                         \
+    grid = std::min<std::int64_t>(std::min(r, max_partials), static_cast<std::int64_t>(nb) * NN); \
+    hipLaunchKernelGGL((BlockSweepKernel<NN, 4, NV, W>), dim3(grid), dim3(NN * NN), 0, s, U,  \
+                       base, out, derived, partial, coef, nb, beta, dtvar, recon, fover, fcor); \
+  }
+#define PTH_SW_LAUNCH(NN, NV)                                                                 \
+  if (waves == 4)                                                                             \
+    PTH_SW_LAUNCH_W(NN, NV, (NV <= 8 ? 4 : 2))                                                \
+  else                                                                                        \
+    PTH_SW_LAUNCH_W(NN, NV, (NV <= 8 ? 3 : 2))
+#define PTH_SW_CASE(NV)                                                                       \
+  case NV:                                                                                    \
+    if (N == 16) {                                                                            \
+      PTH_SW_LAUNCH(16, NV)                                                                   \
+    } else {                                                                                  \
+      PTH_SW_LAUNCH(8, NV)                                                                    \
+    }                                                                                         \
+    break;
+  switch (V) {
+    PTH_SW_CASE(4)
+    PTH_SW_CASE(5)
+    PTH_SW_CASE(6)
+    PTH_SW_CASE(7)
+    PTH_SW_CASE(8)
+    PTH_SW_CASE(9)
+    PTH_SW_CASE(10)
+    PTH_SW_CASE(11)
+    PTH_SW_CASE(12)
+    PTH_SW_CASE(13)
+  }
+#undef PTH_SW_CASE
+#undef PTH_SW_LAUNCH
+#undef PTH_SW_LAUNCH_W
+  PTH_HIP(hipGetLastError());
+  if (groups) *groups = grid;
+  return PTH_COMPLETE;
+}
+
+pth_status FlxcorDirect(const Topology &t, int V, const FluxCorrection &fc, const double *U,
+                        int recon, hipStream_t s) {
+  if (fc.host.empty()) return PTH_COMPLETE;
+  if (V < 4 || V > 16) {
+    SetError("flux correction supports 4 to 16 components");
+    return PTH_FAIL;
+  }
+  ProfScope prof("flxcor_direct", s);
+  const Geo G = MakeGeo(t, V);
+  const dim3 grid(static_cast<unsigned>(fc.host.size()), (fc.max_cells + 63) / 64);
+#define PTH_FD_CASE(NV)                                                                       \
+  case NV:                                                                                    \
+    hipLaunchKernelGGL(FlxcorDirectKernel<NV>, grid, dim3(64), 0, s, fc.dev, U, fc.store, G,  \
+                       recon);                                                                \
+    break;
+  switch (V) {
+    PTH_FD_CASE(4)
+    PTH_FD_CASE(5)
+    PTH_FD_CASE(6)
+    PTH_FD_CASE(7)
+    PTH_FD_CASE(8)
+    PTH_FD_CASE(9)
+    PTH_FD_CASE(10)
+    PTH_FD_CASE(11)
+    PTH_FD_CASE(12)
+    PTH_FD_CASE(13)
+    PTH_FD_CASE(14)
+    PTH_FD_CASE(15)
+    PTH_FD_CASE(16)
+  }
+#undef PTH_FD_CASE
   PTH_HIP(hipGetLastError());
   return PTH_COMPLETE;
 }

@@ -35,8 +35,20 @@ pth_status FlxcorSet(const Topology &t, const FluxCorrection &fc, double *flux, 
 // plane-tiled reconstruction + HLL into the flux array, and the update from stored fluxes
 bool ReconFluxSupported(const Topology &t, int V);
 // part 0: every face; 1: faces independent of ghost cells; 2: the remaining faces
+bool BurgersSweepSupported(const Topology &t, int V);
+// one launch per stage: reconstruction, HLL, flux divergence, RK update, derived, CFL partial
+// per workgroup (partial[0 .. *groups), at most max_partials workgroups); faces with a finer
+// neighbour take fc's corrected fluxes; waves (3 or 4) is the register budget the kernel is
+// compiled for (waves per SIMD)
+pth_status BurgersSweep(const Topology &t, int V, const double *U, const double *base,
+                        double *out, double *derived, double *partial, int max_partials,
+                        const BlockCoefs *coef, double beta, const double *dtvar, int recon,
+                        const FluxCorrection *fc, int waves, int *groups, hipStream_t s);
+// flux-correction send of the sweep: fine fluxes from the state, restricted into fc's store
+pth_status FlxcorDirect(const Topology &t, int V, const FluxCorrection &fc, const double *U,
+                        int recon, hipStream_t s);
 pth_status BurgersReconFlux(const Topology &t, int V, const double *U, double *flux, int recon,
-                            int part, hipStream_t s);
+                            int part, int ppw, hipStream_t s);
 pth_status BurgersFluxUpdate(const Topology &t, int V, const double *flux, const double *U0,
                              const double *base, double *out, double *derived, double *partial,
                              const BlockCoefs *coef, double beta, const double *dtvar,

@@ -25,6 +25,7 @@
 
 #include <cfloat>
 #include <cstring>
+#include <vector>
 
 #include "pth_burgers.hpp"
 
@@ -98,7 +99,8 @@ pth_status Setup(Mesh *m, Ctx &x) {
   if (!m->dtvar) PTH_HIP(hipMalloc(&m->dtvar, sizeof(double) * 4));
   const std::int64_t nd = x.base->store[x.fu].ndata;
   const bool planes = x.base->store[x.fu].flux && ReconFluxSupported(m->topo, x.V);
-  if (!planes && (!m->face || m->face_V != x.V)) {  // face-state scratch of the other path
+  const bool sweep = m->run_sweep && BurgersSweepSupported(m->topo, x.V);
+  if (!planes && !sweep && (!m->face || m->face_V != x.V)) {  // face states of the other path
     if (m->face) (void)hipFree(m->face);
     m->face = nullptr;
     PTH_HIP(hipMalloc(&m->face, sizeof(double) * 6 * nd));
@@ -122,7 +124,7 @@ pth_status Stage(Ctx &x, int stage, hipStream_t s) {
   if (flux && ReconFluxSupported(m.topo, x.V)) {
     // plane-tiled reconstruction + HLL into the flux array, the flux-correction tasks on the
     // stored fluxes (LoadAndSendFluxCorrections, SetFluxCorrections), then the update
-    if (BurgersReconFlux(m.topo, x.V, mc0.store[x.fu].data, flux, m.run_recon, 0, s) !=
+    if (BurgersReconFlux(m.topo, x.V, mc0.store[x.fu].data, flux, m.run_recon, 0, m.rf_ppw[0], s) !=
         PTH_COMPLETE)
       return PTH_FAIL;
     if (x.fc && (FlxcorSend(m.topo, *x.fc, flux, s) != PTH_COMPLETE ||
@@ -152,8 +154,35 @@ pth_status Stage(Ctx &x, int stage, hipStream_t s) {
   return PTH_COMPLETE;
 }
 
+bool Swept(const Ctx &x) {
+  return x.m->run_sweep && BurgersSweepSupported(x.m->topo, x.V);
+}
+
 bool Overlapped(const Ctx &x) {
-  return x.m->run_overlap && x.base->store[x.fu].flux && ReconFluxSupported(x.m->topo, x.V);
+  return !Swept(x) && x.m->run_overlap && x.base->store[x.fu].flux &&
+         ReconFluxSupported(x.m->topo, x.V);
+}
+
+// The stage as one block-sweep launch (after the flux-correction send on multilevel meshes,
+// whose restricted fine fluxes the sweep reads for the faces that have a finer neighbour),
+// then the exchange of its output.
+pth_status StageSweep(Ctx &x, int stage, hipStream_t s) {
+  Mesh &m = *x.m;
+  MeshData &mc0 = stage == 1 ? *x.base : *x.one;
+  MeshData &mc1 = stage == 1 ? *x.one : *x.base;
+  const double beta = stage == 1 ? 1.0 : 0.5;
+  double *derived = x.fd >= 0 ? x.base->store[x.fd].data : nullptr;
+  const double *U = mc0.store[x.fu].data;
+  if (x.fc && FlxcorDirect(m.topo, x.V, *x.fc, U, m.run_recon, s) != PTH_COMPLETE)
+    return PTH_FAIL;
+  int groups = 0;
+  if (BurgersSweep(m.topo, x.V, U, x.base->store[x.fu].data, mc1.store[x.fu].data, derived,
+                   stage == 2 ? m.scratch : nullptr, static_cast<int>(m.scratch_n), m.coef, beta,
+                   m.dtvar, m.run_recon, x.fc, m.sweep_waves, &groups, s) != PTH_COMPLETE)
+    return PTH_FAIL;
+  if (Exchange(m, mc1, s) != PTH_COMPLETE) return PTH_FAIL;
+  if (stage == 2) return BurgersClock(m.scratch, groups, m.run_cfl, m.dtvar, 1, s);
+  return PTH_COMPLETE;
 }
 
 // The stage with its input's boundary exchange moved from the end of the previous stage to
@@ -173,11 +202,11 @@ pth_status StageOverlap(Ctx &x, int stage, hipStream_t s) {
   PTH_HIP(hipEventRecord(m.fork_ev[0], s));
   PTH_HIP(hipStreamWaitEvent(m.side_stream, m.fork_ev[0], 0));
   if (Exchange(m, mc0, m.side_stream) != PTH_COMPLETE) return PTH_FAIL;
-  if (BurgersReconFlux(m.topo, x.V, U, flux, m.run_recon, 1, s) != PTH_COMPLETE)
+  if (BurgersReconFlux(m.topo, x.V, U, flux, m.run_recon, 1, m.rf_ppw[1], s) != PTH_COMPLETE)
     return PTH_FAIL;
   PTH_HIP(hipEventRecord(m.fork_ev[1], m.side_stream));
   PTH_HIP(hipStreamWaitEvent(s, m.fork_ev[1], 0));
-  if (BurgersReconFlux(m.topo, x.V, U, flux, m.run_recon, 2, s) != PTH_COMPLETE)
+  if (BurgersReconFlux(m.topo, x.V, U, flux, m.run_recon, 2, m.rf_ppw[2], s) != PTH_COMPLETE)
     return PTH_FAIL;
   if (x.fc && (FlxcorSend(m.topo, *x.fc, flux, s) != PTH_COMPLETE ||
                FlxcorSet(m.topo, *x.fc, flux, s) != PTH_COMPLETE))
@@ -191,7 +220,174 @@ pth_status StageOverlap(Ctx &x, int stage, hipStream_t s) {
   return PTH_COMPLETE;
 }
 
+// Picks the planes per workgroup of each reconstruction part by timing every choice on the
+// mesh's own data (the flux array is scratch between stages, so the trial launches change
+// nothing); the choice changes the work split only, never a result.
+pth_status TuneReconFlux(Ctx &x, hipStream_t s) {
+  Mesh &m = *x.m;
+  double *flux = x.base->store[x.fu].flux;
+  const bool planes = flux && ReconFluxSupported(m.topo, x.V);
+  const double *U = x.base->store[x.fu].data;
+  hipEvent_t ev[2];
+  PTH_HIP(hipEventCreate(&ev[0]));
+  PTH_HIP(hipEventCreate(&ev[1]));
+  const bool prof = Prof().on;
+  Prof().on = false;
+  pth_status st = PTH_COMPLETE;
+  for (int part = 0; planes && part < 3 && st == PTH_COMPLETE; ++part) {
+    float best = 0.0f;
+    for (int ppw : {1, 2, 4}) {
+      st = BurgersReconFlux(m.topo, x.V, U, flux, m.run_recon, part, ppw, s);  // warm-up
+      if (st != PTH_COMPLETE) break;
+      if (hipEventRecord(ev[0], s) != hipSuccess) st = PTH_FAIL;
+      for (int r = 0; r < 3 && st == PTH_COMPLETE; ++r)
+        st = BurgersReconFlux(m.topo, x.V, U, flux, m.run_recon, part, ppw, s);
+      float ms = 0.0f;
+      if (st != PTH_COMPLETE || hipEventRecord(ev[1], s) != hipSuccess ||
+          hipEventSynchronize(ev[1]) != hipSuccess ||
+          hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess) {
+        if (st == PTH_COMPLETE) SetError("reconstruction tuning: event timing failed");
+        st = PTH_FAIL;
+        break;
+      }
+      if (ppw == 1 || ms < best) {
+        best = ms;
+        m.rf_ppw[part] = ppw;
+      }
+    }
+  }
+  // block sweep: the register budget (3 or 4 waves per SIMD); the trial writes only the
+  // interior of stage "1", which the next stage overwrites
+  if (st == PTH_COMPLETE && BurgersSweepSupported(m.topo, x.V)) {
+    float best = 0.0f;
+    for (int waves : {4, 3}) {
+      st = BurgersSweep(m.topo, x.V, U, U, x.one->store[x.fu].data, nullptr, nullptr,
+                        static_cast<int>(m.scratch_n), m.coef, 1.0, m.dtvar, m.run_recon, x.fc,
+                        waves, nullptr, s);
+      if (st != PTH_COMPLETE) break;
+      if (hipEventRecord(ev[0], s) != hipSuccess) st = PTH_FAIL;
+      for (int r = 0; r < 3 && st == PTH_COMPLETE; ++r)
+        st = BurgersSweep(m.topo, x.V, U, U, x.one->store[x.fu].data, nullptr, nullptr,
+                          static_cast<int>(m.scratch_n), m.coef, 1.0, m.dtvar, m.run_recon,
+                          x.fc, waves, nullptr, s);
+      float ms = 0.0f;
+      if (st != PTH_COMPLETE || hipEventRecord(ev[1], s) != hipSuccess ||
+          hipEventSynchronize(ev[1]) != hipSuccess ||
+          hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess) {
+        if (st == PTH_COMPLETE) SetError("sweep tuning: event timing failed");
+        st = PTH_FAIL;
+        break;
+      }
+      if (waves == 4 || ms < best) {
+        best = ms;
+        m.sweep_waves = waves;
+      }
+    }
+  }
+  Prof().on = prof;
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  if (st == PTH_COMPLETE) m.rf_tuned = true;
+  return st;
+}
+
+pth_status Cycle(Ctx &x, hipStream_t s);
+
+// Picks the stage form (block sweep; plane reconstruction with the exchange on a second
+// stream; plane reconstruction then update then exchange) by timing whole cycles of each on
+// the mesh's own state, which is saved first and restored afterwards (U of "base", the
+// derived field and the device clock). The forms give identical results.
+pth_status TuneForm(Ctx &x, hipStream_t s) {
+  Mesh &m = *x.m;
+  struct Form {
+    int sweep, overlap;
+  };
+  std::vector<Form> forms;
+  if (BurgersSweepSupported(m.topo, x.V)) forms.push_back({1, 0});
+  if (x.base->store[x.fu].flux && ReconFluxSupported(m.topo, x.V)) {
+    forms.push_back({0, 1});
+    forms.push_back({0, 0});
+  }
+  if (forms.size() < 2) {
+    m.form_tuned = true;
+    return PTH_COMPLETE;
+  }
+  const std::int64_t nd = x.base->store[x.fu].ndata;
+  double *U = x.base->store[x.fu].data;
+  double *D = x.fd >= 0 ? x.base->store[x.fd].data : nullptr;
+  const std::int64_t ndd = x.fd >= 0 ? x.base->store[x.fd].ndata : 0;
+  double *save = nullptr;
+  PTH_HIP(hipMalloc(&save, sizeof(double) * (nd + ndd + 4)));
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  pth_status st = PTH_COMPLETE;
+  auto copy = [&](double *dst, const double *src, std::int64_t n) {
+    if (st == PTH_COMPLETE && n > 0 &&
+        hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyDeviceToDevice, s) != hipSuccess) {
+      SetError("stage-form tuning: state copy failed");
+      st = PTH_FAIL;
+    }
+  };
+  copy(save, U, nd);
+  if (D) copy(save + nd, D, ndd);
+  copy(save + nd + ndd, m.dtvar, 4);
+  if (hipEventCreate(&ev[0]) != hipSuccess || hipEventCreate(&ev[1]) != hipSuccess) {
+    SetError("stage-form tuning: event creation failed");
+    st = PTH_FAIL;
+  }
+  const bool prof = Prof().on;
+  Prof().on = false;
+  const int keep_sweep = m.run_sweep, keep_overlap = m.run_overlap;
+  float best = 0.0f;
+  Form pick = forms[0];
+  for (const Form &f : forms) {
+    if (st != PTH_COMPLETE) break;
+    m.run_sweep = f.sweep;
+    m.run_overlap = f.overlap;
+    copy(U, save, nd);
+    copy(m.dtvar, save + nd + ndd, 4);
+    if (st == PTH_COMPLETE) st = Cycle(x, s);  // warm-up
+    if (st == PTH_COMPLETE && hipEventRecord(ev[0], s) != hipSuccess) st = PTH_FAIL;
+    for (int c = 0; c < 2 && st == PTH_COMPLETE; ++c) st = Cycle(x, s);
+    float ms = 0.0f;
+    if (st != PTH_COMPLETE || hipEventRecord(ev[1], s) != hipSuccess ||
+        hipEventSynchronize(ev[1]) != hipSuccess ||
+        hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess) {
+      if (st == PTH_COMPLETE) SetError("stage-form tuning: event timing failed");
+      st = PTH_FAIL;
+      break;
+    }
+    if (&f == &forms[0] || ms < best) {
+      best = ms;
+      pick = f;
+    }
+  }
+  Prof().on = prof;
+  copy(U, save, nd);
+  if (D) copy(D, save + nd, ndd);
+  copy(m.dtvar, save + nd + ndd, 4);
+  if (st == PTH_COMPLETE && hipStreamSynchronize(s) != hipSuccess) {
+    SetError("stage-form tuning: synchronize failed");
+    st = PTH_FAIL;
+  }
+  (void)hipFree(save);
+  for (hipEvent_t e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (st != PTH_COMPLETE) {
+    m.run_sweep = keep_sweep;
+    m.run_overlap = keep_overlap;
+    return st;
+  }
+  m.run_sweep = pick.sweep;
+  m.run_overlap = pick.overlap;
+  m.form_tuned = true;
+  return PTH_COMPLETE;
+}
+
 pth_status Cycle(Ctx &x, hipStream_t s) {
+  if (Swept(x)) {
+    if (StageSweep(x, 1, s) != PTH_COMPLETE) return PTH_FAIL;
+    return StageSweep(x, 2, s);
+  }
   if (Overlapped(x)) {
     if (StageOverlap(x, 1, s) != PTH_COMPLETE) return PTH_FAIL;
     return StageOverlap(x, 2, s);
@@ -288,8 +484,45 @@ pth_status pth_burgers_init(pth_mesh *mh, double cfl, int recon) {
   // the estimate of BurgersDtMin is the un-scaled minimum; apply the CFL and the clock rule
   if (BurgersClock(m->scratch + m->scratch_n, 1, cfl, m->dtvar, 0, s) != PTH_COMPLETE)
     return PTH_FAIL;
+  if (!m->rf_tuned && TuneReconFlux(x, s) != PTH_COMPLETE) return PTH_FAIL;
+  if (m->form_auto && !m->form_tuned && TuneForm(x, s) != PTH_COMPLETE) return PTH_FAIL;
   PTH_HIP(hipStreamSynchronize(s));
   m->run_ready = true;
+  return PTH_COMPLETE;
+}
+
+pth_status pth_burgers_form(pth_mesh *mh, int *sweep, int *overlap) {
+  Mesh *m = Impl(mh);
+  if (sweep) *sweep = m->run_sweep;
+  if (overlap) *overlap = m->run_overlap;
+  return m->form_auto && !m->form_tuned ? PTH_INCOMPLETE : PTH_COMPLETE;
+}
+
+pth_status pth_burgers_tuning(pth_mesh *mh, int planes[4]) {
+  Mesh *m = Impl(mh);
+  for (int p = 0; p < 3; ++p) planes[p] = m->rf_ppw[p];
+  planes[3] = m->sweep_waves;
+  return m->rf_tuned ? PTH_COMPLETE : PTH_INCOMPLETE;
+}
+
+pth_status pth_burgers_set_tuning(pth_mesh *mh, const int planes[4]) {
+  Mesh *m = Impl(mh);
+  for (int p = 0; p < 3; ++p)
+    if (planes[p] != 1 && planes[p] != 2 && planes[p] != 4) {
+      SetError("planes per workgroup must be 1, 2 or 4");
+      return PTH_FAIL;
+    }
+  if (planes[3] != 3 && planes[3] != 4) {
+    SetError("block-sweep waves per SIMD must be 3 or 4");
+    return PTH_FAIL;
+  }
+  for (int p = 0; p < 3; ++p) m->rf_ppw[p] = planes[p];
+  m->sweep_waves = planes[3];
+  m->rf_tuned = true;  // a caller's choice is not re-tuned
+  if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
+  if (m->graph) (void)hipGraphDestroy(m->graph);
+  m->graph_exec = nullptr;
+  m->graph = nullptr;
   return PTH_COMPLETE;
 }
 
@@ -316,9 +549,21 @@ pth_status pth_burgers_run_cycles(pth_mesh *mh, int ncycles, int use_graph) {
   return PTH_COMPLETE;
 }
 
+pth_status pth_burgers_set_sweep(pth_mesh *mh, int on) {
+  Mesh *m = Impl(mh);
+  m->run_sweep = on != 0;
+  m->form_auto = false;
+  if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
+  if (m->graph) (void)hipGraphDestroy(m->graph);
+  m->graph_exec = nullptr;
+  m->graph = nullptr;
+  return PTH_COMPLETE;
+}
+
 pth_status pth_burgers_set_overlap(pth_mesh *mh, int on) {
   Mesh *m = Impl(mh);
   m->run_overlap = on != 0;
+  m->form_auto = false;
   if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
   if (m->graph) (void)hipGraphDestroy(m->graph);
   m->graph_exec = nullptr;

@@ -298,6 +298,12 @@ class Mesh {
   hipStream_t side_stream = nullptr;            // driver: the exchange beside the stencils
   hipEvent_t fork_ev[2] = {nullptr, nullptr};
   int run_overlap = 1;                          // driver: overlap exchange and stencils
+  int rf_ppw[3] = {1, 1, 1};                    // driver: planes per workgroup, recon parts
+  int run_sweep = 1;                            // driver: one block-sweep launch per stage
+  bool form_auto = true;                        // driver: pick sweep / overlap / serial by timing
+  bool form_tuned = false;
+  int sweep_waves = 4;                          // driver: sweep register budget (waves/SIMD)
+  bool rf_tuned = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   double run_cfl = 0.0;

@@ -113,12 +113,12 @@ def test_multilevel_conditions_match_oracle(spec, fused, vector):
     print("bit-exact:", np.array_equal(got, f), flush=True)
 
 
-def burgers_pair(bcs, spheres=(), use_graph=True, overlap=True):
+def burgers_pair(bcs, spheres=(), use_graph=True, overlap=True, sweep=True):
     from parthenon_amd.burgers import DeviceBurgersDriver, make_burgers_mesh
     from oracle_lib import lib
     nx, block, g, ns = (64, 64, 64), (16, 16, 16), 4, 4
     m = make_burgers_mesh(nx, block, g, ns, refine_spheres=spheres, bcs=bcs)
-    d = DeviceBurgersDriver(m, cfl=0.4, use_graph=use_graph, overlap=overlap)
+    d = DeviceBurgersDriver(m, cfl=0.4, use_graph=use_graph, overlap=overlap, sweep=sweep)
     d.initialize("sinusoid", seed=6)
     om = OracleMesh(nx, block, g, periodic=[int(p) for p in periodic_flags(bcs)], bcs=bcs,
                     spheres=spheres)
@@ -139,14 +139,15 @@ OPEN = ("reflect", "outflow", "periodic", "periodic", "outflow", "outflow")
 
 
 @pytest.mark.parametrize("refined", [False, True], ids=["uniform", "sphere"])
-@pytest.mark.parametrize("use_graph,overlap", [(False, False), (True, True)],
-                         ids=["eager-serial", "graph-overlap"])
-def test_burgers_open_boundaries_match_oracle(refined, use_graph, overlap):
-    print(f"case burgers open refined={refined} graph={use_graph} overlap={overlap}",
-          flush=True)
+@pytest.mark.parametrize("use_graph,overlap,sweep", [(False, False, False), (True, True, False),
+                                                     (True, False, True)],
+                         ids=["eager-serial", "graph-overlap", "graph-sweep"])
+def test_burgers_open_boundaries_match_oracle(refined, use_graph, overlap, sweep):
+    print(f"case burgers open refined={refined} graph={use_graph} overlap={overlap} "
+          f"sweep={sweep}", flush=True)
     from oracle_lib import lib
     spheres = [((0.0, 0.0, 0.0), 0.25, 2)] if refined else []
-    m, d, ob = burgers_pair(OPEN, spheres, use_graph, overlap)
+    m, d, ob = burgers_pair(OPEN, spheres, use_graph, overlap, sweep)
     for c in range(3):
         d.cycles(1)
         lib().orc_burgers_cycles(ob.h, 1)

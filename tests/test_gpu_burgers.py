@@ -12,6 +12,11 @@ pytestmark = pytest.mark.gpu
 
 RTOL = 1e-13
 
+# stage forms of the device driver: one block-sweep launch; plane-tiled reconstruction with
+# the exchange on a second stream; plane-tiled reconstruction then update then exchange
+FORMS = {"sweep": dict(sweep=True), "overlap": dict(sweep=False, overlap=True),
+         "serial": dict(sweep=False, overlap=False)}
+
 
 def pair(nx=(32, 32, 32), block=(16, 16, 16), g=4, ns=4, cfl=0.4, recon=0, **kw):
     m = make_burgers_mesh(nx, block, g, ns, **kw)
@@ -77,18 +82,18 @@ def test_ten_cycles_match_oracle(pgen):
     assert abs(d.time - lib().orc_burgers_time(ob.h)) <= RTOL * d.time
 
 
-@pytest.mark.parametrize("overlap", [True, False], ids=["overlap", "serial"])
+@pytest.mark.parametrize("form", list(FORMS))
 @pytest.mark.parametrize("use_graph", [False, True], ids=["eager", "graph"])
-def test_device_driver_matches_oracle_every_cycle(use_graph, overlap):
+def test_device_driver_matches_oracle_every_cycle(use_graph, form):
     # fused stage launches, device-resident clock; with the graph, one captured cycle replayed,
     # with host downloads (eager work) between the replays, each replay checked; with overlap,
     # each stage's exchange runs on a second stream beside the ghost-free reconstruction
-    print(f"case device driver graph={use_graph} overlap={overlap}", flush=True)
+    print(f"case device driver graph={use_graph} form={form}", flush=True)
     from parthenon_amd.burgers import DeviceBurgersDriver
     from oracle_lib import lib
     nx, block, g, ns = (64, 64, 64), (16, 16, 16), 4, 4
     m = make_burgers_mesh(nx, block, g, ns)
-    d = DeviceBurgersDriver(m, cfl=0.4, use_graph=use_graph, overlap=overlap)
+    d = DeviceBurgersDriver(m, cfl=0.4, use_graph=use_graph, **FORMS[form])
     d.initialize("sinusoid", seed=3)
     ob = OracleBurgers(OracleMesh(nx, block, g), ns, 0.4, 0)
     lib().orc_burgers_pgen_sinusoid(ob.h, 3)
@@ -189,20 +194,19 @@ def test_multilevel_task_driver_matches_oracle(name):
         assert d.dt == ob.dt
 
 
-@pytest.mark.parametrize("overlap", [True, False], ids=["overlap", "serial"])
+@pytest.mark.parametrize("form", list(FORMS))
 @pytest.mark.parametrize("use_graph", [False, True], ids=["eager", "graph"])
 @pytest.mark.parametrize("name", ["box120", "sphere960"])
-def test_multilevel_device_driver_matches_oracle(name, use_graph, overlap):
+def test_multilevel_device_driver_matches_oracle(name, use_graph, form):
     # fused stage with the flux-correction launches between reconstruction and update
     spec = REFINED[name]
-    print(f"case multilevel device driver {name} graph={use_graph} overlap={overlap}",
-          flush=True)
+    print(f"case multilevel device driver {name} graph={use_graph} form={form}", flush=True)
     from parthenon_amd.burgers import DeviceBurgersDriver
     from oracle_lib import lib
     m = make_burgers_mesh(spec["nx"], spec["block"], 4, 4,
                           refine_boxes=spec.get("refine_boxes", ()),
                           refine_spheres=spec.get("refine_spheres", ()))
-    d = DeviceBurgersDriver(m, cfl=0.4, use_graph=use_graph, overlap=overlap)
+    d = DeviceBurgersDriver(m, cfl=0.4, use_graph=use_graph, **FORMS[form])
     d.initialize("sinusoid", seed=2)
     om = OracleMesh(spec["nx"], spec["block"], 4, boxes=spec.get("refine_boxes", ()),
                     spheres=spec.get("refine_spheres", ()))
@@ -306,3 +310,31 @@ def test_history_octant_masses_match_oracle(tmp_path):
     assert np.max(rel_diff(table, np.array(rows_o))) < 1e-4
     with open(hst.fname) as f:
         assert f.readline().startswith("#  History data")
+
+
+def test_sweep_tuning_choices_give_identical_bits():
+    # the register budget of the block sweep and the plane split change the work split only
+    print("case tuning independence", flush=True)
+    from parthenon_amd.burgers import DeviceBurgersDriver
+    spec = REFINED["box120"]
+    outs = []
+    for form, tune in (("sweep", [1, 1, 1, 4]), ("sweep", [1, 1, 1, 3]),
+                       ("overlap", [2, 4, 2, 4]), ("serial", [4, 1, 1, 4])):
+        m = make_burgers_mesh(spec["nx"], spec["block"], 4, 4, refine_boxes=spec["refine_boxes"])
+        d = DeviceBurgersDriver(m, cfl=0.4, use_graph=False, **FORMS[form])
+        d.set_tuning(tune)
+        d.initialize("sinusoid", seed=4)
+        assert d.tuning() == tune
+        d.cycles(2)
+        outs.append(d.base.download("U"))
+    # the automatic choice (whole cycles of each form timed on the saved state, which is then
+    # restored) gives the same bits too
+    m = make_burgers_mesh(spec["nx"], spec["block"], 4, 4, refine_boxes=spec["refine_boxes"])
+    d = DeviceBurgersDriver(m, cfl=0.4, use_graph=True)
+    d.initialize("sinusoid", seed=4)
+    print("automatic form:", d.form(), d.tuning(), flush=True)
+    assert d.form() in FORMS
+    d.cycles(2)
+    outs.append(d.base.download("U"))
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
