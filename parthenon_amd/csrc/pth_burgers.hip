@@ -908,7 +908,8 @@ __global__ void __launch_bounds__(N * N, MINW)
                      double *__restrict__ derived, double *__restrict__ partial,
                      const BlockCoefs *__restrict__ coef, int nblocks, double beta,
                      const double *__restrict__ dtvar, int recon, const int *__restrict__ fover,
-                     const double *__restrict__ fcor) {
+                     const double *__restrict__ fcor, const int *__restrict__ push) {
+  static_assert(N >= 2 * NG, "a cell lies in at most one boundary strip per dimension");
   constexpr int E = N + 2 * NG;
   constexpr int NT = N * N;
   constexpr int R = N + 2;  // states per row: ext cells lo - 1 .. hi + 1
@@ -1046,6 +1047,28 @@ __global__ void __launch_bounds__(N * N, MINW)
         const double avg = beta * U[c0 + n * cs] + w2 * base[c0 + n * cs];
         u1[n] = 1.0 * avg + bdt * dudt;
         out[c0 + n * cs] = u1[n];
+      }
+      if (push) {
+        // the exchange's same-level copies: a cell within NG of a face is a ghost cell of the
+        // same-level neighbours across that face, edge or corner (push[27 b + direction]),
+        // at its index shifted by N along each crossed dimension
+        const int lx = tx < NG ? -1 : (tx >= N - NG ? 1 : 0);
+        const int ly = ty < NG ? -1 : (ty >= N - NG ? 1 : 0);
+        const int lz = k < NG ? -1 : (k >= N - NG ? 1 : 0);
+        for (int az = 0; az < (lz ? 2 : 1); ++az)
+          for (int ay = 0; ay < (ly ? 2 : 1); ++ay)
+            for (int ax = 0; ax < (lx ? 2 : 1); ++ax) {
+              const int dx = ax ? lx : 0, dy = ay ? ly : 0, dz = az ? lz : 0;
+              if (!(dx | dy | dz)) continue;
+              const int g = push[27 * b + (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)];
+              if (g < 0) continue;
+              const std::int64_t d0 =
+                  static_cast<std::int64_t>(g) * V * cs +
+                  (static_cast<std::int64_t>(kk - dz * N) * E + NG + ty - dy * N) * E + NG + tx -
+                  dx * N;
+#pragma unroll
+              for (int n = 0; n < V; ++n) out[d0 + n * cs] = u1[n];
+            }
       }
       if (derived)
         derived[static_cast<std::int64_t>(b) * cs + cell] =
@@ -1464,7 +1487,8 @@ int ResidentGroups(const void *kernel, int threads) {
 pth_status BurgersSweep(const Topology &t, int V, const double *U, const double *base,
                         double *out, double *derived, double *partial, int max_partials,
                         const BlockCoefs *coef, double beta, const double *dtvar, int recon,
-                        const FluxCorrection *fc, int waves, int *groups, hipStream_t s) {
+                        const FluxCorrection *fc, int waves, const int *push, int *groups,
+                        hipStream_t s) {
   if (!BurgersSweepSupported(t, V) || (waves != 3 && waves != 4)) {
     SetError("block sweep: unsupported block shape or component count");
     return PTH_FAIL;
@@ -1485,7 +1509,8 @@ pth_status BurgersSweep(const Topology &t, int V, const double *U, const double 
     }                                                                                         \
     grid = std::min<std::int64_t>(std::min(r, max_partials), static_cast<std::int64_t>(nb) * NN); \
     hipLaunchKernelGGL((BlockSweepKernel<NN, 4, NV, W>), dim3(grid), dim3(NN * NN), 0, s, U,  \
-                       base, out, derived, partial, coef, nb, beta, dtvar, recon, fover, fcor); \
+                       base, out, derived, partial, coef, nb, beta, dtvar, recon, fover, fcor, \
+                       push);                                                                 \
   }
 #define PTH_SW_LAUNCH(NN, NV)                                                                 \
   if (waves == 4)                                                                             \

@@ -73,6 +73,8 @@ MeshData::~MeshData() {
     c.prolong.Free();
     c.bc[0].Free();
     c.bc[1].Free();
+    c.pack_rest.Free();
+    c.unpack_rest.Free();
   }
 }
 
@@ -85,6 +87,7 @@ Mesh::~Mesh() {
     if (s.flux) (void)hipFree(s.flux);
   }
   if (arena_base) (void)hipFree(arena_base);
+  if (same_level) (void)hipFree(same_level);
   if (geom) (void)hipFree(geom);
   if (coef) (void)hipFree(coef);
   if (face) (void)hipFree(face);
@@ -425,6 +428,94 @@ pth_status Mesh::BuildCache(MeshData &md, int bt) {
     if (t->Upload() != PTH_COMPLETE) return PTH_FAIL;
   if (BuildBoundaryConditions(md, c) != PTH_COMPLETE) return PTH_FAIL;
   c.built = true;
+  return PTH_COMPLETE;
+}
+
+pth_status Mesh::SameLevelNeighbors() {
+  if (same_level) return PTH_COMPLETE;
+  const int nb = static_cast<int>(topo.blocks.size());
+  std::vector<int> t(static_cast<size_t>(27) * nb, -1);
+  for (int b = 0; b < nb; ++b)
+    for (const Link &l : topo.blocks[b].links)
+      if (!l.is_cell() && l.level == topo.blocks[b].at.level) t[27 * b + l.loc_index()] = l.gid;
+  PTH_HIP(hipMalloc(&same_level, sizeof(int) * t.size()));
+  PTH_HIP(hipMemcpy(same_level, t.data(), sizeof(int) * t.size(), hipMemcpyHostToDevice));
+  return PTH_COMPLETE;
+}
+
+// The direct exchange of every channel except the same-level channels of field f: the pack
+// and unpack regions, both in arena order, pair up channel by channel (a channel's send and
+// receive regions cover the same arena slice in the same element order); a pair whose two
+// regions both address field f's data (not a coarse buffer) is a same-level channel of f.
+pth_status Mesh::BuildRestExchange(MeshData &md, int f, bool *ok) {
+  *ok = false;
+  if (BuildCache(md, PTH_BT_ANY) != PTH_COMPLETE) return PTH_FAIL;
+  BoundCache &c = md.cache[PTH_BT_ANY];
+  if (!CanExchangeDirect(c.pack, c.unpack) || c.pack.host.size() != c.unpack.host.size())
+    return PTH_COMPLETE;
+  if (c.rest_field == f) {
+    *ok = true;
+    return PTH_COMPLETE;
+  }
+  c.pack_rest.Free();
+  c.unpack_rest.Free();
+  std::vector<std::pair<double *, std::int64_t>> arrays;  // the gather slots of BuildCache
+  for (size_t q = 0; q < fields.size(); ++q) {
+    if (!(fields[q].flags & PTH_FILL_GHOST)) continue;
+    arrays.emplace_back(md.store[q].data, md.store[q].ndata);
+    if (md.store[q].coarse) arrays.emplace_back(md.store[q].coarse, md.store[q].ncoarse);
+  }
+  const double *f0 = md.store[f].data, *f1 = f0 + md.store[f].ndata;
+  auto append = [&](const Region &r, std::vector<std::uint32_t> &out) {
+    std::uint32_t slot = 0;
+    std::int64_t rel = 0;
+    for (size_t a = 0; a < arrays.size(); ++a)
+      if (r.a >= arrays[a].first && r.a < arrays[a].first + arrays[a].second) {
+        slot = static_cast<std::uint32_t>(a);
+        rel = r.a - arrays[a].first;
+      }
+    for (int cc = 0; cc < r.nc; ++cc)
+      for (int k = 0; k < r.n[2]; ++k)
+        for (int j = 0; j < r.n[1]; ++j)
+          for (int i = 0; i < r.n[0]; ++i) {
+            const std::int64_t o = rel + static_cast<std::int64_t>(cc) * r.astride[2] +
+                                   static_cast<std::int64_t>(k + r.lo[2]) * r.astride[1] +
+                                   static_cast<std::int64_t>(j + r.lo[1]) * r.astride[0] +
+                                   (i + r.lo[0]);
+            out.push_back((slot << kIdxShift) | static_cast<std::uint32_t>(o));
+          }
+  };
+  std::vector<std::uint32_t> si, ri;
+  for (size_t q = 0; q < c.pack.host.size(); ++q) {
+    const Region &sr = c.pack.host[q], &rr = c.unpack.host[q];
+    if (sr.b != rr.b) {
+      SetError("internal: send and receive regions do not pair up");
+      return PTH_FAIL;
+    }
+    const bool same = sr.a >= f0 && sr.a < f1 && rr.a >= f0 && rr.a < f1;
+    if (same) continue;
+    append(sr, si);
+    append(rr, ri);
+  }
+  if (si.size() != ri.size()) {
+    SetError("internal: send and receive element counts differ");
+    return PTH_FAIL;
+  }
+  for (Table *t : {&c.pack_rest, &c.unpack_rest}) {
+    const std::vector<std::uint32_t> &v = t == &c.pack_rest ? si : ri;
+    const std::vector<double *> &bases = t == &c.pack_rest ? c.pack.bases : c.unpack.bases;
+    t->bases = bases;
+    t->total = static_cast<std::int64_t>(v.size());
+    if (v.empty()) continue;
+    PTH_HIP(hipMalloc(&t->dev_idx, sizeof(std::uint32_t) * v.size()));
+    PTH_HIP(hipMalloc(&t->dev_bases, sizeof(double *) * bases.size()));
+    PTH_HIP(hipMemcpy(t->dev_idx, v.data(), sizeof(std::uint32_t) * v.size(),
+                      hipMemcpyHostToDevice));
+    PTH_HIP(hipMemcpy(t->dev_bases, bases.data(), sizeof(double *) * bases.size(),
+                      hipMemcpyHostToDevice));
+  }
+  c.rest_field = f;
+  *ok = true;
   return PTH_COMPLETE;
 }
 

@@ -35,14 +35,22 @@ namespace {
 // The exchange of AddBoundaryExchangeTasks (boundary_communication.cpp:432-445): send + set
 // (restriction on multilevel meshes), coarse-buffer boundary conditions, prolongation,
 // boundary conditions.
-pth_status Exchange(Mesh &m, MeshData &md, hipStream_t s) {
+// With rest, the same-level channels of field c.rest_field are left out (the block sweep has
+// filled them).
+pth_status Exchange(Mesh &m, MeshData &md, hipStream_t s, bool rest = false) {
   if (m.BuildCache(md, PTH_BT_ANY) != PTH_COMPLETE) return PTH_FAIL;
   BoundCache &c = md.cache[PTH_BT_ANY];
   if (c.nbuffers > 0) {
     if (m.topo.multilevel &&
         LaunchRestrictT(c.restrict_send, m.topo, m.geom, s) != PTH_COMPLETE)
       return PTH_FAIL;
-    if (m.pack_mode == PTH_PACK_PERSISTENT && CanExchangeDirect(c.pack, c.unpack)) {
+    if (rest) {
+      if (c.rest_field < 0) {
+        SetError("internal: exchange without same-level channels was not built");
+        return PTH_FAIL;
+      }
+      if (LaunchExchangeDirect(c.pack_rest, c.unpack_rest, s) != PTH_COMPLETE) return PTH_FAIL;
+    } else if (m.pack_mode == PTH_PACK_PERSISTENT && CanExchangeDirect(c.pack, c.unpack)) {
       if (LaunchExchangeDirect(c.pack, c.unpack, s) != PTH_COMPLETE) return PTH_FAIL;
     } else {
       if (LaunchPack(c.pack, nullptr, s, true, m.pack_mode) != PTH_COMPLETE) return PTH_FAIL;
@@ -65,6 +73,7 @@ struct Ctx {
   MeshData *base, *one;
   int fu, fd, V;
   const FluxCorrection *fc = nullptr;  // multilevel: flux-correction patches of U
+  const int *push = nullptr;           // sweep: same-level neighbours it writes ghosts of
 };
 
 pth_status Setup(Mesh *m, Ctx &x) {
@@ -110,6 +119,17 @@ pth_status Setup(Mesh *m, Ctx &x) {
   if (FusedUpdatePartials(m->topo) > m->scratch_n) {
     SetError("too many CFL partials for the scratch area");
     return PTH_FAIL;
+  }
+  // the sweep writes U's same-level ghost cells itself when the rest of the exchange can run
+  // without those channels (host tables, built before any graph capture)
+  x.push = nullptr;
+  if (sweep) {
+    bool ok_base = false, ok_one = false;
+    if (m->BuildRestExchange(*x.base, x.fu, &ok_base) != PTH_COMPLETE ||
+        m->BuildRestExchange(*x.one, x.fu, &ok_one) != PTH_COMPLETE ||
+        m->SameLevelNeighbors() != PTH_COMPLETE)
+      return PTH_FAIL;
+    if (ok_base && ok_one && m->pack_mode == PTH_PACK_PERSISTENT) x.push = m->same_level;
   }
   return PTH_COMPLETE;
 }
@@ -178,9 +198,10 @@ pth_status StageSweep(Ctx &x, int stage, hipStream_t s) {
   int groups = 0;
   if (BurgersSweep(m.topo, x.V, U, x.base->store[x.fu].data, mc1.store[x.fu].data, derived,
                    stage == 2 ? m.scratch : nullptr, static_cast<int>(m.scratch_n), m.coef, beta,
-                   m.dtvar, m.run_recon, x.fc, m.sweep_waves, &groups, s) != PTH_COMPLETE)
+                   m.dtvar, m.run_recon, x.fc, m.sweep_waves, x.push, &groups, s) !=
+      PTH_COMPLETE)
     return PTH_FAIL;
-  if (Exchange(m, mc1, s) != PTH_COMPLETE) return PTH_FAIL;
+  if (Exchange(m, mc1, s, x.push != nullptr) != PTH_COMPLETE) return PTH_FAIL;
   if (stage == 2) return BurgersClock(m.scratch, groups, m.run_cfl, m.dtvar, 1, s);
   return PTH_COMPLETE;
 }
@@ -263,13 +284,13 @@ pth_status TuneReconFlux(Ctx &x, hipStream_t s) {
     for (int waves : {4, 3}) {
       st = BurgersSweep(m.topo, x.V, U, U, x.one->store[x.fu].data, nullptr, nullptr,
                         static_cast<int>(m.scratch_n), m.coef, 1.0, m.dtvar, m.run_recon, x.fc,
-                        waves, nullptr, s);
+                        waves, nullptr, nullptr, s);
       if (st != PTH_COMPLETE) break;
       if (hipEventRecord(ev[0], s) != hipSuccess) st = PTH_FAIL;
       for (int r = 0; r < 3 && st == PTH_COMPLETE; ++r)
         st = BurgersSweep(m.topo, x.V, U, U, x.one->store[x.fu].data, nullptr, nullptr,
                           static_cast<int>(m.scratch_n), m.coef, 1.0, m.dtvar, m.run_recon,
-                          x.fc, waves, nullptr, s);
+                          x.fc, waves, nullptr, nullptr, s);
       float ms = 0.0f;
       if (st != PTH_COMPLETE || hipEventRecord(ev[1], s) != hipSuccess ||
           hipEventSynchronize(ev[1]) != hipSuccess ||

@@ -260,6 +260,10 @@ struct BoundCache {  // per (MeshData, bound type): the pack / unpack / pro-res 
   Table pack, unpack, restrict_send, restrict_set, prolong;
   BcTable bc[2];  // physical boundary conditions on data (0) and coarse buffers (1)
   std::int64_t nbuffers = 0;
+  // direct-exchange index pairs without the same-level channels of field rest_field (filled
+  // by the block sweep itself); built on first use
+  Table pack_rest, unpack_rest;
+  int rest_field = -1;
 };
 
 struct MeshData {
@@ -327,6 +331,12 @@ class Mesh {
   pth_status Allocate(MeshData &md);
   pth_status BuildCache(MeshData &md, int bt);
   pth_status BuildBoundaryConditions(MeshData &md, BoundCache &c);
+  // pack_rest / unpack_rest of md's exchange cache for field f; false if the exchange has no
+  // gather form (then the caller keeps the full exchange)
+  pth_status BuildRestExchange(MeshData &md, int f, bool *ok);
+  // per block, the same-level neighbour in each of the 27 directions (or -1), on the device
+  int *same_level = nullptr;
+  pth_status SameLevelNeighbors();
   // patches of field f (built on first use); null with an error if f has no fluxes
   FluxCorrection *FluxPatches(int f);
 };
