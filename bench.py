@@ -171,11 +171,12 @@ def gmg_bench(args):
     N, seed = 256, 3
     res = {}
     sols = {}
-    configs = {"mixed": ("mixed", args.inner_cycles), "fp64": ("fp64", 1)}
+    configs = {"mixed": ("mixed", args.inner_cycles, True), "fp64": ("fp64", 1, True)}
     if args.inner_cycles != 1:
-        configs["mixed_inner1"] = ("mixed", 1)
-    for name, (prec, inner) in configs.items():
-        s = MGSolver(N, prec, MGParams(inner_cycles=inner))
+        configs["mixed_inner1"] = ("mixed", 1, True)
+    configs["mixed_cell_sweeps"] = ("mixed", args.inner_cycles, False)
+    for name, (prec, inner, cols) in configs.items():
+        s = MGSolver(N, prec, MGParams(inner_cycles=inner, column_sweeps=cols))
         s.set_rhs_blobs(seed)
         for _ in range(args.warmup):
             s.reset()
@@ -231,6 +232,12 @@ def gmg_bench(args):
                                       res["mixed_inner1"]["wall"],
                                       "fp64_steps": res["mixed_inner1"]["iters"]}
                                      if "mixed_inner1" in res else None),
+        "mixed_cell_sweeps": {"solves_per_s": res["mixed_cell_sweeps"]["steps"] /
+                                              res["mixed_cell_sweeps"]["wall"],
+                              "same_bits": bool(np.array_equal(sols["mixed_cell_sweeps"],
+                                                               sols["mixed"]))},
+        "fp64_kernels": {k: {"launches_per_solve": v[0], "avg_us": v[1] / v[0] * 1e3}
+                         for k, v in sorted(res["fp64"]["prof"].items())},
         "error_vs_fp64_solver": float(np.sqrt(np.mean((sols["mixed"] - sols["fp64"]) ** 2)) /
                                       np.sqrt(np.mean(sols["fp64"] ** 2))),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,

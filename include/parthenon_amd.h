@@ -240,6 +240,10 @@ pth_status pth_gmg_reset(pth_gmg *g);
 /* V-cycles per FP64 refinement step (default 1); cycles after the first continue from the
  * current correction on the same right-hand side. */
 pth_status pth_gmg_set_inner_cycles(pth_gmg *g, int k);
+/* Tuning (no counterpart in the reference): with on != 0 (the default) Jacobi sweeps of levels
+ * whose size is a multiple of 64 run by z columns (z neighbours kept in registers); the
+ * results are the same bits either way. */
+pth_status pth_gmg_set_column_sweeps(pth_gmg *h, int on);
 /* MGSolver::AddTasks iteration (mg_solver.hpp:119-161): V-cycles until the relative L2
  * residual ||b - A u|| / ||b|| < tol or max_iters cycles; iters and final_rel reported. */
 pth_status pth_gmg_solve(pth_gmg *g, double tol, int max_iters, int *iters, double *final_rel);
@@ -258,6 +262,7 @@ void pth_gmg_server_destroy(pth_gmg_server *s);
 pth_status pth_gmg_server_submit(pth_gmg_server *s, int n, uint64_t seed, double arrival_s,
                                  int *id);
 pth_status pth_gmg_server_set_inner_cycles(pth_gmg_server *s, int k);
+pth_status pth_gmg_server_set_column_sweeps(pth_gmg_server *h, int on);
 /* serve every submitted request; realtime: honour arrival times on the wall clock */
 pth_status pth_gmg_server_run(pth_gmg_server *s, int realtime);
 /* out: arrival, admission, completion (s, server clock), latency (s), final relative residual */

@@ -127,6 +127,8 @@ _SIGNATURES = {
     "pth_gmg_reset": (_I, [_P]),
     "pth_gmg_solve": (_I, [_P, _D, _I, C.POINTER(_I), _DP]),
     "pth_gmg_set_inner_cycles": (_I, [_P, _I]),
+    "pth_gmg_set_column_sweeps": (_I, [_P, _I]),
+    "pth_gmg_server_set_column_sweeps": (_I, [_P, _I]),
     "pth_gmg_server_set_inner_cycles": (_I, [_P, _I]),
     "pth_gmg_history": (_I, [_P, _DP, _I]),
     "pth_gmg_server_create": (_P, [_I, _D, _I, _I, _D, _I, _I64, _I, _I]),

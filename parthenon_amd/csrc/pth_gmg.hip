@@ -143,17 +143,44 @@ __device__ __forceinline__ T JacobiPoint(const Nbrs<T> &v, T b, const Lv<T> &o, 
   return w * q + (T(1) - w) * v.u;
 }
 
-// entry lookup: entries sorted by wg_begin, the last one ends at gridDim.x. The logical
-// workgroup id is XCD-remapped, so each XCD walks contiguous rows and planes of a level and
-// finds the y and z neighbours of its rows in its own L2.
-template <typename E>
-__device__ __forceinline__ const E &Find(const E *__restrict__ e, int ne, int &local) {
+// Entry q of a launch. Inline entries are selected with constant indices only, so the kernel
+// arguments are read in place (a runtime index into them would copy them to scratch).
+template <typename T>
+__device__ __forceinline__ GmgEntry<T> At(const GmgArgs<T> &A, int q) {
+  if (A.dev) return A.dev[q];
+  GmgEntry<T> e = A.e[0];
+#pragma unroll
+  for (int i = 1; i < kGmgInline; ++i)
+    if (i == q) e = A.e[i];
+  return e;
+}
+
+// The entry whose workgroup range holds this workgroup: entries sorted by wg_begin, the last
+// one ends at gridDim.x. The logical workgroup id is XCD-remapped, so each XCD walks
+// contiguous rows and planes of a level and finds the y and z neighbours of its rows in its
+// own L2.
+template <typename T>
+__device__ __forceinline__ GmgEntry<T> Find(const GmgArgs<T> &A, int &local) {
   const int w = static_cast<int>(XcdRemap(blockIdx.x, gridDim.x));
-  int lo = 0;
-  for (int q = 1; q < ne; ++q)
-    if (e[q].wg_begin <= w) lo = q;
-  local = w - e[lo].wg_begin;
-  return e[lo];
+  GmgEntry<T> e;
+  if (A.dev) {  // binary search over the staged entries
+    int lo = 0, hi = A.ne - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) / 2;
+      if (A.dev[mid].wg_begin <= w)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    e = A.dev[lo];
+  } else {
+    e = A.e[0];
+#pragma unroll
+    for (int i = 1; i < kGmgInline; ++i)
+      if (i < A.ne && A.e[i].wg_begin <= w) e = A.e[i];
+  }
+  local = w - e.wg_begin;
+  return e;
 }
 
 // ---- kernels --------------------------------------------------------------------------
@@ -161,10 +188,10 @@ __device__ __forceinline__ const E &Find(const E *__restrict__ e, int ne, int &l
 // FP64 outer step: u_new = u_old + e (e may be null), r = b - A u_new, r stored in T as the
 // inner right-hand side, per-workgroup partial sums of r^2.
 template <typename T>
-__global__ void __launch_bounds__(kWG) OuterKernel(const GmgEntry<T> *__restrict__ E, int ne) {
+__global__ void __launch_bounds__(kWG) OuterKernel(const GmgArgs<T> A) {
   __shared__ double wsum[kWG / 64];
   int local;
-  const GmgEntry<T> &e = Find(E, ne, local);
+  const GmgEntry<T> e = Find(A, local);
   const int n = e.n, c = local * kWG + threadIdx.x;
   double rr = 0.0;
   if (c < n * n * n) {
@@ -200,9 +227,9 @@ __global__ void __launch_bounds__(kWG) OuterKernel(const GmgEntry<T> *__restrict
 // per-entry sum of the partials of OuterKernel (or of the rhs norm): one workgroup per entry
 template <typename T>
 __global__ void __launch_bounds__(kWG)
-    NormKernel(const GmgEntry<T> *__restrict__ E, double *__restrict__ out) {
+    NormKernel(const GmgArgs<T> A, double *__restrict__ out) {
   __shared__ double wsum[kWG / 64];
-  const GmgEntry<T> &e = E[blockIdx.x];
+  const GmgEntry<T> e = At(A, blockIdx.x);
   const int n = e.n, np = (n * n * n + kWG - 1) / kWG;
   double s = 0.0;
   for (int p = threadIdx.x; p < np; p += kWG) s += e.partial[p];
@@ -218,10 +245,10 @@ __global__ void __launch_bounds__(kWG)
 
 // partial sums of b^2 (the stop test's denominator)
 template <typename T>
-__global__ void __launch_bounds__(kWG) RhsNormKernel(const GmgEntry<T> *__restrict__ E, int ne) {
+__global__ void __launch_bounds__(kWG) RhsNormKernel(const GmgArgs<T> A) {
   __shared__ double wsum[kWG / 64];
   int local;
-  const GmgEntry<T> &e = Find(E, ne, local);
+  const GmgEntry<T> e = Find(A, local);
   const int n = e.n, c = local * kWG + threadIdx.x;
   double s = 0.0;
   if (c < n * n * n) s = e.b64[c] * e.b64[c];
@@ -240,9 +267,9 @@ __global__ void __launch_bounds__(kWG) RhsNormKernel(const GmgEntry<T> *__restri
 // two separate sweeps, so the same bits).
 template <typename T, bool PRE2>
 __global__ void __launch_bounds__(kWG)
-    PreZeroKernel(const GmgEntry<T> *__restrict__ E, int ne, T w) {
+    PreZeroKernel(const GmgArgs<T> A, T w) {
   int local;
-  const GmgEntry<T> &e = Find(E, ne, local);
+  const GmgEntry<T> e = Find(A, local);
   const int n = e.n, c = local * kWG + threadIdx.x;
   if (c >= n * n * n) return;
   const Cell q = Decode(c, n);
@@ -266,9 +293,9 @@ __global__ void __launch_bounds__(kWG)
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kWG) ZeroKernel(const GmgEntry<T> *__restrict__ E, int ne) {
+__global__ void __launch_bounds__(kWG) ZeroKernel(const GmgArgs<T> A) {
   int local;
-  const GmgEntry<T> &e = Find(E, ne, local);
+  const GmgEntry<T> e = Find(A, local);
   const int c = local * kWG + threadIdx.x;
   if (c < e.n * e.n * e.n) e.x[c] = T(0);
 }
@@ -276,14 +303,51 @@ __global__ void __launch_bounds__(kWG) ZeroKernel(const GmgEntry<T> *__restrict_
 // one Jacobi sweep x -> t (the caller swaps x and t)
 template <typename T>
 __global__ void __launch_bounds__(kWG)
-    JacobiKernel(const GmgEntry<T> *__restrict__ E, int ne, T w) {
+    JacobiKernel(const GmgArgs<T> A, T w) {
   int local;
-  const GmgEntry<T> &e = Find(E, ne, local);
+  const GmgEntry<T> e = Find(A, local);
   const int n = e.n, c = local * kWG + threadIdx.x;
   if (c >= n * n * n) return;
   const Cell q = Decode(c, n);
   const Lv<T> o = MakeLv<T>(n);
   e.t[c] = JacobiPoint(Load(static_cast<const T *>(e.x), c, q, n), e.b[c], o, w);
+}
+
+// One Jacobi sweep x -> t by z columns: a workgroup covers 64 x 4 cells of NZ consecutive
+// planes, each thread marches its column keeping the z neighbours in registers (the x and y
+// neighbours come from the cache). Same operations as JacobiKernel; for n % 64 == 0 and
+// n % NZ == 0.
+template <typename T, int NZ>
+__global__ void __launch_bounds__(kWG) JacobiZKernel(const GmgArgs<T> A, T w) {
+  int local;
+  const GmgEntry<T> e = Find(A, local);
+  const int n = e.n, tx = n / 64, ty = n / 4;
+  const int bx = local % tx, r = local / tx, by = r % ty, bz = r / ty;
+  const int i = bx * 64 + static_cast<int>(threadIdx.x % 64);
+  const int j = by * 4 + static_cast<int>(threadIdx.x / 64);
+  const int k0 = bz * NZ;
+  const Lv<T> o = MakeLv<T>(n);
+  const int s1 = n, s2 = n * n;
+  const T *x = e.x;
+  int c = (k0 * n + j) * n + i;
+  T cur = x[c];
+  T prev = k0 > 0 ? x[c - s2] : T(0);
+  T next = k0 + 1 < n ? x[c + s2] : T(0);
+#pragma unroll 2
+  for (int k = k0; k < k0 + NZ; ++k, c += s2) {
+    Nbrs<T> v;
+    v.u = cur;
+    v.xm = i > 0 ? x[c - 1] : -v.u;
+    v.xp = i < n - 1 ? x[c + 1] : -v.u;
+    v.ym = j > 0 ? x[c - s1] : -v.u;
+    v.yp = j < n - 1 ? x[c + s1] : -v.u;
+    v.zm = k > 0 ? prev : -v.u;
+    v.zp = k < n - 1 ? next : -v.u;
+    e.t[c] = JacobiPoint(v, e.b[c], o, w);
+    prev = cur;
+    cur = next;
+    next = k + 2 < n ? x[c + 2 * s2] : T(0);
+  }
 }
 
 template <typename T>
@@ -297,9 +361,9 @@ __device__ __forceinline__ T Restrict8(const T t[2][2][2], const Lv<T> &o) {
 // coarser level's rhs; one thread per coarse cell
 template <typename T>
 __global__ void __launch_bounds__(kWG)
-    ResidualRestrictKernel(const GmgEntry<T> *__restrict__ E, int ne) {
+    ResidualRestrictKernel(const GmgArgs<T> A) {
   int local;
-  const GmgEntry<T> &e = Find(E, ne, local);
+  const GmgEntry<T> e = Find(A, local);
   const int nc = e.nc, n = e.n, cc = local * kWG + threadIdx.x;
   if (cc >= nc * nc * nc) return;
   const Cell p = Decode(cc, nc);
@@ -324,9 +388,9 @@ __global__ void __launch_bounds__(kWG)
 // x += ProlongateSharedLinear(coarse x); one thread per fine cell, coarse ghosts from the
 // Dirichlet condition (pr_ops.hpp:166-289 term grouping)
 template <typename T>
-__global__ void __launch_bounds__(kWG) ProlongKernel(const GmgEntry<T> *__restrict__ E, int ne) {
+__global__ void __launch_bounds__(kWG) ProlongKernel(const GmgArgs<T> A) {
   int local;
-  const GmgEntry<T> &e = Find(E, ne, local);
+  const GmgEntry<T> e = Find(A, local);
   const int n = e.n, nc = e.nc, c = local * kWG + threadIdx.x;
   if (c >= n * n * n) return;
   const Cell q = Decode(c, n);
@@ -449,10 +513,10 @@ __device__ void BProlong(T *x, const T *xc, int n) {
 // to the coarsest level, every level in LDS.
 template <typename T, int NB>
 __global__ void __launch_bounds__(1024)
-    BottomKernel(const GmgEntry<T> *__restrict__ E, T w, int pre, int post) {
+    BottomKernel(const GmgArgs<T> A, T w, int pre, int post) {
   constexpr int kCap = NB * NB * NB + (NB / 2) * (NB / 2) * (NB / 2) * 8 / 7 + 8;
   __shared__ T sx[kCap], st[kCap], sb[kCap];
-  const GmgEntry<T> &e = E[blockIdx.x];
+  const GmgEntry<T> e = At(A, blockIdx.x);
   int nl = 0, off[8], nn[8];
   for (int n = e.n, o = 0; nl < 8; n /= 2) {
     off[nl] = o;
@@ -514,6 +578,8 @@ __global__ void __launch_bounds__(kWG) FillKernel(double *__restrict__ a, std::i
 }
 
 int Blocks(std::int64_t cells) { return static_cast<int>((cells + kWG - 1) / kWG); }
+
+constexpr int kZCol = 8;  // planes per thread of the column Jacobi sweep
 
 }  // namespace
 
@@ -622,15 +688,22 @@ GmgEntry<T> GmgProblem<T>::Entry(int l) const {
 
 template <typename T>
 pth_status GmgBatch<T>::Stage(const std::vector<GmgEntry<T>> &es, bool coarse_cells,
-                              GmgEntry<T> **dev, int *ne, int *nwg) {
-  // entries are written to a host staging ring and copied once per stage
+                              GmgArgs<T> *args, int *nwg, int per_wg) {
   std::vector<GmgEntry<T>> v = es;
   int wg = 0;
   for (auto &e : v) {
     e.wg_begin = wg;
     const std::int64_t n = coarse_cells ? e.nc : e.n;
-    wg += Blocks(n * n * n);
+    wg += static_cast<int>((n * n * n + per_wg - 1) / per_wg);
   }
+  std::memset(args, 0, sizeof(*args));
+  args->ne = static_cast<int>(v.size());
+  *nwg = wg;
+  if (static_cast<int>(v.size()) <= kGmgInline) {  // in the kernel arguments
+    std::copy(v.begin(), v.end(), args->e);
+    return PTH_COMPLETE;
+  }
+  // larger batches: written to a host staging ring and copied once per stage
   if (static_cast<int>(v.size()) > kStageCap) {
     SetError("GMG: too many batched entries in one launch");
     return PTH_FAIL;
@@ -644,9 +717,7 @@ pth_status GmgBatch<T>::Stage(const std::vector<GmgEntry<T>> &es, bool coarse_ce
   std::copy(v.begin(), v.end(), h);
   PTH_HIP(hipMemcpyAsync(d, h, sizeof(GmgEntry<T>) * v.size(), hipMemcpyHostToDevice, stream));
   stage_used += static_cast<int>(v.size());
-  *dev = d;
-  *ne = static_cast<int>(v.size());
-  *nwg = wg;
+  args->dev = d;
   return PTH_COMPLETE;
 }
 
@@ -678,11 +749,12 @@ template <typename T>
 pth_status GmgBatch<T>::RhsNorms(const std::vector<GmgProblem<T> *> &ps, double *out) {
   std::vector<GmgEntry<T>> es;
   for (auto *p : ps) es.push_back(p->Entry(0));
-  GmgEntry<T> *d;
-  int ne, nwg;
-  if (Stage(es, false, &d, &ne, &nwg) != PTH_COMPLETE) return PTH_FAIL;
-  hipLaunchKernelGGL(RhsNormKernel<T>, dim3(nwg), dim3(kWG), 0, stream, d, ne);
-  hipLaunchKernelGGL(NormKernel<T>, dim3(ne), dim3(kWG), 0, stream, d, norms_dev);
+  GmgArgs<T> a;
+  int nwg;
+  if (Stage(es, false, &a, &nwg) != PTH_COMPLETE) return PTH_FAIL;
+  const int ne = a.ne;
+  hipLaunchKernelGGL(RhsNormKernel<T>, dim3(nwg), dim3(kWG), 0, stream, a);
+  hipLaunchKernelGGL(NormKernel<T>, dim3(ne), dim3(kWG), 0, stream, a, norms_dev);
   PTH_HIP(hipGetLastError());
   PTH_HIP(hipMemcpyAsync(norms_host, norms_dev, sizeof(double) * ne, hipMemcpyDeviceToHost,
                          stream));
@@ -703,14 +775,15 @@ pth_status GmgBatch<T>::Residual(const std::vector<GmgProblem<T> *> &ps,
     if (!correct[q]) e.x = nullptr;
     es.push_back(e);
   }
-  GmgEntry<T> *d;
-  int ne, nwg;
-  if (Stage(es, false, &d, &ne, &nwg) != PTH_COMPLETE) return PTH_FAIL;
+  GmgArgs<T> a;
+  int nwg;
+  if (Stage(es, false, &a, &nwg) != PTH_COMPLETE) return PTH_FAIL;
+  const int ne = a.ne;
   {
     ProfScope prof("gmg_outer", stream);
-    hipLaunchKernelGGL(OuterKernel<T>, dim3(nwg), dim3(kWG), 0, stream, d, ne);
+    hipLaunchKernelGGL(OuterKernel<T>, dim3(nwg), dim3(kWG), 0, stream, a);
   }
-  hipLaunchKernelGGL(NormKernel<T>, dim3(ne), dim3(kWG), 0, stream, d, norms_dev);
+  hipLaunchKernelGGL(NormKernel<T>, dim3(ne), dim3(kWG), 0, stream, a, norms_dev);
   PTH_HIP(hipGetLastError());
   PTH_HIP(hipMemcpyAsync(norms_host, norms_dev, sizeof(double) * ne, hipMemcpyDeviceToHost,
                          stream));
@@ -738,43 +811,47 @@ pth_status GmgBatch<T>::Cycle(const std::vector<GmgProblem<T> *> &ps, double ome
     return es;
   };
   auto sweep = [&](int l) -> pth_status {  // x -> t, then t becomes x
-    GmgEntry<T> *d;
-    int ne, nwg;
-    if (Stage(entries(l), false, &d, &ne, &nwg) != PTH_COMPLETE) return PTH_FAIL;
+    GmgArgs<T> a;
+    int nwg;
+    const std::vector<GmgEntry<T>> es = entries(l);
+    bool zcol = jacobi_columns > 0;  // columns of kZCol planes when every size allows
+    for (const auto &e : es) zcol = zcol && e.n % 64 == 0 && e.n % kZCol == 0;
+    if (Stage(es, false, &a, &nwg, zcol ? kWG * kZCol : kWG) != PTH_COMPLETE) return PTH_FAIL;
     ProfScope prof(l == 0 ? "gmg_jacobi_l0" : "gmg_jacobi", stream);
-    hipLaunchKernelGGL(JacobiKernel<T>, dim3(nwg), dim3(kWG), 0, stream, d, ne, w);
+    if (zcol)
+      hipLaunchKernelGGL((JacobiZKernel<T, kZCol>), dim3(nwg), dim3(kWG), 0, stream, a, w);
+    else
+      hipLaunchKernelGGL(JacobiKernel<T>, dim3(nwg), dim3(kWG), 0, stream, a, w);
     PTH_HIP(hipGetLastError());
     for (auto *p : ps)
       if (l < p->nbottom) std::swap(p->lv[l].x, p->lv[l].t);
     return PTH_COMPLETE;
   };
   for (int l = 0; l < L; ++l) {
-    GmgEntry<T> *d;
-    int ne, nwg;
+    GmgArgs<T> a;
+    int nwg;
     if (warm && l == 0) {
       for (int s = 0; s < pre; ++s)
         if (sweep(l) != PTH_COMPLETE) return PTH_FAIL;
     } else if (pre > 0) {
-      if (Stage(entries(l), false, &d, &ne, &nwg) != PTH_COMPLETE) return PTH_FAIL;
+      if (Stage(entries(l), false, &a, &nwg) != PTH_COMPLETE) return PTH_FAIL;
       ProfScope prof(l == 0 ? "gmg_pre_l0" : "gmg_pre", stream);
       if (pre >= 2)
-        hipLaunchKernelGGL((PreZeroKernel<T, true>), dim3(nwg), dim3(kWG), 0, stream, d, ne,
-                           w);
+        hipLaunchKernelGGL((PreZeroKernel<T, true>), dim3(nwg), dim3(kWG), 0, stream, a, w);
       else
-        hipLaunchKernelGGL((PreZeroKernel<T, false>), dim3(nwg), dim3(kWG), 0, stream, d, ne,
-                           w);
+        hipLaunchKernelGGL((PreZeroKernel<T, false>), dim3(nwg), dim3(kWG), 0, stream, a, w);
       PTH_HIP(hipGetLastError());
     } else {
-      if (Stage(entries(l), false, &d, &ne, &nwg) != PTH_COMPLETE) return PTH_FAIL;
-      hipLaunchKernelGGL(ZeroKernel<T>, dim3(nwg), dim3(kWG), 0, stream, d, ne);
+      if (Stage(entries(l), false, &a, &nwg) != PTH_COMPLETE) return PTH_FAIL;
+      hipLaunchKernelGGL(ZeroKernel<T>, dim3(nwg), dim3(kWG), 0, stream, a);
       PTH_HIP(hipGetLastError());
     }
     if (!(warm && l == 0))
       for (int s = 2; s < pre; ++s)
         if (sweep(l) != PTH_COMPLETE) return PTH_FAIL;
-    if (Stage(entries(l), true, &d, &ne, &nwg) != PTH_COMPLETE) return PTH_FAIL;
+    if (Stage(entries(l), true, &a, &nwg) != PTH_COMPLETE) return PTH_FAIL;
     ProfScope prof(l == 0 ? "gmg_restrict_l0" : "gmg_restrict", stream);
-    hipLaunchKernelGGL(ResidualRestrictKernel<T>, dim3(nwg), dim3(kWG), 0, stream, d, ne);
+    hipLaunchKernelGGL(ResidualRestrictKernel<T>, dim3(nwg), dim3(kWG), 0, stream, a);
     PTH_HIP(hipGetLastError());
   }
   {
@@ -783,21 +860,21 @@ pth_status GmgBatch<T>::Cycle(const std::vector<GmgProblem<T> *> &ps, double ome
       es.push_back(p->Entry(p->nbottom));
       es.back().warm = warm && p->nbottom == 0;
     }
-    GmgEntry<T> *d;
-    int ne, nwg;
-    if (Stage(es, false, &d, &ne, &nwg) != PTH_COMPLETE) return PTH_FAIL;
+    GmgArgs<T> a;
+    int nwg;
+    if (Stage(es, false, &a, &nwg) != PTH_COMPLETE) return PTH_FAIL;
     ProfScope prof("gmg_bottom", stream);
-    hipLaunchKernelGGL((BottomKernel<T, GmgBottom<T>::kN>), dim3(ne), dim3(1024), 0, stream, d,
-                       w, pre, post);
+    hipLaunchKernelGGL((BottomKernel<T, GmgBottom<T>::kN>), dim3(a.ne), dim3(1024), 0, stream,
+                       a, w, pre, post);
     PTH_HIP(hipGetLastError());
   }
   for (int l = L - 1; l >= 0; --l) {
-    GmgEntry<T> *d;
-    int ne, nwg;
-    if (Stage(entries(l), false, &d, &ne, &nwg) != PTH_COMPLETE) return PTH_FAIL;
+    GmgArgs<T> a;
+    int nwg;
+    if (Stage(entries(l), false, &a, &nwg) != PTH_COMPLETE) return PTH_FAIL;
     {
       ProfScope prof(l == 0 ? "gmg_prolong_l0" : "gmg_prolong", stream);
-      hipLaunchKernelGGL(ProlongKernel<T>, dim3(nwg), dim3(kWG), 0, stream, d, ne);
+      hipLaunchKernelGGL(ProlongKernel<T>, dim3(nwg), dim3(kWG), 0, stream, a);
       PTH_HIP(hipGetLastError());
     }
     for (int s = 0; s < post; ++s)

@@ -20,6 +20,16 @@ struct GmgEntry {
   int warm;               // bottom solve: start from x instead of 0
 };
 
+// The entries of one launch, passed as a kernel argument: inline for up to kGmgInline
+// entries (no copy before the launch), else in device memory (dev, staged by GmgBatch).
+constexpr int kGmgInline = 8;
+template <typename T>
+struct GmgArgs {
+  GmgEntry<T> e[kGmgInline];
+  const GmgEntry<T> *dev;
+  int ne;
+};
+
 // largest level handled inside one workgroup in LDS (three n^3 arrays of T plus the coarser
 // levels below the 64 KiB static LDS limit)
 template <typename T>
@@ -62,8 +72,10 @@ struct GmgBatch {
   double *norms_host = nullptr, *norms_dev = nullptr;
   pth_status Init(hipStream_t s);
   void Free();
-  pth_status Stage(const std::vector<GmgEntry<T>> &es, bool coarse_cells, GmgEntry<T> **dev,
-                   int *ne, int *nwg);
+  // per_wg: cells one workgroup of the launch covers
+  pth_status Stage(const std::vector<GmgEntry<T>> &es, bool coarse_cells, GmgArgs<T> *args,
+                   int *nwg, int per_wg = 256);
+  int jacobi_columns = 1;  // Jacobi sweeps by z columns where the level size allows
   pth_status RhsNorms(const std::vector<GmgProblem<T> *> &ps, double *out);
   pth_status Residual(const std::vector<GmgProblem<T> *> &ps, const std::vector<char> &correct,
                       double *out);

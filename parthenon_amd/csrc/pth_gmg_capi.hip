@@ -30,7 +30,8 @@ struct Params {
   int precision;  // 0: FP32 inner cycle (mixed), 1: FP64
   double omega;
   int pre, post;
-  int inner = 1;  // V-cycles per FP64 refinement step
+  int inner = 1;    // V-cycles per FP64 refinement step
+  int columns = 1;  // Jacobi sweeps by z columns where the level size allows
 };
 
 template <typename T>
@@ -62,6 +63,7 @@ struct Solver1 {
       const double r = bnorm > 0 ? rn / bnorm : rn;
       history.push_back(r);
       if (r < tol || it >= maxit) break;
+      batch.jacobi_columns = P.columns;
       for (int c = 0; c < P.inner; ++c)
         if (batch.Cycle(ps, P.omega, P.pre, P.post, c > 0) != PTH_COMPLETE) return PTH_FAIL;
       corr[0] = 1;
@@ -237,6 +239,7 @@ struct Server {
         keep_act.push_back(a);
         cyc.push_back(a.p);
       }
+      batch.jacobi_columns = P.columns;
       for (int c = 0; c < P.inner && !cyc.empty(); ++c)
         if (batch.Cycle(cyc, P.omega, P.pre, P.post, c > 0) != PTH_COMPLETE) return PTH_FAIL;
       max_batch = std::max(max_batch, static_cast<int>(act.size()));
@@ -360,6 +363,11 @@ pth_status pth_gmg_set_inner_cycles(pth_gmg *h, int k) {
   return PTH_COMPLETE;
 }
 
+pth_status pth_gmg_set_column_sweeps(pth_gmg *h, int on) {
+  h->P.columns = on != 0;
+  return PTH_COMPLETE;
+}
+
 pth_status pth_gmg_solve(pth_gmg *h, double tol, int max_iters, int *iters, double *final_rel) {
   return h->Do([&](auto &S) { return S.Solve(h->P, tol, max_iters, iters, final_rel); });
 }
@@ -430,6 +438,13 @@ pth_status pth_gmg_server_set_inner_cycles(pth_gmg_server *h, int k) {
   }
   return h->Do([&](auto &S) {
     S.P.inner = k;
+    return PTH_COMPLETE;
+  });
+}
+
+pth_status pth_gmg_server_set_column_sweeps(pth_gmg_server *h, int on) {
+  return h->Do([&](auto &S) {
+    S.P.columns = on != 0;
     return PTH_COMPLETE;
   });
 }

@@ -24,6 +24,7 @@ class MGParams:  # src/solvers/mg_solver.hpp:37-58, with the smoother weight exp
     pre_sweeps: int = 2
     post_sweeps: int = 2
     inner_cycles: int = 1               # V-cycles per FP64 refinement step
+    column_sweeps: bool = True          # Jacobi by z columns where the level size allows
 
 
 class MGSolver:
@@ -37,6 +38,7 @@ class MGSolver:
         self._h = h
         self.levels = lib().pth_gmg_levels(h)
         _check(lib().pth_gmg_set_inner_cycles(h, p.inner_cycles), "set_inner_cycles")
+        _check(lib().pth_gmg_set_column_sweeps(h, int(p.column_sweeps)), "set_column_sweeps")
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -88,6 +90,8 @@ class Server:
         self._h = h
         self.sizes = {}
         _check(lib().pth_gmg_server_set_inner_cycles(h, p.inner_cycles), "set_inner_cycles")
+        _check(lib().pth_gmg_server_set_column_sweeps(h, int(p.column_sweeps)),
+               "set_column_sweeps")
 
     def __del__(self):
         if getattr(self, "_h", None):
