@@ -224,13 +224,14 @@ __global__ void __launch_bounds__(kWG) OuterKernel(const GmgArgs<T> A) {
   }
 }
 
-// per-entry sum of the partials of OuterKernel (or of the rhs norm): one workgroup per entry
+// per-entry sum of the partials of OuterKernel (or of the rhs norm): one workgroup per entry;
+// per_wg = cells per workgroup of the kernel that wrote the partials
 template <typename T>
 __global__ void __launch_bounds__(kWG)
-    NormKernel(const GmgArgs<T> A, double *__restrict__ out) {
+    NormKernel(const GmgArgs<T> A, double *__restrict__ out, int per_wg) {
   __shared__ double wsum[kWG / 64];
   const GmgEntry<T> e = At(A, blockIdx.x);
-  const int n = e.n, np = (n * n * n + kWG - 1) / kWG;
+  const int n = e.n, np = (n * n * n + per_wg - 1) / per_wg;
   double s = 0.0;
   for (int p = threadIdx.x; p < np; p += kWG) s += e.partial[p];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
@@ -347,6 +348,53 @@ __global__ void __launch_bounds__(kWG) JacobiZKernel(const GmgArgs<T> A, T w) {
     prev = cur;
     cur = next;
     next = k + 2 < n ? x[c + 2 * s2] : T(0);
+  }
+}
+
+// Column form of the fused pre-smoothing: a workgroup covers 64 x 4 cells of NZ consecutive
+// planes and each thread marches its column with the z neighbours of b in registers; the
+// operations per cell are those of PreZeroKernel<T, true>. (A column form of the FP64 outer
+// step measured slower than OuterKernel and is not used.)
+struct ZCell {
+  int i, j, k0, c;
+};
+__device__ __forceinline__ ZCell ZStart(int n, int local, int nz) {
+  const int tx = n / 64, ty = n / 4;
+  const int bx = local % tx, r = local / tx, by = r % ty, bz = r / ty;
+  ZCell z;
+  z.i = bx * 64 + static_cast<int>(threadIdx.x % 64);
+  z.j = by * 4 + static_cast<int>(threadIdx.x / 64);
+  z.k0 = bz * nz;
+  z.c = (z.k0 * n + z.j) * n + z.i;
+  return z;
+}
+
+template <typename T, int NZ>
+__global__ void __launch_bounds__(kWG) PreZero2ZKernel(const GmgArgs<T> A, T w) {
+  int local;
+  const GmgEntry<T> e = Find(A, local);
+  const int n = e.n, s1 = n, s2 = n * n;
+  const ZCell z = ZStart(n, local, NZ);
+  const int i = z.i, j = z.j;
+  int c = z.c;
+  const Lv<T> o = MakeLv<T>(n);
+  const Nbrs<T> zero = {T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
+  const T *b = e.b;
+  T bprev = z.k0 > 0 ? b[c - s2] : T(0), bcur = b[c];
+  T bnext = z.k0 + 1 < n ? b[c + s2] : T(0);
+  for (int k = z.k0; k < z.k0 + NZ; ++k, c += s2) {
+    Nbrs<T> x1;
+    x1.u = JacobiPoint(zero, bcur, o, w);
+    x1.xm = i > 0 ? JacobiPoint(zero, b[c - 1], o, w) : -x1.u;
+    x1.xp = i < n - 1 ? JacobiPoint(zero, b[c + 1], o, w) : -x1.u;
+    x1.ym = j > 0 ? JacobiPoint(zero, b[c - s1], o, w) : -x1.u;
+    x1.yp = j < n - 1 ? JacobiPoint(zero, b[c + s1], o, w) : -x1.u;
+    x1.zm = k > 0 ? JacobiPoint(zero, bprev, o, w) : -x1.u;
+    x1.zp = k < n - 1 ? JacobiPoint(zero, bnext, o, w) : -x1.u;
+    e.x[c] = JacobiPoint(x1, bcur, o, w);
+    bprev = bcur;
+    bcur = bnext;
+    bnext = k + 2 < n ? b[c + 2 * s2] : T(0);
   }
 }
 
@@ -754,7 +802,7 @@ pth_status GmgBatch<T>::RhsNorms(const std::vector<GmgProblem<T> *> &ps, double 
   if (Stage(es, false, &a, &nwg) != PTH_COMPLETE) return PTH_FAIL;
   const int ne = a.ne;
   hipLaunchKernelGGL(RhsNormKernel<T>, dim3(nwg), dim3(kWG), 0, stream, a);
-  hipLaunchKernelGGL(NormKernel<T>, dim3(ne), dim3(kWG), 0, stream, a, norms_dev);
+  hipLaunchKernelGGL(NormKernel<T>, dim3(ne), dim3(kWG), 0, stream, a, norms_dev, kWG);
   PTH_HIP(hipGetLastError());
   PTH_HIP(hipMemcpyAsync(norms_host, norms_dev, sizeof(double) * ne, hipMemcpyDeviceToHost,
                          stream));
@@ -783,7 +831,7 @@ pth_status GmgBatch<T>::Residual(const std::vector<GmgProblem<T> *> &ps,
     ProfScope prof("gmg_outer", stream);
     hipLaunchKernelGGL(OuterKernel<T>, dim3(nwg), dim3(kWG), 0, stream, a);
   }
-  hipLaunchKernelGGL(NormKernel<T>, dim3(ne), dim3(kWG), 0, stream, a, norms_dev);
+  hipLaunchKernelGGL(NormKernel<T>, dim3(ne), dim3(kWG), 0, stream, a, norms_dev, kWG);
   PTH_HIP(hipGetLastError());
   PTH_HIP(hipMemcpyAsync(norms_host, norms_dev, sizeof(double) * ne, hipMemcpyDeviceToHost,
                          stream));
@@ -834,9 +882,14 @@ pth_status GmgBatch<T>::Cycle(const std::vector<GmgProblem<T> *> &ps, double ome
       for (int s = 0; s < pre; ++s)
         if (sweep(l) != PTH_COMPLETE) return PTH_FAIL;
     } else if (pre > 0) {
-      if (Stage(entries(l), false, &a, &nwg) != PTH_COMPLETE) return PTH_FAIL;
+      const std::vector<GmgEntry<T>> es = entries(l);
+      bool zcol = jacobi_columns > 0 && pre >= 2;
+      for (const auto &e : es) zcol = zcol && e.n % 64 == 0 && e.n % kZCol == 0;
+      if (Stage(es, false, &a, &nwg, zcol ? kWG * kZCol : kWG) != PTH_COMPLETE) return PTH_FAIL;
       ProfScope prof(l == 0 ? "gmg_pre_l0" : "gmg_pre", stream);
-      if (pre >= 2)
+      if (zcol)
+        hipLaunchKernelGGL((PreZero2ZKernel<T, kZCol>), dim3(nwg), dim3(kWG), 0, stream, a, w);
+      else if (pre >= 2)
         hipLaunchKernelGGL((PreZeroKernel<T, true>), dim3(nwg), dim3(kWG), 0, stream, a, w);
       else
         hipLaunchKernelGGL((PreZeroKernel<T, false>), dim3(nwg), dim3(kWG), 0, stream, a, w);
