@@ -109,6 +109,13 @@ struct Cell {
 };
 __device__ __forceinline__ Cell Decode(int c, int n) {
   Cell q;
+  if ((n & (n - 1)) == 0) {  // power of two: shifts and masks (n is uniform per workgroup)
+    const int sh = __builtin_ctz(static_cast<unsigned>(n));
+    q.i = c & (n - 1);
+    q.j = (c >> sh) & (n - 1);
+    q.k = c >> (2 * sh);
+    return q;
+  }
   q.i = c % n;
   const int r = c / n;
   q.j = r % n;
@@ -450,39 +457,24 @@ __global__ void __launch_bounds__(kWG) ProlongKernel(const GmgArgs<T> A) {
   const Lv<T> oc = MakeLv<T>(nc);
   const T hc = oc.dx, df = T(0.25) * hc;
   auto grad = [&](T diff) { return oc.mul ? diff * oc.idx : diff / hc; };
-  T g1 = 0, g2 = 0, g3 = 0;  // the one-sided gradient on this child's side
-  if (oi == 0) {
-    const T fm = p.i > 0 ? xc[cc - 1] : -fc;
-    g1 = grad(fc - fm);
-  } else {
-    const T fp = p.i < nc - 1 ? xc[cc + 1] : -fc;
-    g1 = grad(fp - fc);
-  }
-  if (oj == 0) {
-    const T fm = p.j > 0 ? xc[cc - nc] : -fc;
-    g2 = grad(fc - fm);
-  } else {
-    const T fp = p.j < nc - 1 ? xc[cc + nc] : -fc;
-    g2 = grad(fp - fc);
-  }
-  if (ok == 0) {
-    const T fm = p.k > 0 ? xc[cc - nc * nc] : -fc;
-    g3 = grad(fc - fm);
-  } else {
-    const T fp = p.k < nc - 1 ? xc[cc + nc * nc] : -fc;
-    g3 = grad(fp - fc);
-  }
-  T v;
-  switch ((ok << 2) | (oj << 1) | oi) {
-    case 0: v = fc - (g1 * df + g2 * df + g3 * df); break;
-    case 1: v = fc + (g1 * df - g2 * df - g3 * df); break;
-    case 2: v = fc - (g1 * df - g2 * df + g3 * df); break;
-    case 3: v = fc + (g1 * df + g2 * df - g3 * df); break;
-    case 4: v = fc - (g1 * df + g2 * df - g3 * df); break;
-    case 5: v = fc + (g1 * df - g2 * df + g3 * df); break;
-    case 6: v = fc - (g1 * df - g2 * df - g3 * df); break;
-    default: v = fc + (g1 * df + g2 * df + g3 * df); break;
-  }
+  // the one-sided gradient on this child's side of the coarse cell, per dimension
+  const bool in1 = oi ? p.i < nc - 1 : p.i > 0;
+  const bool in2 = oj ? p.j < nc - 1 : p.j > 0;
+  const bool in3 = ok ? p.k < nc - 1 : p.k > 0;
+  const T f1 = in1 ? xc[cc + (oi ? 1 : -1)] : -fc;
+  const T f2 = in2 ? xc[cc + (oj ? nc : -nc)] : -fc;
+  const T f3 = in3 ? xc[cc + (ok ? nc * nc : -nc * nc)] : -fc;
+  const T g1 = grad(oi ? f1 - fc : fc - f1);
+  const T g2 = grad(oj ? f2 - fc : fc - f2);
+  const T g3 = grad(ok ? f3 - fc : fc - f3);
+  // fc +/- (g1 df +/- g2 df +/- g3 df) with the signs of the child's offsets, grouped like the
+  // reference's eight cases (pr_ops.hpp:261-283): the outer sign is the x offset's, the inner
+  // signs are relative to it; a product with -1 and an added negation round exactly like the
+  // subtractions they stand for
+  const T so = oi ? T(1) : T(-1);
+  const T sb = oi == oj ? T(1) : T(-1);
+  const T sc = oi == ok ? T(1) : T(-1);
+  const T v = fc + so * ((g1 * df + sb * (g2 * df)) + sc * (g3 * df));
   e.x[c] += v;
 }
 
