@@ -231,22 +231,33 @@ __global__ void __launch_bounds__(kWG) OuterKernel(const GmgArgs<T> A) {
   }
 }
 
-// per-entry sum of the partials of OuterKernel (or of the rhs norm): one workgroup per entry;
-// per_wg = cells per workgroup of the kernel that wrote the partials
+// per-entry sum of the partials of OuterKernel (or of the rhs norm): one workgroup of kNormWG
+// threads per entry, each thread adding four interleaved streams of partials into separate
+// sums (independent loads in flight), then the workgroup tree; per_wg = cells per workgroup
+// of the kernel that wrote the partials. The order is fixed, so an entry's norm is the same
+// alone and in a batch.
+constexpr int kNormWG = 1024;
 template <typename T>
-__global__ void __launch_bounds__(kWG)
+__global__ void __launch_bounds__(kNormWG)
     NormKernel(const GmgArgs<T> A, double *__restrict__ out, int per_wg) {
-  __shared__ double wsum[kWG / 64];
+  __shared__ double wsum[kNormWG / 64];
   const GmgEntry<T> e = At(A, blockIdx.x);
   const int n = e.n, np = (n * n * n + per_wg - 1) / per_wg;
-  double s = 0.0;
-  for (int p = threadIdx.x; p < np; p += kWG) s += e.partial[p];
+  const double *pa = e.partial;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  for (int p = threadIdx.x; p < np; p += 4 * kNormWG) {
+    s0 += pa[p];
+    if (p + kNormWG < np) s1 += pa[p + kNormWG];
+    if (p + 2 * kNormWG < np) s2 += pa[p + 2 * kNormWG];
+    if (p + 3 * kNormWG < np) s3 += pa[p + 3 * kNormWG];
+  }
+  double s = (s0 + s1) + (s2 + s3);
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
   if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
   __syncthreads();
   if (threadIdx.x == 0) {
     double t = 0.0;
-    for (int w = 0; w < kWG / 64; ++w) t += wsum[w];
+    for (int w = 0; w < kNormWG / 64; ++w) t += wsum[w];
     out[blockIdx.x] = t;
   }
 }
@@ -794,7 +805,7 @@ pth_status GmgBatch<T>::RhsNorms(const std::vector<GmgProblem<T> *> &ps, double 
   if (Stage(es, false, &a, &nwg) != PTH_COMPLETE) return PTH_FAIL;
   const int ne = a.ne;
   hipLaunchKernelGGL(RhsNormKernel<T>, dim3(nwg), dim3(kWG), 0, stream, a);
-  hipLaunchKernelGGL(NormKernel<T>, dim3(ne), dim3(kWG), 0, stream, a, norms_dev, kWG);
+  hipLaunchKernelGGL(NormKernel<T>, dim3(ne), dim3(kNormWG), 0, stream, a, norms_dev, kWG);
   PTH_HIP(hipGetLastError());
   PTH_HIP(hipMemcpyAsync(norms_host, norms_dev, sizeof(double) * ne, hipMemcpyDeviceToHost,
                          stream));
@@ -823,7 +834,7 @@ pth_status GmgBatch<T>::Residual(const std::vector<GmgProblem<T> *> &ps,
     ProfScope prof("gmg_outer", stream);
     hipLaunchKernelGGL(OuterKernel<T>, dim3(nwg), dim3(kWG), 0, stream, a);
   }
-  hipLaunchKernelGGL(NormKernel<T>, dim3(ne), dim3(kWG), 0, stream, a, norms_dev, kWG);
+  hipLaunchKernelGGL(NormKernel<T>, dim3(ne), dim3(kNormWG), 0, stream, a, norms_dev, kWG);
   PTH_HIP(hipGetLastError());
   PTH_HIP(hipMemcpyAsync(norms_host, norms_dev, sizeof(double) * ne, hipMemcpyDeviceToHost,
                          stream));
