@@ -71,6 +71,21 @@ def kernel_bytes(nb, n, g, V, exch_elems):
     }
 
 
+def measured_traffic(workload, kernel):
+    """HBM-side bytes per launch of `kernel` in `workload` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (profiles/traffic.json, scaled by the copy calibration),
+    or None when the summary does not cover that kernel."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    e = t.get(workload)
+    if not e or e.get("kernel") != kernel:
+        return None
+    return (e["fetch_kib"] * t["fetch_scale"] + e["write_kib"] * t["write_scale"]) * 1024
+
+
 def step_bytes(nb, n, g, V, exchange_bytes):
     """Algorithmic bytes of one RK2 cycle in the reference task decomposition: per block the
     reconstruction/flux, divergence, averaging, update, derived and dt tasks, plus the
@@ -241,7 +256,8 @@ def gmg_bench(args):
         "error_vs_fp64_solver": float(np.sqrt(np.mean((sols["mixed"] - sols["fp64"]) ** 2)) /
                                       np.sqrt(np.mean(sols["fp64"] ** 2))),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("gmg", dom),
                      "algorithmic_bytes_per_launch": kb[dom], "avg_launch_us": avg_s * 1e6},
         "iteration_roofline": {"algorithmic_bytes_per_iteration": it_bytes,
                                "ms_per_iteration": per_it * 1e3,
@@ -496,7 +512,9 @@ def main():
                        drv.tuning() if args.driver != "tasks" else None},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-                     "traffic": None, "algorithmic_bytes_per_launch": kb.get(dom),
+                     "traffic": measured_traffic("burgers-amr" if amr else "burgers-uniform",
+                                                 dom),
+                     "algorithmic_bytes_per_launch": kb.get(dom),
                      "avg_launch_us": avg_s * 1e6},
         "step_roofline": {"algorithmic_bytes_per_step": sb,
                           "achieved_GBps": sb / (wall / args.steps) / 1e9,

@@ -253,6 +253,8 @@ struct MGParams {
   Real omega = 2.0 / 3.0;
   int pre_sweeps = 2, post_sweeps = 2;
   bool mixed_precision = true;  // FP32 V-cycles inside FP64 iterative refinement
+  int inner_cycles = 2;          // FP32 V-cycles per FP64 refinement step
+  bool column_sweeps = true;     // Jacobi by z columns where the level size allows
 };
 
 // MGSolver<u, rhs, PoissonEquation> on the uniform n^3 grid of example/poisson_gmg ([0,1]^3,
@@ -262,6 +264,9 @@ class MGSolver {
   MGSolver(int n, MGParams p = MGParams()) : n_(n), p_(p) {
     h_ = pth_gmg_create(n, p.mixed_precision ? 0 : 1, p.omega, p.pre_sweeps, p.post_sweeps);
     if (!h_) throw std::runtime_error(std::string("MGSolver: ") + pth_last_error());
+    detail::Check(pth_gmg_set_inner_cycles(h_, p.mixed_precision ? p.inner_cycles : 1),
+                  "MGSolver");
+    detail::Check(pth_gmg_set_column_sweeps(h_, p.column_sweeps ? 1 : 0), "MGSolver");
   }
   ~MGSolver() {
     if (h_) pth_gmg_destroy(h_);
