@@ -395,6 +395,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--driver", choices=["graph", "eager", "tasks"], default="graph")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="burgers: skip timing the plain one-launch-per-buffer task form")
     ap.add_argument("--form", choices=["auto", "sweep", "overlap", "serial"], default="auto",
                     help="burgers device driver stage form: one block-sweep launch per stage; "
                          "plane reconstruction with the exchange on a second stream; plane "
@@ -521,6 +523,29 @@ def main():
                           "frac": sb / (wall / args.steps) / 1e9 / HBM_PEAK_GBS},
         "kernels": kernels,
     }
+    if args.driver != "tasks" and not args.no_plain:
+        # the plain HIP form of the same step: the reference's task order call by call, one
+        # kernel launch per boundary buffer, no graph, no fused kernels
+        del drv
+        pmesh = make_burgers_mesh(nx, block, g, ns, refine_spheres=spheres)
+        pmesh.set_pack_mode(pa.PackMode.per_buffer)
+        pd = BurgersDriver(pmesh, cfl=0.4)
+        pd.initialize("sinusoid", seed=seed)
+        pd.cycle()
+        pmesh.synchronize()
+        pc = 3
+        tp = time.perf_counter()
+        for _ in range(pc):
+            pd.cycle()
+        pmesh.synchronize()
+        pwall = time.perf_counter() - tp
+        pv = nb * n ** 3 * pc / pwall
+        out["plain_form"] = {
+            "value": pv, "unit": "zone-cycles/s", "ms_per_step": pwall / pc * 1e3,
+            "speedup": value / pv,
+            "form": "task-by-task calls in the reference order, one launch per boundary "
+                    f"buffer, eager ({pc} cycles after 1 warm-up)"}
+        del pd, pmesh
     if not args.no_cpu_baseline:
         threads = min(len(os.sched_getaffinity(0)), 16)
         cw = cpu_baseline(nx, block, g, ns, spheres, seed, cpu_cycles, threads)

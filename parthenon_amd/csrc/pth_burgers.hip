@@ -840,28 +840,27 @@ __device__ __forceinline__ void CellStates(const double *q, unsigned c, unsigned
     LinearRecon(LoadAt(q, c - st), LoadAt(q, c), LoadAt(q, c + st), ql, qr);
 }
 
-// flxcor_send of the block-sweep stage: one thread per coarse face cell of a patch evaluates
-// the HLL fluxes of its 2x2 fine faces straight from the fine block's state (the left state of
-// face f is ql of cell f - 1, the right state qr of cell f, as CalculateFluxes stores them) and
-// writes their restriction to the correction store.
+// flxcor_send of the block-sweep stage: four adjacent threads per coarse face cell of a patch,
+// one per fine face (p, q), evaluate the HLL flux of their face straight from the fine block's
+// state (the left state of face f is ql of cell f - 1, the right state qr of cell f, as
+// CalculateFluxes stores them); the first of the four gathers the other three by lane
+// shuffles and writes their restriction to the correction store.
 template <int V>
 __global__ void __launch_bounds__(64)
     FlxcorDirectKernel(const FluxPatch *__restrict__ patches, const double *__restrict__ U,
                        double *__restrict__ store, Geo G, int recon) {
   const FluxPatch &P = patches[blockIdx.x];
   const int t = blockIdx.y * 64 + threadIdx.x;
-  if (t >= P.n[0] * P.n[1]) return;
-  const int a = t % P.n[0], c = t / P.n[0];
+  const int cell = t >> 2, sub = t & 3, p = sub & 1, q = sub >> 1;
+  const bool live = cell < P.n[0] * P.n[1];
+  const int a = live ? cell % P.n[0] : 0, c = live ? cell / P.n[0] : 0;
   const std::int64_t cs = G.cells;
   const double *Ub = U + static_cast<std::int64_t>(P.fine) * V * cs;
   const unsigned st = 8u * static_cast<unsigned>(P.d == 0 ? 1 : P.d == 1 ? G.e0 : G.e0 * G.e1);
-  // flux of fine face (p, q) of this coarse face cell; zero where the patch has no such face
-  auto face = [&](int p, int q, double *F) {
-    if (p >= P.step[0] || q >= P.step[1]) {
+  double F[V];
 #pragma unroll
-      for (int n = 0; n < V; ++n) F[n] = 0.0;
-      return;
-    }
+  for (int n = 0; n < V; ++n) F[n] = 0.0;
+  if (live && p < P.step[0] && q < P.step[1]) {
     const unsigned o = 8u * static_cast<unsigned>(PatchFineCell(P, G, a, c, p, q));
     double l[V], r[V];
 #pragma unroll
@@ -871,17 +870,17 @@ __global__ void __launch_bounds__(64)
       CellStates(Ub + n * cs, o, st, recon, y, r[n]);
     }
     HllGen<V>([&](int n) { return l[n]; }, [&](int n) { return r[n]; }, P.d, F);
-  };
-  double F00[V], F01[V], F10[V], F11[V];
-  face(0, 0, F00);
-  face(1, 0, F01);
-  face(0, 1, F10);
-  face(1, 1, F11);
+  }
+  const int lane0 = static_cast<int>(threadIdx.x) & ~3;
   const std::int64_t dst = P.dst + a + static_cast<std::int64_t>(c) * P.rows;
 #pragma unroll
-  for (int n = 0; n < V; ++n)
-    store[dst + static_cast<std::int64_t>(n) * P.comp] = RestrictFace(
-        P.d, P.area, F00[n], F01[n], F10[n], F11[n], P.step[0], P.step[1]);
+  for (int n = 0; n < V; ++n) {
+    const double f00 = __shfl(F[n], lane0, 64), f01 = __shfl(F[n], lane0 + 1, 64);
+    const double f10 = __shfl(F[n], lane0 + 2, 64), f11 = __shfl(F[n], lane0 + 3, 64);
+    if (live && sub == 0)
+      store[dst + static_cast<std::int64_t>(n) * P.comp] =
+          RestrictFace(P.d, P.area, f00, f01, f10, f11, P.step[0], P.step[1]);
+  }
 }
 
 // ---- block sweep: reconstruction, HLL, flux divergence and RK update in one launch ----------
@@ -1554,7 +1553,7 @@ pth_status FlxcorDirect(const Topology &t, int V, const FluxCorrection &fc, cons
   }
   ProfScope prof("flxcor_direct", s);
   const Geo G = MakeGeo(t, V);
-  const dim3 grid(static_cast<unsigned>(fc.host.size()), (fc.max_cells + 63) / 64);
+  const dim3 grid(static_cast<unsigned>(fc.host.size()), (4 * fc.max_cells + 63) / 64);
 #define PTH_FD_CASE(NV)                                                                       \
   case NV:                                                                                    \
     hipLaunchKernelGGL(FlxcorDirectKernel<NV>, grid, dim3(64), 0, s, fc.dev, U, fc.store, G,  \
