@@ -346,6 +346,65 @@ pth_status Mesh::BuildCache(MeshData &md, int bt) {
       }
     }
   }
+  // Restriction before the send: the regions of one block's links toward coarser neighbours
+  // overlap (a face region contains the halves of its edge and corner regions), and every
+  // coarse cell restricts to the same value whichever region asks for it; restrict each
+  // block's union once, as disjoint boxes.
+  {
+    std::vector<Region> merged;
+    std::map<std::pair<double *, double *>, std::vector<const Region *>> by_block;
+    std::vector<std::pair<double *, double *>> order;
+    for (const Region &r : c.restrict_send.host) {
+      auto key = std::make_pair(r.a, r.b);
+      if (!by_block.count(key)) order.push_back(key);
+      by_block[key].push_back(&r);
+    }
+    const int E0 = ce.ent(0), E1 = ce.ent(1), E2 = ce.ent(2);
+    std::vector<char> mask(static_cast<size_t>(E0) * E1 * E2);
+    auto at = [&](int i, int j, int k) -> char & {
+      return mask[(static_cast<size_t>(k) * E1 + j) * E0 + i];
+    };
+    for (const auto &key : order) {
+      const auto &rs = by_block[key];
+      std::fill(mask.begin(), mask.end(), 0);
+      for (const Region *r : rs)
+        for (int k = r->lo[2]; k < r->lo[2] + r->n[2]; ++k)
+          for (int j = r->lo[1]; j < r->lo[1] + r->n[1]; ++j)
+            for (int i = r->lo[0]; i < r->lo[0] + r->n[0]; ++i) at(i, j, k) = 1;
+      const Region &r0 = *rs.front();
+      for (int k = 0; k < E2; ++k)
+        for (int j = 0; j < E1; ++j)
+          for (int i = 0; i < E0; ++i) {
+            if (at(i, j, k) != 1) continue;
+            int i1 = i;
+            while (i1 + 1 < E0 && at(i1 + 1, j, k) == 1) ++i1;
+            auto row_ok = [&](int jj, int kk) {
+              for (int ii = i; ii <= i1; ++ii)
+                if (at(ii, jj, kk) != 1) return false;
+              return true;
+            };
+            int j1 = j;
+            while (j1 + 1 < E1 && row_ok(j1 + 1, k)) ++j1;
+            int k1 = k;
+            for (bool grow = true; grow && k1 + 1 < E2;) {
+              for (int jj = j; jj <= j1 && grow; ++jj) grow = row_ok(jj, k1 + 1);
+              if (grow) ++k1;
+            }
+            for (int kk = k; kk <= k1; ++kk)
+              for (int jj = j; jj <= j1; ++jj)
+                for (int ii = i; ii <= i1; ++ii) at(ii, jj, kk) = 2;
+            Box bx;
+            bx.lo[0] = i;
+            bx.lo[1] = j;
+            bx.lo[2] = k;
+            bx.hi[0] = i1;
+            bx.hi[1] = j1;
+            bx.hi[2] = k1;
+            merged.push_back(MakeRegion(r0.a, r0.b, bx, r0.nc, r0.gid, fe, &ce));
+          }
+    }
+    c.restrict_send.host.swap(merged);
+  }
   // Every element a kernel will touch must lie inside its array: pack/unpack boxes inside the
   // field brick, restriction children and prolongation stencils inside fine and coarse bricks.
   auto inside = [](const Region &r, const Extent &e, int halo_lo, int halo_hi, int scale,
