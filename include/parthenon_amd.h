@@ -207,6 +207,43 @@ pth_status pth_burgers_clock(pth_mesh *m, double *dt, double *time);
 /* Waits for the driver's stream. */
 pth_status pth_burgers_synchronize(pth_mesh *m);
 
+/* ---- adaptive mesh refinement ----
+ * <parthenon/mesh> refinement = adaptive with numlevel and derefine_count (mesh.cpp:179,
+ * mesh_refinement.cpp:55-58). Must precede pth_mesh_plan / pth_mesh_finalize; the mesh keeps
+ * coarse buffers on every level. Levels here count from the base grid; the reference's
+ * logical level is root_level + level, root_level = log2 of the largest power of two dividing
+ * the base block count of every dimension (forest.cpp:64-105). */
+pth_status pth_mesh_set_adaptive(pth_mesh *m, int numlevel, int derefine_count);
+/* One <parthenon/refinementN> block (AMRCriteria, amr_criteria.cpp:27-68, 88-106): method 1
+ * derivative_order_1, 2 derivative_order_2, on component comp (vector_i) of field; max_level
+ * <= 0 means numlevel and larger values are clamped to it. As in the reference, max_level is
+ * compared with the logical level (refinement_package.cpp:78). */
+pth_status pth_mesh_add_refinement_criterion(pth_mesh *m, int method, const char *field, int comp,
+                                             double refine_tol, double derefine_tol,
+                                             int max_level);
+/* Refinement::Tag<MeshData> (refinement_package.cpp:153-173): every criterion on md's data in
+ * one launch per criterion, then CheckAllRefinement and MeshRefinement::SetRefinement
+ * (mesh_refinement.cpp:85-117) per block. Synchronous (the flags live on the host, like the
+ * reference's refine_flag_). Ghost cells of the tagged fields must be current. */
+pth_status pth_refinement_tag(pth_meshdata *md);
+/* MeshRefinement::SetRefinement(AmrTag) of one block (tag -1, 0 or 1). */
+pth_status pth_mesh_set_refinement(pth_mesh *m, int gid, int tag);
+/* refine_flag_ and deref_count_ of every block (arrays of pth_mesh_nblocks entries). */
+pth_status pth_mesh_refinement_flags(const pth_mesh *m, int *flags, int *counts);
+int pth_mesh_root_level(const pth_mesh *m);
+/* Mesh::LoadBalancingAndAdaptiveMeshRefinement on one GPU (mesh-amr_loadbalance.cpp:324-351):
+ * UpdateMeshBlockTree (:496-617) from the flags; if the tree changed, the block list, every
+ * MeshData's fields, the region tables and the buffers are rebuilt in place and the data moved
+ * (RedistributeAndRefineMeshBlocks, :627-1003): restriction of derefined blocks, copies of
+ * moved blocks and octants, interior prolongation of refined blocks, then the boundary
+ * exchange of "base" (CommunicateBoundaries). MeshData handles stay valid; data pointers and
+ * block ids change. *modified = 1 when the mesh changed. */
+pth_status pth_mesh_adapt(pth_mesh *m, int *modified);
+/* EvolutionDriver::Execute after a modified mesh (driver.cpp:128-131): FillDerived,
+ * InitializeBlockTimeSteps on the new blocks and SetGlobalTimeStep from the dt of the last
+ * cycle, on the device clock of pth_burgers_init. */
+pth_status pth_burgers_after_remesh(pth_mesh *m);
+
 /* History: the 8 octant masses sum(U^2 * cell volume / mesh volume) of the package's history
  * variables "MS Mass 0..7" (burgers_package.cpp:96-135, MassHistory :406-439), octants in the
  * reference order (x outer, then y, then z); synchronous. */

@@ -6,6 +6,7 @@
 #include <cstring>
 #include <memory>
 
+#include "orc_amr.hpp"
 #include "orc_burgers.hpp"
 #include "orc_gmg.hpp"
 
@@ -534,6 +535,135 @@ void orc_gmg_prolong_add(void *h, int l, const double *coarse, double *fine) {
   for (int k = 0; k < F.n; ++k)
     for (int j = 0; j < F.n; ++j)
       for (int i = 0; i < F.n; ++i) fine[q++] = F.x[F.Idx(k, j, i)];
+}
+
+// ---- adaptive remesh ----
+// The refinement state of one OMesh. Remeshing replaces the OMesh's mesh, so every orc_mesh_*
+// query afterwards sees the new block list.
+struct OAmr {
+  OMesh *om;
+  Adaptive a;
+  std::unique_ptr<Mesh> pending;
+};
+
+void *orc_amr_new(void *mesh, int numlevel, int deref_threshold) {
+  auto *o = new OAmr;
+  o->om = static_cast<OMesh *>(mesh);
+  o->a.Init(*o->om->m, numlevel, deref_threshold);
+  return o;
+}
+void orc_amr_free(void *h) { delete static_cast<OAmr *>(h); }
+
+void orc_amr_add_criterion(void *h, int method, int comp, double refine_tol,
+                           double derefine_tol, int max_level) {
+  Criterion c;
+  c.method = method;
+  c.comp = comp;
+  c.refine_tol = refine_tol;
+  c.derefine_tol = derefine_tol;
+  c.max_level = max_level;
+  static_cast<OAmr *>(h)->a.AddCriterion(c);
+}
+
+int orc_amr_root_level(void *h) { return static_cast<OAmr *>(h)->a.root_level; }
+
+// largest derivative measure of component comp of every block (out[nblocks])
+void orc_amr_derivative_max(void *h, int method, int ncomp, int comp, const double *data,
+                            double *out) {
+  const Mesh &m = *static_cast<OAmr *>(h)->om->m;
+  const std::int64_t sz = m.cellbounds.size();
+  for (int b = 0; b < (int)m.blocks.size(); ++b)
+    out[b] = DerivativeMax(m, data + ((std::int64_t)b * ncomp + comp) * sz, method);
+}
+
+// Refinement::Tag on field data [block][comp][k][j][i]
+void orc_amr_tag(void *h, int ncomp, const double *data) {
+  OAmr &o = *static_cast<OAmr *>(h);
+  Field f;
+  f.ncomp = ncomp;
+  f.data.assign(data, data + o.om->m->blocks.size() * ncomp * o.om->m->cellbounds.size());
+  o.a.Tag(*o.om->m, f);
+}
+
+void orc_amr_set_refinement(void *h, int gid, int tag) {
+  OAmr &o = *static_cast<OAmr *>(h);
+  o.a.SetRefinement(*o.om->m, gid, tag);
+}
+
+void orc_amr_flags(void *h, int *flag, int *count) {
+  OAmr &o = *static_cast<OAmr *>(h);
+  for (size_t b = 0; b < o.a.flag.size(); ++b) {
+    flag[b] = o.a.flag[b];
+    count[b] = o.a.count[b];
+  }
+}
+
+// UpdateMeshBlockTree: the block count of the new list, or 0 when the tree did not change.
+int orc_amr_update(void *h, int *nnew, int *ndel) {
+  OAmr &o = *static_cast<OAmr *>(h);
+  std::unique_ptr<Mesh> tree(new Mesh(o.om->m->d));
+  o.a.UpdateTree(*o.om->m, *tree, *nnew, *ndel);
+  o.pending.reset();
+  if (*nnew == 0 && *ndel == 0) return 0;
+  o.pending = MeshFromTree(*tree);
+  return (int)o.pending->blocks.size();
+}
+
+// Moves one field (data of the current mesh) onto the pending mesh's block list.
+void orc_amr_transfer(void *h, int ncomp, const double *old_data, double *new_data) {
+  OAmr &o = *static_cast<OAmr *>(h);
+  Field of, nf;
+  of.ncomp = ncomp;
+  of.data.assign(old_data, old_data + o.om->m->blocks.size() * ncomp * o.om->m->cellbounds.size());
+  TransferField(*o.om->m, of, *o.pending, nf);
+  std::memcpy(new_data, nf.data.data(), sizeof(double) * nf.data.size());
+}
+
+// Makes the pending mesh current (flags of moved blocks carried, new blocks at 0).
+void orc_amr_commit(void *h) {
+  OAmr &o = *static_cast<OAmr *>(h);
+  CarryFlags(Origins(*o.om->m, *o.pending), o.a);
+  o.om->m = std::move(o.pending);
+}
+
+// Refine / derefine one location of the current tree directly (Tree::Refine, Tree::Derefine)
+// and re-finalize; returns the blocks added or removed.
+int orc_mesh_tree_refine(void *mesh, int level, const std::int64_t *l) {
+  Mesh &m = *static_cast<OMesh *>(mesh)->m;
+  Loc a;
+  a.level = level;
+  for (int k = 0; k < 3; ++k) a.l[k] = l[k];
+  const int n = m.Refine(a);
+  m.multilevel = true;
+  m.Finalize();
+  return n;
+}
+int orc_mesh_tree_derefine(void *mesh, int level, const std::int64_t *l) {
+  Mesh &m = *static_cast<OMesh *>(mesh)->m;
+  Loc a;
+  a.level = level;
+  for (int k = 0; k < 3; ++k) a.l[k] = l[k];
+  const int n = m.Derefine(a);
+  m.Finalize();
+  return n;
+}
+int orc_mesh_is_leaf(void *mesh, int level, const std::int64_t *l) {
+  Mesh &m = *static_cast<OMesh *>(mesh)->m;
+  Loc a;
+  a.level = level;
+  for (int k = 0; k < 3; ++k) a.l[k] = l[k];
+  return m.leaves.count(a) ? 1 : 0;
+}
+
+// Adaptive Burgers: n cycles of the benchmark with tagging and remeshing after each; returns
+// the number of cycles after which the mesh changed. The OBurgers and the OMesh it was built
+// on are updated in place.
+int orc_amr_burgers_cycles(void *amr, void *burgers, int n) {
+  OAmr &o = *static_cast<OAmr *>(amr);
+  Burgers &B = static_cast<OBurgers *>(burgers)->b;
+  int changed = 0;
+  for (int q = 0; q < n; ++q) changed += AdaptiveCycle(o.om->m, B, o.a) ? 1 : 0;
+  return changed;
 }
 
 }  // extern "C"

@@ -23,6 +23,7 @@
 #include <array>
 #include <cmath>
 #include <cstdint>
+#include <limits>
 #include <map>
 #include <set>
 #include <stdexcept>
@@ -193,6 +194,50 @@ struct Mesh {
           }
     }
     return added;
+  }
+
+  // reference: tree.cpp:228-275 (Tree::Derefine with proper nesting). The daughters of
+  // `parent` must all be leaves, and no same-level neighbour of a daughter (the 3^ndim cube
+  // around it, periodic images included) may be an internal node.
+  int Derefine(const Loc &parent) {
+    const std::vector<Loc> ds = Daughters(parent);
+    for (const Loc &dl : ds) {
+      if (!leaves.count(dl)) return 0;
+      for (int k = (ndim > 2 ? -1 : 0); k <= (ndim > 2 ? 1 : 0); ++k)
+        for (int j = (ndim > 1 ? -1 : 0); j <= (ndim > 1 ? 1 : 0); ++j)
+          for (int i = -1; i <= 1; ++i) {
+            Loc n = dl;
+            n.l[0] += i;
+            n.l[1] += j;
+            n.l[2] += k;
+            if (!Wrap(n)) continue;
+            if (internal.count(n)) return 0;
+          }
+    }
+    for (const Loc &dl : ds) leaves.erase(dl);
+    internal.erase(parent);
+    leaves.insert(parent);
+    return static_cast<int>(ds.size()) - 1;
+  }
+
+  // reference: forest.cpp:64-105 (Forest::HyperRectangular). The base lattice is a grid of
+  // trees of 2^root_level blocks per side; root_level = log2 of the largest power of two
+  // dividing the base block count of every dimension (0 when a dimension is inactive).
+  // Logical levels of the reference are root_level + the levels used here.
+  int RootLevel() const {
+    std::int64_t div = std::numeric_limits<int>::max();
+    for (int k = 0; k < 3; ++k) {
+      if (!active[k]) {
+        div = 1;
+        continue;
+      }
+      std::int64_t p = 1;
+      while (nbase[k] % (2 * p) == 0) p *= 2;
+      div = std::min(div, p);
+    }
+    int lev = 0;
+    while ((std::int64_t(1) << (lev + 1)) <= div) ++lev;
+    return lev;
   }
 
   // reference: tree.cpp:72-94

@@ -117,6 +117,14 @@ _SIGNATURES = {
     "pth_burgers_tuning": (_I, [_P, C.POINTER(_I)]),
     "pth_burgers_set_tuning": (_I, [_P, C.POINTER(_I)]),
     "pth_burgers_synchronize": (_I, [_P]),
+    "pth_mesh_set_adaptive": (_I, [_P, _I, _I]),
+    "pth_mesh_add_refinement_criterion": (_I, [_P, _I, C.c_char_p, _I, _D, _D, _I]),
+    "pth_refinement_tag": (_I, [_P]),
+    "pth_mesh_set_refinement": (_I, [_P, _I, _I]),
+    "pth_mesh_refinement_flags": (_I, [_P, C.POINTER(_I), C.POINTER(_I)]),
+    "pth_mesh_root_level": (_I, [_P]),
+    "pth_mesh_adapt": (_I, [_P, C.POINTER(_I)]),
+    "pth_burgers_after_remesh": (_I, [_P]),
     "pth_gmg_create": (_P, [_I, _I, _D, _I, _I]),
     "pth_gmg_destroy": (None, [_P]),
     "pth_gmg_levels": (_I, [_P]),
@@ -285,6 +293,57 @@ class Mesh:
 
     def synchronize(self):
         _check(lib().pth_mesh_synchronize(self._h), "synchronize")
+
+    # -- adaptive refinement (<parthenon/mesh> refinement = adaptive) --
+    def set_adaptive(self, numlevel, derefine_count=10):
+        """numlevel and derefine_count of <parthenon/mesh>; before finalize."""
+        _check(lib().pth_mesh_set_adaptive(self._h, numlevel, derefine_count), "set_adaptive")
+
+    def add_refinement_criterion(self, field, comp=0, method="derivative_order_1",
+                                 refine_tol=0.5, derefine_tol=0.05, max_level=0):
+        """One <parthenon/refinementN> block (field, vector_i, method, tolerances)."""
+        m = {"derivative_order_1": 1, "derivative_order_2": 2}.get(method, method)
+        _check(lib().pth_mesh_add_refinement_criterion(self._h, int(m), field.encode(), comp,
+                                                       refine_tol, derefine_tol, max_level),
+               "add_refinement_criterion")
+
+    @property
+    def root_level(self):
+        return lib().pth_mesh_root_level(self._h)
+
+    def set_refinement(self, gid, tag):
+        """MeshRefinement::SetRefinement(AmrTag) of block gid (-1, 0, 1)."""
+        _check(lib().pth_mesh_set_refinement(self._h, gid, int(tag)), "set_refinement")
+
+    def refinement_flags(self):
+        """(refine_flag_, deref_count_) of every block."""
+        f, c = (_I * self.nblocks)(), (_I * self.nblocks)()
+        _check(lib().pth_mesh_refinement_flags(self._h, f, c), "refinement_flags")
+        return np.array(f[:], dtype=np.int32), np.array(c[:], dtype=np.int32)
+
+    def adapt(self):
+        """Mesh::LoadBalancingAndAdaptiveMeshRefinement: True when the mesh changed. Block
+        ids and counts change; MeshData objects stay valid."""
+        mod = _I()
+        _check(lib().pth_mesh_adapt(self._h, C.byref(mod)), "adapt")
+        if mod.value:
+            self._shapes()
+        return bool(mod.value)
+
+    def locs(self):
+        """(level, (lx1, lx2, lx3)) of every block in gid order."""
+        return [(b["level"], b["l"]) for b in map(self.block, range(self.nblocks))]
+
+
+class Refinement:  # src/amr_criteria/refinement_package.hpp
+    @staticmethod
+    def Tag(md):
+        return _check(lib().pth_refinement_tag(md._h), "Refinement::Tag")
+
+
+def LoadBalancingAndAdaptiveMeshRefinement(mesh):
+    """src/mesh/mesh-amr_loadbalance.cpp:324-351 on one GPU; returns Mesh::modified."""
+    return mesh.adapt()
 
 
 class MeshData:

@@ -12,7 +12,7 @@ min(2 * dt_prev, estimate).
 import math
 
 from . import (ApplyBoundaryConditionsOnCoarseOrFineMD, BoundaryType,
-               LoadAndSendFluxCorrections, Mesh, Metadata, ProlongateBounds,
+               LoadAndSendFluxCorrections, Mesh, Metadata, ProlongateBounds, Refinement,
                ReceiveBoundBufs, ReceiveFluxCorrections, SendBoundBufs, SetBounds,
                SetFluxCorrections, StartReceiveBoundBufs, StartReceiveFluxCorrections,
                TaskStatus, Update, burgers_package)
@@ -21,11 +21,15 @@ RK2_BETA = (1.0, 0.5)
 
 
 def make_burgers_mesh(nx=(64, 64, 64), block=(16, 16, 16), nghost=4, num_scalars=4,
-                      refine_spheres=(), refine_boxes=(), bcs=None):
+                      refine_spheres=(), refine_boxes=(), bcs=None, adaptive=None):
     """Mesh with the fields burgers_package::Initialize registers
     (benchmarks/burgers/burgers_package.cpp:76-94): U with FillGhost/Independent/WithFluxes and
     the OneCopy derived field. bcs: optional face conditions (inner x1, outer x1, ...,
-    outer x3); a dimension whose faces are not periodic is not periodic."""
+    outer x3); a dimension whose faces are not periodic is not periodic. adaptive: optional
+    dict(numlevel=..., derefine_count=..., criteria=[dict(field=, comp=, method=, refine_tol=,
+    derefine_tol=, max_level=)]) for refinement = adaptive; the VIBE input
+    (benchmarks/burgers/burgers.pin:21-55) is numlevel 2, derefine_count 10 and one
+    derivative_order_1 criterion on U component 3 with tolerances 0.5 / 0.2."""
     periodic = (True, True, True)
     if bcs is not None:
         periodic = tuple(bcs[2 * d] == "periodic" and bcs[2 * d + 1] == "periodic"
@@ -35,6 +39,10 @@ def make_burgers_mesh(nx=(64, 64, 64), block=(16, 16, 16), nghost=4, num_scalars
     m.add_field("U", 3 + num_scalars,
                 Metadata.FillGhost | Metadata.Independent | Metadata.WithFluxes)
     m.add_field("derived", 1, Metadata.OneCopy)
+    if adaptive is not None:
+        m.set_adaptive(adaptive.get("numlevel", 1), adaptive.get("derefine_count", 10))
+        for c in adaptive.get("criteria", ()):
+            m.add_refinement_criterion(**c)
     m.finalize()
     return m
 
@@ -77,6 +85,19 @@ class DeviceBurgersDriver:
 
     def synchronize(self):
         self._check(self._lib().pth_burgers_synchronize(self.mesh._h), "synchronize")
+
+    def adaptive_cycles(self, n):
+        """n cycles of EvolutionDriver::Execute on an adaptive mesh (driver.cpp:110-131): a
+        cycle, Refinement::Tag of "base", the remesh and, when the mesh changed, FillDerived and
+        the time step from the new blocks. Returns the number of remeshes."""
+        changed = 0
+        for _ in range(n):
+            self.cycles(1)
+            Refinement.Tag(self.base)
+            if self.mesh.adapt():
+                self._check(self._lib().pth_burgers_after_remesh(self.mesh._h), "after_remesh")
+                changed += 1
+        return changed
 
     def form(self):
         """The stage form in use: "sweep", "overlap" or "serial"."""
@@ -171,3 +192,16 @@ class BurgersDriver:
         self.ncycle += 1
         self.time += self.dt
         self.dt = min(2.0 * self.dt, new_dt)
+
+    def adaptive_cycle(self):
+        """One cycle, then Refinement::Tag (burgers_driver.cpp:139-145) and
+        LoadBalancingAndAdaptiveMeshRefinement; after a remesh FillDerived and
+        InitializeBlockTimeSteps + SetGlobalTimeStep (driver.cpp:128-131). True if remeshed."""
+        dt_used = self.dt
+        self.cycle()
+        Refinement.Tag(self.base)
+        if not self.mesh.adapt():
+            return False
+        burgers_package.CalculateDerived(self.base)
+        self.dt = min(2.0 * dt_used, burgers_package.EstimateTimestepMesh(self.base, self.cfl))
+        return True

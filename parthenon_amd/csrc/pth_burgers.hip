@@ -1153,7 +1153,10 @@ __global__ void __launch_bounds__(64)
 }
 
 // Final CFL minimum and EvolutionDriver::SetGlobalTimeStep on device (driver.cpp:210-235):
-// dtvar = {dt, time, estimate}; time += dt (when advance), dt = min(2 dt, cfl * min).
+// dtvar = {dt, time, estimate, dt of the last cycle}. advance 1 (end of a cycle): time += dt,
+// the cycle's dt is kept, dt = min(2 dt, cfl * min); advance 0 (start): dt = min(2 dt, est)
+// with dt still at its initial maximum; advance 2 (after a remesh): the estimate of the new
+// blocks replaces the old one, dt = min(2 dt_last, cfl * min).
 // One workgroup of kClockWG threads: strided partial minima, wavefront shuffles, then LDS.
 constexpr int kClockWG = 1024;
 __global__ void __launch_bounds__(kClockWG)
@@ -1169,8 +1172,11 @@ __global__ void __launch_bounds__(kClockWG)
     m = wmin[0];
     for (int w = 1; w < kClockWG / 64; ++w) m = DMin(m, wmin[w]);
     const double est = cfl * m;
-    double dt = dtvar[0];
-    if (advance) dtvar[1] = dtvar[1] + dt;
+    double dt = advance == 2 ? dtvar[3] : dtvar[0];
+    if (advance == 1) {
+      dtvar[1] = dtvar[1] + dt;
+      dtvar[3] = dt;
+    }
     if (dt < 0.1 * DBL_MAX) dt *= 2.0;
     dtvar[0] = DMin(dt, est);
     dtvar[2] = est;

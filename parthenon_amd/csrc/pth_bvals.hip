@@ -124,6 +124,32 @@ __global__ void __launch_bounds__(kWG)
   }
 }
 
+// Box copies (remesh): region element e moves from its box in a to the same box position in
+// b, which points at the destination box origin. Loads before stores, like the pack.
+__global__ void __launch_bounds__(kWG)
+    CopyKernel(const Region *__restrict__ reg, const Item *__restrict__ items) {
+  const Item I = items[blockIdx.x];
+  const Region &R = reg[I.region];
+  double v[kPer];
+  std::int64_t off[kPer];
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int local = u * kWG + static_cast<int>(threadIdx.x);
+    if (local < I.count) {
+      const Cell d = Decode(R, static_cast<std::uint32_t>(I.begin + local));
+      v[u] = R.a[VarOffset(R, d)];
+      off[u] = static_cast<std::int64_t>(d.c) * R.bstride[2] +
+               static_cast<std::int64_t>(d.k) * R.bstride[1] +
+               static_cast<std::int64_t>(d.j) * R.bstride[0] + d.i;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int local = u * kWG + static_cast<int>(threadIdx.x);
+    if (local < I.count) R.b[off[u]] = v[u];
+  }
+}
+
 // The plain form: one launch per region, one thread per element.
 __global__ void __launch_bounds__(kWG) PackOneKernel(Region R, int to_buffer) {
   const std::int64_t e = static_cast<std::int64_t>(blockIdx.x) * kWG + threadIdx.x;
@@ -378,6 +404,15 @@ pth_status LaunchPack(const Table &t, double *, hipStream_t s, bool to_buffer, i
     else
       hipLaunchKernelGGL(PackKernel<false>, dim3(GridFor(n)), dim3(kWG), 0, s, t.dev, t.dev_items);
   }
+  PTH_HIP(hipGetLastError());
+  return PTH_COMPLETE;
+}
+
+pth_status LaunchCopy(const Table &t, hipStream_t s) {
+  if (t.total == 0) return PTH_COMPLETE;
+  ProfScope prof("remesh_copy", s);
+  const int n = static_cast<int>(t.items.size());
+  hipLaunchKernelGGL(CopyKernel, dim3(GridFor(n)), dim3(kWG), 0, s, t.dev, t.dev_items);
   PTH_HIP(hipGetLastError());
   return PTH_COMPLETE;
 }

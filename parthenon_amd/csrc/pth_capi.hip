@@ -102,6 +102,7 @@ Mesh::~Mesh() {
   if (dtvar) (void)hipFree(dtvar);
   if (scratch) (void)hipFree(scratch);
   if (pinned) (void)hipHostFree(pinned);
+  if (amr.maxd) (void)hipFree(amr.maxd);
 }
 
 int Mesh::FieldIndex(const std::string &n) const {

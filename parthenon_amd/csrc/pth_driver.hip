@@ -491,7 +491,7 @@ pth_status pth_burgers_init(pth_mesh *mh, double cfl, int recon) {
   hipStream_t s = m->run_stream;
   // order after any work the caller queued on the mesh stream
   PTH_HIP(hipStreamSynchronize(m->stream));
-  const double init[4] = {DBL_MAX, 0.0, 0.0, 0.0};
+  const double init[4] = {DBL_MAX, 0.0, 0.0, DBL_MAX};
   PTH_HIP(hipMemcpyAsync(m->dtvar, init, sizeof(init), hipMemcpyHostToDevice, s));
   // Mesh::Initialize after the problem generator: exchange, FillDerived, first time step
   if (Exchange(*m, *x.base, s) != PTH_COMPLETE) return PTH_FAIL;
@@ -589,6 +589,30 @@ pth_status pth_burgers_set_overlap(pth_mesh *mh, int on) {
   if (m->graph) (void)hipGraphDestroy(m->graph);
   m->graph_exec = nullptr;
   m->graph = nullptr;
+  return PTH_COMPLETE;
+}
+
+pth_status pth_burgers_after_remesh(pth_mesh *mh) {
+  Mesh *m = Impl(mh);
+  if (!m->run_ready) {
+    SetError("call pth_burgers_init first");
+    return PTH_FAIL;
+  }
+  Ctx x;
+  if (Setup(m, x) != PTH_COMPLETE) return PTH_FAIL;
+  hipStream_t s = m->run_stream;
+  PTH_HIP(hipStreamSynchronize(m->stream));
+  // FillDerived, then InitializeBlockTimeSteps and SetGlobalTimeStep from the last cycle's dt
+  if (x.fd >= 0 && BurgersDerived(m->topo, x.V, x.base->store[x.fu].data,
+                                  x.base->store[x.fd].data, s) != PTH_COMPLETE)
+    return PTH_FAIL;
+  const int np = static_cast<int>(m->scratch_n);
+  if (BurgersDtMin(m->topo, x.V, x.base->store[x.fu].data, m->coef, m->scratch, np,
+                   m->scratch + m->scratch_n, s) != PTH_COMPLETE)
+    return PTH_FAIL;
+  if (BurgersClock(m->scratch + m->scratch_n, 1, m->run_cfl, m->dtvar, 2, s) != PTH_COMPLETE)
+    return PTH_FAIL;
+  PTH_HIP(hipStreamSynchronize(s));
   return PTH_COMPLETE;
 }
 

@@ -107,6 +107,17 @@ class Topology {
   void RefineBox(const double box[6], int level);
   void RefineSphere(const double c[3], double r, int level);
   std::string Finalize();
+  // Adaptive refinement of a finalized tree (leaf / inner sets; Finalize rebuilds the blocks).
+  // Split one leaf with 2:1 nesting (reference Tree::Refine); returns the blocks added.
+  int Refine(const Lattice &a);
+  // Merge the children of `parent` back into it if they are all leaves and no same-level
+  // neighbour of a child is an inner node (reference Tree::Derefine); returns blocks removed.
+  int Derefine(const Lattice &parent);
+  // log2 of the largest power of two dividing every active dimension's base block count (0 if
+  // a dimension is inactive): the reference's root level (Forest::HyperRectangular).
+  int RootLevel() const;
+  // true when every neighbour link joins blocks at most one level apart
+  bool Nested() const;
   // Index box of one boundary region (reference CalcIndices, cell-centred fields).
   Box Indices(const Block &b, const Link &l, Side side, bool prores) const;
 
@@ -114,7 +125,7 @@ class Topology {
   std::int64_t Span(int d, int level) const { return nbase[d] << level; }
   bool Fold(Lattice &a) const;
   std::vector<Lattice> Children(const Lattice &a) const;
-  void Split(const Lattice &a);
+  int Split(const Lattice &a);
   void Ensure(const Lattice &a);
   double Position(int d, std::int64_t index, int edge, int level) const;
   void Neighbors(int gid);
@@ -229,6 +240,25 @@ struct FieldStore {
 
 class Mesh;
 
+// Adaptive refinement settings and per-block state (reference MeshRefinement members and the
+// <parthenon/refinementN> criteria).
+struct AmrCriterion {
+  int method;     // 1 derivative_order_1, 2 derivative_order_2
+  int field;      // field index
+  int comp;
+  double refine_tol, derefine_tol;
+  int max_level;  // compared with the logical level root_level + level
+};
+
+struct Adaptive {
+  bool on = false;
+  int numlevel = 1, deref_threshold = 10;
+  std::vector<AmrCriterion> crit;
+  std::vector<int> flag, count;  // refine_flag_, deref_count_ per block
+  double *maxd = nullptr;        // device: [block][criterion] derivative maxima
+  std::int64_t maxd_n = 0;
+};
+
 // One box of ghost cells of one block and field that the physical boundary conditions fill
 // (reference GenericBC, boundary_conditions_generic.hpp:174-255). Along each dimension every
 // cell of the box lies in the same class (inner ghosts, interior, outer ghosts), so each cell
@@ -339,6 +369,14 @@ class Mesh {
   pth_status SameLevelNeighbors();
   // patches of field f (built on first use); null with an error if f has no fluxes
   FluxCorrection *FluxPatches(int f);
+  // adaptive refinement
+  Adaptive amr;
+  void SetRefinement(int b, int tag);          // MeshRefinement::SetRefinement
+  pth_status Tag(MeshData &md);                // Refinement::Tag on md
+  pth_status Adapt(int *modified);             // LoadBalancingAndAdaptiveMeshRefinement
+  // takes the block list, storage and tables of nm (built for the new tree); nm receives the
+  // old ones and frees them when destroyed
+  void Adopt(Mesh &nm);
 };
 
 // Optional per-launch timing with HIP events on the launch stream (instrumentation regions
@@ -375,6 +413,9 @@ pth_status LaunchExchangeDirect(const Table &pack, const Table &unpack, hipStrea
 pth_status LaunchRestrict(const Table &t, const DeviceBlockGeom *geom, hipStream_t s);
 pth_status LaunchProlong(const Table &t, const DeviceBlockGeom *geom, hipStream_t s);
 pth_status LaunchBoundaryConditions(const BcTable &t, hipStream_t s);
+// box copies: region q copies its box from a (at lo, strides astride) to b (pre-offset to the
+// destination box origin, strides bstride)
+pth_status LaunchCopy(const Table &t, hipStream_t s);
 
 std::uint64_t Magic(int d);
 

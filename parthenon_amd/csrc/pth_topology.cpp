@@ -91,14 +91,17 @@ std::vector<Lattice> Topology::Children(const Lattice &a) const {
 }
 
 // Refine one leaf, then refine the leaves on the parent level that touch the corner of the
-// parent nearest to it, so every child keeps neighbours at most one level coarser.
-void Topology::Split(const Lattice &a) {
-  if (!leaf.count(a)) return;
+// parent nearest to it, so every child keeps neighbours at most one level coarser. Returns the
+// number of blocks added (children minus the split leaf, over the whole cascade).
+int Topology::Split(const Lattice &a) {
+  if (!leaf.count(a)) return 0;
   leaf.erase(a);
   inner.insert(a);
-  for (const auto &c : Children(a)) leaf.insert(c);
+  const std::vector<Lattice> kids = Children(a);
+  for (const auto &c : kids) leaf.insert(c);
   multilevel = true;
-  if (a.level == 0) return;
+  int added = static_cast<int>(kids.size()) - 1;
+  if (a.level == 0) return added;
   const Lattice p = Up(a);
   int corner[3];
   for (int d = 0; d < 3; ++d) corner[d] = static_cast<int>(a.x[d] - 2 * p.x[d]);
@@ -109,8 +112,76 @@ void Topology::Split(const Lattice &a) {
         q.x[0] += i + corner[0] - 1;
         if (active[1]) q.x[1] += j + corner[1] - 1;
         if (active[2]) q.x[2] += k + corner[2] - 1;
-        if (Fold(q)) Split(q);
+        if (Fold(q)) added += Split(q);
       }
+  return added;
+}
+
+int Topology::Refine(const Lattice &a) { return Split(a); }
+
+int Topology::Derefine(const Lattice &parent) {
+  const std::vector<Lattice> kids = Children(parent);
+  for (const Lattice &c : kids) {
+    if (!leaf.count(c)) return 0;
+    for (int k = active[2] ? -1 : 0; k <= (active[2] ? 1 : 0); ++k)
+      for (int j = active[1] ? -1 : 0; j <= (active[1] ? 1 : 0); ++j)
+        for (int i = -1; i <= 1; ++i) {
+          Lattice q = c;
+          q.x[0] += i;
+          q.x[1] += j;
+          q.x[2] += k;
+          if (Fold(q) && inner.count(q)) return 0;
+        }
+  }
+  for (const Lattice &c : kids) leaf.erase(c);
+  inner.erase(parent);
+  leaf.insert(parent);
+  return static_cast<int>(kids.size()) - 1;
+}
+
+int Topology::RootLevel() const {
+  std::int64_t div = std::int64_t(1) << 40;
+  for (int d = 0; d < 3; ++d) {
+    if (!active[d]) {
+      div = 1;
+      continue;
+    }
+    std::int64_t p = 1;
+    while (nbase[d] % (2 * p) == 0) p *= 2;
+    div = std::min(div, p);
+  }
+  int lev = 0;
+  while ((std::int64_t(1) << (lev + 1)) <= div) ++lev;
+  return lev;
+}
+
+bool Topology::Nested() const {
+  // every position next to a leaf is a leaf, an inner node whose children touching the leaf
+  // are leaves, or lies inside a leaf one level coarser
+  for (const Lattice &a : leaf)
+    for (int k = active[2] ? -1 : 0; k <= (active[2] ? 1 : 0); ++k)
+      for (int j = active[1] ? -1 : 0; j <= (active[1] ? 1 : 0); ++j)
+        for (int i = -1; i <= 1; ++i) {
+          Lattice seen = a;
+          seen.x[0] += i;
+          seen.x[1] += j;
+          seen.x[2] += k;
+          Lattice q = seen;
+          if (!Fold(q) || leaf.count(q)) continue;
+          if (inner.count(q)) {
+            for (const Lattice &c : Children(seen)) {
+              bool touches = true;
+              for (int d = 0; d < 3; ++d)
+                if (active[d] && (c.x[d] < 2 * a.x[d] - 1 || c.x[d] > 2 * a.x[d] + 2))
+                  touches = false;
+              Lattice cf = c;
+              if (touches && Fold(cf) && !leaf.count(cf)) return false;
+            }
+          } else if (q.level == 0 || !leaf.count(Up(q))) {
+            return false;
+          }
+        }
+  return true;
 }
 
 // Make `a` a block of the mesh by refining its ancestors from the coarsest leaf down.

@@ -81,6 +81,21 @@ def lib():
             "orc_gmg_history": (I, [P, DP, I]),
             "orc_gmg_restrict": (None, [P, I, DP, DP]),
             "orc_gmg_prolong_add": (None, [P, I, DP, DP]),
+            "orc_amr_new": (P, [P, I, I]),
+            "orc_amr_free": (None, [P]),
+            "orc_amr_add_criterion": (None, [P, I, I, D, D, I]),
+            "orc_amr_root_level": (I, [P]),
+            "orc_amr_derivative_max": (None, [P, I, I, I, DP, DP]),
+            "orc_amr_tag": (None, [P, I, DP]),
+            "orc_amr_set_refinement": (None, [P, I, I]),
+            "orc_amr_flags": (None, [P, C.POINTER(I), C.POINTER(I)]),
+            "orc_amr_update": (I, [P, C.POINTER(I), C.POINTER(I)]),
+            "orc_amr_transfer": (None, [P, I, DP, DP]),
+            "orc_amr_commit": (None, [P]),
+            "orc_amr_burgers_cycles": (I, [P, P, I]),
+            "orc_mesh_tree_refine": (I, [P, I, C.POINTER(I64)]),
+            "orc_mesh_tree_derefine": (I, [P, I, C.POINTER(I64)]),
+            "orc_mesh_is_leaf": (I, [P, I, C.POINTER(I64)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -123,8 +138,7 @@ class OracleMesh:
         for center, r, level in spheres:
             L.orc_mesh_refine_sphere(self.h, (D * 3)(*center), r, level)
         L.orc_mesh_finalize(self.h)
-        self.nblocks = L.orc_mesh_nblocks(self.h)
-        self.multilevel = bool(L.orc_mesh_multilevel(self.h))
+        self.refresh()
         s = (I * 12)()
         L.orc_mesh_shapes(self.h, s)
         self.n = tuple(s[0:3])
@@ -138,6 +152,32 @@ class OracleMesh:
         if getattr(self, "h", None) and _LIB is not None:
             _LIB.orc_mesh_free(self.h)
             self.h = None
+
+    def refresh(self):
+        """Re-read the block count after the tree changed (remesh, direct refine/derefine)."""
+        self.nblocks = lib().orc_mesh_nblocks(self.h)
+        self.multilevel = bool(lib().orc_mesh_multilevel(self.h))
+
+    def locs(self):
+        """(level, (l1, l2, l3)) of every block in gid order."""
+        out = []
+        for g in range(self.nblocks):
+            b = self.block(g)
+            out.append((b["level"], b["l"]))
+        return out
+
+    def tree_refine(self, level, l):
+        n = lib().orc_mesh_tree_refine(self.h, level, iarr(l, I64))
+        self.refresh()
+        return n
+
+    def tree_derefine(self, level, l):
+        n = lib().orc_mesh_tree_derefine(self.h, level, iarr(l, I64))
+        self.refresh()
+        return n
+
+    def is_leaf(self, level, l):
+        return bool(lib().orc_mesh_is_leaf(self.h, level, iarr(l, I64)))
 
     def block(self, gid):
         out = (I64 * 4)()
@@ -232,6 +272,72 @@ class OracleBurgers:
         out = np.empty(self.V)
         lib().orc_burgers_totals(self.h, dptr(out))
         return out
+
+
+class OracleAmr:
+    """Adaptive-refinement state of an OracleMesh: criteria, per-block flags, remesh."""
+
+    def __init__(self, mesh, numlevel, deref_threshold=10, criteria=()):
+        self.mesh = mesh
+        self.h = lib().orc_amr_new(mesh.h, numlevel, deref_threshold)
+        for c in criteria:
+            self.add_criterion(**c)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.orc_amr_free(self.h)
+            self.h = None
+
+    def add_criterion(self, method=1, comp=0, refine_tol=0.5, derefine_tol=0.05, max_level=0):
+        lib().orc_amr_add_criterion(self.h, method, comp, refine_tol, derefine_tol, max_level)
+
+    @property
+    def root_level(self):
+        return lib().orc_amr_root_level(self.h)
+
+    def derivative_max(self, data, comp=0, method=1):
+        data = np.ascontiguousarray(data, dtype=np.float64)
+        out = np.empty(self.mesh.nblocks)
+        lib().orc_amr_derivative_max(self.h, method, data.shape[1], comp, dptr(data), dptr(out))
+        return out
+
+    def tag(self, data):
+        data = np.ascontiguousarray(data, dtype=np.float64)
+        lib().orc_amr_tag(self.h, data.shape[1], dptr(data))
+
+    def set_refinement(self, gid, tag):
+        lib().orc_amr_set_refinement(self.h, gid, tag)
+
+    def flags(self):
+        n = self.mesh.nblocks
+        f, c = (I * n)(), (I * n)()
+        lib().orc_amr_flags(self.h, f, c)
+        return np.array(f[:], dtype=np.int32), np.array(c[:], dtype=np.int32)
+
+    def update(self):
+        """UpdateMeshBlockTree: (new block count or 0 if unchanged, nnew, ndel)."""
+        nnew, ndel = I(), I()
+        nb = lib().orc_amr_update(self.h, C.byref(nnew), C.byref(ndel))
+        self.pending = nb
+        return nb, nnew.value, ndel.value
+
+    def transfer(self, data):
+        """Moves one field of the current mesh onto the pending block list."""
+        data = np.ascontiguousarray(data, dtype=np.float64)
+        e = self.mesh.ent
+        out = np.empty((self.pending, data.shape[1], e[2], e[1], e[0]))
+        lib().orc_amr_transfer(self.h, data.shape[1], dptr(data), dptr(out))
+        return out
+
+    def commit(self):
+        lib().orc_amr_commit(self.h)
+        self.mesh.refresh()
+
+    def burgers_cycles(self, burgers, n):
+        """n adaptive Burgers cycles (tag + remesh after each); returns remesh count."""
+        r = lib().orc_amr_burgers_cycles(self.h, burgers.h, n)
+        self.mesh.refresh()
+        return r
 
 
 class OracleGmg:

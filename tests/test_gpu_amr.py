@@ -1,0 +1,210 @@
+"""GPU parity of adaptive mesh refinement through the C ABI against the oracle: the derivative
+criteria of Refinement::Tag (tags bit-exact, probed with tolerances set to the oracle's own
+maxima), the remesh (block list, restriction / copy / prolongation of every container and the
+exchange of "base", bit-exact), and adaptive Burgers runs of both drivers (tag and remesh after
+every cycle; U within 1e-13 relative, block lists, dt and time exact)."""
+import numpy as np
+import pytest
+
+import parthenon_amd as pa
+from parthenon_amd.burgers import BurgersDriver, DeviceBurgersDriver, make_burgers_mesh
+from oracle_lib import OracleAmr, OracleBurgers, OracleMesh, lib
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-13
+FORMS = {"sweep": dict(sweep=True), "overlap": dict(sweep=False, overlap=True),
+         "serial": dict(sweep=False, overlap=False)}
+# 32^3 cells in 8^3 blocks: 4^3 base blocks, root level 2; numlevel 3 lets the base blocks
+# refine once (the criteria compare logical levels)
+NX, BLOCK, G = (32, 32, 32), (8, 8, 8), 4
+BOX = [((-0.5, -0.25, -0.5, -0.25, -0.5, -0.25), 1)]
+
+
+def assert_close(got, exp, what):
+    got, exp = np.asarray(got).ravel(), np.asarray(exp).ravel()
+    assert got.shape == exp.shape, what
+    scale = np.max(np.abs(exp))
+    err = np.max(np.abs(got - exp)) / scale
+    print(f"{what}: max rel err {err:.3e}, bit-exact {np.array_equal(got, exp)}", flush=True)
+    assert err <= RTOL, what
+
+
+def gpu_mesh(criteria=(), numlevel=3, derefine_count=1, boxes=(), ncomp=5, extra=False):
+    m = pa.Mesh(NX, BLOCK, G, refine_boxes=boxes)
+    m.add_field("q", ncomp, pa.Metadata.FillGhost | pa.Metadata.Independent)
+    if extra:
+        m.add_field("w", 2, pa.Metadata.OneCopy)
+    m.set_adaptive(numlevel, derefine_count)
+    for c in criteria:
+        m.add_refinement_criterion("q", **c)
+    m.finalize()
+    return m
+
+
+def seeded(shape, seed):
+    rng = np.random.default_rng(seed)
+    return 1.0 + rng.standard_normal(shape) * 0.3
+
+
+def test_tags_match_oracle_bitwise():
+    # Tolerances at the oracle's exact maxima: a block whose GPU maximum differed by one ulp
+    # would flip between refine, keep and derefine.
+    print("case amr tags", flush=True)
+    om = OracleMesh(NX, BLOCK, G, boxes=BOX)
+    oa = OracleAmr(om, 3, 1)
+    q = seeded(om.field_shape(5), 3)
+    om.exchange_bc([q], coarse=[np.zeros(om.coarse_shape(5))])
+    for method, comp in ((1, 3), (2, 0)):
+        mx = oa.derivative_max(q, comp=comp, method=method)
+        for tol in (np.median(mx), mx[7], mx[om.nblocks - 1]):
+            crit = [dict(comp=comp, method=method, refine_tol=float(tol), derefine_tol=float(tol))]
+            m = gpu_mesh(crit, boxes=BOX)
+            md = m.mesh_data("base")
+            md.upload("q", q)
+            pa.Refinement.Tag(md)
+            oa2 = OracleAmr(om, 3, 1, criteria=[dict(method=method, **{k: v for k, v in
+                                                     crit[0].items() if k != "method"})])
+            oa2.tag(q)
+            f, c = m.refinement_flags()
+            fo, co = oa2.flags()
+            print(f"method {method} tol {tol:.6g}: flags {np.bincount(f + 1, minlength=3)}",
+                  flush=True)
+            assert np.array_equal(f, fo) and np.array_equal(c, co)
+            assert (f == 0).any()   # the block at the tolerance itself keeps its level
+
+
+def test_remesh_moves_every_container_like_oracle():
+    # Refine two base blocks and derefine the refined corner's sibling group in one update:
+    # "base" is moved and exchanged, "1" and the OneCopy field are moved only.
+    print("case amr remesh", flush=True)
+    m = gpu_mesh(boxes=BOX, extra=True)
+    om = OracleMesh(NX, BLOCK, G, boxes=BOX)
+    oa = OracleAmr(om, 3, 1)
+    assert m.locs() == om.locs()
+    base, one = m.mesh_data("base"), m.mesh_data("1")
+    qb, q1, w = seeded(m.field_shape("q"), 5), seeded(m.field_shape("q"), 6), seeded(
+        m.field_shape("w"), 7)
+    base.upload("q", qb)
+    one.upload("q", q1)
+    base.upload("w", w)
+    locs = m.locs()
+    refine = [g for g, (lv, l) in enumerate(locs) if lv == 0 and l in ((3, 3, 3), (2, 3, 3))]
+    group = [g for g, (lv, l) in enumerate(locs) if lv == 1 and max(l) <= 1]
+    assert len(refine) == 2 and len(group) == 8
+    for g in range(m.nblocks):
+        tag = 1 if g in refine else (-1 if g in group else 0)
+        m.set_refinement(g, tag)
+        oa.set_refinement(g, tag)
+    f, c = m.refinement_flags()
+    assert np.array_equal(f, oa.flags()[0]) and np.array_equal(c, oa.flags()[1])
+    assert m.adapt()
+    nb, nnew, ndel = oa.update()
+    eb, e1, ew = oa.transfer(qb), oa.transfer(q1), oa.transfer(w)
+    oa.commit()
+    assert m.nblocks == om.nblocks == nb and m.locs() == om.locs()
+    om.exchange_bc([eb], coarse=[np.zeros(om.coarse_shape(5))])
+    assert np.array_equal(base.download("q"), eb), "base after remesh and exchange"
+    assert np.array_equal(one.download("q"), e1), "stage container moved"
+    assert np.array_equal(m.mesh_data("1").download("w"), ew), "OneCopy field moved"
+    f, c = m.refinement_flags()
+    assert np.array_equal(f, oa.flags()[0]) and np.array_equal(c, oa.flags()[1])
+    # nothing flagged: the next adapt leaves the mesh alone
+    for g in range(m.nblocks):
+        m.set_refinement(g, 0)
+    assert not m.adapt()
+
+
+def test_remesh_round_trip_restores_data():
+    print("case amr round trip", flush=True)
+    m = gpu_mesh()
+    md = m.mesh_data("base")
+    q = seeded(m.field_shape("q"), 9)
+    md.upload("q", q)
+    pa.exchange_fused(md)
+    q = md.download("q")
+    for g in range(m.nblocks):
+        m.set_refinement(g, 1 if g in (5, 6, 40) else 0)
+    assert m.adapt() and m.nblocks > 64
+    for g in range(m.nblocks):
+        m.set_refinement(g, -1 if m.block(g)["level"] == 1 else 0)
+    assert m.adapt() and m.nblocks == 64
+    # interior of a refined-then-derefined block is the restriction of its prolongation:
+    # compare against the oracle doing the same two steps
+    om = OracleMesh(NX, BLOCK, G)
+    oa = OracleAmr(om, 3, 1)
+    x = q.copy()
+    for tags in ("refine", "derefine"):
+        for g in range(om.nblocks):
+            lv = om.block(g)["level"]
+            oa.set_refinement(g, (1 if g in (5, 6, 40) else 0) if tags == "refine"
+                              else (-1 if lv == 1 else 0))
+        oa.update()
+        x = oa.transfer(x)
+        oa.commit()
+        om.exchange_bc([x], coarse=[np.zeros(om.coarse_shape(5))])
+    assert np.array_equal(md.download("q"), x)
+
+
+def test_adapt_errors():
+    m = pa.Mesh(NX, BLOCK, G)
+    m.add_field("q", 1)
+    m.finalize()
+    with pytest.raises(RuntimeError):
+        m.adapt()
+    m2 = gpu_mesh()
+    with pytest.raises(RuntimeError):
+        m2.set_refinement(m2.nblocks, 1)
+    with pytest.raises(RuntimeError):
+        m2.set_adaptive(2, 10)   # after finalize
+
+
+VIBE_CRIT = [dict(field="U", comp=3, method="derivative_order_1", refine_tol=0.5,
+                  derefine_tol=0.2)]
+
+
+def oracle_adaptive(derefine_count):
+    om = OracleMesh(NX, BLOCK, G)
+    oa = OracleAmr(om, 3, derefine_count,
+                   criteria=[dict(method=1, comp=3, refine_tol=0.5, derefine_tol=0.2)])
+    ob = OracleBurgers(om, 4, 0.4, 0)
+    lib().orc_burgers_pgen_sinusoid(ob.h, 4)
+    lib().orc_burgers_initialize(ob.h)
+    return om, oa, ob
+
+
+@pytest.mark.parametrize("form", list(FORMS))
+@pytest.mark.parametrize("use_graph", [False, True], ids=["eager", "graph"])
+def test_adaptive_device_driver_matches_oracle(form, use_graph):
+    print(f"case amr device driver form={form} graph={use_graph}", flush=True)
+    m = make_burgers_mesh(NX, BLOCK, G, 4, adaptive=dict(numlevel=3, derefine_count=2,
+                                                         criteria=VIBE_CRIT))
+    d = DeviceBurgersDriver(m, cfl=0.4, use_graph=use_graph, **FORMS[form])
+    d.initialize("sinusoid", seed=4)
+    om, oa, ob = oracle_adaptive(2)
+    remeshed = 0
+    for c in range(4):
+        r = d.adaptive_cycles(1)
+        ro = oa.burgers_cycles(ob, 1)
+        remeshed += r
+        assert r == ro and m.locs() == om.locs(), f"block list after cycle {c + 1}"
+        assert_close(d.base.download("U"), ob.get(0), f"U after cycle {c + 1}")
+        dt, t = d.clock()
+        assert dt == ob.dt and t == lib().orc_burgers_time(ob.h)
+        assert np.array_equal(m.refinement_flags()[0], oa.flags()[0])
+    assert remeshed >= 1 and m.nblocks > 64
+
+
+def test_adaptive_task_driver_matches_oracle():
+    print("case amr task driver", flush=True)
+    m = make_burgers_mesh(NX, BLOCK, G, 4, adaptive=dict(numlevel=3, derefine_count=2,
+                                                         criteria=VIBE_CRIT))
+    d = BurgersDriver(m, cfl=0.4)
+    d.initialize("sinusoid", seed=4)
+    om, oa, ob = oracle_adaptive(2)
+    for c in range(3):
+        r = d.adaptive_cycle()
+        ro = oa.burgers_cycles(ob, 1)
+        assert int(r) == ro and m.locs() == om.locs(), f"block list after cycle {c + 1}"
+        assert_close(d.base.download("U"), ob.get(0), f"U after cycle {c + 1}")
+        assert d.dt == ob.dt
