@@ -139,6 +139,34 @@ class Mesh {
   }
   void Initialize() { detail::Check(pth_mesh_finalize(h_), "Initialize"); }
 
+  // <parthenon/mesh> refinement = adaptive, numlevel, derefine_count (before Initialize)
+  void SetAdaptive(int numlevel, int derefine_count = 10) {
+    detail::Check(pth_mesh_set_adaptive(h_, numlevel, derefine_count), "SetAdaptive");
+  }
+  // <parthenon/refinementN>: method "derivative_order_1" or "derivative_order_2" on
+  // component vector_i of field (AMRCriteria::MakeAMRCriteria, amr_criteria.cpp:70-80)
+  void AddRefinementCriterion(const std::string &method, const std::string &field, int vector_i,
+                              Real refine_tol = 0.5, Real derefine_tol = 0.05,
+                              int max_level = 0) {
+    const int m = method == "derivative_order_1" ? 1 : method == "derivative_order_2" ? 2 : -1;
+    if (m < 0) throw std::invalid_argument("Invalid selection for refinement method: " + method);
+    detail::Check(pth_mesh_add_refinement_criterion(h_, m, field.c_str(),
+                                                    vector_i, refine_tol, derefine_tol,
+                                                    max_level),
+                  "AddRefinementCriterion");
+  }
+  // Mesh::LoadBalancingAndAdaptiveMeshRefinement (mesh-amr_loadbalance.cpp:324-351); the
+  // result is Mesh::modified. MeshData objects from GetOrAdd stay valid.
+  bool LoadBalancingAndAdaptiveMeshRefinement() {
+    int modified = 0;
+    detail::Check(pth_mesh_adapt(h_, &modified), "LoadBalancingAndAdaptiveMeshRefinement");
+    return modified != 0;
+  }
+  // MeshRefinement::SetRefinement of block gid
+  void SetRefinement(int gid, int tag) {
+    detail::Check(pth_mesh_set_refinement(h_, gid, tag), "SetRefinement");
+  }
+
   int nbtotal() const { return pth_mesh_nblocks(h_); }
   bool multilevel() const { return pth_mesh_multilevel(h_) != 0; }
   pth_mesh *handle() const { return h_; }
@@ -160,6 +188,16 @@ class Mesh {
   pth_mesh *h_ = nullptr;
   std::map<std::string, std::shared_ptr<MeshData<Real>>> data_;
 };
+
+// ---- refinement tagging (src/amr_criteria/refinement_package.hpp) ----
+namespace Refinement {
+template <typename T>
+TaskStatus Tag(T *md);
+template <>
+inline TaskStatus Tag(MeshData<Real> *md) {
+  return detail::Check(pth_refinement_tag(md->handle()), "Refinement::Tag");
+}
+}  // namespace Refinement
 
 // ---- boundary communication tasks (src/bvals/comms/bvals_in_one.hpp:41-67) ----
 template <BoundaryType BT>

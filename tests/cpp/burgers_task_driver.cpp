@@ -3,13 +3,18 @@
 // (benchmarks/burgers/burgers_driver.cpp:53-148) with a sequential task list, the RK2
 // integrator (low_storage_integrator.cpp:67-83) and the driver's dt rule (driver.cpp:210-235).
 //
-// usage: burgers_task_driver <ncycles> <out.bin> [sphere_radius] [open]
+// usage: burgers_task_driver <ncycles> <out.bin> [sphere_radius] [open|periodic] [adaptive]
 // writes the final "base" U (float64, [block][comp][k][j][i]) followed by dt and time. With a
 // sphere radius the mesh is statically refined and the stage adds the flux-correction tasks
 // (burgers_driver.cpp:89-96) and ProlongateBounds after the sets (the library exchange order
 // of AddBoundaryExchangeTasks, boundary_communication.cpp:406-445). "open" makes x1 and x3
 // non-periodic (inner x1 reflecting, the other three faces outflow) and adds the boundary
 // condition tasks (burgers_driver.cpp:130-137; coarse buffers before the prolongation).
+// "adaptive" makes the mesh adaptive (numlevel 3, derefine_count 2, derivative_order_1 on U
+// component 3 with the VIBE tolerances 0.5 / 0.2): Refinement::Tag closes the last stage
+// (burgers_driver.cpp:139-145) and every cycle ends with
+// LoadBalancingAndAdaptiveMeshRefinement and, on a change, FillDerived and the time step of
+// the new blocks (driver.cpp:110-131).
 #include <cstdio>
 #include <cstdlib>
 #include <functional>
@@ -48,12 +53,14 @@ struct Integrator {
 
 int main(int argc, char **argv) {
   if (argc < 3) {
-    std::fprintf(stderr, "usage: %s ncycles out.bin [sphere_radius] [open]\n", argv[0]);
+    std::fprintf(stderr, "usage: %s ncycles out.bin [sphere_radius] [open|periodic] [adaptive]\n",
+                 argv[0]);
     return 2;
   }
   const int ncycles = std::atoi(argv[1]);
   const Real radius = argc > 3 ? std::atof(argv[3]) : 0.0;
   const bool open = argc > 4 && std::string(argv[4]) == "open";
+  const bool adaptive = argc > 5 && std::string(argv[5]) == "adaptive";
   const Real cfl = 0.4;
   try {
     // burgers_package::Initialize: U (3 + 4 scalars) and the OneCopy derived field; with a
@@ -64,6 +71,10 @@ int main(int argc, char **argv) {
     if (radius > 0.0) mesh.AddRefinementSphere({0.0, 0.0, 0.0}, radius, 2);
     mesh.AddField("U", 7, Metadata::FillGhost | Metadata::Independent | Metadata::WithFluxes);
     mesh.AddField("derived", 1, Metadata::OneCopy);
+    if (adaptive) {
+      mesh.SetAdaptive(3, 2);
+      mesh.AddRefinementCriterion("derivative_order_1", "U", 3, 0.5, 0.2);
+    }
     mesh.Initialize();
     auto &base = mesh.GetOrAdd("base", 0);
     if (pth_burgers_pgen_sinusoid(base->handle(), 0) != PTH_COMPLETE)
@@ -127,9 +138,17 @@ int main(int argc, char **argv) {
             new_dt = burgers_package::EstimateTimestepMesh(mc1.get(), cfl);
             return TaskStatus::complete;
           });
+        if (stage == integ.nstages && adaptive)
+          tl.AddTask([&] { return Refinement::Tag(mc1.get()); });
         tl.Execute();
       }
       time += integ.dt;
+      if (adaptive && mesh.LoadBalancingAndAdaptiveMeshRefinement()) {
+        // InitializeBlockTimeSteps on the new blocks (FillDerived ran in the remesh)
+        auto &mb = mesh.GetOrAdd("base", 0);
+        burgers_package::CalculateDerived(mb.get());
+        new_dt = burgers_package::EstimateTimestepMesh(mb.get(), cfl);
+      }
       // SetGlobalTimeStep: growth limited to 2x
       integ.dt = std::min(2.0 * integ.dt, new_dt);
     }

@@ -208,3 +208,27 @@ def test_adaptive_task_driver_matches_oracle():
         assert int(r) == ro and m.locs() == om.locs(), f"block list after cycle {c + 1}"
         assert_close(d.base.download("U"), ob.get(0), f"U after cycle {c + 1}")
         assert d.dt == ob.dt
+
+
+def test_cpp_task_driver_adaptive_matches_oracle(tmp_path):
+    # the reference's task order with Refinement::Tag and the remesh, in C++ against
+    # include/parthenon_amd.hpp (64^3 in 16^3 blocks: root level 2, numlevel 3)
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "burgers_task_driver")
+    out = str(tmp_path / "u.bin")
+    print("case cpp task driver adaptive 3 cycles", flush=True)
+    r = subprocess.run([exe, "3", out, "0", "periodic", "adaptive"], stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = np.fromfile(out, dtype=np.float64)
+    U, dt, t = raw[:-2], raw[-2], raw[-1]
+    om = OracleMesh((64, 64, 64), (16, 16, 16), 4)
+    oa = OracleAmr(om, 3, 2, criteria=[dict(method=1, comp=3, refine_tol=0.5, derefine_tol=0.2)])
+    ob = OracleBurgers(om, 4, 0.4, 0)
+    lib().orc_burgers_pgen_sinusoid(ob.h, 0)
+    lib().orc_burgers_initialize(ob.h)
+    assert oa.burgers_cycles(ob, 3) >= 1
+    assert U.size == ob.get(0).size and om.nblocks > 64
+    assert_close(U, ob.get(0), "C++ adaptive driver U after 3 cycles")
+    assert dt == ob.dt and t == lib().orc_burgers_time(ob.h)
