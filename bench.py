@@ -332,6 +332,92 @@ def gmg_serve_bench(args):
     print(json.dumps(out))
 
 
+def remesh_copy_elements(old, new, n, g, comps):
+    """Elements the remesh box-copy launch moves: a moved block's whole brick, a refined
+    block's coarse-shaped octant of its old parent, a derefined block's 8 restricted octants;
+    comps lists the component counts of every moved (container, field) array."""
+    E, C, I = (n + 2 * g) ** 3, (n // 2 + 2 * g) ** 3, n ** 3
+    olds = set(old)
+    per = 0
+    for lv, l in new:
+        if (lv, l) in olds:
+            per += E
+        elif lv > 0 and (lv - 1, tuple(x // 2 for x in l)) in olds:
+            per += C
+        else:
+            per += I
+    return per * sum(comps)
+
+
+def adaptive_bench(args, quiet=False):
+    """Adaptive Burgers (EvolutionDriver::Execute with refinement = adaptive): the VIBE input's
+    criterion (derivative_order_1 on U component 3, refine 0.5 / derefine 0.2, derefine_count
+    10) on a 64^3 mesh of 16^3 blocks with numlevel 3. The first cycle's remesh refines every
+    block (64 -> 512); its parts are timed separately, then adaptive cycles (cycle + tag +
+    remesh check) are compared with plain cycles on the refined mesh."""
+    import parthenon_amd as pa
+    from parthenon_amd.burgers import DeviceBurgersDriver, make_burgers_mesh
+    nx, block, g, ns = (64, 64, 64), (16, 16, 16), 4, 4
+    crit = [dict(field="U", comp=3, method="derivative_order_1", refine_tol=0.5,
+                 derefine_tol=0.2)]
+    mesh = make_burgers_mesh(nx, block, g, ns,
+                             adaptive=dict(numlevel=3, derefine_count=10, criteria=crit))
+    drv = DeviceBurgersDriver(mesh, cfl=0.4, use_graph=True)
+    drv.initialize("sinusoid", seed=2)
+    base = mesh.mesh_data("base")
+    drv.cycles(1)
+    drv.synchronize()
+    old = mesh.locs()
+    pa.profile_enable(True)
+    pa.profile_report()
+    t0 = time.perf_counter()
+    pa.Refinement.Tag(base)
+    t1 = time.perf_counter()
+    changed = mesh.adapt()
+    t2 = time.perf_counter()
+    pa.lib().pth_burgers_after_remesh(mesh._h)
+    t3 = time.perf_counter()
+    prof = pa.profile_report()
+    pa.profile_enable(False)
+    new = mesh.locs()
+    V = 3 + ns
+    copied = remesh_copy_elements(old, new, block[0], g, [V, V, 1])
+    kern = {k: {"launches": c, "avg_us": ms / c * 1e3} for k, (c, ms) in sorted(prof.items())}
+    cp = prof.get("remesh_copy")
+    copy_gbps = 16 * copied / (cp[1] / cp[0] * 1e-3) / 1e9 if cp else None
+    # steady state on the refined mesh: plain cycles, then adaptive cycles (tag + remesh check)
+    k = max(4, min(args.steps, 40))
+    drv.cycles(2)
+    drv.synchronize()
+    ta = time.perf_counter()
+    drv.cycles(k)
+    drv.synchronize()
+    plain = time.perf_counter() - ta
+    ta = time.perf_counter()
+    remeshes = drv.adaptive_cycles(k)
+    drv.synchronize()
+    adaptive = time.perf_counter() - ta
+    nb = mesh.nblocks
+    zones = nb * block[0] ** 3
+    out = {
+        "workload": "burgers-adaptive 64^3 mesh of 16^3 blocks, numlevel 3, derivative_order_1 "
+                    "on U[3] (refine 0.5, derefine 0.2, derefine_count 10), nghost 4, 3 "
+                    "velocities + 4 scalars, CFL 0.4, periodic",
+        "first_remesh": {"changed": bool(changed), "blocks_before": len(old),
+                         "blocks_after": len(new), "tag_ms": (t1 - t0) * 1e3,
+                         "adapt_ms": (t2 - t1) * 1e3, "after_remesh_ms": (t3 - t2) * 1e3,
+                         "copy_elements": copied, "remesh_copy_GBps": copy_gbps,
+                         "kernels": kern},
+        "steady": {"cycles": k, "blocks": nb, "remeshes": remeshes,
+                   "adaptive_zone_cycles_per_s": zones * k / adaptive,
+                   "plain_zone_cycles_per_s": zones * k / plain,
+                   "tag_and_check_ms_per_cycle": (adaptive - plain) / k * 1e3},
+    }
+    if not quiet:
+        print(json.dumps(out))
+    return out
+
+
 def exchange_bench(args):
     """Ghost-exchange microbenchmark: 128^3 mesh in 512 blocks of 16^3, nghost 3, 8 FP64
     one-component fields, U(-1,1) interiors; one step = SendBoundBufs + ReceiveBoundBufs +
@@ -387,8 +473,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload",
-                    choices=["burgers-amr", "burgers-uniform", "exchange", "copy", "gmg",
-                             "gmg-serve"],
+                    choices=["burgers-amr", "burgers-uniform", "burgers-adaptive", "exchange",
+                             "copy", "gmg", "gmg-serve"],
                     default="burgers-amr")
     ap.add_argument("--pack", choices=["persistent", "per_buffer", "fused"], default="persistent")
     ap.add_argument("--steps", type=int, default=200)
@@ -414,6 +500,8 @@ def main():
         return exchange_bench(args)
     if args.workload == "copy":
         return copy_bench(args)
+    if args.workload == "burgers-adaptive":
+        return adaptive_bench(args)
     if args.workload == "gmg":
         if "--steps" not in sys.argv:
             args.steps = 10
@@ -546,6 +634,8 @@ def main():
             "form": "task-by-task calls in the reference order, one launch per boundary "
                     f"buffer, eager ({pc} cycles after 1 warm-up)"}
         del pd, pmesh
+    if amr and args.driver != "tasks":
+        out["adaptive"] = adaptive_bench(args, quiet=True)
     if not args.no_cpu_baseline:
         threads = min(len(os.sched_getaffinity(0)), 16)
         cw = cpu_baseline(nx, block, g, ns, spheres, seed, cpu_cycles, threads)

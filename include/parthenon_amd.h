@@ -232,16 +232,18 @@ pth_status pth_mesh_set_refinement(pth_mesh *m, int gid, int tag);
 pth_status pth_mesh_refinement_flags(const pth_mesh *m, int *flags, int *counts);
 int pth_mesh_root_level(const pth_mesh *m);
 /* Mesh::LoadBalancingAndAdaptiveMeshRefinement on one GPU (mesh-amr_loadbalance.cpp:324-351):
- * UpdateMeshBlockTree (:496-617) from the flags; if the tree changed, the block list, every
+ * UpdateMeshBlockTree (:496-626) from the flags; if the tree changed, the block list, every
  * MeshData's fields, the region tables and the buffers are rebuilt in place and the data moved
- * (RedistributeAndRefineMeshBlocks, :627-1003): restriction of derefined blocks, copies of
+ * (RedistributeAndRefineMeshBlocks, :663-1003): restriction of derefined blocks, copies of
  * moved blocks and octants, interior prolongation of refined blocks, then the boundary
  * exchange of "base" (CommunicateBoundaries). MeshData handles stay valid; data pointers and
  * block ids change. *modified = 1 when the mesh changed. */
 pth_status pth_mesh_adapt(pth_mesh *m, int *modified);
 /* EvolutionDriver::Execute after a modified mesh (driver.cpp:128-131): FillDerived,
  * InitializeBlockTimeSteps on the new blocks and SetGlobalTimeStep from the dt of the last
- * cycle, on the device clock of pth_burgers_init. */
+ * cycle, on the device clock of pth_burgers_init. When the block count changed by a factor of
+ * two or more since the stage form and tuning were timed, and the caller fixed neither, both
+ * are timed again on the new blocks (the state is restored). */
 pth_status pth_burgers_after_remesh(pth_mesh *m);
 
 /* History: the 8 octant masses sum(U^2 * cell volume / mesh volume) of the package's history

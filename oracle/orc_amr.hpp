@@ -11,8 +11,8 @@
 //   src/interface/state_descriptor.hpp:338-341         package default: derefine
 //   src/mesh/mesh_refinement.cpp:55-58,85-117          deref_count threshold, SetRefinement
 //   src/mesh/mesh-amr_loadbalance.cpp:324-351          LoadBalancingAndAdaptiveMeshRefinement
-//   src/mesh/mesh-amr_loadbalance.cpp:496-617          UpdateMeshBlockTree
-//   src/mesh/mesh-amr_loadbalance.cpp:627-760          new block list, restriction of f2c blocks
+//   src/mesh/mesh-amr_loadbalance.cpp:496-626          UpdateMeshBlockTree
+//   src/mesh/mesh-amr_loadbalance.cpp:663-760          new block list, restriction of f2c blocks
 //   src/mesh/mesh-amr_loadbalance.cpp:83-140,178-240   TryRecvCoarseToFine / TryRecvFineToCoarse
 //   src/mesh/mesh-amr_loadbalance.cpp:864-894,995-1003 interior prolongation, CommunicateBoundaries
 //   src/bvals/comms/bnd_info.cpp:355-385,198-203       interior restrict / prolongate index ranges

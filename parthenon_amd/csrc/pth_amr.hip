@@ -7,7 +7,7 @@
 //   derivative criteria         src/amr_criteria/refinement_package.cpp:51-151,
 //                               src/amr_criteria/amr_criteria.cpp:27-68, 88-106
 //   refinement flags            src/mesh/mesh_refinement.cpp:55-58, 85-117
-//   tree update                 src/mesh/mesh-amr_loadbalance.cpp:496-617,
+//   tree update                 src/mesh/mesh-amr_loadbalance.cpp:496-626,
 //                               src/mesh/forest/tree.cpp:96-137, 228-275
 //   data movement               src/mesh/mesh-amr_loadbalance.cpp:83-140, 178-240, 739-900
 //   interior index ranges       src/bvals/comms/bnd_info.cpp:105-223, 355-385

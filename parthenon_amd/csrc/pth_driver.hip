@@ -507,6 +507,7 @@ pth_status pth_burgers_init(pth_mesh *mh, double cfl, int recon) {
     return PTH_FAIL;
   if (!m->rf_tuned && TuneReconFlux(x, s) != PTH_COMPLETE) return PTH_FAIL;
   if (m->form_auto && !m->form_tuned && TuneForm(x, s) != PTH_COMPLETE) return PTH_FAIL;
+  m->tuned_blocks = static_cast<int>(m->topo.blocks.size());
   PTH_HIP(hipStreamSynchronize(s));
   m->run_ready = true;
   return PTH_COMPLETE;
@@ -540,6 +541,7 @@ pth_status pth_burgers_set_tuning(pth_mesh *mh, const int planes[4]) {
   for (int p = 0; p < 3; ++p) m->rf_ppw[p] = planes[p];
   m->sweep_waves = planes[3];
   m->rf_tuned = true;  // a caller's choice is not re-tuned
+  m->rf_auto = false;
   if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
   if (m->graph) (void)hipGraphDestroy(m->graph);
   m->graph_exec = nullptr;
@@ -612,6 +614,25 @@ pth_status pth_burgers_after_remesh(pth_mesh *mh) {
     return PTH_FAIL;
   if (BurgersClock(m->scratch + m->scratch_n, 1, m->run_cfl, m->dtvar, 2, s) != PTH_COMPLETE)
     return PTH_FAIL;
+  // choices timed on a block count at least twice smaller or larger are timed again (the
+  // tuners restore U, the derived field and the clock)
+  const int nb = static_cast<int>(m->topo.blocks.size());
+  if ((m->rf_auto || m->form_auto) && (nb >= 2 * m->tuned_blocks || 2 * nb <= m->tuned_blocks)) {
+    if (m->rf_auto) m->rf_tuned = false;
+    if (m->form_auto) {
+      m->form_tuned = false;
+      m->run_sweep = 1;  // so that Setup builds the sweep's tables for its trial
+    }
+    Ctx y;
+    if (Setup(m, y) != PTH_COMPLETE) return PTH_FAIL;
+    if (!m->rf_tuned && TuneReconFlux(y, s) != PTH_COMPLETE) return PTH_FAIL;
+    if (m->form_auto && !m->form_tuned && TuneForm(y, s) != PTH_COMPLETE) return PTH_FAIL;
+    m->tuned_blocks = nb;
+    if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
+    if (m->graph) (void)hipGraphDestroy(m->graph);
+    m->graph_exec = nullptr;
+    m->graph = nullptr;
+  }
   PTH_HIP(hipStreamSynchronize(s));
   return PTH_COMPLETE;
 }

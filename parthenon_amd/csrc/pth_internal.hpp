@@ -338,6 +338,8 @@ class Mesh {
   bool form_tuned = false;
   int sweep_waves = 4;                          // driver: sweep register budget (waves/SIMD)
   bool rf_tuned = false;
+  bool rf_auto = true;                          // driver: tuning chosen by timing (not set)
+  int tuned_blocks = 0;                         // driver: block count the choices were timed on
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   double run_cfl = 0.0;
