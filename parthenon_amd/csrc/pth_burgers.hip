@@ -887,13 +887,15 @@ __global__ void __launch_bounds__(64)
 //
 // One thread per (i, j) column of a cubic block of N cells per side, marching k over interior
 // planes. Per plane: the x states of the plane's rows (extended by one cell each way) go to
-// LDS and each thread takes the HLL fluxes of its cell's two x faces; then the same for y;
-// then the z states of the plane above, whose qr closes the high z face (the low z face of
-// plane k is the high z face of plane k - 1, carried in registers). The flux divergence sums
-// x, y, z in the order of FluxDivHelper (update.hpp:43-57), then the RK averaging and update,
-// the derived field and the CFL minimum follow as in FluxUpdateKernel. Every face flux is the
-// same expression as in the other forms (shared faces are evaluated by both neighbouring
-// cells), so the stage output is the same bits, and no flux or face state reaches HBM. On
+// LDS, the HLL flux of every x face of the plane is evaluated once into LDS (each thread the
+// high face of its cell, the first N threads the low face of one row each), and each thread
+// forms its cell's x difference; then the same for y; then the z states of the plane above,
+// whose qr closes the high z face (the low z face of plane k is the high z face of plane
+// k - 1, carried in registers). The flux divergence sums x, y, z in the order of
+// FluxDivHelper (update.hpp:43-57), then the RK averaging and update, the derived field and
+// the CFL minimum follow as in FluxUpdateKernel. Every face flux is the same expression as in
+// the other forms, so the stage output is the same bits, and no flux or face state reaches
+// HBM. On
 // multilevel meshes, faces with a finer neighbour take the restricted fine fluxes from the
 // correction store (fover / fcor as in FusedUpdateKernel).
 // The grid is persistent: workgroup g takes planes [g T / G, (g + 1) T / G) of the
@@ -913,7 +915,9 @@ __global__ void __launch_bounds__(N * N, MINW)
   constexpr int NT = N * N;
   constexpr int R = N + 2;  // states per row: ext cells lo - 1 .. hi + 1
   constexpr int P = R * N;  // states of one plane per component
+  constexpr int FP = (N + 1) * N;  // faces of one plane per component and direction
   __shared__ double sl[V * P], sr[V * P];
+  __shared__ double fx[V * FP];  // x (then y) face fluxes of the plane, each face once
   __shared__ double wmin[(NT + 63) / 64];
   const int tx = threadIdx.x % N, ty = threadIdx.x / N;
   constexpr std::int64_t cs = static_cast<std::int64_t>(E) * E * E;
@@ -972,25 +976,36 @@ __global__ void __launch_bounds__(N * N, MINW)
       }
       __syncthreads();
       {
-        // du = A F(hi) - A F(lo); A F(lo) is formed first so one flux array is live at a time
+        // every x face once: each thread the high face of its cell, threads 0 .. N - 1 also
+        // the low face of row threadIdx.x; fluxes to LDS as [n][row][face]
         double F[V];
         const double *L = sl + ty * R + tx, *Rr = sr + ty * R + tx;
-        const double A = C.area[0];
-        HllGen<V>([&](int n) { return L[n * P]; }, [&](int n) { return Rr[n * P + 1]; }, 0, F);
-        if (tx == 0 && ox0 >= 0)
-#pragma unroll
-          for (int n = 0; n < V; ++n) F[n] = fcor[ox0 + ty + k * N + n * comp];
-#pragma unroll
-        for (int n = 0; n < V; ++n) du[n] = A * F[n];
         HllGen<V>([&](int n) { return L[n * P + 1]; }, [&](int n) { return Rr[n * P + 2]; }, 0,
                   F);
         if (tx == N - 1 && ox1 >= 0)
 #pragma unroll
           for (int n = 0; n < V; ++n) F[n] = fcor[ox1 + ty + k * N + n * comp];
 #pragma unroll
-        for (int n = 0; n < V; ++n) du[n] = (A * F[n] - du[n]);
+        for (int n = 0; n < V; ++n) fx[n * FP + ty * (N + 1) + tx + 1] = F[n];
+        if (static_cast<int>(threadIdx.x) < N) {
+          const int row = threadIdx.x;
+          HllGen<V>([&](int n) { return sl[n * P + row * R]; },
+                    [&](int n) { return sr[n * P + row * R + 1]; }, 0, F);
+          if (ox0 >= 0)
+#pragma unroll
+            for (int n = 0; n < V; ++n) F[n] = fcor[ox0 + row + k * N + n * comp];
+#pragma unroll
+          for (int n = 0; n < V; ++n) fx[n * FP + row * (N + 1)] = F[n];
+        }
       }
-      __syncthreads();  // the x states have been read
+      __syncthreads();  // the x fluxes are in LDS and the x states have been read
+      {
+        // du = A F(hi) - A F(lo)
+        const double A = C.area[0];
+        const double *f = fx + ty * (N + 1) + tx;
+#pragma unroll
+        for (int n = 0; n < V; ++n) du[n] = (A * f[n * FP + 1] - A * f[n * FP]);
+      }
       // ---- y: states of ext cells lo - 1 .. hi + 1 of every column, [n][a][i]
       for (int t = threadIdx.x; t < V * P; t += NT) {
         const int n = t / P, u = t % P;
@@ -998,29 +1013,39 @@ __global__ void __launch_bounds__(N * N, MINW)
         const unsigned c = 8u * static_cast<unsigned>((kk * E + NG - 1 + a) * E + NG + i);
         CellStates(Ub + n * cs, c, sy, recon, sl[t], sr[t]);
       }
-      __syncthreads();
+      __syncthreads();  // also: every thread has read its x fluxes
       {
-        double F[V], alo[V];
+        // every y face once, as [n][face][i]
+        double F[V];
         const double *L = sl + ty * N + tx, *Rr = sr + ty * N + tx;
-        const double A = C.area[1];
-        HllGen<V>([&](int n) { return L[n * P]; }, [&](int n) { return Rr[n * P + N]; }, 1, F);
-        if (ty == 0 && oy0 >= 0)
-#pragma unroll
-          for (int n = 0; n < V; ++n) F[n] = fcor[oy0 + tx + k * N + n * comp];
-#pragma unroll
-        for (int n = 0; n < V; ++n) alo[n] = A * F[n];
         HllGen<V>([&](int n) { return L[n * P + N]; }, [&](int n) { return Rr[n * P + 2 * N]; },
                   1, F);
         if (ty == N - 1 && oy1 >= 0)
 #pragma unroll
           for (int n = 0; n < V; ++n) F[n] = fcor[oy1 + tx + k * N + n * comp];
 #pragma unroll
-        for (int n = 0; n < V; ++n) du[n] = du[n] + (A * F[n] - alo[n]);
+        for (int n = 0; n < V; ++n) fx[n * FP + (ty + 1) * N + tx] = F[n];
+        if (static_cast<int>(threadIdx.x) < N) {
+          const int col = threadIdx.x;
+          HllGen<V>([&](int n) { return sl[n * P + col]; },
+                    [&](int n) { return sr[n * P + N + col]; }, 1, F);
+          if (oy0 >= 0)
+#pragma unroll
+            for (int n = 0; n < V; ++n) F[n] = fcor[oy0 + col + k * N + n * comp];
+#pragma unroll
+          for (int n = 0; n < V; ++n) fx[n * FP + col] = F[n];
+        }
       }
-      __syncthreads();  // the y states have been read
+      __syncthreads();  // the y fluxes are in LDS and the y states have been read
+      {
+        const double A = C.area[1];
+        const double *f = fx + ty * N + tx;
+#pragma unroll
+        for (int n = 0; n < V; ++n) du[n] = du[n] + (A * f[n * FP + N] - A * f[n * FP]);
+      }
       // ---- z: qr of the cell above this plane gives the high z face; its ql moves down
       zstates(kk + 1);
-      __syncthreads();
+      __syncthreads();  // also: every thread has read its y fluxes
       {
         double Fhi[V];
         HllGen<V>([&](int n) { return zl[n]; }, [&](int n) { return sr[n * P + threadIdx.x]; }, 2,
