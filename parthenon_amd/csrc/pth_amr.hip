@@ -411,6 +411,7 @@ pth_status Mesh::Adapt(int *modified) {
     pt.Free();
     return PTH_FAIL;
   };
+  rt.item_elems = pt.item_elems = kProResItemElems;
   if (rt.Upload() != PTH_COMPLETE || ct.Upload() != PTH_COMPLETE || pt.Upload() != PTH_COMPLETE)
     return fail();
   if (LaunchRestrictT(rt, topo, geom, stream) != PTH_COMPLETE) return fail();

@@ -341,7 +341,7 @@ pth_status Table::Upload() {
   Free();
   total = 0;
   items.clear();
-  // every region is cut into items of at most kItemElems elements; one workgroup per item
+  // every region is cut into items of at most item_elems elements; one workgroup per item
   for (size_t q = 0; q < host.size(); ++q) {
     Region &r = host[q];
     r.start = total;
@@ -350,9 +350,10 @@ pth_status Table::Upload() {
       SetError("boundary region too large for the 32-bit element decode");
       return PTH_FAIL;
     }
-    for (std::int64_t b = 0; b < count; b += kItemElems)
+    const int step = std::min(std::max(item_elems, 1), kItemElems);
+    for (std::int64_t b = 0; b < count; b += step)
       items.push_back(Item{static_cast<int>(q), 1, static_cast<int>(b),
-                           static_cast<int>(std::min<std::int64_t>(count - b, kItemElems))});
+                           static_cast<int>(std::min<std::int64_t>(count - b, step))});
     total += count;
   }
   if (host.empty()) return PTH_COMPLETE;

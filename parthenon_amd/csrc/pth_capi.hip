@@ -484,6 +484,7 @@ pth_status Mesh::BuildCache(MeshData &md, int bt) {
     }
     t->arena = arena_base;
   }
+  for (Table *t : {&c.restrict_send, &c.restrict_set, &c.prolong}) t->item_elems = kProResItemElems;
   for (Table *t : {&c.pack, &c.unpack, &c.restrict_send, &c.restrict_set, &c.prolong})
     if (t->Upload() != PTH_COMPLETE) return PTH_FAIL;
   if (BuildBoundaryConditions(md, c) != PTH_COMPLETE) return PTH_FAIL;

@@ -168,6 +168,9 @@ struct Item {  // a run of consecutive elements over up to kItemRegions consecut
 };
 constexpr int kItemRegions = 8;
 constexpr int kItemElems = 4096;
+// restriction / prolongation items: one thread does a few coarse cells (each with 8 dependent
+// fine loads or stores), so these tables are cut finer to keep enough workgroups in flight
+constexpr int kProResItemElems = 512;
 
 struct Table {
   std::vector<Region> host;
@@ -175,6 +178,7 @@ struct Table {
   Region *dev = nullptr;
   Item *dev_items = nullptr;
   std::int64_t total = 0;  // total elements
+  int item_elems = kItemElems;  // elements per work item (at most kItemElems)
   // Gather form of a pack/unpack table whose elements are in arena order: element e moves
   // between arena[e] and bases[idx[e] >> kIdxShift][idx[e] & kIdxMask].
   std::vector<std::uint32_t> idx;
