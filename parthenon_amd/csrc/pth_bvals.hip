@@ -150,6 +150,21 @@ __global__ void __launch_bounds__(kWG)
   }
 }
 
+// Gather words: element e of region q gets words[q] plus its offset from the region's array
+// origin (i fastest, then j, k, component), at position start + e of the table's walk.
+__global__ void __launch_bounds__(kWG)
+    GatherWordsKernel(const Region *__restrict__ reg, const Item *__restrict__ items,
+                      const std::uint32_t *__restrict__ words, std::uint32_t *__restrict__ out) {
+  const Item I = items[blockIdx.x];
+  const Region &R = reg[I.region];
+  const std::uint32_t w = words[I.region];
+  for (int local = static_cast<int>(threadIdx.x); local < I.count; local += kWG) {
+    const int e = I.begin + local;
+    out[R.start + e] =
+        w + static_cast<std::uint32_t>(VarOffset(R, Decode(R, static_cast<std::uint32_t>(e))));
+  }
+}
+
 // The plain form: one launch per region, one thread per element.
 __global__ void __launch_bounds__(kWG) PackOneKernel(Region R, int to_buffer) {
   const std::int64_t e = static_cast<std::int64_t>(blockIdx.x) * kWG + threadIdx.x;
@@ -361,21 +376,54 @@ pth_status Table::Upload() {
   PTH_HIP(hipMalloc(&dev_items, sizeof(Item) * items.size()));
   PTH_HIP(hipMemcpy(dev, host.data(), sizeof(Region) * host.size(), hipMemcpyHostToDevice));
   PTH_HIP(hipMemcpy(dev_items, items.data(), sizeof(Item) * items.size(), hipMemcpyHostToDevice));
-  if (!idx.empty()) {
-    if (static_cast<std::int64_t>(idx.size()) != total || bases.empty() ||
+  if (!words.empty()) {
+    if (words.size() != host.size() || bases.empty() ||
         static_cast<int>(bases.size()) > kIdxSlots) {
       SetError("internal: gather index does not match its table");
       return PTH_FAIL;
     }
-    PTH_HIP(hipMalloc(&dev_idx, sizeof(std::uint32_t) * idx.size()));
+    std::uint32_t *dw = nullptr;
+    PTH_HIP(hipMalloc(&dev_idx, sizeof(std::uint32_t) * total));
     PTH_HIP(hipMalloc(&dev_bases, sizeof(double *) * bases.size()));
-    PTH_HIP(hipMemcpy(dev_idx, idx.data(), sizeof(std::uint32_t) * idx.size(),
-                      hipMemcpyHostToDevice));
+    PTH_HIP(hipMalloc(&dw, sizeof(std::uint32_t) * words.size()));
     PTH_HIP(hipMemcpy(dev_bases, bases.data(), sizeof(double *) * bases.size(),
                       hipMemcpyHostToDevice));
-    idx.clear();
-    idx.shrink_to_fit();
+    PTH_HIP(hipMemcpy(dw, words.data(), sizeof(std::uint32_t) * words.size(),
+                      hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(GatherWordsKernel, dim3(static_cast<unsigned>(items.size())), dim3(kWG), 0,
+                       nullptr, dev, dev_items, dw, dev_idx);
+    const hipError_t launch = hipGetLastError();
+    const hipError_t done = hipStreamSynchronize(nullptr);
+    (void)hipFree(dw);
+    if (launch != hipSuccess || done != hipSuccess) {
+      SetError(std::string("gather index expansion failed: ") +
+               hipGetErrorString(launch != hipSuccess ? launch : done));
+      return PTH_FAIL;
+    }
+    words.clear();
+    words.shrink_to_fit();
   }
+  return PTH_COMPLETE;
+}
+
+pth_status ExpandGatherWords(const std::vector<Region> &regions,
+                             const std::vector<std::uint32_t> &words, std::uint32_t **dev_idx,
+                             std::int64_t *total) {
+  *dev_idx = nullptr;
+  *total = 0;
+  if (regions.empty()) return PTH_COMPLETE;
+  Table t;
+  t.host = regions;
+  t.words = words;
+  t.bases.push_back(nullptr);  // slots are resolved by the caller's own base list
+  if (t.Upload() != PTH_COMPLETE) {
+    t.Free();
+    return PTH_FAIL;
+  }
+  *dev_idx = t.dev_idx;
+  *total = t.total;
+  t.dev_idx = nullptr;
+  t.Free();
   return PTH_COMPLETE;
 }
 

@@ -162,6 +162,19 @@ Region MakeRegion(double *a, double *b, const Box &box, int nc, int gid, const E
 
 bool IsExchangeType(int bt) { return bt == PTH_BT_ANY || bt == PTH_BT_LOCAL || bt == PTH_BT_NONLOCAL; }
 
+// gather word of a region's array origin: the slot of the array holding it, then its offset
+std::uint32_t GatherWord(const std::vector<std::pair<double *, std::int64_t>> &arrays,
+                         const double *origin) {
+  std::uint32_t slot = 0;
+  std::int64_t rel = 0;
+  for (size_t s = 0; s < arrays.size(); ++s)
+    if (origin >= arrays[s].first && origin < arrays[s].first + arrays[s].second) {
+      slot = static_cast<std::uint32_t>(s);
+      rel = origin - arrays[s].first;
+    }
+  return (slot << kIdxShift) | static_cast<std::uint32_t>(rel);
+}
+
 }  // namespace
 
 namespace pth {
@@ -449,7 +462,7 @@ pth_status Mesh::BuildCache(MeshData &md, int bt) {
     if (md.store[f].coarse) arrays.emplace_back(md.store[f].coarse, md.store[f].ncoarse);
   }
   for (Table *t : {&c.pack, &c.unpack}) {
-    t->idx.clear();
+    t->words.clear();
     t->bases.clear();
     t->arena = nullptr;
     bool gather = static_cast<int>(arrays.size()) <= kIdxSlots && arena_base != nullptr;
@@ -462,26 +475,9 @@ pth_status Mesh::BuildCache(MeshData &md, int bt) {
     gather = gather && pos == arena_total;
     if (!gather) continue;  // the table keeps its region walk
     for (const auto &a : arrays) t->bases.push_back(a.first);
-    t->idx.reserve(pos);
-    for (const Region &r : t->host) {
-      std::uint32_t slot = 0;
-      std::int64_t rel = 0;
-      for (size_t s = 0; s < arrays.size(); ++s)
-        if (r.a >= arrays[s].first && r.a < arrays[s].first + arrays[s].second) {
-          slot = static_cast<std::uint32_t>(s);
-          rel = r.a - arrays[s].first;
-        }
-      for (int cc = 0; cc < r.nc; ++cc)
-        for (int k = 0; k < r.n[2]; ++k)
-          for (int j = 0; j < r.n[1]; ++j)
-            for (int i = 0; i < r.n[0]; ++i) {
-              const std::int64_t o = rel + static_cast<std::int64_t>(cc) * r.astride[2] +
-                                     static_cast<std::int64_t>(k + r.lo[2]) * r.astride[1] +
-                                     static_cast<std::int64_t>(j + r.lo[1]) * r.astride[0] +
-                                     (i + r.lo[0]);
-              t->idx.push_back((slot << kIdxShift) | static_cast<std::uint32_t>(o));
-            }
-    }
+    // one word per region; Upload expands it per element on the device
+    t->words.reserve(t->host.size());
+    for (const Region &r : t->host) t->words.push_back(GatherWord(arrays, r.a));
     t->arena = arena_base;
   }
   for (Table *t : {&c.restrict_send, &c.restrict_set, &c.prolong}) t->item_elems = kProResItemElems;
@@ -527,26 +523,8 @@ pth_status Mesh::BuildRestExchange(MeshData &md, int f, bool *ok) {
     if (md.store[q].coarse) arrays.emplace_back(md.store[q].coarse, md.store[q].ncoarse);
   }
   const double *f0 = md.store[f].data, *f1 = f0 + md.store[f].ndata;
-  auto append = [&](const Region &r, std::vector<std::uint32_t> &out) {
-    std::uint32_t slot = 0;
-    std::int64_t rel = 0;
-    for (size_t a = 0; a < arrays.size(); ++a)
-      if (r.a >= arrays[a].first && r.a < arrays[a].first + arrays[a].second) {
-        slot = static_cast<std::uint32_t>(a);
-        rel = r.a - arrays[a].first;
-      }
-    for (int cc = 0; cc < r.nc; ++cc)
-      for (int k = 0; k < r.n[2]; ++k)
-        for (int j = 0; j < r.n[1]; ++j)
-          for (int i = 0; i < r.n[0]; ++i) {
-            const std::int64_t o = rel + static_cast<std::int64_t>(cc) * r.astride[2] +
-                                   static_cast<std::int64_t>(k + r.lo[2]) * r.astride[1] +
-                                   static_cast<std::int64_t>(j + r.lo[1]) * r.astride[0] +
-                                   (i + r.lo[0]);
-            out.push_back((slot << kIdxShift) | static_cast<std::uint32_t>(o));
-          }
-  };
-  std::vector<std::uint32_t> si, ri;
+  std::vector<Region> sregs, rregs;
+  std::vector<std::uint32_t> sw, rw;
   for (size_t q = 0; q < c.pack.host.size(); ++q) {
     const Region &sr = c.pack.host[q], &rr = c.unpack.host[q];
     if (sr.b != rr.b) {
@@ -555,25 +533,26 @@ pth_status Mesh::BuildRestExchange(MeshData &md, int f, bool *ok) {
     }
     const bool same = sr.a >= f0 && sr.a < f1 && rr.a >= f0 && rr.a < f1;
     if (same) continue;
-    append(sr, si);
-    append(rr, ri);
-  }
-  if (si.size() != ri.size()) {
-    SetError("internal: send and receive element counts differ");
-    return PTH_FAIL;
+    sregs.push_back(sr);
+    sw.push_back(GatherWord(arrays, sr.a));
+    rregs.push_back(rr);
+    rw.push_back(GatherWord(arrays, rr.a));
   }
   for (Table *t : {&c.pack_rest, &c.unpack_rest}) {
-    const std::vector<std::uint32_t> &v = t == &c.pack_rest ? si : ri;
-    const std::vector<double *> &bases = t == &c.pack_rest ? c.pack.bases : c.unpack.bases;
+    const bool send = t == &c.pack_rest;
+    const std::vector<double *> &bases = send ? c.pack.bases : c.unpack.bases;
     t->bases = bases;
-    t->total = static_cast<std::int64_t>(v.size());
-    if (v.empty()) continue;
-    PTH_HIP(hipMalloc(&t->dev_idx, sizeof(std::uint32_t) * v.size()));
+    if (ExpandGatherWords(send ? sregs : rregs, send ? sw : rw, &t->dev_idx, &t->total) !=
+        PTH_COMPLETE)
+      return PTH_FAIL;
+    if (t->total == 0) continue;
     PTH_HIP(hipMalloc(&t->dev_bases, sizeof(double *) * bases.size()));
-    PTH_HIP(hipMemcpy(t->dev_idx, v.data(), sizeof(std::uint32_t) * v.size(),
-                      hipMemcpyHostToDevice));
     PTH_HIP(hipMemcpy(t->dev_bases, bases.data(), sizeof(double *) * bases.size(),
                       hipMemcpyHostToDevice));
+  }
+  if (c.pack_rest.total != c.unpack_rest.total) {
+    SetError("internal: send and receive element counts differ");
+    return PTH_FAIL;
   }
   c.rest_field = f;
   *ok = true;
