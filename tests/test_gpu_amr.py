@@ -232,3 +232,63 @@ def test_cpp_task_driver_adaptive_matches_oracle(tmp_path):
     assert U.size == ob.get(0).size and om.nblocks > 64
     assert_close(U, ob.get(0), "C++ adaptive driver U after 3 cycles")
     assert dt == ob.dt and t == lib().orc_burgers_time(ob.h)
+
+
+def test_remesh_2d_matches_oracle():
+    # a 2-D mesh (nleaf 4): refine two blocks, then derefine one complete sibling group
+    print("case amr 2-D remesh", flush=True)
+    nx, block = (64, 64, 1), (16, 16, 1)
+    m = pa.Mesh(nx, block, 2, periodic=(True, True, False))
+    m.add_field("q", 3, pa.Metadata.FillGhost | pa.Metadata.Independent)
+    m.set_adaptive(3, 1)
+    m.finalize()
+    om = OracleMesh(nx, block, 2, periodic=(1, 1, 0))
+    oa = OracleAmr(om, 3, 1)
+    md = m.mesh_data("base")
+    q = seeded(m.field_shape("q"), 21)
+    md.upload("q", q)
+    pa.exchange_fused(md)
+    x = md.download("q")
+    for step in ("refine", "derefine"):
+        locs = m.locs()
+        assert locs == om.locs()
+        for g, (lv, l) in enumerate(locs):
+            if step == "refine":
+                tag = 1 if l in ((1, 1, 0), (2, 2, 0)) else 0
+            else:
+                tag = -1 if lv == 1 and l[0] // 2 == 1 and l[1] // 2 == 1 else 0
+            m.set_refinement(g, tag)
+            oa.set_refinement(g, tag)
+        assert m.adapt()
+        nb, nnew, ndel = oa.update()
+        x = oa.transfer(x)
+        oa.commit()
+        om.exchange_bc([x], coarse=[np.zeros(om.coarse_shape(3))])
+        assert m.locs() == om.locs() and m.nblocks == nb
+        assert np.array_equal(md.download("q"), x), f"2-D data after {step}"
+    assert m.nblocks == 16 + 3
+
+
+def test_adaptive_open_boundaries_match_oracle():
+    # reflecting and outflow faces: the remesh's exchange applies the conditions (coarse
+    # buffers before the prolongation), every cycle's state against the oracle
+    print("case amr open boundaries", flush=True)
+    bcs = ("reflect", "outflow", "periodic", "periodic", "outflow", "reflect")
+    m = make_burgers_mesh(NX, BLOCK, G, 4, bcs=bcs,
+                          adaptive=dict(numlevel=3, derefine_count=2, criteria=VIBE_CRIT))
+    d = DeviceBurgersDriver(m, cfl=0.4, use_graph=True, sweep=True)
+    d.initialize("sinusoid", seed=4)
+    om = OracleMesh(NX, BLOCK, G, periodic=(0, 1, 0), bcs=bcs)
+    oa = OracleAmr(om, 3, 2, criteria=[dict(method=1, comp=3, refine_tol=0.5, derefine_tol=0.2)])
+    ob = OracleBurgers(om, 4, 0.4, 0)
+    lib().orc_burgers_pgen_sinusoid(ob.h, 4)
+    lib().orc_burgers_initialize(ob.h)
+    remeshed = 0
+    for c in range(3):
+        r = d.adaptive_cycles(1)
+        assert r == oa.burgers_cycles(ob, 1) and m.locs() == om.locs(), f"cycle {c + 1}"
+        remeshed += r
+        assert_close(d.base.download("U"), ob.get(0), f"U after cycle {c + 1}")
+        dt, t = d.clock()
+        assert dt == ob.dt and t == lib().orc_burgers_time(ob.h)
+    assert remeshed >= 1
