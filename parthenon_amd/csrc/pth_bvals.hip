@@ -79,51 +79,6 @@ __global__ void __launch_bounds__(kWG)
   }
 }
 
-// Gather form: element e of the arena-ordered table moves between arena[e] and
-// bases[idx >> 27][idx & mask]. The table is walked with a persistent grid-stride loop; the
-// array base pointers sit in LDS.
-template <bool TO_BUFFER>
-__global__ void __launch_bounds__(kWG)
-    PackIdxKernel(double *const *__restrict__ bases, int nbases,
-                  const std::uint32_t *__restrict__ idx, double *__restrict__ arena,
-                  std::int64_t total) {
-  __shared__ double *sb[kIdxSlots];
-  if (static_cast<int>(threadIdx.x) < nbases) sb[threadIdx.x] = bases[threadIdx.x];
-  __syncthreads();
-  for (std::int64_t e = static_cast<std::int64_t>(blockIdx.x) * kWG + threadIdx.x; e < total;
-       e += static_cast<std::int64_t>(gridDim.x) * kWG) {
-    const std::uint32_t u = idx[e];
-    double *p = sb[u >> kIdxShift] + (u & kIdxMask);
-    if (TO_BUFFER)
-      arena[e] = *p;
-    else
-      *p = arena[e];
-  }
-}
-
-// Fused exchange on one GPU: element e of the arena goes straight from its sending cell to its
-// receiving cell. Send regions read only interior cells (of the field or of its coarse buffer)
-// and receive regions write only ghost cells, so no element is both read and written.
-__global__ void __launch_bounds__(kWG)
-    ExchangeIdxKernel(double *const *__restrict__ sbases, const std::uint32_t *__restrict__ sidx,
-                      double *const *__restrict__ rbases, const std::uint32_t *__restrict__ ridx,
-                      int nbases, std::int64_t total) {
-  __shared__ double *ss[kIdxSlots];
-  __shared__ double *rs[kIdxSlots];
-  if (static_cast<int>(threadIdx.x) < nbases) {
-    ss[threadIdx.x] = sbases[threadIdx.x];
-    rs[threadIdx.x] = rbases[threadIdx.x];
-  }
-  __syncthreads();
-  // grid-stride: the resident workgroups sweep the arena together (measured faster than one
-  // contiguous chunk per workgroup, with or without an XCD-aware order)
-  for (std::int64_t e = static_cast<std::int64_t>(blockIdx.x) * kWG + threadIdx.x; e < total;
-       e += static_cast<std::int64_t>(gridDim.x) * kWG) {
-    const std::uint32_t us = sidx[e], ur = ridx[e];
-    rs[ur >> kIdxShift][ur & kIdxMask] = ss[us >> kIdxShift][us & kIdxMask];
-  }
-}
-
 // Box copies (remesh): region element e moves from its box in a to the same box position in
 // b, which points at the destination box origin. Loads before stores, like the pack.
 __global__ void __launch_bounds__(kWG)
@@ -150,18 +105,31 @@ __global__ void __launch_bounds__(kWG)
   }
 }
 
-// Gather words: element e of region q gets words[q] plus its offset from the region's array
-// origin (i fastest, then j, k, component), at position start + e of the table's walk.
+// Fused exchange by region pairs: send region q and receive region q cover the same slice of
+// the channel arena in the same element order, so element e of the pair moves from its cell
+// in the send box to its cell in the receive box. Both records are uniform per work item; no
+// per-element index is read. Loads before stores, like the pack.
 __global__ void __launch_bounds__(kWG)
-    GatherWordsKernel(const Region *__restrict__ reg, const Item *__restrict__ items,
-                      const std::uint32_t *__restrict__ words, std::uint32_t *__restrict__ out) {
+    ExchangeRegionKernel(const Region *__restrict__ sreg, const Region *__restrict__ rreg,
+                         const Item *__restrict__ items) {
   const Item I = items[blockIdx.x];
-  const Region &R = reg[I.region];
-  const std::uint32_t w = words[I.region];
-  for (int local = static_cast<int>(threadIdx.x); local < I.count; local += kWG) {
-    const int e = I.begin + local;
-    out[R.start + e] =
-        w + static_cast<std::uint32_t>(VarOffset(R, Decode(R, static_cast<std::uint32_t>(e))));
+  const Region &S = sreg[I.region];
+  const Region &D = rreg[I.region];
+  double v[kPer];
+  std::int64_t off[kPer];
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int local = u * kWG + static_cast<int>(threadIdx.x);
+    if (local < I.count) {
+      const std::uint32_t e = static_cast<std::uint32_t>(I.begin + local);
+      v[u] = S.a[VarOffset(S, Decode(S, e))];
+      off[u] = VarOffset(D, Decode(D, e));
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int local = u * kWG + static_cast<int>(threadIdx.x);
+    if (local < I.count) D.a[off[u]] = v[u];
   }
 }
 
@@ -343,12 +311,8 @@ std::uint64_t Magic(int d) {
 void Table::Free() {
   if (dev) (void)hipFree(dev);
   if (dev_items) (void)hipFree(dev_items);
-  if (dev_idx) (void)hipFree(dev_idx);
-  if (dev_bases) (void)hipFree(dev_bases);
   dev = nullptr;
   dev_items = nullptr;
-  dev_idx = nullptr;
-  dev_bases = nullptr;
 }
 
 // Assigns region start offsets, cuts regions into work items and copies both to the device.
@@ -376,54 +340,6 @@ pth_status Table::Upload() {
   PTH_HIP(hipMalloc(&dev_items, sizeof(Item) * items.size()));
   PTH_HIP(hipMemcpy(dev, host.data(), sizeof(Region) * host.size(), hipMemcpyHostToDevice));
   PTH_HIP(hipMemcpy(dev_items, items.data(), sizeof(Item) * items.size(), hipMemcpyHostToDevice));
-  if (!words.empty()) {
-    if (words.size() != host.size() || bases.empty() ||
-        static_cast<int>(bases.size()) > kIdxSlots) {
-      SetError("internal: gather index does not match its table");
-      return PTH_FAIL;
-    }
-    std::uint32_t *dw = nullptr;
-    PTH_HIP(hipMalloc(&dev_idx, sizeof(std::uint32_t) * total));
-    PTH_HIP(hipMalloc(&dev_bases, sizeof(double *) * bases.size()));
-    PTH_HIP(hipMalloc(&dw, sizeof(std::uint32_t) * words.size()));
-    PTH_HIP(hipMemcpy(dev_bases, bases.data(), sizeof(double *) * bases.size(),
-                      hipMemcpyHostToDevice));
-    PTH_HIP(hipMemcpy(dw, words.data(), sizeof(std::uint32_t) * words.size(),
-                      hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(GatherWordsKernel, dim3(static_cast<unsigned>(items.size())), dim3(kWG), 0,
-                       nullptr, dev, dev_items, dw, dev_idx);
-    const hipError_t launch = hipGetLastError();
-    const hipError_t done = hipStreamSynchronize(nullptr);
-    (void)hipFree(dw);
-    if (launch != hipSuccess || done != hipSuccess) {
-      SetError(std::string("gather index expansion failed: ") +
-               hipGetErrorString(launch != hipSuccess ? launch : done));
-      return PTH_FAIL;
-    }
-    words.clear();
-    words.shrink_to_fit();
-  }
-  return PTH_COMPLETE;
-}
-
-pth_status ExpandGatherWords(const std::vector<Region> &regions,
-                             const std::vector<std::uint32_t> &words, std::uint32_t **dev_idx,
-                             std::int64_t *total) {
-  *dev_idx = nullptr;
-  *total = 0;
-  if (regions.empty()) return PTH_COMPLETE;
-  Table t;
-  t.host = regions;
-  t.words = words;
-  t.bases.push_back(nullptr);  // slots are resolved by the caller's own base list
-  if (t.Upload() != PTH_COMPLETE) {
-    t.Free();
-    return PTH_FAIL;
-  }
-  *dev_idx = t.dev_idx;
-  *total = t.total;
-  t.dev_idx = nullptr;
-  t.Free();
   return PTH_COMPLETE;
 }
 
@@ -436,16 +352,6 @@ pth_status LaunchPack(const Table &t, double *, hipStream_t s, bool to_buffer, i
       hipLaunchKernelGGL(PackOneKernel, dim3(static_cast<unsigned>((count + kWG - 1) / kWG)),
                          dim3(kWG), 0, s, R, to_buffer ? 1 : 0);
     }
-  } else if (t.dev_idx) {
-    const std::int64_t want = (t.total + kWG - 1) / kWG;
-    const int grid = static_cast<int>(want < 2048 ? want : 2048);
-    const int nb = static_cast<int>(t.bases.size());
-    if (to_buffer)
-      hipLaunchKernelGGL(PackIdxKernel<true>, dim3(grid), dim3(kWG), 0, s, t.dev_bases, nb,
-                         t.dev_idx, t.arena, t.total);
-    else
-      hipLaunchKernelGGL(PackIdxKernel<false>, dim3(grid), dim3(kWG), 0, s, t.dev_bases, nb,
-                         t.dev_idx, t.arena, t.total);
   } else {
     const int n = static_cast<int>(t.items.size());
     if (to_buffer)
@@ -467,22 +373,19 @@ pth_status LaunchCopy(const Table &t, hipStream_t s) {
 }
 
 bool CanExchangeDirect(const Table &pack, const Table &unpack) {
-  return pack.dev_idx && unpack.dev_idx && pack.total == unpack.total &&
-         pack.bases.size() == unpack.bases.size();
+  return pack.dev && unpack.dev && pack.host.size() == unpack.host.size() &&
+         pack.total == unpack.total;
 }
 
 pth_status LaunchExchangeDirect(const Table &pack, const Table &unpack, hipStream_t s) {
   if (pack.total == 0) return PTH_COMPLETE;
   if (!CanExchangeDirect(pack, unpack)) {
-    SetError("internal: direct exchange needs gather tables on both sides");
+    SetError("internal: direct exchange needs paired send and receive regions");
     return PTH_FAIL;
   }
   ProfScope prof("exchange_direct", s);
-  const std::int64_t want = (pack.total + kWG - 1) / kWG;
-  const int grid = static_cast<int>(want < 2048 ? want : 2048);
-  hipLaunchKernelGGL(ExchangeIdxKernel, dim3(grid), dim3(kWG), 0, s, pack.dev_bases, pack.dev_idx,
-                     unpack.dev_bases, unpack.dev_idx, static_cast<int>(pack.bases.size()),
-                     pack.total);
+  hipLaunchKernelGGL(ExchangeRegionKernel, dim3(GridFor(static_cast<int>(pack.items.size()))),
+                     dim3(kWG), 0, s, pack.dev, unpack.dev, pack.dev_items);
   PTH_HIP(hipGetLastError());
   return PTH_COMPLETE;
 }

@@ -162,19 +162,6 @@ Region MakeRegion(double *a, double *b, const Box &box, int nc, int gid, const E
 
 bool IsExchangeType(int bt) { return bt == PTH_BT_ANY || bt == PTH_BT_LOCAL || bt == PTH_BT_NONLOCAL; }
 
-// gather word of a region's array origin: the slot of the array holding it, then its offset
-std::uint32_t GatherWord(const std::vector<std::pair<double *, std::int64_t>> &arrays,
-                         const double *origin) {
-  std::uint32_t slot = 0;
-  std::int64_t rel = 0;
-  for (size_t s = 0; s < arrays.size(); ++s)
-    if (origin >= arrays[s].first && origin < arrays[s].first + arrays[s].second) {
-      slot = static_cast<std::uint32_t>(s);
-      rel = origin - arrays[s].first;
-    }
-  return (slot << kIdxShift) | static_cast<std::uint32_t>(rel);
-}
-
 }  // namespace
 
 namespace pth {
@@ -450,36 +437,11 @@ pth_status Mesh::BuildCache(MeshData &md, int bt) {
     return PTH_FAIL;
   }
   // The order of regions within a pack or an unpack is free (their writes are disjoint), so
-  // order both like the arena; then both tables walk the arena front to back and get a
-  // per-element gather index.
+  // order both like the arena; then region q of the pack and of the unpack cover the same
+  // arena slice, which the direct exchange walks pair by pair.
   for (Table *t : {&c.pack, &c.unpack})
     std::sort(t->host.begin(), t->host.end(),
               [](const Region &x, const Region &y) { return x.b < y.b; });
-  std::vector<std::pair<double *, std::int64_t>> arrays;  // candidate gather bases
-  for (size_t f = 0; f < fields.size(); ++f) {
-    if (!(fields[f].flags & PTH_FILL_GHOST)) continue;
-    arrays.emplace_back(md.store[f].data, md.store[f].ndata);
-    if (md.store[f].coarse) arrays.emplace_back(md.store[f].coarse, md.store[f].ncoarse);
-  }
-  for (Table *t : {&c.pack, &c.unpack}) {
-    t->words.clear();
-    t->bases.clear();
-    t->arena = nullptr;
-    bool gather = static_cast<int>(arrays.size()) <= kIdxSlots && arena_base != nullptr;
-    for (const auto &a : arrays) gather = gather && a.second <= static_cast<std::int64_t>(kIdxMask);
-    std::int64_t pos = 0;
-    for (const Region &r : t->host) {
-      gather = gather && r.b == arena_base + pos;
-      pos += static_cast<std::int64_t>(r.n[0]) * r.n[1] * r.n[2] * r.nc;
-    }
-    gather = gather && pos == arena_total;
-    if (!gather) continue;  // the table keeps its region walk
-    for (const auto &a : arrays) t->bases.push_back(a.first);
-    // one word per region; Upload expands it per element on the device
-    t->words.reserve(t->host.size());
-    for (const Region &r : t->host) t->words.push_back(GatherWord(arrays, r.a));
-    t->arena = arena_base;
-  }
   for (Table *t : {&c.restrict_send, &c.restrict_set, &c.prolong}) t->item_elems = kProResItemElems;
   for (Table *t : {&c.pack, &c.unpack, &c.restrict_send, &c.restrict_set, &c.prolong})
     if (t->Upload() != PTH_COMPLETE) return PTH_FAIL;
@@ -516,15 +478,8 @@ pth_status Mesh::BuildRestExchange(MeshData &md, int f, bool *ok) {
   }
   c.pack_rest.Free();
   c.unpack_rest.Free();
-  std::vector<std::pair<double *, std::int64_t>> arrays;  // the gather slots of BuildCache
-  for (size_t q = 0; q < fields.size(); ++q) {
-    if (!(fields[q].flags & PTH_FILL_GHOST)) continue;
-    arrays.emplace_back(md.store[q].data, md.store[q].ndata);
-    if (md.store[q].coarse) arrays.emplace_back(md.store[q].coarse, md.store[q].ncoarse);
-  }
   const double *f0 = md.store[f].data, *f1 = f0 + md.store[f].ndata;
   std::vector<Region> sregs, rregs;
-  std::vector<std::uint32_t> sw, rw;
   for (size_t q = 0; q < c.pack.host.size(); ++q) {
     const Region &sr = c.pack.host[q], &rr = c.unpack.host[q];
     if (sr.b != rr.b) {
@@ -534,21 +489,11 @@ pth_status Mesh::BuildRestExchange(MeshData &md, int f, bool *ok) {
     const bool same = sr.a >= f0 && sr.a < f1 && rr.a >= f0 && rr.a < f1;
     if (same) continue;
     sregs.push_back(sr);
-    sw.push_back(GatherWord(arrays, sr.a));
     rregs.push_back(rr);
-    rw.push_back(GatherWord(arrays, rr.a));
   }
   for (Table *t : {&c.pack_rest, &c.unpack_rest}) {
-    const bool send = t == &c.pack_rest;
-    const std::vector<double *> &bases = send ? c.pack.bases : c.unpack.bases;
-    t->bases = bases;
-    if (ExpandGatherWords(send ? sregs : rregs, send ? sw : rw, &t->dev_idx, &t->total) !=
-        PTH_COMPLETE)
-      return PTH_FAIL;
-    if (t->total == 0) continue;
-    PTH_HIP(hipMalloc(&t->dev_bases, sizeof(double *) * bases.size()));
-    PTH_HIP(hipMemcpy(t->dev_bases, bases.data(), sizeof(double *) * bases.size(),
-                      hipMemcpyHostToDevice));
+    t->host = t == &c.pack_rest ? sregs : rregs;
+    if (t->Upload() != PTH_COMPLETE) return PTH_FAIL;
   }
   if (c.pack_rest.total != c.unpack_rest.total) {
     SetError("internal: send and receive element counts differ");

@@ -179,21 +179,9 @@ struct Table {
   Item *dev_items = nullptr;
   std::int64_t total = 0;  // total elements
   int item_elems = kItemElems;  // elements per work item (at most kItemElems)
-  // Gather form of a pack/unpack table whose elements are in arena order: element e moves
-  // between arena[e] and bases[idx[e] >> kIdxShift][idx[e] & kIdxMask]. The host gives one
-  // word per region (slot << kIdxShift | offset of the region's array origin in its slot);
-  // Upload expands it on the device into dev_idx, one word per element.
-  std::vector<std::uint32_t> words;
-  std::vector<double *> bases;
-  std::uint32_t *dev_idx = nullptr;
-  double **dev_bases = nullptr;
-  double *arena = nullptr;
   void Free();
   pth_status Upload();
 };
-constexpr int kIdxShift = 27;
-constexpr std::uint32_t kIdxMask = (1u << kIdxShift) - 1;
-constexpr int kIdxSlots = 32;
 
 // Flux correction (reference flxcor_send / flxcor_recv, boundary_communication.cpp:454-461; the
 // face-element index boxes of bnd_info.cpp:69-101, 106-212, 273-305 and 387-403; the neighbour
@@ -424,11 +412,6 @@ pth_status LaunchBoundaryConditions(const BcTable &t, hipStream_t s);
 // box copies: region q copies its box from a (at lo, strides astride) to b (pre-offset to the
 // destination box origin, strides bstride)
 pth_status LaunchCopy(const Table &t, hipStream_t s);
-// Per-element gather words of a region walk (regions in element order, one base word each),
-// expanded on the device into a new array of *total words (synchronous).
-pth_status ExpandGatherWords(const std::vector<Region> &regions,
-                             const std::vector<std::uint32_t> &words, std::uint32_t **dev_idx,
-                             std::int64_t *total);
 
 std::uint64_t Magic(int d);
 
