@@ -488,6 +488,8 @@ def main():
                          "plane reconstruction with the exchange on a second stream; plane "
                          "reconstruction, update, exchange on one stream; auto = timed at "
                          "initialization")
+    ap.add_argument("--no-adaptive", action="store_true",
+                    help="burgers-amr: skip the adaptive remesh measurement in the same line")
     ap.add_argument("--cpu-cycles", type=int, default=None)
     ap.add_argument("--inner-cycles", type=int, default=2,
                     help="gmg: FP32 V-cycles per FP64 refinement step")
@@ -634,7 +636,7 @@ def main():
             "form": "task-by-task calls in the reference order, one launch per boundary "
                     f"buffer, eager ({pc} cycles after 1 warm-up)"}
         del pd, pmesh
-    if amr and args.driver != "tasks":
+    if amr and args.driver != "tasks" and not args.no_adaptive:
         out["adaptive"] = adaptive_bench(args, quiet=True)
     if not args.no_cpu_baseline:
         threads = min(len(os.sched_getaffinity(0)), 16)
