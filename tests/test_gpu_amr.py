@@ -292,3 +292,19 @@ def test_adaptive_open_boundaries_match_oracle():
         dt, t = d.clock()
         assert dt == ob.dt and t == lib().orc_burgers_time(ob.h)
     assert remeshed >= 1
+
+
+def test_vibe_input_never_refines():
+    # benchmarks/burgers/burgers.pin:21-55: 128^3 in 16^3 blocks (one tree, root level 3),
+    # numlevel 2, derivative_order_1 on U[3] with 0.5 / 0.2. The criterion's max_level (2) is
+    # below every logical level, so no block is tagged for refinement, and root-level blocks
+    # never derefine: the mesh stays uniform, as in the oracle.
+    print("case amr VIBE input", flush=True)
+    m = make_burgers_mesh((128, 128, 128), (16, 16, 16), 4, 4,
+                          adaptive=dict(numlevel=2, derefine_count=10, criteria=VIBE_CRIT))
+    assert m.root_level == 3
+    d = DeviceBurgersDriver(m, cfl=0.4, use_graph=True)
+    d.initialize("vibe")
+    assert d.adaptive_cycles(2) == 0 and m.nblocks == 512
+    f, c = m.refinement_flags()
+    assert not f.any() and not c.any()
