@@ -86,6 +86,20 @@ def measured_traffic(workload, kernel):
     return (e["fetch_kib"] * t["fetch_scale"] + e["write_kib"] * t["write_scale"]) * 1024
 
 
+VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4  # wave64 VALU instructions/s: 1024 SIMDs, 4 cycles each
+
+
+def measured_valu(workload, kernel):
+    """Wave-level VALU instructions per launch of `kernel` from the committed SQ_INSTS_VALU
+    counter pass (profiles/traffic.json), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            e = json.load(f).get(workload) or {}
+    except (OSError, ValueError):
+        return None
+    return e.get("sq_insts_valu") if e.get("kernel") == kernel else None
+
+
 def step_bytes(nb, n, g, V, exchange_bytes):
     """Algorithmic bytes of one RK2 cycle in the reference task decomposition: per block the
     reconstruction/flux, divergence, averaging, update, derived and dt tasks, plus the
@@ -608,6 +622,15 @@ def main():
                                                  dom),
                      "algorithmic_bytes_per_launch": kb.get(dom),
                      "avg_launch_us": avg_s * 1e6},
+        # the block sweep is bound by FP64 instruction issue, not by HBM: its share of the
+        # VALU issue capacity over the live launch time, from the committed instruction count
+        "issue_roofline": ({"bound": "valu",
+                            "valu_insts_per_launch": measured_valu(
+                                "burgers-amr" if amr else "burgers-uniform", dom),
+                            "frac": measured_valu("burgers-amr" if amr else "burgers-uniform",
+                                                  dom) / (VALU_ISSUE_PER_S * avg_s)}
+                           if measured_valu("burgers-amr" if amr else "burgers-uniform", dom)
+                           else None),
         "step_roofline": {"algorithmic_bytes_per_step": sb,
                           "achieved_GBps": sb / (wall / args.steps) / 1e9,
                           "frac": sb / (wall / args.steps) / 1e9 / HBM_PEAK_GBS},
