@@ -1517,9 +1517,9 @@ int ResidentGroups(const void *kernel, int threads) {
 pth_status BurgersSweep(const Topology &t, int V, const double *U, const double *base,
                         double *out, double *derived, double *partial, int max_partials,
                         const BlockCoefs *coef, double beta, const double *dtvar, int recon,
-                        const FluxCorrection *fc, int waves, const int *push, int *groups,
-                        hipStream_t s) {
-  if (!BurgersSweepSupported(t, V) || (waves != 3 && waves != 4)) {
+                        const FluxCorrection *fc, int waves, int spread, const int *push,
+                        int *groups, hipStream_t s) {
+  if (!BurgersSweepSupported(t, V) || (waves != 3 && waves != 4) || spread < 1 || spread > 3) {
     SetError("block sweep: unsupported block shape or component count");
     return PTH_FAIL;
   }
@@ -1537,7 +1537,8 @@ pth_status BurgersSweep(const Topology &t, int V, const double *U, const double 
       SetError("block sweep: occupancy query failed");                                        \
       return PTH_FAIL;                                                                        \
     }                                                                                         \
-    grid = std::min<std::int64_t>(std::min(r, max_partials), static_cast<std::int64_t>(nb) * NN); \
+    grid = std::min<std::int64_t>(std::min(std::max(r / spread, 1), max_partials),             \
+                                  static_cast<std::int64_t>(nb) * NN);                        \
     hipLaunchKernelGGL((BlockSweepKernel<NN, 4, NV, W>), dim3(grid), dim3(NN * NN), 0, s, U,  \
                        base, out, derived, partial, coef, nb, beta, dtvar, recon, fover, fcor, \
                        push);                                                                 \

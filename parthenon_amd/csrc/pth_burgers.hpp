@@ -39,13 +39,15 @@ bool BurgersSweepSupported(const Topology &t, int V);
 // one launch per stage: reconstruction, HLL, flux divergence, RK update, derived, CFL partial
 // per workgroup (partial[0 .. *groups), at most max_partials workgroups); faces with a finer
 // neighbour take fc's corrected fluxes; waves (3 or 4) is the register budget the kernel is
-// compiled for (waves per SIMD); with push (27 same-level neighbours per block, or -1) the
-// output's boundary cells are also written into those neighbours' ghost cells of out
+// compiled for (waves per SIMD); spread (1, 2 or 3) divides the resident workgroup count, so
+// each workgroup takes longer plane segments (fewer z-state rebuilds on small meshes); with
+// push (27 same-level neighbours per block, or -1) the output's boundary cells are also
+// written into those neighbours' ghost cells of out
 pth_status BurgersSweep(const Topology &t, int V, const double *U, const double *base,
                         double *out, double *derived, double *partial, int max_partials,
                         const BlockCoefs *coef, double beta, const double *dtvar, int recon,
-                        const FluxCorrection *fc, int waves, const int *push, int *groups,
-                        hipStream_t s);
+                        const FluxCorrection *fc, int waves, int spread, const int *push,
+                        int *groups, hipStream_t s);
 // flux-correction send of the sweep: fine fluxes from the state, restricted into fc's store
 pth_status FlxcorDirect(const Topology &t, int V, const FluxCorrection &fc, const double *U,
                         int recon, hipStream_t s);

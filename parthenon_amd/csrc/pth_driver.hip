@@ -198,7 +198,8 @@ pth_status StageSweep(Ctx &x, int stage, hipStream_t s) {
   int groups = 0;
   if (BurgersSweep(m.topo, x.V, U, x.base->store[x.fu].data, mc1.store[x.fu].data, derived,
                    stage == 2 ? m.scratch : nullptr, static_cast<int>(m.scratch_n), m.coef, beta,
-                   m.dtvar, m.run_recon, x.fc, m.sweep_waves, x.push, &groups, s) !=
+                   m.dtvar, m.run_recon, x.fc, m.sweep_waves, m.sweep_spread, x.push, &groups,
+                   s) !=
       PTH_COMPLETE)
     return PTH_FAIL;
   if (Exchange(m, mc1, s, x.push != nullptr) != PTH_COMPLETE) return PTH_FAIL;
@@ -284,13 +285,13 @@ pth_status TuneReconFlux(Ctx &x, hipStream_t s) {
     for (int waves : {4, 3}) {
       st = BurgersSweep(m.topo, x.V, U, U, x.one->store[x.fu].data, nullptr, nullptr,
                         static_cast<int>(m.scratch_n), m.coef, 1.0, m.dtvar, m.run_recon, x.fc,
-                        waves, nullptr, nullptr, s);
+                        waves, 1, nullptr, nullptr, s);
       if (st != PTH_COMPLETE) break;
       if (hipEventRecord(ev[0], s) != hipSuccess) st = PTH_FAIL;
       for (int r = 0; r < 3 && st == PTH_COMPLETE; ++r)
         st = BurgersSweep(m.topo, x.V, U, U, x.one->store[x.fu].data, nullptr, nullptr,
                           static_cast<int>(m.scratch_n), m.coef, 1.0, m.dtvar, m.run_recon,
-                          x.fc, waves, nullptr, nullptr, s);
+                          x.fc, waves, 1, nullptr, nullptr, s);
       float ms = 0.0f;
       if (st != PTH_COMPLETE || hipEventRecord(ev[1], s) != hipSuccess ||
           hipEventSynchronize(ev[1]) != hipSuccess ||
@@ -302,6 +303,30 @@ pth_status TuneReconFlux(Ctx &x, hipStream_t s) {
       if (waves == 4 || ms < best) {
         best = ms;
         m.sweep_waves = waves;
+      }
+    }
+    // then how many resident workgroups share the planes (longer segments on small meshes)
+    m.sweep_spread = 1;
+    for (int spread : {1, 2, 3}) {
+      if (st != PTH_COMPLETE) break;
+      for (int r = 0; r < 4 && st == PTH_COMPLETE; ++r) {
+        if (r == 1 && hipEventRecord(ev[0], s) != hipSuccess) st = PTH_FAIL;
+        if (st == PTH_COMPLETE)
+          st = BurgersSweep(m.topo, x.V, U, U, x.one->store[x.fu].data, nullptr, nullptr,
+                            static_cast<int>(m.scratch_n), m.coef, 1.0, m.dtvar, m.run_recon,
+                            x.fc, m.sweep_waves, spread, nullptr, nullptr, s);
+      }
+      float ms = 0.0f;
+      if (st != PTH_COMPLETE || hipEventRecord(ev[1], s) != hipSuccess ||
+          hipEventSynchronize(ev[1]) != hipSuccess ||
+          hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess) {
+        if (st == PTH_COMPLETE) SetError("sweep tuning: event timing failed");
+        st = PTH_FAIL;
+        break;
+      }
+      if (spread == 1 || ms < best) {
+        best = ms;
+        m.sweep_spread = spread;
       }
     }
   }

@@ -331,6 +331,7 @@ class Mesh {
   bool form_auto = true;                        // driver: pick sweep / overlap / serial by timing
   bool form_tuned = false;
   int sweep_waves = 4;                          // driver: sweep register budget (waves/SIMD)
+  int sweep_spread = 1;                         // driver: resident sweep workgroups / spread
   bool rf_tuned = false;
   bool rf_auto = true;                          // driver: tuning chosen by timing (not set)
   int tuned_blocks = 0;                         // driver: block count the choices were timed on
