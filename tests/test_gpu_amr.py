@@ -308,3 +308,33 @@ def test_vibe_input_never_refines():
     assert d.adaptive_cycles(2) == 0 and m.nblocks == 512
     f, c = m.refinement_flags()
     assert not f.any() and not c.any()
+
+
+def test_two_criteria_match_oracle():
+    # first- and second-derivative criteria on different components, tolerances at the
+    # oracle's exact maxima of each: the flags (max over criteria, then SetRefinement) agree
+    print("case amr two criteria", flush=True)
+    nx, block, box = (48, 48, 48), (16, 16, 16), [((-0.5, -0.2, -0.5, -0.2, -0.5, -0.2), 1)]
+    om = OracleMesh(nx, block, 4, boxes=box)
+    q = seeded(om.field_shape(2), 31)
+    om.exchange_bc([q], coarse=[np.zeros(om.coarse_shape(2))])
+    base = OracleAmr(om, 2, 1)
+    m1 = base.derivative_max(q, comp=0, method=1)
+    m2 = base.derivative_max(q, comp=1, method=2)
+    for t1, t2 in ((np.median(m1), np.median(m2)), (m1.min(), m2.max()), (m1.max(), m2.min())):
+        crit = [dict(method=1, comp=0, refine_tol=float(t1), derefine_tol=float(t1) * 0.9),
+                dict(method=2, comp=1, refine_tol=float(t2), derefine_tol=float(t2) * 0.9)]
+        m = pa.Mesh(nx, block, 4, refine_boxes=box)
+        m.add_field("q", 2, pa.Metadata.FillGhost | pa.Metadata.Independent)
+        m.set_adaptive(2, 1)
+        for c in crit:
+            m.add_refinement_criterion("q", **c)
+        m.finalize()
+        md = m.mesh_data("base")
+        md.upload("q", q)
+        pa.Refinement.Tag(md)
+        oa = OracleAmr(om, 2, 1, criteria=crit)
+        oa.tag(q)
+        f, c = m.refinement_flags()
+        print(f"tolerances {t1:.4g} {t2:.4g}: flags {np.bincount(f + 1, minlength=3)}", flush=True)
+        assert np.array_equal(f, oa.flags()[0]) and np.array_equal(c, oa.flags()[1])

@@ -236,3 +236,28 @@ def test_adaptive_burgers_conserves_totals():
     assert r >= 1 and m.nblocks != 64
     assert _levels_nested(m)
     np.testing.assert_allclose(b.totals(), t0, rtol=1e-12, atol=1e-12)
+
+
+def test_criteria_combine_by_maximum():
+    # CheckAllRefinement keeps the largest recommendation over the criteria: comp 0 = 100 + i
+    # has a first-derivative maximum of 1/(100 + g), comp 1 = 100 + i^2 a second-derivative
+    # maximum of 1/(201 + 2 g^2); the tolerances below put each criterion on either side.
+    m = O.OracleMesh((48, 48, 48), (16, 16, 16), 4, boxes=[((-0.5, -0.2, -0.5, -0.2, -0.5, -0.2), 1)])
+    lv = np.array([l for l, _ in m.locs()])
+    q = np.concatenate([_index_field(m, lambda k, j, i: 100.0 + i),
+                        _index_field(m, lambda k, j, i: 100.0 + i * i)], axis=1)
+    d1, d2 = 1.0 / 104.0, 1.0 / (201.0 + 2 * 16)
+    second = dict(method=2, comp=1, refine_tol=2 * d2, derefine_tol=1.5 * d2)   # derefine
+    for first, want in ((dict(refine_tol=0.5 * d1, derefine_tol=0.1 * d1), 1),    # refine
+                        (dict(refine_tol=2 * d1, derefine_tol=0.5 * d1), 0),      # keep
+                        (dict(refine_tol=4 * d1, derefine_tol=2 * d1), -1)):      # derefine
+        a = O.OracleAmr(m, numlevel=2, deref_threshold=1,
+                        criteria=[dict(method=1, comp=0, **first), second])
+        a.tag(q)
+        f = a.flags()[0]
+        if want == 1:      # base blocks refine; level-1 blocks are at the maximum level
+            assert set(f[lv == 0]) == {1} and set(f[lv == 1]) == {0}
+        elif want == 0:
+            assert not f.any()
+        else:              # root blocks never derefine; level-1 blocks without finer neighbours do
+            assert set(f[lv == 0]) == {0} and set(f[lv == 1]) == {-1}
