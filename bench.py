@@ -478,8 +478,10 @@ def exchange_bench(args):
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes_per_launch": 16 * nel, "avg_launch_us": us[dom]},
+        # the buffered figure (pack R+W, unpack R+W) per exchange, also for the fused form
         "exchange_roofline": {"algorithmic_bytes": alg,
-                              "achieved_GBps": alg / (wall / args.steps) / 1e9},
+                              "achieved_GBps": alg / (wall / args.steps) / 1e9,
+                              "frac": alg / (wall / args.steps) / 1e9 / HBM_PEAK_GBS},
         "kernels_avg_us": us}))
 
 
