@@ -338,3 +338,38 @@ def test_two_criteria_match_oracle():
         f, c = m.refinement_flags()
         print(f"tolerances {t1:.4g} {t2:.4g}: flags {np.bincount(f + 1, minlength=3)}", flush=True)
         assert np.array_equal(f, oa.flags()[0]) and np.array_equal(c, oa.flags()[1])
+
+
+def test_remesh_1d_matches_oracle():
+    # a 1-D mesh (nleaf 2): refine three blocks (two neighbours and one apart), then derefine
+    # the pair whose parent has no finer neighbour
+    print("case amr 1-D remesh", flush=True)
+    nx, block = (64, 1, 1), (8, 1, 1)
+    m = pa.Mesh(nx, block, 2, periodic=(True, False, False))
+    m.add_field("q", 2, pa.Metadata.FillGhost | pa.Metadata.Independent)
+    m.set_adaptive(3, 1)
+    m.finalize()
+    om = OracleMesh(nx, block, 2, periodic=(1, 0, 0))
+    oa = OracleAmr(om, 3, 1)
+    md = m.mesh_data("base")
+    md.upload("q", seeded(m.field_shape("q"), 41))
+    pa.exchange_fused(md)
+    x = md.download("q")
+    for step in ("refine", "derefine"):
+        locs = m.locs()
+        assert locs == om.locs()
+        for g, (lv, l) in enumerate(locs):
+            if step == "refine":
+                tag = 1 if l[0] in (2, 3, 6) and lv == 0 else 0
+            else:
+                tag = -1 if lv == 1 and l[0] // 2 == 6 else 0
+            m.set_refinement(g, tag)
+            oa.set_refinement(g, tag)
+        assert m.adapt()
+        nb, nnew, ndel = oa.update()
+        x = oa.transfer(x)
+        oa.commit()
+        om.exchange_bc([x], coarse=[np.zeros(om.coarse_shape(2))])
+        assert m.locs() == om.locs() and m.nblocks == nb
+        assert np.array_equal(md.download("q"), x), f"1-D data after {step}"
+    assert m.nblocks == 8 + 2
