@@ -520,11 +520,17 @@ pth_status pth_mesh_set_refinement(pth_mesh *mh, int gid, int tag) {
 }
 
 pth_status pth_mesh_refinement_flags(const pth_mesh *mh, int *flags, int *counts) {
-  Mesh *m = const_cast<Mesh *>(Impl(mh));
-  if (!AmrReady(m)) return PTH_FAIL;
-  for (size_t b = 0; b < m->amr.flag.size(); ++b) {
-    if (flags) flags[b] = m->amr.flag[b];
-    if (counts) counts[b] = m->amr.count[b];
+  const Mesh *m = Impl(mh);
+  if (!m->finalized || !m->amr.on) {
+    SetError("needs a finalized adaptive mesh");
+    return PTH_FAIL;
+  }
+  // blocks not tagged yet (flags not sized) report 0, as a new MeshRefinement holds
+  const size_t nb = m->topo.blocks.size();
+  const bool sized = m->amr.flag.size() == nb;
+  for (size_t b = 0; b < nb; ++b) {
+    if (flags) flags[b] = sized ? m->amr.flag[b] : 0;
+    if (counts) counts[b] = sized ? m->amr.count[b] : 0;
   }
   return PTH_COMPLETE;
 }
