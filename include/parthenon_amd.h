@@ -196,7 +196,8 @@ pth_status pth_burgers_set_sweep(pth_mesh *m, int on);
 pth_status pth_burgers_form(pth_mesh *m, int *sweep, int *overlap);
 /* Driver tuning (no counterpart in the reference): planes per workgroup (1, 2 or 4) of the
  * plane-tiled reconstruction for its three forms (all faces, ghost-free faces, the rest), and
- * the block sweep's register budget (3 or 4 waves per SIMD) as planes[3] of
+ * the block sweep's register budget (3 or 4: the waves per SIMD it is compiled for; with the
+ * sweep's LDS the 4-wave build runs at 2 waves per SIMD without spills) as planes[3] of
  * pth_burgers_tuning. pth_burgers_init picks them by timing each choice on the mesh's data
  * unless set here first; pth_burgers_tuning returns PTH_INCOMPLETE while no choice has been
  * made. Results do not depend on the choice. */

@@ -278,8 +278,8 @@ pth_status TuneReconFlux(Ctx &x, hipStream_t s) {
       }
     }
   }
-  // block sweep: the register budget (3 or 4 waves per SIMD); the trial writes only the
-  // interior of stage "1", which the next stage overwrites
+  // block sweep: the register budget (built for 3 or 4 waves per SIMD); the trial writes only
+  // the interior of stage "1", which the next stage overwrites
   if (st == PTH_COMPLETE && BurgersSweepSupported(m.topo, x.V)) {
     float best = 0.0f;
     for (int waves : {4, 3}) {
