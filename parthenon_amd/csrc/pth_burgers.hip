@@ -9,97 +9,25 @@
 //   DerivedKernel    burgers_package::CalculateDerived (burgers_package.cpp:143-167)
 //   DtKernel         burgers_package::EstimateTimestepMesh (burgers_package.cpp:170-200):
 //                    per-workgroup min via 64-lane wavefront shuffles, then one final pass
-// Expression order matches the reference operator by operator and the library is built with
-// -ffp-contract=off, so results are reproducible bit for bit by an IEEE CPU evaluation.
+// Expression order matches the reference operator by operator and this file is built with
+// -ffp-contract=off, so results are reproducible bit for bit by an IEEE CPU evaluation (the
+// block-sweep form, in pth_sweep.hip, is built with contraction instead).
 // Arrays are [block][comp][k][j][i] bricks of (n+2g)^3 with i fastest; consecutive lanes
 // take consecutive i, so every load and store is a coalesced run along x.
 #include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <map>
 
 #include <algorithm>
 #include <cfloat>
 
 #include "pth_burgers.hpp"
 #include "pth_device.hpp"
+#include "pth_stencil.hpp"
 
 namespace pth {
 
 namespace {
 
 constexpr int kWG = 256;
-
-struct Geo {   // brick shape
-  int e0, e1, e2;       // entire extents
-  int lo0, lo1, lo2;    // interior starts
-  int hi0, hi1, hi2;    // interior ends
-  int V;                // components
-  int ndim;
-  std::int64_t cells;   // e0*e1*e2
-};
-
-__device__ __forceinline__ double DMin(double a, double b) { return b < a ? b : a; }
-__device__ __forceinline__ double DMax(double a, double b) { return a < b ? b : a; }
-
-__device__ __forceinline__ double Mc(double dm, double dp) {
-  const double dc = (dm * dp > 0.0) * 0.5 * (dm + dp);
-  return copysign(DMin(fabs(dc), 2.0 * DMin(fabs(dm), fabs(dp))), dc);
-}
-
-__device__ __forceinline__ void LinearRecon(double qm, double q0, double qp, double &ql,
-                                            double &qr) {
-  double dq = qp - q0;
-  dq = 0.5 * Mc(q0 - qm, dq);
-  ql = q0 + dq;
-  qr = q0 - dq;
-}
-
-__device__ __forceinline__ void Weno5Z(double q0, double q1, double q2, double q3, double q4,
-                                       double &ql, double &qr) {
-  const double a00 = 1.0 / 3.0, a01 = -7.0 / 6.0, a02 = 11.0 / 6.0;
-  const double a10 = -1.0 / 6.0, a11 = 5.0 / 6.0, a12 = 1.0 / 3.0;
-  const double a20 = 1.0 / 3.0, a21 = 5.0 / 6.0, a22 = -1.0 / 6.0;
-  const double g0 = 0.1, g1 = 0.6, g2 = 0.3;
-  const double eps = 10.0 * DBL_EPSILON;  // robust::EPS (src/utils/robust.hpp:38-41)
-  const double t13 = 13.0 / 3.0;
-  double a = q0 - 2 * q1 + q2;
-  double b = q0 - 4.0 * q1 + 3.0 * q2;
-  double be0 = t13 * a * a + b * b + eps;
-  a = q1 - 2.0 * q2 + q3;
-  b = q3 - q1;
-  double be1 = t13 * a * a + b * b + eps;
-  a = q2 - 2.0 * q3 + q4;
-  b = q4 - 4.0 * q3 + 3.0 * q2;
-  double be2 = t13 * a * a + b * b + eps;
-  const double tau5 = fabs(be2 - be0);
-  be0 = (be0 + tau5) / be0;
-  be1 = (be1 + tau5) / be1;
-  be2 = (be2 + tau5) / be2;
-  double w0 = g0 * be0 + eps;
-  double w1 = g1 * be1 + eps;
-  double w2 = g2 * be2 + eps;
-  double ws = 1.0 / (w0 + w1 + w2);
-  ql = w0 * (a00 * q0 + a01 * q1 + a02 * q2);
-  ql += w1 * (a10 * q1 + a11 * q2 + a12 * q3);
-  ql += w2 * (a20 * q2 + a21 * q3 + a22 * q4);
-  ql *= ws;
-  const double al = 3.0 * ws * w0 * w1 * w2 / (g2 * w0 * w1 + g1 * w0 * w2 + g0 * w1 * w2) + eps;
-  w0 = g0 * be2 + eps;
-  w1 = g1 * be1 + eps;
-  w2 = g2 * be0 + eps;
-  ws = 1.0 / (w0 + w1 + w2);
-  qr = w0 * (a00 * q4 + a01 * q3 + a02 * q2);
-  qr += w1 * (a10 * q3 + a11 * q2 + a12 * q1);
-  qr += w2 * (a20 * q2 + a21 * q1 + a22 * q0);
-  qr *= ws;
-  const double ar = 3.0 * ws * w0 * w1 * w2 / (g2 * w0 * w1 + g1 * w0 * w2 + g0 * w1 * w2) + eps;
-  double dq = q3 - q2;
-  dq = 0.5 * Mc(q2 - q1, dq);
-  const double alin = 2.0 * al * ar / (al + ar);
-  ql = alin * ql + (1.0 - alin) * (q2 + dq);
-  qr = alin * qr + (1.0 - alin) * (q2 - dq);
-}
 
 // One thread per (block, comp, k, j, i) of the box extended by one cell around the interior.
 // face layout: [6][block][comp][k][j][i] for Ulx, Urx, Uly, Ury, Ulz, Urz.
@@ -260,12 +188,6 @@ __global__ void __launch_bounds__(kWG)
   }
 }
 
-__device__ __forceinline__ double WaveMin(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = DMin(v, __shfl_xor(v, off, 64));
-  return v;
-}
-
 // Per-workgroup minimum of the CFL time step over interior cells; partial[blockIdx.x].
 __global__ void __launch_bounds__(kWG)
     DtKernel(const double *__restrict__ U, const BlockCoefs *__restrict__ coef, Geo G,
@@ -319,12 +241,6 @@ __global__ void __launch_bounds__(64) MinFinalKernel(const double *__restrict__ 
 // cubic blocks (N interior cells, NG ghosts; N == 0 reads the shape at run time), the direction
 // is uniform per workgroup, and loads and stores use 32-bit byte offsets from per-workgroup
 // base pointers.
-__device__ __forceinline__ double LoadAt(const double *base, unsigned byte_off) {
-  return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(base) + byte_off);
-}
-__device__ __forceinline__ void StoreAt(double *base, unsigned byte_off, double v) {
-  *reinterpret_cast<double *>(reinterpret_cast<char *>(base) + byte_off) = v;
-}
 
 template <int N, int NG, int D>
 __device__ __forceinline__ void ReconTask(const double *__restrict__ q, double *__restrict__ fl,
@@ -374,27 +290,6 @@ __global__ void __launch_bounds__(kWG)
     ReconTask<N, NG, 1>(q, fl, fr, G, t, recon);
   else
     ReconTask<N, NG, 2>(q, fl, fr, G, t, recon);
-}
-
-// HLL flux of every component at one face from the face states (burgers_package.cpp:307-401
-// with lr_to_flux, burgers_package.hpp:33-43). l(n), r(n) give component n of the left and
-// right states; d = normal direction.
-template <int V, class LS, class RS>
-__device__ __forceinline__ void HllGen(LS l, RS r, int d, double *F) {
-  const double upl = l(d), upr = r(d);
-  const double sl = DMin(DMin(upl, upr), 0.0);
-  const double sr = DMax(DMax(upl, upr), 0.0);
-  const double islsr = 1.0 / (sr - sl + (sl * sr == 0.0));
-#pragma unroll
-  for (int n = 0; n < 3; ++n) {
-    const double ul = l(n), ur = r(n);
-    F[n] = 0.5 * (sr * ul * upl - sl * ur * upr + sl * sr * (ur - ul)) * islsr;
-  }
-#pragma unroll
-  for (int n = 3; n < V; ++n) {
-    const double ql = l(n), qr = r(n);
-    F[n] = (sr * upl * ql - sl * upr * qr + sl * sr * (qr - ql)) / (sr - sl + (sl * sr == 0.0));
-  }
 }
 
 // HllGen over state arrays: L, R point at component 0 of the face; component stride cs.
@@ -696,66 +591,6 @@ __global__ void __launch_bounds__(kWG)
   }
 }
 
-// RestrictAverage::Do for face element F(D+1) (pr_ops.hpp:105-164): area-weighted average of
-// the fine face fluxes fqp (p along the lower transverse dimension t1, q along t2) sharing one
-// coarse face, in the reference's fixed summation order; faces beyond the active dimensions
-// carry zero weight and a zero term, as in the reference loop.
-template <int D>
-__device__ __forceinline__ double RestrictFaceValue(double area, double f00, double f01,
-                                                    double f10, double f11, int s1, int s2) {
-  double T[2][2][2], W[2][2][2];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-      for (int c = 0; c < 2; ++c) T[a][b][c] = W[a][b][c] = 0.0;
-  const double f[2][2] = {{f00, f01}, {f10, f11}};
-#pragma unroll
-  for (int q = 0; q < 2; ++q)
-#pragma unroll
-    for (int p = 0; p < 2; ++p) {
-      if (p >= s1 || q >= s2) continue;
-      // [ok][oj][oi]: D=0 -> (t2 = z, t1 = y); D=1 -> (t2 = z, t1 = x); D=2 -> (t2 = y, t1 = x)
-      double &t = D == 0 ? T[q][p][0] : D == 1 ? T[q][0][p] : T[0][q][p];
-      double &w = D == 0 ? W[q][p][0] : D == 1 ? W[q][0][p] : W[0][q][p];
-      w = area;
-      t = w * f[q][p];
-    }
-  const double tvol = ((W[0][0][0] + W[0][1][0]) + (W[0][0][1] + W[0][1][1])) +
-                      ((W[1][0][0] + W[1][1][0]) + (W[1][0][1] + W[1][1][1]));
-  return (((T[0][0][0] + T[0][1][0]) + (T[0][0][1] + T[0][1][1])) +
-          ((T[1][0][0] + T[1][1][0]) + (T[1][0][1] + T[1][1][1]))) /
-         tvol;
-}
-
-__device__ __forceinline__ double RestrictFace(int d, double area, double f00, double f01,
-                                               double f10, double f11, int s1, int s2) {
-  if (d == 0) return RestrictFaceValue<0>(area, f00, f01, f10, f11, s1, s2);
-  if (d == 1) return RestrictFaceValue<1>(area, f00, f01, f10, f11, s1, s2);
-  return RestrictFaceValue<2>(area, f00, f01, f10, f11, s1, s2);
-}
-
-// Brick offset of fine face (p, q) of patch cell (a, c): the face index along d, and transverse
-// fine cells step * (a, c) + (p, q) from the patch start.
-__device__ __forceinline__ std::int64_t PatchFineCell(const FluxPatch &P, const Geo &G, int a,
-                                                      int c, int p, int q) {
-  const int u = P.ft[0] + P.step[0] * a + p, v = P.ft[1] + P.step[1] * c + q;
-  const int i = P.d == 0 ? P.fface : u;
-  const int j = P.d == 1 ? P.fface : (P.d == 0 ? u : v);
-  const int k = P.d == 2 ? P.fface : v;
-  return (static_cast<std::int64_t>(k) * G.e1 + j) * G.e0 + i;
-}
-
-__device__ __forceinline__ std::int64_t PatchCoarseCell(const FluxPatch &P, const Geo &G, int a,
-                                                        int c) {
-  const int u = P.ct[0] + a, v = P.ct[1] + c;
-  const int i = P.d == 0 ? P.cface : u;
-  const int j = P.d == 1 ? P.cface : (P.d == 0 ? u : v);
-  const int k = P.d == 2 ? P.cface : v;
-  return (static_cast<std::int64_t>(k) * G.e1 + j) * G.e0 + i;
-}
-
 // flxcor_send of the fused stage: one thread per coarse face cell of a patch evaluates the HLL
 // fluxes of its 2x2 fine faces from the fine block's face states (the values CalculateFluxes
 // would store there) and writes their restriction to the correction store.
@@ -827,292 +662,6 @@ __global__ void __launch_bounds__(64)
               PatchCoarseCell(P, G, a, c);
   const std::int64_t src = P.dst + a + static_cast<std::int64_t>(c) * P.rows;
   for (int n = 0; n < nc; ++n) F[n * cs] = store[src + static_cast<std::int64_t>(n) * P.comp];
-}
-
-// Left and right states of one cell from its stencil along a direction (byte offsets c of the
-// cell and st of one step): WENO5Z or the MC-limited linear reconstruction.
-__device__ __forceinline__ void CellStates(const double *q, unsigned c, unsigned st, int recon,
-                                           double &ql, double &qr) {
-  if (recon == 0)
-    Weno5Z(LoadAt(q, c - 2 * st), LoadAt(q, c - st), LoadAt(q, c), LoadAt(q, c + st),
-           LoadAt(q, c + 2 * st), ql, qr);
-  else
-    LinearRecon(LoadAt(q, c - st), LoadAt(q, c), LoadAt(q, c + st), ql, qr);
-}
-
-// flxcor_send of the block-sweep stage: four adjacent threads per coarse face cell of a patch,
-// one per fine face (p, q), evaluate the HLL flux of their face straight from the fine block's
-// state (the left state of face f is ql of cell f - 1, the right state qr of cell f, as
-// CalculateFluxes stores them); the first of the four gathers the other three by lane
-// shuffles and writes their restriction to the correction store.
-template <int V>
-__global__ void __launch_bounds__(64)
-    FlxcorDirectKernel(const FluxPatch *__restrict__ patches, const double *__restrict__ U,
-                       double *__restrict__ store, Geo G, int recon) {
-  const FluxPatch &P = patches[blockIdx.x];
-  const int t = blockIdx.y * 64 + threadIdx.x;
-  const int cell = t >> 2, sub = t & 3, p = sub & 1, q = sub >> 1;
-  const bool live = cell < P.n[0] * P.n[1];
-  const int a = live ? cell % P.n[0] : 0, c = live ? cell / P.n[0] : 0;
-  const std::int64_t cs = G.cells;
-  const double *Ub = U + static_cast<std::int64_t>(P.fine) * V * cs;
-  const unsigned st = 8u * static_cast<unsigned>(P.d == 0 ? 1 : P.d == 1 ? G.e0 : G.e0 * G.e1);
-  double F[V];
-#pragma unroll
-  for (int n = 0; n < V; ++n) F[n] = 0.0;
-  if (live && p < P.step[0] && q < P.step[1]) {
-    const unsigned o = 8u * static_cast<unsigned>(PatchFineCell(P, G, a, c, p, q));
-    double l[V], r[V];
-#pragma unroll
-    for (int n = 0; n < V; ++n) {
-      double x, y;
-      CellStates(Ub + n * cs, o - st, st, recon, l[n], x);
-      CellStates(Ub + n * cs, o, st, recon, y, r[n]);
-    }
-    HllGen<V>([&](int n) { return l[n]; }, [&](int n) { return r[n]; }, P.d, F);
-  }
-  const int lane0 = static_cast<int>(threadIdx.x) & ~3;
-  const std::int64_t dst = P.dst + a + static_cast<std::int64_t>(c) * P.rows;
-#pragma unroll
-  for (int n = 0; n < V; ++n) {
-    const double f00 = __shfl(F[n], lane0, 64), f01 = __shfl(F[n], lane0 + 1, 64);
-    const double f10 = __shfl(F[n], lane0 + 2, 64), f11 = __shfl(F[n], lane0 + 3, 64);
-    if (live && sub == 0)
-      store[dst + static_cast<std::int64_t>(n) * P.comp] =
-          RestrictFace(P.d, P.area, f00, f01, f10, f11, P.step[0], P.step[1]);
-  }
-}
-
-// ---- block sweep: reconstruction, HLL, flux divergence and RK update in one launch ----------
-//
-// One thread per (i, j) column of a cubic block of N cells per side, marching k over interior
-// planes. Per plane: the x states of the plane's rows (extended by one cell each way) go to
-// LDS, the HLL flux of every x face of the plane is evaluated once into LDS (each thread the
-// high face of its cell, the first N threads the low face of one row each), and each thread
-// forms its cell's x difference; then the same for y; then the z states of the plane above,
-// whose qr closes the high z face (the low z face of plane k is the high z face of plane
-// k - 1, carried in registers). The flux divergence sums x, y, z in the order of
-// FluxDivHelper (update.hpp:43-57), then the RK averaging and update, the derived field and
-// the CFL minimum follow as in FluxUpdateKernel. Every face flux is the same expression as in
-// the other forms, so the stage output is the same bits, and no flux or face state reaches
-// HBM. On
-// multilevel meshes, faces with a finer neighbour take the restricted fine fluxes from the
-// correction store (fover / fcor as in FusedUpdateKernel).
-// The grid is persistent: workgroup g takes planes [g T / G, (g + 1) T / G) of the
-// T = nblocks * N planes in block order, so every workgroup does the same number of planes
-// (a segment that starts inside a block first rebuilds the z state below its first plane).
-// partial[g] is the workgroup's CFL minimum.
-
-template <int N, int NG, int V, int MINW>
-__global__ void __launch_bounds__(N * N, MINW)
-    BlockSweepKernel(const double *__restrict__ U, const double *base, double *out,
-                     double *__restrict__ derived, double *__restrict__ partial,
-                     const BlockCoefs *__restrict__ coef, int nblocks, double beta,
-                     const double *__restrict__ dtvar, int recon, const int *__restrict__ fover,
-                     const double *__restrict__ fcor, const int *__restrict__ push) {
-  static_assert(N >= 2 * NG, "a cell lies in at most one boundary strip per dimension");
-  constexpr int E = N + 2 * NG;
-  constexpr int NT = N * N;
-  constexpr int R = N + 2;  // states per row: ext cells lo - 1 .. hi + 1
-  constexpr int P = R * N;  // states of one plane per component
-  constexpr int FP = (N + 1) * N;  // faces of one plane per component and direction
-  __shared__ double sl[V * P], sr[V * P];
-  __shared__ double fx[V * FP];  // x (then y) face fluxes of the plane, each face once
-  __shared__ double wmin[(NT + 63) / 64];
-  const int tx = threadIdx.x % N, ty = threadIdx.x / N;
-  constexpr std::int64_t cs = static_cast<std::int64_t>(E) * E * E;
-  const double bdt = beta * dtvar[0];
-  const double w2 = 1.0 - beta;
-  constexpr int comp = N * N;  // correction-store stride between components
-  constexpr unsigned sx = 8u, sy = 8u * E, sz = 8u * E * E;
-  double m = DBL_MAX;
-  const std::int64_t T = static_cast<std::int64_t>(nblocks) * N;
-  std::int64_t pnext = T * blockIdx.x / gridDim.x;
-  const std::int64_t pend = T * (blockIdx.x + 1) / gridDim.x;
-  while (pnext < pend) {
-    const int b = static_cast<int>(pnext / N);
-    const int k0 = static_cast<int>(pnext % N);
-    const int k1 = static_cast<int>(pend - pnext < N - k0 ? k0 + (pend - pnext) : N);
-    pnext += k1 - k0;
-    const double *Ub = U + static_cast<std::int64_t>(b) * V * cs;
-    const BlockCoefs &C = coef[b];
-    const int ox0 = fover ? fover[6 * b + 0] : -1, ox1 = fover ? fover[6 * b + 1] : -1;
-    const int oy0 = fover ? fover[6 * b + 2] : -1, oy1 = fover ? fover[6 * b + 3] : -1;
-    const int oz0 = fover ? fover[6 * b + 4] : -1, oz1 = fover ? fover[6 * b + 5] : -1;
-    // z states of one plane of cells kc for every column, [n][j][i] (a flat loop like x and y)
-    auto zstates = [&](int kc) {
-      for (int t = threadIdx.x; t < V * NT; t += NT) {
-        const int n = t / NT, u = t % NT;
-        const unsigned c = 8u * static_cast<unsigned>((kc * E + NG + u / N) * E + NG + u % N);
-        CellStates(Ub + n * cs, c, sz, recon, sl[n * P + u], sr[n * P + u]);
-      }
-    };
-    // z faces: zl = ql of the cell below the current plane, zf = flux on the current low z face
-    double zl[V], zf[V];
-    __syncthreads();  // the previous segment's states have been read
-    zstates(NG + k0 - 1);
-    __syncthreads();
-#pragma unroll
-    for (int n = 0; n < V; ++n) zl[n] = sl[n * P + threadIdx.x];
-    __syncthreads();
-    zstates(NG + k0);
-    __syncthreads();
-    HllGen<V>([&](int n) { return zl[n]; }, [&](int n) { return sr[n * P + threadIdx.x]; }, 2, zf);
-    if (k0 == 0 && oz0 >= 0)
-#pragma unroll
-      for (int n = 0; n < V; ++n) zf[n] = fcor[oz0 + tx + ty * N + n * comp];
-#pragma unroll
-    for (int n = 0; n < V; ++n) zl[n] = sl[n * P + threadIdx.x];
-    for (int k = k0; k < k1; ++k) {
-      const int kk = NG + k;
-      double du[V];
-      // ---- x: states of ext cells lo - 1 .. hi + 1 of every row of the plane, [n][row][a]
-      __syncthreads();  // the previous states have been read
-      for (int t = threadIdx.x; t < V * P; t += NT) {
-        const int n = t / P, u = t % P;
-        const int a = u % R, row = u / R;
-        const unsigned c = 8u * static_cast<unsigned>((kk * E + NG + row) * E + NG - 1 + a);
-        CellStates(Ub + n * cs, c, sx, recon, sl[t], sr[t]);
-      }
-      __syncthreads();
-      {
-        // every x face once: each thread the high face of its cell, threads 0 .. N - 1 also
-        // the low face of row threadIdx.x; fluxes to LDS as [n][row][face]
-        double F[V];
-        const double *L = sl + ty * R + tx, *Rr = sr + ty * R + tx;
-        HllGen<V>([&](int n) { return L[n * P + 1]; }, [&](int n) { return Rr[n * P + 2]; }, 0,
-                  F);
-        if (tx == N - 1 && ox1 >= 0)
-#pragma unroll
-          for (int n = 0; n < V; ++n) F[n] = fcor[ox1 + ty + k * N + n * comp];
-#pragma unroll
-        for (int n = 0; n < V; ++n) fx[n * FP + ty * (N + 1) + tx + 1] = F[n];
-        if (static_cast<int>(threadIdx.x) < N) {
-          const int row = threadIdx.x;
-          HllGen<V>([&](int n) { return sl[n * P + row * R]; },
-                    [&](int n) { return sr[n * P + row * R + 1]; }, 0, F);
-          if (ox0 >= 0)
-#pragma unroll
-            for (int n = 0; n < V; ++n) F[n] = fcor[ox0 + row + k * N + n * comp];
-#pragma unroll
-          for (int n = 0; n < V; ++n) fx[n * FP + row * (N + 1)] = F[n];
-        }
-      }
-      __syncthreads();  // the x fluxes are in LDS and the x states have been read
-      {
-        // du = A F(hi) - A F(lo)
-        const double A = C.area[0];
-        const double *f = fx + ty * (N + 1) + tx;
-#pragma unroll
-        for (int n = 0; n < V; ++n) du[n] = (A * f[n * FP + 1] - A * f[n * FP]);
-      }
-      // ---- y: states of ext cells lo - 1 .. hi + 1 of every column, [n][a][i]
-      for (int t = threadIdx.x; t < V * P; t += NT) {
-        const int n = t / P, u = t % P;
-        const int i = u % N, a = u / N;
-        const unsigned c = 8u * static_cast<unsigned>((kk * E + NG - 1 + a) * E + NG + i);
-        CellStates(Ub + n * cs, c, sy, recon, sl[t], sr[t]);
-      }
-      __syncthreads();  // also: every thread has read its x fluxes
-      {
-        // every y face once, as [n][face][i]
-        double F[V];
-        const double *L = sl + ty * N + tx, *Rr = sr + ty * N + tx;
-        HllGen<V>([&](int n) { return L[n * P + N]; }, [&](int n) { return Rr[n * P + 2 * N]; },
-                  1, F);
-        if (ty == N - 1 && oy1 >= 0)
-#pragma unroll
-          for (int n = 0; n < V; ++n) F[n] = fcor[oy1 + tx + k * N + n * comp];
-#pragma unroll
-        for (int n = 0; n < V; ++n) fx[n * FP + (ty + 1) * N + tx] = F[n];
-        if (static_cast<int>(threadIdx.x) < N) {
-          const int col = threadIdx.x;
-          HllGen<V>([&](int n) { return sl[n * P + col]; },
-                    [&](int n) { return sr[n * P + N + col]; }, 1, F);
-          if (oy0 >= 0)
-#pragma unroll
-            for (int n = 0; n < V; ++n) F[n] = fcor[oy0 + col + k * N + n * comp];
-#pragma unroll
-          for (int n = 0; n < V; ++n) fx[n * FP + col] = F[n];
-        }
-      }
-      __syncthreads();  // the y fluxes are in LDS and the y states have been read
-      {
-        const double A = C.area[1];
-        const double *f = fx + ty * N + tx;
-#pragma unroll
-        for (int n = 0; n < V; ++n) du[n] = du[n] + (A * f[n * FP + N] - A * f[n * FP]);
-      }
-      // ---- z: qr of the cell above this plane gives the high z face; its ql moves down
-      zstates(kk + 1);
-      __syncthreads();  // also: every thread has read its y fluxes
-      {
-        double Fhi[V];
-        HllGen<V>([&](int n) { return zl[n]; }, [&](int n) { return sr[n * P + threadIdx.x]; }, 2,
-                  Fhi);
-        if (k == N - 1 && oz1 >= 0)
-#pragma unroll
-          for (int n = 0; n < V; ++n) Fhi[n] = fcor[oz1 + tx + ty * N + n * comp];
-        const double A = C.area[2];
-#pragma unroll
-        for (int n = 0; n < V; ++n) {
-          du[n] = du[n] + (A * Fhi[n] - A * zf[n]);
-          zf[n] = Fhi[n];
-          zl[n] = sl[n * P + threadIdx.x];
-        }
-      }
-      // ---- divergence, RK averaging and update, derived field, CFL
-      const std::int64_t cell = (static_cast<std::int64_t>(kk) * E + NG + ty) * E + NG + tx;
-      const std::int64_t c0 = static_cast<std::int64_t>(b) * V * cs + cell;
-      double u1[V];
-#pragma unroll
-      for (int n = 0; n < V; ++n) {
-        const double dudt = -du[n] / C.vol;
-        const double avg = beta * U[c0 + n * cs] + w2 * base[c0 + n * cs];
-        u1[n] = 1.0 * avg + bdt * dudt;
-        out[c0 + n * cs] = u1[n];
-      }
-      if (push) {
-        // the exchange's same-level copies: a cell within NG of a face is a ghost cell of the
-        // same-level neighbours across that face, edge or corner (push[27 b + direction]),
-        // at its index shifted by N along each crossed dimension
-        const int lx = tx < NG ? -1 : (tx >= N - NG ? 1 : 0);
-        const int ly = ty < NG ? -1 : (ty >= N - NG ? 1 : 0);
-        const int lz = k < NG ? -1 : (k >= N - NG ? 1 : 0);
-        for (int az = 0; az < (lz ? 2 : 1); ++az)
-          for (int ay = 0; ay < (ly ? 2 : 1); ++ay)
-            for (int ax = 0; ax < (lx ? 2 : 1); ++ax) {
-              const int dx = ax ? lx : 0, dy = ay ? ly : 0, dz = az ? lz : 0;
-              if (!(dx | dy | dz)) continue;
-              const int g = push[27 * b + (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)];
-              if (g < 0) continue;
-              const std::int64_t d0 =
-                  static_cast<std::int64_t>(g) * V * cs +
-                  (static_cast<std::int64_t>(kk - dz * N) * E + NG + ty - dy * N) * E + NG + tx -
-                  dx * N;
-#pragma unroll
-              for (int n = 0; n < V; ++n) out[d0 + n * cs] = u1[n];
-            }
-      }
-      if (derived)
-        derived[static_cast<std::int64_t>(b) * cs + cell] =
-            0.5 * u1[3] * (u1[0] * u1[0] + u1[1] * u1[1] + u1[2] * u1[2]);
-      double sp = fabs(u1[0]) / C.dx[0];
-      sp = sp + fabs(u1[1]) / C.dx[1];
-      sp = sp + fabs(u1[2]) / C.dx[2];
-      m = DMin(m, 1.0 / sp);
-    }
-  }
-  if (!partial) return;
-  m = WaveMin(m);
-  if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = m;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double v = wmin[0];
-#pragma unroll
-    for (int w = 1; w < (NT + 63) / 64; ++w) v = DMin(v, wmin[w]);
-    partial[blockIdx.x] = v;
-  }
 }
 
 // burgers_package::MassHistory (benchmarks/burgers/burgers_package.cpp:406-439) for the 8
@@ -1211,23 +760,6 @@ __global__ void __launch_bounds__(kClockWG)
 int Grid(std::int64_t total) {
   const std::int64_t want = (total + kWG - 1) / kWG;
   return static_cast<int>(want < 8192 ? want : 8192);
-}
-
-Geo MakeGeo(const Topology &t, int V) {
-  Geo g;
-  g.e0 = t.fine.ent(0);
-  g.e1 = t.fine.ent(1);
-  g.e2 = t.fine.ent(2);
-  g.lo0 = t.fine.lo(0);
-  g.lo1 = t.fine.lo(1);
-  g.lo2 = t.fine.lo(2);
-  g.hi0 = t.fine.hi(0);
-  g.hi1 = t.fine.hi(1);
-  g.hi2 = t.fine.hi(2);
-  g.V = V;
-  g.ndim = t.ndim;
-  g.cells = t.fine.cells();
-  return g;
 }
 
 }  // namespace
@@ -1493,120 +1025,6 @@ pth_status BurgersReconFlux(const Topology &t, int V, const double *U, double *f
 #undef PTH_RF_PARTS
 #undef PTH_RF_PPW
 #undef PTH_RF_LAUNCH
-  PTH_HIP(hipGetLastError());
-  return PTH_COMPLETE;
-}
-
-bool BurgersSweepSupported(const Topology &t, int V) { return ReconFluxSupported(t, V); }
-
-// Workgroups of a kernel resident on the whole device at once (occupancy x compute units).
-int ResidentGroups(const void *kernel, int threads) {
-  static std::map<const void *, int> cache;
-  auto it = cache.find(kernel);
-  if (it != cache.end()) return it->second;
-  int dev = 0, cus = 0, per = 0;
-  if (hipGetDevice(&dev) != hipSuccess ||
-      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess ||
-      per < 1 || cus < 1)
-    return -1;
-  cache[kernel] = per * cus;
-  return per * cus;
-}
-
-pth_status BurgersSweep(const Topology &t, int V, const double *U, const double *base,
-                        double *out, double *derived, double *partial, int max_partials,
-                        const BlockCoefs *coef, double beta, const double *dtvar, int recon,
-                        const FluxCorrection *fc, int waves, int spread, const int *push,
-                        int *groups, hipStream_t s) {
-  if (!BurgersSweepSupported(t, V) || (waves != 3 && waves != 4) || spread < 1 || spread > 3) {
-    SetError("block sweep: unsupported block shape or component count");
-    return PTH_FAIL;
-  }
-  ProfScope prof("block_sweep", s);
-  const int nb = static_cast<int>(t.blocks.size());
-  const int N = t.fine.n[0];
-  const int *fover = fc ? fc->dev_face : nullptr;
-  const double *fcor = fc ? fc->store : nullptr;
-  int grid = 0;
-#define PTH_SW_LAUNCH_W(NN, NV, W)                                                            \
-  {                                                                                           \
-    const int r = ResidentGroups(reinterpret_cast<const void *>(&BlockSweepKernel<NN, 4, NV, W>), \
-                                 NN * NN);                                                    \
-    if (r < 1) {                                                                              \
-      SetError("block sweep: occupancy query failed");                                        \
-      return PTH_FAIL;                                                                        \
-    }                                                                                         \
-    grid = std::min<std::int64_t>(std::min(std::max(r / spread, 1), max_partials),             \
-                                  static_cast<std::int64_t>(nb) * NN);                        \
-    hipLaunchKernelGGL((BlockSweepKernel<NN, 4, NV, W>), dim3(grid), dim3(NN * NN), 0, s, U,  \
-                       base, out, derived, partial, coef, nb, beta, dtvar, recon, fover, fcor, \
-                       push);                                                                 \
-  }
-#define PTH_SW_LAUNCH(NN, NV)                                                                 \
-  if (waves == 4)                                                                             \
-    PTH_SW_LAUNCH_W(NN, NV, (NV <= 8 ? 4 : 2))                                                \
-  else                                                                                        \
-    PTH_SW_LAUNCH_W(NN, NV, (NV <= 8 ? 3 : 2))
-#define PTH_SW_CASE(NV)                                                                       \
-  case NV:                                                                                    \
-    if (N == 16) {                                                                            \
-      PTH_SW_LAUNCH(16, NV)                                                                   \
-    } else {                                                                                  \
-      PTH_SW_LAUNCH(8, NV)                                                                    \
-    }                                                                                         \
-    break;
-  switch (V) {
-    PTH_SW_CASE(4)
-    PTH_SW_CASE(5)
-    PTH_SW_CASE(6)
-    PTH_SW_CASE(7)
-    PTH_SW_CASE(8)
-    PTH_SW_CASE(9)
-    PTH_SW_CASE(10)
-    PTH_SW_CASE(11)
-    PTH_SW_CASE(12)
-    PTH_SW_CASE(13)
-  }
-#undef PTH_SW_CASE
-#undef PTH_SW_LAUNCH
-#undef PTH_SW_LAUNCH_W
-  PTH_HIP(hipGetLastError());
-  if (groups) *groups = grid;
-  return PTH_COMPLETE;
-}
-
-pth_status FlxcorDirect(const Topology &t, int V, const FluxCorrection &fc, const double *U,
-                        int recon, hipStream_t s) {
-  if (fc.host.empty()) return PTH_COMPLETE;
-  if (V < 4 || V > 16) {
-    SetError("flux correction supports 4 to 16 components");
-    return PTH_FAIL;
-  }
-  ProfScope prof("flxcor_direct", s);
-  const Geo G = MakeGeo(t, V);
-  const dim3 grid(static_cast<unsigned>(fc.host.size()), (4 * fc.max_cells + 63) / 64);
-#define PTH_FD_CASE(NV)                                                                       \
-  case NV:                                                                                    \
-    hipLaunchKernelGGL(FlxcorDirectKernel<NV>, grid, dim3(64), 0, s, fc.dev, U, fc.store, G,  \
-                       recon);                                                                \
-    break;
-  switch (V) {
-    PTH_FD_CASE(4)
-    PTH_FD_CASE(5)
-    PTH_FD_CASE(6)
-    PTH_FD_CASE(7)
-    PTH_FD_CASE(8)
-    PTH_FD_CASE(9)
-    PTH_FD_CASE(10)
-    PTH_FD_CASE(11)
-    PTH_FD_CASE(12)
-    PTH_FD_CASE(13)
-    PTH_FD_CASE(14)
-    PTH_FD_CASE(15)
-    PTH_FD_CASE(16)
-  }
-#undef PTH_FD_CASE
   PTH_HIP(hipGetLastError());
   return PTH_COMPLETE;
 }

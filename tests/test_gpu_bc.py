@@ -153,7 +153,11 @@ def test_burgers_open_boundaries_match_oracle(refined, use_graph, overlap, sweep
         lib().orc_burgers_cycles(ob.h, 1)
         assert_close(d.base.download("U"), ob.get(0), f"U after cycle {c + 1}")
         dt, t = d.clock()
-        assert dt == ob.dt and t == lib().orc_burgers_time(ob.h)
+        t_exp = lib().orc_burgers_time(ob.h)
+        if sweep:  # the block sweep is built with FMA contraction: dt and time within 1e-13
+            assert abs(dt - ob.dt) <= 1e-13 * ob.dt and abs(t - t_exp) <= 1e-13 * t_exp
+        else:
+            assert dt == ob.dt and t == t_exp
     d.cycles(5)
     lib().orc_burgers_cycles(ob.h, 5)
     assert_close(d.base.download("U"), ob.get(0), "U after 8 cycles")
