@@ -25,6 +25,89 @@ namespace pth {
 
 namespace {
 
+// WENO5Z of recon.hpp:28-99 with its eight divides folded into three: one reciprocal of
+// be0 * be1 * be2 gives the three 1 / be_k, one reciprocal of (S_l D_l)(S_r D_r) gives both
+// sides' 1 / (S D) (S = w0 + w1 + w2, D = g2 w0 w1 + g1 w0 w2 + g0 w1 w2, so that
+// ql = N_l / S = N_l D_l / (S_l D_l) and al = 3 w0 w1 w2 / (S_l D_l) + eps), and alin keeps
+// its own. The values are the reference's to rounding; the magnitudes stay far inside the
+// FP64 range (be_k >= eps, w_k >= g_k, and the w_k are bounded by g_k tau / eps).
+__device__ __forceinline__ void Weno5ZSweep(double q0, double q1, double q2, double q3,
+                                            double q4, double &ql, double &qr) {
+  const double a00 = 1.0 / 3.0, a01 = -7.0 / 6.0, a02 = 11.0 / 6.0;
+  const double a10 = -1.0 / 6.0, a11 = 5.0 / 6.0, a12 = 1.0 / 3.0;
+  const double a20 = 1.0 / 3.0, a21 = 5.0 / 6.0, a22 = -1.0 / 6.0;
+  const double g0 = 0.1, g1 = 0.6, g2 = 0.3;
+  const double eps = 10.0 * DBL_EPSILON;  // robust::EPS (src/utils/robust.hpp:38-41)
+  const double t13 = 13.0 / 3.0;
+  double a = q0 - 2 * q1 + q2;
+  double b = q0 - 4.0 * q1 + 3.0 * q2;
+  const double be0 = t13 * a * a + b * b + eps;
+  a = q1 - 2.0 * q2 + q3;
+  b = q3 - q1;
+  const double be1 = t13 * a * a + b * b + eps;
+  a = q2 - 2.0 * q3 + q4;
+  b = q4 - 4.0 * q3 + 3.0 * q2;
+  const double be2 = t13 * a * a + b * b + eps;
+  const double tau5 = fabs(be2 - be0);
+  const double p12 = be1 * be2, p02 = be0 * be2, p01 = be0 * be1;
+  const double rp = 1.0 / (be0 * p12);
+  const double z0 = (be0 + tau5) * (rp * p12);
+  const double z1 = (be1 + tau5) * (rp * p02);
+  const double z2 = (be2 + tau5) * (rp * p01);
+  // left
+  const double l0 = g0 * z0 + eps, l1 = g1 * z1 + eps, l2 = g2 * z2 + eps;
+  const double nl = l0 * (a00 * q0 + a01 * q1 + a02 * q2) + l1 * (a10 * q1 + a11 * q2 + a12 * q3) +
+                    l2 * (a20 * q2 + a21 * q3 + a22 * q4);
+  const double dl = g2 * l0 * l1 + g1 * l0 * l2 + g0 * l1 * l2;
+  const double sdl = (l0 + l1 + l2) * dl;
+  // right
+  const double r0 = g0 * z2 + eps, r1 = g1 * z1 + eps, r2 = g2 * z0 + eps;
+  const double nr = r0 * (a00 * q4 + a01 * q3 + a02 * q2) + r1 * (a10 * q3 + a11 * q2 + a12 * q1) +
+                    r2 * (a20 * q2 + a21 * q1 + a22 * q0);
+  const double dr = g2 * r0 * r1 + g1 * r0 * r2 + g0 * r1 * r2;
+  const double sdr = (r0 + r1 + r2) * dr;
+  const double rr = 1.0 / (sdl * sdr);
+  const double il = rr * sdr, ir = rr * sdl;  // 1 / (S_l D_l), 1 / (S_r D_r)
+  const double wl = nl * dl * il, wr = nr * dr * ir;
+  const double al = 3.0 * l0 * l1 * l2 * il + eps;
+  const double ar = 3.0 * r0 * r1 * r2 * ir + eps;
+  double dq = q3 - q2;
+  dq = 0.5 * Mc(q2 - q1, dq);
+  const double alin = 2.0 * al * ar / (al + ar);
+  ql = alin * wl + (1.0 - alin) * (q2 + dq);
+  qr = alin * wr + (1.0 - alin) * (q2 - dq);
+}
+
+// CellStates with the sweep's WENO5Z.
+__device__ __forceinline__ void SweepStates(const double *q, unsigned c, unsigned st, int recon,
+                                            double &ql, double &qr) {
+  if (recon == 0)
+    Weno5ZSweep(LoadAt(q, c - 2 * st), LoadAt(q, c - st), LoadAt(q, c), LoadAt(q, c + st),
+                LoadAt(q, c + 2 * st), ql, qr);
+  else
+    LinearRecon(LoadAt(q, c - st), LoadAt(q, c), LoadAt(q, c + st), ql, qr);
+}
+
+// HllGen with the scalar components multiplied by the reciprocal the velocity components
+// already use (burgers_package.cpp:307-401 divides them by the same denominator).
+template <int V, class LS, class RS>
+__device__ __forceinline__ void SweepHll(LS l, RS r, int d, double *F) {
+  const double upl = l(d), upr = r(d);
+  const double sl = DMin(DMin(upl, upr), 0.0);
+  const double sr = DMax(DMax(upl, upr), 0.0);
+  const double islsr = 1.0 / (sr - sl + (sl * sr == 0.0));
+#pragma unroll
+  for (int n = 0; n < 3; ++n) {
+    const double ul = l(n), ur = r(n);
+    F[n] = 0.5 * (sr * ul * upl - sl * ur * upr + sl * sr * (ur - ul)) * islsr;
+  }
+#pragma unroll
+  for (int n = 3; n < V; ++n) {
+    const double ql = l(n), qr = r(n);
+    F[n] = (sr * upl * ql - sl * upr * qr + sl * sr * (qr - ql)) * islsr;
+  }
+}
+
 // flxcor_send of the block-sweep stage: four adjacent threads per coarse face cell of a patch,
 // one per fine face (p, q), evaluate the HLL flux of their face straight from the fine block's
 // state (the left state of face f is ql of cell f - 1, the right state qr of cell f, as
@@ -51,10 +134,10 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
     for (int n = 0; n < V; ++n) {
       double x, y;
-      CellStates(Ub + n * cs, o - st, st, recon, l[n], x);
-      CellStates(Ub + n * cs, o, st, recon, y, r[n]);
+      SweepStates(Ub + n * cs, o - st, st, recon, l[n], x);
+      SweepStates(Ub + n * cs, o, st, recon, y, r[n]);
     }
-    HllGen<V>([&](int n) { return l[n]; }, [&](int n) { return r[n]; }, P.d, F);
+    SweepHll<V>([&](int n) { return l[n]; }, [&](int n) { return r[n]; }, P.d, F);
   }
   const int lane0 = static_cast<int>(threadIdx.x) & ~3;
   const std::int64_t dst = P.dst + a + static_cast<std::int64_t>(c) * P.rows;
@@ -129,7 +212,7 @@ __global__ void __launch_bounds__(N * N, MINW)
       for (int t = threadIdx.x; t < V * NT; t += NT) {
         const int n = t / NT, u = t % NT;
         const unsigned c = 8u * static_cast<unsigned>((kc * E + NG + u / N) * E + NG + u % N);
-        CellStates(Ub + n * cs, c, sz, recon, sl[n * P + u], sr[n * P + u]);
+        SweepStates(Ub + n * cs, c, sz, recon, sl[n * P + u], sr[n * P + u]);
       }
     };
     // z faces: zl = ql of the cell below the current plane, zf = flux on the current low z face
@@ -142,7 +225,7 @@ __global__ void __launch_bounds__(N * N, MINW)
     __syncthreads();
     zstates(NG + k0);
     __syncthreads();
-    HllGen<V>([&](int n) { return zl[n]; }, [&](int n) { return sr[n * P + threadIdx.x]; }, 2, zf);
+    SweepHll<V>([&](int n) { return zl[n]; }, [&](int n) { return sr[n * P + threadIdx.x]; }, 2, zf);
     if (k0 == 0 && oz0 >= 0)
 #pragma unroll
       for (int n = 0; n < V; ++n) zf[n] = fcor[oz0 + tx + ty * N + n * comp];
@@ -157,7 +240,7 @@ __global__ void __launch_bounds__(N * N, MINW)
         const int n = t / P, u = t % P;
         const int a = u % R, row = u / R;
         const unsigned c = 8u * static_cast<unsigned>((kk * E + NG + row) * E + NG - 1 + a);
-        CellStates(Ub + n * cs, c, sx, recon, sl[t], sr[t]);
+        SweepStates(Ub + n * cs, c, sx, recon, sl[t], sr[t]);
       }
       __syncthreads();
       {
@@ -165,7 +248,7 @@ __global__ void __launch_bounds__(N * N, MINW)
         // the low face of row threadIdx.x; fluxes to LDS as [n][row][face]
         double F[V];
         const double *L = sl + ty * R + tx, *Rr = sr + ty * R + tx;
-        HllGen<V>([&](int n) { return L[n * P + 1]; }, [&](int n) { return Rr[n * P + 2]; }, 0,
+        SweepHll<V>([&](int n) { return L[n * P + 1]; }, [&](int n) { return Rr[n * P + 2]; }, 0,
                   F);
         if (tx == N - 1 && ox1 >= 0)
 #pragma unroll
@@ -174,7 +257,7 @@ __global__ void __launch_bounds__(N * N, MINW)
         for (int n = 0; n < V; ++n) fx[n * FP + ty * (N + 1) + tx + 1] = F[n];
         if (static_cast<int>(threadIdx.x) < N) {
           const int row = threadIdx.x;
-          HllGen<V>([&](int n) { return sl[n * P + row * R]; },
+          SweepHll<V>([&](int n) { return sl[n * P + row * R]; },
                     [&](int n) { return sr[n * P + row * R + 1]; }, 0, F);
           if (ox0 >= 0)
 #pragma unroll
@@ -196,14 +279,14 @@ __global__ void __launch_bounds__(N * N, MINW)
         const int n = t / P, u = t % P;
         const int i = u % N, a = u / N;
         const unsigned c = 8u * static_cast<unsigned>((kk * E + NG - 1 + a) * E + NG + i);
-        CellStates(Ub + n * cs, c, sy, recon, sl[t], sr[t]);
+        SweepStates(Ub + n * cs, c, sy, recon, sl[t], sr[t]);
       }
       __syncthreads();  // also: every thread has read its x fluxes
       {
         // every y face once, as [n][face][i]
         double F[V];
         const double *L = sl + ty * N + tx, *Rr = sr + ty * N + tx;
-        HllGen<V>([&](int n) { return L[n * P + N]; }, [&](int n) { return Rr[n * P + 2 * N]; },
+        SweepHll<V>([&](int n) { return L[n * P + N]; }, [&](int n) { return Rr[n * P + 2 * N]; },
                   1, F);
         if (ty == N - 1 && oy1 >= 0)
 #pragma unroll
@@ -212,7 +295,7 @@ __global__ void __launch_bounds__(N * N, MINW)
         for (int n = 0; n < V; ++n) fx[n * FP + (ty + 1) * N + tx] = F[n];
         if (static_cast<int>(threadIdx.x) < N) {
           const int col = threadIdx.x;
-          HllGen<V>([&](int n) { return sl[n * P + col]; },
+          SweepHll<V>([&](int n) { return sl[n * P + col]; },
                     [&](int n) { return sr[n * P + N + col]; }, 1, F);
           if (oy0 >= 0)
 #pragma unroll
@@ -233,7 +316,7 @@ __global__ void __launch_bounds__(N * N, MINW)
       __syncthreads();  // also: every thread has read its y fluxes
       {
         double Fhi[V];
-        HllGen<V>([&](int n) { return zl[n]; }, [&](int n) { return sr[n * P + threadIdx.x]; }, 2,
+        SweepHll<V>([&](int n) { return zl[n]; }, [&](int n) { return sr[n * P + threadIdx.x]; }, 2,
                   Fhi);
         if (k == N - 1 && oz1 >= 0)
 #pragma unroll
