@@ -29,8 +29,10 @@ namespace {
 // be0 * be1 * be2 gives the three 1 / be_k, one reciprocal of (S_l D_l)(S_r D_r) gives both
 // sides' 1 / (S D) (S = w0 + w1 + w2, D = g2 w0 w1 + g1 w0 w2 + g0 w1 w2, so that
 // ql = N_l / S = N_l D_l / (S_l D_l) and al = 3 w0 w1 w2 / (S_l D_l) + eps), and alin keeps
-// its own. The values are the reference's to rounding; the magnitudes stay far inside the
-// FP64 range (be_k >= eps, w_k >= g_k, and the w_k are bounded by g_k tau / eps).
+// its own. The values are the reference's to rounding. Range: be_k >= eps and w_k >= g_k,
+// and w_k <= g_k (1 + tau5 / eps), so (S_l D_l)(S_r D_r) <= ~w^6 stays finite while tau5 is
+// below ~1e36 (a jump in q of ~1e18 beside a flat stencil); the reference's own form
+// overflows at ~w^3, tau5 ~1e87.
 __device__ __forceinline__ void Weno5ZSweep(double q0, double q1, double q2, double q3,
                                             double q4, double &ql, double &qr) {
   const double a00 = 1.0 / 3.0, a01 = -7.0 / 6.0, a02 = 11.0 / 6.0;
