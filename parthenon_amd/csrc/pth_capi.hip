@@ -351,16 +351,29 @@ pth_status Mesh::BuildCache(MeshData &md, int bt) {
   // overlap (a face region contains the halves of its edge and corner regions), and every
   // coarse cell restricts to the same value whichever region asks for it; restrict each
   // block's union once, as disjoint boxes.
-  {
+  // Restriction after the set: the reference restricts every same-level and finer ghost
+  // region of a block with a coarser neighbour, "a superset of the boundaries that actually
+  // need to be restricted" (src/bvals/comms/bnd_info.cpp:409-411). The prolongation reads
+  // the coarse buffer only within one coarse cell of its own boxes (the minmod stencil), so
+  // on meshes whose active dimensions are all periodic the set is cut to those cells. With a
+  // physical boundary the coarse-buffer conditions can copy other halo cells into that
+  // reach, and the whole superset is kept.
+  bool all_periodic = true;
+  for (int d = 0; d < 3; ++d) all_periodic = all_periodic && (topo.periodic[d] || !topo.active[d]);
+  auto merge = [&](std::vector<Region> &host, const std::vector<Region> *reach) {
     std::vector<Region> merged;
     std::map<std::pair<double *, double *>, std::vector<const Region *>> by_block;
+    std::map<std::pair<double *, double *>, std::vector<const Region *>> reach_of;
     std::vector<std::pair<double *, double *>> order;
-    for (const Region &r : c.restrict_send.host) {
+    for (const Region &r : host) {
       auto key = std::make_pair(r.a, r.b);
       if (!by_block.count(key)) order.push_back(key);
       by_block[key].push_back(&r);
     }
+    if (reach)
+      for (const Region &r : *reach) reach_of[std::make_pair(r.a, r.b)].push_back(&r);
     const int E0 = ce.ent(0), E1 = ce.ent(1), E2 = ce.ent(2);
+    const int E[3] = {E0, E1, E2};
     std::vector<char> mask(static_cast<size_t>(E0) * E1 * E2);
     auto at = [&](int i, int j, int k) -> char & {
       return mask[(static_cast<size_t>(k) * E1 + j) * E0 + i];
@@ -368,10 +381,27 @@ pth_status Mesh::BuildCache(MeshData &md, int bt) {
     for (const auto &key : order) {
       const auto &rs = by_block[key];
       std::fill(mask.begin(), mask.end(), 0);
+      // 3 = wanted; with a reach, only the cells it covers become 1 (to restrict)
+      const char want = reach ? 3 : 1;
       for (const Region *r : rs)
         for (int k = r->lo[2]; k < r->lo[2] + r->n[2]; ++k)
           for (int j = r->lo[1]; j < r->lo[1] + r->n[1]; ++j)
-            for (int i = r->lo[0]; i < r->lo[0] + r->n[0]; ++i) at(i, j, k) = 1;
+            for (int i = r->lo[0]; i < r->lo[0] + r->n[0]; ++i) at(i, j, k) = want;
+      if (reach) {
+        auto it = reach_of.find(key);
+        if (it != reach_of.end())
+          for (const Region *p : it->second) {
+            int lo[3], hi[3];
+            for (int d = 0; d < 3; ++d) {
+              lo[d] = std::max(p->lo[d] - 1, 0);
+              hi[d] = std::min(p->lo[d] + p->n[d], E[d] - 1);
+            }
+            for (int k = lo[2]; k <= hi[2]; ++k)
+              for (int j = lo[1]; j <= hi[1]; ++j)
+                for (int i = lo[0]; i <= hi[0]; ++i)
+                  if (at(i, j, k) == 3) at(i, j, k) = 1;
+          }
+      }
       const Region &r0 = *rs.front();
       for (int k = 0; k < E2; ++k)
         for (int j = 0; j < E1; ++j)
@@ -404,8 +434,10 @@ pth_status Mesh::BuildCache(MeshData &md, int bt) {
             merged.push_back(MakeRegion(r0.a, r0.b, bx, r0.nc, r0.gid, fe, &ce));
           }
     }
-    c.restrict_send.host.swap(merged);
-  }
+    host.swap(merged);
+  };
+  merge(c.restrict_send.host, nullptr);
+  if (all_periodic) merge(c.restrict_set.host, &c.prolong.host);
   // Every element a kernel will touch must lie inside its array: pack/unpack boxes inside the
   // field brick, restriction children and prolongation stencils inside fine and coarse bricks.
   auto inside = [](const Region &r, const Extent &e, int halo_lo, int halo_hi, int scale,

@@ -116,14 +116,21 @@ def test_multilevel_random_data_matches_oracle(spec, fused):
     f = interior_uniform(m.field_shape("q0"), m.ng, seed=7, fill=0.0)
     c = np.zeros(m.coarse_shape("q0"))
     md.upload("q0", f)
-    md.upload("q0", c, which=1)
+    # coarse buffers start as NaN: a cell the prolongation reads that the restriction skipped
+    # would turn the prolongated ghosts into NaN
+    md.upload("q0", np.full(m.coarse_shape("q0"), np.nan), which=1)
     if fused:
         pa.exchange_fused(md)
     else:
         pa.exchange(md)
     o.exchange([f], coarse=[c], prolong=1)
     got, gotc = md.download("q0"), md.download("q0", which=1)
-    assert np.array_equal(gotc, c)           # restriction into coarse buffers: bit-exact
+    # restriction into coarse buffers: bit-exact on every cell written. On these periodic
+    # meshes the restriction after the set covers only the prolongation stencils' reach, a
+    # subset of the reference's (and the oracle's) halo restriction.
+    w = ~np.isnan(gotc)
+    assert np.array_equal(gotc[w], c[w])
+    print(f"coarse cells written: {w.mean():.3f} of the buffer", flush=True)
     mask = interior_mask(m)
     # copies into ghosts: bit-exact; prolongated ghosts within 1e-13 relative
     np.testing.assert_allclose(got, f, rtol=1e-13, atol=0)
