@@ -297,6 +297,16 @@ void orc_lr_to_flux(int n, const double *in, double *out) {
   }
 }
 
+// in: 4 per face (upl upr ql qr); out: the scalar flux with the face's Davis speeds
+void orc_scalar_flux(int n, const double *in, double *out) {
+  for (int i = 0; i < n; ++i) {
+    const double *a = in + 4 * i;
+    const double sl = std::min(std::min(a[0], a[1]), 0.0);
+    const double sr = std::max(std::max(a[0], a[1]), 0.0);
+    out[i] = qflux(a[0], a[1], a[2], a[3], sl, sr);
+  }
+}
+
 // ---- Burgers benchmark ----
 struct OBurgers {
   Burgers b;

@@ -95,6 +95,11 @@ inline void lr_to_flux(double uxl, double uxr, double uyl, double uyr, double uz
   fuz = 0.5 * (sr * uzl * upl - sl * uzr * upr + sl * sr * (uzr - uzl)) * islsr;
 }
 
+// The scalar (passive) flux of burgers_package.cpp:335-340 with the face's sl, sr.
+inline double qflux(double upl, double upr, double ql, double qr, double sl, double sr) {
+  return (sr * upl * ql - sl * upr * qr + sl * sr * (qr - ql)) / (sr - sl + (sl * sr == 0.0));
+}
+
 // State of one Burgers container ("base", "1") : U and its three face-flux arrays.
 struct Container {
   Field U;
@@ -220,12 +225,9 @@ struct Burgers {
               u(c.flux[d], b, 1, k, j, i) = f1;
               u(c.flux[d], b, 2, k, j, i) = f2;
               const double upl = u(L, b, d, k, j, i), upr = u(R, b, d, k, j, i);
-              for (int n = 3; n < V; ++n) {
-                const double ql = u(L, b, n, k, j, i), qr = u(R, b, n, k, j, i);
+              for (int n = 3; n < V; ++n)
                 u(c.flux[d], b, n, k, j, i) =
-                    (sr * upl * ql - sl * upr * qr + sl * sr * (qr - ql)) /
-                    (sr - sl + (sl * sr == 0.0));
-              }
+                    qflux(upl, upr, u(L, b, n, k, j, i), u(R, b, n, k, j, i), sl, sr);
             }
           }
         }

@@ -21,7 +21,9 @@ DP = C.POINTER(C.c_double)
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(ROOT, "oracle", "liboracle.so")
+        # PTH_ORACLE_LIB points the tests at another build of the restatement (a planted
+        # mistake, to show that the pins catch it)
+        path = os.environ.get("PTH_ORACLE_LIB") or os.path.join(ROOT, "oracle", "liboracle.so")
         if not os.path.exists(path):
             raise RuntimeError("oracle/liboracle.so is not built; run __graft_entry__.build()")
         L = C.CDLL(path)
@@ -49,6 +51,7 @@ def lib():
             "orc_weno5z": (None, [I, DP, DP, DP]),
             "orc_linear": (None, [I, DP, DP, DP]),
             "orc_lr_to_flux": (None, [I, DP, DP]),
+            "orc_scalar_flux": (None, [I, DP, DP]),
             "orc_burgers_new": (P, [P, I, D, I]),
             "orc_burgers_free": (None, [P]),
             "orc_burgers_pgen_sinusoid": (None, [P, C.c_uint64]),

@@ -219,3 +219,159 @@ def test_flux_correction_overwrites_each_coarse_face_with_the_fine_face_average(
     assert e.max() < 1e-14
     m, b, _, _ = refined_burgers(False, 1)
     assert coarse_fine_face_errors(m, b).max() > 1e-6
+
+
+def qflux(a):
+    a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1, 4)
+    out = np.empty(len(a))
+    lib().orc_scalar_flux(len(a), dptr(a), out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
+
+
+def textbook_hll(fl, fr, ul, ur, sl, sr):
+    """Harten-Lax-van Leer flux for sl < 0 < sr (Toro, Riemann Solvers and Numerical Methods
+    for Fluid Dynamics, 3rd ed., eq. 10.21): (sr F_L - sl F_R + sl sr (U_R - U_L)) / (sr - sl)."""
+    return (sr * fl - sl * fr + sl * sr * (ur - ul)) / (sr - sl)
+
+
+def straddling_states(n, seed):
+    r = np.random.default_rng(seed)
+    a = r.uniform(-2, 2, size=(n, 8))
+    # normal velocities of opposite signs, in either order, so that sl < 0 < sr
+    up = r.uniform(0.05, 2, size=(n, 2)) * np.where(r.random((n, 1)) < 0.5, [1, -1], [-1, 1])
+    a[:, 6:8] = up
+    return a
+
+
+def test_hll_is_the_textbook_flux_when_the_speeds_straddle_zero():
+    # burgers_package.hpp:33-43 and burgers_package.cpp:335-340: the Davis speeds sl = min(upl,
+    # upr, 0), sr = max(upl, upr, 0); with sl < 0 < sr the guard term is 0 and the fluxes are
+    # HLL of the Burgers flux: velocity n carries F = 1/2 u_n u_p with the 1/2 on the whole
+    # expression (the diffusive term included), a passive scalar q carries F = q u_p.
+    a = straddling_states(2000, 11)
+    upl, upr = a[:, 6], a[:, 7]
+    sl, sr = np.minimum(upl, upr), np.maximum(upl, upr)
+    assert np.all(sl < 0) and np.all(sr > 0)
+    f = hll(a)
+    np.testing.assert_array_equal(f[:, 0], sl)
+    np.testing.assert_array_equal(f[:, 1], sr)
+    for n in range(3):
+        ul, ur = a[:, 2 * n], a[:, 2 * n + 1]
+        exp = 0.5 * textbook_hll(ul * upl, ur * upr, ul, ur, sl, sr)
+        # the terms are O(1): fluxes that cancel to ~0 carry ~1e-15 of rounding either way
+        np.testing.assert_allclose(f[:, 2 + n], exp, rtol=1e-13, atol=1e-14)
+    q = np.random.default_rng(12).uniform(-3, 3, size=(2000, 2))
+    g = qflux(np.column_stack([upl, upr, q]))
+    exp = textbook_hll(q[:, 0] * upl, q[:, 1] * upr, q[:, 0], q[:, 1], sl, sr)
+    np.testing.assert_allclose(g, exp, rtol=1e-13, atol=1e-14)
+
+
+def test_hll_one_signed_speeds_follow_the_guard():
+    # With both normal velocities of one sign, sl * sr == 0 and the reference's denominator is
+    # sr - sl + 1: the velocity flux is 1/2 sr u_n(L) upl / (sr + 1) for upl, upr >= 0 and
+    # -1/2 sl u_n(R) upr / (1 - sl) for upl, upr <= 0; scalars the same without the 1/2.
+    # Hand-worked cases (exact in binary):
+    #   upl = 2, upr = 1:  sl = 0, sr = 2, u_x(L) = 3 -> 1/2 * 2 * 3 * 2 / 3 = 2; q(L) = 3 -> 4
+    #   upl = -1, upr = -3: sl = -3, sr = 0, u_x(R) = 5 -> -1/2 * -3 * 5 * -3 / 4 = -45/8;
+    #                       q(R) = 5 -> -45/4
+    #   upl = upr = 0: sl = sr = 0 -> every flux 0
+    #   upl = 0, upr = 1: sl = 0, sr = 1 -> upwind value 1/2 * 1 * u(L) * 0 / 2 = 0
+    cases = np.array([[3, 7, 1, 1, 1, 1, 2, 1],
+                      [7, 5, 1, 1, 1, 1, -1, -3],
+                      [3, 7, 1, 1, 1, 1, 0, 0],
+                      [3, 7, 1, 1, 1, 1, 0, 1]], dtype=np.float64)
+    f = hll(cases)
+    assert f[0, 2] == 2.0 and f[1, 2] == -45.0 / 8.0 and f[2, 2] == 0.0 and f[3, 2] == 0.0
+    assert f[0, 3] == 0.5 * 2 * 1 * 2 / 3
+    g = qflux(np.array([[2, 1, 3, 7], [-1, -3, 7, 5], [0, 0, 3, 7], [0, 1, 3, 7]], dtype=float))
+    assert g[0] == 4.0 and g[1] == -45.0 / 4.0 and g[2] == 0.0 and g[3] == 0.0
+    # random one-signed states against the closed forms
+    r = np.random.default_rng(13)
+    a = r.uniform(-2, 2, size=(1000, 8))
+    a[:500, 6:8] = r.uniform(0, 2, size=(500, 2))
+    a[500:, 6:8] = -r.uniform(0, 2, size=(500, 2))
+    upl, upr = a[:, 6], a[:, 7]
+    sl, sr = np.minimum(np.minimum(upl, upr), 0), np.maximum(np.maximum(upl, upr), 0)
+    f = hll(a)
+    pos = np.arange(1000) < 500
+    for n in range(3):
+        ul, ur = a[:, 2 * n], a[:, 2 * n + 1]
+        exp = np.where(pos, 0.5 * sr * ul * upl / (sr + 1), -0.5 * sl * ur * upr / (1 - sl))
+        np.testing.assert_allclose(f[:, 2 + n], exp, rtol=1e-13, atol=1e-15)
+    q = r.uniform(-3, 3, size=(1000, 2))
+    g = qflux(np.column_stack([upl, upr, q]))
+    exp = np.where(pos, sr * upl * q[:, 0] / (sr + 1), -sl * upr * q[:, 1] / (1 - sl))
+    np.testing.assert_allclose(g, exp, rtol=1e-13, atol=1e-15)
+
+
+def test_calculate_fluxes_on_a_uniform_state_is_the_guarded_flux():
+    # CalculateFluxes over whole blocks: with a uniform state every face has ql = qr = the
+    # state, so every face flux is the one-signed closed form of its direction's velocity,
+    # F_n = 1/2 |u_d| u_n u_d / (|u_d| + 1) (velocities), |u_d| q u_d / (|u_d| + 1) (scalars)
+    # for u_d > 0, and -u_d ... / (1 - u_d) for u_d < 0; stored at every face of the block.
+    m, b = burgers(nx=(16, 16, 16), block=(8, 8, 8), ns=2)
+    U = b.get(0).reshape(m.field_shape(b.V))
+    state = (0.75, -0.5, 0.25, 2.0, -1.5)
+    for c in range(b.V):
+        U[:, c] = state[c]
+    b.set(0, U)
+    lib().orc_burgers_fluxes(b.h)
+    g, n = m.ng[0], m.n[0]
+    for d in range(3):
+        F = b.get(4 + d).reshape(U.shape)
+        ud = state[d]
+        s = abs(ud) / (abs(ud) + 1.0)
+        hi = [g + n] * 3
+        hi[d] += 1  # faces lo .. hi of the flux direction
+        sub = F[:, :, g:hi[2], g:hi[1], g:hi[0]]
+        for c in range(b.V):
+            exp = (0.5 if c < 3 else 1.0) * s * state[c] * ud
+            np.testing.assert_allclose(sub[:, c], exp, rtol=1e-15, atol=0)
+
+
+def octant_boxes(lo=-0.5, hi=0.5):
+    mid = 0.5 * (lo + hi)
+    out = []
+    for s1 in (0, 1):           # the reference's octant order: x outer, then y, then z
+        for s2 in (0, 1):
+            for s3 in (0, 1):
+                out.append(([mid if s else lo for s in (s1, s2, s3)],
+                            [hi if s else mid for s in (s1, s2, s3)]))
+    return out
+
+
+def octant_constant_field(m, V, coef):
+    """U = coef[o, v] on every cell whose centre lies in octant o (octant 4 s1 + 2 s2 + s3)."""
+    U = np.zeros(m.field_shape(V))
+    g, n = m.ng[0], m.n[0]
+    for gid in range(m.nblocks):
+        bk = m.block(gid)
+        xc = [bk["xmin"][d] + (np.arange(-g, n + g) + 0.5) * (bk["xmax"][d] - bk["xmin"][d]) / n
+              for d in range(3)]
+        s = [(x > 0).astype(int) for x in xc]
+        o = 4 * s[0][None, None, :] + 2 * s[1][None, :, None] + s[2][:, None, None]
+        for v in range(V):
+            U[gid, v] = coef[o, v]
+    return U
+
+
+@pytest.mark.parametrize("refined", [False, True])
+def test_mass_history_octant_constant_fields(refined):
+    # burgers_package.cpp:96-135, 406-439: mass_o = sum over cells of mask_o(x_c) U^2 vol /
+    # mesh_vol. With U = c[o, v] on octant o every octant holds 1/8 of the unit mesh volume at
+    # any refinement, so mass_o = sum_v c[o, v]^2 / 8, in the order x outer, then y, then z.
+    boxes = [((-0.1, 0.1, -0.1, 0.1, -0.1, 0.1), 1)] if refined else []
+    m = OracleMesh((32, 32, 32), (8, 8, 8), 4, boxes=boxes)
+    b = OracleBurgers(m, 2, 0.4, 0)
+    coef = np.array([[1, 2, 3, 4, 5], [-1, 0.5, 0, 2, 1], [3, 1, 1, 1, 1], [0, 0, 0, 0, 4],
+                     [2, 2, 2, 2, 2], [0.25, -3, 1, 0, 1], [1, 1, -1, -1, 1], [5, 0, 0, 0, 0]],
+                    dtype=np.float64)
+    b.set(0, octant_constant_field(m, b.V, coef))
+    got = [lib().orc_burgers_mass_history(b.h, (C.c_double * 3)(*lo), (C.c_double * 3)(*hi), 1.0)
+           for lo, hi in octant_boxes()]
+    exp = (coef ** 2).sum(axis=1) / 8.0
+    np.testing.assert_allclose(got, exp, rtol=1e-13)
+    # the mesh volume divides: a domain volume of 2 halves every mass
+    half = lib().orc_burgers_mass_history(b.h, (C.c_double * 3)(*octant_boxes()[0][0]),
+                                          (C.c_double * 3)(*octant_boxes()[0][1]), 2.0)
+    assert abs(half - exp[0] / 2) < 1e-13 * exp[0]

@@ -89,3 +89,50 @@ def test_sphere_mesh_ghosts_are_all_filled():
     m.exchange([f], coarse=[cf], prolong=1)
     # constant data: every ghost cell filled, restriction and prolongation keep the constant
     assert np.array_equal(f, np.ones_like(f))
+
+
+def restrict_interior(m, ncomp, data):
+    from oracle_lib import dptr, lib
+    coarse = np.full(m.coarse_shape(ncomp), np.nan)
+    lib().orc_restrict_interior(m.h, ncomp, dptr(data), dptr(coarse))
+    return coarse
+
+
+def brute_force_restriction(m, data):
+    """2x2x2 averages of every block interior by direct indexing: coarse interior cell
+    (ck, cj, ci) is the mean of fine interior cells 2ck..2ck+1, 2cj..2cj+1, 2ci..2ci+1, summed
+    in the pairwise order of pr_ops.hpp:157-162 (off-centred pairs first, j pairs inside i
+    pairs inside k halves). Cell volumes are equal powers of two here, so the volume weights
+    cancel exactly."""
+    g, n, cg, cn = m.ng[0], m.n[0], m.cng[0], m.cn[0]
+    f = data[:, :, g:g + n, g:g + n, g:g + n]
+    t = {(a, b, c): f[:, :, a::2, b::2, c::2] for a in (0, 1) for b in (0, 1) for c in (0, 1)}
+    s = (((t[0, 0, 0] + t[0, 1, 0]) + (t[0, 0, 1] + t[0, 1, 1])) +
+         ((t[1, 0, 0] + t[1, 1, 0]) + (t[1, 0, 1] + t[1, 1, 1])))
+    return s / 8.0, (slice(None), slice(None), slice(cg, cg + cn), slice(cg, cg + cn),
+                     slice(cg, cg + cn))
+
+
+@pytest.mark.parametrize("dyadic", [True, False])
+def test_restriction_equals_brute_force_average(dyadic):
+    # cfg-2 block shape (16^3, g = 4) on a two-level mesh, 3 components: bit-exact against the
+    # direct 2x2x2 average; with dyadic data (multiples of 2^-12 in [-4, 4]) the sum is exact in
+    # any order, so the order-free numpy mean must match too
+    m = OracleMesh((32, 32, 32), (16, 16, 16), 4, boxes=[((-0.2, 0.2, -0.2, 0.2, -0.2, 0.2), 1)])
+    r = np.random.default_rng(5)
+    shape = m.field_shape(3)
+    data = (np.round(r.uniform(-4, 4, shape) * 4096) / 4096) if dyadic else r.normal(size=shape)
+    got = restrict_interior(m, 3, data)
+    exp, sl = brute_force_restriction(m, data)
+    assert np.array_equal(got[sl], exp)
+    if dyadic:
+        g, n = m.ng[0], m.n[0]
+        f = data[:, :, g:g + n, g:g + n, g:g + n]
+        mean = f.reshape(f.shape[0], 3, n // 2, 2, n // 2, 2, n // 2, 2).mean(axis=(3, 5, 7))
+        assert np.array_equal(got[sl], mean)
+    else:
+        # the pairwise order is mirror-symmetric: reversing the fine cells along every axis
+        # reverses the coarse cells bit for bit
+        rev = np.ascontiguousarray(data[:, :, ::-1, ::-1, ::-1])
+        got2 = restrict_interior(m, 3, rev)
+        assert np.array_equal(got2[sl], got[sl][:, :, ::-1, ::-1, ::-1])
