@@ -203,8 +203,16 @@ pth_status pth_burgers_form(pth_mesh *m, int *sweep, int *overlap);
  * made. Results do not depend on the choice. */
 pth_status pth_burgers_tuning(pth_mesh *m, int planes[4]);
 pth_status pth_burgers_set_tuning(pth_mesh *m, const int planes[4]);
-/* Synchronous read of the device clock: current dt and simulated time. */
+/* Synchronous read of the device clock: the driver's dt (tm.dt) and simulated time. */
 pth_status pth_burgers_clock(pth_mesh *m, double *dt, double *time);
+/* <parthenon/time> limits of EvolutionDriver (src/driver/driver.hpp:66-94; applied as in
+ * SetGlobalTimeStep, driver.cpp:210-269): lim = {tlim, dt_init, dt_user, dt_force, dt_floor,
+ * dt_ceil}; the reference defaults are {inf, DBL_MAX, DBL_MAX, -DBL_MAX, DBL_MIN, DBL_MAX},
+ * dt_init_force 0. The last cycle before tlim is shortened to end on it. The device cannot
+ * end the host's loop, so a cycle run after time >= tlim (where SimTime::KeepGoing,
+ * src/basic_types.hpp:241, would have stopped the reference) advances nothing: the state,
+ * time and dt stay as they are. Takes effect from the next pth_burgers_init or cycle. */
+pth_status pth_burgers_set_time_limits(pth_mesh *m, const double lim[6], int dt_init_force);
 /* Waits for the driver's stream. */
 pth_status pth_burgers_synchronize(pth_mesh *m);
 
@@ -292,8 +300,11 @@ int pth_gmg_history(const pth_gmg *g, double *out, int max);
 
 /* Serving loop of independent Poisson solves (no counterpart in the reference, which holds
  * one Mesh per process, src/parthenon_manager.cpp:170-173). Requests arrive at given times;
- * they are admitted while the device-memory budget and the active-slot limit allow; every
- * outer iteration runs the same V-cycle stage of all active solves as one launch. */
+ * they are admitted in arrival order while a slot is free (max_active) and the device's free
+ * HBM (hipMemGetInfo) holds the request's hierarchy plus a reserve (default 1 GiB); a
+ * budget_bytes > 0 also caps the bytes the server holds (<= 0: no cap). Pooled hierarchies of
+ * other sizes are freed first when memory is short. Every outer iteration runs the same
+ * V-cycle stage of all active solves as one launch. */
 typedef struct pth_gmg_server pth_gmg_server;
 pth_gmg_server *pth_gmg_server_create(int precision, double omega, int pre, int post, double tol,
                                       int max_iters, int64_t budget_bytes, int max_active,
@@ -303,6 +314,8 @@ pth_status pth_gmg_server_submit(pth_gmg_server *s, int n, uint64_t seed, double
                                  int *id);
 pth_status pth_gmg_server_set_inner_cycles(pth_gmg_server *s, int k);
 pth_status pth_gmg_server_set_column_sweeps(pth_gmg_server *h, int on);
+/* Free HBM (bytes) that admissions leave unallocated (default 1 GiB). */
+pth_status pth_gmg_server_set_hbm_reserve(pth_gmg_server *s, int64_t bytes);
 /* serve every submitted request; realtime: honour arrival times on the wall clock */
 pth_status pth_gmg_server_run(pth_gmg_server *s, int realtime);
 /* out: arrival, admission, completion (s, server clock), latency (s), final relative residual */
@@ -318,6 +331,9 @@ pth_status pth_gmg_server_stats(const pth_gmg_server *s, int64_t out[3], double 
  * records and returns the number of lines. No counterpart in the reference's FFI. */
 pth_status pth_profile_enable(int enable);
 int pth_profile_report(char *buf, int len);
+
+/* Free and total memory of the current device (hipMemGetInfo); no counterpart in the reference. */
+pth_status pth_device_memory(int64_t *free_bytes, int64_t *total_bytes);
 
 const char *pth_last_error(void);
 const char *pth_version(void);

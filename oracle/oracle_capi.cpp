@@ -379,9 +379,28 @@ void orc_burgers_pgen_vibe(void *h) {
 
 void orc_burgers_initialize(void *h) { static_cast<OBurgers *>(h)->b.Initialize(); }
 
-void orc_burgers_cycles(void *h, int n) {
-  for (int q = 0; q < n; ++q) static_cast<OBurgers *>(h)->b.Cycle();
+// EvolutionDriver::Execute's loop (driver.cpp:98-151): at most n cycles while KeepGoing().
+int orc_burgers_cycles(void *h, int n) {
+  Burgers &B = static_cast<OBurgers *>(h)->b;
+  int q = 0;
+  for (; q < n && B.KeepGoing(); ++q) B.Cycle();
+  return q;
 }
+
+// <parthenon/time> tlim, dt_init, dt_init_force, dt_user, dt_force, dt_floor, dt_ceil, nlim;
+// call before orc_burgers_initialize.
+void orc_burgers_set_time_limits(void *h, const double *v, int dt_init_force, int nlim) {
+  Burgers &B = static_cast<OBurgers *>(h)->b;
+  B.lim.tlim = v[0];
+  B.lim.dt_init = v[1];
+  B.lim.dt_init_force = dt_init_force != 0;
+  B.lim.dt_user = v[2];
+  B.lim.dt_force = v[3];
+  B.lim.dt_floor = v[4];
+  B.lim.dt_ceil = v[5];
+  B.lim.nlim = nlim;
+}
+int orc_burgers_ncycle(void *h) { return static_cast<OBurgers *>(h)->b.ncycle; }
 
 // Stage-level entry points for kernel parity tests on the "base" container.
 void orc_burgers_fluxes(void *h) {

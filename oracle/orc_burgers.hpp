@@ -121,6 +121,17 @@ struct Burgers {
   std::vector<Field> fb, fo;    // field lists for the exchanges
   double dt = std::numeric_limits<double>::max(), time = 0.0;
   int ncycle = 0;
+  // <parthenon/time> limits with the defaults of EvolutionDriver (src/driver/driver.hpp:66-94)
+  struct Limits {
+    double tlim = std::numeric_limits<double>::infinity();
+    double dt_init = std::numeric_limits<double>::max();
+    bool dt_init_force = false;
+    double dt_user = std::numeric_limits<double>::max();
+    double dt_force = std::numeric_limits<double>::lowest();
+    double dt_floor = std::numeric_limits<double>::min();
+    double dt_ceil = std::numeric_limits<double>::max();
+    int nlim = -1;
+  } lim;
 
   std::int64_t slab() const { return m->cellbounds.size(); }
 
@@ -325,10 +336,25 @@ struct Burgers {
     SetGlobalTimeStep(EstimateTimestep(base));
   }
 
-  void SetGlobalTimeStep(double block_dt) {  // driver.cpp:210-235 with default limits
-    if (dt < 0.1 * std::numeric_limits<double>::max()) dt *= 2.0;
-    dt = std::min(dt, block_dt);
+  // EvolutionDriver::SetGlobalTimeStep (driver.cpp:210-269) with dt_factor 2; the dt_min /
+  // dt_max cycle-limit checks only abort and change no value.
+  void SetGlobalTimeStep(double block_dt) {
+    if (lim.dt_force > 0.0) {
+      dt = lim.dt_force;
+    } else if (ncycle == 0 && lim.dt_init_force && lim.dt_init > 0.0) {
+      dt = lim.dt_init;
+    } else {
+      if (dt < 0.1 * std::numeric_limits<double>::max()) dt *= 2.0;
+      if (ncycle == 0) dt = std::min(dt, lim.dt_init);
+      dt = std::min(dt, block_dt);
+      dt = std::min(dt, lim.dt_user);
+      dt = std::max(lim.dt_floor, std::min(dt, lim.dt_ceil));
+    }
+    if (time < lim.tlim && (lim.tlim - time) < dt) dt = lim.tlim - time;
   }
+
+  // SimTime::KeepGoing (src/basic_types.hpp:241): time < tlim and ncycle < nlim
+  bool KeepGoing() const { return time < lim.tlim && (lim.nlim < 0 || ncycle < lim.nlim); }
 
   // One RK2 cycle in the task order of burgers_driver.cpp:53-148: fluxes, flux correction
   // (multilevel), divergence, averaging, update, exchange (+ restriction / prolongation),

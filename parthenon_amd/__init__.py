@@ -111,6 +111,7 @@ _SIGNATURES = {
     "pth_burgers_init": (_I, [_P, _D, _I]),
     "pth_burgers_run_cycles": (_I, [_P, _I, _I]),
     "pth_burgers_clock": (_I, [_P, _DP, _DP]),
+    "pth_burgers_set_time_limits": (_I, [_P, _DP, _I]),
     "pth_burgers_set_overlap": (_I, [_P, _I]),
     "pth_burgers_set_sweep": (_I, [_P, _I]),
     "pth_burgers_form": (_I, [_P, C.POINTER(_I), C.POINTER(_I)]),
@@ -137,6 +138,8 @@ _SIGNATURES = {
     "pth_gmg_set_inner_cycles": (_I, [_P, _I]),
     "pth_gmg_set_column_sweeps": (_I, [_P, _I]),
     "pth_gmg_server_set_column_sweeps": (_I, [_P, _I]),
+    "pth_gmg_server_set_hbm_reserve": (_I, [_P, C.c_int64]),
+    "pth_device_memory": (_I, [C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "pth_gmg_server_set_inner_cycles": (_I, [_P, _I]),
     "pth_gmg_history": (_I, [_P, _DP, _I]),
     "pth_gmg_server_create": (_P, [_I, _D, _I, _I, _D, _I, _I64, _I, _I]),
@@ -477,6 +480,13 @@ class Update:  # src/interface/update.hpp
     @staticmethod
     def UpdateIndependentData(md_in, dudt, dt, out):
         return Update.WeightedSumData(md_in, dudt, 1.0, dt, out)
+
+
+def device_memory():
+    """(free, total) bytes of the current device's memory."""
+    f, t = C.c_int64(), C.c_int64()
+    _check(lib().pth_device_memory(C.byref(f), C.byref(t)), "device_memory")
+    return f.value, t.value
 
 
 def profile_enable(on=True):

@@ -6,10 +6,11 @@ library exchange order (AddBoundaryExchangeTasks, src/bvals/comms/boundary_commu
 406-445; the shipped benchmark never prolongates inside a stage) and
 the low-storage RK2 integrator (src/time_integration/low_storage_integrator.cpp:67-83, stage
 names "base", "1", "base") and EvolutionDriver's time-step control
-(src/driver/driver.cpp:194-235): the first dt is the mesh estimate, later ones are
-min(2 * dt_prev, estimate).
+(src/driver/driver.cpp:210-269): the first dt is the mesh estimate, later ones are
+min(2 * dt_prev, estimate), within the <parthenon/time> limits and cut to end on tlim.
 """
 import math
+import sys
 
 from . import (ApplyBoundaryConditionsOnCoarseOrFineMD, BoundaryType,
                LoadAndSendFluxCorrections, Mesh, Metadata, ProlongateBounds, Refinement,
@@ -18,6 +19,38 @@ from . import (ApplyBoundaryConditionsOnCoarseOrFineMD, BoundaryType,
                TaskStatus, Update, burgers_package)
 
 RK2_BETA = (1.0, 0.5)
+
+
+class TimeLimits:
+    """<parthenon/time> tlim, nlim and dt bounds with EvolutionDriver's defaults
+    (src/driver/driver.hpp:66-94)."""
+
+    def __init__(self, tlim=math.inf, nlim=-1, dt_init=sys.float_info.max, dt_init_force=False,
+                 dt_user=sys.float_info.max, dt_force=-sys.float_info.max,
+                 dt_floor=sys.float_info.min, dt_ceil=sys.float_info.max):
+        self.tlim, self.nlim = tlim, nlim
+        self.dt_init, self.dt_init_force = dt_init, dt_init_force
+        self.dt_user, self.dt_force = dt_user, dt_force
+        self.dt_floor, self.dt_ceil = dt_floor, dt_ceil
+
+
+def set_global_time_step(dt, block_dt, time, ncycle, lim):
+    """EvolutionDriver::SetGlobalTimeStep (driver.cpp:210-269, dt_factor 2): the new dt."""
+    if lim.dt_force > 0.0:
+        dt = lim.dt_force
+    elif ncycle == 0 and lim.dt_init_force and lim.dt_init > 0.0:
+        dt = lim.dt_init
+    else:
+        if dt < 0.1 * sys.float_info.max:
+            dt *= 2.0
+        if ncycle == 0:
+            dt = min(dt, lim.dt_init)
+        dt = min(dt, block_dt)
+        dt = min(dt, lim.dt_user)
+        dt = max(lim.dt_floor, min(dt, lim.dt_ceil))
+    if time < lim.tlim and (lim.tlim - time) < dt:
+        dt = lim.tlim - time
+    return dt
 
 
 def make_burgers_mesh(nx=(64, 64, 64), block=(16, 16, 16), nghost=4, num_scalars=4,
@@ -125,18 +158,34 @@ class DeviceBurgersDriver:
         self._check(self._lib().pth_burgers_clock(self.mesh._h, C.byref(dt), C.byref(t)), "clock")
         return dt.value, t.value
 
+    def set_time_limits(self, tlim=math.inf, dt_init=sys.float_info.max, dt_init_force=False,
+                        dt_user=sys.float_info.max, dt_force=-sys.float_info.max,
+                        dt_floor=sys.float_info.min, dt_ceil=sys.float_info.max):
+        """<parthenon/time> tlim and dt bounds (EvolutionDriver, src/driver/driver.hpp:66-94).
+        Cycles queued after time reaches tlim leave the state and the clock unchanged."""
+        C = self._C
+        v = (C.c_double * 6)(tlim, dt_init, dt_user, dt_force, dt_floor, dt_ceil)
+        self._check(self._lib().pth_burgers_set_time_limits(self.mesh._h, v, int(dt_init_force)),
+                    "set_time_limits")
+
 
 class BurgersDriver:
-    def __init__(self, mesh, cfl=0.4, recon=burgers_package.WENO5):
+    def __init__(self, mesh, cfl=0.4, recon=burgers_package.WENO5, limits=None):
         self.mesh = mesh
         self.cfl = cfl
         self.recon = recon
+        self.limits = limits or TimeLimits()
         self.base = mesh.mesh_data("base")
         self.one = mesh.mesh_data("1")
         self.dudt = mesh.mesh_data("dUdt")
-        self.dt = math.inf
+        self.dt = sys.float_info.max  # <parthenon/time> dt default
         self.time = 0.0
         self.ncycle = 0
+
+    def keep_going(self):
+        """SimTime::KeepGoing (src/basic_types.hpp:241)."""
+        lim = self.limits
+        return self.time < lim.tlim and (lim.nlim < 0 or self.ncycle < lim.nlim)
 
     def _exchange(self, md):
         any_ = BoundaryType.any
@@ -164,7 +213,8 @@ class BurgersDriver:
         StartReceiveBoundBufs(self.base, BoundaryType.any)
         self._exchange(self.base)
         burgers_package.CalculateDerived(self.base)
-        self.dt = burgers_package.EstimateTimestepMesh(self.base, self.cfl)
+        self.dt = set_global_time_step(self.dt, burgers_package.EstimateTimestepMesh(
+            self.base, self.cfl), self.time, self.ncycle, self.limits)
 
     def stage(self, stage):
         beta = RK2_BETA[stage - 1]
@@ -191,7 +241,16 @@ class BurgersDriver:
         new_dt = self.stage(2)
         self.ncycle += 1
         self.time += self.dt
-        self.dt = min(2.0 * self.dt, new_dt)
+        self.dt = set_global_time_step(self.dt, new_dt, self.time, self.ncycle, self.limits)
+
+    def run(self, n):
+        """EvolutionDriver::Execute's loop: at most n cycles while keep_going(); returns the
+        number run."""
+        q = 0
+        while q < n and self.keep_going():
+            self.cycle()
+            q += 1
+        return q
 
     def adaptive_cycle(self):
         """One cycle, then Refinement::Tag (burgers_driver.cpp:139-145) and
@@ -203,5 +262,6 @@ class BurgersDriver:
         if not self.mesh.adapt():
             return False
         burgers_package.CalculateDerived(self.base)
-        self.dt = min(2.0 * dt_used, burgers_package.EstimateTimestepMesh(self.base, self.cfl))
+        self.dt = set_global_time_step(dt_used, burgers_package.EstimateTimestepMesh(
+            self.base, self.cfl), self.time, self.ncycle, self.limits)
         return True

@@ -726,16 +726,21 @@ __global__ void __launch_bounds__(64)
   out[threadIdx.x] = v;
 }
 
-// Final CFL minimum and EvolutionDriver::SetGlobalTimeStep on device (driver.cpp:210-235):
-// dtvar = {dt, time, estimate, dt of the last cycle}. advance 1 (end of a cycle): time += dt,
-// the cycle's dt is kept, dt = min(2 dt, cfl * min); advance 0 (start): dt = min(2 dt, est)
-// with dt still at its initial maximum; advance 2 (after a remesh): the estimate of the new
-// blocks replaces the old one, dt = min(2 dt_last, cfl * min).
+// Final CFL minimum and EvolutionDriver::SetGlobalTimeStep on device (driver.cpp:210-269,
+// dt_factor 2), with SimTime::KeepGoing's stop at tlim (basic_types.hpp:241).
+// dtvar = {dt of the next cycle, time, estimate, dt of the last cycle, tm.dt}.
+// advance 0 (start, ncycle 0): tm.dt is still the initial maximum and dt_init applies;
+// advance 1 (end of a cycle): time += dt and the cycle's dt is kept before the rule;
+// advance 2 (after a remesh): the estimate of the new blocks replaces the old one and the rule
+// starts again from the dt of the last cycle.
+// The loop of EvolutionDriver::Execute ends once time >= tlim. A cycle queued after that runs
+// with dt = 0 (slot 0), which leaves the state, the time and the clock unchanged, so a host
+// loop may queue cycles without reading the clock; slot 4 keeps the reference's tm.dt.
 // One workgroup of kClockWG threads: strided partial minima, wavefront shuffles, then LDS.
 constexpr int kClockWG = 1024;
 __global__ void __launch_bounds__(kClockWG)
     ClockKernel(const double *__restrict__ partial, int n, double cfl, double *dtvar,
-                int advance) {
+                int advance, ClockLimits L) {
   __shared__ double wmin[kClockWG / 64];
   double m = DBL_MAX;
   for (int t = threadIdx.x; t < n; t += kClockWG) m = DMin(m, partial[t]);
@@ -746,13 +751,28 @@ __global__ void __launch_bounds__(kClockWG)
     m = wmin[0];
     for (int w = 1; w < kClockWG / 64; ++w) m = DMin(m, wmin[w]);
     const double est = cfl * m;
-    double dt = advance == 2 ? dtvar[3] : dtvar[0];
+    double time = dtvar[1];
     if (advance == 1) {
-      dtvar[1] = dtvar[1] + dt;
-      dtvar[3] = dt;
+      if (!(time < L.tlim)) return;  // a cycle after the end: nothing moved
+      time = time + dtvar[0];
+      dtvar[1] = time;
+      dtvar[3] = dtvar[0];
     }
-    if (dt < 0.1 * DBL_MAX) dt *= 2.0;
-    dtvar[0] = DMin(dt, est);
+    double dt = advance == 0 ? dtvar[4] : dtvar[3];
+    if (L.dt_force > 0.0) {
+      dt = L.dt_force;
+    } else if (advance == 0 && L.dt_init_force && L.dt_init > 0.0) {
+      dt = L.dt_init;
+    } else {
+      if (dt < 0.1 * DBL_MAX) dt *= 2.0;
+      if (advance == 0) dt = DMin(dt, L.dt_init);
+      dt = DMin(dt, est);
+      dt = DMin(dt, L.dt_user);
+      dt = DMax(L.dt_floor, DMin(dt, L.dt_ceil));
+    }
+    if (time < L.tlim && (L.tlim - time) < dt) dt = L.tlim - time;
+    dtvar[4] = dt;
+    dtvar[0] = time < L.tlim ? dt : 0.0;
     dtvar[2] = est;
   }
 }
@@ -1103,10 +1123,10 @@ pth_status BurgersMassHistory(const Topology &t, int V, const double *U,
 }
 
 pth_status BurgersClock(const double *partial, int n, double cfl, double *dtvar, int advance,
-                        hipStream_t s) {
+                        const ClockLimits &lim, hipStream_t s) {
   ProfScope prof("clock", s);
   hipLaunchKernelGGL(ClockKernel, dim3(1), dim3(kClockWG), 0, s, partial, n, cfl, dtvar,
-                     advance);
+                     advance, lim);
   PTH_HIP(hipGetLastError());
   return PTH_COMPLETE;
 }

@@ -57,8 +57,11 @@ pth_status BurgersFluxUpdate(const Topology &t, int V, const double *flux, const
                              const double *base, double *out, double *derived, double *partial,
                              const BlockCoefs *coef, double beta, const double *dtvar,
                              hipStream_t s);
+// device clock slots: {dt of the next cycle (0 once time >= tlim), time, CFL estimate,
+// dt of the last cycle run, the driver's tm.dt}
+constexpr int kClockSlots = 5;
 pth_status BurgersClock(const double *partial, int n, double cfl, double *dtvar, int advance,
-                        hipStream_t s);
+                        const ClockLimits &lim, hipStream_t s);
 // octant sums of U^2 * volume / mesh volume (8 values into out, device memory)
 pth_status BurgersMassHistory(const Topology &t, int V, const double *U,
                               const DeviceBlockGeom *geom, const BlockCoefs *coef,

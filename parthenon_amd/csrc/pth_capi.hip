@@ -774,7 +774,15 @@ struct SplitMix64 {
 extern "C" {
 
 const char *pth_last_error(void) { return ErrorText(); }
-const char *pth_version(void) { return "parthenon_amd 0.1 (gfx950)"; }
+const char *pth_version(void) { return "parthenon_amd 0.2 (gfx950)"; }
+
+pth_status pth_device_memory(int64_t *free_bytes, int64_t *total_bytes) {
+  size_t f = 0, t = 0;
+  PTH_HIP(hipMemGetInfo(&f, &t));
+  if (free_bytes) *free_bytes = static_cast<int64_t>(f);
+  if (total_bytes) *total_bytes = static_cast<int64_t>(t);
+  return PTH_COMPLETE;
+}
 
 pth_mesh *pth_mesh_create(const int nx[3], const int bnx[3], int nghost, const double xmin[3],
                           const double xmax[3], const int periodic[3]) {

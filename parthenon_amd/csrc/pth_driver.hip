@@ -105,7 +105,7 @@ pth_status Setup(Mesh *m, Ctx &x) {
     PTH_HIP(hipStreamCreateWithFlags(&m->side_stream, hipStreamNonBlocking));
   for (hipEvent_t &e : m->fork_ev)
     if (!e) PTH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-  if (!m->dtvar) PTH_HIP(hipMalloc(&m->dtvar, sizeof(double) * 4));
+  if (!m->dtvar) PTH_HIP(hipMalloc(&m->dtvar, sizeof(double) * kClockSlots));
   const std::int64_t nd = x.base->store[x.fu].ndata;
   const bool planes = x.base->store[x.fu].flux && ReconFluxSupported(m->topo, x.V);
   const bool sweep = m->run_sweep && BurgersSweepSupported(m->topo, x.V);
@@ -170,7 +170,8 @@ pth_status Stage(Ctx &x, int stage, hipStream_t s) {
   }
   if (Exchange(m, mc1, s) != PTH_COMPLETE) return PTH_FAIL;
   if (stage == 2)
-    return BurgersClock(m.scratch, FusedUpdatePartials(m.topo), m.run_cfl, m.dtvar, 1, s);
+    return BurgersClock(m.scratch, FusedUpdatePartials(m.topo), m.run_cfl, m.dtvar, 1, m.run_lim,
+                        s);
   return PTH_COMPLETE;
 }
 
@@ -203,7 +204,7 @@ pth_status StageSweep(Ctx &x, int stage, hipStream_t s) {
       PTH_COMPLETE)
     return PTH_FAIL;
   if (Exchange(m, mc1, s, x.push != nullptr) != PTH_COMPLETE) return PTH_FAIL;
-  if (stage == 2) return BurgersClock(m.scratch, groups, m.run_cfl, m.dtvar, 1, s);
+  if (stage == 2) return BurgersClock(m.scratch, groups, m.run_cfl, m.dtvar, 1, m.run_lim, s);
   return PTH_COMPLETE;
 }
 
@@ -238,7 +239,8 @@ pth_status StageOverlap(Ctx &x, int stage, hipStream_t s) {
                         s) != PTH_COMPLETE)
     return PTH_FAIL;
   if (stage == 2)
-    return BurgersClock(m.scratch, FusedUpdatePartials(m.topo), m.run_cfl, m.dtvar, 1, s);
+    return BurgersClock(m.scratch, FusedUpdatePartials(m.topo), m.run_cfl, m.dtvar, 1, m.run_lim,
+                        s);
   return PTH_COMPLETE;
 }
 
@@ -363,7 +365,7 @@ pth_status TuneForm(Ctx &x, hipStream_t s) {
   double *D = x.fd >= 0 ? x.base->store[x.fd].data : nullptr;
   const std::int64_t ndd = x.fd >= 0 ? x.base->store[x.fd].ndata : 0;
   double *save = nullptr;
-  PTH_HIP(hipMalloc(&save, sizeof(double) * (nd + ndd + 4)));
+  PTH_HIP(hipMalloc(&save, sizeof(double) * (nd + ndd + kClockSlots)));
   hipEvent_t ev[2] = {nullptr, nullptr};
   pth_status st = PTH_COMPLETE;
   auto copy = [&](double *dst, const double *src, std::int64_t n) {
@@ -375,7 +377,7 @@ pth_status TuneForm(Ctx &x, hipStream_t s) {
   };
   copy(save, U, nd);
   if (D) copy(save + nd, D, ndd);
-  copy(save + nd + ndd, m.dtvar, 4);
+  copy(save + nd + ndd, m.dtvar, kClockSlots);
   if (hipEventCreate(&ev[0]) != hipSuccess || hipEventCreate(&ev[1]) != hipSuccess) {
     SetError("stage-form tuning: event creation failed");
     st = PTH_FAIL;
@@ -390,7 +392,7 @@ pth_status TuneForm(Ctx &x, hipStream_t s) {
     m.run_sweep = f.sweep;
     m.run_overlap = f.overlap;
     copy(U, save, nd);
-    copy(m.dtvar, save + nd + ndd, 4);
+    copy(m.dtvar, save + nd + ndd, kClockSlots);
     if (st == PTH_COMPLETE) st = Cycle(x, s);  // warm-up
     if (st == PTH_COMPLETE && hipEventRecord(ev[0], s) != hipSuccess) st = PTH_FAIL;
     for (int c = 0; c < 2 && st == PTH_COMPLETE; ++c) st = Cycle(x, s);
@@ -410,7 +412,7 @@ pth_status TuneForm(Ctx &x, hipStream_t s) {
   Prof().on = prof;
   copy(U, save, nd);
   if (D) copy(D, save + nd, ndd);
-  copy(m.dtvar, save + nd + ndd, 4);
+  copy(m.dtvar, save + nd + ndd, kClockSlots);
   if (st == PTH_COMPLETE && hipStreamSynchronize(s) != hipSuccess) {
     SetError("stage-form tuning: synchronize failed");
     st = PTH_FAIL;
@@ -516,7 +518,7 @@ pth_status pth_burgers_init(pth_mesh *mh, double cfl, int recon) {
   hipStream_t s = m->run_stream;
   // order after any work the caller queued on the mesh stream
   PTH_HIP(hipStreamSynchronize(m->stream));
-  const double init[4] = {DBL_MAX, 0.0, 0.0, DBL_MAX};
+  const double init[kClockSlots] = {DBL_MAX, 0.0, 0.0, DBL_MAX, DBL_MAX};
   PTH_HIP(hipMemcpyAsync(m->dtvar, init, sizeof(init), hipMemcpyHostToDevice, s));
   // Mesh::Initialize after the problem generator: exchange, FillDerived, first time step
   if (Exchange(*m, *x.base, s) != PTH_COMPLETE) return PTH_FAIL;
@@ -528,7 +530,8 @@ pth_status pth_burgers_init(pth_mesh *mh, double cfl, int recon) {
                    m->scratch + m->scratch_n, s) != PTH_COMPLETE)
     return PTH_FAIL;
   // the estimate of BurgersDtMin is the un-scaled minimum; apply the CFL and the clock rule
-  if (BurgersClock(m->scratch + m->scratch_n, 1, cfl, m->dtvar, 0, s) != PTH_COMPLETE)
+  if (BurgersClock(m->scratch + m->scratch_n, 1, cfl, m->dtvar, 0, m->run_lim, s) !=
+      PTH_COMPLETE)
     return PTH_FAIL;
   if (!m->rf_tuned && TuneReconFlux(x, s) != PTH_COMPLETE) return PTH_FAIL;
   if (m->form_auto && !m->form_tuned && TuneForm(x, s) != PTH_COMPLETE) return PTH_FAIL;
@@ -637,7 +640,8 @@ pth_status pth_burgers_after_remesh(pth_mesh *mh) {
   if (BurgersDtMin(m->topo, x.V, x.base->store[x.fu].data, m->coef, m->scratch, np,
                    m->scratch + m->scratch_n, s) != PTH_COMPLETE)
     return PTH_FAIL;
-  if (BurgersClock(m->scratch + m->scratch_n, 1, m->run_cfl, m->dtvar, 2, s) != PTH_COMPLETE)
+  if (BurgersClock(m->scratch + m->scratch_n, 1, m->run_cfl, m->dtvar, 2, m->run_lim, s) !=
+      PTH_COMPLETE)
     return PTH_FAIL;
   // choices timed on a block count at least twice smaller or larger are timed again (the
   // tuners restore U, the derived field and the clock)
@@ -668,11 +672,34 @@ pth_status pth_burgers_clock(pth_mesh *mh, double *dt, double *time) {
     SetError("call pth_burgers_init first");
     return PTH_FAIL;
   }
-  double v[4];
+  double v[kClockSlots];
   PTH_HIP(hipStreamSynchronize(m->run_stream));
   PTH_HIP(hipMemcpy(v, m->dtvar, sizeof(v), hipMemcpyDeviceToHost));
-  if (dt) *dt = v[0];
+  if (dt) *dt = v[4];
   if (time) *time = v[1];
+  return PTH_COMPLETE;
+}
+
+pth_status pth_burgers_set_time_limits(pth_mesh *mh, const double lim[6], int dt_init_force) {
+  Mesh *m = Impl(mh);
+  if (!lim) {
+    SetError("time limits: null array");
+    return PTH_FAIL;
+  }
+  ClockLimits L;
+  L.tlim = lim[0];
+  L.dt_init = lim[1];
+  L.dt_user = lim[2];
+  L.dt_force = lim[3];
+  L.dt_floor = lim[4];
+  L.dt_ceil = lim[5];
+  L.dt_init_force = dt_init_force != 0;
+  m->run_lim = L;
+  // the limits travel in the clock launches' arguments: a captured cycle is captured again
+  if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
+  if (m->graph) (void)hipGraphDestroy(m->graph);
+  m->graph_exec = nullptr;
+  m->graph = nullptr;
   return PTH_COMPLETE;
 }
 

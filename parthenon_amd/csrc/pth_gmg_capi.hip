@@ -102,7 +102,8 @@ struct Server {
   Params P;
   double tol;
   int maxit, max_active;
-  std::int64_t budget;
+  std::int64_t budget;           // cap on bytes held (<= 0: none beyond free HBM)
+  std::int64_t reserve = 1LL << 30;  // free HBM left unallocated by admissions
   bool keep;
   hipStream_t s = nullptr;
   GmgBatch<T> batch;
@@ -124,6 +125,14 @@ struct Server {
     PTH_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     return batch.Init(s);
   }
+  // Admission by memory: the device's free HBM (hipMemGetInfo) must hold the problem and the
+  // reserve, and the bytes held must stay within the cap when one is set.
+  bool Fits(std::int64_t need) const {
+    if (budget > 0 && held + need > budget) return false;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return false;
+    return static_cast<std::int64_t>(free_b) >= need + reserve;
+  }
   GmgProblem<T> *Take(int n) {
     auto &v = pool[n];
     if (!v.empty()) {
@@ -132,16 +141,16 @@ struct Server {
       return p;
     }
     const std::int64_t need = GmgProblem<T>::Bytes(n);
-    if (held + need > budget) {  // evict pooled problems of other sizes first
+    if (!Fits(need)) {  // evict pooled problems of other sizes first
       for (auto &kv : pool)
-        while (!kv.second.empty() && held + need > budget) {
+        while (!kv.second.empty() && !Fits(need)) {
           GmgProblem<T> *q = kv.second.back();
           kv.second.pop_back();
           held -= GmgProblem<T>::Bytes(q->N);
           q->Free();
           delete q;
         }
-      if (held + need > budget) return nullptr;
+      if (!Fits(need)) return nullptr;
     }
     auto *p = new GmgProblem<T>();
     if (p->Allocate(n, GmgBottom<T>::kN) != PTH_COMPLETE) {
@@ -195,7 +204,7 @@ struct Server {
       }
       if (act.empty()) {
         if (!waiting.empty()) {
-          SetError("GMG server: a request does not fit the memory budget");
+          SetError("GMG server: a request does not fit in free device memory (or the cap)");
           return PTH_FAIL;
         }
         if (next < order.size() && realtime) {
@@ -382,7 +391,7 @@ int pth_gmg_history(const pth_gmg *h, double *out, int max) {
 pth_gmg_server *pth_gmg_server_create(int precision, double omega, int pre, int post, double tol,
                                       int max_iters, int64_t budget_bytes, int max_active,
                                       int keep_solutions) {
-  if ((precision != 0 && precision != 1) || max_active < 1 || budget_bytes <= 0) {
+  if ((precision != 0 && precision != 1) || max_active < 1) {
     SetError("pth_gmg_server_create: bad arguments");
     return nullptr;
   }
@@ -438,6 +447,17 @@ pth_status pth_gmg_server_set_inner_cycles(pth_gmg_server *h, int k) {
   }
   return h->Do([&](auto &S) {
     S.P.inner = k;
+    return PTH_COMPLETE;
+  });
+}
+
+pth_status pth_gmg_server_set_hbm_reserve(pth_gmg_server *h, int64_t bytes) {
+  if (bytes < 0) {
+    SetError("pth_gmg_server_set_hbm_reserve: bytes must be >= 0");
+    return PTH_FAIL;
+  }
+  return h->Do([&](auto &S) {
+    S.reserve = bytes;
     return PTH_COMPLETE;
   });
 }

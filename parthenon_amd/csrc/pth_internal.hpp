@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <cfloat>
+#include <cmath>
 #include <cstdint>
 #include <map>
 #include <memory>
@@ -290,6 +292,17 @@ struct BoundCache {  // per (MeshData, bound type): the pack / unpack / pro-res 
   int rest_field = -1;
 };
 
+// <parthenon/time> limits of EvolutionDriver with its defaults (src/driver/driver.hpp:66-94)
+struct ClockLimits {
+  double tlim = HUGE_VAL;
+  double dt_init = DBL_MAX;
+  double dt_user = DBL_MAX;
+  double dt_force = -DBL_MAX;
+  double dt_floor = DBL_MIN;
+  double dt_ceil = DBL_MAX;
+  int dt_init_force = 0;
+};
+
 struct MeshData {
   Mesh *mesh;
   std::string name;
@@ -319,8 +332,10 @@ class Mesh {
   BlockCoefs *coef = nullptr;
   double *face = nullptr;                       // Burgers face-state scratch
   int face_V = 0;
-  // device-resident Burgers driver state: clock {dt, time, estimate} and the captured cycle
+  // device-resident Burgers driver state: clock (kClockSlots doubles), its limits and the
+  // captured cycle
   double *dtvar = nullptr;
+  ClockLimits run_lim;
   hipStream_t run_stream = nullptr;
   hipEvent_t order_ev[2] = {nullptr, nullptr};  // mesh stream <-> driver stream ordering
   hipStream_t side_stream = nullptr;            // driver: the exchange beside the stencils

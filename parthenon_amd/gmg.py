@@ -78,8 +78,11 @@ class MGSolver:
 
 
 class Server:
-    def __init__(self, precision="mixed", params=None, budget_bytes=64 << 30, max_active=64,
-                 keep_solutions=False):
+    """Continuous-batching request loop: admits solves by free device memory (and an optional
+    cap on the bytes held, budget_bytes > 0) and free slots (max_active)."""
+
+    def __init__(self, precision="mixed", params=None, budget_bytes=0, max_active=64,
+                 keep_solutions=False, hbm_reserve=None):
         p = params or MGParams()
         self.params = p
         h = lib().pth_gmg_server_create(_PREC[precision], p.omega, p.pre_sweeps, p.post_sweeps,
@@ -92,6 +95,8 @@ class Server:
         _check(lib().pth_gmg_server_set_inner_cycles(h, p.inner_cycles), "set_inner_cycles")
         _check(lib().pth_gmg_server_set_column_sweeps(h, int(p.column_sweeps)),
                "set_column_sweeps")
+        if hbm_reserve is not None:
+            _check(lib().pth_gmg_server_set_hbm_reserve(h, int(hbm_reserve)), "set_hbm_reserve")
 
     def __del__(self):
         if getattr(self, "_h", None):

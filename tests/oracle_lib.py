@@ -4,7 +4,9 @@ Test infrastructure only: imported by tests/, the smoke check and bench.py's CPU
 leg, never by the product package.
 """
 import ctypes as C
+import math
 import os
+import sys
 
 import numpy as np
 
@@ -57,7 +59,9 @@ def lib():
             "orc_burgers_pgen_sinusoid": (None, [P, C.c_uint64]),
             "orc_burgers_pgen_vibe": (None, [P]),
             "orc_burgers_initialize": (None, [P]),
-            "orc_burgers_cycles": (None, [P, I]),
+            "orc_burgers_cycles": (I, [P, I]),
+            "orc_burgers_set_time_limits": (None, [P, DP, I, I]),
+            "orc_burgers_ncycle": (I, [P]),
             "orc_burgers_fluxes": (None, [P]),
             "orc_burgers_fluxdiv": (None, [P]),
             "orc_burgers_derived": (None, [P]),
@@ -270,6 +274,25 @@ class OracleBurgers:
     @property
     def dt(self):
         return lib().orc_burgers_dt(self.h)
+
+    @property
+    def time(self):
+        return lib().orc_burgers_time(self.h)
+
+    @property
+    def ncycle(self):
+        return lib().orc_burgers_ncycle(self.h)
+
+    def set_time_limits(self, tlim=math.inf, dt_init=sys.float_info.max, dt_init_force=False,
+                        dt_user=sys.float_info.max, dt_force=-sys.float_info.max,
+                        dt_floor=sys.float_info.min, dt_ceil=sys.float_info.max, nlim=-1):
+        """<parthenon/time> limits (EvolutionDriver defaults); before initialize()."""
+        v = np.array([tlim, dt_init, dt_user, dt_force, dt_floor, dt_ceil], dtype=np.float64)
+        lib().orc_burgers_set_time_limits(self.h, dptr(v), int(dt_init_force), nlim)
+
+    def cycles(self, n):
+        """EvolutionDriver loop: at most n cycles while KeepGoing(); returns cycles run."""
+        return lib().orc_burgers_cycles(self.h, n)
 
     def totals(self):
         out = np.empty(self.V)

@@ -353,3 +353,65 @@ def test_tuning_choices_give_identical_bits():
     assert np.array_equal(outs[3], outs[2])
     assert np.array_equal(auto, outs[0] if d.form() == "sweep" else outs[2])
     assert_close(outs[0], outs[2], "block sweep vs plane-tiled form")
+
+
+@pytest.mark.parametrize("form", ["sweep", "serial"])
+@pytest.mark.parametrize("use_graph", [False, True], ids=["eager", "graph"])
+def test_device_driver_time_limits_match_oracle(form, use_graph):
+    # SetGlobalTimeStep's <parthenon/time> bounds and the end on tlim (driver.cpp:210-269) on
+    # the device clock: a forced first dt, a dt ceiling, and tlim in the middle of the third
+    # cycle; cycles queued after the end change nothing
+    print(f"case device driver time limits form={form} graph={use_graph}", flush=True)
+    from parthenon_amd.burgers import DeviceBurgersDriver
+    probe = OracleBurgers(OracleMesh((32, 32, 32), (16, 16, 16), 4), 4, 0.4, 0)
+    probe.set_time_limits(dt_init=1e-4, dt_init_force=True, dt_ceil=6e-4)
+    from oracle_lib import lib
+    lib().orc_burgers_pgen_sinusoid(probe.h, 5)
+    lib().orc_burgers_initialize(probe.h)
+    probe.cycles(2)
+    lim = dict(tlim=probe.time + 0.37 * probe.dt, dt_init=1e-4, dt_init_force=True,
+               dt_ceil=6e-4)
+    ob = OracleBurgers(OracleMesh((32, 32, 32), (16, 16, 16), 4), 4, 0.4, 0)
+    ob.set_time_limits(**lim)
+    lib().orc_burgers_pgen_sinusoid(ob.h, 5)
+    lib().orc_burgers_initialize(ob.h)
+    assert ob.cycles(10) == 3 and ob.time == lim["tlim"]
+    m = make_burgers_mesh((32, 32, 32), (16, 16, 16), 4, 4)
+    d = DeviceBurgersDriver(m, cfl=0.4, use_graph=use_graph, **FORMS[form])
+    d.set_time_limits(**lim)
+    d.initialize("sinusoid", seed=5)
+    assert d.clock()[0] == 1e-4
+    d.cycles(6)
+    assert_close(d.base.download("U"), ob.get(0), "U at tlim")
+    assert_clock(*d.clock(), ob, exact=form != "sweep")
+
+
+def test_sweep_form_sphere960_full_run_matches_oracle():
+    # the benchmark's workload at its own length: the 960-block sphere mesh, 20 cycles of the
+    # block-sweep form (built with FMA contraction), eager and graph-replayed side by side,
+    # against the oracle at 1e-13 after cycles 1, 10 and 20
+    print("case sphere960 sweep 20 cycles eager+graph", flush=True)
+    from parthenon_amd.burgers import DeviceBurgersDriver
+    from oracle_lib import lib
+    spec = REFINED["sphere960"]
+    drivers = []
+    for use_graph in (False, True):
+        m = make_burgers_mesh(spec["nx"], spec["block"], 4, 4,
+                              refine_spheres=spec["refine_spheres"])
+        d = DeviceBurgersDriver(m, cfl=0.4, use_graph=use_graph, sweep=True)
+        d.initialize("sinusoid", seed=2)
+        drivers.append(d)
+    om = OracleMesh(spec["nx"], spec["block"], 4, spheres=spec["refine_spheres"])
+    ob = OracleBurgers(om, 4, 0.4, 0)
+    lib().orc_burgers_pgen_sinusoid(ob.h, 2)
+    lib().orc_burgers_initialize(ob.h)
+    done = 0
+    for upto in (1, 10, 20):
+        for d in drivers:
+            d.cycles(upto - done)
+        ob.cycles(upto - done)
+        done = upto
+        exp = ob.get(0)
+        for d, name in zip(drivers, ("eager", "graph")):
+            assert_close(d.base.download("U"), exp, f"sweep {name} U after {upto} cycles")
+            assert_clock(*d.clock(), ob, exact=False)

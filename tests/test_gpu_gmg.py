@@ -130,3 +130,75 @@ def test_served_stream_in_real_time():
         assert r["admit"] >= r["arrival"] - 1e-9 and r["done"] >= r["admit"]
         lat.append(r["latency"])
     assert np.all(np.array(lat) > 0)
+
+
+def test_mixed_256_blob_problem_at_full_size():
+    # the 256^3 problem of the GMG benchmark (16 blobs, seed 3): the mixed solver's residual,
+    # recomputed in FP64 with the oracle's operator, and its distance to the FP64 oracle solve
+    print("case gmg mixed N=256 seed 3", flush=True)
+    N = 256
+    g, b, it, hist = oracle_solve(N, seed=3)
+    s = MGSolver(N, "mixed", MGParams(inner_cycles=2))
+    assert s.levels == g.levels == 9
+    s.set_rhs(b)
+    its, rel = s.solve()
+    assert rel < 1e-10, (its, rel)
+    u = s.solution()
+    r = b - g.apply(u)
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-10
+    err = rel_rms(u, g.u())
+    print(f"N=256: {its} FP64 steps (oracle {it} V-cycles), error vs oracle {err:.2e}", flush=True)
+    assert err < 1e-8
+
+
+def test_served_stream_with_large_grids_matches_solves_alone():
+    # a seeded arrival stream that includes 128^3 and 192^3 requests, served in real time with
+    # overlapping solves; every result equals the same request solved alone, bit for bit
+    print("case gmg server stream with 128^3 and 192^3", flush=True)
+    rng = np.random.default_rng(7)
+    sizes = [32, 128, 64, 192, 32, 128, 64, 32, 192, 64, 32, 128]
+    t, reqs = 0.0, []
+    for q, n in enumerate(sizes):
+        t += rng.exponential(1 / 50.0)
+        reqs.append((n, 300 + q, t))
+    srv = Server("mixed", max_active=16, keep_solutions=True)
+    ids = [srv.submit(n, seed, at) for n, seed, at in reqs]
+    srv.run(realtime=True)
+    st = srv.stats()
+    print(f"server stats {st}", flush=True)
+    assert st["max_batch"] >= 2
+    for (n, seed, _), rid in zip(reqs, ids):
+        res = srv.result(rid)
+        assert res["rel_residual"] < 1e-10
+        alone = MGSolver(n, "mixed")
+        alone.set_rhs_blobs(seed)
+        its, rel = alone.solve()
+        assert its == res["iters"], (n, seed)
+        assert np.array_equal(srv.solution(rid), alone.solution()), (n, seed)
+
+
+def test_admission_by_free_device_memory():
+    # with a reserve that leaves room for two 64^3 hierarchies in the device's free memory,
+    # at most two of six requests present at t = 0 are active together; results are unchanged
+    import gc
+    import parthenon_amd as pa
+    print("case gmg server admission by free HBM", flush=True)
+    gc.collect()
+    probe = Server("mixed", max_active=8)
+    probe.submit(64, 1)
+    probe.run(realtime=False)
+    per = probe.stats()["bytes_held"]
+    del probe
+    srv = Server("mixed", max_active=8, keep_solutions=True)
+    free, total = pa.device_memory()
+    assert 0 < free <= total
+    pa.lib().pth_gmg_server_set_hbm_reserve(srv._h, int(free - 2.5 * per))
+    ids = [srv.submit(64, 40 + q, 0.0) for q in range(6)]
+    srv.run(realtime=False)
+    st = srv.stats()
+    assert st["max_batch"] == 2, st
+    for q, rid in enumerate(ids):
+        alone = MGSolver(64, "mixed")
+        alone.set_rhs_blobs(40 + q)
+        alone.solve()
+        assert np.array_equal(srv.solution(rid), alone.solution())

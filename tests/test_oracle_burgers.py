@@ -375,3 +375,56 @@ def test_mass_history_octant_constant_fields(refined):
     half = lib().orc_burgers_mass_history(b.h, (C.c_double * 3)(*octant_boxes()[0][0]),
                                           (C.c_double * 3)(*octant_boxes()[0][1]), 2.0)
     assert abs(half - exp[0] / 2) < 1e-13 * exp[0]
+
+
+def uniform_run(cfl=0.375, **limits):
+    """Uniform state (0.75, -0.5, 0.25) on 32^3 cells of width 1/32: the CFL estimate is
+    cfl / (1.5 * 32) = 2^-7 exactly for cfl = 0.375, and the state is steady."""
+    m, b = burgers(cfl=cfl)
+    U = b.get(0).reshape(m.field_shape(b.V))
+    vel = (0.75, -0.5, 0.25)
+    for c in range(b.V):
+        U[:, c] = vel[c] if c < 3 else 2.0
+    b.set(0, U)
+    b.set_time_limits(**limits)
+    lib().orc_burgers_initialize(b.h)
+    return b, U
+
+
+def test_time_limits_tlim_cuts_the_last_cycle_and_stops():
+    # SetGlobalTimeStep's end clamp (driver.cpp:267-268) and KeepGoing (basic_types.hpp:241):
+    # with dt = 2^-7 and tlim = 2.5 * 2^-7 the third cycle is 2^-8 long, ends on tlim exactly,
+    # and the loop stops; the dt computed after it is the unclamped 2^-7
+    b, U = uniform_run(tlim=2.5 * 2.0 ** -7)
+    assert b.dt == 2.0 ** -7
+    assert b.cycles(10) == 3
+    assert b.time == 2.5 * 2.0 ** -7 and b.ncycle == 3 and b.dt == 2.0 ** -7
+    assert b.cycles(1) == 0
+    assert np.array_equal(b.get(0).reshape(U.shape), U)
+
+
+def test_time_limits_nlim_stops():
+    b, _ = uniform_run(nlim=2)
+    assert b.cycles(10) == 2 and b.time == 2.0 ** -6
+
+
+@pytest.mark.parametrize("limits,dts", [
+    (dict(dt_user=2.0 ** -9), [2.0 ** -9] * 4),
+    (dict(dt_force=2.0 ** -8), [2.0 ** -8] * 4),
+    # dt_init bounds only the first dt; then dt doubles until the estimate binds
+    (dict(dt_init=2.0 ** -10), [2.0 ** -10, 2.0 ** -9, 2.0 ** -8, 2.0 ** -7, 2.0 ** -7]),
+    # forced, dt_init may exceed the estimate on the first cycle only
+    (dict(dt_init=2.0 ** -5, dt_init_force=True), [2.0 ** -5, 2.0 ** -7, 2.0 ** -7]),
+    (dict(dt_ceil=2.0 ** -8), [2.0 ** -8] * 3),
+    (dict(dt_floor=2.0 ** -6), [2.0 ** -6] * 3),
+], ids=["dt_user", "dt_force", "dt_init", "dt_init_force", "dt_ceil", "dt_floor"])
+def test_time_limits_dt_bounds(limits, dts):
+    # SetGlobalTimeStep (driver.cpp:210-235) with each <parthenon/time> bound against the
+    # estimate 2^-7: the dt of every cycle and the time they add up to
+    b, _ = uniform_run(**limits)
+    t = 0.0
+    for dt in dts:
+        assert b.dt == dt, (limits, b.ncycle, b.dt, dt)
+        assert b.cycles(1) == 1
+        t += dt
+        assert b.time == t
