@@ -27,20 +27,16 @@ typedef enum {
   PTH_FAIL = 3
 } pth_status;
 
-/* BoundaryType: src/basic_types.hpp:156 (any, local, nonlocal, flxcor_send, flxcor_recv,
- * gmg_same, gmg_restrict_send, gmg_restrict_recv, gmg_prolongate_send, gmg_prolongate_recv).
- * On one GPU every neighbour is local, so any == local and nonlocal has no buffers. */
+/* BoundaryType: src/basic_types.hpp:156 (any, local, nonlocal, flxcor_send, flxcor_recv;
+ * the reference's gmg_* types are not offered: the multigrid solver keeps its level transfers
+ * inside pth_gmg_*). On one GPU every neighbour is local, so any == local and
+ * nonlocal has no buffers. */
 typedef enum {
   PTH_BT_ANY = 0,
   PTH_BT_LOCAL = 1,
   PTH_BT_NONLOCAL = 2,
   PTH_BT_FLXCOR_SEND = 3,
-  PTH_BT_FLXCOR_RECV = 4,
-  PTH_BT_GMG_SAME = 5,
-  PTH_BT_GMG_RESTRICT_SEND = 6,
-  PTH_BT_GMG_RESTRICT_RECV = 7,
-  PTH_BT_GMG_PROLONGATE_SEND = 8,
-  PTH_BT_GMG_PROLONGATE_RECV = 9
+  PTH_BT_FLXCOR_RECV = 4
 } pth_boundary_type;
 
 /* Metadata flags used by the path: src/interface/metadata.hpp (FillGhost, Independent,
@@ -181,8 +177,9 @@ pth_status pth_burgers_init(pth_mesh *m, double cfl, int recon);
 /* ncycles RK2 cycles; use_graph != 0 captures one cycle into a hipGraph on first use and
  * replays it. Asynchronous on the driver's own stream. */
 pth_status pth_burgers_run_cycles(pth_mesh *m, int ncycles, int use_graph);
-/* Driver stage form (no counterpart in the reference; results are identical in every form).
- * For cubic blocks of 8 or 16 cells with 4 ghosts a stage is either one block-sweep launch
+/* Driver stage form (no counterpart in the reference). The two plane forms give the same
+ * bits; the block sweep is built with FMA contraction and agrees with them to rounding (about
+ * 1e-15 relative after a few cycles). For cubic blocks of 8 or 16 cells with 4 ghosts a stage is either one block-sweep launch
  * (reconstruction, HLL, flux divergence and RK update per block, no face state or flux in
  * HBM) followed by the exchange, or the plane-tiled reconstruction into the flux array and
  * the update, with the stage input's exchange either beside the reconstruction of the faces
@@ -194,6 +191,10 @@ pth_status pth_burgers_run_cycles(pth_mesh *m, int ncycles, int use_graph);
 pth_status pth_burgers_set_overlap(pth_mesh *m, int on);
 pth_status pth_burgers_set_sweep(pth_mesh *m, int on);
 pth_status pth_burgers_form(pth_mesh *m, int *sweep, int *overlap);
+/* With on != 0 the automatic choice of pth_burgers_init considers only the bit-exact plane
+ * forms (when the mesh supports them), so a run's bits cannot depend on timing noise; must
+ * precede pth_burgers_init. No counterpart in the reference. */
+pth_status pth_burgers_set_bit_exact(pth_mesh *m, int on);
 /* Driver tuning (no counterpart in the reference): planes per workgroup (1, 2 or 4) of the
  * plane-tiled reconstruction for its three forms (all faces, ghost-free faces, the rest), and
  * the block sweep's register budget (3 or 4: the waves per SIMD it is compiled for; with the

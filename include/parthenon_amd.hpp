@@ -37,12 +37,7 @@ enum class BoundaryType : int {  // src/basic_types.hpp:156
   local = PTH_BT_LOCAL,
   nonlocal = PTH_BT_NONLOCAL,
   flxcor_send = PTH_BT_FLXCOR_SEND,
-  flxcor_recv = PTH_BT_FLXCOR_RECV,
-  gmg_same = PTH_BT_GMG_SAME,
-  gmg_restrict_send = PTH_BT_GMG_RESTRICT_SEND,
-  gmg_restrict_recv = PTH_BT_GMG_RESTRICT_RECV,
-  gmg_prolongate_send = PTH_BT_GMG_PROLONGATE_SEND,
-  gmg_prolongate_recv = PTH_BT_GMG_PROLONGATE_RECV
+  flxcor_recv = PTH_BT_FLXCOR_RECV
 };
 
 struct Metadata {  // the cell-variable flags used on this path (src/interface/metadata.hpp)

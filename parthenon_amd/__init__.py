@@ -38,11 +38,6 @@ class BoundaryType(enum.IntEnum):  # src/basic_types.hpp:156
     nonlocal_ = 2
     flxcor_send = 3
     flxcor_recv = 4
-    gmg_same = 5
-    gmg_restrict_send = 6
-    gmg_restrict_recv = 7
-    gmg_prolongate_send = 8
-    gmg_prolongate_recv = 9
 
 
 class Metadata(enum.IntFlag):  # the flags of src/interface/metadata.hpp used on this path
@@ -114,6 +109,7 @@ _SIGNATURES = {
     "pth_burgers_set_time_limits": (_I, [_P, _DP, _I]),
     "pth_burgers_set_overlap": (_I, [_P, _I]),
     "pth_burgers_set_sweep": (_I, [_P, _I]),
+    "pth_burgers_set_bit_exact": (_I, [_P, _I]),
     "pth_burgers_form": (_I, [_P, C.POINTER(_I), C.POINTER(_I)]),
     "pth_burgers_tuning": (_I, [_P, C.POINTER(_I)]),
     "pth_burgers_set_tuning": (_I, [_P, C.POINTER(_I)]),

@@ -86,7 +86,7 @@ class DeviceBurgersDriver:
     (use_graph) one captured hipGraph replayed per cycle."""
 
     def __init__(self, mesh, cfl=0.4, recon=burgers_package.WENO5, use_graph=True,
-                 overlap=None, sweep=None):
+                 overlap=None, sweep=None, bit_exact=False):
         import ctypes as C
         from . import _check, lib
         self._C, self._check, self._lib = C, _check, lib
@@ -96,10 +96,13 @@ class DeviceBurgersDriver:
         self.use_graph = use_graph
         self.base = mesh.mesh_data("base")
         self.one = mesh.mesh_data("1")
-        # stage form (same results): sweep = one block-sweep launch per stage; otherwise the
-        # plane reconstruction, with overlap = the stage input's exchange beside the ghost-free
-        # reconstruction on a second stream. Left as None, initialize() times the forms and
-        # keeps the fastest.
+        # stage form: sweep = one block-sweep launch per stage (FMA-contracted, equal to the
+        # others to rounding); otherwise the plane reconstruction (bit-exact), with overlap =
+        # the stage input's exchange beside the ghost-free reconstruction on a second stream.
+        # Left as None, initialize() times the forms and keeps the fastest; bit_exact limits
+        # that choice to the bit-exact plane forms.
+        if bit_exact:
+            _check(lib().pth_burgers_set_bit_exact(mesh._h, 1), "set_bit_exact")
         if overlap is not None:
             _check(lib().pth_burgers_set_overlap(mesh._h, int(overlap)), "set_overlap")
         if sweep is not None:

@@ -344,14 +344,19 @@ pth_status Cycle(Ctx &x, hipStream_t s);
 // Picks the stage form (block sweep; plane reconstruction with the exchange on a second
 // stream; plane reconstruction then update then exchange) by timing whole cycles of each on
 // the mesh's own state, which is saved first and restored afterwards (U of "base", the
-// derived field and the device clock). The forms give identical results.
+// derived field and the device clock). The two plane forms give the same bits; the sweep is
+// built with FMA contraction and agrees with them to rounding (about 1e-15 relative), so
+// with form_exact set only the plane forms are candidates and the result cannot depend on
+// which form the timing picked.
 pth_status TuneForm(Ctx &x, hipStream_t s) {
   Mesh &m = *x.m;
   struct Form {
     int sweep, overlap;
   };
   std::vector<Form> forms;
-  if (BurgersSweepSupported(m.topo, x.V)) forms.push_back({1, 0});
+  if (BurgersSweepSupported(m.topo, x.V) &&
+      !(m.form_exact && x.base->store[x.fu].flux && ReconFluxSupported(m.topo, x.V)))
+    forms.push_back({1, 0});
   if (x.base->store[x.fu].flux && ReconFluxSupported(m.topo, x.V)) {
     forms.push_back({0, 1});
     forms.push_back({0, 0});
@@ -608,6 +613,16 @@ pth_status pth_burgers_set_sweep(pth_mesh *mh, int on) {
   if (m->graph) (void)hipGraphDestroy(m->graph);
   m->graph_exec = nullptr;
   m->graph = nullptr;
+  return PTH_COMPLETE;
+}
+
+pth_status pth_burgers_set_bit_exact(pth_mesh *mh, int on) {
+  Mesh *m = Impl(mh);
+  if (m->run_ready && m->form_auto) {
+    SetError("pth_burgers_set_bit_exact must precede pth_burgers_init");
+    return PTH_FAIL;
+  }
+  m->form_exact = on != 0;
   return PTH_COMPLETE;
 }
 

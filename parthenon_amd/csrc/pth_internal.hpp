@@ -344,6 +344,7 @@ class Mesh {
   int rf_ppw[3] = {1, 1, 1};                    // driver: planes per workgroup, recon parts
   int run_sweep = 1;                            // driver: one block-sweep launch per stage
   bool form_auto = true;                        // driver: pick sweep / overlap / serial by timing
+  bool form_exact = false;                      // driver: timing picks among bit-exact forms only
   bool form_tuned = false;
   int sweep_waves = 4;                          // driver: sweep register budget (waves/SIMD)
   int sweep_spread = 1;                         // driver: resident sweep workgroups / spread

@@ -164,8 +164,8 @@ __global__ void __launch_bounds__(64)
 // k - 1, carried in registers). The flux divergence sums x, y, z in the order of
 // FluxDivHelper (update.hpp:43-57), then the RK averaging and update, the derived field and
 // the CFL minimum follow as in FluxUpdateKernel. Every face flux is the same expression as in
-// the other forms, so the stage output is the same bits, and no flux or face state reaches
-// HBM. On
+// the other forms up to FMA contraction and the folded divides, so the stage output agrees
+// with theirs to rounding, and no flux or face state reaches HBM. On
 // multilevel meshes, faces with a finer neighbour take the restricted fine fluxes from the
 // correction store (fover / fcor as in FusedUpdateKernel).
 // The grid is persistent: workgroup g takes planes [g T / G, (g + 1) T / G) of the
