@@ -165,6 +165,14 @@ pth_status pth_weighted_sum_independent(pth_meshdata *in1, pth_meshdata *in2, do
 pth_status pth_burgers_calculate_derived(pth_meshdata *md);
 /* burgers_package::EstimateTimestepMesh (burgers_package.cpp:170-200); synchronous. */
 pth_status pth_burgers_estimate_timestep(pth_meshdata *md, double cfl, double *dt);
+/* The same estimate without allocation or host synchronization; no counterpart in the
+ * reference. For callers that cannot allocate device or pinned memory (code under hipGraph
+ * capture, tight serving loops): asynchronous on the mesh's stream, it writes cfl * min into
+ * dt_device[0] (device memory, the same bits as pth_burgers_estimate_timestep) and keeps the
+ * reduction's workgroup partials in a static 64 KiB device array per GPU (at most 8192
+ * partials; a larger need is rejected). Calls that overlap on one GPU share that array and
+ * race: order them on one stream. It calls no hipMalloc, hipFree or allocating routine. */
+pth_status pth_burgers_estimate_timestep_noalloc(pth_meshdata *md, double cfl, double *dt_device);
 
 /* ---- device-resident Burgers driver (the task order of BurgersDriver::MakeTaskCollection,
  * benchmarks/burgers/burgers_driver.cpp:53-148, RK2 of low_storage_integrator.cpp:67-83 and

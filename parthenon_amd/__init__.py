@@ -99,6 +99,7 @@ _SIGNATURES = {
     "pth_weighted_sum_independent": (_I, [_P, _P, _D, _D, _P]),
     "pth_burgers_calculate_derived": (_I, [_P]),
     "pth_burgers_estimate_timestep": (_I, [_P, _D, _DP]),
+    "pth_burgers_estimate_timestep_noalloc": (_I, [_P, _D, _P]),
     "pth_burgers_pgen_sinusoid": (_I, [_P, C.c_uint64]),
     "pth_burgers_pgen_vibe": (_I, [_P]),
     "pth_burgers_mass_history": (_I, [_P, _DP]),

@@ -231,6 +231,16 @@ __global__ void __launch_bounds__(64) MinFinalKernel(const double *__restrict__ 
   if (threadIdx.x == 0) out[0] = m;
 }
 
+// cfl times the minimum of the partials, into out[0] (the product the host forms after
+// MinFinalKernel in the synchronous form, so the two give the same bits)
+__global__ void __launch_bounds__(64) MinScaleKernel(const double *__restrict__ partial, int n,
+                                                     double cfl, double *__restrict__ out) {
+  double m = DBL_MAX;
+  for (int t = threadIdx.x; t < n; t += 64) m = DMin(m, partial[t]);
+  m = WaveMin(m);
+  if (threadIdx.x == 0) out[0] = cfl * m;
+}
+
 // ---- fused stage path ----------------------------------------------------------------
 //
 // ReconTaskKernel: one thread = one WENO5Z (or linear) reconstruction of one cell in one
@@ -1127,6 +1137,28 @@ pth_status BurgersClock(const double *partial, int n, double cfl, double *dtvar,
   ProfScope prof("clock", s);
   hipLaunchKernelGGL(ClockKernel, dim3(1), dim3(kClockWG), 0, s, partial, n, cfl, dtvar,
                      advance, lim);
+  PTH_HIP(hipGetLastError());
+  return PTH_COMPLETE;
+}
+
+int BurgersDtPartials(const Topology &t) {
+  return Grid(static_cast<std::int64_t>(t.fine.n[0]) * t.fine.n[1] * t.fine.n[2] *
+              static_cast<std::int64_t>(t.blocks.size()));
+}
+
+pth_status BurgersDtToDevice(const Topology &t, int V, const double *U, const BlockCoefs *coef,
+                             double *partial, int npartial, double cfl, double *out,
+                             hipStream_t s) {
+  ProfScope prof("dt_reduce", s);
+  const Geo G = MakeGeo(t, V);
+  const int nb = static_cast<int>(t.blocks.size());
+  const int grid = BurgersDtPartials(t);
+  if (grid > npartial) {
+    SetError("CFL reduction: more partials than the scratch holds");
+    return PTH_FAIL;
+  }
+  hipLaunchKernelGGL(DtKernel, dim3(grid), dim3(kWG), 0, s, U, coef, G, nb, partial);
+  hipLaunchKernelGGL(MinScaleKernel, dim3(1), dim3(64), 0, s, partial, grid, cfl, out);
   PTH_HIP(hipGetLastError());
   return PTH_COMPLETE;
 }

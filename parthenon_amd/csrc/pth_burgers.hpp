@@ -16,6 +16,13 @@ pth_status BurgersDerived(const Topology &t, int V, const double *U, double *out
                           hipStream_t s);
 pth_status BurgersDtMin(const Topology &t, int V, const double *U, const BlockCoefs *coef,
                         double *partial, int npartial, double *out, hipStream_t s);
+// workgroup partials the CFL reduction uses on topology t (at most 8192)
+int BurgersDtPartials(const Topology &t);
+// out[0] = cfl * (the CFL minimum), device memory, asynchronous on s; partial holds npartial
+// doubles and must cover BurgersDtPartials(t)
+pth_status BurgersDtToDevice(const Topology &t, int V, const double *U, const BlockCoefs *coef,
+                             double *partial, int npartial, double cfl, double *out,
+                             hipStream_t s);
 
 // fused stage path
 pth_status BurgersReconTasks(const Topology &t, int V, const double *U, double *face, int recon,

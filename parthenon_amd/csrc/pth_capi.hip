@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "pth_burgers.hpp"
@@ -1434,3 +1435,61 @@ int pth_profile_report(char *buf, int len) {
 }
 
 }  // extern "C"
+
+// ---- CFL estimate without allocation ---------------------------------------------------
+// The workgroup partials of the reduction live in one file-scope device array per GPU,
+// `scratch`, instead of the mesh's scratch area. In host code `scratch` names a host-side
+// stand-in, never the device memory: the device address of each GPU's copy is taken once,
+// with that GPU current, by hipGetSymbolAddress and kept per device; only that address is
+// passed to kernels.
+namespace {
+
+constexpr int kStaticPartials = 8192;  // 64 KiB per GPU, held until the process ends
+__device__ __align__(256) double scratch[kStaticPartials];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// device address of the current GPU's `scratch` (looked up on the first call per device)
+pth_status StaticScratch(double **out) {
+  int dev = -1;
+  PTH_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    SetError("static scratch: device ordinal out of range");
+    return PTH_FAIL;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    PTH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return PTH_COMPLETE;
+}
+
+}  // namespace
+
+extern "C" pth_status pth_burgers_estimate_timestep_noalloc(pth_meshdata *mdh, double cfl,
+                                                            double *dt_device) {
+  MeshData *md = Impl(mdh);
+  Mesh *m = md->mesh;
+  const int fu = NeedField(md, "U", 3);
+  if (fu < 0) return PTH_FAIL;
+  if (!dt_device) {
+    SetError("pth_burgers_estimate_timestep_noalloc: dt_device is null");
+    return PTH_FAIL;
+  }
+  const int need = BurgersDtPartials(m->topo);
+  if (need > kStaticPartials) {
+    SetError("pth_burgers_estimate_timestep_noalloc: the mesh needs " + std::to_string(need) +
+             " reduction partials, more than the " + std::to_string(kStaticPartials) +
+             " of the static scratch");
+    return PTH_FAIL;
+  }
+  double *partial = nullptr;
+  if (StaticScratch(&partial) != PTH_COMPLETE) return PTH_FAIL;
+  return BurgersDtToDevice(m->topo, m->fields[fu].ncomp, md->store[fu].data, m->coef, partial,
+                           kStaticPartials, cfl, dt_device, m->stream);
+}
