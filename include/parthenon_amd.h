@@ -77,6 +77,16 @@ pth_status pth_mesh_refine_sphere(pth_mesh *m, const double center[3], double ra
 /* StateDescriptor::AddField (src/interface/state_descriptor.cpp) for a cell-centred field of
  * ncomp components. Must precede pth_mesh_finalize. */
 pth_status pth_mesh_add_field(pth_mesh *m, const char *name, int ncomp, unsigned flags);
+/* Metadata::RegisterRefinementOps<ProlongationOp, RestrictionOp> (src/interface/metadata.hpp:
+ * 555) of one field, resolved like StateDescriptor::RefinementFncsToIDs (state_descriptor.hpp:
+ * 256): the shared prolongation of its coarse buffers into fine ghost cells, and of refined
+ * blocks' interiors on a remesh, is ProlongateSharedMinMod (the default) or
+ * ProlongateSharedLinear (src/prolong_restrict/pr_ops.hpp:166-289); restriction is
+ * RestrictAverage. Must precede pth_mesh_finalize. */
+enum { PTH_PROLONG_MINMOD = 0, PTH_PROLONG_LINEAR = 1 };
+enum { PTH_RESTRICT_AVERAGE = 0 };
+pth_status pth_mesh_register_refinement_ops(pth_mesh *m, const char *field, int prolong,
+                                            int restrict_op);
 /* Builds blocks and neighbour links on the host only (no device work); idempotent.
  * pth_mesh_finalize calls it first. */
 pth_status pth_mesh_plan(pth_mesh *m);
@@ -122,6 +132,40 @@ pth_status pth_field_upload(pth_meshdata *md, const char *field, int which, cons
                             size_t n);
 pth_status pth_field_download(pth_meshdata *md, const char *field, int which, double *host,
                               size_t n);
+
+/* ---- packs: SparsePack / VariablePack / VariableFluxPack over a MeshData ----
+ * PDOpt (src/interface/sparse_pack_base.hpp:46): with fluxes, coarse buffers, flattened
+ * (block, variable) index. */
+enum { PTH_PACK_WITH_FLUXES = 1u << 0, PTH_PACK_COARSE = 1u << 1, PTH_PACK_FLATTEN = 1u << 2 };
+
+/* Device view of one pack. Every pointer is device memory owned by the library and valid
+ * until the mesh is remeshed. Pack index v runs over the requested fields in order, a field
+ * of ncomp components taking ncomp consecutive indices; every block holds every field (no
+ * sparse fields on this path), so bounds are the same for all blocks. */
+typedef struct {
+  double *const *data;   /* [nblocks * nvar]: brick of (block, v), [k][j][i] over ent[] */
+  double *const *flux;   /* [3 * nblocks * nvar]: face-flux bricks of direction dir - 1, or NULL */
+  const int *bounds;     /* [2 * nfields]: first pack index of field f at f, last at nfields + f */
+  const double *coords;  /* [nblocks * 8]: dx1, dx2, dx3, area1, area2, area3, volume, 0 */
+  const double *geom;    /* [nblocks * 12]: x0[3], dx[3], coarse x0[3], coarse dx[3]; cell
+                            centre of index i along d is x0[d] + (i + 0.5) dx[d] */
+  int nblocks, nvar, nfields, options;
+  int ent[3];            /* brick extents (i, j, k); the coarse extents with PTH_PACK_COARSE */
+  int is, ie, js, je, ks, ke; /* interior index range (inclusive) of those bricks */
+} pth_pack_view;
+
+typedef struct pth_pack pth_pack;
+/* MakePackDescriptor(md, vars, {}, options).GetPack(md) and MeshData::PackVariables(
+ * AndFluxes)(vars) (src/interface/make_pack_descriptor.hpp:42-105, sparse_pack.hpp:53-409,
+ * sparse_pack_base.cpp:95-346, mesh_data.hpp:349-430): the pack of the named fields, built on
+ * first use and cached in md by (fields, options); after a remesh the next call rebuilds it.
+ * Fails (NULL) for an unknown field, for fluxes of a field without them, for coarse buffers
+ * on a single-level mesh. */
+pth_pack *pth_pack_get(pth_meshdata *md, const char *const *fields, int nfields,
+                       unsigned options);
+pth_status pth_pack_view_get(const pth_pack *p, pth_pack_view *view);
+/* GetLowerBoundHost / GetUpperBoundHost (sparse_pack.hpp:164-190) of requested field f. */
+pth_status pth_pack_bounds(const pth_pack *p, int field, int *lo, int *hi);
 
 /* ---- boundary communication (src/bvals/comms/bvals_in_one.hpp:41-93) ---- */
 /* SendBoundBufs<bt>: boundary_communication.cpp:49-159 (restrict + pack). */
@@ -205,8 +249,8 @@ pth_status pth_burgers_form(pth_mesh *m, int *sweep, int *overlap);
 pth_status pth_burgers_set_bit_exact(pth_mesh *m, int on);
 /* Driver tuning (no counterpart in the reference): planes per workgroup (1, 2 or 4) of the
  * plane-tiled reconstruction for its three forms (all faces, ghost-free faces, the rest), and
- * the block sweep's register budget (3 or 4: the waves per SIMD it is compiled for; with the
- * sweep's LDS the 4-wave build runs at 2 waves per SIMD without spills) as planes[3] of
+ * the block sweep's register budget (the waves per SIMD it is compiled for: 2 or 3 for 16^3
+ * blocks with 4 ghosts, 3 or 4 otherwise; other values are clamped) as planes[3] of
  * pth_burgers_tuning. pth_burgers_init picks them by timing each choice on the mesh's data
  * unless set here first; pth_burgers_tuning returns PTH_INCOMPLETE while no choice has been
  * made. Results do not depend on the choice. */

@@ -19,6 +19,7 @@
 #include <array>
 #include <cstdint>
 #include <map>
+#include <set>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -82,12 +83,62 @@ class MeshData {
     detail::Check(pth_field_download(h_, field.c_str(), which, host.data(), host.size()),
                   "Download");
   }
+  // MeshData::PackVariables / PackVariablesAndFluxes (mesh_data.hpp:349-430): the device view
+  // of the named fields (a parthenon::SparsePack in HIP code, parthenon_amd_pack.hpp)
+  pth_pack_view PackVariables(const std::vector<std::string> &vars) const {
+    return Pack(vars, 0);
+  }
+  pth_pack_view PackVariablesAndFluxes(const std::vector<std::string> &vars) const {
+    return Pack(vars, PTH_PACK_WITH_FLUXES);
+  }
+  pth_pack_view Pack(const std::vector<std::string> &vars, unsigned options) const {
+    std::vector<const char *> names;
+    for (const auto &v : vars) names.push_back(v.c_str());
+    pth_pack *p = pth_pack_get(h_, names.data(), static_cast<int>(names.size()), options);
+    if (!p) throw std::runtime_error(std::string("pack: ") + pth_last_error());
+    pth_pack_view view;
+    detail::Check(pth_pack_view_get(p, &view), "pack");
+    return view;
+  }
 
  private:
   Mesh *mesh_;
   pth_meshdata *h_;
   std::string name_;
 };
+
+// PDOpt (src/interface/sparse_pack_base.hpp:46)
+enum class PDOpt { WithFluxes, Coarse, Flatten };
+
+// MakePackDescriptor(md, vars, {}, options) (make_pack_descriptor.hpp:42-105) and
+// Descriptor::GetPack (sparse_pack.hpp:71-108): a named-field descriptor whose packs are
+// cached per MeshData; GetIdx(name) gives the field's PackIdx number.
+class PackDescriptor {
+ public:
+  PackDescriptor(std::vector<std::string> vars, const std::set<PDOpt> &options)
+      : vars_(std::move(vars)) {
+    for (PDOpt o : options)
+      opt_ |= o == PDOpt::WithFluxes ? PTH_PACK_WITH_FLUXES
+              : o == PDOpt::Coarse   ? PTH_PACK_COARSE
+                                     : PTH_PACK_FLATTEN;
+  }
+  pth_pack_view GetPack(const MeshData<Real> *md) const { return md->Pack(vars_, opt_); }
+  int GetIdx(const std::string &name) const {
+    for (size_t q = 0; q < vars_.size(); ++q)
+      if (vars_[q] == name) return static_cast<int>(q);
+    throw std::runtime_error("PackDescriptor: no variable " + name);
+  }
+  const std::vector<std::string> &vars() const { return vars_; }
+
+ private:
+  std::vector<std::string> vars_;
+  unsigned opt_ = 0;
+};
+
+inline PackDescriptor MakePackDescriptor(const MeshData<Real> *, std::vector<std::string> vars,
+                                         const std::set<PDOpt> &options = {}) {
+  return PackDescriptor(std::move(vars), options);
+}
 
 // Mesh: a uniform or statically refined Cartesian mesh of equal meshblocks on one GPU.
 class Mesh {
@@ -131,6 +182,14 @@ class Mesh {
   // StateDescriptor::AddField for a cell-centred field
   void AddField(const std::string &name, int ncomp, unsigned flags) {
     detail::Check(pth_mesh_add_field(h_, name.c_str(), ncomp, flags), "AddField");
+  }
+  // Metadata::RegisterRefinementOps<ProlongateSharedLinear, RestrictAverage>() of a field
+  // (metadata.hpp:555); ProlongateSharedMinMod is the default
+  enum class ProlongationOp { SharedMinMod = PTH_PROLONG_MINMOD, SharedLinear = PTH_PROLONG_LINEAR };
+  void RegisterRefinementOps(const std::string &field, ProlongationOp op) {
+    detail::Check(pth_mesh_register_refinement_ops(h_, field.c_str(), static_cast<int>(op),
+                                                   PTH_RESTRICT_AVERAGE),
+                  "RegisterRefinementOps");
   }
   void Initialize() { detail::Check(pth_mesh_finalize(h_), "Initialize"); }
 

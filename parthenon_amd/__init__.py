@@ -40,6 +40,12 @@ class BoundaryType(enum.IntEnum):  # src/basic_types.hpp:156
     flxcor_recv = 4
 
 
+class PDOpt(enum.IntFlag):  # src/interface/sparse_pack_base.hpp:46
+    WithFluxes = 1
+    Coarse = 2
+    Flatten = 4
+
+
 class Metadata(enum.IntFlag):  # the flags of src/interface/metadata.hpp used on this path
     FillGhost = 1
     Independent = 2
@@ -65,6 +71,15 @@ _I64 = C.c_int64
 _P = C.c_void_p
 _DP = C.POINTER(C.c_double)
 
+
+class PackView(C.Structure):
+    """pth_pack_view of include/parthenon_amd.h: device pointers and shape of one pack."""
+    _fields_ = [("data", C.c_void_p), ("flux", C.c_void_p), ("bounds", C.c_void_p),
+                ("coords", C.c_void_p), ("geom", C.c_void_p), ("nblocks", C.c_int),
+                ("nvar", C.c_int), ("nfields", C.c_int), ("options", C.c_int),
+                ("ent", C.c_int * 3), ("is_", C.c_int), ("ie", C.c_int), ("js", C.c_int),
+                ("je", C.c_int), ("ks", C.c_int), ("ke", C.c_int)]
+
 _SIGNATURES = {
     "pth_mesh_create": (_P, [C.POINTER(_I), C.POINTER(_I), _I, _DP, _DP, C.POINTER(_I)]),
     "pth_mesh_refine_box": (_I, [_P, _DP, _I]),
@@ -73,6 +88,7 @@ _SIGNATURES = {
     "pth_mesh_refine_sphere": (_I, [_P, _DP, _D, _I]),
     "pth_mesh_add_field": (_I, [_P, C.c_char_p, _I, C.c_uint]),
     "pth_mesh_plan": (_I, [_P]),
+    "pth_mesh_register_refinement_ops": (_I, [_P, C.c_char_p, _I, _I]),
     "pth_mesh_finalize": (_I, [_P]),
     "pth_mesh_destroy": (None, [_P]),
     "pth_mesh_set_stream": (_I, [_P, _P]),
@@ -87,6 +103,9 @@ _SIGNATURES = {
     "pth_mesh_exchange_counts": (_I, [_P, _I, C.POINTER(_I64), C.POINTER(_I64)]),
     "pth_meshdata_get": (_P, [_P, C.c_char_p]),
     "pth_field_pointers": (_I, [_P, C.c_char_p, C.POINTER(_DP), C.POINTER(_DP), C.POINTER(_DP)]),
+    "pth_pack_get": (_P, [_P, C.POINTER(C.c_char_p), _I, C.c_uint]),
+    "pth_pack_view_get": (_I, [_P, C.POINTER(PackView)]),
+    "pth_pack_bounds": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I)]),
     "pth_field_upload": (_I, [_P, C.c_char_p, _I, _DP, C.c_size_t]),
     "pth_field_download": (_I, [_P, C.c_char_p, _I, _DP, C.c_size_t]),
     "pth_send_bound_bufs": (_I, [_P, _I]),
@@ -223,6 +242,14 @@ class Mesh:
     def add_field(self, name, ncomp=1, flags=Metadata.FillGhost | Metadata.Independent):
         _check(lib().pth_mesh_add_field(self._h, name.encode(), ncomp, int(flags)), "add_field")
         self.fields[name] = (ncomp, int(flags))
+
+    def register_refinement_ops(self, name, prolong="minmod"):
+        """Metadata::RegisterRefinementOps for field name: shared prolongation "minmod"
+        (ProlongateSharedMinMod, the default) or "linear" (ProlongateSharedLinear);
+        restriction is RestrictAverage. Before finalize."""
+        op = {"minmod": 0, "linear": 1}[prolong]
+        _check(lib().pth_mesh_register_refinement_ops(self._h, name.encode(), op, 0),
+               "register_refinement_ops")
 
     def plan(self):
         _check(lib().pth_mesh_plan(self._h), "plan")
@@ -369,6 +396,22 @@ class MeshData:
         _check(lib().pth_field_download(self._h, field.encode(), which, _dptr(out), out.size),
                "download")
         return out
+
+    def pack(self, fields, options=0):
+        """MakePackDescriptor(md, fields, {}, options).GetPack(md) / PackVariables: the pack's
+        handle and its device view (PackView); options from PDOpt."""
+        names = (C.c_char_p * len(fields))(*[f.encode() for f in fields])
+        h = lib().pth_pack_get(self._h, names, len(fields), int(options))
+        if not h:
+            raise RuntimeError(f"pack failed: {lib().pth_last_error().decode()}")
+        view = PackView()
+        _check(lib().pth_pack_view_get(h, C.byref(view)), "pack view")
+        return h, view
+
+    def pack_bounds(self, handle, field):
+        lo, hi = _I(), _I()
+        _check(lib().pth_pack_bounds(handle, field, C.byref(lo), C.byref(hi)), "pack bounds")
+        return lo.value, hi.value
 
     def pointers(self, field):
         d, c, f = _DP(), _DP(), _DP()

@@ -234,6 +234,7 @@ void Mesh::Adopt(Mesh &nm) {
   std::swap(same_level, nm.same_level);
   for (BufState &st : state) st = BufState::kStale;
   flxcor_state = BufState::kStale;
+  ++generation;
 }
 
 pth_status Mesh::Adapt(int *modified) {
@@ -346,10 +347,10 @@ pth_status Mesh::Adapt(int *modified) {
         }
   }
   // region tables: restriction on the old list, box copies old -> new, prolongation on the new
-  Table rt, ct, pt;
+  Table rt, ct, pt, pl;  // pl: prolongation of fields registered with the linear operator
   const Extent &F = topo.fine, &Cx = topo.coarse;
   const std::int64_t fcells = F.cells(), ccells = Cx.cells();
-  auto add_field = [&](const FieldStore &os, const FieldStore &ns, int nc) -> bool {
+  auto add_field = [&](const FieldStore &os, const FieldStore &ns, int nc, bool linear) -> bool {
     if (!os.data || !ns.data) return true;
     if (!os.coarse || !ns.coarse) return false;
     for (int n = 0; n < nbn; ++n) {
@@ -371,7 +372,7 @@ pth_status Mesh::Adapt(int *modified) {
         double *cdst = ns.coarse + static_cast<std::int64_t>(n) * nc * ccells;
         ct.host.push_back(MakeBox(os.data + static_cast<std::int64_t>(o.old[0]) * nc * fcells,
                                   cdst, lo, sz, nc, n, F, Cx));
-        pt.host.push_back(MakeBox(dst, cdst, plo, psz, nc, n, F, Cx));
+        (linear ? pl : pt).host.push_back(MakeBox(dst, cdst, plo, psz, nc, n, F, Cx));
       } else {
         for (int q = 0; q < o.n; ++q) {
           double *fsrc = os.data + static_cast<std::int64_t>(o.old[q]) * nc * fcells;
@@ -393,7 +394,8 @@ pth_status Mesh::Adapt(int *modified) {
     MeshData &nmd = *nm->data.at(kv.first);
     for (size_t f = 0; f < fields.size(); ++f) {
       if (fields[f].flags & PTH_ONE_COPY) continue;
-      if (!add_field(om.store[f], nmd.store[f], fields[f].ncomp)) {
+      if (!add_field(om.store[f], nmd.store[f], fields[f].ncomp,
+                     fields[f].prolong == PTH_PROLONG_LINEAR)) {
         SetError("internal: remesh needs coarse buffers");
         return PTH_FAIL;
       }
@@ -401,7 +403,8 @@ pth_status Mesh::Adapt(int *modified) {
   }
   for (size_t f = 0; f < fields.size(); ++f)
     if (fields[f].flags & PTH_ONE_COPY)
-      if (!add_field(one_copy[f], nm->one_copy[f], fields[f].ncomp)) {
+      if (!add_field(one_copy[f], nm->one_copy[f], fields[f].ncomp,
+                     fields[f].prolong == PTH_PROLONG_LINEAR)) {
         SetError("internal: remesh needs coarse buffers");
         return PTH_FAIL;
       }
@@ -409,14 +412,17 @@ pth_status Mesh::Adapt(int *modified) {
     rt.Free();
     ct.Free();
     pt.Free();
+    pl.Free();
     return PTH_FAIL;
   };
-  rt.item_elems = pt.item_elems = kProResItemElems;
-  if (rt.Upload() != PTH_COMPLETE || ct.Upload() != PTH_COMPLETE || pt.Upload() != PTH_COMPLETE)
+  rt.item_elems = pt.item_elems = pl.item_elems = kProResItemElems;
+  if (rt.Upload() != PTH_COMPLETE || ct.Upload() != PTH_COMPLETE || pt.Upload() != PTH_COMPLETE ||
+      pl.Upload() != PTH_COMPLETE)
     return fail();
   if (LaunchRestrictT(rt, topo, geom, stream) != PTH_COMPLETE) return fail();
   if (LaunchCopy(ct, stream) != PTH_COMPLETE) return fail();
   if (LaunchProlongT(pt, nt, nm->geom, stream, true) != PTH_COMPLETE) return fail();
+  if (LaunchProlongT(pl, nt, nm->geom, stream, false) != PTH_COMPLETE) return fail();
   if (hipStreamSynchronize(stream) != hipSuccess) {
     SetError("remesh transfer failed on the device");
     return fail();
@@ -424,6 +430,7 @@ pth_status Mesh::Adapt(int *modified) {
   rt.Free();
   ct.Free();
   pt.Free();
+  pl.Free();
   // flags: moved blocks keep theirs, new blocks start at 0 (MeshRefinement is rebuilt)
   std::vector<int> fl(nbn, 0), cnt(nbn, 0);
   for (int n = 0; n < nbn; ++n)

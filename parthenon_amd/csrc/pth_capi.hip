@@ -72,11 +72,18 @@ MeshData::~MeshData() {
     c.restrict_send.Free();
     c.restrict_set.Free();
     c.prolong.Free();
+    c.prolong_linear.Free();
     c.bc[0].Free();
     c.bc[1].Free();
     c.pack_rest.Free();
     c.unpack_rest.Free();
   }
+  for (auto &kv : packs) kv.second.Free();
+}
+
+void PackEntry::Free() {
+  if (dev) (void)hipFree(dev);
+  dev = nullptr;
 }
 
 Mesh::~Mesh() {
@@ -298,6 +305,7 @@ pth_status Mesh::BuildCache(MeshData &md, int bt) {
   c.restrict_send.host.clear();
   c.restrict_set.host.clear();
   c.prolong.host.clear();
+  c.prolong_linear.host.clear();
   c.nbuffers = 0;
   if (bt == PTH_BT_NONLOCAL) {  // one rank: no neighbour is remote
     c.built = true;
@@ -339,8 +347,10 @@ pth_status Mesh::BuildCache(MeshData &md, int bt) {
         if (coarser) {
           c.restrict_send.host.push_back(MakeRegion(
               fine, crs, topo.Indices(B, l, Side::kSendBoundary, true), nc, b, fe, &ce));
-          c.prolong.host.push_back(MakeRegion(
-              fine, crs, topo.Indices(B, l, Side::kRecvBoundary, true), nc, b, fe, &ce));
+          (fields[f].prolong == PTH_PROLONG_LINEAR ? c.prolong_linear : c.prolong)
+              .host.push_back(MakeRegion(fine, crs,
+                                         topo.Indices(B, l, Side::kRecvBoundary, true), nc, b,
+                                         fe, &ce));
         } else if (B.coarser_neighbor && B.at.level > 0) {
           c.restrict_set.host.push_back(MakeRegion(
               fine, crs, topo.Indices(B, l, Side::kRecvBoundary, true), nc, b, fe, &ce));
@@ -438,7 +448,11 @@ pth_status Mesh::BuildCache(MeshData &md, int bt) {
     host.swap(merged);
   };
   merge(c.restrict_send.host, nullptr);
-  if (all_periodic) merge(c.restrict_set.host, &c.prolong.host);
+  if (all_periodic) {
+    std::vector<Region> reach = c.prolong.host;
+    reach.insert(reach.end(), c.prolong_linear.host.begin(), c.prolong_linear.host.end());
+    merge(c.restrict_set.host, &reach);
+  }
   // Every element a kernel will touch must lie inside its array: pack/unpack boxes inside the
   // field brick, restriction children and prolongation stencils inside fine and coarse bricks.
   auto inside = [](const Region &r, const Extent &e, int halo_lo, int halo_hi, int scale,
@@ -463,8 +477,9 @@ pth_status Mesh::BuildCache(MeshData &md, int bt) {
   for (const Table *t : {&c.restrict_send, &c.restrict_set})
     for (const Region &r : t->host)
       ok = ok && inside(r, ce, 0, 0, 1, 0, 0) && inside(r, fe, 0, 0, 2, ce.lo(0), fe.lo(0));
-  for (const Region &r : c.prolong.host)
-    ok = ok && inside(r, ce, 1, 1, 1, 0, 0) && inside(r, fe, 0, 0, 2, ce.lo(0), fe.lo(0));
+  for (const Table *t : {&c.prolong, &c.prolong_linear})
+    for (const Region &r : t->host)
+      ok = ok && inside(r, ce, 1, 1, 1, 0, 0) && inside(r, fe, 0, 0, 2, ce.lo(0), fe.lo(0));
   if (!ok) {
     SetError("internal: a boundary region exceeds its array extents");
     return PTH_FAIL;
@@ -475,8 +490,10 @@ pth_status Mesh::BuildCache(MeshData &md, int bt) {
   for (Table *t : {&c.pack, &c.unpack})
     std::sort(t->host.begin(), t->host.end(),
               [](const Region &x, const Region &y) { return x.b < y.b; });
-  for (Table *t : {&c.restrict_send, &c.restrict_set, &c.prolong}) t->item_elems = kProResItemElems;
-  for (Table *t : {&c.pack, &c.unpack, &c.restrict_send, &c.restrict_set, &c.prolong})
+  for (Table *t : {&c.restrict_send, &c.restrict_set, &c.prolong, &c.prolong_linear})
+    t->item_elems = kProResItemElems;
+  for (Table *t : {&c.pack, &c.unpack, &c.restrict_send, &c.restrict_set, &c.prolong,
+                   &c.prolong_linear})
     if (t->Upload() != PTH_COMPLETE) return PTH_FAIL;
   if (BuildBoundaryConditions(md, c) != PTH_COMPLETE) return PTH_FAIL;
   c.built = true;
@@ -856,6 +873,27 @@ pth_status pth_mesh_add_field(pth_mesh *mh, const char *name, int ncomp, unsigne
   return PTH_COMPLETE;
 }
 
+pth_status pth_mesh_register_refinement_ops(pth_mesh *mh, const char *field, int prolong,
+                                            int restrict_op) {
+  Mesh *m = Impl(mh);
+  if (m->finalized) {
+    SetError("refinement operators must be registered before finalize");
+    return PTH_FAIL;
+  }
+  const int f = field ? m->FieldIndex(field) : -1;
+  if (f < 0) {
+    SetError(std::string("no field ") + (field ? field : "(null)"));
+    return PTH_FAIL;
+  }
+  if ((prolong != PTH_PROLONG_MINMOD && prolong != PTH_PROLONG_LINEAR) ||
+      restrict_op != PTH_RESTRICT_AVERAGE) {
+    SetError("unknown refinement operator");
+    return PTH_FAIL;
+  }
+  m->fields[f].prolong = prolong;
+  return PTH_COMPLETE;
+}
+
 pth_status pth_mesh_plan(pth_mesh *mh) { return Impl(mh)->Plan(); }
 
 pth_status pth_mesh_finalize(pth_mesh *mh) { return Impl(mh)->Finalize(); }
@@ -1041,6 +1079,123 @@ pth_status pth_field_pointers(pth_meshdata *mdh, const char *field, double **dat
   return PTH_COMPLETE;
 }
 
+// ---- packs ------------------------------------------------------------------------------
+pth_pack *pth_pack_get(pth_meshdata *mdh, const char *const *fields, int nfields,
+                       unsigned options) {
+  MeshData *md = Impl(mdh);
+  Mesh *m = md->mesh;
+  if (!fields || nfields < 1) {
+    SetError("pth_pack_get: no fields");
+    return nullptr;
+  }
+  if (options & ~static_cast<unsigned>(PTH_PACK_WITH_FLUXES | PTH_PACK_COARSE |
+                                       PTH_PACK_FLATTEN)) {
+    SetError("pth_pack_get: unknown option");
+    return nullptr;
+  }
+  const bool with_flux = options & PTH_PACK_WITH_FLUXES, coarse = options & PTH_PACK_COARSE;
+  std::string key = std::to_string(options);
+  std::vector<int> fid;
+  for (int q = 0; q < nfields; ++q) {
+    const int f = fields[q] ? m->FieldIndex(fields[q]) : -1;
+    if (f < 0) {
+      SetError(std::string("pth_pack_get: no field ") + (fields[q] ? fields[q] : "(null)"));
+      return nullptr;
+    }
+    if (with_flux && !md->store[f].flux) {
+      SetError(std::string("pth_pack_get: field ") + fields[q] + " has no fluxes");
+      return nullptr;
+    }
+    if (coarse && !md->store[f].coarse) {
+      SetError(std::string("pth_pack_get: field ") + fields[q] +
+               " has no coarse buffer (single-level mesh)");
+      return nullptr;
+    }
+    fid.push_back(f);
+    key += "|";
+    key += fields[q];
+  }
+  PackEntry &e = md->packs[key];
+  if (e.dev && e.generation == m->generation)
+    return reinterpret_cast<pth_pack *>(&e);
+  e.Free();
+  const int nb = static_cast<int>(m->topo.blocks.size());
+  const Extent &x = coarse ? m->topo.coarse : m->topo.fine;
+  const std::int64_t cells = x.cells();
+  int nvar = 0;
+  e.bounds.assign(2 * nfields, 0);
+  for (int q = 0; q < nfields; ++q) {
+    e.bounds[q] = nvar;
+    nvar += m->fields[fid[q]].ncomp;
+    e.bounds[nfields + q] = nvar - 1;
+  }
+  std::vector<double *> ptr(static_cast<size_t>(nb) * nvar * (with_flux ? 4 : 1));
+  for (int b = 0; b < nb; ++b)
+    for (int q = 0; q < nfields; ++q) {
+      const FieldStore &fs = md->store[fid[q]];
+      const int nc = m->fields[fid[q]].ncomp;
+      for (int c = 0; c < nc; ++c) {
+        const std::int64_t off = (static_cast<std::int64_t>(b) * nc + c) * cells;
+        const int v = e.bounds[q] + c;
+        ptr[static_cast<size_t>(b) * nvar + v] = (coarse ? fs.coarse : fs.data) + off;
+        if (with_flux)
+          for (int d = 0; d < 3; ++d)
+            ptr[static_cast<size_t>(nb) * nvar * (1 + d) + static_cast<size_t>(b) * nvar + v] =
+                fs.flux + d * fs.ndata + off;
+      }
+    }
+  const size_t pbytes = ptr.size() * sizeof(double *);
+  const size_t bbytes = e.bounds.size() * sizeof(int);
+  if (hipMalloc(&e.dev, pbytes + bbytes) != hipSuccess ||
+      hipMemcpy(e.dev, ptr.data(), pbytes, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(static_cast<char *>(e.dev) + pbytes, e.bounds.data(), bbytes,
+                hipMemcpyHostToDevice) != hipSuccess) {
+    SetError("pth_pack_get: device table upload failed");
+    e.Free();
+    return nullptr;
+  }
+  pth_pack_view &v = e.view;
+  v.data = static_cast<double *const *>(e.dev);
+  v.flux = with_flux ? static_cast<double *const *>(e.dev) + static_cast<size_t>(nb) * nvar
+                     : nullptr;
+  v.bounds = reinterpret_cast<const int *>(static_cast<char *>(e.dev) + pbytes);
+  v.coords = reinterpret_cast<const double *>(m->coef);
+  v.geom = reinterpret_cast<const double *>(m->geom);
+  v.nblocks = nb;
+  v.nvar = nvar;
+  v.nfields = nfields;
+  v.options = static_cast<int>(options);
+  for (int d = 0; d < 3; ++d) v.ent[d] = x.ent(d);
+  v.is = x.lo(0);
+  v.ie = x.hi(0);
+  v.js = x.lo(1);
+  v.je = x.hi(1);
+  v.ks = x.lo(2);
+  v.ke = x.hi(2);
+  e.generation = m->generation;
+  return reinterpret_cast<pth_pack *>(&e);
+}
+
+pth_status pth_pack_view_get(const pth_pack *p, pth_pack_view *view) {
+  if (!p || !view) {
+    SetError("pth_pack_view_get: null argument");
+    return PTH_FAIL;
+  }
+  *view = reinterpret_cast<const PackEntry *>(p)->view;
+  return PTH_COMPLETE;
+}
+
+pth_status pth_pack_bounds(const pth_pack *p, int field, int *lo, int *hi) {
+  const PackEntry *e = reinterpret_cast<const PackEntry *>(p);
+  if (!e || field < 0 || field >= e->view.nfields) {
+    SetError("pth_pack_bounds: no such field in the pack");
+    return PTH_FAIL;
+  }
+  if (lo) *lo = e->bounds[field];
+  if (hi) *hi = e->bounds[e->view.nfields + field];
+  return PTH_COMPLETE;
+}
+
 static pth_status FieldBuffer(MeshData *md, const char *field, int which, double **p,
                               std::int64_t *n) {
   const int f = md->mesh->FieldIndex(field);
@@ -1200,7 +1355,9 @@ pth_status pth_prolongate_bounds(pth_meshdata *mdh, int bt) {
   if (m->BuildCache(*md, bt) != PTH_COMPLETE) return PTH_FAIL;
   BoundCache &c = md->cache[bt];
   if (c.nbuffers == 0 || !m->topo.multilevel) return PTH_COMPLETE;
-  return LaunchProlongT(c.prolong, m->topo, m->geom, m->stream, true);
+  if (LaunchProlongT(c.prolong, m->topo, m->geom, m->stream, true) != PTH_COMPLETE)
+    return PTH_FAIL;
+  return LaunchProlongT(c.prolong_linear, m->topo, m->geom, m->stream, false);
 }
 
 // ---- Burgers package and Update:: tasks ----

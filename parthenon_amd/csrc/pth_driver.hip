@@ -64,6 +64,9 @@ pth_status Exchange(Mesh &m, MeshData &md, hipStream_t s, bool rest = false) {
     if (LaunchBoundaryConditions(c.bc[1], s) != PTH_COMPLETE) return PTH_FAIL;
     if (c.nbuffers > 0 && LaunchProlongT(c.prolong, m.topo, m.geom, s, true) != PTH_COMPLETE)
       return PTH_FAIL;
+    if (c.nbuffers > 0 &&
+        LaunchProlongT(c.prolong_linear, m.topo, m.geom, s, false) != PTH_COMPLETE)
+      return PTH_FAIL;
   }
   return LaunchBoundaryConditions(c.bc[0], s);
 }
@@ -284,7 +287,11 @@ pth_status TuneReconFlux(Ctx &x, hipStream_t s) {
   // the interior of stage "1", which the next stage overwrites
   if (st == PTH_COMPLETE && BurgersSweepSupported(m.topo, x.V)) {
     float best = 0.0f;
-    for (int waves : {4, 3}) {
+    // the register-ring kernel of 16^3 blocks is built for 3 or 2 waves per SIMD, the general
+    // one for 4 or 3
+    const bool ring = m.topo.fine.n[0] == 16 && m.topo.fine.g[0] == 4;
+    const int budgets[2] = {ring ? 3 : 4, ring ? 2 : 3};
+    for (int waves : budgets) {
       st = BurgersSweep(m.topo, x.V, U, U, x.one->store[x.fu].data, nullptr, nullptr,
                         static_cast<int>(m.scratch_n), m.coef, 1.0, m.dtvar, m.run_recon, x.fc,
                         waves, 1, nullptr, nullptr, s);
@@ -302,7 +309,7 @@ pth_status TuneReconFlux(Ctx &x, hipStream_t s) {
         st = PTH_FAIL;
         break;
       }
-      if (waves == 4 || ms < best) {
+      if (waves == budgets[0] || ms < best) {
         best = ms;
         m.sweep_waves = waves;
       }
@@ -567,8 +574,8 @@ pth_status pth_burgers_set_tuning(pth_mesh *mh, const int planes[4]) {
       SetError("planes per workgroup must be 1, 2 or 4");
       return PTH_FAIL;
     }
-  if (planes[3] != 3 && planes[3] != 4) {
-    SetError("block-sweep waves per SIMD must be 3 or 4");
+  if (planes[3] < 2 || planes[3] > 4) {
+    SetError("block-sweep waves per SIMD must be 2, 3 or 4");
     return PTH_FAIL;
   }
   for (int p = 0; p < 3; ++p) m->rf_ppw[p] = planes[p];

@@ -226,6 +226,7 @@ struct FieldSpec {
   std::string name;
   int ncomp;
   unsigned flags;
+  int prolong = PTH_PROLONG_MINMOD;  // registered shared-prolongation operator
 };
 
 struct FieldStore {
@@ -284,6 +285,7 @@ struct BcTable {
 struct BoundCache {  // per (MeshData, bound type): the pack / unpack / pro-res tables
   bool built = false;
   Table pack, unpack, restrict_send, restrict_set, prolong;
+  Table prolong_linear;  // prolongation regions of fields registered with the linear operator
   BcTable bc[2];  // physical boundary conditions on data (0) and coarse buffers (1)
   std::int64_t nbuffers = 0;
   // direct-exchange index pairs without the same-level channels of field rest_field (filled
@@ -303,11 +305,25 @@ struct ClockLimits {
   int dt_init_force = 0;
 };
 
+// One pack over a MeshData (reference SparsePack / VariablePack / VariableFluxPack): a device
+// table of (block, variable) brick pointers, the face-flux bricks with PTH_PACK_WITH_FLUXES,
+// and the first / last pack index of every requested field. Built on first request and cached
+// by (fields, options) like SparsePackCache (sparse_pack_base.cpp:319-346); a remesh (which
+// moves every brick) makes the next request rebuild it.
+struct PackEntry {
+  void *dev = nullptr;              // one allocation: data pointers, flux pointers, bounds
+  pth_pack_view view{};
+  std::vector<int> bounds;          // host copy, [2][nfields]
+  int generation = -1;              // mesh generation the pointers belong to
+  void Free();
+};
+
 struct MeshData {
   Mesh *mesh;
   std::string name;
   std::vector<FieldStore> store;  // indexed like Mesh::fields
   BoundCache cache[10];
+  std::map<std::string, PackEntry> packs;  // keyed by options and field names
   ~MeshData();
 };
 
@@ -358,6 +374,7 @@ class Mesh {
   bool run_ready = false;
   hipStream_t stream = nullptr;
   int pack_mode = PTH_PACK_PERSISTENT;
+  int generation = 0;      // counts remeshes (packs built before one are rebuilt)
   bool planned = false;    // host topology built
   bool finalized = false;  // device storage, tables and buffers built
   double *scratch = nullptr;                    // reduction partials

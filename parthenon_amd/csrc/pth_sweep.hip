@@ -110,12 +110,118 @@ __device__ __forceinline__ void SweepHll(LS l, RS r, int d, double *F) {
   }
 }
 
+// ---- helpers of the 16^3 block sweep ---------------------------------------------------
+//
+// Reciprocal by v_rcp_f64 and two Newton-Raphson steps r <- r + r (1 - x r) (each step
+// roughly doubles the correct bits; the hardware estimate is good to well over 20 bits, so
+// the result is within about an ulp), five instructions instead of the ten of an IEEE divide.
+// The callers' arguments are positive normal numbers (sums of squares plus robust::EPS, or
+// HLL denominators that are >= 1 whenever the speeds' product underflows).
+__device__ __forceinline__ double Recip(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// WENO5Z of recon.hpp:28-99 for the 16^3 block sweep, straight-line code with six
+// reciprocals by Recip instead of eight divides: one reciprocal of be0 be1 be2 gives the three
+// 1 / be_k; each side's weights enter through 1 / (S D) (S = w0 + w1 + w2, D = g2 w0 w1 +
+// g1 w0 w2 + g0 w1 w2, so that ql = N D / (S D) and al = 3 w0 w1 w2 / (S D) + eps); alin's
+// harmonic mean takes the sixth. The values are the reference's to rounding. Range: be_k >=
+// eps and w_k <= g_k (1 + tau5 / eps), so be0 be1 be2 stays finite below a jump in q of about
+// 1e50 and S D ~ w^3 below a jump of about 1e43 beside a flat stencil; the reference's own
+// expression overflows (w^2) near 1e69.
+__device__ __forceinline__ void Weno5ZFast(double q0, double q1, double q2, double q3,
+                                           double q4, double &ql, double &qr) {
+  const double a00 = 1.0 / 3.0, a01 = -7.0 / 6.0, a02 = 11.0 / 6.0;
+  const double a10 = -1.0 / 6.0, a11 = 5.0 / 6.0, a12 = 1.0 / 3.0;
+  const double a20 = 1.0 / 3.0, a21 = 5.0 / 6.0, a22 = -1.0 / 6.0;
+  const double g0 = 0.1, g1 = 0.6, g2 = 0.3;
+  const double eps = 10.0 * DBL_EPSILON;  // robust::EPS (src/utils/robust.hpp:38-41)
+  const double t13 = 13.0 / 3.0;
+  double a = q0 - 2 * q1 + q2;
+  double b = q0 - 4.0 * q1 + 3.0 * q2;
+  const double be0 = t13 * a * a + b * b + eps;
+  a = q1 - 2.0 * q2 + q3;
+  b = q3 - q1;
+  const double be1 = t13 * a * a + b * b + eps;
+  a = q2 - 2.0 * q3 + q4;
+  b = q4 - 4.0 * q3 + 3.0 * q2;
+  const double be2 = t13 * a * a + b * b + eps;
+  const double tau5 = fabs(be2 - be0);
+  const double p12 = be1 * be2, p02 = be0 * be2, p01 = be0 * be1;
+  const double rp = Recip(be0 * p12);
+  const double z0 = (be0 + tau5) * (rp * p12);
+  const double z1 = (be1 + tau5) * (rp * p02);
+  const double z2 = (be2 + tau5) * (rp * p01);
+  const double l0 = g0 * z0 + eps, l1 = g1 * z1 + eps, l2 = g2 * z2 + eps;
+  const double nl = l0 * (a00 * q0 + a01 * q1 + a02 * q2) + l1 * (a10 * q1 + a11 * q2 + a12 * q3) +
+                    l2 * (a20 * q2 + a21 * q3 + a22 * q4);
+  const double dl = g2 * l0 * l1 + g1 * l0 * l2 + g0 * l1 * l2;
+  const double il = Recip((l0 + l1 + l2) * dl);  // 1 / (S_l D_l)
+  const double r0 = g0 * z2 + eps, r1 = g1 * z1 + eps, r2 = g2 * z0 + eps;
+  const double nr = r0 * (a00 * q4 + a01 * q3 + a02 * q2) + r1 * (a10 * q3 + a11 * q2 + a12 * q1) +
+                    r2 * (a20 * q2 + a21 * q1 + a22 * q0);
+  const double dr = g2 * r0 * r1 + g1 * r0 * r2 + g0 * r1 * r2;
+  const double ir = Recip((r0 + r1 + r2) * dr);  // 1 / (S_r D_r)
+  const double wl = nl * dl * il, wr = nr * dr * ir;
+  const double al = 3.0 * l0 * l1 * l2 * il + eps;
+  const double ar = 3.0 * r0 * r1 * r2 * ir + eps;
+  double dq = q3 - q2;
+  dq = 0.5 * Mc(q2 - q1, dq);
+  const double alin = 2.0 * al * ar * Recip(al + ar);
+  ql = alin * wl + (1.0 - alin) * (q2 + dq);
+  qr = alin * wr + (1.0 - alin) * (q2 - dq);
+}
+
+__device__ __forceinline__ void FastStates(int recon, double q0, double q1, double q2, double q3,
+                                           double q4, double &ql, double &qr) {
+  if (recon == 0)
+    Weno5ZFast(q0, q1, q2, q3, q4, ql, qr);
+  else
+    LinearRecon(q1, q2, q3, ql, qr);
+}
+
+// HLL of burgers_package.cpp:307-401 (lr_to_flux, burgers_package.hpp:33-43) as two
+// coefficients per face: with sl, sr the signal speeds and islsr = 1 / (sr - sl + (sl sr ==
+// 0)), the scalar flux (sr upl ql - sl upr qr + sl sr (qr - ql)) islsr equals a ql + b qr with
+// a = islsr sr (upl - sl), b = islsr sl (sr - upr), and the velocity flux is half of the same
+// combination of ul, ur. So the coupling of the components goes through (a, b) only.
+__device__ __forceinline__ void HllCoef(double upl, double upr, double &a, double &b) {
+  const double sl = DMin(DMin(upl, upr), 0.0);
+  const double sr = DMax(DMax(upl, upr), 0.0);
+  const double is = Recip(sr - sl + (sl * sr == 0.0));
+  a = (is * sr) * (upl - sl);
+  b = (is * sl) * (sr - upr);
+}
+
+// flux of component n from the face coefficients and its left / right states
+__device__ __forceinline__ double HllFluxN(int n, double a, double b, double l, double r) {
+  if (n < 3) return fma(0.5 * a, l, (0.5 * b) * r);
+  return fma(a, l, b * r);
+}
+
+// value of lane - 1 / lane + 1 within each 16-lane DPP row (the row's first / last lane
+// gets 0; callers replace those)
+__device__ __forceinline__ double FromLeft16(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x111, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x111, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double FromRight16(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x101, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x101, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
 // flxcor_send of the block-sweep stage: four adjacent threads per coarse face cell of a patch,
 // one per fine face (p, q), evaluate the HLL flux of their face straight from the fine block's
 // state (the left state of face f is ql of cell f - 1, the right state qr of cell f, as
 // CalculateFluxes stores them); the first of the four gathers the other three by lane
 // shuffles and writes their restriction to the correction store.
-template <int V>
+template <int V, bool RING>
 __global__ void __launch_bounds__(64)
     FlxcorDirectKernel(const FluxPatch *__restrict__ patches, const double *__restrict__ U,
                        double *__restrict__ store, Geo G, int recon) {
@@ -133,13 +239,30 @@ __global__ void __launch_bounds__(64)
   if (live && p < P.step[0] && q < P.step[1]) {
     const unsigned o = 8u * static_cast<unsigned>(PatchFineCell(P, G, a, c, p, q));
     double l[V], r[V];
+    if (RING) {
+      // the expressions of BlockSweep16Kernel: FastStates and the face coefficients
 #pragma unroll
-    for (int n = 0; n < V; ++n) {
-      double x, y;
-      SweepStates(Ub + n * cs, o - st, st, recon, l[n], x);
-      SweepStates(Ub + n * cs, o, st, recon, y, r[n]);
+      for (int n = 0; n < V; ++n) {
+        const double *q = Ub + n * cs;
+        double x, y;
+        FastStates(recon, LoadAt(q, o - 3 * st), LoadAt(q, o - 2 * st), LoadAt(q, o - st),
+                   LoadAt(q, o), LoadAt(q, o + st), l[n], x);
+        FastStates(recon, LoadAt(q, o - 2 * st), LoadAt(q, o - st), LoadAt(q, o),
+                   LoadAt(q, o + st), LoadAt(q, o + 2 * st), y, r[n]);
+      }
+      double a, bb;
+      HllCoef(l[P.d], r[P.d], a, bb);
+#pragma unroll
+      for (int n = 0; n < V; ++n) F[n] = HllFluxN(n, a, bb, l[n], r[n]);
+    } else {
+#pragma unroll
+      for (int n = 0; n < V; ++n) {
+        double x, y;
+        SweepStates(Ub + n * cs, o - st, st, recon, l[n], x);
+        SweepStates(Ub + n * cs, o, st, recon, y, r[n]);
+      }
+      SweepHll<V>([&](int n) { return l[n]; }, [&](int n) { return r[n]; }, P.d, F);
     }
-    SweepHll<V>([&](int n) { return l[n]; }, [&](int n) { return r[n]; }, P.d, F);
   }
   const int lane0 = static_cast<int>(threadIdx.x) & ~3;
   const std::int64_t dst = P.dst + a + static_cast<std::int64_t>(c) * P.rows;
@@ -384,6 +507,290 @@ __global__ void __launch_bounds__(N * N, MINW)
     partial[blockIdx.x] = v;
   }
 }
+
+// ---- block sweep for 16^3 blocks: register-ring z stencils, LDS plane, DPP neighbours ------
+//
+// One workgroup of 256 threads marches the z planes of a block; thread (tx, ty) owns column
+// (i, j) = (tx, ty). Per plane kk:
+//   * the plane's U (all components, with 3-cell x and y halos) is staged in LDS; its interior
+//     comes from the thread's register ring, the halos from HBM;
+//   * x: each thread reconstructs its own cell (stencil read from LDS); the neighbouring
+//     cells' states come by DPP row shifts (a row of 16 cells is a 16-lane DPP row), those of
+//     the cells beyond the row ends from a small pass over all components; the face
+//     coefficients (a, b) of HllCoef follow from the x velocity's states, then every
+//     component's two face fluxes are two multiply-adds each;
+//   * y: the same with the thread mapping transposed (thread (tx, ty) takes cell i = ty,
+//     j = tx, so a column is a DPP row); the y differences reach the owning thread through
+//     LDS;
+//   * z: each thread keeps U of its column at planes kk - 1, kk + 1, kk + 2, kk + 3 in
+//     registers (kk itself is in LDS), so the z stencil of the cell above needs no load; the
+//     high z face's flux becomes the next plane's low face flux; the load of plane kk + 4 is
+//     issued a whole plane ahead of its use.
+// The divergence adds x, y and z in the order of FluxDivHelper (update.hpp:43-57); the RK
+// averaging and update, derived field, CFL minimum, flux-correction faces and same-level push
+// are those of BlockSweepKernel. Each U element is read from HBM once per workgroup (plus
+// the halos and the segment's first planes), and face states and fluxes never leave the CU.
+template <int V, int MINW>
+__global__ void __launch_bounds__(256, MINW)
+    BlockSweep16Kernel(const double *__restrict__ U, const double *base, double *out,
+                       double *__restrict__ derived, double *__restrict__ partial,
+                       const BlockCoefs *__restrict__ coef, int nblocks, double beta,
+                       const double *__restrict__ dtvar, int recon, const int *__restrict__ fover,
+                       const double *__restrict__ fcor, const int *__restrict__ push) {
+  constexpr int N = 16, NG = 4, E = N + 2 * NG, NT = N * N;
+  constexpr int H = 3;             // halo of the staged plane
+  constexpr int PW = N + 2 * H;    // staged rows / columns
+  constexpr int PP = PW + 1;       // row pitch (odd, fewer bank conflicts on column reads)
+  constexpr int PS = PW * PP;      // one component of the staged plane
+  constexpr int SP = N + 1;        // pitch of the difference tile
+  constexpr int NX = V * N * 2;    // beyond-the-row states of one direction
+  constexpr std::int64_t cs = static_cast<std::int64_t>(E) * E * E;
+  constexpr int comp = N * N;      // correction-store stride between components
+  static_assert(V * NT <= PS * V && V * NT <= V * N * SP, "segment start scratch fits");
+  __shared__ double pl[V * PS];      // [n][j + H][i + H]
+  __shared__ double ext[2 * NX];     // [dir][n][line][side]: side 0 ql of cell -1, 1 qr of cell N
+  __shared__ double sd[V * N * SP];  // x + y differences [n][j][i]
+  __shared__ double wmin[NT / 64];
+  const int tx = threadIdx.x % N, ty = threadIdx.x / N;
+  const double bdt = beta * dtvar[0];
+  const double w2 = 1.0 - beta;
+  const bool same_base = base == U;
+  double m = DBL_MAX;
+  const std::int64_t T = static_cast<std::int64_t>(nblocks) * N;
+  std::int64_t pnext = T * blockIdx.x / gridDim.x;
+  const std::int64_t pend = T * (blockIdx.x + 1) / gridDim.x;
+  const double *own_pl = pl + (ty + H) * PP + tx + H;  // own cell, component 0
+  const int own_sd = ty * SP + tx;                      // own cell in the difference tile
+  while (pnext < pend) {
+    const int b = static_cast<int>(pnext / N);
+    const int k0 = static_cast<int>(pnext % N);
+    const int k1 = static_cast<int>(pend - pnext < N - k0 ? k0 + (pend - pnext) : N);
+    pnext += k1 - k0;
+    const double *Ub = U + static_cast<std::int64_t>(b) * V * cs;
+    const BlockCoefs &C = coef[b];
+    const double ivol = 1.0 / C.vol;
+    const int ox0 = fover ? fover[6 * b + 0] : -1, ox1 = fover ? fover[6 * b + 1] : -1;
+    const int oy0 = fover ? fover[6 * b + 2] : -1, oy1 = fover ? fover[6 * b + 3] : -1;
+    const int oz0 = fover ? fover[6 * b + 4] : -1, oz1 = fover ? fover[6 * b + 5] : -1;
+    const std::int64_t col = static_cast<std::int64_t>(NG + ty) * E + NG + tx;  // in a plane
+    const double *Uc = Ub + col;  // this thread's column, component 0, plane 0
+    // ---- z state at the segment's first plane kk0: ql of cell kk0 (zl) and the flux of face
+    // kk0 - 1/2 (flo); the z velocity first gives the face coefficients. The per-component
+    // part runs as a loop through LDS (once per segment, kept out of the unrolled code).
+    const int kk0 = NG + k0;
+    double az, bz;
+    {
+      const double *q = Uc + 2 * cs + static_cast<std::int64_t>(kk0 - 3) * E * E;
+      constexpr std::int64_t sz = static_cast<std::int64_t>(E) * E;
+      double qm, qx, ql, qr;
+      FastStates(recon, q[0], q[sz], q[2 * sz], q[3 * sz], q[4 * sz], qm, qx);
+      FastStates(recon, q[sz], q[2 * sz], q[3 * sz], q[4 * sz], q[5 * sz], ql, qr);
+      HllCoef(qm, qr, az, bz);
+    }
+    __syncthreads();  // the previous segment's LDS has been read
+#pragma unroll 1
+    for (int n = 0; n < V; ++n) {
+      const double *q = Uc + n * cs + static_cast<std::int64_t>(kk0 - 3) * E * E;
+      constexpr std::int64_t sz = static_cast<std::int64_t>(E) * E;
+      double qm, qx, ql, qr;
+      FastStates(recon, q[0], q[sz], q[2 * sz], q[3 * sz], q[4 * sz], qm, qx);
+      FastStates(recon, q[sz], q[2 * sz], q[3 * sz], q[4 * sz], q[5 * sz], ql, qr);
+      double f = HllFluxN(n, az, bz, qm, qr);
+      if (k0 == 0 && oz0 >= 0) f = fcor[oz0 + tx + ty * N + n * comp];
+      pl[n * NT + threadIdx.x] = f;
+      sd[n * NT + threadIdx.x] = ql;
+    }
+    double r0[V], r1[V], r2[V], r3[V], r4[V], zl[V], flo[V];
+#pragma unroll
+    for (int n = 0; n < V; ++n) {
+      flo[n] = pl[n * NT + threadIdx.x];
+      zl[n] = sd[n * NT + threadIdx.x];
+      const double *q = Uc + n * cs + static_cast<std::int64_t>(kk0 - 1) * E * E;
+      constexpr std::int64_t sz = static_cast<std::int64_t>(E) * E;
+      r0[n] = q[0];
+      r1[n] = q[sz];
+      r2[n] = q[2 * sz];
+      r3[n] = q[3 * sz];
+      r4[n] = q[4 * sz];
+    }
+    for (int k = k0; k < k1; ++k) {
+      const int kk = NG + k;
+      __syncthreads();  // the previous plane's (or the segment start's) LDS has been read
+      // ---- stage plane kk: interior from the ring, halos from HBM
+#pragma unroll
+      for (int n = 0; n < V; ++n) pl[n * PS + (ty + H) * PP + tx + H] = r1[n];
+      for (int t = threadIdx.x; t < V * 12 * N; t += NT) {
+        const int n = t / (12 * N), r = t % (12 * N);
+        int i, j;
+        if (r < 6 * N) {  // y strips: rows -3 .. -1 and N .. N + 2, all interior columns
+          const int yy = r / N;
+          i = r % N;
+          j = yy < 3 ? yy - 3 : N + yy - 3;
+        } else {          // x strips: all interior rows, columns -3 .. -1 and N .. N + 2
+          const int q = r - 6 * N;
+          j = q / 6;
+          const int c = q % 6;
+          i = c < 3 ? c - 3 : N + c - 3;
+        }
+        pl[n * PS + (j + H) * PP + i + H] =
+            Ub[n * cs + (static_cast<std::int64_t>(kk) * E + NG + j) * E + NG + i];
+      }
+      __syncthreads();
+      // ---- states of the cells just beyond each row (x) and column (y), every component
+      for (int t = threadIdx.x; t < 2 * NX; t += NT) {
+        const int dir = t / NX, r = t % NX;
+        const int n = r / (2 * N), q = r % (2 * N);
+        const int line = q >> 1, side = q & 1;
+        const int cell = side ? N : -1;
+        const int st = dir ? PP : 1;
+        const double *c = pl + n * PS + (dir ? (cell + H) * PP + line + H
+                                             : (line + H) * PP + cell + H);
+        double ql, qr;
+        FastStates(recon, c[-2 * st], c[-st], c[0], c[st], c[2 * st], ql, qr);
+        ext[t] = side ? qr : ql;
+      }
+      __syncthreads();
+      // ---- x: own cell (tx, ty); differences to sd[n][ty][tx]
+      {
+        const double A = C.area[0];
+        double ql0, qr0;
+        FastStates(recon, own_pl[-2], own_pl[-1], own_pl[0], own_pl[1], own_pl[2], ql0, qr0);
+        double qlm = FromLeft16(ql0), qrp = FromRight16(qr0);
+        if (tx == 0) qlm = ext[ty * 2];
+        if (tx == N - 1) qrp = ext[ty * 2 + 1];
+        double alo, blo, ahi, bhi;
+        HllCoef(qlm, qr0, alo, blo);
+        HllCoef(ql0, qrp, ahi, bhi);
+#pragma unroll 1
+        for (int n = 0; n < V; ++n) {
+          double ql = ql0, qr = qr0;
+          if (n > 0) {
+            const double *cn = own_pl + n * PS;
+            FastStates(recon, cn[-2], cn[-1], cn[0], cn[1], cn[2], ql, qr);
+          }
+          double lm = FromLeft16(ql), rp = FromRight16(qr);
+          if (tx == 0) lm = ext[(n * N + ty) * 2];
+          if (tx == N - 1) rp = ext[(n * N + ty) * 2 + 1];
+          double fl = HllFluxN(n, alo, blo, lm, qr);
+          double fh = HllFluxN(n, ahi, bhi, ql, rp);
+          if (tx == 0 && ox0 >= 0) fl = fcor[ox0 + ty + k * N + n * comp];
+          if (tx == N - 1 && ox1 >= 0) fh = fcor[ox1 + ty + k * N + n * comp];
+          sd[n * N * SP + own_sd] = A * fh - A * fl;
+        }
+      }
+      __syncthreads();  // the x differences are in the tile
+      // ---- y: cell i = ty, j = tx (a column per DPP row); adds to sd[n][j][i]
+      {
+        const double A = C.area[1];
+        const double *c = pl + (tx + H) * PP + ty + H;
+        double ql1, qr1;
+        FastStates(recon, c[PS - 2 * PP], c[PS - PP], c[PS], c[PS + PP], c[PS + 2 * PP], ql1,
+                   qr1);
+        double qlm = FromLeft16(ql1), qrp = FromRight16(qr1);
+        if (tx == 0) qlm = ext[NX + (N + ty) * 2];
+        if (tx == N - 1) qrp = ext[NX + (N + ty) * 2 + 1];
+        double alo, blo, ahi, bhi;
+        HllCoef(qlm, qr1, alo, blo);
+        HllCoef(ql1, qrp, ahi, bhi);
+#pragma unroll 1
+        for (int n = 0; n < V; ++n) {
+          double ql = ql1, qr = qr1;
+          if (n != 1) {
+            const double *cn = c + n * PS;
+            FastStates(recon, cn[-2 * PP], cn[-PP], cn[0], cn[PP], cn[2 * PP], ql, qr);
+          }
+          double lm = FromLeft16(ql), rp = FromRight16(qr);
+          if (tx == 0) lm = ext[NX + (n * N + ty) * 2];
+          if (tx == N - 1) rp = ext[NX + (n * N + ty) * 2 + 1];
+          double fl = HllFluxN(n, alo, blo, lm, qr);
+          double fh = HllFluxN(n, ahi, bhi, ql, rp);
+          if (tx == 0 && oy0 >= 0) fl = fcor[oy0 + ty + k * N + n * comp];
+          if (tx == N - 1 && oy1 >= 0) fh = fcor[oy1 + ty + k * N + n * comp];
+          double &slot = sd[n * N * SP + tx * SP + ty];
+          slot = slot + (A * fh - A * fl);
+        }
+      }
+      __syncthreads();  // x + y differences complete
+      // ---- z: the cell above (kk + 1) from the ring; its low face closes this cell
+      double bv[V];
+      const std::int64_t c0 = static_cast<std::int64_t>(b) * V * cs +
+                              static_cast<std::int64_t>(kk) * E * E + col;
+#pragma unroll
+      for (int n = 0; n < V; ++n) bv[n] = same_base ? 0.0 : base[c0 + n * cs];
+      const double A = C.area[2];
+      {
+        double ql2, qr2;
+        FastStates(recon, r0[2], own_pl[2 * PS], r2[2], r3[2], r4[2], ql2, qr2);
+        HllCoef(zl[2], qr2, az, bz);
+      }
+#pragma unroll
+      for (int n = 0; n < V; ++n) {
+        const double uo = own_pl[n * PS];
+        double ql, qr;
+        FastStates(recon, r0[n], uo, r2[n], r3[n], r4[n], ql, qr);
+        double fh = HllFluxN(n, az, bz, zl[n], qr);
+        if (k == N - 1 && oz1 >= 0) fh = fcor[oz1 + tx + ty * N + n * comp];
+        double &slot = sd[n * N * SP + own_sd];
+        const double du = slot + (A * fh - A * flo[n]);
+        flo[n] = fh;
+        zl[n] = ql;
+        const double dudt = -du * ivol;
+        const double avg = beta * uo + w2 * (same_base ? uo : bv[n]);
+        const double u1 = 1.0 * avg + bdt * dudt;
+        out[c0 + n * cs] = u1;
+        slot = u1;  // the new state waits in the thread's own slot for the push below
+        r0[n] = uo;  // the ring moves up one plane
+        r1[n] = r2[n];
+        r2[n] = r3[n];
+        r3[n] = r4[n];
+      }
+      double u1[V];
+#pragma unroll
+      for (int n = 0; n < V; ++n) u1[n] = sd[n * N * SP + own_sd];
+      if (k + 1 < k1) {
+#pragma unroll
+        for (int n = 0; n < V; ++n) r4[n] = Uc[n * cs + static_cast<std::int64_t>(kk + 4) * E * E];
+      }
+      if (push) {
+        // same-level ghost copies, as in BlockSweepKernel
+        const int lx = tx < NG ? -1 : (tx >= N - NG ? 1 : 0);
+        const int ly = ty < NG ? -1 : (ty >= N - NG ? 1 : 0);
+        const int lz = k < NG ? -1 : (k >= N - NG ? 1 : 0);
+        for (int az_ = 0; az_ < (lz ? 2 : 1); ++az_)
+          for (int ay = 0; ay < (ly ? 2 : 1); ++ay)
+            for (int ax = 0; ax < (lx ? 2 : 1); ++ax) {
+              const int dx = ax ? lx : 0, dy = ay ? ly : 0, dz = az_ ? lz : 0;
+              if (!(dx | dy | dz)) continue;
+              const int g = push[27 * b + (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)];
+              if (g < 0) continue;
+              const std::int64_t d0 =
+                  static_cast<std::int64_t>(g) * V * cs +
+                  (static_cast<std::int64_t>(kk - dz * N) * E + NG + ty - dy * N) * E + NG + tx -
+                  dx * N;
+#pragma unroll
+              for (int n = 0; n < V; ++n) out[d0 + n * cs] = u1[n];
+            }
+      }
+      if (derived)
+        derived[static_cast<std::int64_t>(b) * cs + static_cast<std::int64_t>(kk) * E * E + col] =
+            0.5 * u1[3] * (u1[0] * u1[0] + u1[1] * u1[1] + u1[2] * u1[2]);
+      double sp = fabs(u1[0]) / C.dx[0];
+      sp = sp + fabs(u1[1]) / C.dx[1];
+      sp = sp + fabs(u1[2]) / C.dx[2];
+      m = DMin(m, 1.0 / sp);
+    }
+  }
+  if (!partial) return;
+  m = WaveMin(m);
+  if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v = wmin[0];
+#pragma unroll
+    for (int w = 1; w < NT / 64; ++w) v = DMin(v, wmin[w]);
+    partial[blockIdx.x] = v;
+  }
+}
 }  // namespace
 
 bool BurgersSweepSupported(const Topology &t, int V) { return ReconFluxSupported(t, V); }
@@ -408,13 +815,59 @@ pth_status BurgersSweep(const Topology &t, int V, const double *U, const double 
                         const BlockCoefs *coef, double beta, const double *dtvar, int recon,
                         const FluxCorrection *fc, int waves, int spread, const int *push,
                         int *groups, hipStream_t s) {
-  if (!BurgersSweepSupported(t, V) || (waves != 3 && waves != 4) || spread < 1 || spread > 3) {
+  if (!BurgersSweepSupported(t, V) || waves < 2 || waves > 4 || spread < 1 || spread > 3) {
     SetError("block sweep: unsupported block shape or component count");
     return PTH_FAIL;
   }
   ProfScope prof("block_sweep", s);
   const int nb = static_cast<int>(t.blocks.size());
   const int N = t.fine.n[0];
+  if (N == 16 && t.fine.g[0] == 4) {
+    // register-ring kernel, compiled for 2 or 3 waves per SIMD
+    const int minw = waves >= 3 ? 3 : 2;
+    const int *fover16 = fc ? fc->dev_face : nullptr;
+    const double *fcor16 = fc ? fc->store : nullptr;
+    int grid16 = 0;
+#define PTH_S16_W(NV, W)                                                                       \
+  {                                                                                            \
+    const int r = ResidentGroups(reinterpret_cast<const void *>(&BlockSweep16Kernel<NV, W>), 256); \
+    if (r < 1) {                                                                               \
+      SetError("block sweep: occupancy query failed");                                         \
+      return PTH_FAIL;                                                                         \
+    }                                                                                          \
+    grid16 = static_cast<int>(std::min<std::int64_t>(std::min(std::max(r / spread, 1),          \
+                                                              max_partials),                   \
+                                                     static_cast<std::int64_t>(nb) * 16));     \
+    hipLaunchKernelGGL((BlockSweep16Kernel<NV, W>), dim3(grid16), dim3(256), 0, s, U, base,     \
+                       out, derived, partial, coef, nb, beta, dtvar, recon, fover16, fcor16,   \
+                       push);                                                                  \
+  }
+#define PTH_S16_CASE(NV)                                                                       \
+  case NV:                                                                                     \
+    if (minw == 3)                                                                             \
+      PTH_S16_W(NV, 3)                                                                         \
+    else                                                                                       \
+      PTH_S16_W(NV, 2)                                                                         \
+    break;
+    switch (V) {
+      PTH_S16_CASE(4)
+      PTH_S16_CASE(5)
+      PTH_S16_CASE(6)
+      PTH_S16_CASE(7)
+      PTH_S16_CASE(8)
+      PTH_S16_CASE(9)
+      PTH_S16_CASE(10)
+      PTH_S16_CASE(11)
+      PTH_S16_CASE(12)
+      PTH_S16_CASE(13)
+    }
+#undef PTH_S16_CASE
+#undef PTH_S16_W
+    PTH_HIP(hipGetLastError());
+    if (groups) *groups = grid16;
+    return PTH_COMPLETE;
+  }
+  if (waves == 2) waves = 3;
   const int *fover = fc ? fc->dev_face : nullptr;
   const double *fcor = fc ? fc->store : nullptr;
   int grid = 0;
@@ -475,10 +928,15 @@ pth_status FlxcorDirect(const Topology &t, int V, const FluxCorrection &fc, cons
   ProfScope prof("flxcor_direct", s);
   const Geo G = MakeGeo(t, V);
   const dim3 grid(static_cast<unsigned>(fc.host.size()), (4 * fc.max_cells + 63) / 64);
+  const bool ring = t.fine.n[0] == 16 && t.fine.g[0] == 4;
 #define PTH_FD_CASE(NV)                                                                       \
   case NV:                                                                                    \
-    hipLaunchKernelGGL(FlxcorDirectKernel<NV>, grid, dim3(64), 0, s, fc.dev, U, fc.store, G,  \
-                       recon);                                                                \
+    if (ring)                                                                                 \
+      hipLaunchKernelGGL((FlxcorDirectKernel<NV, true>), grid, dim3(64), 0, s, fc.dev, U,     \
+                         fc.store, G, recon);                                                 \
+    else                                                                                      \
+      hipLaunchKernelGGL((FlxcorDirectKernel<NV, false>), grid, dim3(64), 0, s, fc.dev, U,    \
+                         fc.store, G, recon);                                                 \
     break;
   switch (V) {
     PTH_FD_CASE(4)

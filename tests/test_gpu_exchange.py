@@ -137,3 +137,38 @@ def test_multilevel_random_data_matches_oracle(spec, fused):
     assert np.array_equal(got[:, :, mask], f[:, :, mask])
     bit_exact = np.array_equal(got, f)
     print("prolongation bit-exact:", bit_exact, flush=True)
+
+
+@pytest.mark.parametrize("fused", [False, True], ids=["tasks", "fused"])
+def test_registered_linear_prolongation_per_field(fused):
+    # Metadata::RegisterRefinementOps per field: "lin" registers ProlongateSharedLinear, "mm"
+    # keeps ProlongateSharedMinMod; one exchange serves both, each field against the oracle's
+    # exchange with its own operator (the oracle's operators are pinned by the parabola
+    # closed forms in test_oracle_exchange.py)
+    print(f"case per-field prolongation fused={fused}", flush=True)
+    spec = dict(nx=(64, 64, 64), block=(16, 16, 16), g=4)
+    spheres = [((0.0, 0.0, 0.0), 0.25, 2)]
+    m = pa.Mesh(spec["nx"], spec["block"], spec["g"], refine_spheres=spheres)
+    for name in ("mm", "lin"):
+        m.add_field(name, 2, pa.Metadata.FillGhost | pa.Metadata.Independent)
+    m.register_refinement_ops("lin", "linear")
+    m.finalize()
+    md = m.mesh_data("base")
+    results = {}
+    for name, prolong, seed in (("mm", 1, 3), ("lin", 2, 4)):
+        f = interior_uniform(m.field_shape(name), m.ng, seed=seed, fill=0.0)
+        md.upload(name, f)
+        o = OracleMesh(spec["nx"], spec["block"], spec["g"], spheres=spheres)
+        c = np.zeros(m.coarse_shape(name))
+        o.exchange([f], coarse=[c], prolong=prolong)
+        results[name] = f
+    (pa.exchange_fused if fused else pa.exchange)(md)
+    for name in ("mm", "lin"):
+        got = md.download(name)
+        np.testing.assert_allclose(got, results[name], rtol=1e-13, atol=0)
+        print(f"{name}: prolongation bit-exact {np.array_equal(got, results[name])}", flush=True)
+    # the two operators really differ on this data
+    f = interior_uniform(m.field_shape("mm"), m.ng, seed=4, fill=0.0)
+    o = OracleMesh(spec["nx"], spec["block"], spec["g"], spheres=spheres)
+    o.exchange([f], coarse=[np.zeros(m.coarse_shape("mm"))], prolong=1)
+    assert not np.allclose(f, results["lin"], rtol=1e-13, atol=0)

@@ -3,7 +3,7 @@ prolongation) against closed forms, before it is trusted as the checker of the H
 import numpy as np
 import pytest
 
-from meshcheck import global_index_field, interior_mask, linear_field
+from meshcheck import cell_centers, global_index_field, interior_mask, linear_field
 from oracle_lib import OracleMesh
 
 
@@ -136,3 +136,53 @@ def test_restriction_equals_brute_force_average(dyadic):
         rev = np.ascontiguousarray(data[:, :, ::-1, ::-1, ::-1])
         got2 = restrict_interior(m, 3, rev)
         assert np.array_equal(got2[sl], got[sl][:, :, ::-1, ::-1, ::-1])
+
+
+@pytest.mark.parametrize("prolong", [1, 2], ids=["minmod", "linear"])
+def test_prolongation_of_a_parabola_closed_form(prolong):
+    # Interior cells hold the cell averages x^2 + h^2/12 of the parabola x^2 (h the cell
+    # width); fine blocks sit in the middle of x with coarser blocks on both x sides. Each
+    # fine ghost cell across such an x face comes from prolongating its parent coarse cell
+    # (centre c, width h, average C = c^2 + h^2/12): its neighbours' averages differ by
+    # (2c - h) h below and (2c + h) h above, so the one-sided slopes are 2c - h and 2c + h.
+    # ProlongateSharedLinear puts the child at c -/+ h/4 on C -/+ slope * h/4 with the slope of
+    # its own side; ProlongateSharedMinMod uses the minmod of the two for both children. The
+    # closed forms differ by h^2/2 on the upper child, so a swapped side or limiter fails.
+    m = OracleMesh((32, 8, 8), (8, 8, 8), 4, boxes=[((-0.25, 0.25, -0.5, 0.5, -0.5, 0.5), 1)])
+    assert m.multilevel
+    f = np.full(m.field_shape(1), np.nan)
+    g = m.ng[0]
+    for gid in range(m.nblocks):
+        x, _, _ = cell_centers(m, gid)
+        h = (m.block(gid)["xmax"][0] - m.block(gid)["xmin"][0]) / m.n[0]
+        f[gid, 0, g:-g, g:-g, g:-g] = (x[g:-g] ** 2 + h * h / 12.0)[None, None, :]
+    cf = np.full(m.coarse_shape(1), np.nan)
+    m.exchange([f], coarse=[cf], prolong=prolong)
+    checked = 0
+    for gid in range(m.nblocks):
+        b = m.block(gid)
+        if b["level"] != 1:
+            continue
+        hf = (b["xmax"][0] - b["xmin"][0]) / m.n[0]
+        h = 2 * hf
+        x, _, _ = cell_centers(m, gid)
+        for side, cells in (("lo", range(0, g)), ("hi", range(m.n[0] + g, m.ent[0]))):
+            edge = b["xmin"][0] if side == "lo" else b["xmax"][0]
+            if abs(edge) != 0.25:  # only the faces toward the coarse blocks
+                continue
+            for i in cells:
+                xf = x[i]
+                c = (np.floor(xf / h) + 0.5) * h
+                C = c * c + h * h / 12.0
+                lower = xf < c
+                slope_m, slope_p = 2 * c - h, 2 * c + h
+                if prolong == 2:
+                    exp = C - slope_m * h / 4 if lower else C + slope_p * h / 4
+                else:
+                    gc = (np.sign(slope_m) + np.sign(slope_p)) / 2 * min(abs(slope_m),
+                                                                         abs(slope_p))
+                    exp = C - gc * h / 4 if lower else C + gc * h / 4
+                got = f[gid, 0, g:-g, g:-g, i]
+                assert np.max(np.abs(got - exp)) < 1e-15, (gid, side, i, got.ravel()[0], exp)
+                checked += 1
+    assert checked >= 8
