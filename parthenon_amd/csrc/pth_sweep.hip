@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <map>
+#include <type_traits>
 
 #include "pth_burgers.hpp"
 #include "pth_device.hpp"
@@ -202,6 +203,8 @@ __device__ __forceinline__ double HllFluxN(int n, double a, double b, double l, 
   if (n < 3) return fma(0.5 * a, l, (0.5 * b) * r);
   return fma(a, l, b * r);
 }
+
+__device__ __forceinline__ int DMinI(int a, int b) { return b < a ? b : a; }
 
 // value of lane - 1 / lane + 1 within each 16-lane DPP row (the row's first / last lane
 // gets 0; callers replace those)
@@ -530,12 +533,12 @@ __global__ void __launch_bounds__(N * N, MINW)
 // averaging and update, derived field, CFL minimum, flux-correction faces and same-level push
 // are those of BlockSweepKernel. Each U element is read from HBM once per workgroup (plus
 // the halos and the segment's first planes), and face states and fluxes never leave the CU.
-template <int V, int MINW>
+template <int V, int MINW, int RECON>
 __global__ void __launch_bounds__(256, MINW)
     BlockSweep16Kernel(const double *__restrict__ U, const double *base, double *out,
                        double *__restrict__ derived, double *__restrict__ partial,
                        const BlockCoefs *__restrict__ coef, int nblocks, double beta,
-                       const double *__restrict__ dtvar, int recon, const int *__restrict__ fover,
+                       const double *__restrict__ dtvar, const int *__restrict__ fover,
                        const double *__restrict__ fcor, const int *__restrict__ push) {
   constexpr int N = 16, NG = 4, E = N + 2 * NG, NT = N * N;
   constexpr int H = 3;             // halo of the staged plane
@@ -544,12 +547,18 @@ __global__ void __launch_bounds__(256, MINW)
   constexpr int PS = PW * PP;      // one component of the staged plane
   constexpr int SP = N + 1;        // pitch of the difference tile
   constexpr int NX = V * N * 2;    // beyond-the-row states of one direction
+  constexpr int NH = V * 12 * N;   // halo cells of one staged plane (all components)
+  constexpr int QH = (NH + NT - 1) / NT;  // halo cells per thread
   constexpr std::int64_t cs = static_cast<std::int64_t>(E) * E * E;
+  constexpr std::int64_t sz = static_cast<std::int64_t>(E) * E;
   constexpr int comp = N * N;      // correction-store stride between components
-  static_assert(V * NT <= PS * V && V * NT <= V * N * SP, "segment start scratch fits");
   __shared__ double pl[V * PS];      // [n][j + H][i + H]
   __shared__ double ext[2 * NX];     // [dir][n][line][side]: side 0 ql of cell -1, 1 qr of cell N
-  __shared__ double sd[V * N * SP];  // x + y differences [n][j][i]
+  __shared__ double sd[V * N * SP];  // x + y differences [n][j][i], then the new state
+  __shared__ double zs[2 * V * NT];  // this thread's z state: [0][n] ql of its cell, [1][n] flux
+                                     // of its low z face (own slots, no barriers)
+  __shared__ double fxy[2 * V * 2 * N];  // corrected x / y face fluxes of the plane,
+                                          // [dir][n][side][line] (blocks with such faces)
   __shared__ double wmin[NT / 64];
   const int tx = threadIdx.x % N, ty = threadIdx.x / N;
   const double bdt = beta * dtvar[0];
@@ -561,6 +570,69 @@ __global__ void __launch_bounds__(256, MINW)
   const std::int64_t pend = T * (blockIdx.x + 1) / gridDim.x;
   const double *own_pl = pl + (ty + H) * PP + tx + H;  // own cell, component 0
   const int own_sd = ty * SP + tx;                      // own cell in the difference tile
+  // halo cell q of this thread (q < QH): its component, offset within a plane of the brick
+  // and place in the staged plane; cells past the last share its address (loads are
+  // unconditional, so the compiler can count them in its memory waits)
+  auto halo = [&](int q, std::int64_t &goff, int &loff) {
+    const int t = DMinI(static_cast<int>(threadIdx.x) + q * NT, NH - 1);
+    const int n = t / (12 * N), r = t % (12 * N);
+    int i, j;
+    if (r < 6 * N) {  // y strips: rows -3 .. -1 and N .. N + 2, all interior columns
+      const int yy = r / N;
+      i = r % N;
+      j = yy < 3 ? yy - 3 : N + yy - 3;
+    } else {          // x strips: all interior rows, columns -3 .. -1 and N .. N + 2
+      const int qq = r - 6 * N;
+      j = qq / 6;
+      const int c = qq % 6;
+      i = c < 3 ? c - 3 : N + c - 3;
+    }
+    goff = n * cs + static_cast<std::int64_t>(NG + j) * E + NG + i;
+    loff = n * PS + (j + H) * PP + i + H;
+  };
+  // the new state of plane kk (in the thread's own slots of sd) to out, the same-level
+  // neighbours' ghosts, the derived field and the CFL minimum
+  auto flush = [&](int b, int k) {
+    const int kk = NG + k;
+    const std::int64_t cell = static_cast<std::int64_t>(kk) * sz + (NG + ty) * E + NG + tx;
+    const std::int64_t c0 = static_cast<std::int64_t>(b) * V * cs + cell;
+    double u1[V];
+#pragma unroll
+    for (int n = 0; n < V; ++n) {
+      u1[n] = sd[n * N * SP + own_sd];
+      out[c0 + n * cs] = u1[n];
+    }
+    if (push) {
+      // same-level ghost copies: a cell within NG of a face is a ghost cell of the
+      // same-level neighbours across that face, edge or corner (push[27 b + direction]), at
+      // its index shifted by N along each crossed dimension
+      const int lx = tx < NG ? -1 : (tx >= N - NG ? 1 : 0);
+      const int ly = ty < NG ? -1 : (ty >= N - NG ? 1 : 0);
+      const int lz = k < NG ? -1 : (k >= N - NG ? 1 : 0);
+      for (int az_ = 0; az_ < (lz ? 2 : 1); ++az_)
+        for (int ay = 0; ay < (ly ? 2 : 1); ++ay)
+          for (int ax = 0; ax < (lx ? 2 : 1); ++ax) {
+            const int dx = ax ? lx : 0, dy = ay ? ly : 0, dz = az_ ? lz : 0;
+            if (!(dx | dy | dz)) continue;
+            const int g = push[27 * b + (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)];
+            if (g < 0) continue;
+            const std::int64_t d0 =
+                static_cast<std::int64_t>(g) * V * cs +
+                (static_cast<std::int64_t>(kk - dz * N) * E + NG + ty - dy * N) * E + NG + tx -
+                dx * N;
+#pragma unroll
+            for (int n = 0; n < V; ++n) out[d0 + n * cs] = u1[n];
+          }
+    }
+    if (derived)
+      derived[static_cast<std::int64_t>(b) * cs + cell] =
+          0.5 * u1[3] * (u1[0] * u1[0] + u1[1] * u1[1] + u1[2] * u1[2]);
+    const BlockCoefs &Cb = coef[b];
+    double sp = fabs(u1[0]) / Cb.dx[0];
+    sp = sp + fabs(u1[1]) / Cb.dx[1];
+    sp = sp + fabs(u1[2]) / Cb.dx[2];
+    m = DMin(m, 1.0 / sp);
+  };
   while (pnext < pend) {
     const int b = static_cast<int>(pnext / N);
     const int k0 = static_cast<int>(pnext % N);
@@ -574,39 +646,42 @@ __global__ void __launch_bounds__(256, MINW)
     const int oz0 = fover ? fover[6 * b + 4] : -1, oz1 = fover ? fover[6 * b + 5] : -1;
     const std::int64_t col = static_cast<std::int64_t>(NG + ty) * E + NG + tx;  // in a plane
     const double *Uc = Ub + col;  // this thread's column, component 0, plane 0
+    const int kk0 = NG + k0;
+    // halo of the first plane
+    double hv[QH];
+#pragma unroll
+    for (int q = 0; q < QH; ++q) {
+      std::int64_t go;
+      int lo;
+      halo(q, go, lo);
+      hv[q] = Ub[go + kk0 * sz];
+    }
     // ---- z state at the segment's first plane kk0: ql of cell kk0 (zl) and the flux of face
     // kk0 - 1/2 (flo); the z velocity first gives the face coefficients. The per-component
     // part runs as a loop through LDS (once per segment, kept out of the unrolled code).
-    const int kk0 = NG + k0;
     double az, bz;
     {
-      const double *q = Uc + 2 * cs + static_cast<std::int64_t>(kk0 - 3) * E * E;
-      constexpr std::int64_t sz = static_cast<std::int64_t>(E) * E;
+      const double *q = Uc + 2 * cs + static_cast<std::int64_t>(kk0 - 3) * sz;
       double qm, qx, ql, qr;
-      FastStates(recon, q[0], q[sz], q[2 * sz], q[3 * sz], q[4 * sz], qm, qx);
-      FastStates(recon, q[sz], q[2 * sz], q[3 * sz], q[4 * sz], q[5 * sz], ql, qr);
+      FastStates(RECON, q[0], q[sz], q[2 * sz], q[3 * sz], q[4 * sz], qm, qx);
+      FastStates(RECON, q[sz], q[2 * sz], q[3 * sz], q[4 * sz], q[5 * sz], ql, qr);
       HllCoef(qm, qr, az, bz);
     }
-    __syncthreads();  // the previous segment's LDS has been read
 #pragma unroll 1
     for (int n = 0; n < V; ++n) {
-      const double *q = Uc + n * cs + static_cast<std::int64_t>(kk0 - 3) * E * E;
-      constexpr std::int64_t sz = static_cast<std::int64_t>(E) * E;
+      const double *q = Uc + n * cs + static_cast<std::int64_t>(kk0 - 3) * sz;
       double qm, qx, ql, qr;
-      FastStates(recon, q[0], q[sz], q[2 * sz], q[3 * sz], q[4 * sz], qm, qx);
-      FastStates(recon, q[sz], q[2 * sz], q[3 * sz], q[4 * sz], q[5 * sz], ql, qr);
+      FastStates(RECON, q[0], q[sz], q[2 * sz], q[3 * sz], q[4 * sz], qm, qx);
+      FastStates(RECON, q[sz], q[2 * sz], q[3 * sz], q[4 * sz], q[5 * sz], ql, qr);
       double f = HllFluxN(n, az, bz, qm, qr);
       if (k0 == 0 && oz0 >= 0) f = fcor[oz0 + tx + ty * N + n * comp];
-      pl[n * NT + threadIdx.x] = f;
-      sd[n * NT + threadIdx.x] = ql;
+      zs[(V + n) * NT + threadIdx.x] = f;
+      zs[n * NT + threadIdx.x] = ql;
     }
-    double r0[V], r1[V], r2[V], r3[V], r4[V], zl[V], flo[V];
+    double r0[V], r1[V], r2[V], r3[V], r4[V];
 #pragma unroll
     for (int n = 0; n < V; ++n) {
-      flo[n] = pl[n * NT + threadIdx.x];
-      zl[n] = sd[n * NT + threadIdx.x];
-      const double *q = Uc + n * cs + static_cast<std::int64_t>(kk0 - 1) * E * E;
-      constexpr std::int64_t sz = static_cast<std::int64_t>(E) * E;
+      const double *q = Uc + n * cs + static_cast<std::int64_t>(kk0 - 1) * sz;
       r0[n] = q[0];
       r1[n] = q[sz];
       r2[n] = q[2 * sz];
@@ -616,24 +691,33 @@ __global__ void __launch_bounds__(256, MINW)
     for (int k = k0; k < k1; ++k) {
       const int kk = NG + k;
       __syncthreads();  // the previous plane's (or the segment start's) LDS has been read
-      // ---- stage plane kk: interior from the ring, halos from HBM
+      // ---- stage plane kk: interior from the ring, halos from the prefetch
 #pragma unroll
       for (int n = 0; n < V; ++n) pl[n * PS + (ty + H) * PP + tx + H] = r1[n];
-      for (int t = threadIdx.x; t < V * 12 * N; t += NT) {
-        const int n = t / (12 * N), r = t % (12 * N);
-        int i, j;
-        if (r < 6 * N) {  // y strips: rows -3 .. -1 and N .. N + 2, all interior columns
-          const int yy = r / N;
-          i = r % N;
-          j = yy < 3 ? yy - 3 : N + yy - 3;
-        } else {          // x strips: all interior rows, columns -3 .. -1 and N .. N + 2
-          const int q = r - 6 * N;
-          j = q / 6;
-          const int c = q % 6;
-          i = c < 3 ? c - 3 : N + c - 3;
+#pragma unroll
+      for (int q = 0; q < QH; ++q) {
+        std::int64_t go;
+        int lo;
+        halo(q, go, lo);
+        pl[lo] = hv[q];
+      }
+      // corrected fluxes of this plane's x and y faces (only blocks next to finer ones):
+      // staged here, so the loops below read LDS and issue no conditional load
+      if (ox0 >= 0 || ox1 >= 0 || oy0 >= 0 || oy1 >= 0) {
+        for (int t = threadIdx.x; t < 2 * V * 2 * N; t += NT) {
+          const int dir = t / (2 * V * N), r = t % (2 * V * N);
+          const int n = r / (2 * N), side = (r / N) & 1, line = r % N;
+          const int o = dir == 0 ? (side ? ox1 : ox0) : (side ? oy1 : oy0);
+          fxy[t] = o >= 0 ? fcor[o + line + k * N + n * comp] : 0.0;
         }
-        pl[n * PS + (j + H) * PP + i + H] =
-            Ub[n * cs + (static_cast<std::int64_t>(kk) * E + NG + j) * E + NG + i];
+      }
+      // the previous plane's results leave now, well before the next wait on a load; then the
+      // ring's top plane (kk + 3) for this plane's z stencils, issued after those stores so
+      // that the wait for it counts only loads
+      if (k > k0) {
+        flush(b, k - 1);
+#pragma unroll
+        for (int n = 0; n < V; ++n) r4[n] = Uc[n * cs + static_cast<std::int64_t>(kk + 3) * sz];
       }
       __syncthreads();
       // ---- states of the cells just beyond each row (x) and column (y), every component
@@ -646,7 +730,7 @@ __global__ void __launch_bounds__(256, MINW)
         const double *c = pl + n * PS + (dir ? (cell + H) * PP + line + H
                                              : (line + H) * PP + cell + H);
         double ql, qr;
-        FastStates(recon, c[-2 * st], c[-st], c[0], c[st], c[2 * st], ql, qr);
+        FastStates(RECON, c[-2 * st], c[-st], c[0], c[st], c[2 * st], ql, qr);
         ext[t] = side ? qr : ql;
       }
       __syncthreads();
@@ -654,27 +738,27 @@ __global__ void __launch_bounds__(256, MINW)
       {
         const double A = C.area[0];
         double ql0, qr0;
-        FastStates(recon, own_pl[-2], own_pl[-1], own_pl[0], own_pl[1], own_pl[2], ql0, qr0);
+        FastStates(RECON, own_pl[-2], own_pl[-1], own_pl[0], own_pl[1], own_pl[2], ql0, qr0);
         double qlm = FromLeft16(ql0), qrp = FromRight16(qr0);
         if (tx == 0) qlm = ext[ty * 2];
         if (tx == N - 1) qrp = ext[ty * 2 + 1];
         double alo, blo, ahi, bhi;
         HllCoef(qlm, qr0, alo, blo);
         HllCoef(ql0, qrp, ahi, bhi);
-#pragma unroll 1
+#pragma unroll 2
         for (int n = 0; n < V; ++n) {
           double ql = ql0, qr = qr0;
           if (n > 0) {
             const double *cn = own_pl + n * PS;
-            FastStates(recon, cn[-2], cn[-1], cn[0], cn[1], cn[2], ql, qr);
+            FastStates(RECON, cn[-2], cn[-1], cn[0], cn[1], cn[2], ql, qr);
           }
           double lm = FromLeft16(ql), rp = FromRight16(qr);
           if (tx == 0) lm = ext[(n * N + ty) * 2];
           if (tx == N - 1) rp = ext[(n * N + ty) * 2 + 1];
           double fl = HllFluxN(n, alo, blo, lm, qr);
           double fh = HllFluxN(n, ahi, bhi, ql, rp);
-          if (tx == 0 && ox0 >= 0) fl = fcor[ox0 + ty + k * N + n * comp];
-          if (tx == N - 1 && ox1 >= 0) fh = fcor[ox1 + ty + k * N + n * comp];
+          if (tx == 0 && ox0 >= 0) fl = fxy[(n * 2 + 0) * N + ty];
+          if (tx == N - 1 && ox1 >= 0) fh = fxy[(n * 2 + 1) * N + ty];
           sd[n * N * SP + own_sd] = A * fh - A * fl;
         }
       }
@@ -684,7 +768,7 @@ __global__ void __launch_bounds__(256, MINW)
         const double A = C.area[1];
         const double *c = pl + (tx + H) * PP + ty + H;
         double ql1, qr1;
-        FastStates(recon, c[PS - 2 * PP], c[PS - PP], c[PS], c[PS + PP], c[PS + 2 * PP], ql1,
+        FastStates(RECON, c[PS - 2 * PP], c[PS - PP], c[PS], c[PS + PP], c[PS + 2 * PP], ql1,
                    qr1);
         double qlm = FromLeft16(ql1), qrp = FromRight16(qr1);
         if (tx == 0) qlm = ext[NX + (N + ty) * 2];
@@ -692,93 +776,82 @@ __global__ void __launch_bounds__(256, MINW)
         double alo, blo, ahi, bhi;
         HllCoef(qlm, qr1, alo, blo);
         HllCoef(ql1, qrp, ahi, bhi);
-#pragma unroll 1
+#pragma unroll 2
         for (int n = 0; n < V; ++n) {
           double ql = ql1, qr = qr1;
           if (n != 1) {
             const double *cn = c + n * PS;
-            FastStates(recon, cn[-2 * PP], cn[-PP], cn[0], cn[PP], cn[2 * PP], ql, qr);
+            FastStates(RECON, cn[-2 * PP], cn[-PP], cn[0], cn[PP], cn[2 * PP], ql, qr);
           }
           double lm = FromLeft16(ql), rp = FromRight16(qr);
           if (tx == 0) lm = ext[NX + (n * N + ty) * 2];
           if (tx == N - 1) rp = ext[NX + (n * N + ty) * 2 + 1];
           double fl = HllFluxN(n, alo, blo, lm, qr);
           double fh = HllFluxN(n, ahi, bhi, ql, rp);
-          if (tx == 0 && oy0 >= 0) fl = fcor[oy0 + ty + k * N + n * comp];
-          if (tx == N - 1 && oy1 >= 0) fh = fcor[oy1 + ty + k * N + n * comp];
+          if (tx == 0 && oy0 >= 0) fl = fxy[((V + n) * 2 + 0) * N + ty];
+          if (tx == N - 1 && oy1 >= 0) fh = fxy[((V + n) * 2 + 1) * N + ty];
           double &slot = sd[n * N * SP + tx * SP + ty];
           slot = slot + (A * fh - A * fl);
         }
       }
       __syncthreads();  // x + y differences complete
-      // ---- z: the cell above (kk + 1) from the ring; its low face closes this cell
-      double bv[V];
+      // the base state of this plane (second stage), then the next plane's halo; both are
+      // issued before this plane's results are stored, so no wait drains a store
       const std::int64_t c0 = static_cast<std::int64_t>(b) * V * cs +
-                              static_cast<std::int64_t>(kk) * E * E + col;
+                              static_cast<std::int64_t>(kk) * sz + col;
+      double bv[V];
 #pragma unroll
       for (int n = 0; n < V; ++n) bv[n] = same_base ? 0.0 : base[c0 + n * cs];
+      if (k + 1 < k1) {
+#pragma unroll
+        for (int q = 0; q < QH; ++q) {
+          std::int64_t go;
+          int lo;
+          halo(q, go, lo);
+          hv[q] = Ub[go + (kk + 1) * sz];
+        }
+      }
+      // ---- z: the cell above (kk + 1) from the ring; its low face closes this cell
       const double A = C.area[2];
       {
         double ql2, qr2;
-        FastStates(recon, r0[2], own_pl[2 * PS], r2[2], r3[2], r4[2], ql2, qr2);
-        HllCoef(zl[2], qr2, az, bz);
+        FastStates(RECON, r0[2], own_pl[2 * PS], r2[2], r3[2], r4[2], ql2, qr2);
+        HllCoef(zs[2 * NT + threadIdx.x], qr2, az, bz);
       }
-#pragma unroll
-      for (int n = 0; n < V; ++n) {
-        const double uo = own_pl[n * PS];
-        double ql, qr;
-        FastStates(recon, r0[n], uo, r2[n], r3[n], r4[n], ql, qr);
-        double fh = HllFluxN(n, az, bz, zl[n], qr);
-        if (k == N - 1 && oz1 >= 0) fh = fcor[oz1 + tx + ty * N + n * comp];
-        double &slot = sd[n * N * SP + own_sd];
-        const double du = slot + (A * fh - A * flo[n]);
-        flo[n] = fh;
-        zl[n] = ql;
-        const double dudt = -du * ivol;
-        const double avg = beta * uo + w2 * (same_base ? uo : bv[n]);
-        const double u1 = 1.0 * avg + bdt * dudt;
-        out[c0 + n * cs] = u1;
-        slot = u1;  // the new state waits in the thread's own slot for the push below
-        r0[n] = uo;  // the ring moves up one plane
-        r1[n] = r2[n];
-        r2[n] = r3[n];
-        r3[n] = r4[n];
-      }
-      double u1[V];
-#pragma unroll
-      for (int n = 0; n < V; ++n) u1[n] = sd[n * N * SP + own_sd];
-      if (k + 1 < k1) {
-#pragma unroll
-        for (int n = 0; n < V; ++n) r4[n] = Uc[n * cs + static_cast<std::int64_t>(kk + 4) * E * E];
-      }
-      if (push) {
-        // same-level ghost copies, as in BlockSweepKernel
-        const int lx = tx < NG ? -1 : (tx >= N - NG ? 1 : 0);
-        const int ly = ty < NG ? -1 : (ty >= N - NG ? 1 : 0);
-        const int lz = k < NG ? -1 : (k >= N - NG ? 1 : 0);
-        for (int az_ = 0; az_ < (lz ? 2 : 1); ++az_)
-          for (int ay = 0; ay < (ly ? 2 : 1); ++ay)
-            for (int ax = 0; ax < (lx ? 2 : 1); ++ax) {
-              const int dx = ax ? lx : 0, dy = ay ? ly : 0, dz = az_ ? lz : 0;
-              if (!(dx | dy | dz)) continue;
-              const int g = push[27 * b + (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)];
-              if (g < 0) continue;
-              const std::int64_t d0 =
-                  static_cast<std::int64_t>(g) * V * cs +
-                  (static_cast<std::int64_t>(kk - dz * N) * E + NG + ty - dy * N) * E + NG + tx -
-                  dx * N;
-#pragma unroll
-              for (int n = 0; n < V; ++n) out[d0 + n * cs] = u1[n];
-            }
-      }
-      if (derived)
-        derived[static_cast<std::int64_t>(b) * cs + static_cast<std::int64_t>(kk) * E * E + col] =
-            0.5 * u1[3] * (u1[0] * u1[0] + u1[1] * u1[1] + u1[2] * u1[2]);
-      double sp = fabs(u1[0]) / C.dx[0];
-      sp = sp + fabs(u1[1]) / C.dx[1];
-      sp = sp + fabs(u1[2]) / C.dx[2];
-      m = DMin(m, 1.0 / sp);
+      // the z loop in two copies: the top plane of a block whose upper z face takes the
+      // correction store's fluxes loads them; no other plane has a load in this loop
+      auto zloop = [&](auto top_tag) {
+        constexpr bool TOP = decltype(top_tag)::value;
+  #pragma unroll
+        for (int n = 0; n < V; ++n) {
+          const double uo = own_pl[n * PS];
+          double ql, qr;
+          FastStates(RECON, r0[n], uo, r2[n], r3[n], r4[n], ql, qr);
+          double &zl = zs[n * NT + threadIdx.x], &flo = zs[(V + n) * NT + threadIdx.x];
+          double fh = HllFluxN(n, az, bz, zl, qr);
+          if (TOP) fh = fcor[oz1 + tx + ty * N + n * comp];
+          double &slot = sd[n * N * SP + own_sd];
+          const double du = slot + (A * fh - A * flo);
+          flo = fh;
+          zl = ql;
+          const double dudt = -du * ivol;
+          const double avg = beta * uo + w2 * (same_base ? uo : bv[n]);
+          slot = 1.0 * avg + bdt * dudt;  // the new state waits here until the next plane
+          r0[n] = uo;  // the ring moves up one plane
+          r1[n] = r2[n];
+          r2[n] = r3[n];
+          r3[n] = r4[n];
+          // one component at a time: interleaving the seven reconstructions needs more
+          // registers than the budget holds
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      };
+      if (k == N - 1 && oz1 >= 0)
+        zloop(std::true_type());
+      else
+        zloop(std::false_type());
     }
+    flush(b, k1 - 1);  // the segment's last plane (own slots only: no barrier needed)
   }
   if (!partial) return;
   m = WaveMin(m);
@@ -828,9 +901,10 @@ pth_status BurgersSweep(const Topology &t, int V, const double *U, const double 
     const int *fover16 = fc ? fc->dev_face : nullptr;
     const double *fcor16 = fc ? fc->store : nullptr;
     int grid16 = 0;
-#define PTH_S16_W(NV, W)                                                                       \
+#define PTH_S16_R(NV, W, RC)                                                                   \
   {                                                                                            \
-    const int r = ResidentGroups(reinterpret_cast<const void *>(&BlockSweep16Kernel<NV, W>), 256); \
+    const int r =                                                                              \
+        ResidentGroups(reinterpret_cast<const void *>(&BlockSweep16Kernel<NV, W, RC>), 256);    \
     if (r < 1) {                                                                               \
       SetError("block sweep: occupancy query failed");                                         \
       return PTH_FAIL;                                                                         \
@@ -838,10 +912,14 @@ pth_status BurgersSweep(const Topology &t, int V, const double *U, const double 
     grid16 = static_cast<int>(std::min<std::int64_t>(std::min(std::max(r / spread, 1),          \
                                                               max_partials),                   \
                                                      static_cast<std::int64_t>(nb) * 16));     \
-    hipLaunchKernelGGL((BlockSweep16Kernel<NV, W>), dim3(grid16), dim3(256), 0, s, U, base,     \
-                       out, derived, partial, coef, nb, beta, dtvar, recon, fover16, fcor16,   \
-                       push);                                                                  \
+    hipLaunchKernelGGL((BlockSweep16Kernel<NV, W, RC>), dim3(grid16), dim3(256), 0, s, U, base, \
+                       out, derived, partial, coef, nb, beta, dtvar, fover16, fcor16, push);   \
   }
+#define PTH_S16_W(NV, W)                                                                       \
+  if (recon == 0)                                                                              \
+    PTH_S16_R(NV, W, 0)                                                                        \
+  else                                                                                         \
+    PTH_S16_R(NV, W, 1)
 #define PTH_S16_CASE(NV)                                                                       \
   case NV:                                                                                     \
     if (minw == 3)                                                                             \
@@ -863,6 +941,7 @@ pth_status BurgersSweep(const Topology &t, int V, const double *U, const double 
     }
 #undef PTH_S16_CASE
 #undef PTH_S16_W
+#undef PTH_S16_R
     PTH_HIP(hipGetLastError());
     if (groups) *groups = grid16;
     return PTH_COMPLETE;
