@@ -472,12 +472,19 @@ def adaptive_bench(args, quiet=False):
     return out
 
 
-def exchange_bench(args):
+def exchange_bench(args, quiet=False, pack=None, steps=None, warmup=None):
     """Ghost-exchange microbenchmark: 128^3 mesh in 512 blocks of 16^3, nghost 3, 8 FP64
     one-component fields, U(-1,1) interiors; one step = SendBoundBufs + ReceiveBoundBufs +
-    SetBounds over all 26 neighbour buffers of every block."""
+    SetBounds over all 26 neighbour buffers of every block. Returns the line's dict."""
     import parthenon_amd as pa
     from parthenon_amd.inputs import interior_uniform
+    args = argparse.Namespace(**vars(args))
+    if pack is not None:
+        args.pack = pack
+    if steps is not None:
+        args.steps = steps
+    if warmup is not None:
+        args.warmup = warmup
 
     m = pa.Mesh((128, 128, 128), (16, 16, 16), 3)
     for f in range(8):
@@ -508,7 +515,7 @@ def exchange_bench(args):
     us = {k: ms / c * 1e3 for k, (c, ms) in prof.items()}
     dom = max(prof, key=lambda k: prof[k][1])
     achieved = 16 * nel / (us[dom] * 1e-6) / 1e9
-    print(json.dumps({
+    out = {
         "metric": "boundary exchanges/s (Send+Recv+Set, 26 buffers per block)",
         "value": args.steps / wall, "unit": "exchanges/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
@@ -522,7 +529,11 @@ def exchange_bench(args):
         "exchange_roofline": {"algorithmic_bytes": alg,
                               "achieved_GBps": alg / (wall / args.steps) / 1e9,
                               "frac": alg / (wall / args.steps) / 1e9 / HBM_PEAK_GBS},
-        "kernels_avg_us": us}))
+        "kernels_avg_us": us}
+    del md, m
+    if not quiet:
+        print(json.dumps(out))
+    return out
 
 
 def main():
@@ -546,6 +557,8 @@ def main():
                          "initialization")
     ap.add_argument("--no-adaptive", action="store_true",
                     help="burgers-amr: skip the adaptive remesh measurement in the same line")
+    ap.add_argument("--no-exchange", action="store_true",
+                    help="leave out the exchange sub-measurement of the default line")
     ap.add_argument("--no-gmg", action="store_true",
                     help="burgers-amr: skip the GMG solve and served-stream sub-objects")
     ap.add_argument("--cpu-cycles", type=int, default=None)
@@ -718,6 +731,13 @@ def main():
                                                   no_cpu_baseline=args.no_cpu_baseline),
                                quick=True)
         out["gmg_serve"] = gmg_serve_bench(args, nreq=600, quick=True)
+    if amr and args.driver != "tasks" and not args.no_exchange:
+        # the ghost-exchange microbenchmark (128^3 in 512 blocks of 16^3, nghost 3, 8 fields)
+        # in its fused (direct copy) and buffered (pack + unpack) forms
+        drv = None
+        out["exchange"] = {
+            form: exchange_bench(args, quiet=True, pack=pk, steps=100, warmup=5)
+            for form, pk in (("fused", "fused"), ("buffered", "persistent"))}
     if not args.no_cpu_baseline:
         threads, host = host_cores()
         cw = cpu_baseline(nx, block, g, ns, spheres, seed, cpu_cycles, threads)

@@ -26,71 +26,6 @@ namespace pth {
 
 namespace {
 
-// WENO5Z of recon.hpp:28-99 with its eight divides folded into three: one reciprocal of
-// be0 * be1 * be2 gives the three 1 / be_k, one reciprocal of (S_l D_l)(S_r D_r) gives both
-// sides' 1 / (S D) (S = w0 + w1 + w2, D = g2 w0 w1 + g1 w0 w2 + g0 w1 w2, so that
-// ql = N_l / S = N_l D_l / (S_l D_l) and al = 3 w0 w1 w2 / (S_l D_l) + eps), and alin keeps
-// its own. The values are the reference's to rounding. Range: be_k >= eps and w_k >= g_k,
-// and w_k <= g_k (1 + tau5 / eps), so (S_l D_l)(S_r D_r) <= ~w^6 stays finite while tau5 is
-// below ~1e36 (a jump in q of ~1e18 beside a flat stencil); the reference's own form
-// overflows at ~w^3, tau5 ~1e87.
-__device__ __forceinline__ void Weno5ZSweep(double q0, double q1, double q2, double q3,
-                                            double q4, double &ql, double &qr) {
-  const double a00 = 1.0 / 3.0, a01 = -7.0 / 6.0, a02 = 11.0 / 6.0;
-  const double a10 = -1.0 / 6.0, a11 = 5.0 / 6.0, a12 = 1.0 / 3.0;
-  const double a20 = 1.0 / 3.0, a21 = 5.0 / 6.0, a22 = -1.0 / 6.0;
-  const double g0 = 0.1, g1 = 0.6, g2 = 0.3;
-  const double eps = 10.0 * DBL_EPSILON;  // robust::EPS (src/utils/robust.hpp:38-41)
-  const double t13 = 13.0 / 3.0;
-  double a = q0 - 2 * q1 + q2;
-  double b = q0 - 4.0 * q1 + 3.0 * q2;
-  const double be0 = t13 * a * a + b * b + eps;
-  a = q1 - 2.0 * q2 + q3;
-  b = q3 - q1;
-  const double be1 = t13 * a * a + b * b + eps;
-  a = q2 - 2.0 * q3 + q4;
-  b = q4 - 4.0 * q3 + 3.0 * q2;
-  const double be2 = t13 * a * a + b * b + eps;
-  const double tau5 = fabs(be2 - be0);
-  const double p12 = be1 * be2, p02 = be0 * be2, p01 = be0 * be1;
-  const double rp = 1.0 / (be0 * p12);
-  const double z0 = (be0 + tau5) * (rp * p12);
-  const double z1 = (be1 + tau5) * (rp * p02);
-  const double z2 = (be2 + tau5) * (rp * p01);
-  // left
-  const double l0 = g0 * z0 + eps, l1 = g1 * z1 + eps, l2 = g2 * z2 + eps;
-  const double nl = l0 * (a00 * q0 + a01 * q1 + a02 * q2) + l1 * (a10 * q1 + a11 * q2 + a12 * q3) +
-                    l2 * (a20 * q2 + a21 * q3 + a22 * q4);
-  const double dl = g2 * l0 * l1 + g1 * l0 * l2 + g0 * l1 * l2;
-  const double sdl = (l0 + l1 + l2) * dl;
-  // right
-  const double r0 = g0 * z2 + eps, r1 = g1 * z1 + eps, r2 = g2 * z0 + eps;
-  const double nr = r0 * (a00 * q4 + a01 * q3 + a02 * q2) + r1 * (a10 * q3 + a11 * q2 + a12 * q1) +
-                    r2 * (a20 * q2 + a21 * q1 + a22 * q0);
-  const double dr = g2 * r0 * r1 + g1 * r0 * r2 + g0 * r1 * r2;
-  const double sdr = (r0 + r1 + r2) * dr;
-  const double rr = 1.0 / (sdl * sdr);
-  const double il = rr * sdr, ir = rr * sdl;  // 1 / (S_l D_l), 1 / (S_r D_r)
-  const double wl = nl * dl * il, wr = nr * dr * ir;
-  const double al = 3.0 * l0 * l1 * l2 * il + eps;
-  const double ar = 3.0 * r0 * r1 * r2 * ir + eps;
-  double dq = q3 - q2;
-  dq = 0.5 * Mc(q2 - q1, dq);
-  const double alin = 2.0 * al * ar / (al + ar);
-  ql = alin * wl + (1.0 - alin) * (q2 + dq);
-  qr = alin * wr + (1.0 - alin) * (q2 - dq);
-}
-
-// CellStates with the sweep's WENO5Z.
-__device__ __forceinline__ void SweepStates(const double *q, unsigned c, unsigned st, int recon,
-                                            double &ql, double &qr) {
-  if (recon == 0)
-    Weno5ZSweep(LoadAt(q, c - 2 * st), LoadAt(q, c - st), LoadAt(q, c), LoadAt(q, c + st),
-                LoadAt(q, c + 2 * st), ql, qr);
-  else
-    LinearRecon(LoadAt(q, c - st), LoadAt(q, c), LoadAt(q, c + st), ql, qr);
-}
-
 // HllGen with the scalar components multiplied by the reciprocal the velocity components
 // already use (burgers_package.cpp:307-401 divides them by the same denominator).
 template <int V, class LS, class RS>
@@ -183,6 +118,16 @@ __device__ __forceinline__ void FastStates(int recon, double q0, double q1, doub
     Weno5ZFast(q0, q1, q2, q3, q4, ql, qr);
   else
     LinearRecon(q1, q2, q3, ql, qr);
+}
+
+// CellStates with the sweep's WENO5Z (the 8^3 block sweep and its flux-correction send).
+__device__ __forceinline__ void SweepStates(const double *q, unsigned c, unsigned st, int recon,
+                                            double &ql, double &qr) {
+  if (recon == 0)
+    Weno5ZFast(LoadAt(q, c - 2 * st), LoadAt(q, c - st), LoadAt(q, c), LoadAt(q, c + st),
+               LoadAt(q, c + 2 * st), ql, qr);
+  else
+    LinearRecon(LoadAt(q, c - st), LoadAt(q, c), LoadAt(q, c + st), ql, qr);
 }
 
 // HLL of burgers_package.cpp:307-401 (lr_to_flux, burgers_package.hpp:33-43) as two
