@@ -55,9 +55,17 @@ typedef enum { PTH_PACK_PERSISTENT = 0, PTH_PACK_PER_BUFFER = 1 } pth_pack_mode;
 
 /* Boundary condition of a domain face (<parthenon/mesh> ix1_bc ... ox3_bc, src/mesh/mesh.cpp:
  * 1180-1185; the reference's "outflow" and "reflecting" functions, src/bvals/
- * boundary_conditions.cpp:89-137). Faces are numbered like BoundaryFace: 0 inner x1,
- * 1 outer x1, 2 inner x2, 3 outer x2, 4 inner x3, 5 outer x3. */
-typedef enum { PTH_BC_PERIODIC = 0, PTH_BC_OUTFLOW = 1, PTH_BC_REFLECT = 2 } pth_bc;
+ * boundary_conditions.cpp:89-137; FixedFace with value 0, GenericBC<..., BCType::FixedFace>
+ * of boundary_conditions_generic.hpp:229-232 as the poisson_gmg package registers it,
+ * example/poisson_gmg/poisson_package.cpp:60-78: ghost = 2 * 0 - mirror cell, every
+ * component). Faces are numbered like BoundaryFace: 0 inner x1, 1 outer x1, 2 inner x2,
+ * 3 outer x2, 4 inner x3, 5 outer x3. */
+typedef enum {
+  PTH_BC_PERIODIC = 0,
+  PTH_BC_OUTFLOW = 1,
+  PTH_BC_REFLECT = 2,
+  PTH_BC_FIXED_FACE = 3
+} pth_bc;
 
 typedef struct pth_mesh pth_mesh;         /* Mesh: blocks, neighbours, fields, comm buffers */
 typedef struct pth_meshdata pth_meshdata; /* MeshData: one named container over all blocks */

@@ -162,10 +162,14 @@ class Mesh {
   Mesh(const Mesh &) = delete;
   Mesh &operator=(const Mesh &) = delete;
 
-  // <parthenon/mesh> ix1_bc ... ox3_bc (src/mesh/mesh.cpp:1180-1185): "outflow" or
-  // "reflecting" on a face of a non-periodic dimension (its default is outflow)
+  // <parthenon/mesh> ix1_bc ... ox3_bc (src/mesh/mesh.cpp:1180-1185): "outflow",
+  // "reflecting" or "fixed_face" (FixedFace with value 0, the poisson_gmg package's
+  // condition) on a face of a non-periodic dimension (its default is outflow)
   void SetBoundaryCondition(BoundaryFace face, const std::string &name) {
-    int type = name == "outflow" ? PTH_BC_OUTFLOW : name == "reflecting" ? PTH_BC_REFLECT : -1;
+    int type = name == "outflow"      ? PTH_BC_OUTFLOW
+               : name == "reflecting" ? PTH_BC_REFLECT
+               : name == "fixed_face" ? PTH_BC_FIXED_FACE
+                                      : -1;
     if (type < 0) throw std::runtime_error("SetBoundaryCondition: unknown condition " + name);
     detail::Check(pth_mesh_set_boundary(h_, static_cast<int>(face), type),
                   "SetBoundaryCondition");

@@ -57,7 +57,7 @@ void *orc_mesh_new(const int *nx, const int *bnx, int nghost, const double *xmin
 int orc_mesh_set_boundary(void *h, int face, int type) {
   Mesh &m = *static_cast<OMesh *>(h)->m;
   if (face < 0 || face >= 6 || m.d.periodic[face / 2]) return -1;
-  if (type != kBcOutflow && type != kBcReflect) return -1;
+  if (type != kBcOutflow && type != kBcReflect && type != kBcFixedFace) return -1;
   m.d.bc[face] = type;
   return 0;
 }

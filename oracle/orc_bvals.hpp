@@ -415,11 +415,12 @@ inline bool PhysicalFace(const Mesh &m, const Block &b, int face) {
   return face % 2 == 0 ? b.loc.l[d] == 0 : b.loc.l[d] == N - 1;
 }
 
-// reference GenericBC<DIR, SIDE, Outflow|Reflect> (boundary_conditions_generic.hpp:174-255)
-// for one cell-centred FillGhost field of one block: the loop of par_for_bndry over
+// reference GenericBC<DIR, SIDE, Outflow|Reflect|FixedFace> (boundary_conditions_generic.hpp:
+// 174-255) for one cell-centred FillGhost field of one block: the loop of par_for_bndry over
 // IndexDomain::inner_xD / outer_xD (ghost layers along D, the entire range along the other
 // dimensions; domain.hpp:184-252), Outflow copying the first / last interior cell (ref),
-// Reflect copying the mirror cell (offset - index) and negating the vector component D.
+// Reflect copying the mirror cell (offset - index) and negating the vector component D,
+// FixedFace with value 0 setting 2 * 0 - the mirror cell (poisson_package.cpp:60-78).
 inline void GenericBC(const Mesh &m, const Block &b, Field &f, bool coarse, int dir, bool inner,
                       int type) {
   const Shape &sh = coarse ? m.c_cellbounds : m.cellbounds;
@@ -443,9 +444,13 @@ inline void GenericBC(const Mesh &m, const Block &b, Field &f, bool coarse, int 
       for (int j = lo[1]; j <= hi[1]; ++j)
         for (int i = lo[0]; i <= hi[0]; ++i) {
           int src[3] = {i, j, k};
-          src[dir] = type == kBcReflect ? offset - src[dir] : ref;
+          const bool mirror = type == kBcReflect || type == kBcFixedFace;
+          src[dir] = mirror ? offset - src[dir] : ref;
           const double q = At(slab, sh, l, src[2], src[1], src[0]);
-          At(slab, sh, l, k, j, i) = type == kBcReflect ? (reflect ? -1.0 : 1.0) * q : q;
+          const double val = 0.0;
+          At(slab, sh, l, k, j, i) = type == kBcFixedFace ? 2.0 * val - q
+                                     : type == kBcReflect ? (reflect ? -1.0 : 1.0) * q
+                                                          : q;
         }
   }
 }

@@ -100,7 +100,7 @@ struct Block {
 
 // Boundary condition of a domain face (reference <parthenon/mesh> ix1_bc ... ox3_bc,
 // mesh.cpp:1180-1185, default "outflow"; "periodic" on both faces of a periodic dimension).
-enum BcType { kBcPeriodic = 0, kBcOutflow = 1, kBcReflect = 2 };
+enum BcType { kBcPeriodic = 0, kBcOutflow = 1, kBcReflect = 2, kBcFixedFace = 3 };
 
 struct MeshDesc {
   int nx[3];

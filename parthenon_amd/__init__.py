@@ -58,6 +58,7 @@ class BoundaryCondition(enum.IntEnum):  # <parthenon/mesh> ix1_bc ... ox3_bc
     periodic = 0
     outflow = 1
     reflect = 2
+    fixed_face = 3  # FixedFace with value 0 (the poisson_gmg package's condition)
 
 
 class PackMode(enum.IntEnum):

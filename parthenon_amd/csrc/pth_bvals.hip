@@ -273,9 +273,16 @@ __global__ void __launch_bounds__(kWG) BoundaryConditionKernel(const BcBox *__re
     const int k = B.lo[2] + r / n01;
     const int si = B.a[0] + B.b[0] * i, sj = B.a[1] + B.b[1] * j, sk = B.a[2] + B.b[2] * k;
     double *q = B.slab + c * cs;
-    const double v = q[(static_cast<std::int64_t>(sk) * B.e[1] + sj) * B.e[0] + si];
-    const bool neg = B.vec && c < 3 && ((B.flip >> c) & 1);
-    q[(static_cast<std::int64_t>(k) * B.e[1] + j) * B.e[0] + i] = neg ? -v : v;
+    double v = q[(static_cast<std::int64_t>(sk) * B.e[1] + sj) * B.e[0] + si];
+    // the faces' operations in the reference's face order (x, then y, then z)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if ((B.fixed >> d) & 1)
+        v = 2.0 * 0.0 - v;
+      else if (B.vec && c == d && ((B.flip >> d) & 1))
+        v = -1.0 * v;
+    }
+    q[(static_cast<std::int64_t>(k) * B.e[1] + j) * B.e[0] + i] = v;
   }
 }
 

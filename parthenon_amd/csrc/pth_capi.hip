@@ -611,10 +611,13 @@ pth_status Mesh::BuildBoundaryConditions(MeshData &md, BoundCache &c) {
                 if (face < 0 || !phys[face]) continue;
                 mapped = true;
                 const int ref = cls[d] == 0 ? lo : hi;
-                if (topo.bc[face] == PTH_BC_REFLECT) {
+                if (topo.bc[face] == PTH_BC_REFLECT || topo.bc[face] == PTH_BC_FIXED_FACE) {
                   x.a[d] = 2 * ref + (cls[d] == 0 ? -1 : 1);
                   x.b[d] = -1;
-                  x.flip |= 1 << d;
+                  if (topo.bc[face] == PTH_BC_REFLECT)
+                    x.flip |= 1 << d;
+                  else
+                    x.fixed |= 1 << d;
                 } else {
                   x.a[d] = ref;
                   x.b[d] = 0;
@@ -826,8 +829,8 @@ pth_status pth_mesh_set_boundary(pth_mesh *mh, int face, int type) {
     SetError("boundary face must be 0..5 (inner x1, outer x1, ..., outer x3)");
     return PTH_FAIL;
   }
-  if (type != PTH_BC_OUTFLOW && type != PTH_BC_REFLECT) {
-    SetError("boundary condition must be PTH_BC_OUTFLOW or PTH_BC_REFLECT");
+  if (type != PTH_BC_OUTFLOW && type != PTH_BC_REFLECT && type != PTH_BC_FIXED_FACE) {
+    SetError("boundary condition must be PTH_BC_OUTFLOW, PTH_BC_REFLECT or PTH_BC_FIXED_FACE");
     return PTH_FAIL;
   }
   if (m->topo.periodic[face / 2]) {

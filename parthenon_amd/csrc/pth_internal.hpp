@@ -272,6 +272,8 @@ struct BcBox {
   int ncomp;
   int flip;      // reflecting dimensions
   int vec;       // 1 for a Metadata::Vector field
+  int fixed;     // FixedFace-0 dimensions (every component: 2 * 0 - value, in face order)
+  int pad;
 };
 static_assert(sizeof(BcBox) % 8 == 0, "bc box alignment");
 

@@ -77,19 +77,21 @@ def bc_index_field(mesh_like, ncomp, nx, bcs, vector=False):
     """Single-level mesh with physical boundaries: value = sign * (c*1e7 + global cell id) of
     the source cell of every cell. Along each dimension a global index outside the mesh maps
     back inside by its face's condition: periodic wraps, outflow copies the first / last cell,
-    reflect mirrors across the face. Component c < 3 of a vector field changes sign once per
-    reflecting face it crosses along dimension c."""
+    reflect and fixed_face mirror across the face. Component c < 3 of a vector field changes
+    sign once per reflecting face it crosses along dimension c; every component changes sign
+    once per fixed_face face it crosses (ghost = 2 * 0 - mirror cell, a zero face value)."""
     shape = (mesh_like.nblocks, ncomp) + tuple(reversed(mesh_like.ent))
     expected = np.empty(shape)
     for gid in range(mesh_like.nblocks):
         b = mesh_like.block(gid)
         assert b["level"] == 0
-        idx, refl = [], []
+        idx, refl, fixed = [], [], []
         for d in range(3):
             g = mesh_like.ng[d]
             gi = b["l"][d] * mesh_like.n[d] + np.arange(mesh_like.ent[d]) - g
             src = gi.copy()
             r = np.zeros(gi.shape, dtype=bool)
+            f = np.zeros(gi.shape, dtype=bool)
             lo, hi = gi < 0, gi >= nx[d]
             for mask, face, edge, mirror in ((lo, bcs[2 * d], 0, -1 - gi),
                                              (hi, bcs[2 * d + 1], nx[d] - 1,
@@ -98,17 +100,23 @@ def bc_index_field(mesh_like, ncomp, nx, bcs, vector=False):
                     src[mask] = np.mod(gi[mask], nx[d])
                 elif face == "outflow":
                     src[mask] = edge
+                elif face == "fixed_face":
+                    src[mask] = mirror[mask]
+                    f |= mask
                 else:
                     src[mask] = mirror[mask]
                     r |= mask
             idx.append(src)
             refl.append(r)
+            fixed.append(f)
         gk, gj, gi = np.meshgrid(idx[2], idx[1], idx[0], indexing="ij")
         rk, rj, ri = np.meshgrid(refl[2], refl[1], refl[0], indexing="ij")
+        fk, fj, fi = np.meshgrid(fixed[2], fixed[1], fixed[0], indexing="ij")
+        nfix = fi.astype(int) + fj.astype(int) + fk.astype(int)
         cell = (gk * nx[1] + gj) * nx[0] + gi
         for c in range(ncomp):
             v = c * 1.0e7 + cell.astype(np.float64)
             if vector and c < 3:
                 v = np.where((ri, rj, rk)[c], -v, v)
-            expected[gid, c] = v
+            expected[gid, c] = np.where(nfix % 2 == 1, -v, v)
     return expected

@@ -121,7 +121,7 @@ def iarr(vals, t=C.c_int):
     return (t * len(vals))(*vals)
 
 
-BC_TYPES = {"periodic": 0, "outflow": 1, "reflect": 2}
+BC_TYPES = {"periodic": 0, "outflow": 1, "reflect": 2, "fixed_face": 3}
 
 
 class OracleMesh:

@@ -21,6 +21,9 @@ CASES = [
                                   "reflect")),
     ((16, 16, 32), (8, 8, 8), 4, ("outflow", "outflow", "reflect", "reflect", "periodic",
                                   "periodic")),
+    ((8, 8, 8), (8, 8, 8), 3, ("fixed_face",) * 6),
+    ((16, 16, 16), (8, 8, 8), 4, ("fixed_face", "outflow", "reflect", "fixed_face",
+                                  "periodic", "periodic")),
 ]
 
 

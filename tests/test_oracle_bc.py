@@ -21,6 +21,11 @@ CASES = [
                                   "reflect")),
     ((16, 16, 32), (8, 8, 8), 4, ("outflow", "outflow", "reflect", "reflect", "periodic",
                                   "periodic")),
+    # fixed_face (zero face value, the Dirichlet faces of the Poisson solvers): the ghost is
+    # the negated mirror cell, and corners crossing two fixed faces get the sign back
+    ((8, 8, 8), (8, 8, 8), 3, ("fixed_face",) * 6),
+    ((16, 16, 16), (8, 8, 8), 4, ("fixed_face", "outflow", "reflect", "fixed_face",
+                                  "periodic", "periodic")),
 ]
 
 
