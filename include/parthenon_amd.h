@@ -29,7 +29,7 @@ typedef enum {
 
 /* BoundaryType: src/basic_types.hpp:156 (any, local, nonlocal, flxcor_send, flxcor_recv;
  * the reference's gmg_* types are not offered: the multigrid solver keeps its level transfers
- * inside pth_gmg_*). On one GPU every neighbour is local, so any == local and
+ * inside pth_gmg_* and pth_mg_*). On one GPU every neighbour is local, so any == local and
  * nonlocal has no buffers. */
 typedef enum {
   PTH_BT_ANY = 0,
@@ -358,6 +358,69 @@ pth_status pth_gmg_set_column_sweeps(pth_gmg *h, int on);
 pth_status pth_gmg_solve(pth_gmg *g, double tol, int max_iters, int *iters, double *final_rel);
 /* relative residual before every V-cycle and after the last; returns the count */
 int pth_gmg_history(const pth_gmg *g, double *out, int max);
+
+/* ---- solver field utilities over MeshData (src/solvers/solver_utils.hpp:151-277) ----
+ * Cell fields of md with equal component counts; asynchronous on the mesh stream except the
+ * dot product, which returns its value.
+ * copy_data: CopyData<in, out> (:151-178), every cell (ghosts included), every component.
+ * add_fields_and_store: AddFieldsAndStore<a, b, out> (:180-222), out = wa a + wb b on every
+ *   cell (ghosts included); out may be a or b.
+ * set_to_zero: SetToZero<var> (:224-250), interior cells.
+ * dot_product: DotProductLocal<a, b> (:252-277) over every block of md, the sum over interior
+ *   cells and components of a * b (fixed summation order: the same bits on every call). */
+pth_status pth_solver_copy_data(pth_meshdata *md, const char *in, const char *out);
+pth_status pth_solver_add_fields_and_store(pth_meshdata *md, const char *a, const char *b,
+                                           const char *out, double wa, double wb);
+pth_status pth_solver_set_to_zero(pth_meshdata *md, const char *var);
+pth_status pth_solver_dot_product(pth_meshdata *md, const char *a, const char *b, double *out);
+
+/* ---- the Poisson equation over MeshData (example/poisson_gmg/poisson_equation.hpp:41-205) ----
+ * div(D grad u) - diagonal_alpha u = rhs with a constant D (the reference's face field D held
+ * constant, poisson_package.cpp:112-118) and the package's diagonal_alpha (:83-84). */
+typedef struct pth_poisson_equation {
+  double D;
+  double diagonal_alpha;
+} pth_poisson_equation;
+/* PoissonEquation::Ax<x, out> (:41-53): CalculateFluxes (:131-184) then FluxMultiplyMatrix
+ * (:186-205) on the interior cells of every block of md, without flux correction:
+ * out = -alpha x + sum_d (F_d(lo) - F_d(hi)) / dx_d from x's cells and one ghost layer (fill
+ * them first: the boundary exchange with its conditions). x and out: distinct one-component
+ * fields. The x3 face flux is the 7-point operator's (the reference stores the top x3 flux in
+ * the x2 slot, :153-157). */
+pth_status pth_poisson_ax(pth_meshdata *md, const pth_poisson_equation *eq, const char *x,
+                          const char *out);
+
+/* ---- MGSolver<u, rhs, PoissonEquation> over MeshData (src/solvers/mg_solver.hpp:37-203) ----
+ * Mesh: uniform, single-level, 3-D, N^3 cells on a cube of side L, FixedFace (fixed_face) on
+ * all six faces; u (FillGhost) and rhs one-component fields; constant D > 0 and
+ * diagonal_alpha = 0. The hierarchy is the uniform one of src/mesh/mesh-gmg.cpp:103-154 (cell
+ * counts halving while even) held as whole-level arrays; the problem is solved on the unit
+ * cube with rhs scaled by L^2 / D (exact when L = D = 1). */
+typedef struct pth_mg_params {
+  int max_iters;             /* MGParams::max_iters (mg_solver.hpp:37-58): V-cycles at most */
+  double residual_tolerance; /* stop when the RMS residual sqrt(sum r^2 / ncells) of
+                                rhs - A u is below it (mg_solver.hpp:147-158) */
+  int precision;             /* 0: FP32 V-cycles inside FP64 iterative refinement; 1: FP64 */
+  int inner_cycles;          /* FP32 V-cycles per FP64 refinement step (precision 0), >= 1 */
+  double omega;              /* weighted-Jacobi weight (the reference fixes SRJ weights,
+                                mg_solver.hpp:312-345) */
+  int pre_sweeps, post_sweeps;
+} pth_mg_params;
+typedef struct pth_mg pth_mg;
+pth_mg *pth_mg_create(pth_mesh *m, const char *u, const char *rhs, const pth_mg_params *p,
+                      const pth_poisson_equation *eq);
+void pth_mg_destroy(pth_mg *mg);
+/* MGSolver::AddSetupTasks (mg_solver.hpp:188-203): gathers rhs and the initial u of md's
+ * blocks into the solver's FP64 finest level (after the work queued on the mesh stream) */
+pth_status pth_mg_setup(pth_mg *mg, pth_meshdata *md);
+/* MGSolver::AddTasks (mg_solver.hpp:119-161): at least one V-cycle, then V-cycles until the
+ * RMS residual is below the tolerance or max_iters cycles ran; writes u's interior cells and
+ * refills its ghosts (the boundary exchange of md, with the FixedFace conditions) */
+pth_status pth_mg_solve(pth_mg *mg, pth_meshdata *md);
+/* GetFinalIterations / GetFinalResidual (mg_solver.hpp:207-208): V-cycles run, RMS residual */
+pth_status pth_mg_result(const pth_mg *mg, int *iterations, double *rms_residual);
+/* RMS residual before every V-cycle and after the last; returns the count */
+int pth_mg_history(const pth_mg *mg, double *rms, int max);
 
 /* Serving loop of independent Poisson solves (no counterpart in the reference, which holds
  * one Mesh per process, src/parthenon_manager.cpp:170-173). Requests arrive at given times;

@@ -17,12 +17,15 @@
 #define PARTHENON_AMD_HPP_
 
 #include <array>
+#include <cmath>
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <set>
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "parthenon_amd.h"
@@ -63,6 +66,98 @@ inline TaskStatus Check(pth_status s, const char *what) {
   return static_cast<TaskStatus>(s);
 }
 }  // namespace detail
+
+// ---- tasks (src/tasks/tasks.hpp) ----
+// On one GPU a partition holds every block and the library orders device work on the mesh
+// stream, so a task list runs its tasks in the order they were added (every dependency was
+// added before the task that names it). A task returning incomplete is polled again, like
+// Task::operator() re-enqueueing itself (tasks.hpp:93-118); fail and iterate end the list
+// with an exception (iterative sublists are not offered: the solvers iterate inside the
+// library).
+struct TaskID {
+  int id = -1;  // -1: no dependency
+  TaskID() = default;
+  explicit TaskID(int i) : id(i) {}
+  TaskID operator|(const TaskID &o) const { return TaskID(id > o.id ? id : o.id); }
+};
+
+class TaskList {
+ public:
+  // AddTask(dependency, function, arguments...): the function is called with copies of the
+  // arguments when the list runs
+  template <class F, class... Args>
+  TaskID AddTask(TaskID /*dependency*/, F &&f, Args &&...args) {
+    tasks_.emplace_back(
+        [f = std::forward<F>(f), a = std::make_tuple(std::forward<Args>(args)...)]() mutable {
+          return std::apply(f, a);
+        });
+    return TaskID(static_cast<int>(tasks_.size()) - 1);
+  }
+  template <class F>
+  TaskID AddTask(F &&f) {
+    return AddTask(TaskID(), std::forward<F>(f));
+  }
+  int size() const { return static_cast<int>(tasks_.size()); }
+  void Execute() {
+    for (size_t q = 0; q < tasks_.size(); ++q) {
+      TaskStatus s;
+      int polls = 0;
+      while ((s = tasks_[q]()) == TaskStatus::incomplete)
+        if (++polls > kMaxPolls)
+          throw std::runtime_error("TaskList: task " + std::to_string(q) + " never completed");
+      if (s != TaskStatus::complete)
+        throw std::runtime_error("TaskList: task " + std::to_string(q) +
+                                 (s == TaskStatus::fail ? " failed" : " asked to iterate"));
+    }
+  }
+
+ private:
+  static constexpr int kMaxPolls = 1000;
+  std::vector<std::function<TaskStatus()>> tasks_;
+};
+
+// TaskRegion / TaskCollection (tasks.hpp): regions run one after the other, the lists of a
+// region (one per partition; one partition per GPU) one after the other
+class TaskRegion {
+ public:
+  explicit TaskRegion(int n) : lists_(n) {}
+  TaskList &operator[](int i) { return lists_.at(i); }
+  int size() const { return static_cast<int>(lists_.size()); }
+  void Execute() {
+    for (auto &l : lists_) l.Execute();
+  }
+
+ private:
+  std::vector<TaskList> lists_;
+};
+
+class TaskCollection {
+ public:
+  TaskRegion &AddRegion(int n) {
+    regions_.emplace_back(new TaskRegion(n));
+    return *regions_.back();
+  }
+  TaskStatus Execute() {
+    for (auto &r : regions_) r->Execute();
+    return TaskStatus::complete;
+  }
+
+ private:
+  std::vector<std::unique_ptr<TaskRegion>> regions_;
+};
+
+// the reference's task-function wrapper; plain functions and lambdas are tasks here
+#ifndef TF
+#define TF(...) __VA_ARGS__
+#endif
+
+// AllReduce<T> (src/utils/reductions.hpp): on one GPU the local value is the global one
+template <class T>
+struct AllReduce {
+  T val{};
+  TaskStatus StartReduce(int = 0) { return TaskStatus::complete; }
+  TaskStatus CheckReduce() { return TaskStatus::complete; }
+};
 
 class Mesh;
 
@@ -318,6 +413,24 @@ inline TaskStatus SetFluxCorrections(std::shared_ptr<MeshData<Real>> &md) {
 }
 
 // ---- Update:: tasks (src/interface/update.hpp) ----
+// AddBoundaryExchangeTasks<any> (src/bvals/comms/boundary_communication.cpp:405-445): send,
+// receive, set; on a multilevel mesh the coarse-buffer conditions and the prolongation; then
+// the physical boundary conditions
+template <BoundaryType BT>
+TaskID AddBoundaryExchangeTasks(TaskID dependency, TaskList &tl,
+                                std::shared_ptr<MeshData<Real>> &md, bool multilevel) {
+  static_assert(BT == BoundaryType::any, "AddBoundaryExchangeTasks: BoundaryType::any only");
+  auto send = tl.AddTask(dependency, SendBoundBufs<BT>, md);
+  auto recv = tl.AddTask(dependency, ReceiveBoundBufs<BT>, md);
+  auto set = tl.AddTask(send | recv, SetBounds<BT>, md);
+  auto pro = set;
+  if (multilevel) {
+    auto cbound = tl.AddTask(set, ApplyBoundaryConditionsOnCoarseOrFineMD, md, true);
+    pro = tl.AddTask(cbound, ProlongateBounds<BT>, md);
+  }
+  return tl.AddTask(pro, ApplyBoundaryConditionsOnCoarseOrFineMD, md, false);
+}
+
 namespace Update {
 template <typename T>
 TaskStatus FluxDivergence(T *in, T *dudt) {
@@ -341,41 +454,190 @@ TaskStatus AverageIndependentData(T *c1, T *c2, const Real wgt1) {
 
 // ---- multigrid Poisson solver (src/solvers/mg_solver.hpp:37-521) ----
 namespace solvers {
-// MGParams (mg_solver.hpp:37-58) with the weighted-Jacobi weight and sweep counts explicit;
-// residual_tolerance is relative: ||b - A u|| / ||b||.
+
+// Field types name fields the way the reference's VARIABLE(ns, varname) types do
+// (src/interface/make_pack_descriptor.hpp): a type with a static name().
+#define PARTHENON_AMD_VARIABLE(ns, varname)                     \
+  struct varname {                                              \
+    static std::string name() { return #ns "." #varname; }     \
+  }
+
+// solver_utils (src/solvers/solver_utils.hpp:151-301) over every block of md
+namespace utils {
+template <class in_t, class out_t>
+TaskStatus CopyData(std::shared_ptr<MeshData<Real>> &md) {  // :151-178, ghosts included
+  return detail::Check(
+      pth_solver_copy_data(md->handle(), in_t::name().c_str(), out_t::name().c_str()),
+      "CopyData");
+}
+template <class a_t, class b_t, class out_t>
+TaskStatus AddFieldsAndStore(std::shared_ptr<MeshData<Real>> &md, Real wa = 1.0,
+                             Real wb = 1.0) {  // :180-222, out = wa a + wb b
+  return detail::Check(pth_solver_add_fields_and_store(md->handle(), a_t::name().c_str(),
+                                                       b_t::name().c_str(),
+                                                       out_t::name().c_str(), wa, wb),
+                       "AddFieldsAndStore");
+}
+template <class var_t>
+TaskStatus SetToZero(std::shared_ptr<MeshData<Real>> &md) {  // :224-250, interior cells
+  return detail::Check(pth_solver_set_to_zero(md->handle(), var_t::name().c_str()),
+                       "SetToZero");
+}
+template <class a_t, class b_t>
+TaskStatus DotProductLocal(std::shared_ptr<MeshData<Real>> &md, AllReduce<Real> *adotb) {
+  Real v = 0;
+  TaskStatus s = detail::Check(
+      pth_solver_dot_product(md->handle(), a_t::name().c_str(), b_t::name().c_str(), &v),
+      "DotProduct");
+  adotb->val += v;
+  return s;
+}
+// DotProduct (:279-301): zero the result, add this partition's sum (one partition per GPU)
+template <class a_t, class b_t>
+TaskID DotProduct(TaskID dependency_in, TaskList &tl, AllReduce<Real> *adotb,
+                  std::shared_ptr<MeshData<Real>> &md) {
+  auto zero = tl.AddTask(dependency_in, [adotb]() {
+    adotb->val = 0.0;
+    return TaskStatus::complete;
+  });
+  return tl.AddTask(zero, DotProductLocal<a_t, b_t>, md, adotb);
+}
+}  // namespace utils
+
+// The poisson_gmg example's PoissonEquation (example/poisson_gmg/poisson_equation.hpp:41-205)
+// with a constant D and the package's diagonal_alpha. do_flux_cor: flux correction at
+// fine-coarse faces; the operator here runs on single-level meshes, where it changes nothing.
+class PoissonEquation {
+ public:
+  Real D = 1.0;
+  Real diagonal_alpha = 0.0;
+  bool do_flux_cor = false;
+  pth_poisson_equation Coefficients() const { return {D, diagonal_alpha}; }
+  // Ax<x_t, out_t> (:41-53): out = A x on every block's interior; x's ghosts must be filled
+  template <class x_t, class out_t>
+  TaskID Ax(TaskList &tl, TaskID depends_on, std::shared_ptr<MeshData<Real>> &md) const {
+    const pth_poisson_equation e = Coefficients();
+    return tl.AddTask(depends_on, [e, md]() {
+      return detail::Check(
+          pth_poisson_ax(md->handle(), &e, x_t::name().c_str(), out_t::name().c_str()), "Ax");
+    });
+  }
+};
+
+// MGParams (mg_solver.hpp:37-58): max_iters and residual_tolerance (RMS residual
+// sqrt(sum r^2 / ncells), the reference's stop test, :147-158) as the reference; the smoother
+// is weighted Jacobi with an explicit weight and sweep counts (the reference's SRJ tables fix
+// them, :312-345), inside FP64 iterative refinement when mixed_precision is set.
 struct MGParams {
   int max_iters = 1000;
-  Real residual_tolerance = 1.e-10;
+  Real residual_tolerance = 1.e-12;
   Real omega = 2.0 / 3.0;
   int pre_sweeps = 2, post_sweeps = 2;
   bool mixed_precision = true;  // FP32 V-cycles inside FP64 iterative refinement
   int inner_cycles = 2;          // FP32 V-cycles per FP64 refinement step
-  bool column_sweeps = true;     // Jacobi by z columns where the level size allows
 };
 
-// MGSolver<u, rhs, PoissonEquation> on the uniform n^3 grid of example/poisson_gmg ([0,1]^3,
-// FixedFace 0): set the right-hand side, Solve(), read the solution; host arrays in [k][j][i].
+// MGSolver<u, rhs, equations> (mg_solver.hpp:73-208) over the blocks of a MeshData: u and rhs
+// are field types (static name()), equations carries the operator's coefficients
+// (Coefficients(), as PoissonEquation). The mesh must be uniform, 3-D, N^3 cells on a cube,
+// with fixed_face conditions on every face (the poisson_gmg package's FixedFace).
+template <class u, class rhs, class equations>
 class MGSolver {
  public:
-  MGSolver(int n, MGParams p = MGParams()) : n_(n), p_(p) {
-    h_ = pth_gmg_create(n, p.mixed_precision ? 0 : 1, p.omega, p.pre_sweeps, p.post_sweeps);
-    if (!h_) throw std::runtime_error(std::string("MGSolver: ") + pth_last_error());
-    detail::Check(pth_gmg_set_inner_cycles(h_, p.mixed_precision ? p.inner_cycles : 1),
-                  "MGSolver");
-    detail::Check(pth_gmg_set_column_sweeps(h_, p.column_sweeps ? 1 : 0), "MGSolver");
-  }
+  explicit MGSolver(MGParams params_in = MGParams(), equations eq_in = equations(),
+                    std::string container = "base")
+      : params_(params_in), eqs_(eq_in), container_(std::move(container)) {}
   ~MGSolver() {
-    if (h_) pth_gmg_destroy(h_);
+    if (h_) pth_mg_destroy(h_);
   }
   MGSolver(const MGSolver &) = delete;
   MGSolver &operator=(const MGSolver &) = delete;
+
+  // AddSetupTasks (:188-203): gathers rhs and the current u of the partition's blocks
+  template <class TL_t>
+  TaskID AddSetupTasks(TL_t &tl, TaskID dependence, int partition, Mesh *pmesh) {
+    auto &md = pmesh->GetOrAdd(container_, partition);
+    return tl.AddTask(dependence, [this, pmesh, md]() {
+      if (!h_) Create(pmesh);
+      return detail::Check(pth_mg_setup(h_, md->handle()), "MGSolver::AddSetupTasks");
+    });
+  }
+  // AddTasks (:119-161): V-cycles until the RMS residual is below residual_tolerance (at
+  // least one, at most max_iters); u's interior and ghosts hold the solution afterwards
+  TaskID AddTasks(TaskList &tl, TaskID dependence, Mesh *pmesh, int partition) {
+    auto &md = pmesh->GetOrAdd(container_, partition);
+    return tl.AddTask(dependence, [this, md]() {
+      if (!h_) throw std::runtime_error("MGSolver::AddTasks: AddSetupTasks has not run");
+      TaskStatus s = detail::Check(pth_mg_solve(h_, md->handle()), "MGSolver::AddTasks");
+      detail::Check(pth_mg_result(h_, &final_iteration_, &final_residual_), "MGSolver");
+      return s;
+    });
+  }
+  Real GetFinalResidual() const { return final_residual_; }
+  int GetFinalIterations() const { return final_iteration_; }
+  // RMS residual before every V-cycle of the last solve and after its last
+  std::vector<Real> GetResidualHistory() const {
+    std::vector<Real> h(1024);
+    const int n = h_ ? pth_mg_history(h_, h.data(), static_cast<int>(h.size())) : 0;
+    h.resize(n < 1024 ? n : 1024);
+    return h;
+  }
+
+ private:
+  void Create(Mesh *pmesh) {
+    pth_mg_params p{};
+    p.max_iters = params_.max_iters;
+    p.residual_tolerance = params_.residual_tolerance;
+    p.precision = params_.mixed_precision ? 0 : 1;
+    p.inner_cycles = params_.inner_cycles;
+    p.omega = params_.omega;
+    p.pre_sweeps = params_.pre_sweeps;
+    p.post_sweeps = params_.post_sweeps;
+    const pth_poisson_equation e = eqs_.Coefficients();
+    h_ = pth_mg_create(pmesh->handle(), u::name().c_str(), rhs::name().c_str(), &p, &e);
+    if (!h_) throw std::runtime_error(std::string("MGSolver: ") + pth_last_error());
+  }
+  MGParams params_;
+  equations eqs_;
+  std::string container_;
+  pth_mg *h_ = nullptr;
+  Real final_residual_ = 0;
+  int final_iteration_ = 0;
+};
+
+// The standalone solver on the uniform n^3 grid of example/poisson_gmg ([0,1]^3, FixedFace
+// 0) without a mesh: set the right-hand side, Solve(), read the solution; host arrays in
+// [k][j][i]; residual_tolerance is relative here, ||b - A u|| / ||b||.
+struct UniformMGParams {
+  int max_iters = 1000;
+  Real residual_tolerance = 1.e-10;
+  Real omega = 2.0 / 3.0;
+  int pre_sweeps = 2, post_sweeps = 2;
+  bool mixed_precision = true;
+  int inner_cycles = 2;
+  bool column_sweeps = true;     // Jacobi by z columns where the level size allows
+};
+
+class UniformMGSolver {
+ public:
+  UniformMGSolver(int n, UniformMGParams p = UniformMGParams()) : n_(n), p_(p) {
+    h_ = pth_gmg_create(n, p.mixed_precision ? 0 : 1, p.omega, p.pre_sweeps, p.post_sweeps);
+    if (!h_) throw std::runtime_error(std::string("UniformMGSolver: ") + pth_last_error());
+    detail::Check(pth_gmg_set_inner_cycles(h_, p.mixed_precision ? p.inner_cycles : 1),
+                  "UniformMGSolver");
+    detail::Check(pth_gmg_set_column_sweeps(h_, p.column_sweeps ? 1 : 0), "UniformMGSolver");
+  }
+  ~UniformMGSolver() {
+    if (h_) pth_gmg_destroy(h_);
+  }
+  UniformMGSolver(const UniformMGSolver &) = delete;
+  UniformMGSolver &operator=(const UniformMGSolver &) = delete;
   void SetRhs(const std::vector<Real> &b) {
     detail::Check(pth_gmg_set_rhs(h_, b.data(), b.size()), "SetRhs");
   }
   void SetRhsBlobs(std::uint64_t seed) {
     detail::Check(pth_gmg_set_rhs_blobs(h_, seed), "SetRhsBlobs");
   }
-  // the AddTasks iteration (mg_solver.hpp:119-161) run to completion
   int Solve() {
     int it = 0;
     detail::Check(pth_gmg_solve(h_, p_.residual_tolerance, p_.max_iters, &it, &final_residual_),
@@ -392,7 +654,7 @@ class MGSolver {
 
  private:
   int n_;
-  MGParams p_;
+  UniformMGParams p_;
   pth_gmg *h_ = nullptr;
   Real final_residual_ = 0;
   int final_iteration_ = 0;

@@ -81,6 +81,19 @@ class PackView(C.Structure):
                 ("ent", C.c_int * 3), ("is_", C.c_int), ("ie", C.c_int), ("js", C.c_int),
                 ("je", C.c_int), ("ks", C.c_int), ("ke", C.c_int)]
 
+
+class PoissonEquationC(C.Structure):
+    """pth_poisson_equation of include/parthenon_amd.h."""
+    _fields_ = [("D", C.c_double), ("diagonal_alpha", C.c_double)]
+
+
+class MgParamsC(C.Structure):
+    """pth_mg_params of include/parthenon_amd.h."""
+    _fields_ = [("max_iters", C.c_int), ("residual_tolerance", C.c_double),
+                ("precision", C.c_int), ("inner_cycles", C.c_int), ("omega", C.c_double),
+                ("pre_sweeps", C.c_int), ("post_sweeps", C.c_int)]
+
+
 _SIGNATURES = {
     "pth_mesh_create": (_P, [C.POINTER(_I), C.POINTER(_I), _I, _DP, _DP, C.POINTER(_I)]),
     "pth_mesh_refine_box": (_I, [_P, _DP, _I]),
@@ -159,6 +172,18 @@ _SIGNATURES = {
     "pth_device_memory": (_I, [C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "pth_gmg_server_set_inner_cycles": (_I, [_P, _I]),
     "pth_gmg_history": (_I, [_P, _DP, _I]),
+    "pth_solver_copy_data": (_I, [_P, C.c_char_p, C.c_char_p]),
+    "pth_solver_add_fields_and_store": (_I, [_P, C.c_char_p, C.c_char_p, C.c_char_p, _D, _D]),
+    "pth_solver_set_to_zero": (_I, [_P, C.c_char_p]),
+    "pth_solver_dot_product": (_I, [_P, C.c_char_p, C.c_char_p, _DP]),
+    "pth_poisson_ax": (_I, [_P, C.POINTER(PoissonEquationC), C.c_char_p, C.c_char_p]),
+    "pth_mg_create": (_P, [_P, C.c_char_p, C.c_char_p, C.POINTER(MgParamsC),
+                           C.POINTER(PoissonEquationC)]),
+    "pth_mg_destroy": (None, [_P]),
+    "pth_mg_setup": (_I, [_P, _P]),
+    "pth_mg_solve": (_I, [_P, _P]),
+    "pth_mg_result": (_I, [_P, C.POINTER(_I), _DP]),
+    "pth_mg_history": (_I, [_P, _DP, _I]),
     "pth_gmg_server_create": (_P, [_I, _D, _I, _I, _D, _I, _I64, _I, _I]),
     "pth_gmg_server_destroy": (None, [_P]),
     "pth_gmg_server_submit": (_I, [_P, _I, C.c_uint64, _D, C.POINTER(_I)]),

@@ -87,6 +87,59 @@ struct GmgBatch {
   pth_status Reset(GmgProblem<T> &p);
 };
 
+struct GmgParams {
+  int precision;  // 0: FP32 inner cycle (mixed), 1: FP64
+  double omega;
+  int pre, post;
+  int inner = 1;    // V-cycles per FP64 refinement step
+  int columns = 1;  // Jacobi sweeps by z columns where the level size allows
+};
+
+// One solve of one problem: the MGSolver::AddTasks iteration (mg_solver.hpp:119-161) as a host
+// loop over the batched kernels: residual, stop test, V-cycle(s).
+template <typename T>
+struct GmgSolver1 {
+  GmgProblem<T> p;
+  GmgBatch<T> batch;
+  hipStream_t s = nullptr;
+  double bnorm = 0.0;
+  std::vector<double> history;
+  ~GmgSolver1() {
+    p.Free();
+    batch.Free();
+    if (s) (void)hipStreamDestroy(s);
+  }
+  pth_status Init(int N) {
+    PTH_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    if (batch.Init(s) != PTH_COMPLETE) return PTH_FAIL;
+    return p.Allocate(N, GmgBottom<T>::kN);
+  }
+  pth_status Norm() { return batch.RhsNorms({&p}, &bnorm); }
+  // relative residual ||b - A u|| / ||b|| (||r|| when b = 0) below tol ends the loop once
+  // min_cycles V-cycles have run; max_iters cycles at most
+  pth_status Solve(const GmgParams &P, double tol, int maxit, int *iters, double *rel,
+                   int min_cycles = 0) {
+    history.clear();
+    std::vector<GmgProblem<T> *> ps = {&p};
+    std::vector<char> corr = {0};
+    int it = 0;
+    for (;; ++it) {
+      double rn;
+      if (batch.Residual(ps, corr, &rn) != PTH_COMPLETE) return PTH_FAIL;
+      const double r = bnorm > 0 ? rn / bnorm : rn;
+      history.push_back(r);
+      if ((r < tol && it >= min_cycles) || it >= maxit) break;
+      batch.jacobi_columns = P.columns;
+      for (int c = 0; c < P.inner; ++c)
+        if (batch.Cycle(ps, P.omega, P.pre, P.post, c > 0) != PTH_COMPLETE) return PTH_FAIL;
+      corr[0] = 1;
+    }
+    if (iters) *iters = it;
+    if (rel) *rel = history.back();
+    return PTH_COMPLETE;
+  }
+};
+
 void GmgBlobs(std::uint64_t seed, double out[80]);
 std::vector<int> GmgLevelSizes(int N);
 

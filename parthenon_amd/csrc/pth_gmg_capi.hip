@@ -26,59 +26,11 @@
 namespace pth {
 namespace {
 
-struct Params {
-  int precision;  // 0: FP32 inner cycle (mixed), 1: FP64
-  double omega;
-  int pre, post;
-  int inner = 1;    // V-cycles per FP64 refinement step
-  int columns = 1;  // Jacobi sweeps by z columns where the level size allows
-};
-
-template <typename T>
-struct Solver1 {
-  GmgProblem<T> p;
-  GmgBatch<T> batch;
-  hipStream_t s = nullptr;
-  double bnorm = 0.0;
-  std::vector<double> history;
-  ~Solver1() {
-    p.Free();
-    batch.Free();
-    if (s) (void)hipStreamDestroy(s);
-  }
-  pth_status Init(int N) {
-    PTH_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    if (batch.Init(s) != PTH_COMPLETE) return PTH_FAIL;
-    return p.Allocate(N, GmgBottom<T>::kN);
-  }
-  pth_status Norm() { return batch.RhsNorms({&p}, &bnorm); }
-  pth_status Solve(const Params &P, double tol, int maxit, int *iters, double *rel) {
-    history.clear();
-    std::vector<GmgProblem<T> *> ps = {&p};
-    std::vector<char> corr = {0};
-    int it = 0;
-    for (;; ++it) {
-      double rn;
-      if (batch.Residual(ps, corr, &rn) != PTH_COMPLETE) return PTH_FAIL;
-      const double r = bnorm > 0 ? rn / bnorm : rn;
-      history.push_back(r);
-      if (r < tol || it >= maxit) break;
-      batch.jacobi_columns = P.columns;
-      for (int c = 0; c < P.inner; ++c)
-        if (batch.Cycle(ps, P.omega, P.pre, P.post, c > 0) != PTH_COMPLETE) return PTH_FAIL;
-      corr[0] = 1;
-    }
-    if (iters) *iters = it;
-    if (rel) *rel = history.back();
-    return PTH_COMPLETE;
-  }
-};
-
 struct GmgHandle {
-  Params P;
+  GmgParams P;
   int N = 0;
-  std::unique_ptr<Solver1<float>> f;
-  std::unique_ptr<Solver1<double>> d;
+  std::unique_ptr<GmgSolver1<float>> f;
+  std::unique_ptr<GmgSolver1<double>> d;
   template <typename F>
   pth_status Do(F fn) {
     return f ? fn(*f) : fn(*d);
@@ -99,7 +51,7 @@ struct Request {
 
 template <typename T>
 struct Server {
-  Params P;
+  GmgParams P;
   double tol;
   int maxit, max_active;
   std::int64_t budget;           // cap on bytes held (<= 0: none beyond free HBM)
@@ -290,10 +242,10 @@ pth_gmg *pth_gmg_create(int n, int precision, double omega, int pre, int post) {
   h->N = n;
   pth_status st;
   if (precision == 0) {
-    h->f.reset(new Solver1<float>());
+    h->f.reset(new GmgSolver1<float>());
     st = h->f->Init(n);
   } else {
-    h->d.reset(new Solver1<double>());
+    h->d.reset(new GmgSolver1<double>());
     st = h->d->Init(n);
   }
   if (st != PTH_COMPLETE) {

@@ -130,3 +130,97 @@ class Server:
         busy = C.c_double()
         _check(lib().pth_gmg_server_stats(self._h, out, C.byref(busy)), "stats")
         return dict(iterations=out[0], max_batch=out[1], bytes_held=out[2], busy_s=busy.value)
+
+
+# ---- solver utilities, the Poisson equation and MGSolver over MeshData ----------------------
+def copy_data(md, src, dst):
+    """solvers::utils::CopyData<src, dst> (src/solvers/solver_utils.hpp:151-178)."""
+    _check(lib().pth_solver_copy_data(md._h, src.encode(), dst.encode()), "CopyData")
+
+
+def add_fields_and_store(md, a, b, out, wa=1.0, wb=1.0):
+    """solvers::utils::AddFieldsAndStore<a, b, out> (solver_utils.hpp:180-222)."""
+    _check(lib().pth_solver_add_fields_and_store(md._h, a.encode(), b.encode(), out.encode(),
+                                                 wa, wb), "AddFieldsAndStore")
+
+
+def set_to_zero(md, var):
+    """solvers::utils::SetToZero<var> (solver_utils.hpp:224-250): interior cells."""
+    _check(lib().pth_solver_set_to_zero(md._h, var.encode()), "SetToZero")
+
+
+def dot_product(md, a, b):
+    """solvers::utils::DotProduct<a, b> (solver_utils.hpp:252-301) on one GPU."""
+    out = C.c_double()
+    _check(lib().pth_solver_dot_product(md._h, a.encode(), b.encode(), C.byref(out)),
+           "DotProduct")
+    return out.value
+
+
+@dataclass
+class PoissonEquation:  # example/poisson_gmg/poisson_equation.hpp:41-205, constant D
+    D: float = 1.0
+    diagonal_alpha: float = 0.0
+
+    def _c(self):
+        from . import PoissonEquationC
+        return PoissonEquationC(self.D, self.diagonal_alpha)
+
+    def Ax(self, md, x, out):
+        """PoissonEquation::Ax<x, out> (:41-53): needs x's ghosts filled."""
+        e = self._c()
+        _check(lib().pth_poisson_ax(md._h, C.byref(e), x.encode(), out.encode()), "Ax")
+
+
+@dataclass
+class MeshMGParams:  # src/solvers/mg_solver.hpp:37-58, with the smoother explicit
+    max_iters: int = 1000
+    residual_tolerance: float = 1e-12   # RMS residual sqrt(sum r^2 / ncells), as the reference
+    precision: str = "mixed"
+    inner_cycles: int = 2               # FP32 V-cycles per FP64 refinement step (mixed)
+    omega: float = 2.0 / 3.0
+    pre_sweeps: int = 2
+    post_sweeps: int = 2
+
+
+class MeshMGSolver:
+    """MGSolver<u, rhs, PoissonEquation> over the blocks of a MeshData (mg_solver.hpp:37-203):
+    setup() is AddSetupTasks, solve() is AddTasks run to completion. The mesh is uniform,
+    3-D, N^3 cells on a cube, fixed_face on every face."""
+
+    def __init__(self, mesh, u, rhs, params=None, equation=None):
+        from . import MgParamsC
+        self.params = params or MeshMGParams()
+        self.equation = equation or PoissonEquation()
+        p = self.params
+        cp = MgParamsC(p.max_iters, p.residual_tolerance, _PREC[p.precision], p.inner_cycles,
+                       p.omega, p.pre_sweeps, p.post_sweeps)
+        e = self.equation._c()
+        h = lib().pth_mg_create(mesh._h, u.encode(), rhs.encode(), C.byref(cp), C.byref(e))
+        if not h:
+            raise RuntimeError("pth_mg_create: " + lib().pth_last_error().decode())
+        self._h = h
+        self._mesh = mesh  # the solver refers to the mesh: keep it alive
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().pth_mg_destroy(self._h)
+            self._h = None
+
+    def setup(self, md):
+        _check(lib().pth_mg_setup(self._h, md._h), "MGSolver::AddSetupTasks")
+
+    def solve(self, md):
+        """Returns (V-cycles, final RMS residual)."""
+        _check(lib().pth_mg_solve(self._h, md._h), "MGSolver::AddTasks")
+        return self.result()
+
+    def result(self):
+        it, rms = C.c_int(), C.c_double()
+        _check(lib().pth_mg_result(self._h, C.byref(it), C.byref(rms)), "MGSolver result")
+        return it.value, rms.value
+
+    def history(self):
+        buf = (C.c_double * 1024)()
+        n = lib().pth_mg_history(self._h, buf, 1024)
+        return np.array(buf[:min(n, 1024)])

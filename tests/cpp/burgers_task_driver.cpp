@@ -1,6 +1,6 @@
 // A C++ driver written against the reference's task-level names, run through
 // include/parthenon_amd.hpp: the task order of BurgersDriver::MakeTaskCollection
-// (benchmarks/burgers/burgers_driver.cpp:53-148) with a sequential task list, the RK2
+// (benchmarks/burgers/burgers_driver.cpp:53-148) on the header's TaskList, the RK2
 // integrator (low_storage_integrator.cpp:67-83) and the driver's dt rule (driver.cpp:210-235).
 //
 // usage: burgers_task_driver <ncycles> <out.bin> [sphere_radius] [open|periodic] [adaptive]
@@ -27,20 +27,6 @@
 using namespace parthenon;
 
 namespace {
-
-// Minimal sequential stand-in for the reference's TaskList: each task is re-polled while it
-// returns incomplete, like Task::operator() re-enqueueing itself (src/tasks/tasks.hpp:93-118).
-struct TaskList {
-  std::vector<std::function<TaskStatus()>> tasks;
-  void AddTask(std::function<TaskStatus()> f) { tasks.push_back(std::move(f)); }
-  void Execute() {
-    for (auto &t : tasks) {
-      int polls = 0;
-      while (t() == TaskStatus::incomplete)
-        if (++polls > 1000) throw std::runtime_error("task never completed");
-    }
-  }
-};
 
 struct Integrator {
   int nstages = 2;
