@@ -16,6 +16,7 @@
 //   pack / unpack loops     src/bvals/comms/boundary_communication.cpp:95-139, 273-334
 #include <hip/hip_runtime.h>
 
+#include "pth_device.hpp"
 #include "pth_internal.hpp"
 
 namespace pth {
@@ -24,6 +25,10 @@ namespace {
 
 constexpr int kWG = 256;  // threads per workgroup (4 wavefronts of 64)
 constexpr int kPer = kItemElems / kWG;  // elements per lane per item (16 loads in flight)
+
+// The region tables hold generic pointers; every array they address is device global
+// memory, accessed through Glob (global loads and stores).
+typedef __attribute__((address_space(1))) double gdouble;
 
 struct Cell {
   int i, j, k, c;
@@ -48,6 +53,16 @@ __device__ __forceinline__ std::int64_t VarOffset(const Region &R, const Cell &d
          static_cast<std::int64_t>(d.j + R.lo[1]) * R.astride[0] + (d.i + R.lo[0]);
 }
 
+// Element of lane slot u within an item. Lanes past the item's end take its last element:
+// they load and store the same value at the same address as the lane that owns it, so every
+// load and store is unconditional. (gfx9 counts loads and stores on one in-order counter; a
+// store under a branch makes the compiler wait for every earlier access before it, which
+// serializes the stores.)
+__device__ __forceinline__ int Lane(const Item &I, int u) {
+  const int local = u * kWG + static_cast<int>(threadIdx.x);
+  return local < I.count ? local : I.count - 1;
+}
+
 // Pack (TO_BUFFER) or unpack: one launch over the whole table, one workgroup per work item.
 // The item and its region record are uniform (scalar loads); each lane issues its kPer loads
 // before its kPer stores, so a workgroup keeps 32 KiB in flight.
@@ -57,25 +72,19 @@ __global__ void __launch_bounds__(kWG)
   const Item I = items[blockIdx.x];
   const Region &R = reg[I.region];
   double v[kPer];
-  int off[kPer];
+  int off[kPer], e[kPer];
 #pragma unroll
   for (int u = 0; u < kPer; ++u) {
-    const int local = u * kWG + static_cast<int>(threadIdx.x);
-    if (local < I.count) {
-      const int e = I.begin + local;
-      off[u] = static_cast<int>(VarOffset(R, Decode(R, static_cast<std::uint32_t>(e))));
-      v[u] = TO_BUFFER ? R.a[off[u]] : R.b[e];
-    }
+    e[u] = I.begin + Lane(I, u);
+    off[u] = static_cast<int>(VarOffset(R, Decode(R, static_cast<std::uint32_t>(e[u]))));
+    v[u] = TO_BUFFER ? Glob(R.a)[off[u]] : Glob(R.b)[e[u]];
   }
 #pragma unroll
   for (int u = 0; u < kPer; ++u) {
-    const int local = u * kWG + static_cast<int>(threadIdx.x);
-    if (local < I.count) {
-      if (TO_BUFFER)
-        R.b[I.begin + local] = v[u];
-      else
-        R.a[off[u]] = v[u];
-    }
+    if (TO_BUFFER)
+      Glob(R.b)[e[u]] = v[u];
+    else
+      Glob(R.a)[off[u]] = v[u];
   }
 }
 
@@ -89,20 +98,14 @@ __global__ void __launch_bounds__(kWG)
   std::int64_t off[kPer];
 #pragma unroll
   for (int u = 0; u < kPer; ++u) {
-    const int local = u * kWG + static_cast<int>(threadIdx.x);
-    if (local < I.count) {
-      const Cell d = Decode(R, static_cast<std::uint32_t>(I.begin + local));
-      v[u] = R.a[VarOffset(R, d)];
-      off[u] = static_cast<std::int64_t>(d.c) * R.bstride[2] +
-               static_cast<std::int64_t>(d.k) * R.bstride[1] +
-               static_cast<std::int64_t>(d.j) * R.bstride[0] + d.i;
-    }
+    const Cell d = Decode(R, static_cast<std::uint32_t>(I.begin + Lane(I, u)));
+    v[u] = Glob(R.a)[VarOffset(R, d)];
+    off[u] = static_cast<std::int64_t>(d.c) * R.bstride[2] +
+             static_cast<std::int64_t>(d.k) * R.bstride[1] +
+             static_cast<std::int64_t>(d.j) * R.bstride[0] + d.i;
   }
 #pragma unroll
-  for (int u = 0; u < kPer; ++u) {
-    const int local = u * kWG + static_cast<int>(threadIdx.x);
-    if (local < I.count) R.b[off[u]] = v[u];
-  }
+  for (int u = 0; u < kPer; ++u) Glob(R.b)[off[u]] = v[u];
 }
 
 // Fused exchange by region pairs: send region q and receive region q cover the same slice of
@@ -119,17 +122,82 @@ __global__ void __launch_bounds__(kWG)
   std::int64_t off[kPer];
 #pragma unroll
   for (int u = 0; u < kPer; ++u) {
-    const int local = u * kWG + static_cast<int>(threadIdx.x);
-    if (local < I.count) {
-      const std::uint32_t e = static_cast<std::uint32_t>(I.begin + local);
-      v[u] = S.a[VarOffset(S, Decode(S, e))];
-      off[u] = VarOffset(D, Decode(D, e));
-    }
+    const std::uint32_t e = static_cast<std::uint32_t>(I.begin + Lane(I, u));
+    v[u] = Glob(S.a)[VarOffset(S, Decode(S, e))];
+    off[u] = VarOffset(D, Decode(D, e));
   }
 #pragma unroll
-  for (int u = 0; u < kPer; ++u) {
-    const int local = u * kWG + static_cast<int>(threadIdx.x);
-    if (local < I.count) D.a[off[u]] = v[u];
+  for (int u = 0; u < kPer; ++u) Glob(D.a)[off[u]] = v[u];
+}
+
+// Same-level ghost shells: one workgroup per (field brick q, receiving block b) fills every
+// ghost cell of the brick whose neighbour across it is a block of the same level
+// (same[27 b + dir] >= 0) from that neighbour's interior: cell (i, j, k) in the ghosts of
+// direction (ox, oy, oz) is cell (i - ox N, j - oy N, k - oz N) of the neighbour's brick. The
+// ghost cells are numbered in brick order, skipping the interior, so a wavefront writes
+// consecutive addresses: whole planes below and above the interior, whole rows in the y ghost
+// bands and, between them, the x ghost runs of consecutive rows (the high run of one row and
+// the low run of the next are adjacent). Each lane issues its U loads before its U stores.
+// Cells whose neighbour is at another level or across a physical face keep their value here
+// and are filled by the cross-level channels and the boundary conditions after this launch.
+template <int N, int G>
+__global__ void __launch_bounds__(kWG) ShellKernel(const ShellArgs A) {
+  constexpr int E = N + 2 * G, P = E * E, PI = P - N * N, S = 2 * G * P + N * PI;
+  // loads in flight per lane: two passes over the shell where that takes at most 16
+  constexpr int U = (S + 2 * kWG - 1) / (2 * kWG) < 16 ? (S + 2 * kWG - 1) / (2 * kWG) : 16;
+  __shared__ int nbr[27];
+  const unsigned L = XcdRemap(blockIdx.x, gridDim.x);  // = q * nblocks + b
+  const int b = static_cast<int>(L % static_cast<unsigned>(A.nblocks));
+  const int q = static_cast<int>(L / static_cast<unsigned>(A.nblocks));
+  if (threadIdx.x < 27) nbr[threadIdx.x] = A.same[27 * b + threadIdx.x];
+  __syncthreads();
+  const double *src0 = A.brick[q];
+  const std::int64_t bs = A.stride[q];
+  double *dst = A.brick[q] + b * bs;
+  for (int e0 = static_cast<int>(threadIdx.x); e0 < S; e0 += U * kWG) {
+    double v[U];
+    int off[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = e0 + u * kWG < S ? e0 + u * kWG : S - 1;  // past the end: repeat the last
+      {
+        int i, j, k;
+        if (e < G * P || e >= G * P + N * PI) {  // whole ghost planes
+          const int t = e < G * P ? e : e - N * PI;
+          k = t / P;
+          k += k >= G ? N : 0;
+          const int r = t % P;
+          j = r / E;
+          i = r % E;
+        } else {
+          const int t = e - G * P;
+          k = G + t / PI;
+          const int r = t % PI;
+          if (r < G * E || r >= G * E + 2 * G * N) {  // whole ghost rows
+            const int rr = r < G * E ? r : r - 2 * G * N;
+            j = rr / E;
+            j += j >= G ? N : 0;
+            i = rr % E;
+          } else {  // the x ghost runs of an interior row
+            const int rr = r - G * E;
+            j = G + rr / (2 * G);
+            const int x = rr % (2 * G);
+            i = x < G ? x : x + N;
+          }
+        }
+        const int ox = (i >= G) + (i >= G + N) - 1, oy = (j >= G) + (j >= G + N) - 1,
+                  oz = (k >= G) + (k >= G + N) - 1;
+        const int sb = nbr[(ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)];
+        off[u] = (k * E + j) * E + i;
+        // a cell without a same-level neighbour keeps its value (it is rewritten here and
+        // filled later by the cross-level channels or the boundary conditions), so every
+        // load and store is unconditional
+        v[u] = sb >= 0 ? src0[sb * bs + off[u] - (ox * N + oy * N * E + oz * N * P)]
+                       : dst[off[u]];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) dst[off[u]] = v[u];
   }
 }
 
@@ -152,15 +220,53 @@ struct ProResParams {
 };
 
 // Cell-average restriction of one coarse cell from its 2x2x2 fine children; the summation
-// order is the reference's fixed pairwise order.
-__device__ __forceinline__ void RestrictOne(const Region &R, const Cell &d,
-                                            const DeviceBlockGeom &G, const ProResParams &p) {
+// order is the reference's fixed pairwise order. Split into the loads (RestrictLoad: the
+// eight children, an inactive dimension repeating its first child so every load is in
+// bounds and unconditional) and the arithmetic and store (RestrictStore), so a lane can issue
+// the loads of several cells before any store.
+struct RestrictIn {
+  double x[8];
+  std::int64_t out;
+  int c;
+};
+
+// active dimension q: every dimension when ALL3 (known at compile time), else the table's
+template <bool ALL3>
+__device__ __forceinline__ int Act(const ProResParams &p, int q) {
+  return ALL3 ? 1 : p.act[q];
+}
+
+template <bool ALL3>
+__device__ __forceinline__ RestrictIn RestrictLoad(const Region &R, const Cell &d,
+                                                   const ProResParams &p) {
   const int ci = d.i + R.lo[0], cj = d.j + R.lo[1], ck = d.k + R.lo[2];
-  const int fi = p.act[0] ? (ci - p.clo[0]) * 2 + p.flo[0] : p.flo[0];
-  const int fj = p.act[1] ? (cj - p.clo[1]) * 2 + p.flo[1] : p.flo[1];
-  const int fk = p.act[2] ? (ck - p.clo[2]) * 2 + p.flo[2] : p.flo[2];
+  const int a0 = Act<ALL3>(p, 0), a1 = Act<ALL3>(p, 1), a2 = Act<ALL3>(p, 2);
+  const int fi = a0 ? (ci - p.clo[0]) * 2 + p.flo[0] : p.flo[0];
+  const int fj = a1 ? (cj - p.clo[1]) * 2 + p.flo[1] : p.flo[1];
+  const int fk = a2 ? (ck - p.clo[2]) * 2 + p.flo[2] : p.flo[2];
+  const gdouble *f = Glob(R.a) + static_cast<std::int64_t>(d.c) * R.astride[2];
+  RestrictIn in;
+#pragma unroll
+  for (int ok = 0; ok < 2; ++ok)
+#pragma unroll
+    for (int oj = 0; oj < 2; ++oj)
+#pragma unroll
+      for (int oi = 0; oi < 2; ++oi)
+        in.x[(ok * 2 + oj) * 2 + oi] =
+            f[static_cast<std::int64_t>(fk + ok * a2) * R.astride[1] +
+              static_cast<std::int64_t>(fj + oj * a1) * R.astride[0] + (fi + oi * a0)];
+  in.out = static_cast<std::int64_t>(d.c) * R.bstride[2] +
+           static_cast<std::int64_t>(ck) * R.bstride[1] +
+           static_cast<std::int64_t>(cj) * R.bstride[0] + ci;
+  in.c = d.c;
+  return in;
+}
+
+template <bool ALL3>
+__device__ __forceinline__ void RestrictStore(const Region &R, const RestrictIn &in,
+                                              const DeviceBlockGeom &G,
+                                              const ProResParams &p) {
   const double v = G.dx[0] * G.dx[1] * G.dx[2];
-  const double *f = R.a + static_cast<std::int64_t>(d.c) * R.astride[2];
   double vol[2][2][2], t[2][2][2];
 #pragma unroll
   for (int ok = 0; ok < 2; ++ok)
@@ -168,53 +274,77 @@ __device__ __forceinline__ void RestrictOne(const Region &R, const Cell &d,
     for (int oj = 0; oj < 2; ++oj)
 #pragma unroll
       for (int oi = 0; oi < 2; ++oi) {
-        const bool use = (ok == 0 || p.act[2]) && (oj == 0 || p.act[1]) && (oi == 0 || p.act[0]);
+        const bool use = (ok == 0 || Act<ALL3>(p, 2)) && (oj == 0 || Act<ALL3>(p, 1)) &&
+                         (oi == 0 || Act<ALL3>(p, 0));
         vol[ok][oj][oi] = use ? v : 0.0;
-        t[ok][oj][oi] = use ? v * f[static_cast<std::int64_t>(fk + ok) * R.astride[1] +
-                                    static_cast<std::int64_t>(fj + oj) * R.astride[0] + (fi + oi)]
-                            : 0.0;
+        t[ok][oj][oi] = use ? v * in.x[(ok * 2 + oj) * 2 + oi] : 0.0;
       }
   const double tvol = ((vol[0][0][0] + vol[0][1][0]) + (vol[0][0][1] + vol[0][1][1])) +
                       ((vol[1][0][0] + vol[1][1][0]) + (vol[1][0][1] + vol[1][1][1]));
-  R.b[static_cast<std::int64_t>(d.c) * R.bstride[2] + static_cast<std::int64_t>(ck) * R.bstride[1] +
-      static_cast<std::int64_t>(cj) * R.bstride[0] + ci] =
-      (((t[0][0][0] + t[0][1][0]) + (t[0][0][1] + t[0][1][1])) +
-       ((t[1][0][0] + t[1][1][0]) + (t[1][0][1] + t[1][1][1]))) /
-      tvol;
+  Glob(R.b)[in.out] = (((t[0][0][0] + t[0][1][0]) + (t[0][0][1] + t[0][1][1])) +
+                       ((t[1][0][0] + t[1][1][0]) + (t[1][0][1] + t[1][1][1]))) /
+                      tvol;
 }
 
 __device__ __forceinline__ double Sgn(double x) { return x < 0 ? -1.0 : 1.0; }
 __device__ __forceinline__ double Min(double a, double b) { return b < a ? b : a; }
 
 // Shared prolongation of one coarse cell into its 2x2x2 fine cells; minmod selects the
-// limited slope, otherwise the one-sided slopes (ProlongateSharedLinear).
-__device__ __forceinline__ void ProlongOne(const Region &R, const Cell &d,
-                                           const DeviceBlockGeom &G, const ProResParams &p,
-                                           bool minmod) {
-  const int ci[3] = {d.i + R.lo[0], d.j + R.lo[1], d.k + R.lo[2]};
+// limited slope, otherwise the one-sided slopes (ProlongateSharedLinear). ProlongLoad reads
+// the cell and its two neighbours per dimension (an inactive dimension reads the cell
+// itself), ProlongStore does the arithmetic and the stores.
+struct ProlongIn {
+  double fc, lo[3], hi[3];
+  int ci[3], c;
+};
+
+template <bool ALL3>
+__device__ __forceinline__ ProlongIn ProlongLoad(const Region &R, const Cell &d,
+                                                 const ProResParams &p) {
+  ProlongIn in;
+  in.ci[0] = d.i + R.lo[0];
+  in.ci[1] = d.j + R.lo[1];
+  in.ci[2] = d.k + R.lo[2];
+  in.c = d.c;
+  const gdouble *cz = Glob(R.b) + static_cast<std::int64_t>(d.c) * R.bstride[2];
+  const std::int64_t cstr[3] = {Act<ALL3>(p, 0),
+                                Act<ALL3>(p, 1) * static_cast<std::int64_t>(R.bstride[0]),
+                                Act<ALL3>(p, 2) * static_cast<std::int64_t>(R.bstride[1])};
+  const std::int64_t cc = static_cast<std::int64_t>(in.ci[2]) * R.bstride[1] +
+                          static_cast<std::int64_t>(in.ci[1]) * R.bstride[0] + in.ci[0];
+  in.fc = cz[cc];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    in.lo[q] = cz[cc - cstr[q]];
+    in.hi[q] = cz[cc + cstr[q]];
+  }
+  return in;
+}
+
+template <bool ALL3>
+__device__ __forceinline__ void ProlongStore(const Region &R, const ProlongIn &in,
+                                             const DeviceBlockGeom &G, const ProResParams &p,
+                                             bool minmod) {
   int fi[3];
 #pragma unroll
-  for (int q = 0; q < 3; ++q) fi[q] = p.act[q] ? (ci[q] - p.clo[q]) * 2 + p.flo[q] : p.flo[q];
-  const double *cz = R.b + static_cast<std::int64_t>(d.c) * R.bstride[2];
-  double *fz = R.a + static_cast<std::int64_t>(d.c) * R.astride[2];
-  const std::int64_t cstr[3] = {1, R.bstride[0], R.bstride[1]};
-  const std::int64_t cc = static_cast<std::int64_t>(ci[2]) * R.bstride[1] +
-                          static_cast<std::int64_t>(ci[1]) * R.bstride[0] + ci[0];
-  const double fc = cz[cc];
+  for (int q = 0; q < 3; ++q)
+    fi[q] = Act<ALL3>(p, q) ? (in.ci[q] - p.clo[q]) * 2 + p.flo[q] : p.flo[q];
+  gdouble *fz = Glob(R.a) + static_cast<std::int64_t>(in.c) * R.astride[2];
+  const double fc = in.fc;
   double gm[3] = {0, 0, 0}, gp[3] = {0, 0, 0}, dfm[3] = {0, 0, 0}, dfp[3] = {0, 0, 0};
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    if (!p.act[q]) continue;
-    const double xm = G.cx0[q] + (ci[q] - 1 + 0.5) * G.cdx[q];
-    const double xc = G.cx0[q] + (ci[q] + 0.5) * G.cdx[q];
-    const double xp = G.cx0[q] + (ci[q] + 1 + 0.5) * G.cdx[q];
+    if (!Act<ALL3>(p, q)) continue;
+    const double xm = G.cx0[q] + (in.ci[q] - 1 + 0.5) * G.cdx[q];
+    const double xc = G.cx0[q] + (in.ci[q] + 0.5) * G.cdx[q];
+    const double xp = G.cx0[q] + (in.ci[q] + 1 + 0.5) * G.cdx[q];
     const double dxm = xc - xm, dxp = xp - xc;
     const double fxm = G.x0[q] + (fi[q] + 0.5) * G.dx[q];
     const double fxp = G.x0[q] + (fi[q] + 1 + 0.5) * G.dx[q];
     dfm[q] = xc - fxm;
     dfp[q] = fxp - xc;
-    const double gxm = (fc - cz[cc - cstr[q]]) / dxm;
-    const double gxp = (cz[cc + cstr[q]] - fc) / dxp;
+    const double gxm = (fc - in.lo[q]) / dxm;
+    const double gxp = (in.hi[q] - fc) / dxp;
     if (minmod) {
       const double gc = 0.5 * (Sgn(gxm) + Sgn(gxp)) * Min(fabs(gxm), fabs(gxp));
       gm[q] = gc;
@@ -227,7 +357,7 @@ __device__ __forceinline__ void ProlongOne(const Region &R, const Cell &d,
   const std::int64_t f0 = static_cast<std::int64_t>(fi[2]) * R.astride[1] +
                           static_cast<std::int64_t>(fi[1]) * R.astride[0] + fi[0];
   const std::int64_t sj = R.astride[0], sk = R.astride[1];
-  const bool X1 = p.act[0], X2 = p.act[1], X3 = p.act[2];
+  const bool X1 = Act<ALL3>(p, 0), X2 = Act<ALL3>(p, 1), X3 = Act<ALL3>(p, 2);
   fz[f0] = fc - (gm[0] * dfm[0] + gm[1] * dfm[1] + gm[2] * dfm[2]);
   if (X1) fz[f0 + 1] = fc + (gp[0] * dfp[0] - gm[1] * dfm[1] - gm[2] * dfm[2]);
   if (X2) fz[f0 + sj] = fc - (gm[0] * dfm[0] - gp[1] * dfp[1] + gm[2] * dfm[2]);
@@ -239,20 +369,36 @@ __device__ __forceinline__ void ProlongOne(const Region &R, const Cell &d,
     fz[f0 + sk + sj + 1] = fc + (gp[0] * dfp[0] + gp[1] * dfp[1] + gp[2] * dfp[2]);
 }
 
-// OP 0: restriction, 1: minmod prolongation, 2: linear prolongation
-template <int OP>
+// OP 0: restriction, 1: minmod prolongation, 2: linear prolongation; ALL3: every dimension
+// active (3-D meshes). A lane takes kProRes cells of the item at a time: their loads first,
+// then their arithmetic and stores (lanes past the item's end repeat its last cell, which
+// writes the same value to the same place).
+constexpr int kProRes = 4;
+template <int OP, bool ALL3>
 __global__ void __launch_bounds__(kWG)
     ProResKernel(const Region *__restrict__ reg, const Item *__restrict__ items, int nitems,
                  const DeviceBlockGeom *__restrict__ geom, ProResParams p) {
   const Item I = items[blockIdx.x];
   const Region &R = reg[I.region];
   const DeviceBlockGeom &G = geom[R.gid];
-  for (int local = static_cast<int>(threadIdx.x); local < I.count; local += kWG) {
-    const Cell d = Decode(R, static_cast<std::uint32_t>(I.begin + local));
-    if (OP == 0)
-      RestrictOne(R, d, G, p);
-    else
-      ProlongOne(R, d, G, p, OP == 1);
+  for (int u0 = 0; u0 * kWG < I.count; u0 += kProRes) {
+    if (OP == 0) {
+      RestrictIn in[kProRes];
+#pragma unroll
+      for (int u = 0; u < kProRes; ++u)
+        in[u] = RestrictLoad<ALL3>(
+            R, Decode(R, static_cast<std::uint32_t>(I.begin + Lane(I, u0 + u))), p);
+#pragma unroll
+      for (int u = 0; u < kProRes; ++u) RestrictStore<ALL3>(R, in[u], G, p);
+    } else {
+      ProlongIn in[kProRes];
+#pragma unroll
+      for (int u = 0; u < kProRes; ++u)
+        in[u] = ProlongLoad<ALL3>(
+            R, Decode(R, static_cast<std::uint32_t>(I.begin + Lane(I, u0 + u))), p);
+#pragma unroll
+      for (int u = 0; u < kProRes; ++u) ProlongStore<ALL3>(R, in[u], G, p, OP == 1);
+    }
   }
 }
 
@@ -397,6 +543,46 @@ pth_status LaunchExchangeDirect(const Table &pack, const Table &unpack, hipStrea
   return PTH_COMPLETE;
 }
 
+bool ShellSupported(const Topology &t) {
+  const Extent &x = t.fine;
+  if (t.ndim != 3 || x.n[0] != x.n[1] || x.n[0] != x.n[2] || x.g[0] != x.g[1] ||
+      x.g[0] != x.g[2])
+    return false;
+  const int n = x.n[0], g = x.g[0];
+  return (n == 8 || n == 16 || n == 32) && g >= 2 && g <= 4;
+}
+
+pth_status LaunchShell(const Topology &t, const std::vector<ShellBrick> &bricks,
+                       const int *same, hipStream_t s) {
+  const int nb = static_cast<int>(t.blocks.size());
+  if (bricks.empty() || nb == 0) return PTH_COMPLETE;
+  if (!ShellSupported(t) || !same) {
+    SetError("internal: ghost-shell exchange needs cubic 8/16/32 blocks and 2-4 ghosts");
+    return PTH_FAIL;
+  }
+  ProfScope prof("exchange_shell", s);
+  const int n = t.fine.n[0], g = t.fine.g[0];
+  for (size_t q0 = 0; q0 < bricks.size(); q0 += kShellBricks) {
+    ShellArgs A{};
+    const int nq = static_cast<int>(std::min<size_t>(kShellBricks, bricks.size() - q0));
+    for (int q = 0; q < nq; ++q) {
+      A.brick[q] = bricks[q0 + q].base;
+      A.stride[q] = bricks[q0 + q].stride;
+    }
+    A.same = same;
+    A.nblocks = nb;
+    const dim3 grid(static_cast<unsigned>(nq) * nb);
+#define PTH_SHELL(NN, GG)                                                              \
+  if (n == NN && g == GG) hipLaunchKernelGGL((ShellKernel<NN, GG>), grid, dim3(kWG), 0, s, A);
+    PTH_SHELL(8, 2) PTH_SHELL(8, 3) PTH_SHELL(8, 4)
+    PTH_SHELL(16, 2) PTH_SHELL(16, 3) PTH_SHELL(16, 4)
+    PTH_SHELL(32, 2) PTH_SHELL(32, 3) PTH_SHELL(32, 4)
+#undef PTH_SHELL
+    PTH_HIP(hipGetLastError());
+  }
+  return PTH_COMPLETE;
+}
+
 static ProResParams Params(const Topology &topo) {
   ProResParams p;
   for (int d = 0; d < 3; ++d) {
@@ -407,13 +593,24 @@ static ProResParams Params(const Topology &topo) {
   return p;
 }
 
+template <int OP>
+static void LaunchProRes(const Table &t, const Topology &topo, const DeviceBlockGeom *geom,
+                         hipStream_t s) {
+  const int n = static_cast<int>(t.items.size());
+  const ProResParams P = Params(topo);
+  if (P.act[0] && P.act[1] && P.act[2])
+    hipLaunchKernelGGL((ProResKernel<OP, true>), dim3(GridFor(n)), dim3(kWG), 0, s, t.dev,
+                       t.dev_items, n, geom, P);
+  else
+    hipLaunchKernelGGL((ProResKernel<OP, false>), dim3(GridFor(n)), dim3(kWG), 0, s, t.dev,
+                       t.dev_items, n, geom, P);
+}
+
 pth_status LaunchRestrictT(const Table &t, const Topology &topo, const DeviceBlockGeom *geom,
                            hipStream_t s) {
   if (t.total == 0) return PTH_COMPLETE;
   ProfScope prof("restrict", s);
-  const int n = static_cast<int>(t.items.size());
-  hipLaunchKernelGGL(ProResKernel<0>, dim3(GridFor(n)), dim3(kWG), 0, s, t.dev, t.dev_items, n,
-                     geom, Params(topo));
+  LaunchProRes<0>(t, topo, geom, s);
   PTH_HIP(hipGetLastError());
   return PTH_COMPLETE;
 }
@@ -422,13 +619,10 @@ pth_status LaunchProlongT(const Table &t, const Topology &topo, const DeviceBloc
                           hipStream_t s, bool minmod) {
   if (t.total == 0) return PTH_COMPLETE;
   ProfScope prof("prolong", s);
-  const int n = static_cast<int>(t.items.size());
   if (minmod)
-    hipLaunchKernelGGL(ProResKernel<1>, dim3(GridFor(n)), dim3(kWG), 0, s, t.dev, t.dev_items, n,
-                       geom, Params(topo));
+    LaunchProRes<1>(t, topo, geom, s);
   else
-    hipLaunchKernelGGL(ProResKernel<2>, dim3(GridFor(n)), dim3(kWG), 0, s, t.dev, t.dev_items, n,
-                       geom, Params(topo));
+    LaunchProRes<2>(t, topo, geom, s);
   PTH_HIP(hipGetLastError());
   return PTH_COMPLETE;
 }

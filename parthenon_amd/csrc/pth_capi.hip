@@ -77,6 +77,8 @@ MeshData::~MeshData() {
     c.bc[1].Free();
     c.pack_rest.Free();
     c.unpack_rest.Free();
+    c.pack_cross.Free();
+    c.unpack_cross.Free();
   }
   for (auto &kv : packs) kv.second.Free();
 }
@@ -550,6 +552,61 @@ pth_status Mesh::BuildRestExchange(MeshData &md, int f, bool *ok) {
     return PTH_FAIL;
   }
   c.rest_field = f;
+  *ok = true;
+  return PTH_COMPLETE;
+}
+
+// The direct exchange split for the ghost-shell kernel: the same-level channels of every
+// field go to ShellKernel (one entry per field component), the other region pairs (the
+// channels between levels, through coarse buffers) stay in pack_cross / unpack_cross. ok is
+// false where the shell kernel does not apply (block shape) or the tables have no direct form.
+pth_status Mesh::BuildCrossExchange(MeshData &md, bool *ok) {
+  *ok = false;
+  if (!ShellSupported(topo)) return PTH_COMPLETE;
+  if (BuildCache(md, PTH_BT_ANY) != PTH_COMPLETE) return PTH_FAIL;
+  BoundCache &c = md.cache[PTH_BT_ANY];
+  if (!CanExchangeDirect(c.pack, c.unpack) || c.pack.host.size() != c.unpack.host.size())
+    return PTH_COMPLETE;
+  if (c.cross_built) {
+    *ok = true;
+    return PTH_COMPLETE;
+  }
+  if (SameLevelNeighbors() != PTH_COMPLETE) return PTH_FAIL;
+  c.pack_cross.Free();
+  c.unpack_cross.Free();
+  c.shell.clear();
+  std::vector<std::pair<const double *, const double *>> ranges;
+  const std::int64_t cells = topo.fine.cells();
+  for (size_t f = 0; f < fields.size(); ++f) {
+    if (!(fields[f].flags & PTH_FILL_GHOST)) continue;
+    const FieldStore &st = md.store[f];
+    ranges.emplace_back(st.data, st.data + st.ndata);
+    const int nc = fields[f].ncomp;
+    for (int v = 0; v < nc; ++v) c.shell.push_back({st.data + v * cells, nc * cells});
+  }
+  std::vector<Region> sregs, rregs;
+  for (size_t q = 0; q < c.pack.host.size(); ++q) {
+    const Region &sr = c.pack.host[q], &rr = c.unpack.host[q];
+    if (sr.b != rr.b) {
+      SetError("internal: send and receive regions do not pair up");
+      return PTH_FAIL;
+    }
+    bool same = false;
+    for (const auto &r : ranges)
+      same = same || (sr.a >= r.first && sr.a < r.second && rr.a >= r.first && rr.a < r.second);
+    if (same) continue;
+    sregs.push_back(sr);
+    rregs.push_back(rr);
+  }
+  for (Table *t : {&c.pack_cross, &c.unpack_cross}) {
+    t->host = t == &c.pack_cross ? sregs : rregs;
+    if (t->Upload() != PTH_COMPLETE) return PTH_FAIL;
+  }
+  if (c.pack_cross.total != c.unpack_cross.total) {
+    SetError("internal: send and receive element counts differ");
+    return PTH_FAIL;
+  }
+  c.cross_built = true;
   *ok = true;
   return PTH_COMPLETE;
 }

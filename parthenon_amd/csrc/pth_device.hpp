@@ -14,4 +14,12 @@ __device__ __forceinline__ unsigned XcdRemap(unsigned w, unsigned W) {
   return x * per + (x < rem ? x : rem) + i;
 }
 
+// Global-address-space view of a device pointer. Loads and stores through it are global_load
+// / global_store, counted on vmcnt alone; through a generic pointer read from memory (a table
+// entry, a struct of arrays) the compiler emits flat accesses, which also count on lgkmcnt.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *Glob(T *p) {
+  return (__attribute__((address_space(1))) T *)p;
+}
+
 }  // namespace pth

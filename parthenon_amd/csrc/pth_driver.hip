@@ -51,7 +51,16 @@ pth_status Exchange(Mesh &m, MeshData &md, hipStream_t s, bool rest = false) {
       }
       if (LaunchExchangeDirect(c.pack_rest, c.unpack_rest, s) != PTH_COMPLETE) return PTH_FAIL;
     } else if (m.pack_mode == PTH_PACK_PERSISTENT && CanExchangeDirect(c.pack, c.unpack)) {
-      if (LaunchExchangeDirect(c.pack, c.unpack, s) != PTH_COMPLETE) return PTH_FAIL;
+      // same-level channels by ghost shells where the block shape allows, the rest by regions
+      bool shell = false;
+      if (m.BuildCrossExchange(md, &shell) != PTH_COMPLETE) return PTH_FAIL;
+      if (shell) {
+        if (LaunchShell(m.topo, c.shell, m.same_level, s) != PTH_COMPLETE) return PTH_FAIL;
+        if (LaunchExchangeDirect(c.pack_cross, c.unpack_cross, s) != PTH_COMPLETE)
+          return PTH_FAIL;
+      } else if (LaunchExchangeDirect(c.pack, c.unpack, s) != PTH_COMPLETE) {
+        return PTH_FAIL;
+      }
     } else {
       if (LaunchPack(c.pack, nullptr, s, true, m.pack_mode) != PTH_COMPLETE) return PTH_FAIL;
       if (LaunchPack(c.unpack, nullptr, s, false, m.pack_mode) != PTH_COMPLETE) return PTH_FAIL;
@@ -467,6 +476,9 @@ pth_status RunCycles(Mesh &m, Ctx &x, hipStream_t s, int ncycles, bool use_graph
     // tables must exist before capture (their uploads are synchronous copies)
     if (m.BuildCache(*x.base, PTH_BT_ANY) != PTH_COMPLETE) return PTH_FAIL;
     if (m.BuildCache(*x.one, PTH_BT_ANY) != PTH_COMPLETE) return PTH_FAIL;
+    bool shell = false;
+    if (m.BuildCrossExchange(*x.base, &shell) != PTH_COMPLETE) return PTH_FAIL;
+    if (m.BuildCrossExchange(*x.one, &shell) != PTH_COMPLETE) return PTH_FAIL;
     const bool prof = Prof().on;
     Prof().on = false;  // no event records inside the captured cycle
     PTH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));

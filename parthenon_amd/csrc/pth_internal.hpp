@@ -162,6 +162,20 @@ struct BlockCoefs {  // per-block uniform-Cartesian metrics used by the stencils
   double pad;
 };
 
+// Same-level ghost-shell exchange (ShellKernel, pth_bvals.hip): per (field, component), the
+// brick of block 0 and the element stride between consecutive blocks' bricks.
+struct ShellBrick {
+  double *base;
+  std::int64_t stride;
+};
+constexpr int kShellBricks = 32;  // bricks per launch (kernel arguments)
+struct ShellArgs {
+  double *brick[kShellBricks];
+  std::int64_t stride[kShellBricks];
+  const int *same;  // [block][27] same-level neighbour or -1
+  int nblocks;
+};
+
 struct Item {  // a run of consecutive elements over up to kItemRegions consecutive regions
   int region;    // first region
   int nreg;      // regions touched
@@ -294,6 +308,11 @@ struct BoundCache {  // per (MeshData, bound type): the pack / unpack / pro-res 
   // by the block sweep itself); built on first use
   Table pack_rest, unpack_rest;
   int rest_field = -1;
+  // direct-exchange pairs of the channels between blocks of different levels, with the
+  // same-level channels of every field served by the ghost-shell kernel; built on first use
+  Table pack_cross, unpack_cross;
+  std::vector<ShellBrick> shell;
+  bool cross_built = false;
 };
 
 // <parthenon/time> limits of EvolutionDriver with its defaults (src/driver/driver.hpp:66-94)
@@ -396,6 +415,9 @@ class Mesh {
   // pack_rest / unpack_rest of md's exchange cache for field f; false if the exchange has no
   // gather form (then the caller keeps the full exchange)
   pth_status BuildRestExchange(MeshData &md, int f, bool *ok);
+  // same-level channels of every field by ghost shells, the others in pack_cross /
+  // unpack_cross; false in ok where the shell kernel does not apply
+  pth_status BuildCrossExchange(MeshData &md, bool *ok);
   // per block, the same-level neighbour in each of the 27 directions (or -1), on the device
   int *same_level = nullptr;
   pth_status SameLevelNeighbors();
@@ -442,6 +464,11 @@ pth_status LaunchPack(const Table &t, double *arena_unused, hipStream_t s, bool 
                       int mode);
 bool CanExchangeDirect(const Table &pack, const Table &unpack);
 pth_status LaunchExchangeDirect(const Table &pack, const Table &unpack, hipStream_t s);
+// same-level channels of the listed bricks by ghost shells (cubic blocks of 8, 16 or 32 cells
+// with 2 to 4 ghosts: ShellSupported)
+bool ShellSupported(const Topology &t);
+pth_status LaunchShell(const Topology &t, const std::vector<ShellBrick> &bricks,
+                       const int *same, hipStream_t s);
 pth_status LaunchRestrict(const Table &t, const DeviceBlockGeom *geom, hipStream_t s);
 pth_status LaunchProlong(const Table &t, const DeviceBlockGeom *geom, hipStream_t s);
 pth_status LaunchBoundaryConditions(const BcTable &t, hipStream_t s);
