@@ -53,39 +53,94 @@ __device__ __forceinline__ std::int64_t VarOffset(const Region &R, const Cell &d
          static_cast<std::int64_t>(d.j + R.lo[1]) * R.astride[0] + (d.i + R.lo[0]);
 }
 
-// Element of lane slot u within an item. Lanes past the item's end take its last element:
-// they load and store the same value at the same address as the lane that owns it, so every
-// load and store is unconditional. (gfx9 counts loads and stores on one in-order counter; a
-// store under a branch makes the compiler wait for every earlier access before it, which
-// serializes the stores.)
+// Element of lane slot u within an item: lanes past the item's end take its last element
+// and load and store the same value at the same address as the lane that owns it, so no
+// load or store sits under a per-lane branch. (gfx9 counts loads and stores on
+// one in-order counter; a store under a per-lane branch makes the compiler wait for every
+// earlier access before it, which serializes the stores.)
 __device__ __forceinline__ int Lane(const Item &I, int u) {
   const int local = u * kWG + static_cast<int>(threadIdx.x);
   return local < I.count ? local : I.count - 1;
 }
+__device__ __forceinline__ int Slots(const Item &I) { return (I.count + kWG - 1) / kWG; }
 
-// Pack (TO_BUFFER) or unpack: one launch over the whole table, one workgroup per work item.
-// The item and its region record are uniform (scalar loads); each lane issues its kPer loads
-// before its kPer stores, so a workgroup keeps 32 KiB in flight.
-template <bool TO_BUFFER>
-__global__ void __launch_bounds__(kWG)
-    PackKernel(const Region *__restrict__ reg, const Item *__restrict__ items) {
-  const Item I = items[blockIdx.x];
-  const Region &R = reg[I.region];
-  double v[kPer];
-  int off[kPer], e[kPer];
+// Straight-line bodies for NS slots: the NS loads of a lane, then its NS stores. The kernels
+// below pick NS = Slots(I) with a workgroup-uniform switch, so a small item (a corner, an
+// edge) runs only the slots it fills and no load or store sits under a branch.
+template <int NS, bool TO_BUFFER>
+__device__ __forceinline__ void PackSlots(const Region &R, const Item &I) {
+  double v[NS];
+  int off[NS], e[NS];
 #pragma unroll
-  for (int u = 0; u < kPer; ++u) {
+  for (int u = 0; u < NS; ++u) {
     e[u] = I.begin + Lane(I, u);
     off[u] = static_cast<int>(VarOffset(R, Decode(R, static_cast<std::uint32_t>(e[u]))));
     v[u] = TO_BUFFER ? Glob(R.a)[off[u]] : Glob(R.b)[e[u]];
   }
 #pragma unroll
-  for (int u = 0; u < kPer; ++u) {
+  for (int u = 0; u < NS; ++u) {
     if (TO_BUFFER)
       Glob(R.b)[e[u]] = v[u];
     else
       Glob(R.a)[off[u]] = v[u];
   }
+}
+
+template <int NS>
+__device__ __forceinline__ void CopySlots(const Region &R, const Item &I) {
+  double v[NS];
+  std::int64_t off[NS];
+#pragma unroll
+  for (int u = 0; u < NS; ++u) {
+    const Cell d = Decode(R, static_cast<std::uint32_t>(I.begin + Lane(I, u)));
+    v[u] = Glob(R.a)[VarOffset(R, d)];
+    off[u] = static_cast<std::int64_t>(d.c) * R.bstride[2] +
+             static_cast<std::int64_t>(d.k) * R.bstride[1] +
+             static_cast<std::int64_t>(d.j) * R.bstride[0] + d.i;
+  }
+#pragma unroll
+  for (int u = 0; u < NS; ++u) Glob(R.b)[off[u]] = v[u];
+}
+
+template <int NS>
+__device__ __forceinline__ void ExchangeSlots(const Region &S, const Region &D, const Item &I) {
+  double v[NS];
+  std::int64_t off[NS];
+#pragma unroll
+  for (int u = 0; u < NS; ++u) {
+    const std::uint32_t e = static_cast<std::uint32_t>(I.begin + Lane(I, u));
+    v[u] = Glob(S.a)[VarOffset(S, Decode(S, e))];
+    off[u] = VarOffset(D, Decode(D, e));
+  }
+#pragma unroll
+  for (int u = 0; u < NS; ++u) Glob(D.a)[off[u]] = v[u];
+}
+
+// slot counts 1, 2, 4, 8 and kPer (rounded up) keep the code small
+#define PTH_SLOTS(CALL)             \
+  switch (Slots(I)) {               \
+    case 1: CALL(1); break;         \
+    case 2: CALL(2); break;         \
+    case 3:                         \
+    case 4: CALL(4); break;         \
+    case 5:                         \
+    case 6:                         \
+    case 7:                         \
+    case 8: CALL(8); break;         \
+    default: CALL(kPer); break;     \
+  }
+
+// Pack (TO_BUFFER) or unpack: one launch over the whole table, one workgroup per work item.
+// The item and its region record are uniform (scalar loads); each lane issues its loads
+// before its stores (up to kPer of each), so a workgroup keeps up to 32 KiB in flight.
+template <bool TO_BUFFER>
+__global__ void __launch_bounds__(kWG)
+    PackKernel(const Region *__restrict__ reg, const Item *__restrict__ items) {
+  const Item I = items[blockIdx.x];
+  const Region &R = reg[I.region];
+#define PTH_PACK(NS) PackSlots<NS, TO_BUFFER>(R, I)
+  PTH_SLOTS(PTH_PACK)
+#undef PTH_PACK
 }
 
 // Box copies (remesh): region element e moves from its box in a to the same box position in
@@ -94,18 +149,9 @@ __global__ void __launch_bounds__(kWG)
     CopyKernel(const Region *__restrict__ reg, const Item *__restrict__ items) {
   const Item I = items[blockIdx.x];
   const Region &R = reg[I.region];
-  double v[kPer];
-  std::int64_t off[kPer];
-#pragma unroll
-  for (int u = 0; u < kPer; ++u) {
-    const Cell d = Decode(R, static_cast<std::uint32_t>(I.begin + Lane(I, u)));
-    v[u] = Glob(R.a)[VarOffset(R, d)];
-    off[u] = static_cast<std::int64_t>(d.c) * R.bstride[2] +
-             static_cast<std::int64_t>(d.k) * R.bstride[1] +
-             static_cast<std::int64_t>(d.j) * R.bstride[0] + d.i;
-  }
-#pragma unroll
-  for (int u = 0; u < kPer; ++u) Glob(R.b)[off[u]] = v[u];
+#define PTH_COPY(NS) CopySlots<NS>(R, I)
+  PTH_SLOTS(PTH_COPY)
+#undef PTH_COPY
 }
 
 // Fused exchange by region pairs: send region q and receive region q cover the same slice of
@@ -118,16 +164,9 @@ __global__ void __launch_bounds__(kWG)
   const Item I = items[blockIdx.x];
   const Region &S = sreg[I.region];
   const Region &D = rreg[I.region];
-  double v[kPer];
-  std::int64_t off[kPer];
-#pragma unroll
-  for (int u = 0; u < kPer; ++u) {
-    const std::uint32_t e = static_cast<std::uint32_t>(I.begin + Lane(I, u));
-    v[u] = Glob(S.a)[VarOffset(S, Decode(S, e))];
-    off[u] = VarOffset(D, Decode(D, e));
-  }
-#pragma unroll
-  for (int u = 0; u < kPer; ++u) Glob(D.a)[off[u]] = v[u];
+#define PTH_XCHG(NS) ExchangeSlots<NS>(S, D, I)
+  PTH_SLOTS(PTH_XCHG)
+#undef PTH_XCHG
 }
 
 // Same-level ghost shells: one workgroup per (field brick q, receiving block b) fills every
@@ -146,8 +185,10 @@ __global__ void __launch_bounds__(kWG) ShellKernel(const ShellArgs A) {
   // loads in flight per lane: two passes over the shell where that takes at most 16
   constexpr int U = (S + 2 * kWG - 1) / (2 * kWG) < 16 ? (S + 2 * kWG - 1) / (2 * kWG) : 16;
   __shared__ int nbr[27];
-  const unsigned L = XcdRemap(blockIdx.x, gridDim.x);  // = q * nblocks + b
-  const int b = static_cast<int>(L % static_cast<unsigned>(A.nblocks));
+  // logical id q * nblocks + m: each XCD takes a contiguous range of one brick's blocks, in
+  // Morton order so that the neighbours reading a block's boundary run close in time
+  const unsigned L = XcdRemap(blockIdx.x, gridDim.x);
+  const int b = A.order[L % static_cast<unsigned>(A.nblocks)];
   const int q = static_cast<int>(L / static_cast<unsigned>(A.nblocks));
   if (threadIdx.x < 27) nbr[threadIdx.x] = A.same[27 * b + threadIdx.x];
   __syncthreads();
@@ -196,8 +237,10 @@ __global__ void __launch_bounds__(kWG) ShellKernel(const ShellArgs A) {
                        : dst[off[u]];
       }
     }
+    // streaming stores: the ghost lines are not read again by this launch, so they should
+    // not push the neighbours' boundary lines out of the L2
 #pragma unroll
-    for (int u = 0; u < U; ++u) dst[off[u]] = v[u];
+    for (int u = 0; u < U; ++u) __builtin_nontemporal_store(v[u], &dst[off[u]]);
   }
 }
 
@@ -373,7 +416,7 @@ __device__ __forceinline__ void ProlongStore(const Region &R, const ProlongIn &i
 // active (3-D meshes). A lane takes kProRes cells of the item at a time: their loads first,
 // then their arithmetic and stores (lanes past the item's end repeat its last cell, which
 // writes the same value to the same place).
-constexpr int kProRes = 4;
+constexpr int kProRes = kProResItemElems / kWG;
 template <int OP, bool ALL3>
 __global__ void __launch_bounds__(kWG)
     ProResKernel(const Region *__restrict__ reg, const Item *__restrict__ items, int nitems,
@@ -553,10 +596,10 @@ bool ShellSupported(const Topology &t) {
 }
 
 pth_status LaunchShell(const Topology &t, const std::vector<ShellBrick> &bricks,
-                       const int *same, hipStream_t s) {
+                       const int *same, const int *order, hipStream_t s) {
   const int nb = static_cast<int>(t.blocks.size());
   if (bricks.empty() || nb == 0) return PTH_COMPLETE;
-  if (!ShellSupported(t) || !same) {
+  if (!ShellSupported(t) || !same || !order) {
     SetError("internal: ghost-shell exchange needs cubic 8/16/32 blocks and 2-4 ghosts");
     return PTH_FAIL;
   }
@@ -570,6 +613,7 @@ pth_status LaunchShell(const Topology &t, const std::vector<ShellBrick> &bricks,
       A.stride[q] = bricks[q0 + q].stride;
     }
     A.same = same;
+    A.order = order;
     A.nblocks = nb;
     const dim3 grid(static_cast<unsigned>(nq) * nb);
 #define PTH_SHELL(NN, GG)                                                              \

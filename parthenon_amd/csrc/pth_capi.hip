@@ -79,6 +79,8 @@ MeshData::~MeshData() {
     c.unpack_rest.Free();
     c.pack_cross.Free();
     c.unpack_cross.Free();
+    if (c.shell_order) (void)hipFree(c.shell_order);
+    c.shell_order = nullptr;
   }
   for (auto &kv : packs) kv.second.Free();
 }
@@ -575,6 +577,32 @@ pth_status Mesh::BuildCrossExchange(MeshData &md, bool *ok) {
   c.pack_cross.Free();
   c.unpack_cross.Free();
   c.shell.clear();
+  if (c.shell_order) (void)hipFree(c.shell_order);
+  c.shell_order = nullptr;
+  {  // blocks in Morton order of their positions on the finest level's lattice
+    const int nb = static_cast<int>(topo.blocks.size());
+    int top = 0;
+    for (const Block &B : topo.blocks) top = std::max(top, B.at.level);
+    auto spread = [](std::uint64_t v) {  // bits of v to every third bit
+      std::uint64_t r = 0;
+      for (int bit = 0; bit < 21; ++bit) r |= ((v >> bit) & 1ull) << (3 * bit);
+      return r;
+    };
+    std::vector<std::pair<std::uint64_t, int>> key(nb);
+    for (int b = 0; b < nb; ++b) {
+      const Lattice &L = topo.blocks[b].at;
+      const int sh = top - L.level;
+      key[b] = {spread(static_cast<std::uint64_t>(L.x[0]) << sh) |
+                    (spread(static_cast<std::uint64_t>(L.x[1]) << sh) << 1) |
+                    (spread(static_cast<std::uint64_t>(L.x[2]) << sh) << 2),
+                b};
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<int> order(nb);
+    for (int b = 0; b < nb; ++b) order[b] = key[b].second;
+    PTH_HIP(hipMalloc(&c.shell_order, sizeof(int) * std::max(nb, 1)));
+    PTH_HIP(hipMemcpy(c.shell_order, order.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
+  }
   std::vector<std::pair<const double *, const double *>> ranges;
   const std::int64_t cells = topo.fine.cells();
   for (size_t f = 0; f < fields.size(); ++f) {

@@ -55,7 +55,8 @@ pth_status Exchange(Mesh &m, MeshData &md, hipStream_t s, bool rest = false) {
       bool shell = false;
       if (m.BuildCrossExchange(md, &shell) != PTH_COMPLETE) return PTH_FAIL;
       if (shell) {
-        if (LaunchShell(m.topo, c.shell, m.same_level, s) != PTH_COMPLETE) return PTH_FAIL;
+        if (LaunchShell(m.topo, c.shell, m.same_level, c.shell_order, s) != PTH_COMPLETE)
+          return PTH_FAIL;
         if (LaunchExchangeDirect(c.pack_cross, c.unpack_cross, s) != PTH_COMPLETE)
           return PTH_FAIL;
       } else if (LaunchExchangeDirect(c.pack, c.unpack, s) != PTH_COMPLETE) {

@@ -142,6 +142,29 @@ __device__ __forceinline__ Nbrs<T> Load(const T *__restrict__ x, int c, const Ce
   return v;
 }
 
+// The same from an array in global memory, every load unconditional: a neighbour outside the
+// array reads the cell itself and the ghost value -u replaces it afterwards. (gfx9 counts
+// loads and stores on one in-order counter; loads under a per-lane branch make the compiler
+// wait for each before the next.)
+template <typename T>
+__device__ __forceinline__ Nbrs<T> LoadG(const T *x0, int c, const Cell &q, int n) {
+  const auto *x = Glob(x0);
+  const int s1 = n, s2 = n * n;
+  const T u = x[c];
+  const T xm = x[q.i > 0 ? c - 1 : c], xp = x[q.i < n - 1 ? c + 1 : c];
+  const T ym = x[q.j > 0 ? c - s1 : c], yp = x[q.j < n - 1 ? c + s1 : c];
+  const T zm = x[q.k > 0 ? c - s2 : c], zp = x[q.k < n - 1 ? c + s2 : c];
+  Nbrs<T> v;
+  v.u = u;
+  v.xm = q.i > 0 ? xm : -u;
+  v.xp = q.i < n - 1 ? xp : -u;
+  v.ym = q.j > 0 ? ym : -u;
+  v.yp = q.j < n - 1 ? yp : -u;
+  v.zm = q.k > 0 ? zm : -u;
+  v.zp = q.k < n - 1 ? zp : -u;
+  return v;
+}
+
 template <typename T>
 __device__ __forceinline__ T JacobiPoint(const Nbrs<T> &v, T b, const Lv<T> &o, T w) {
   const T off = ApplyA(v.u, v.xm, v.xp, v.ym, v.yp, v.zm, v.zp, o) - o.D * v.u;
@@ -194,6 +217,29 @@ __device__ __forceinline__ GmgEntry<T> Find(const GmgArgs<T> &A, int &local) {
 
 // FP64 outer step: u_new = u_old + e (e may be null), r = b - A u_new, r stored in T as the
 // inner right-hand side, per-workgroup partial sums of r^2.
+// One cell of the outer step; CORR: a correction is pending (e.x). Every load comes before
+// the first store, so a lane has its 8 (or 15) loads in flight at once.
+template <typename T, bool CORR>
+__device__ __forceinline__ double OuterCell(const GmgEntry<T> &e, int c, const Cell &q, int n) {
+  const Lv<double> o = MakeLv<double>(n);
+  const double bv = Glob(static_cast<const double *>(e.b64))[c];
+  Nbrs<double> v = LoadG(static_cast<const double *>(e.u_old), c, q, n);
+  if (CORR) {  // add the correction at the 7 points (the ghost follows the updated value)
+    const Nbrs<T> d = LoadG(static_cast<const T *>(e.x), c, q, n);
+    v.u += static_cast<double>(d.u);
+    v.xm = q.i > 0 ? v.xm + static_cast<double>(d.xm) : -v.u;
+    v.xp = q.i < n - 1 ? v.xp + static_cast<double>(d.xp) : -v.u;
+    v.ym = q.j > 0 ? v.ym + static_cast<double>(d.ym) : -v.u;
+    v.yp = q.j < n - 1 ? v.yp + static_cast<double>(d.yp) : -v.u;
+    v.zm = q.k > 0 ? v.zm + static_cast<double>(d.zm) : -v.u;
+    v.zp = q.k < n - 1 ? v.zp + static_cast<double>(d.zp) : -v.u;
+    Glob(e.u_new)[c] = v.u;
+  }
+  const double r = bv - ApplyA(v.u, v.xm, v.xp, v.ym, v.yp, v.zm, v.zp, o);
+  Glob(e.b)[c] = static_cast<T>(r);
+  return r;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kWG) OuterKernel(const GmgArgs<T> A) {
   __shared__ double wsum[kWG / 64];
@@ -203,22 +249,7 @@ __global__ void __launch_bounds__(kWG) OuterKernel(const GmgArgs<T> A) {
   double rr = 0.0;
   if (c < n * n * n) {
     const Cell q = Decode(c, n);
-    const Lv<double> o = MakeLv<double>(n);
-    const double *uo = e.u_old;
-    Nbrs<double> v = Load(uo, c, q, n);
-    if (e.x) {  // add the correction at the 7 points (the ghost follows the updated value)
-      const Nbrs<T> d = Load(static_cast<const T *>(e.x), c, q, n);
-      v.u += static_cast<double>(d.u);
-      v.xm = q.i > 0 ? v.xm + static_cast<double>(d.xm) : -v.u;
-      v.xp = q.i < n - 1 ? v.xp + static_cast<double>(d.xp) : -v.u;
-      v.ym = q.j > 0 ? v.ym + static_cast<double>(d.ym) : -v.u;
-      v.yp = q.j < n - 1 ? v.yp + static_cast<double>(d.yp) : -v.u;
-      v.zm = q.k > 0 ? v.zm + static_cast<double>(d.zm) : -v.u;
-      v.zp = q.k < n - 1 ? v.zp + static_cast<double>(d.zp) : -v.u;
-      e.u_new[c] = v.u;
-    }
-    const double r = e.b64[c] - ApplyA(v.u, v.xm, v.xp, v.ym, v.yp, v.zm, v.zp, o);
-    e.b[c] = static_cast<T>(r);
+    const double r = e.x ? OuterCell<T, true>(e, c, q, n) : OuterCell<T, false>(e, c, q, n);
     rr = r * r;
   }
   for (int off = 32; off > 0; off >>= 1) rr += __shfl_xor(rr, off, 64);
@@ -270,7 +301,7 @@ __global__ void __launch_bounds__(kWG) RhsNormKernel(const GmgArgs<T> A) {
   const GmgEntry<T> e = Find(A, local);
   const int n = e.n, c = local * kWG + threadIdx.x;
   double s = 0.0;
-  if (c < n * n * n) s = e.b64[c] * e.b64[c];
+  if (c < n * n * n) s = Glob(e.b64)[c] * Glob(e.b64)[c];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
   if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
   __syncthreads();
@@ -296,10 +327,10 @@ __global__ void __launch_bounds__(kWG)
   const Nbrs<T> zero = {T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
   const T *b = e.b;
   if (!PRE2) {
-    e.x[c] = JacobiPoint(zero, b[c], o, w);
+    Glob(e.x)[c] = JacobiPoint(zero, Glob(b)[c], o, w);
     return;
   }
-  const Nbrs<T> bb = Load(b, c, q, n);
+  const Nbrs<T> bb = LoadG(b, c, q, n);
   Nbrs<T> x1;
   x1.u = JacobiPoint(zero, bb.u, o, w);
   x1.xm = q.i > 0 ? JacobiPoint(zero, bb.xm, o, w) : -x1.u;
@@ -308,7 +339,7 @@ __global__ void __launch_bounds__(kWG)
   x1.yp = q.j < n - 1 ? JacobiPoint(zero, bb.yp, o, w) : -x1.u;
   x1.zm = q.k > 0 ? JacobiPoint(zero, bb.zm, o, w) : -x1.u;
   x1.zp = q.k < n - 1 ? JacobiPoint(zero, bb.zp, o, w) : -x1.u;
-  e.x[c] = JacobiPoint(x1, bb.u, o, w);
+  Glob(e.x)[c] = JacobiPoint(x1, bb.u, o, w);
 }
 
 template <typename T>
@@ -316,7 +347,7 @@ __global__ void __launch_bounds__(kWG) ZeroKernel(const GmgArgs<T> A) {
   int local;
   const GmgEntry<T> e = Find(A, local);
   const int c = local * kWG + threadIdx.x;
-  if (c < e.n * e.n * e.n) e.x[c] = T(0);
+  if (c < e.n * e.n * e.n) Glob(e.x)[c] = T(0);
 }
 
 // one Jacobi sweep x -> t (the caller swaps x and t)
@@ -329,7 +360,7 @@ __global__ void __launch_bounds__(kWG)
   if (c >= n * n * n) return;
   const Cell q = Decode(c, n);
   const Lv<T> o = MakeLv<T>(n);
-  e.t[c] = JacobiPoint(Load(static_cast<const T *>(e.x), c, q, n), e.b[c], o, w);
+  Glob(e.t)[c] = JacobiPoint(LoadG(static_cast<const T *>(e.x), c, q, n), Glob(e.b)[c], o, w);
 }
 
 // One Jacobi sweep x -> t by z columns: a workgroup covers 64 x 4 cells of NZ consecutive
@@ -347,25 +378,32 @@ __global__ void __launch_bounds__(kWG) JacobiZKernel(const GmgArgs<T> A, T w) {
   const int k0 = bz * NZ;
   const Lv<T> o = MakeLv<T>(n);
   const int s1 = n, s2 = n * n;
-  const T *x = e.x;
+  const auto *x = Glob(static_cast<const T *>(e.x));
+  const auto *b = Glob(static_cast<const T *>(e.b));
+  auto *t = Glob(e.t);
   int c = (k0 * n + j) * n + i;
+  // neighbours outside the array read the cell itself (unconditional loads) and are replaced
   T cur = x[c];
-  T prev = k0 > 0 ? x[c - s2] : T(0);
-  T next = k0 + 1 < n ? x[c + s2] : T(0);
+  T prev = x[k0 > 0 ? c - s2 : c];
+  T next = x[k0 + 1 < n ? c + s2 : c];
 #pragma unroll 2
   for (int k = k0; k < k0 + NZ; ++k, c += s2) {
+    const T xm = x[i > 0 ? c - 1 : c], xp = x[i < n - 1 ? c + 1 : c];
+    const T ym = x[j > 0 ? c - s1 : c], yp = x[j < n - 1 ? c + s1 : c];
+    const T bc = b[c];
+    const T nn = x[k + 2 < n ? c + 2 * s2 : c];
     Nbrs<T> v;
     v.u = cur;
-    v.xm = i > 0 ? x[c - 1] : -v.u;
-    v.xp = i < n - 1 ? x[c + 1] : -v.u;
-    v.ym = j > 0 ? x[c - s1] : -v.u;
-    v.yp = j < n - 1 ? x[c + s1] : -v.u;
+    v.xm = i > 0 ? xm : -v.u;
+    v.xp = i < n - 1 ? xp : -v.u;
+    v.ym = j > 0 ? ym : -v.u;
+    v.yp = j < n - 1 ? yp : -v.u;
     v.zm = k > 0 ? prev : -v.u;
     v.zp = k < n - 1 ? next : -v.u;
-    e.t[c] = JacobiPoint(v, e.b[c], o, w);
+    t[c] = JacobiPoint(v, bc, o, w);
     prev = cur;
     cur = next;
-    next = k + 2 < n ? x[c + 2 * s2] : T(0);
+    next = nn;
   }
 }
 
@@ -397,22 +435,27 @@ __global__ void __launch_bounds__(kWG) PreZero2ZKernel(const GmgArgs<T> A, T w) 
   int c = z.c;
   const Lv<T> o = MakeLv<T>(n);
   const Nbrs<T> zero = {T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
-  const T *b = e.b;
-  T bprev = z.k0 > 0 ? b[c - s2] : T(0), bcur = b[c];
-  T bnext = z.k0 + 1 < n ? b[c + s2] : T(0);
+  const auto *b = Glob(static_cast<const T *>(e.b));
+  auto *x = Glob(e.x);
+  // neighbours outside the array read the cell itself (unconditional loads) and are replaced
+  T bprev = b[z.k0 > 0 ? c - s2 : c], bcur = b[c];
+  T bnext = b[z.k0 + 1 < n ? c + s2 : c];
   for (int k = z.k0; k < z.k0 + NZ; ++k, c += s2) {
+    const T bxm = b[i > 0 ? c - 1 : c], bxp = b[i < n - 1 ? c + 1 : c];
+    const T bym = b[j > 0 ? c - s1 : c], byp = b[j < n - 1 ? c + s1 : c];
+    const T bnn = b[k + 2 < n ? c + 2 * s2 : c];
     Nbrs<T> x1;
     x1.u = JacobiPoint(zero, bcur, o, w);
-    x1.xm = i > 0 ? JacobiPoint(zero, b[c - 1], o, w) : -x1.u;
-    x1.xp = i < n - 1 ? JacobiPoint(zero, b[c + 1], o, w) : -x1.u;
-    x1.ym = j > 0 ? JacobiPoint(zero, b[c - s1], o, w) : -x1.u;
-    x1.yp = j < n - 1 ? JacobiPoint(zero, b[c + s1], o, w) : -x1.u;
+    x1.xm = i > 0 ? JacobiPoint(zero, bxm, o, w) : -x1.u;
+    x1.xp = i < n - 1 ? JacobiPoint(zero, bxp, o, w) : -x1.u;
+    x1.ym = j > 0 ? JacobiPoint(zero, bym, o, w) : -x1.u;
+    x1.yp = j < n - 1 ? JacobiPoint(zero, byp, o, w) : -x1.u;
     x1.zm = k > 0 ? JacobiPoint(zero, bprev, o, w) : -x1.u;
     x1.zp = k < n - 1 ? JacobiPoint(zero, bnext, o, w) : -x1.u;
-    e.x[c] = JacobiPoint(x1, bcur, o, w);
+    x[c] = JacobiPoint(x1, bcur, o, w);
     bprev = bcur;
     bcur = bnext;
-    bnext = k + 2 < n ? b[c + 2 * s2] : T(0);
+    bnext = bnn;
   }
 }
 
@@ -444,11 +487,11 @@ __global__ void __launch_bounds__(kWG)
       for (int c2 = 0; c2 < 2; ++c2) {
         const Cell q = {2 * p.i + c2, 2 * p.j + bb, 2 * p.k + a};
         const int c = (q.k * n + q.j) * n + q.i;
-        const Nbrs<T> v = Load(static_cast<const T *>(e.x), c, q, n);
-        const T r = e.b[c] - ApplyA(v.u, v.xm, v.xp, v.ym, v.yp, v.zm, v.zp, o);
+        const Nbrs<T> v = LoadG(static_cast<const T *>(e.x), c, q, n);
+        const T r = Glob(e.b)[c] - ApplyA(v.u, v.xm, v.xp, v.ym, v.yp, v.zm, v.zp, o);
         t[a][bb][c2] = vol * r;
       }
-  e.bc[cc] = Restrict8(t, o);
+  Glob(e.bc)[cc] = Restrict8(t, o);
 }
 
 // x += ProlongateSharedLinear(coarse x); one thread per fine cell, coarse ghosts from the
@@ -462,7 +505,7 @@ __global__ void __launch_bounds__(kWG) ProlongKernel(const GmgArgs<T> A) {
   const Cell q = Decode(c, n);
   const Cell p = {q.i >> 1, q.j >> 1, q.k >> 1};
   const int oi = q.i & 1, oj = q.j & 1, ok = q.k & 1;
-  const T *xc = e.xc;
+  const auto *xc = Glob(static_cast<const T *>(e.xc));
   const int cc = (p.k * nc + p.j) * nc + p.i;
   const T fc = xc[cc];
   const Lv<T> oc = MakeLv<T>(nc);
@@ -472,9 +515,13 @@ __global__ void __launch_bounds__(kWG) ProlongKernel(const GmgArgs<T> A) {
   const bool in1 = oi ? p.i < nc - 1 : p.i > 0;
   const bool in2 = oj ? p.j < nc - 1 : p.j > 0;
   const bool in3 = ok ? p.k < nc - 1 : p.k > 0;
-  const T f1 = in1 ? xc[cc + (oi ? 1 : -1)] : -fc;
-  const T f2 = in2 ? xc[cc + (oj ? nc : -nc)] : -fc;
-  const T f3 = in3 ? xc[cc + (ok ? nc * nc : -nc * nc)] : -fc;
+  // unconditional loads (a neighbour outside the level reads the cell itself), then the ghost
+  const T l1 = xc[in1 ? cc + (oi ? 1 : -1) : cc];
+  const T l2 = xc[in2 ? cc + (oj ? nc : -nc) : cc];
+  const T l3 = xc[in3 ? cc + (ok ? nc * nc : -nc * nc) : cc];
+  const T f1 = in1 ? l1 : -fc;
+  const T f2 = in2 ? l2 : -fc;
+  const T f3 = in3 ? l3 : -fc;
   const T g1 = grad(oi ? f1 - fc : fc - f1);
   const T g2 = grad(oj ? f2 - fc : fc - f2);
   const T g3 = grad(ok ? f3 - fc : fc - f3);
@@ -486,7 +533,7 @@ __global__ void __launch_bounds__(kWG) ProlongKernel(const GmgArgs<T> A) {
   const T sb = oi == oj ? T(1) : T(-1);
   const T sc = oi == ok ? T(1) : T(-1);
   const T v = fc + so * ((g1 * df + sb * (g2 * df)) + sc * (g3 * df));
-  e.x[c] += v;
+  Glob(e.x)[c] += v;
 }
 
 // ---- bottom of the V-cycle in one workgroup (levels of at most NB cells per side) --------

@@ -172,7 +172,8 @@ constexpr int kShellBricks = 32;  // bricks per launch (kernel arguments)
 struct ShellArgs {
   double *brick[kShellBricks];
   std::int64_t stride[kShellBricks];
-  const int *same;  // [block][27] same-level neighbour or -1
+  const int *same;   // [block][27] same-level neighbour or -1
+  const int *order;  // blocks in Morton order of their positions
   int nblocks;
 };
 
@@ -312,6 +313,7 @@ struct BoundCache {  // per (MeshData, bound type): the pack / unpack / pro-res 
   // same-level channels of every field served by the ghost-shell kernel; built on first use
   Table pack_cross, unpack_cross;
   std::vector<ShellBrick> shell;
+  int *shell_order = nullptr;  // device: block ids in Morton order
   bool cross_built = false;
 };
 
@@ -468,7 +470,7 @@ pth_status LaunchExchangeDirect(const Table &pack, const Table &unpack, hipStrea
 // with 2 to 4 ghosts: ShellSupported)
 bool ShellSupported(const Topology &t);
 pth_status LaunchShell(const Topology &t, const std::vector<ShellBrick> &bricks,
-                       const int *same, hipStream_t s);
+                       const int *same, const int *order, hipStream_t s);
 pth_status LaunchRestrict(const Table &t, const DeviceBlockGeom *geom, hipStream_t s);
 pth_status LaunchProlong(const Table &t, const DeviceBlockGeom *geom, hipStream_t s);
 pth_status LaunchBoundaryConditions(const BcTable &t, hipStream_t s);
