@@ -494,18 +494,14 @@ __global__ void __launch_bounds__(kWG)
   Glob(e.bc)[cc] = Restrict8(t, o);
 }
 
-// x += ProlongateSharedLinear(coarse x); one thread per fine cell, coarse ghosts from the
-// Dirichlet condition (pr_ops.hpp:166-289 term grouping)
-template <typename T>
-__global__ void __launch_bounds__(kWG) ProlongKernel(const GmgArgs<T> A) {
-  int local;
-  const GmgEntry<T> e = Find(A, local);
-  const int n = e.n, nc = e.nc, c = local * kWG + threadIdx.x;
-  if (c >= n * n * n) return;
-  const Cell q = Decode(c, n);
+// ProlongateSharedLinear of the coarse correction xc at fine cell q of an n^3 level (coarse
+// ghosts from the Dirichlet condition; pr_ops.hpp:166-289 term grouping). Every load is
+// unconditional: a coarse neighbour outside the level reads the cell itself and the ghost
+// value replaces it.
+template <typename T, typename P>
+__device__ __forceinline__ T ProlongValue(const P *xc, int nc, const Cell &q) {
   const Cell p = {q.i >> 1, q.j >> 1, q.k >> 1};
   const int oi = q.i & 1, oj = q.j & 1, ok = q.k & 1;
-  const auto *xc = Glob(static_cast<const T *>(e.xc));
   const int cc = (p.k * nc + p.j) * nc + p.i;
   const T fc = xc[cc];
   const Lv<T> oc = MakeLv<T>(nc);
@@ -515,7 +511,6 @@ __global__ void __launch_bounds__(kWG) ProlongKernel(const GmgArgs<T> A) {
   const bool in1 = oi ? p.i < nc - 1 : p.i > 0;
   const bool in2 = oj ? p.j < nc - 1 : p.j > 0;
   const bool in3 = ok ? p.k < nc - 1 : p.k > 0;
-  // unconditional loads (a neighbour outside the level reads the cell itself), then the ghost
   const T l1 = xc[in1 ? cc + (oi ? 1 : -1) : cc];
   const T l2 = xc[in2 ? cc + (oj ? nc : -nc) : cc];
   const T l3 = xc[in3 ? cc + (ok ? nc * nc : -nc * nc) : cc];
@@ -532,7 +527,18 @@ __global__ void __launch_bounds__(kWG) ProlongKernel(const GmgArgs<T> A) {
   const T so = oi ? T(1) : T(-1);
   const T sb = oi == oj ? T(1) : T(-1);
   const T sc = oi == ok ? T(1) : T(-1);
-  const T v = fc + so * ((g1 * df + sb * (g2 * df)) + sc * (g3 * df));
+  return fc + so * ((g1 * df + sb * (g2 * df)) + sc * (g3 * df));
+}
+
+// x += ProlongateSharedLinear(coarse x); one thread per fine cell
+template <typename T>
+__global__ void __launch_bounds__(kWG) ProlongKernel(const GmgArgs<T> A) {
+  int local;
+  const GmgEntry<T> e = Find(A, local);
+  const int n = e.n, c = local * kWG + threadIdx.x;
+  if (c >= n * n * n) return;
+  const Cell q = Decode(c, n);
+  const T v = ProlongValue<T>(Glob(static_cast<const T *>(e.xc)), e.nc, q);
   Glob(e.x)[c] += v;
 }
 
