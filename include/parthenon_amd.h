@@ -390,21 +390,33 @@ typedef struct pth_poisson_equation {
 pth_status pth_poisson_ax(pth_meshdata *md, const pth_poisson_equation *eq, const char *x,
                           const char *out);
 
-/* ---- MGSolver<u, rhs, PoissonEquation> over MeshData (src/solvers/mg_solver.hpp:37-203) ----
- * Mesh: uniform, single-level, 3-D, N^3 cells on a cube of side L, FixedFace (fixed_face) on
- * all six faces; u (FillGhost) and rhs one-component fields; constant D > 0 and
- * diagonal_alpha = 0. The hierarchy is the uniform one of src/mesh/mesh-gmg.cpp:103-154 (cell
- * counts halving while even) held as whole-level arrays; the problem is solved on the unit
- * cube with rhs scaled by L^2 / D (exact when L = D = 1). */
+/* ---- MGSolver<u, rhs, PoissonEquation> over MeshData (src/solvers/mg_solver.hpp:37-520) ----
+ * u (FillGhost) and rhs: one-component fields; constant D > 0 and diagonal_alpha >= 0.
+ * Two hierarchies, both stopping on the RMS residual of the MeshData problem:
+ * - uniform single-level 3-D meshes of N^3 cells on a cube with fixed_face (FixedFace) on all
+ *   six faces and smoother 0: whole-level arrays (cell counts halving while even, as
+ *   src/mesh/mesh-gmg.cpp:103-154 gives on such a mesh), FP32 V-cycles inside FP64 iterative
+ *   refinement (precision 0) or FP64 (precision 1); the problem is solved on the unit cube
+ *   with rhs scaled by L^2 / D (exact when L = D = 1), diagonal_alpha = 0;
+ * - every other mesh (static refinement, 1-D / 2-D, periodic or fixed_face faces, or
+ *   smoother 1-3): a hierarchy of block meshes (the leaf mesh, the mesh capped at each coarser
+ *   refinement level, then the forest's merged and cell-halved levels below the base grid) in
+ *   FP64 (precision 1 only); each level's ghosts come from its own boundary exchange with
+ *   the faces' conditions, restriction is RestrictAverage and prolongation of the correction
+ *   ProlongateSharedLinear. u's registered prolongation operator defines the leaf problem's
+ *   fine-coarse ghosts (the poisson_gmg package registers ProlongateSharedLinear). */
 typedef struct pth_mg_params {
   int max_iters;             /* MGParams::max_iters (mg_solver.hpp:37-58): V-cycles at most */
   double residual_tolerance; /* stop when the RMS residual sqrt(sum r^2 / ncells) of
                                 rhs - A u is below it (mg_solver.hpp:147-158) */
   int precision;             /* 0: FP32 V-cycles inside FP64 iterative refinement; 1: FP64 */
   int inner_cycles;          /* FP32 V-cycles per FP64 refinement step (precision 0), >= 1 */
-  double omega;              /* weighted-Jacobi weight (the reference fixes SRJ weights,
-                                mg_solver.hpp:312-345) */
-  int pre_sweeps, post_sweeps;
+  double omega;              /* weighted-Jacobi weight (smoother 0) */
+  int pre_sweeps, post_sweeps; /* weighted-Jacobi sweeps (smoother 0) */
+  int smoother;              /* 0: weighted Jacobi (omega, pre_sweeps, post_sweeps); 1, 2, 3:
+                                MGParams::smoother "SRJ1" / "SRJ2" / "SRJ3", the reference's
+                                stage weights by dimension (mg_solver.hpp:312-345), that many
+                                stages before and after the coarser level */
 } pth_mg_params;
 typedef struct pth_mg pth_mg;
 pth_mg *pth_mg_create(pth_mesh *m, const char *u, const char *rhs, const pth_mg_params *p,
@@ -419,6 +431,8 @@ pth_status pth_mg_setup(pth_mg *mg, pth_meshdata *md);
 pth_status pth_mg_solve(pth_mg *mg, pth_meshdata *md);
 /* GetFinalIterations / GetFinalResidual (mg_solver.hpp:207-208): V-cycles run, RMS residual */
 pth_status pth_mg_result(const pth_mg *mg, int *iterations, double *rms_residual);
+/* levels of the solver's hierarchy, finest first (GetGMGMaxLevel - GetGMGMinLevel + 1) */
+int pth_mg_levels(const pth_mg *mg);
 /* RMS residual before every V-cycle and after the last; returns the count */
 int pth_mg_history(const pth_mg *mg, double *rms, int max);
 

@@ -526,8 +526,9 @@ class PoissonEquation {
 
 // MGParams (mg_solver.hpp:37-58): max_iters and residual_tolerance (RMS residual
 // sqrt(sum r^2 / ncells), the reference's stop test, :147-158) as the reference; the smoother
-// is weighted Jacobi with an explicit weight and sweep counts (the reference's SRJ tables fix
-// them, :312-345), inside FP64 iterative refinement when mixed_precision is set.
+// is weighted Jacobi with an explicit weight and sweep counts ("jacobi") or the reference's
+// "SRJ1" / "SRJ2" / "SRJ3" stage weights (:312-345); mixed_precision (FP32 V-cycles inside
+// FP64 iterative refinement) applies to the uniform N^3 cube with fixed_face faces only.
 struct MGParams {
   int max_iters = 1000;
   Real residual_tolerance = 1.e-12;
@@ -535,12 +536,15 @@ struct MGParams {
   int pre_sweeps = 2, post_sweeps = 2;
   bool mixed_precision = true;  // FP32 V-cycles inside FP64 iterative refinement
   int inner_cycles = 2;          // FP32 V-cycles per FP64 refinement step
+  std::string smoother = "jacobi";
 };
 
-// MGSolver<u, rhs, equations> (mg_solver.hpp:73-208) over the blocks of a MeshData: u and rhs
+// MGSolver<u, rhs, equations> (mg_solver.hpp:73-520) over the blocks of a MeshData: u and rhs
 // are field types (static name()), equations carries the operator's coefficients
-// (Coefficients(), as PoissonEquation). The mesh must be uniform, 3-D, N^3 cells on a cube,
-// with fixed_face conditions on every face (the poisson_gmg package's FixedFace).
+// (Coefficients(), as PoissonEquation). Faces are fixed_face (the poisson_gmg package's
+// FixedFace) or periodic; the mesh may be statically refined, 1-D, 2-D or 3-D (a uniform
+// 3-D N^3 cube with fixed_face faces runs on whole-level arrays, any other mesh on a
+// hierarchy of block meshes in FP64).
 template <class u, class rhs, class equations>
 class MGSolver {
  public:
@@ -593,6 +597,12 @@ class MGSolver {
     p.omega = params_.omega;
     p.pre_sweeps = params_.pre_sweeps;
     p.post_sweeps = params_.post_sweeps;
+    p.smoother = params_.smoother == "SRJ1"   ? 1
+                 : params_.smoother == "SRJ2" ? 2
+                 : params_.smoother == "SRJ3" ? 3
+                 : params_.smoother == "jacobi"
+                     ? 0
+                     : throw std::runtime_error("MGSolver: unknown smoother " + params_.smoother);
     const pth_poisson_equation e = eqs_.Coefficients();
     h_ = pth_mg_create(pmesh->handle(), u::name().c_str(), rhs::name().c_str(), &p, &e);
     if (!h_) throw std::runtime_error(std::string("MGSolver: ") + pth_last_error());

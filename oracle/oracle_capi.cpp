@@ -9,6 +9,7 @@
 #include "orc_amr.hpp"
 #include "orc_burgers.hpp"
 #include "orc_gmg.hpp"
+#include "orc_poisson_mesh.hpp"
 
 using namespace orc;
 
@@ -693,6 +694,27 @@ int orc_amr_burgers_cycles(void *amr, void *burgers, int n) {
   int changed = 0;
   for (int q = 0; q < n; ++q) changed += AdaptiveCycle(o.om->m, B, o.a) ? 1 : 0;
   return changed;
+}
+
+// The Poisson problem on the leaf grid of a (possibly refined) mesh: x and out hold the
+// interior cells of every block in block order, then [k][j][i].
+void orc_poisson_leaf_apply(void *mesh, double D, double alpha, const double *x, double *out) {
+  const Mesh &m = *static_cast<OMesh *>(mesh)->m;
+  poisson::LeafProblem P(m, D, alpha);
+  std::vector<double> xv(x, x + P.Cells()), ov;
+  P.Apply(xv, ov);
+  std::memcpy(out, ov.data(), sizeof(double) * ov.size());
+}
+
+// BiCGSTAB on the leaf grid from x (in/out) to RMS residual tol; returns the iterations or -1.
+int orc_poisson_leaf_solve(void *mesh, double D, double alpha, const double *b, double *x,
+                           double tol, int maxit, double *rms) {
+  const Mesh &m = *static_cast<OMesh *>(mesh)->m;
+  poisson::LeafProblem P(m, D, alpha);
+  std::vector<double> bv(b, b + P.Cells()), xv(x, x + P.Cells());
+  const int it = P.Solve(bv, xv, tol, maxit, rms);
+  std::memcpy(x, xv.data(), sizeof(double) * xv.size());
+  return it;
 }
 
 }  // extern "C"

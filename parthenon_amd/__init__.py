@@ -91,7 +91,7 @@ class MgParamsC(C.Structure):
     """pth_mg_params of include/parthenon_amd.h."""
     _fields_ = [("max_iters", C.c_int), ("residual_tolerance", C.c_double),
                 ("precision", C.c_int), ("inner_cycles", C.c_int), ("omega", C.c_double),
-                ("pre_sweeps", C.c_int), ("post_sweeps", C.c_int)]
+                ("pre_sweeps", C.c_int), ("post_sweeps", C.c_int), ("smoother", C.c_int)]
 
 
 _SIGNATURES = {
@@ -184,6 +184,7 @@ _SIGNATURES = {
     "pth_mg_solve": (_I, [_P, _P]),
     "pth_mg_result": (_I, [_P, C.POINTER(_I), _DP]),
     "pth_mg_history": (_I, [_P, _DP, _I]),
+    "pth_mg_levels": (_I, [_P]),
     "pth_gmg_server_create": (_P, [_I, _D, _I, _I, _D, _I, _I64, _I, _I]),
     "pth_gmg_server_destroy": (None, [_P]),
     "pth_gmg_server_submit": (_I, [_P, _I, C.c_uint64, _D, C.POINTER(_I)]),

@@ -140,7 +140,7 @@ pth_status Alloc(double **p, std::int64_t n) {
 pth_status AllocStore(const Mesh &m, const FieldSpec &f, FieldStore &s) {
   const std::int64_t nb = static_cast<std::int64_t>(m.topo.blocks.size());
   s.ndata = nb * f.ncomp * m.topo.fine.cells();
-  s.ncoarse = m.topo.multilevel ? nb * f.ncomp * m.topo.coarse.cells() : 0;
+  s.ncoarse = (m.topo.multilevel || m.force_coarse) ? nb * f.ncomp * m.topo.coarse.cells() : 0;
   s.nflux = (f.flags & PTH_WITH_FLUXES) ? 3 * s.ndata : 0;
   if (Alloc(&s.data, s.ndata) != PTH_COMPLETE) return PTH_FAIL;
   if (Alloc(&s.coarse, s.ncoarse) != PTH_COMPLETE) return PTH_FAIL;

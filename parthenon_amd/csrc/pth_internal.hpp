@@ -398,6 +398,7 @@ class Mesh {
   hipStream_t stream = nullptr;
   int pack_mode = PTH_PACK_PERSISTENT;
   int generation = 0;      // counts remeshes (packs built before one are rebuilt)
+  bool force_coarse = false;  // coarse buffers on a single-level mesh (multigrid levels)
   bool planned = false;    // host topology built
   bool finalized = false;  // device storage, tables and buffers built
   double *scratch = nullptr;                    // reduction partials

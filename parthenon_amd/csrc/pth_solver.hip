@@ -37,6 +37,7 @@
 #include "pth_device.hpp"
 #include "pth_gmg.hpp"
 #include "pth_internal.hpp"
+#include "pth_mgtree.hpp"
 
 namespace pth {
 namespace {
@@ -232,7 +233,9 @@ struct MgMesh {
   int iterations = 0;
   double rms = 0.0;
   std::vector<double> history;  // RMS residual before every V-cycle and after the last
+  MgTree *tree = nullptr;       // block-mesh hierarchy (meshes other than the uniform cube)
   ~MgMesh() {
+    if (tree) MgTreeDestroy(tree);
     if (map) (void)hipFree(map);
     if (ev) (void)hipEventDestroy(ev);
   }
@@ -413,29 +416,14 @@ pth_mg *pth_mg_create(pth_mesh *mh, const char *u, const char *rhs, const pth_mg
   }
   if (p->max_iters < 1 || !(p->residual_tolerance >= 0.0) ||
       (p->precision != 0 && p->precision != 1) || p->inner_cycles < 1 || p->pre_sweeps < 0 ||
-      p->post_sweeps < 0 || !(p->omega > 0.0)) {
+      p->post_sweeps < 0 || !(p->omega > 0.0) || p->smoother < 0 || p->smoother > 3) {
     SetError("MGSolver: bad parameters");
     return nullptr;
   }
-  if (!(eq->D > 0.0) || eq->diagonal_alpha != 0.0) {
-    SetError("MGSolver: the equation needs a constant D > 0 and diagonal_alpha = 0");
+  if (!(eq->D > 0.0) || !(eq->diagonal_alpha >= 0.0)) {
+    SetError("MGSolver: the equation needs a constant D > 0 and diagonal_alpha >= 0");
     return nullptr;
   }
-  const Topology &t = m->topo;
-  if (t.ndim != 3 || t.multilevel || t.nx[0] != t.nx[1] || t.nx[0] != t.nx[2]) {
-    SetError("MGSolver: needs a uniform single-level 3-D mesh of N^3 cells");
-    return nullptr;
-  }
-  const double L = t.xmax[0] - t.xmin[0];
-  if (t.xmax[1] - t.xmin[1] != L || t.xmax[2] - t.xmin[2] != L) {
-    SetError("MGSolver: needs a cubic domain");
-    return nullptr;
-  }
-  for (int face = 0; face < 6; ++face)
-    if (t.bc[face] != PTH_BC_FIXED_FACE) {
-      SetError("MGSolver: every domain face must be fixed_face (FixedFace, value 0)");
-      return nullptr;
-    }
   const int fu = m->FieldIndex(u ? u : ""), fr = m->FieldIndex(rhs ? rhs : "");
   if (fu < 0 || fr < 0 || fu == fr) {
     SetError("MGSolver: u and rhs must be two fields of the mesh");
@@ -445,6 +433,37 @@ pth_mg *pth_mg_create(pth_mesh *mh, const char *u, const char *rhs, const pth_mg
       !(m->fields[fu].flags & PTH_FILL_GHOST)) {
     SetError("MGSolver: u (FillGhost) and rhs must have one component");
     return nullptr;
+  }
+  const Topology &t = m->topo;
+  const double L = t.xmax[0] - t.xmin[0];
+  bool cube = t.ndim == 3 && !t.multilevel && t.nx[0] == t.nx[1] && t.nx[0] == t.nx[2] &&
+              t.xmax[1] - t.xmin[1] == L && t.xmax[2] - t.xmin[2] == L &&
+              eq->diagonal_alpha == 0.0 && p->smoother == 0;
+  for (int face = 0; face < 6; ++face) cube = cube && t.bc[face] == PTH_BC_FIXED_FACE;
+  if (!cube) {
+    // block-mesh hierarchy, FP64
+    if (p->precision != 1) {
+      SetError("MGSolver: mixed precision needs a uniform N^3 cube with fixed_face faces and "
+               "the weighted-Jacobi smoother; use precision 1 on this mesh");
+      return nullptr;
+    }
+    for (int face = 0; face < 2 * t.ndim; ++face)
+      if (t.bc[face] != PTH_BC_FIXED_FACE && t.bc[face] != PTH_BC_PERIODIC) {
+        SetError("MGSolver: every face must be fixed_face or periodic (the poisson_gmg "
+                 "package's FixedFace conditions)");
+        return nullptr;
+      }
+    auto *h = new pth_mg();
+    h->mesh = m;
+    h->fu = fu;
+    h->frhs = fr;
+    h->P = *p;
+    h->tree = MgTreeCreate(m, fu, fr, *p, eq->D, eq->diagonal_alpha);
+    if (!h->tree) {
+      delete h;
+      return nullptr;
+    }
+    return h;
   }
   auto *h = new pth_mg();
   h->mesh = m;
@@ -483,6 +502,10 @@ pth_status pth_mg_setup(pth_mg *h, pth_meshdata *mdh) {
     SetError("MGSolver::AddSetupTasks: MeshData of another mesh");
     return PTH_FAIL;
   }
+  if (h->tree) {
+    h->ready = MgTreeSetup(h->tree, *md) == PTH_COMPLETE;
+    return h->ready ? PTH_COMPLETE : PTH_FAIL;
+  }
   if (h->Map() != PTH_COMPLETE) return PTH_FAIL;
   Mesh *m = h->mesh;
   const Shape s = ShapeOf(m->topo.fine);
@@ -511,6 +534,13 @@ pth_status pth_mg_solve(pth_mg *h, pth_meshdata *mdh) {
   if (!h->ready) {
     SetError("MGSolver::AddTasks: the setup has not run");
     return PTH_FAIL;
+  }
+  if (h->tree) {
+    h->ready = false;
+    if (MgTreeSolve(h->tree, *md, &h->iterations, &h->history) != PTH_COMPLETE)
+      return PTH_FAIL;
+    h->rms = h->history.back();
+    return pth_exchange_fused(mdh, PTH_BT_ANY);
   }
   if (h->Map() != PTH_COMPLETE) return PTH_FAIL;
   Mesh *m = h->mesh;
@@ -549,6 +579,12 @@ pth_status pth_mg_result(const pth_mg *h, int *iterations, double *rms_residual)
   if (iterations) *iterations = h->iterations;
   if (rms_residual) *rms_residual = h->rms;
   return PTH_COMPLETE;
+}
+
+int pth_mg_levels(const pth_mg *h) {
+  if (!h) return 0;
+  if (h->tree) return MgTreeLevels(h->tree);
+  return h->f ? static_cast<int>(h->f->p.sizes.size()) : static_cast<int>(h->d->p.sizes.size());
 }
 
 int pth_mg_history(const pth_mg *h, double *out, int max) {

@@ -172,21 +172,26 @@ class PoissonEquation:  # example/poisson_gmg/poisson_equation.hpp:41-205, const
         _check(lib().pth_poisson_ax(md._h, C.byref(e), x.encode(), out.encode()), "Ax")
 
 
+_SMOOTHER = {"jacobi": 0, "SRJ1": 1, "SRJ2": 2, "SRJ3": 3}
+
+
 @dataclass
 class MeshMGParams:  # src/solvers/mg_solver.hpp:37-58, with the smoother explicit
     max_iters: int = 1000
     residual_tolerance: float = 1e-12   # RMS residual sqrt(sum r^2 / ncells), as the reference
-    precision: str = "mixed"
+    precision: str = "mixed"            # "fp64" on meshes other than the uniform N^3 cube
     inner_cycles: int = 2               # FP32 V-cycles per FP64 refinement step (mixed)
-    omega: float = 2.0 / 3.0
+    omega: float = 2.0 / 3.0            # weighted Jacobi ("jacobi")
     pre_sweeps: int = 2
     post_sweeps: int = 2
+    smoother: str = "jacobi"            # or the reference's "SRJ1" / "SRJ2" / "SRJ3"
 
 
 class MeshMGSolver:
-    """MGSolver<u, rhs, PoissonEquation> over the blocks of a MeshData (mg_solver.hpp:37-203):
-    setup() is AddSetupTasks, solve() is AddTasks run to completion. The mesh is uniform,
-    3-D, N^3 cells on a cube, fixed_face on every face."""
+    """MGSolver<u, rhs, PoissonEquation> over the blocks of a MeshData (mg_solver.hpp:37-520):
+    setup() is AddSetupTasks, solve() is AddTasks run to completion. A uniform 3-D N^3 cube
+    with fixed_face faces runs on whole-level arrays (mixed precision or FP64); any other mesh
+    (static refinement, 1-D / 2-D, periodic faces) on a hierarchy of block meshes in FP64."""
 
     def __init__(self, mesh, u, rhs, params=None, equation=None):
         from . import MgParamsC
@@ -194,7 +199,7 @@ class MeshMGSolver:
         self.equation = equation or PoissonEquation()
         p = self.params
         cp = MgParamsC(p.max_iters, p.residual_tolerance, _PREC[p.precision], p.inner_cycles,
-                       p.omega, p.pre_sweeps, p.post_sweeps)
+                       p.omega, p.pre_sweeps, p.post_sweeps, _SMOOTHER[p.smoother])
         e = self.equation._c()
         h = lib().pth_mg_create(mesh._h, u.encode(), rhs.encode(), C.byref(cp), C.byref(e))
         if not h:
@@ -224,3 +229,7 @@ class MeshMGSolver:
         buf = (C.c_double * 1024)()
         n = lib().pth_mg_history(self._h, buf, 1024)
         return np.array(buf[:min(n, 1024)])
+
+    def levels(self):
+        """Levels of the hierarchy, finest first."""
+        return lib().pth_mg_levels(self._h)

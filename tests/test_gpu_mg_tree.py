@@ -1,0 +1,176 @@
+"""GPU parity of MGSolver<u, rhs, PoissonEquation> on statically refined and 2-D meshes (the
+block-mesh hierarchy), through the C ABI.
+
+* The reference's own regression deck (tst/regression/test_suites/poisson_gmg/
+  parthinput.poisson): 2-D, 64^2 cells on [-1, 1]^2 in 32^2 blocks, static refinement of
+  [-1, -0.75]^2 to logical level 3 (two levels above the base grid), FixedFace faces, SRJ2,
+  max_iterations 15, residual_tolerance 1e-12, use_exact_rhs (rhs = A exact with
+  exact = -exp(-10 r^2), poisson_driver.cpp:87-95, parthenon_app_inputs.cpp:53-75). rhs is
+  bit-equal to the oracle's leaf operator applied to exact; the solve passes the reference's
+  thresholds (RMS residual <= 1e-10, RMS error <= 1e-12, example/poisson_gmg/main.cpp:52)
+  and matches the oracle's BiCGSTAB solution of the same leaf problem.
+* The deck's own right-hand side (1 inside r < 0.5, parthenon_app_inputs.cpp:53-68) and a 3-D
+  sphere-refined mesh: the solution against the oracle's, 1e-9 relative RMS, residual
+  recomputed with the oracle's leaf operator below the tolerance.
+"""
+import numpy as np
+import pytest
+
+import parthenon_amd as pa
+from oracle_lib import OracleMesh
+from parthenon_amd.gmg import MeshMGParams, MeshMGSolver, PoissonEquation
+
+pytestmark = pytest.mark.gpu
+
+DECK_BOX = ((-1.0, -0.75, -1.0, -0.75, 0.0, 1.0), 2)
+
+
+def make_meshes(nx, bnx, xmin, xmax, boxes=(), spheres=(), ng=2):
+    nd = sum(1 for v in nx if v > 1)
+    bcs = tuple(("fixed_face" if d < nd else "periodic") for d in range(3) for _ in range(2))
+    periodic = tuple(0 if d < nd else 1 for d in range(3))
+    m = pa.Mesh(nx, bnx, ng, xmin=xmin, xmax=xmax, periodic=periodic, bcs=bcs,
+                refine_boxes=boxes, refine_spheres=spheres)
+    m.add_field("u", 1, pa.Metadata.FillGhost | pa.Metadata.Independent)
+    m.register_refinement_ops("u", "linear")  # poisson_package.cpp:122-126
+    m.add_field("rhs", 1)
+    m.add_field("Ax", 1)
+    m.finalize()
+    om = OracleMesh(nx, bnx, ng, xmin=xmin, xmax=xmax, periodic=periodic, bcs=bcs,
+                    boxes=boxes, spheres=spheres)
+    assert om.nblocks == m.nblocks
+    for gid in range(m.nblocks):
+        a, b = m.block(gid), om.block(gid)
+        assert (a["level"], tuple(a["l"])) == (b["level"], tuple(b["l"]))
+    return m, om
+
+
+def centres(m):
+    """Cell centres of every interior cell, block order then [k][j][i]."""
+    n = m.n
+    out = []
+    for gid in range(m.nblocks):
+        b = m.block(gid)
+        lo, hi = b["xmin"], b["xmax"]
+        ax = [lo[d] + (np.arange(n[d]) + 0.5) * (hi[d] - lo[d]) / n[d] for d in range(3)]
+        z, y, x = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+        out.append(np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1))
+    return np.concatenate(out)
+
+
+def to_bricks(m, v, name="u", fill=0.0):
+    out = np.full(m.field_shape(name), fill)
+    n, g = m.n, m.ng
+    per = n[0] * n[1] * n[2]
+    for gid in range(m.nblocks):
+        out[gid, 0, g[2]:g[2] + n[2], g[1]:g[1] + n[1], g[0]:g[0] + n[0]] = \
+            v[gid * per:(gid + 1) * per].reshape(n[2], n[1], n[0])
+    return out
+
+
+def from_bricks(m, f):
+    n, g = m.n, m.ng
+    return np.concatenate([f[gid, 0, g[2]:g[2] + n[2], g[1]:g[1] + n[1], g[0]:g[0] + n[0]]
+                           .ravel() for gid in range(m.nblocks)])
+
+
+def rms(v):
+    return float(np.sqrt(np.mean(np.square(v))))
+
+
+def deck():
+    return make_meshes((64, 64, 1), (32, 32, 1), (-1.0, -1.0, 0.0), (1.0, 1.0, 1.0),
+                       boxes=[DECK_BOX])
+
+
+def test_deck_use_exact_rhs_passes_reference_thresholds():
+    print("case poisson_gmg deck, use_exact_rhs", flush=True)
+    m, om = deck()
+    md = m.mesh_data("base")
+    c = centres(m)
+    r = np.sqrt(c[:, 0] ** 2 + c[:, 1] ** 2)  # ndim = 2: x3 does not enter the radius
+    exact = -np.exp(-10.0 * r * r)
+    # rhs <- A exact: CopyData<exact, u>, boundary exchange, Ax<u, rhs>
+    md.upload("u", to_bricks(m, exact, fill=np.nan))
+    pa.exchange_fused(md)
+    PoissonEquation().Ax(md, "u", "rhs")
+    rhs = from_bricks(m, md.download("rhs"))
+    ref_rhs = om.poisson_apply(exact)
+    assert np.array_equal(rhs, ref_rhs), np.max(np.abs(rhs - ref_rhs))
+    md.upload("u", np.zeros(m.field_shape("u")))
+    p = MeshMGParams(max_iters=15, residual_tolerance=1e-12, precision="fp64",
+                     smoother="SRJ2")
+    s = MeshMGSolver(m, "u", "rhs", p)
+    # leaf level, the two capped levels, the merged tree level, 16^2 .. 2^2 blocks
+    assert s.levels() == 3 + 1 + 4
+    s.setup(md)
+    it, res = s.solve(md)
+    h = s.history()
+    print(f"deck: {it} V-cycles, rms residual {res:.3e}, history {h}", flush=True)
+    u = from_bricks(m, md.download("u"))
+    err = rms(u - exact)
+    print(f"deck: rms error {err:.3e}", flush=True)
+    assert res <= 1e-10 and err <= 1e-12  # example/poisson_gmg/main.cpp:52
+    assert res == pytest.approx(rms(ref_rhs - om.poisson_apply(u)), rel=1e-6, abs=1e-15)
+    xo, _, _ = om.poisson_solve(ref_rhs, tol=1e-13)
+    assert rms(u - xo) <= 1e-12
+    assert np.all(np.diff(h[:it]) < 0)  # monotone until the tolerance
+
+
+@pytest.mark.parametrize("smoother", ["SRJ2", "jacobi"])
+def test_deck_rhs_matches_oracle(smoother):
+    print(f"case poisson_gmg deck rhs, {smoother}", flush=True)
+    m, om = deck()
+    md = m.mesh_data("base")
+    c = centres(m)
+    b = np.where(np.sqrt(c[:, 0] ** 2 + c[:, 1] ** 2) < 0.5, 1.0, 0.0)
+    md.upload("rhs", to_bricks(m, b, name="rhs"))
+    md.upload("u", np.zeros(m.field_shape("u")))
+    p = MeshMGParams(max_iters=60, residual_tolerance=1e-11, precision="fp64",
+                     smoother=smoother)
+    s = MeshMGSolver(m, "u", "rhs", p)
+    s.setup(md)
+    it, res = s.solve(md)
+    print(f"deck rhs {smoother}: {it} V-cycles, rms residual {res:.3e}", flush=True)
+    assert res <= 1e-11
+    u = from_bricks(m, md.download("u"))
+    assert rms(b - om.poisson_apply(u)) <= 1.01e-11
+    xo, _, _ = om.poisson_solve(b, tol=1e-13)
+    assert rms(u - xo) <= 1e-9 * rms(xo)
+    # u's ghosts were refilled by the boundary exchange after the solve
+    full = md.download("u")
+    md.upload("u", full)
+    pa.exchange_fused(md)
+    assert np.array_equal(md.download("u"), full)
+
+
+def test_sphere_refined_3d_matches_oracle():
+    print("case 3-D sphere-refined mesh", flush=True)
+    m, om = make_meshes((32, 32, 32), (8, 8, 8), (-0.5,) * 3, (0.5,) * 3,
+                        spheres=[((0.1, -0.05, 0.0), 0.15, 1)])
+    md = m.mesh_data("base")
+    c = centres(m)
+    rng = np.random.default_rng(11)
+    b = np.zeros(len(c))
+    for _ in range(4):
+        x0 = rng.uniform(-0.3, 0.3, 3)
+        b += rng.choice([-1.0, 1.0]) * np.exp(-np.sum((c - x0) ** 2, axis=1) / 0.01)
+    md.upload("rhs", to_bricks(m, b, name="rhs"))
+    md.upload("u", np.zeros(m.field_shape("u")))
+    p = MeshMGParams(max_iters=60, residual_tolerance=1e-10, precision="fp64",
+                     smoother="SRJ2")
+    s = MeshMGSolver(m, "u", "rhs", p)
+    s.setup(md)
+    it, res = s.solve(md)
+    print(f"3-D: {it} V-cycles, rms residual {res:.3e}, levels {s.levels()}", flush=True)
+    assert res <= 1e-10
+    u = from_bricks(m, md.download("u"))
+    assert rms(b - om.poisson_apply(u)) <= 1.01e-10
+    xo, _, _ = om.poisson_solve(b, tol=1e-13)
+    assert rms(u - xo) <= 1e-8 * rms(xo)
+
+
+def test_mixed_precision_is_refused_on_refined_meshes():
+    m, _ = deck()
+    with pytest.raises(RuntimeError, match="mixed precision"):
+        MeshMGSolver(m, "u", "rhs", MeshMGParams(precision="mixed"))
