@@ -9,10 +9,12 @@
 //     it, the reference's gmg_restrict_send to gmg_coarser_neighbors), every other block is
 //     the same block on both levels (the reference shares those MeshBlocks between its grids:
 //     gmg_leaf_neighbors, same_to_same buffers, bvals_utils.hpp:49-63);
-//   * below the base grid, the tree levels of the forest (2^d base blocks merge into one
-//     while every base block count is even), then levels whose blocks keep their positions
-//     and halve their cells (the reference's negative logical levels) while every block size
-//     is even and stays >= nghost.
+//   * below the base grid, the tree levels of the forest (2^d base blocks merge into one, as
+//     many times as the forest's root level, forest.cpp:64-105), then levels whose blocks keep
+//     their positions and halve their cells (the reference's negative logical levels) while
+//     every block size is even, down to one cell per block (one ghost layer on these levels)
+//     and two cells per dimension for the level (where the reference's single-tree forest
+//     ends with one cell).
 // Every level is a full tiling of the domain, so each level's boundary exchange is the leaf
 // exchange of that level's mesh (same-level copies, restriction into coarse buffers and
 // linear prolongation at fine-coarse faces, FixedFace conditions). The reference's grids
@@ -303,31 +305,39 @@ pth_status MgTree::Build() {
     if (NewLevel(t, true, xp) != PTH_COMPLETE) return PTH_FAIL;
     lv.back()->to_coarser = Coarsen::kCap;
   }
-  // below the base grid: merge base blocks, then halve the cells
-  int nx[3], bnx[3];
+  // below the base grid: the forest's tree levels (root_level merges of 2^d base blocks,
+  // forest.cpp:64-105), then levels that halve every block's cells as many times as the block
+  // size has trailing binary zeros (mesh-gmg.cpp:106-118), with one ghost layer
+  int nx[3], bnx[3], ng = ut.ng;
   for (int d = 0; d < 3; ++d) {
     nx[d] = ut.nx[d];
     bnx[d] = ut.bnx[d];
   }
-  for (;;) {
-    bool merge = true, shrink = true, any = false;
-    for (int d = 0; d < 3; ++d) {
-      if (!ut.active[d]) continue;
-      any = true;
-      const int nb = nx[d] / bnx[d];
-      merge = merge && nb % 2 == 0;
-      shrink = shrink && bnx[d] % 2 == 0 && bnx[d] / 2 >= ut.ng;
-    }
-    if (!any || (!merge && !shrink)) break;
+  int merges = ut.RootLevel(), shrinks = 1 << 30;
+  for (int d = 0; d < 3; ++d) {
+    if (!ut.active[d]) continue;
+    int z = 0;
+    while (((bnx[d] >> z) & 1) == 0) ++z;
+    shrinks = std::min(shrinks, z);
+  }
+  if (shrinks == (1 << 30)) shrinks = 0;
+  for (int q = 0; q < merges + shrinks; ++q) {
+    const bool merge = q < merges;
+    // a level keeps at least two cells along every active dimension (a mesh of one cell
+    // along a dimension is a mesh without that dimension here)
+    bool fits = true;
+    for (int d = 0; d < 3; ++d) fits = fits && (!ut.active[d] || nx[d] / 2 >= 2);
+    if (!fits) break;
     lv.back()->to_coarser = merge ? Coarsen::kMerge : Coarsen::kShrink;
     for (int d = 0; d < 3; ++d) {
       if (!ut.active[d]) continue;
       nx[d] /= 2;
       if (!merge) bnx[d] /= 2;
     }
+    if (!merge) ng = 1;
     Topology t;
     const int per[3] = {ut.periodic[0], ut.periodic[1], ut.periodic[2]};
-    const std::string err = t.Init(nx, bnx, ut.ng, ut.xmin, ut.xmax, per);
+    const std::string err = t.Init(nx, bnx, ng, ut.xmin, ut.xmax, per);
     if (!err.empty()) {
       SetError("MGSolver hierarchy: " + err);
       return PTH_FAIL;
@@ -393,17 +403,25 @@ pth_status MgTree::BuildTransfers(int l) {
       src_lo[d] = dst[d] - ce.g[d];
     }
     F.copy_down.host.push_back(Rec(r_c, b_C + Offset(Ce, dst[0], dst[1], dst[2]), ci, fb, ce, &Ce));
-    // the target's cells over the block and one coarse ghost band -> t's coarse buffer
+    // the target's cells over the block and a ghost band of m >= 1 cells (the prolongation
+    // stencil reads one) -> t's coarse buffer
     Box sb;
     for (int d = 0; d < 3; ++d) {
-      sb.lo[d] = src_lo[d];
-      sb.hi[d] = src_lo[d] + ce.ent(d) - 1;
-      if (sb.lo[d] < 0 || sb.hi[d] >= Ce.ent(d)) {
-        SetError("MGSolver hierarchy: prolongation source outside the coarser block");
+      const int m = std::min(ce.g[d], Ce.g[d]);
+      sb.lo[d] = src_lo[d] + ce.g[d] - m;
+      sb.hi[d] = sb.lo[d] + ce.n[d] + 2 * m - 1;
+      if (sb.lo[d] < 0 || sb.hi[d] >= Ce.ent(d) || (ft.active[d] && m < 1)) {
+        SetError("MGSolver hierarchy: prolongation source outside the coarser block (level " +
+                 std::to_string(l) + ", block " + std::to_string(fb) + ", dimension " +
+                 std::to_string(d) + ": cells " + std::to_string(sb.lo[d]) + ".." +
+                 std::to_string(sb.hi[d]) + " of " + std::to_string(Ce.ent(d)) + ")");
         return PTH_FAIL;
       }
     }
-    F.copy_up.host.push_back(Rec(x_C, t_c, sb, cbk, Ce, &ce));
+    const int m0 = std::min(ce.g[0], Ce.g[0]), m1 = std::min(ce.g[1], Ce.g[1]),
+              m2 = std::min(ce.g[2], Ce.g[2]);
+    F.copy_up.host.push_back(
+        Rec(x_C, t_c + Offset(ce, ce.g[0] - m0, ce.g[1] - m1, ce.g[2] - m2), sb, cbk, Ce, &ce));
     F.prolong_t.host.push_back(Rec(t_f, t_c, ci, fb, fe, &ce));
   }
   for (Table *q : {&F.restrict_r, &F.copy_down, &F.copy_up, &F.prolong_t})

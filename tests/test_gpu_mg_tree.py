@@ -3,7 +3,8 @@ block-mesh hierarchy), through the C ABI.
 
 * The reference's own regression deck (tst/regression/test_suites/poisson_gmg/
   parthinput.poisson): 2-D, 64^2 cells on [-1, 1]^2 in 32^2 blocks, static refinement of
-  [-1, -0.75]^2 to logical level 3 (two levels above the base grid), FixedFace faces, SRJ2,
+  [-1, -0.75]^2 to logical level 3 (three levels above the base grid: a 2-D forest has root
+  level 0), FixedFace faces, SRJ2,
   max_iterations 15, residual_tolerance 1e-12, use_exact_rhs (rhs = A exact with
   exact = -exp(-10 r^2), poisson_driver.cpp:87-95, parthenon_app_inputs.cpp:53-75). rhs is
   bit-equal to the oracle's leaf operator applied to exact; the solve passes the reference's
@@ -12,7 +13,12 @@ block-mesh hierarchy), through the C ABI.
 * The deck's own right-hand side (1 inside r < 0.5, parthenon_app_inputs.cpp:53-68) and a 3-D
   sphere-refined mesh: the solution against the oracle's, 1e-9 relative RMS, residual
   recomputed with the oracle's leaf operator below the tolerance.
+* tests/cpp/poisson_deck_driver on the deck file itself (tests/golden/parthinput.poisson):
+  the same thresholds and the oracle's solution, through the C++ task-level mirror.
 """
+import os
+import subprocess
+
 import numpy as np
 import pytest
 
@@ -22,7 +28,9 @@ from parthenon_amd.gmg import MeshMGParams, MeshMGSolver, PoissonEquation
 
 pytestmark = pytest.mark.gpu
 
-DECK_BOX = ((-1.0, -0.75, -1.0, -0.75, 0.0, 1.0), 2)
+# <parthenon/static_refinement0> of the deck: logical level 3; a 2-D forest has root level 0
+# (forest.cpp:74-80), so three levels above the base grid
+DECK_BOX = ((-1.0, -0.75, -1.0, -0.75, 0.0, 1.0), 3)
 
 
 def make_meshes(nx, bnx, xmin, xmax, boxes=(), spheres=(), ng=2):
@@ -101,8 +109,9 @@ def test_deck_use_exact_rhs_passes_reference_thresholds():
     p = MeshMGParams(max_iters=15, residual_tolerance=1e-12, precision="fp64",
                      smoother="SRJ2")
     s = MeshMGSolver(m, "u", "rhs", p)
-    # leaf level, the two capped levels, the merged tree level, 16^2 .. 2^2 blocks
-    assert s.levels() == 3 + 1 + 4
+    # the reference's GMG levels -5 .. 3 (mesh-gmg.cpp:103-118): the leaf mesh, the mesh capped
+    # at logical levels 2, 1, 0, then blocks of 16^2 .. 1^2 cells
+    assert m.nblocks == 13 and s.levels() == 9
     s.setup(md)
     it, res = s.solve(md)
     h = s.history()
@@ -174,3 +183,27 @@ def test_mixed_precision_is_refused_on_refined_meshes():
     m, _ = deck()
     with pytest.raises(RuntimeError, match="mixed precision"):
         MeshMGSolver(m, "u", "rhs", MeshMGParams(precision="mixed"))
+
+
+def test_cpp_deck_driver_runs_reference_deck(tmp_path):
+    """tests/cpp/poisson_deck_driver reads the reference's regression deck (copied unchanged
+    into tests/golden) and runs PoissonDriver's task collection through the C++ mirror."""
+    print("case cpp deck driver on parthinput.poisson", flush=True)
+    here = os.path.dirname(os.path.abspath(__file__))
+    exe = os.path.join(here, "cpp", "poisson_deck_driver")
+    out = str(tmp_path / "deck.bin")
+    r = subprocess.run([exe, os.path.join(here, "golden", "parthinput.poisson"), out],
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    print(r.stdout.strip(), flush=True)
+    raw = np.fromfile(out, dtype=np.float64)
+    res, err, its, nb, per = raw[:5]
+    n = int(nb) * int(per)
+    u, rhs, exact = (raw[5 + q * n:5 + (q + 1) * n] for q in range(3))
+    # the reference's pass test (example/poisson_gmg/main.cpp:52), within its 15 V-cycles
+    assert res <= 1e-10 and err <= 1e-12 and its <= 15
+    _, om = deck()
+    assert np.array_equal(rhs, om.poisson_apply(exact))
+    assert abs(err - rms(u - exact)) <= 1e-6 * err
+    xo, _, _ = om.poisson_solve(rhs, tol=1e-13)
+    assert rms(u - xo) <= 1e-12
