@@ -375,35 +375,43 @@ pth_status pth_solver_set_to_zero(pth_meshdata *md, const char *var);
 pth_status pth_solver_dot_product(pth_meshdata *md, const char *a, const char *b, double *out);
 
 /* ---- the Poisson equation over MeshData (example/poisson_gmg/poisson_equation.hpp:41-205) ----
- * div(D grad u) - diagonal_alpha u = rhs with a constant D (the reference's face field D held
- * constant, poisson_package.cpp:112-118) and the package's diagonal_alpha (:83-84). */
+ * div(D grad u) - diagonal_alpha u = rhs with the package's diagonal_alpha (:83-84) and D either
+ * constant or the reference's face field D (Metadata::Face, poisson_package.cpp:112-118):
+ * D_field names a 3-component cell field whose component d at cell (i, j, k) holds D on the
+ * cell's lower face along dimension d (the reference's pack(b, TE::F(d+1), D(), k, j, i)); a
+ * block's upper faces along d sit in its first ghost cell along d, so fill D over the whole
+ * brick (the problem generator's IndexDomain::entire, parthenon_app_inputs.cpp:44-78). */
 typedef struct pth_poisson_equation {
-  double D;
+  double D;              /* constant D (D_field null) */
   double diagonal_alpha;
+  const char *D_field;   /* null, or the face field D */
 } pth_poisson_equation;
 /* PoissonEquation::Ax<x, out> (:41-53): CalculateFluxes (:131-184) then FluxMultiplyMatrix
  * (:186-205) on the interior cells of every block of md, without flux correction:
- * out = -alpha x + sum_d (F_d(lo) - F_d(hi)) / dx_d from x's cells and one ghost layer (fill
- * them first: the boundary exchange with its conditions). x and out: distinct one-component
- * fields. The x3 face flux is the 7-point operator's (the reference stores the top x3 flux in
+ * out = -alpha x + sum_d (F_d(lo) - F_d(hi)) / dx_d, F_d = D_d(face) (x(lo side) - x) / dx_d,
+ * from x's cells and one ghost layer (fill them first: the boundary exchange with its
+ * conditions). x and out: distinct one-component fields. The x3 face flux is the 7-point operator's (the reference stores the top x3 flux in
  * the x2 slot, :153-157). */
 pth_status pth_poisson_ax(pth_meshdata *md, const pth_poisson_equation *eq, const char *x,
                           const char *out);
 
 /* ---- MGSolver<u, rhs, PoissonEquation> over MeshData (src/solvers/mg_solver.hpp:37-520) ----
- * u (FillGhost) and rhs: one-component fields; constant D > 0 and diagonal_alpha >= 0.
+ * u (FillGhost) and rhs: one-component fields; D > 0 (constant, or the face field D) and
+ * diagonal_alpha >= 0.
  * Two hierarchies, both stopping on the RMS residual of the MeshData problem:
  * - uniform single-level 3-D meshes of N^3 cells on a cube with fixed_face (FixedFace) on all
  *   six faces and smoother 0: whole-level arrays (cell counts halving while even, as
  *   src/mesh/mesh-gmg.cpp:103-154 gives on such a mesh), FP32 V-cycles inside FP64 iterative
  *   refinement (precision 0) or FP64 (precision 1); the problem is solved on the unit cube
  *   with rhs scaled by L^2 / D (exact when L = D = 1), diagonal_alpha = 0;
- * - every other mesh (static refinement, 1-D / 2-D, periodic or fixed_face faces, or
- *   smoother 1-3): a hierarchy of block meshes (the leaf mesh, the mesh capped at each coarser
- *   refinement level, then the forest's merged and cell-halved levels below the base grid) in
- *   FP64 (precision 1 only); each level's ghosts come from its own boundary exchange with
- *   the faces' conditions, restriction is RestrictAverage and prolongation of the correction
- *   ProlongateSharedLinear. u's registered prolongation operator defines the leaf problem's
+ * - every other mesh or equation (static refinement, 1-D / 2-D, periodic faces, a face field
+ *   D, diagonal_alpha != 0, or smoother 1-3): a hierarchy of block meshes (the leaf mesh, the
+ *   mesh capped at each coarser refinement level, then the forest's merged and cell-halved
+ *   levels below the base grid) in FP64 (precision 1 only); each level's ghosts come from
+ *   its own boundary exchange with the faces' conditions, restriction is RestrictAverage and
+ *   prolongation of the correction ProlongateSharedLinear; a face field D is read at pth_mg_setup and restricted down the
+ *   levels by face averages (D is GMGRestrict in the reference, AddSetupTasks restricts it,
+ *   mg_solver.hpp:188-203). u's registered prolongation operator defines the leaf problem's
  *   fine-coarse ghosts (the poisson_gmg package registers ProlongateSharedLinear). */
 typedef struct pth_mg_params {
   int max_iters;             /* MGParams::max_iters (mg_solver.hpp:37-58): V-cycles at most */

@@ -509,15 +509,20 @@ TaskID DotProduct(TaskID dependency_in, TaskList &tl, AllReduce<Real> *adotb,
 // fine-coarse faces; the operator here runs on single-level meshes, where it changes nothing.
 class PoissonEquation {
  public:
-  Real D = 1.0;
+  Real D = 1.0;               // constant D, or
+  std::string D_field;        // the face field D: a 3-component field, component d = D on
+                              // the lower face along d (the reference's Metadata::Face D)
   Real diagonal_alpha = 0.0;
   bool do_flux_cor = false;
-  pth_poisson_equation Coefficients() const { return {D, diagonal_alpha}; }
+  pth_poisson_equation Coefficients() const {
+    return {D, diagonal_alpha, D_field.empty() ? nullptr : D_field.c_str()};
+  }
   // Ax<x_t, out_t> (:41-53): out = A x on every block's interior; x's ghosts must be filled
   template <class x_t, class out_t>
   TaskID Ax(TaskList &tl, TaskID depends_on, std::shared_ptr<MeshData<Real>> &md) const {
-    const pth_poisson_equation e = Coefficients();
-    return tl.AddTask(depends_on, [e, md]() {
+    // the task keeps its own copy (the face field's name outlives this object)
+    return tl.AddTask(depends_on, [eq = *this, md]() {
+      const pth_poisson_equation e = eq.Coefficients();
       return detail::Check(
           pth_poisson_ax(md->handle(), &e, x_t::name().c_str(), out_t::name().c_str()), "Ax");
     });

@@ -697,20 +697,24 @@ int orc_amr_burgers_cycles(void *amr, void *burgers, int n) {
 }
 
 // The Poisson problem on the leaf grid of a (possibly refined) mesh: x and out hold the
-// interior cells of every block in block order, then [k][j][i].
-void orc_poisson_leaf_apply(void *mesh, double D, double alpha, const double *x, double *out) {
+// interior cells of every block in block order, then [k][j][i]; Df: the face field D as
+// [block][3] bricks (component d = D on the lower face along d), or null for the constant D.
+void orc_poisson_leaf_apply(void *mesh, double D, double alpha, const double *Df,
+                            const double *x, double *out) {
   const Mesh &m = *static_cast<OMesh *>(mesh)->m;
   poisson::LeafProblem P(m, D, alpha);
+  P.Df = Df;
   std::vector<double> xv(x, x + P.Cells()), ov;
   P.Apply(xv, ov);
   std::memcpy(out, ov.data(), sizeof(double) * ov.size());
 }
 
 // BiCGSTAB on the leaf grid from x (in/out) to RMS residual tol; returns the iterations or -1.
-int orc_poisson_leaf_solve(void *mesh, double D, double alpha, const double *b, double *x,
-                           double tol, int maxit, double *rms) {
+int orc_poisson_leaf_solve(void *mesh, double D, double alpha, const double *Df,
+                           const double *b, double *x, double tol, int maxit, double *rms) {
   const Mesh &m = *static_cast<OMesh *>(mesh)->m;
   poisson::LeafProblem P(m, D, alpha);
+  P.Df = Df;
   std::vector<double> bv(b, b + P.Cells()), xv(x, x + P.Cells());
   const int it = P.Solve(bv, xv, tol, maxit, rms);
   std::memcpy(x, xv.data(), sizeof(double) * xv.size());

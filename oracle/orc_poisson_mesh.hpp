@@ -10,7 +10,7 @@
 //                  FixedFace-0 conditions, :60-78), then PoissonEquation::Ax =
 //                  CalculateFluxes + FluxMultiplyMatrix (poisson_equation.hpp:41-53,
 //                  131-205) without flux correction (the deck's flux_correct = false), D
-//                  constant; the top x3 face flux is the 7-point operator's (the reference
+//                  constant or the face field D; the top x3 face flux is the 7-point operator's (the reference
 //                  writes it into the x2 slot, :153-157)
 //   solve          the fixed point every convergent iteration reaches: A u = rhs on the leaf
 //                  grid, by BiCGSTAB (van der Vorst, SIAM J. Sci. Stat. Comput. 13 (1992)
@@ -29,6 +29,9 @@ namespace poisson {
 struct LeafProblem {
   const Mesh *m;
   double D = 1.0, alpha = 0.0;
+  // the face field D (Metadata::Face, poisson_package.cpp:112-118), or null for the constant
+  // D: [block][3][cellbounds], component d at cell (i, j, k) = D on the lower face along d
+  const double *Df = nullptr;
   Exchange ex;
   std::vector<Field> fs;  // the work field u with ghosts and coarse buffers (the exchange's
                           // regions point into it, so it stays in place)
@@ -83,23 +86,26 @@ struct LeafProblem {
     std::int64_t q = 0;
     for (const auto &b : m->blocks) {
       const double *p = u.data.data() + (std::int64_t)b.gid * s.size();
+      const double *df = Df ? Df + (std::int64_t)b.gid * 3 * s.size() : nullptr;
+      // D on face (d, cell c): the face field's component d, or the constant
+      auto Dat = [&](int d, std::int64_t c) { return df ? df[d * s.size() + c] : D; };
       const double dx1 = b.coords.dx[0], dx2 = b.coords.dx[1], dx3 = b.coords.dx[2];
       for (int k = s.s(2); k <= s.e(2); ++k)
         for (int j = s.s(1); j <= s.e(1); ++j)
           for (int i = s.s(0); i <= s.e(0); ++i) {
             const std::int64_t c = ((std::int64_t)k * s.ent(1) + j) * s.ent(0) + i;
             double r = -alpha * p[c];
-            const double f1l = D / dx1 * (p[c - 1] - p[c]);
-            const double f1r = D / dx1 * (p[c] - p[c + 1]);
+            const double f1l = Dat(0, c) / dx1 * (p[c - 1] - p[c]);
+            const double f1r = Dat(0, c + 1) / dx1 * (p[c] - p[c + 1]);
             r += (f1l - f1r) / dx1;
             if (ndim > 1) {
-              const double f2l = D * (p[c - sj] - p[c]) / dx2;
-              const double f2r = D * (p[c] - p[c + sj]) / dx2;
+              const double f2l = Dat(1, c) * (p[c - sj] - p[c]) / dx2;
+              const double f2r = Dat(1, c + sj) * (p[c] - p[c + sj]) / dx2;
               r += (f2l - f2r) / dx2;
             }
             if (ndim > 2) {
-              const double f3l = D * (p[c - sk] - p[c]) / dx3;
-              const double f3r = D * (p[c] - p[c + sk]) / dx3;
+              const double f3l = Dat(2, c) * (p[c - sk] - p[c]) / dx3;
+              const double f3r = Dat(2, c + sk) * (p[c] - p[c + sk]) / dx3;
               r += (f3l - f3r) / dx3;
             }
             out[q++] = r;

@@ -84,7 +84,7 @@ class PackView(C.Structure):
 
 class PoissonEquationC(C.Structure):
     """pth_poisson_equation of include/parthenon_amd.h."""
-    _fields_ = [("D", C.c_double), ("diagonal_alpha", C.c_double)]
+    _fields_ = [("D", C.c_double), ("diagonal_alpha", C.c_double), ("D_field", C.c_char_p)]
 
 
 class MgParamsC(C.Structure):

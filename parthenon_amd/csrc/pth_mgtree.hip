@@ -55,8 +55,9 @@ namespace pth {
 namespace {
 
 constexpr int kWG = 256;
-enum { kX = 0, kT = 1, kB = 2, kR = 3 };  // level fields: solution/correction, Jacobi
-                                          // output, right-hand side, residual
+enum { kX = 0, kT = 1, kB = 2, kR = 3, kD = 4 };  // level fields: solution/correction,
+                                                  // Jacobi output, right-hand side,
+                                                  // residual, face field D (3 components)
 
 struct BrickShape {
   int e0, e1;
@@ -91,37 +92,44 @@ __device__ __forceinline__ long long Cell(const BrickShape &s, long long t) {
 
 // MODE 0: Jacobi stage out = w (b - (A x - diag x)) / diag + (1 - w) x
 // MODE 1: residual out = 1.0 b + (-1.0) A x (AddFieldsAndStoreInteriorSelect<rhs, Ax, res>)
-// A x in the operation order of CalculateFluxes + FluxMultiplyMatrix, diag as SetDiagonal.
+// A x in the operation order of CalculateFluxes + FluxMultiplyMatrix, diag as SetDiagonal;
+// D on the cell's faces from the face field Df ([block][3] bricks, component d = D on the
+// lower face along d) or the constant D when Df is null.
 template <int MODE>
 __global__ void __launch_bounds__(kWG)
     LevelCellKernel(const double *__restrict__ x, const double *__restrict__ b,
                     double *__restrict__ out, BrickShape s, const BlockCoefs *__restrict__ coef,
-                    double D, double alpha, double w, int ndim, long long total) {
+                    double D, const double *__restrict__ Df, double alpha, double w, int ndim,
+                    long long total) {
   const long long t = static_cast<long long>(blockIdx.x) * kWG + threadIdx.x;
   if (t >= total) return;
   const long long blk = t / (static_cast<long long>(s.n0) * s.n1 * s.n2);
   const long long c = Cell(s, t);
+  const long long p = static_cast<long long>(s.e0) * s.e1;
+  const double *d0 = Df ? Df + blk * 3 * s.cells + (c - blk * s.cells) : nullptr;  // comp 0
   const double u = x[c];
   const double dx1 = coef[blk].dx[0];
+  const double D1l = Df ? d0[0] : D, D1h = Df ? d0[1] : D;
   double ax = -alpha * u;
-  const double f1l = D / dx1 * (x[c - 1] - u);
-  const double f1r = D / dx1 * (u - x[c + 1]);
+  const double f1l = D1l / dx1 * (x[c - 1] - u);
+  const double f1r = D1h / dx1 * (u - x[c + 1]);
   ax += (f1l - f1r) / dx1;
-  double diag = -(D + D) / (dx1 * dx1) - alpha;
+  double diag = -(D1l + D1h) / (dx1 * dx1) - alpha;
   if (ndim > 1) {
+    const double D2l = Df ? d0[s.cells] : D, D2h = Df ? d0[s.cells + s.e0] : D;
     const double dx2 = coef[blk].dx[1];
-    const double f2l = D * (x[c - s.e0] - u) / dx2;
-    const double f2r = D * (u - x[c + s.e0]) / dx2;
+    const double f2l = D2l * (x[c - s.e0] - u) / dx2;
+    const double f2r = D2h * (u - x[c + s.e0]) / dx2;
     ax += (f2l - f2r) / dx2;
-    diag -= (D + D) / (dx2 * dx2);
+    diag -= (D2l + D2h) / (dx2 * dx2);
   }
   if (ndim > 2) {
-    const long long p = static_cast<long long>(s.e0) * s.e1;
+    const double D3l = Df ? d0[2 * s.cells] : D, D3h = Df ? d0[2 * s.cells + p] : D;
     const double dx3 = coef[blk].dx[2];
-    const double f3l = D * (x[c - p] - u) / dx3;
-    const double f3r = D * (u - x[c + p]) / dx3;
+    const double f3l = D3l * (x[c - p] - u) / dx3;
+    const double f3r = D3h * (u - x[c + p]) / dx3;
     ax += (f3l - f3r) / dx3;
-    diag -= (D + D) / (dx3 * dx3);
+    diag -= (D3l + D3h) / (dx3 * dx3);
   }
   if (MODE == 0) {
     const double off_diag = ax - diag * u;
@@ -130,6 +138,60 @@ __global__ void __launch_bounds__(kWG)
   } else {
     out[c] = 1.0 * b[c] + (-1.0) * ax;
   }
+}
+
+// Face-field restriction (RestrictAverage on faces) from a block into its coarser-level
+// counterpart: coarse face (I, J, K) along direction d, with I in [0, nc_d] and the transverse
+// indices in [0, nc), is the mean of the fine faces it covers (2 x 2 in 3-D, 2 in 2-D, one in
+// 1-D), fine face index 2 I along d.
+struct FaceXfer {
+  const double *f;  // the fine block's D brick, component 0
+  double *c;        // the coarse block's D brick, component 0
+  int o[3];         // octant of the fine block in the coarse one
+  int pad;
+};
+struct FaceGeom {
+  int fe0, fe1, ce0, ce1;
+  long long fcells, ccells;
+  int fg[3], cg[3], nc[3], act[3];
+  int ndim;
+};
+
+__global__ void __launch_bounds__(kWG)
+    FaceRestrictKernel(const FaceXfer *__restrict__ recs, FaceGeom G, long long per_rec,
+                       long long total) {
+  const long long t = static_cast<long long>(blockIdx.x) * kWG + threadIdx.x;
+  if (t >= total) return;
+  const FaceXfer R = recs[t / per_rec];
+  long long q = t % per_rec;
+  int P[3];
+  for (int e = 0; e < 3; ++e) P[e] = G.act[e] ? G.nc[e] + 1 : 1;
+  const long long per_d = static_cast<long long>(P[0]) * P[1] * P[2];
+  const int d = static_cast<int>(q / per_d);
+  q %= per_d;
+  int I[3];
+  I[0] = static_cast<int>(q % P[0]);
+  I[1] = static_cast<int>((q / P[0]) % P[1]);
+  I[2] = static_cast<int>(q / (static_cast<long long>(P[0]) * P[1]));
+  for (int e = 0; e < 3; ++e)
+    if (e != d && G.act[e] && I[e] >= G.nc[e]) return;  // padding of a transverse range
+  int f[3], n2[3];
+  for (int e = 0; e < 3; ++e) {
+    f[e] = G.act[e] ? G.fg[e] + 2 * I[e] : 0;
+    n2[e] = (e != d && G.act[e]) ? 2 : 1;
+  }
+  double sum = 0.0;
+  for (int kk = 0; kk < n2[2]; ++kk)
+    for (int jj = 0; jj < n2[1]; ++jj)
+      for (int ii = 0; ii < n2[0]; ++ii)
+        sum += R.f[d * G.fcells + (static_cast<long long>(f[2] + kk) * G.fe1 + (f[1] + jj)) *
+                                      G.fe0 +
+                   (f[0] + ii)];
+  const int cnt = n2[0] * n2[1] * n2[2];
+  int C[3];
+  for (int e = 0; e < 3; ++e) C[e] = G.act[e] ? G.cg[e] + R.o[e] * G.nc[e] + I[e] : 0;
+  R.c[d * G.ccells + (static_cast<long long>(C[2]) * G.ce1 + C[1]) * G.ce0 + C[0]] =
+      sum / cnt;
 }
 
 // interior cells of every block: MODE 0: a = 0; MODE 1: a += b
@@ -173,11 +235,12 @@ Box Whole(const Extent &e) {
   return b;
 }
 
-Region Rec(double *a, double *b, const Box &box, int gid, const Extent &ea, const Extent *eb) {
+Region Rec(double *a, double *b, const Box &box, int gid, const Extent &ea, const Extent *eb,
+           int nc = 1) {
   Region r{};
   r.a = a;
   r.b = b;
-  r.nc = 1;
+  r.nc = nc;
   r.gid = gid;
   for (int d = 0; d < 3; ++d) {
     r.lo[d] = box.lo[d];
@@ -219,19 +282,26 @@ struct TreeLevel {
   // to the next coarser level: r -> r's coarse buffers (restrict), then boxes -> its b (copy);
   // its x -> t's coarse buffers or t (copy), then t's coarse buffers -> t (prolong)
   Table restrict_r, copy_down, copy_up, prolong_t;
+  // the face field D to the next coarser level: same blocks copied, the others face-averaged
+  Table copy_D;
+  std::vector<FaceXfer> face_host;
+  FaceXfer *face = nullptr;
+  FaceGeom face_geom{};
   long long cells = 0;  // interior cells of all blocks
   ~TreeLevel() {
     restrict_r.Free();
     copy_down.Free();
     copy_up.Free();
     prolong_t.Free();
+    copy_D.Free();
+    if (face) (void)hipFree(face);
   }
 };
 
 class MgTree {
  public:
   Mesh *user = nullptr;
-  int fu = -1, frhs = -1;
+  int fu = -1, frhs = -1, fD = -1;
   pth_mg_params P{};
   double D = 1.0, alpha = 0.0;
   std::vector<std::unique_ptr<TreeLevel>> lv;  // finest (a copy of the user's mesh) first
@@ -264,6 +334,7 @@ pth_status MgTree::NewLevel(const Topology &t, bool coarse_buffers, int x_prolon
   m.fields.push_back(FieldSpec{"t", 1, 0u, PTH_PROLONG_LINEAR});
   m.fields.push_back(FieldSpec{"b", 1, 0u, PTH_PROLONG_LINEAR});
   m.fields.push_back(FieldSpec{"r", 1, 0u, PTH_PROLONG_LINEAR});
+  m.fields.push_back(FieldSpec{"D", 3, 0u, PTH_PROLONG_LINEAR});
   if (m.Finalize() != PTH_COMPLETE) return PTH_FAIL;
   L->md = m.Get("base");
   if (!L->md) return PTH_FAIL;
@@ -392,8 +463,14 @@ pth_status MgTree::BuildTransfers(int l) {
           Rec(r_f, b_C + Offset(Ce, Ce.g[0], Ce.g[1], Ce.g[2]), Interior(fe), fb, fe, &Ce));
       F.copy_up.host.push_back(
           Rec(x_C, t_f + Offset(fe, fe.g[0], fe.g[1], fe.g[2]), Interior(Ce), cbk, Ce, &fe));
+      F.copy_D.host.push_back(Rec(F.md->store[kD].data + fb * 3 * fe.cells(),
+                                  C.md->store[kD].data + cbk * 3 * Ce.cells(), Whole(fe), fb,
+                                  fe, &Ce, 3));
       continue;
     }
+    F.face_host.push_back(FaceXfer{F.md->store[kD].data + fb * 3 * fe.cells(),
+                                   C.md->store[kD].data + cbk * 3 * Ce.cells(),
+                                   {o[0], o[1], o[2]}, 0});
     // restriction of the block's interior into its coarse buffer, then into the target
     const Box ci = Interior(ce);
     F.restrict_r.host.push_back(Rec(r_f, r_c, ci, fb, fe, &ce));
@@ -424,8 +501,26 @@ pth_status MgTree::BuildTransfers(int l) {
         Rec(x_C, t_c + Offset(ce, ce.g[0] - m0, ce.g[1] - m1, ce.g[2] - m2), sb, cbk, Ce, &ce));
     F.prolong_t.host.push_back(Rec(t_f, t_c, ci, fb, fe, &ce));
   }
-  for (Table *q : {&F.restrict_r, &F.copy_down, &F.copy_up, &F.prolong_t})
+  for (Table *q : {&F.restrict_r, &F.copy_down, &F.copy_up, &F.prolong_t, &F.copy_D})
     if (q->Upload() != PTH_COMPLETE) return PTH_FAIL;
+  FaceGeom &G = F.face_geom;
+  G.fe0 = fe.ent(0), G.fe1 = fe.ent(1), G.ce0 = Ce.ent(0), G.ce1 = Ce.ent(1);
+  G.fcells = fe.cells(), G.ccells = Ce.cells();
+  for (int d = 0; d < 3; ++d) {
+    G.fg[d] = fe.g[d], G.cg[d] = Ce.g[d], G.nc[d] = ce.n[d];
+    G.act[d] = ft.active[d] ? 1 : 0;
+    // the upper face of the last cell lies in the first ghost cell on both levels
+    if (ft.active[d] && (fe.g[d] < 1 || Ce.g[d] < 1)) {
+      SetError("MGSolver hierarchy: the face field needs one ghost layer");
+      return PTH_FAIL;
+    }
+  }
+  G.ndim = ft.ndim;
+  if (!F.face_host.empty()) {
+    PTH_HIP(hipMalloc(&F.face, sizeof(FaceXfer) * F.face_host.size()));
+    PTH_HIP(hipMemcpy(F.face, F.face_host.data(), sizeof(FaceXfer) * F.face_host.size(),
+                      hipMemcpyHostToDevice));
+  }
   return PTH_COMPLETE;
 }
 
@@ -438,13 +533,14 @@ pth_status MgTree::Cells(int l, int mode, int in, int out, double w) {
   Mesh &m = *lv[l]->mesh;
   const long long total = lv[l]->cells;
   const BrickShape s = ShapeOf(m.topo.fine);
+  const double *Df = fD >= 0 ? Data(l, kD) : nullptr;
   if (mode == 0)
     hipLaunchKernelGGL(LevelCellKernel<0>, dim3(Grid(total)), dim3(kWG), 0, m.stream,
-                       Data(l, in), Data(l, kB), Data(l, out), s, m.coef, D, alpha, w,
+                       Data(l, in), Data(l, kB), Data(l, out), s, m.coef, D, Df, alpha, w,
                        m.topo.ndim, total);
   else
     hipLaunchKernelGGL(LevelCellKernel<1>, dim3(Grid(total)), dim3(kWG), 0, m.stream,
-                       Data(l, in), Data(l, kB), Data(l, out), s, m.coef, D, alpha, w,
+                       Data(l, in), Data(l, kB), Data(l, out), s, m.coef, D, Df, alpha, w,
                        m.topo.ndim, total);
   PTH_HIP(hipGetLastError());
   return PTH_COMPLETE;
@@ -517,6 +613,22 @@ pth_status MgTree::Setup(MeshData &md) {
   PTH_HIP(hipMemcpyAsync(Data(0, kX), md.store[fu].data, n, hipMemcpyDeviceToDevice, m0.stream));
   PTH_HIP(hipMemcpyAsync(Data(0, kB), md.store[frhs].data, n, hipMemcpyDeviceToDevice,
                          m0.stream));
+  if (fD < 0) return PTH_COMPLETE;
+  // AddSetupTasks: the face field D (GMGRestrict) down the levels
+  PTH_HIP(hipMemcpyAsync(Data(0, kD), md.store[fD].data, sizeof(double) * md.store[fD].ndata,
+                         hipMemcpyDeviceToDevice, m0.stream));
+  for (int l = 0; l + 1 < static_cast<int>(lv.size()); ++l) {
+    TreeLevel &F = *lv[l];
+    if (LaunchCopy(F.copy_D, F.mesh->stream) != PTH_COMPLETE) return PTH_FAIL;
+    if (F.face_host.empty()) continue;
+    const FaceGeom &G = F.face_geom;
+    long long per_rec = G.ndim;
+    for (int d = 0; d < 3; ++d) per_rec *= G.act[d] ? G.nc[d] + 1 : 1;
+    const long long total = per_rec * static_cast<long long>(F.face_host.size());
+    hipLaunchKernelGGL(FaceRestrictKernel, dim3(Grid(total)), dim3(kWG), 0, F.mesh->stream,
+                       F.face, G, per_rec, total);
+    PTH_HIP(hipGetLastError());
+  }
   return PTH_COMPLETE;
 }
 
@@ -539,11 +651,13 @@ pth_status MgTree::Solve(MeshData &md, int *iters, std::vector<double> *history)
   return PTH_COMPLETE;
 }
 
-MgTree *MgTreeCreate(Mesh *m, int fu, int frhs, const pth_mg_params &p, double D, double alpha) {
+MgTree *MgTreeCreate(Mesh *m, int fu, int frhs, int fD, const pth_mg_params &p, double D,
+                     double alpha) {
   std::unique_ptr<MgTree> t(new MgTree());
   t->user = m;
   t->fu = fu;
   t->frhs = frhs;
+  t->fD = fD;
   t->P = p;
   t->D = D;
   t->alpha = alpha;

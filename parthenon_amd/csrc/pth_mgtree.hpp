@@ -9,11 +9,13 @@
 namespace pth {
 
 class MgTree;
-// nullptr with the error set on failure
-MgTree *MgTreeCreate(Mesh *m, int fu, int frhs, const pth_mg_params &p, double D, double alpha);
+// nullptr with the error set on failure; fD: the face field D (3 components) or -1 for the
+// constant D
+MgTree *MgTreeCreate(Mesh *m, int fu, int frhs, int fD, const pth_mg_params &p, double D,
+                     double alpha);
 void MgTreeDestroy(MgTree *t);
 int MgTreeLevels(const MgTree *t);
-// copies u and rhs of md into the leaf level
+// copies u, rhs (and the face field D, restricted down the levels) of md into the hierarchy
 pth_status MgTreeSetup(MgTree *t, MeshData &md);
 // V-cycles until the RMS residual is below the tolerance (at least one, at most max_iters);
 // writes u's cells (ghosts included, stale: the caller refills them); history: the RMS

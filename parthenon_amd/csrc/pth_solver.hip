@@ -130,33 +130,41 @@ __global__ void __launch_bounds__(kWG)
   if (threadIdx.x == 0) *out = sum;
 }
 
-// Ax on the interior cells of every block; coef[b].dx = the block's cell widths
+// Ax on the interior cells of every block; coef[b].dx = the block's cell widths. Df: the face
+// field D ([block][3][k][j][i], component d = D on the lower face along d) or null for the
+// constant D.
 __global__ void __launch_bounds__(kWG)
     PoissonAxKernel(const double *__restrict__ x, double *__restrict__ out, Shape s,
-                    const BlockCoefs *__restrict__ coef, double D, double alpha, int ndim,
-                    long long total) {
+                    const BlockCoefs *__restrict__ coef, double D,
+                    const double *__restrict__ Df, double alpha, int ndim, long long total) {
   const long long t = static_cast<long long>(blockIdx.x) * kWG + threadIdx.x;
   if (t >= total) return;
   const long long per = static_cast<long long>(s.n0) * s.n1 * s.n2;
   const long long blk = t / per;
-  const long long c = blk * s.cells + InteriorCell(s, t % per);
+  const long long cl = InteriorCell(s, t % per);
+  const long long c = blk * s.cells + cl;
+  const long long p = static_cast<long long>(s.e0) * s.e1;
+  // D on the lower and upper face of the cell along each dimension
+  const double *d0 = Df ? Df + blk * 3 * s.cells + cl : nullptr;
+  const double D1l = Df ? d0[0] : D, D1h = Df ? d0[1] : D;
   const double u = x[c];
   const double dx1 = coef[blk].dx[0];
   double r = -alpha * u;
-  const double f1l = D / dx1 * (x[c - 1] - u);
-  const double f1r = D / dx1 * (u - x[c + 1]);
+  const double f1l = D1l / dx1 * (x[c - 1] - u);
+  const double f1r = D1h / dx1 * (u - x[c + 1]);
   r += (f1l - f1r) / dx1;
   if (ndim > 1) {
+    const double D2l = Df ? d0[s.cells] : D, D2h = Df ? d0[s.cells + s.e0] : D;
     const double dx2 = coef[blk].dx[1];
-    const double f2l = D * (x[c - s.e0] - u) / dx2;
-    const double f2r = D * (u - x[c + s.e0]) / dx2;
+    const double f2l = D2l * (x[c - s.e0] - u) / dx2;
+    const double f2r = D2h * (u - x[c + s.e0]) / dx2;
     r += (f2l - f2r) / dx2;
   }
   if (ndim > 2) {
-    const long long p = static_cast<long long>(s.e0) * s.e1;
+    const double D3l = Df ? d0[2 * s.cells] : D, D3h = Df ? d0[2 * s.cells + p] : D;
     const double dx3 = coef[blk].dx[2];
-    const double f3l = D * (x[c - p] - u) / dx3;
-    const double f3r = D * (u - x[c + p]) / dx3;
+    const double f3l = D3l * (x[c - p] - u) / dx3;
+    const double f3r = D3h * (u - x[c + p]) / dx3;
     r += (f3l - f3r) / dx3;
   }
   out[c] = r;
@@ -395,11 +403,21 @@ pth_status pth_poisson_ax(pth_meshdata *mdh, const pth_poisson_equation *eq, con
       SetError("PoissonEquation::Ax: needs one ghost layer");
       return PTH_FAIL;
     }
+  const double *Df = nullptr;
+  if (eq->D_field) {
+    const int fd = SolverField(md, eq->D_field, "PoissonEquation::Ax");
+    if (fd < 0) return PTH_FAIL;
+    if (m->fields[fd].ncomp != 3) {
+      SetError("PoissonEquation::Ax: the face field D needs 3 components");
+      return PTH_FAIL;
+    }
+    Df = md->store[fd].data;
+  }
   const Shape s = ShapeOf(t.fine);
   const long long total = static_cast<long long>(t.blocks.size()) * s.n0 * s.n1 * s.n2;
   if (total == 0) return PTH_COMPLETE;
   hipLaunchKernelGGL(PoissonAxKernel, dim3(Blocks(total)), dim3(kWG), 0, m->stream,
-                     md->store[fx].data, md->store[fo].data, s, m->coef, eq->D,
+                     md->store[fx].data, md->store[fo].data, s, m->coef, eq->D, Df,
                      eq->diagonal_alpha, t.ndim, total);
   PTH_HIP(hipGetLastError());
   return PTH_COMPLETE;
@@ -420,9 +438,17 @@ pth_mg *pth_mg_create(pth_mesh *mh, const char *u, const char *rhs, const pth_mg
     SetError("MGSolver: bad parameters");
     return nullptr;
   }
-  if (!(eq->D > 0.0) || !(eq->diagonal_alpha >= 0.0)) {
-    SetError("MGSolver: the equation needs a constant D > 0 and diagonal_alpha >= 0");
+  if ((!eq->D_field && !(eq->D > 0.0)) || !(eq->diagonal_alpha >= 0.0)) {
+    SetError("MGSolver: the equation needs D > 0 and diagonal_alpha >= 0");
     return nullptr;
+  }
+  int fD = -1;
+  if (eq->D_field) {
+    fD = m->FieldIndex(eq->D_field);
+    if (fD < 0 || m->fields[fD].ncomp != 3) {
+      SetError("MGSolver: the face field D must be a 3-component field of the mesh");
+      return nullptr;
+    }
   }
   const int fu = m->FieldIndex(u ? u : ""), fr = m->FieldIndex(rhs ? rhs : "");
   if (fu < 0 || fr < 0 || fu == fr) {
@@ -438,7 +464,7 @@ pth_mg *pth_mg_create(pth_mesh *mh, const char *u, const char *rhs, const pth_mg
   const double L = t.xmax[0] - t.xmin[0];
   bool cube = t.ndim == 3 && !t.multilevel && t.nx[0] == t.nx[1] && t.nx[0] == t.nx[2] &&
               t.xmax[1] - t.xmin[1] == L && t.xmax[2] - t.xmin[2] == L &&
-              eq->diagonal_alpha == 0.0 && p->smoother == 0;
+              eq->diagonal_alpha == 0.0 && p->smoother == 0 && fD < 0;
   for (int face = 0; face < 6; ++face) cube = cube && t.bc[face] == PTH_BC_FIXED_FACE;
   if (!cube) {
     // block-mesh hierarchy, FP64
@@ -458,7 +484,7 @@ pth_mg *pth_mg_create(pth_mesh *mh, const char *u, const char *rhs, const pth_mg
     h->fu = fu;
     h->frhs = fr;
     h->P = *p;
-    h->tree = MgTreeCreate(m, fu, fr, *p, eq->D, eq->diagonal_alpha);
+    h->tree = MgTreeCreate(m, fu, fr, fD, *p, eq->D, eq->diagonal_alpha);
     if (!h->tree) {
       delete h;
       return nullptr;

@@ -8,6 +8,7 @@ in FP64. `Server` admits independent solves as they arrive and batches their V-c
 """
 import ctypes as C
 from dataclasses import dataclass
+from typing import Optional
 
 import numpy as np
 
@@ -158,13 +159,16 @@ def dot_product(md, a, b):
 
 
 @dataclass
-class PoissonEquation:  # example/poisson_gmg/poisson_equation.hpp:41-205, constant D
-    D: float = 1.0
+class PoissonEquation:  # example/poisson_gmg/poisson_equation.hpp:41-205
+    D: float = 1.0                 # constant D, or
     diagonal_alpha: float = 0.0
+    D_field: Optional[str] = None  # a 3-component field: component d = D on the lower face
+                                   # along d (the reference's face field D)
 
     def _c(self):
         from . import PoissonEquationC
-        return PoissonEquationC(self.D, self.diagonal_alpha)
+        return PoissonEquationC(self.D, self.diagonal_alpha,
+                                self.D_field.encode() if self.D_field else None)
 
     def Ax(self, md, x, out):
         """PoissonEquation::Ax<x, out> (:41-53): needs x's ghosts filled."""

@@ -7,11 +7,13 @@
 // periodic; every other face gets the package's FixedFace conditions, which it registers by
 // hand, poisson_package.cpp:39-78), <parthenon/meshblock> nx1..3, <parthenon/static_refinementN>
 // boxes with their logical level, <poisson> use_exact_rhs, x0 y0 z0 radius, interior_D /
-// exterior_D (equal: D is constant here), diagonal_alpha, <poisson/solver_params>
+// exterior_D (the face field D), diagonal_alpha, <poisson/solver_params>
 // max_iterations, residual_tolerance, smoother. do_FAS is read and not needed (the correction
 // scheme gives the same iterates for this linear operator); the BiCGSTAB keys are ignored.
 // The problem generator is parthenon_app_inputs.cpp:32-82: rhs = 1 inside the radius,
-// exact = -exp(-10 r^2) (r over the active dimensions; x3 from the x3 centre).
+// exact = -exp(-10 r^2) (r over the active dimensions; x3 from the x3 centre), D on each face
+// interior_D inside the two boxes of inside_region and exterior_D elsewhere (x3 of the face
+// from the x3 coordinates, where the reference reads X<2>).
 //
 // usage: poisson_deck_driver <deck> <out.bin>
 // writes doubles: final RMS residual, final RMS error (or -1), V-cycles, blocks, cells per
@@ -33,6 +35,7 @@ namespace poisson_package {
 PARTHENON_AMD_VARIABLE(poisson, u);
 PARTHENON_AMD_VARIABLE(poisson, rhs);
 PARTHENON_AMD_VARIABLE(poisson, exact);
+PARTHENON_AMD_VARIABLE(poisson, D);
 }  // namespace poisson_package
 
 namespace {
@@ -109,8 +112,6 @@ int main(int argc, char **argv) {
     const int ng = GetInt(deck, M, "nghost", 2);
     const Real interior_D = GetReal(deck, P, "interior_D", 1.0),
                exterior_D = GetReal(deck, P, "exterior_D", 1.0);
-    if (interior_D != exterior_D)
-      throw std::runtime_error("interior_D != exterior_D: D is constant in this build");
     const bool use_exact_rhs = GetBool(deck, P, "use_exact_rhs", false);
     const Real x0 = GetReal(deck, P, "x0", 0.0), y0 = GetReal(deck, P, "y0", 0.0),
                z0 = GetReal(deck, P, "z0", 0.0), radius0 = GetReal(deck, P, "radius", 0.1);
@@ -136,6 +137,7 @@ int main(int argc, char **argv) {
     mesh.RegisterRefinementOps(u::name(), Mesh::ProlongationOp::SharedLinear);
     mesh.AddField(rhs::name(), 1, Metadata::OneCopy);
     mesh.AddField(exact::name(), 1, Metadata::OneCopy);
+    mesh.AddField(D::name(), 3, Metadata::OneCopy);  // component d: D on the lower face along d
     mesh.Initialize();
 
     // ProblemGenerator (parthenon_app_inputs.cpp:32-82) on every cell, ghosts included
@@ -146,7 +148,17 @@ int main(int argc, char **argv) {
     const int e[3] = {n[0] + 2 * g[0], n[1] + 2 * g[1], n[2] + 2 * g[2]};
     const size_t cells = static_cast<size_t>(e[0]) * e[1] * e[2];
     const int nblocks = mesh.nbtotal();
-    std::vector<Real> hr(nblocks * cells), hu(hr.size(), 0.0), he(hr.size());
+    std::vector<Real> hr(nblocks * cells), hu(hr.size(), 0.0), he(hr.size()),
+        hd(3 * hr.size());
+    auto inside_region = [ndim](Real x, Real y, Real z) {
+      bool inside1 = (x < -0.25) && (x > -0.75);
+      if (ndim > 1) inside1 = inside1 && (y < 0.5) && (y > -0.5);
+      if (ndim > 2) inside1 = inside1 && (z < 0.25) && (z > -0.25);
+      bool inside2 = (x < 0.5) && (x > -0.75);
+      if (ndim > 1) inside2 = inside2 && (y < -0.25) && (y > -0.75);
+      if (ndim > 2) inside2 = inside2 && (z < 0.25) && (z > -0.25);
+      return inside1 || inside2;
+    };
     for (int b = 0; b < nblocks; ++b) {
       int64_t loc[4];
       double x[6];
@@ -166,16 +178,22 @@ int main(int argc, char **argv) {
             const size_t c = b * cells + (static_cast<size_t>(k) * e[1] + j) * e[0] + i;
             hr[c] = rad < radius0 ? 1.0 : 0.0;
             he[c] = -std::exp(-10.0 * rad * rad);
+            const Real x1f = x1 - 0.5 * dx[0], x2f = x2 - 0.5 * dx[1], x3f = x3 - 0.5 * dx[2];
+            const size_t dc = b * 3 * cells + (static_cast<size_t>(k) * e[1] + j) * e[0] + i;
+            hd[dc] = inside_region(x1f, x2, x3) ? interior_D : exterior_D;
+            hd[dc + cells] = inside_region(x1, x2f, x3) ? interior_D : exterior_D;
+            hd[dc + 2 * cells] = inside_region(x1, x2, x3f) ? interior_D : exterior_D;
           }
     }
     auto &base = mesh.GetOrAdd("base", 0);
     base->Upload(rhs::name(), hr);
     base->Upload(u::name(), hu);
     base->Upload(exact::name(), he);
+    base->Upload(D::name(), hd);
 
     // the package's solver (poisson_package.cpp:80-98) with the deck's solver_params
     solvers::PoissonEquation eq;
-    eq.D = interior_D;
+    eq.D_field = D::name();
     eq.diagonal_alpha = GetReal(deck, P, "diagonal_alpha", 0.0);
     solvers::MGParams params;
     params.max_iters = GetInt(deck, S, "max_iterations", 1000);

@@ -50,8 +50,8 @@ def lib():
             "orc_apply_bc": (None, [P, I, C.POINTER(I), C.POINTER(I), C.POINTER(DP), I]),
             "orc_restrict_interior": (None, [P, I, DP, DP]),
             "orc_prolongate_cells": (None, [P, I, DP, DP, I, C.POINTER(I), I]),
-            "orc_poisson_leaf_apply": (None, [P, D, D, DP, DP]),
-            "orc_poisson_leaf_solve": (I, [P, D, D, DP, DP, D, I, DP]),
+            "orc_poisson_leaf_apply": (None, [P, D, D, DP, DP, DP]),
+            "orc_poisson_leaf_solve": (I, [P, D, D, DP, DP, DP, D, I, DP]),
             "orc_weno5z": (None, [I, DP, DP, DP]),
             "orc_linear": (None, [I, DP, DP, DP]),
             "orc_lr_to_flux": (None, [I, DP, DP]),
@@ -162,21 +162,25 @@ class OracleMesh:
             _LIB.orc_mesh_free(self.h)
             self.h = None
 
-    def poisson_apply(self, x, D=1.0, alpha=0.0):
+    def poisson_apply(self, x, D=1.0, alpha=0.0, Df=None):
         """Leaf-grid Poisson operator (exchange with linear prolongation and the face
-        conditions, then Ax) on interior values in block, then [k][j][i] order."""
+        conditions, then Ax) on interior values in block, then [k][j][i] order; Df: the face
+        field D as [block][3] bricks, or None for the constant D."""
         x = np.ascontiguousarray(x, dtype=np.float64).ravel()
         out = np.empty_like(x)
-        lib().orc_poisson_leaf_apply(self.h, D, alpha, dptr(x), dptr(out))
+        df = None if Df is None else np.ascontiguousarray(Df, dtype=np.float64)
+        lib().orc_poisson_leaf_apply(self.h, D, alpha, None if df is None else dptr(df),
+                                     dptr(x), dptr(out))
         return out
 
-    def poisson_solve(self, b, x0=None, D=1.0, alpha=0.0, tol=1e-14, maxit=5000):
+    def poisson_solve(self, b, x0=None, D=1.0, alpha=0.0, Df=None, tol=1e-14, maxit=5000):
         """BiCGSTAB on the leaf grid to RMS residual tol; returns (x, iterations, rms)."""
         b = np.ascontiguousarray(b, dtype=np.float64).ravel()
         x = np.zeros_like(b) if x0 is None else np.array(x0, dtype=np.float64).ravel()
+        df = None if Df is None else np.ascontiguousarray(Df, dtype=np.float64)
         rms = C.c_double(0.0)
-        it = lib().orc_poisson_leaf_solve(self.h, D, alpha, dptr(b), dptr(x), tol, maxit,
-                                          C.byref(rms))
+        it = lib().orc_poisson_leaf_solve(self.h, D, alpha, None if df is None else dptr(df),
+                                          dptr(b), dptr(x), tol, maxit, C.byref(rms))
         return x, it, rms.value
 
     def refresh(self):

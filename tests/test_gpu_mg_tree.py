@@ -33,7 +33,7 @@ pytestmark = pytest.mark.gpu
 DECK_BOX = ((-1.0, -0.75, -1.0, -0.75, 0.0, 1.0), 3)
 
 
-def make_meshes(nx, bnx, xmin, xmax, boxes=(), spheres=(), ng=2):
+def make_meshes(nx, bnx, xmin, xmax, boxes=(), spheres=(), ng=2, face_D=False):
     nd = sum(1 for v in nx if v > 1)
     bcs = tuple(("fixed_face" if d < nd else "periodic") for d in range(3) for _ in range(2))
     periodic = tuple(0 if d < nd else 1 for d in range(3))
@@ -43,6 +43,8 @@ def make_meshes(nx, bnx, xmin, xmax, boxes=(), spheres=(), ng=2):
     m.register_refinement_ops("u", "linear")  # poisson_package.cpp:122-126
     m.add_field("rhs", 1)
     m.add_field("Ax", 1)
+    if face_D:
+        m.add_field("D", 3, 0)  # the face field D: component d on each cell's lower face
     m.finalize()
     om = OracleMesh(nx, bnx, ng, xmin=xmin, xmax=xmax, periodic=periodic, bcs=bcs,
                     boxes=boxes, spheres=spheres)
@@ -86,9 +88,31 @@ def rms(v):
     return float(np.sqrt(np.mean(np.square(v))))
 
 
-def deck():
+def deck(face_D=False):
     return make_meshes((64, 64, 1), (32, 32, 1), (-1.0, -1.0, 0.0), (1.0, 1.0, 1.0),
-                       boxes=[DECK_BOX])
+                       boxes=[DECK_BOX], face_D=face_D)
+
+
+def deck_face_D(m, interior_D, exterior_D):
+    """The problem generator's D (parthenon_app_inputs.cpp:58-75, 2-D): interior_D on faces
+    inside the two rectangles, exterior_D elsewhere, as [block][3] bricks."""
+    def inside(x, y):
+        in1 = (x < -0.25) & (x > -0.75) & (y < 0.5) & (y > -0.5)
+        in2 = (x < 0.5) & (x > -0.75) & (y < -0.25) & (y > -0.75)
+        return in1 | in2
+    n, g = m.n, m.ng
+    e = tuple(n[d] + 2 * g[d] for d in range(3))
+    out = np.zeros((m.nblocks, 3, e[2], e[1], e[0]))
+    for gid in range(m.nblocks):
+        b = m.block(gid)
+        dx = [(b["xmax"][d] - b["xmin"][d]) / n[d] for d in range(3)]
+        cc = [b["xmin"][d] + (np.arange(e[d]) - g[d] + 0.5) * dx[d] for d in range(3)]
+        for d in range(3):
+            ax = [a.copy() for a in cc]
+            ax[d] = ax[d] - 0.5 * dx[d]
+            z, y, x = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+            out[gid, d] = np.where(inside(x, y), interior_D, exterior_D)
+    return out
 
 
 def test_deck_use_exact_rhs_passes_reference_thresholds():
@@ -207,3 +231,64 @@ def test_cpp_deck_driver_runs_reference_deck(tmp_path):
     assert abs(err - rms(u - exact)) <= 1e-6 * err
     xo, _, _ = om.poisson_solve(rhs, tol=1e-13)
     assert rms(u - xo) <= 1e-12
+
+
+def test_deck_with_face_field_D_matches_oracle():
+    """The deck with interior_D = 10, exterior_D = 1 (the package's face field D,
+    poisson_package.cpp:112-118): rhs = A exact bit-equal to the oracle's, the solution
+    against the oracle's solve with the same faces, and the exact solution recovered."""
+    print("case poisson_gmg deck, face field D", flush=True)
+    m, om = deck(face_D=True)
+    md = m.mesh_data("base")
+    Df = deck_face_D(m, 10.0, 1.0)
+    md.upload("D", Df)
+    eq = PoissonEquation(D_field="D")
+    c = centres(m)
+    exact = -np.exp(-10.0 * (c[:, 0] ** 2 + c[:, 1] ** 2))
+    md.upload("u", to_bricks(m, exact, fill=np.nan))
+    pa.exchange_fused(md)
+    eq.Ax(md, "u", "rhs")
+    rhs = from_bricks(m, md.download("rhs"))
+    ref_rhs = om.poisson_apply(exact, Df=Df)
+    assert np.array_equal(rhs, ref_rhs), np.max(np.abs(rhs - ref_rhs))
+    md.upload("u", np.zeros(m.field_shape("u")))
+    p = MeshMGParams(max_iters=60, residual_tolerance=1e-11, precision="fp64",
+                     smoother="SRJ2")
+    s = MeshMGSolver(m, "u", "rhs", p, eq)
+    s.setup(md)
+    it, res = s.solve(md)
+    print(f"face D: {it} V-cycles, rms residual {res:.3e}, history {s.history()[:6]}",
+          flush=True)
+    assert res <= 1e-11
+    u = from_bricks(m, md.download("u"))
+    assert rms(ref_rhs - om.poisson_apply(u, Df=Df)) <= 1.01e-11
+    xo, _, _ = om.poisson_solve(ref_rhs, Df=Df, tol=1e-13)
+    assert rms(u - xo) <= 1e-10 * rms(xo)
+    assert rms(u - exact) <= 1e-10
+
+
+def test_cpp_deck_driver_with_face_field_D(tmp_path):
+    """The regression deck with interior_D = 10: the driver builds the face field D as the
+    problem generator does; its rhs and solution match the oracle's with the same faces."""
+    print("case cpp deck driver, interior_D = 10", flush=True)
+    here = os.path.dirname(os.path.abspath(__file__))
+    text = open(os.path.join(here, "golden", "parthinput.poisson")).read()
+    assert "interior_D = 1.0" in text
+    deck_file = tmp_path / "parthinput.poisson_D10"
+    deck_file.write_text(text.replace("interior_D = 1.0", "interior_D = 10.0")
+                         .replace("max_iterations = 15", "max_iterations = 60"))
+    out = str(tmp_path / "deckD.bin")
+    r = subprocess.run([os.path.join(here, "cpp", "poisson_deck_driver"), str(deck_file), out],
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    print(r.stdout.strip(), flush=True)
+    raw = np.fromfile(out, dtype=np.float64)
+    res, err, its, nb, per = raw[:5]
+    n = int(nb) * int(per)
+    u, rhs, exact = (raw[5 + q * n:5 + (q + 1) * n] for q in range(3))
+    m, om = deck()
+    Df = deck_face_D(m, 10.0, 1.0)
+    assert np.array_equal(rhs, om.poisson_apply(exact, Df=Df))
+    assert res <= 1e-12 and err <= 1e-11
+    xo, _, _ = om.poisson_solve(rhs, Df=Df, tol=1e-13)
+    assert rms(u - xo) <= 1e-11
