@@ -335,6 +335,104 @@ def gmg_bench(args, quick=False):
     print(json.dumps(out))
 
 
+def gmg_refined_bench(quick=True):
+    """MGSolver<u, rhs, PoissonEquation> on statically refined meshes (the block-mesh
+    hierarchy, FP64, SRJ2): the reference's poisson_gmg regression deck (2-D, 64^2 cells in
+    32^2 blocks on [-1, 1]^2, [-1, -0.75]^2 refined to logical level 3, use_exact_rhs, 15
+    V-cycles to 1e-12 RMS residual) and a 3-D mesh (128^3 cells in 16^3 blocks on [-1, 1]^3,
+    the blocks meeting the sphere r = 0.25 refined one level, a seeded Gaussian-blob rhs, to
+    1e-10 RMS residual). One step = one solve from u = 0, setup excluded."""
+    import numpy as np
+    import parthenon_amd as pa
+    from parthenon_amd.gmg import MeshMGParams, MeshMGSolver, PoissonEquation
+
+    def build(nx, bnx, lo, hi, boxes=(), spheres=()):
+        nd = sum(1 for v in nx if v > 1)
+        bcs = tuple(("fixed_face" if d < nd else "periodic") for d in range(3) for _ in range(2))
+        m = pa.Mesh(nx, bnx, 2, xmin=lo, xmax=hi, periodic=tuple(0 if d < nd else 1
+                                                                 for d in range(3)),
+                    bcs=bcs, refine_boxes=boxes, refine_spheres=spheres)
+        m.add_field("u", 1, pa.Metadata.FillGhost | pa.Metadata.Independent)
+        m.register_refinement_ops("u", "linear")
+        m.add_field("rhs", 1)
+        m.finalize()
+        return m
+
+    def bricks(m, f):  # cell-centre function -> the field's bricks (ghosts too)
+        out = np.zeros(m.field_shape("u"))
+        n, g = m.n, m.ng
+        for gid in range(m.nblocks):
+            b = m.block(gid)
+            ax = [b["xmin"][d] + (np.arange(n[d] + 2 * g[d]) - g[d] + 0.5) *
+                  (b["xmax"][d] - b["xmin"][d]) / n[d] for d in range(3)]
+            z, y, x = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+            out[gid, 0] = f(x, y, z)
+        return out
+
+    def run(m, rhs_fn, exact_fn, tol, maxit, steps):
+        md = m.mesh_data("base")
+        if exact_fn is not None:  # use_exact_rhs: rhs <- A exact (poisson_driver.cpp:87-95)
+            md.upload("u", bricks(m, exact_fn))
+            pa.exchange_fused(md)
+            PoissonEquation().Ax(md, "u", "rhs")
+        else:
+            md.upload("rhs", bricks(m, rhs_fn))
+        zero = np.zeros(m.field_shape("u"))
+        s = MeshMGSolver(m, "u", "rhs", MeshMGParams(max_iters=maxit, residual_tolerance=tol,
+                                                     precision="fp64", smoother="SRJ2"))
+        walls = []
+        for q in range(steps + 1):
+            md.upload("u", zero)
+            s.setup(md)
+            m.synchronize()
+            t0 = time.perf_counter()
+            its, res = s.solve(md)
+            m.synchronize()
+            walls.append(time.perf_counter() - t0)
+        pa.profile_enable(True)
+        pa.profile_report()
+        md.upload("u", zero)
+        s.setup(md)
+        s.solve(md)
+        prof = pa.profile_report()
+        pa.profile_enable(False)
+        out = {"blocks": m.nblocks, "levels": s.levels(), "v_cycles": its,
+               "rms_residual": res, "ms_per_solve": 1e3 * float(np.median(walls[1:])),
+               "solves_per_s": 1.0 / float(np.median(walls[1:])),
+               "kernels": {k: {"launches_per_solve": v[0], "avg_us": v[1] / v[0] * 1e3}
+                           for k, v in sorted(prof.items())}}
+        if exact_fn is not None:
+            u = md.download("u")
+            ex = bricks(m, exact_fn)
+            n, g = m.n, m.ng
+            sl = (slice(None), 0, slice(g[2], g[2] + n[2]), slice(g[1], g[1] + n[1]),
+                  slice(g[0], g[0] + n[0]))
+            out["rms_error"] = float(np.sqrt(np.mean((u[sl] - ex[sl]) ** 2)))
+        return out
+
+    deck = build((64, 64, 1), (32, 32, 1), (-1.0, -1.0, 0.0), (1.0, 1.0, 1.0),
+                 boxes=[((-1.0, -0.75, -1.0, -0.75, 0.0, 1.0), 3)])
+    res_deck = run(deck, None, lambda x, y, z: -np.exp(-10.0 * (x * x + y * y)), 1e-12, 15, 5)
+    rng = np.random.default_rng(3)
+    blobs = [(rng.uniform(-0.8, 0.8, 3), rng.uniform(0.05, 0.2), rng.choice([-1.0, 1.0]))
+             for _ in range(16)]
+
+    def blob_rhs(x, y, z):
+        v = np.zeros_like(x)
+        for c, w, a in blobs:
+            v += a * np.exp(-((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2) / (w * w))
+        return v
+    m3 = build((128, 128, 128), (16, 16, 16), (-1.0,) * 3, (1.0,) * 3,
+               spheres=[((0.0, 0.0, 0.0), 0.25, 1)])
+    res3 = run(m3, blob_rhs, None, 1e-10, 40, 3)
+    return {"metric": "refined-mesh Poisson MG solves/s (FP64, SRJ2)",
+            "value": res3["solves_per_s"], "unit": "solves/s", "higher_is_better": True,
+            "dtype": "f64", "data": "synthetic (16 Gaussian blobs, numpy seed 3)",
+            "config": {"workload": "poisson-gmg 128^3 on [-1,1]^3 in 16^3 blocks, sphere "
+                                   "r=0.25 refined one level, FixedFace 0, SRJ2, 1e-10 RMS"},
+            "mesh_3d": res3, "regression_deck": res_deck}
+
+
 def gmg_serve_bench(args, nreq=None, quick=False):
     """Served Poisson solves: seeded Poisson-process arrivals at 50 requests/s, grid sizes
     {32, 64, 128, 192}^3 with weights {0.4, 0.3, 0.2, 0.1}, block size 16^3 or 32^3 (50/50,
@@ -754,6 +852,7 @@ def main():
                                                   no_cpu_baseline=args.no_cpu_baseline),
                                quick=True)
         out["gmg_serve"] = gmg_serve_bench(args, nreq=2000, quick=True)  # cfg 4: 2000 requests
+        out["gmg_refined"] = gmg_refined_bench()
     if amr and args.driver != "tasks" and not args.no_exchange:
         # the ghost-exchange microbenchmark (128^3 in 512 blocks of 16^3, nghost 3, 8 fields)
         # in its fused (direct copy) and buffered (pack + unpack) forms

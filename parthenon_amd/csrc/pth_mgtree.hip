@@ -534,6 +534,7 @@ pth_status MgTree::Cells(int l, int mode, int in, int out, double w) {
   const long long total = lv[l]->cells;
   const BrickShape s = ShapeOf(m.topo.fine);
   const double *Df = fD >= 0 ? Data(l, kD) : nullptr;
+  ProfScope prof(mode == 0 ? "mg_level_jacobi" : "mg_level_residual", m.stream);
   if (mode == 0)
     hipLaunchKernelGGL(LevelCellKernel<0>, dim3(Grid(total)), dim3(kWG), 0, m.stream,
                        Data(l, in), Data(l, kB), Data(l, out), s, m.coef, D, Df, alpha, w,
