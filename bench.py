@@ -272,14 +272,21 @@ def gmg_bench(args, quick=False):
     # algorithmic bytes per launch of the finest-level kernels
     kb = {"gmg_outer": 32 * c, "gmg_pre_l0": 2 * es * c, "gmg_jacobi_l0": 3 * es * c,
           "gmg_restrict_l0": 2 * es * c + es * c // 8,
-          "gmg_prolong_l0": 2 * es * c + es * c // 8}
+          "gmg_prolong_l0": 2 * es * c + es * c // 8,
+          "gmg_prolong_jacobi_l0": 3 * es * c + es * c // 8}
     prof = m["prof"]
     dom = max((k for k in prof if k in kb), key=lambda k: prof[k][1])
     n_l, ms = prof[dom]
     avg_s = ms / n_l * 1e-3
     achieved = kb[dom] / avg_s / 1e9
-    # per outer iteration: the finest level's traffic plus 1/7 of it for the coarser levels
-    it_bytes = int(32 * c + (2 * es + 2 * 3 * es + 2 * (2 * es + es / 8)) * c * 8 / 7)
+    # per outer iteration: the finest level's traffic plus 1/7 of it for the coarser levels;
+    # a V-cycle's level traffic: pre-smoothing (b read, x written), residual-restriction (x, b
+    # read, coarse b written), prolongation inside the first post-sweep (x, b, coarse x read,
+    # t written) and the second post-sweep (x, b read, t written)
+    fused = "gmg_prolong_jacobi_l0" in prof
+    vc = (2 * es + (2 * es + es / 8) + (3 * es + es / 8) + 3 * es if fused
+          else 2 * es + 2 * 3 * es + 2 * (2 * es + es / 8))
+    it_bytes = int(32 * c + vc * c * 8 / 7)
     per_it = m["wall"] / m["steps"] / max(m["iters"] * args.inner_cycles, 1)
     out = {
         "metric": "Poisson GMG solves/s (256^3, 1e-10 relative residual)",

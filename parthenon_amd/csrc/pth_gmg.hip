@@ -542,6 +542,66 @@ __global__ void __launch_bounds__(kWG) ProlongKernel(const GmgArgs<T> A) {
   Glob(e.x)[c] += v;
 }
 
+// Prolongation fused into the first post-smoothing sweep, column form: t = J(x + P xc), the
+// same values as ProlongKernel followed by JacobiZKernel. A workgroup covers 64 x 4 cells of
+// NZ consecutive planes; per plane, the corrected values x' = x + P xc of its cells and of the
+// one-cell ring around them (rows j0 - 1 and j0 + 4, columns i0 - 1 and i0 + 64) are staged
+// in LDS for the x and y neighbours, and each thread keeps x' of its column at k - 1, k, k + 1
+// in registers. The level's x is read, never written (after the launch t holds the iterate).
+template <typename T, int NZ>
+__global__ void __launch_bounds__(kWG) ProlongJacobiZKernel(const GmgArgs<T> A, T w) {
+  constexpr int TW = 66;  // tile width: 64 cells and one on each side
+  __shared__ T tile[6][TW];
+  int local;
+  const GmgEntry<T> e = Find(A, local);
+  const int n = e.n, nc = e.nc, s2 = n * n;
+  const ZCell z = ZStart(n, local, NZ);
+  const int i = z.i, j = z.j, lx = static_cast<int>(threadIdx.x % 64),
+            ly = static_cast<int>(threadIdx.x / 64);
+  const int i0 = i - lx, j0 = j - ly;
+  int c = z.c;
+  const Lv<T> o = MakeLv<T>(n);
+  const auto *x = Glob(static_cast<const T *>(e.x));
+  const auto *b = Glob(static_cast<const T *>(e.b));
+  const auto *xc = Glob(static_cast<const T *>(e.xc));
+  auto *t = Glob(e.t);
+  // x + P xc at a cell of the level (callers clamp positions into the level)
+  auto xp1 = [&](int ii, int jj, int kk) -> T {
+    const Cell q = {ii, jj, kk};
+    return x[(kk * n + jj) * n + ii] + ProlongValue<T>(xc, nc, q);
+  };
+  // the ring cell this thread fills: 66 + 66 cells of rows 0 and 5, then columns 0 and 65 of
+  // rows 1..4 (140 cells; threads 140..255 fill none)
+  const int h = static_cast<int>(threadIdx.x);
+  const int hr = h < TW ? 0 : h < 2 * TW ? 5 : 1 + (h - 2 * TW) / 2;
+  const int hc = h < TW ? h : h < 2 * TW ? h - TW : ((h - 2 * TW) % 2 ? TW - 1 : 0);
+  const int hi = min(max(i0 - 1 + hc, 0), n - 1), hj = min(max(j0 - 1 + hr, 0), n - 1);
+  const bool ring = h < 2 * TW + 8;
+  T prev = xp1(i, j, max(z.k0 - 1, 0));
+  T cur = xp1(i, j, z.k0);
+  T next = xp1(i, j, min(z.k0 + 1, n - 1));
+  for (int k = z.k0; k < z.k0 + NZ; ++k, c += s2) {
+    tile[ly + 1][lx + 1] = cur;
+    if (ring) tile[hr][hc] = xp1(hi, hj, k);
+    const T bc = b[c];
+    const T nn = xp1(i, j, min(k + 2, n - 1));
+    __syncthreads();
+    Nbrs<T> v;
+    v.u = cur;
+    v.xm = i > 0 ? tile[ly + 1][lx] : -v.u;
+    v.xp = i < n - 1 ? tile[ly + 1][lx + 2] : -v.u;
+    v.ym = j > 0 ? tile[ly][lx + 1] : -v.u;
+    v.yp = j < n - 1 ? tile[ly + 2][lx + 1] : -v.u;
+    v.zm = k > 0 ? prev : -v.u;
+    v.zp = k < n - 1 ? next : -v.u;
+    t[c] = JacobiPoint(v, bc, o, w);
+    __syncthreads();  // the next plane overwrites the tile
+    prev = cur;
+    cur = next;
+    next = nn;
+  }
+}
+
 // ---- bottom of the V-cycle in one workgroup (levels of at most NB cells per side) --------
 template <typename T>
 __device__ void BSweep(T *x, T *t, const T *b, int n, T w, bool zero) {
@@ -683,7 +743,8 @@ __global__ void __launch_bounds__(kWG) FillKernel(double *__restrict__ a, std::i
 
 int Blocks(std::int64_t cells) { return static_cast<int>((cells + kWG - 1) / kWG); }
 
-constexpr int kZCol = 8;  // planes per thread of the column Jacobi sweep
+constexpr int kZCol = 8;    // planes per thread of the column Jacobi sweep
+constexpr int kPJCol = 8;   // planes per thread of the prolongation-and-sweep kernel
 
 }  // namespace
 
@@ -980,13 +1041,26 @@ pth_status GmgBatch<T>::Cycle(const std::vector<GmgProblem<T> *> &ps, double ome
   for (int l = L - 1; l >= 0; --l) {
     GmgArgs<T> a;
     int nwg;
-    if (Stage(entries(l), false, &a, &nwg) != PTH_COMPLETE) return PTH_FAIL;
-    {
+    const std::vector<GmgEntry<T>> es = entries(l);
+    bool fuse = jacobi_columns > 0 && post >= 1;  // prolongation inside the first sweep
+    for (const auto &e : es) fuse = fuse && e.n % 64 == 0 && e.n % kPJCol == 0;
+    if (fuse) {
+      if (Stage(es, false, &a, &nwg, kWG * kPJCol) != PTH_COMPLETE) return PTH_FAIL;
+      {
+        ProfScope prof(l == 0 ? "gmg_prolong_jacobi_l0" : "gmg_prolong_jacobi", stream);
+        hipLaunchKernelGGL((ProlongJacobiZKernel<T, kPJCol>), dim3(nwg), dim3(kWG), 0, stream,
+                           a, w);
+        PTH_HIP(hipGetLastError());
+      }
+      for (auto *p : ps)
+        if (l < p->nbottom) std::swap(p->lv[l].x, p->lv[l].t);
+    } else {
+      if (Stage(es, false, &a, &nwg) != PTH_COMPLETE) return PTH_FAIL;
       ProfScope prof(l == 0 ? "gmg_prolong_l0" : "gmg_prolong", stream);
       hipLaunchKernelGGL(ProlongKernel<T>, dim3(nwg), dim3(kWG), 0, stream, a);
       PTH_HIP(hipGetLastError());
     }
-    for (int s = 0; s < post; ++s)
+    for (int s = fuse ? 1 : 0; s < post; ++s)
       if (sweep(l) != PTH_COMPLETE) return PTH_FAIL;
   }
   return PTH_COMPLETE;
