@@ -97,7 +97,11 @@ def measured_traffic(workload, kernel):
     except (OSError, ValueError):
         return None
     e = t.get(workload)
-    if not e or e.get("kernel") != kernel:
+    if e and "kernels" in e:
+        e = e["kernels"].get(kernel)
+    elif e and e.get("kernel") != kernel:
+        e = None
+    if not e:
         return None
     return (e["fetch_kib"] * t["fetch_scale"] + e["write_kib"] * t["write_scale"]) * 1024
 
