@@ -130,8 +130,9 @@ VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4  # wave64 VALU instructions/s: 1024 SIMDs, 4
 
 
 def measured_valu(workload, kernel):
-    """Wave-level VALU instructions per launch of `kernel` from the committed SQ_INSTS_VALU
-    counter pass (profiles/traffic.json), or None."""
+    """Wave-level VALU instructions of `kernel` per launch (for the block sweep: per RK stage,
+    summed over the stage's launches) from the committed SQ_INSTS_VALU counter pass
+    (profiles/traffic.json), or None."""
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
             e = json.load(f).get(workload) or {}
@@ -761,7 +762,13 @@ def main():
     dom = max(prof, key=lambda k: prof[k][1])
     launches, total_ms = prof[dom]
     avg_s = total_ms / launches * 1e-3
-    achieved = kb[dom] / avg_s / 1e9 if dom in kb else None
+    # the block sweep covers every block once per RK stage, in one launch or, split on a
+    # refined mesh, in two (the blocks without cross-level channels, then the others): its
+    # algorithmic bytes and instruction count are per stage, over the stage's sweep time
+    lps = {k: (c / (2 * prof_steps) if k == "block_sweep" else 1.0)
+           for k, (c, ms) in prof.items()}
+    stage_s = avg_s * lps[dom]
+    achieved = kb[dom] / stage_s / 1e9 if dom in kb else None
     form = drv.form() if args.driver != "tasks" else "tasks"
     push_bytes = None
     if form == "sweep":
@@ -769,7 +776,7 @@ def main():
         kb.pop("exchange_direct", None)
         push_bytes = 8 * V * same_level_ghost_cells(mesh)
     kernels = {k: {"launches_per_step": c / prof_steps, "avg_us": ms / c * 1e3,
-                   "GBps": (kb[k] / (ms / c * 1e-3) / 1e9) if k in kb else None}
+                   "GBps": (kb[k] / (ms / c * 1e-3 * lps[k]) / 1e9) if k in kb else None}
                for k, (c, ms) in sorted(prof.items())}
     # interior part of the reference task decomposition plus the exchange's buffer traffic
     # counted from the live table (send R+W and set R+W per element, two stages); restriction,
@@ -804,20 +811,22 @@ def main():
                      "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": measured_traffic("burgers-amr" if amr else "burgers-uniform",
                                                  dom),
-                     "algorithmic_bytes_per_launch": kb.get(dom),
+                     "algorithmic_bytes_per_stage": kb.get(dom),
+                     "launches_per_stage": lps[dom],
                      "avg_launch_us": avg_s * 1e6,
+                     "stage_us": stage_s * 1e6,
                      # the block sweep also writes every same-level ghost cell of its output
                      # (the exchange's copies, done in the same launch); counted here apart
-                     "push_bytes_per_launch": push_bytes if dom == "block_sweep" else None,
-                     "frac_with_push": ((kb[dom] + push_bytes) / avg_s / 1e9 / HBM_PEAK_GBS
+                     "push_bytes_per_stage": push_bytes if dom == "block_sweep" else None,
+                     "frac_with_push": ((kb[dom] + push_bytes) / stage_s / 1e9 / HBM_PEAK_GBS
                                         if dom == "block_sweep" and push_bytes else None)},
         # the block sweep is bound by FP64 instruction issue, not by HBM: its share of the
         # VALU issue capacity over the live launch time, from the committed instruction count
         "issue_roofline": ({"bound": "valu",
-                            "valu_insts_per_launch": measured_valu(
+                            "valu_insts_per_stage": measured_valu(
                                 "burgers-amr" if amr else "burgers-uniform", dom),
                             "frac": measured_valu("burgers-amr" if amr else "burgers-uniform",
-                                                  dom) / (VALU_ISSUE_PER_S * avg_s)}
+                                                  dom) / (VALU_ISSUE_PER_S * stage_s)}
                            if measured_valu("burgers-amr" if amr else "burgers-uniform", dom)
                            else None),
         # an effective rate, not achieved bandwidth: the bytes the reference's unfused task

@@ -255,6 +255,13 @@ pth_status pth_burgers_form(pth_mesh *m, int *sweep, int *overlap);
  * forms (when the mesh supports them), so a run's bits cannot depend on timing noise; must
  * precede pth_burgers_init. No counterpart in the reference. */
 pth_status pth_burgers_set_bit_exact(pth_mesh *m, int on);
+/* Sweep form on statically refined meshes of 16^3 blocks with 4 ghosts (on by default; no
+ * counterpart in the reference): each stage sweeps the blocks without a neighbour at another
+ * level or a physical face first, while the cross-level part of the stage input's exchange
+ * (restriction, channels between levels, prolongation, conditions) runs on a second stream,
+ * then the flux-correction send and the sweep of the other blocks; the last cycle's
+ * cross-level exchange runs at the end of pth_burgers_run_cycles. The same bits as on = 0. */
+pth_status pth_burgers_set_sweep_split(pth_mesh *m, int on);
 /* Driver tuning (no counterpart in the reference): planes per workgroup (1, 2 or 4) of the
  * plane-tiled reconstruction for its three forms (all faces, ghost-free faces, the rest), and
  * the block sweep's register budget (the waves per SIMD it is compiled for: 2 or 3 for 16^3

@@ -143,6 +143,7 @@ _SIGNATURES = {
     "pth_burgers_set_time_limits": (_I, [_P, _DP, _I]),
     "pth_burgers_set_overlap": (_I, [_P, _I]),
     "pth_burgers_set_sweep": (_I, [_P, _I]),
+    "pth_burgers_set_sweep_split": (_I, [_P, _I]),
     "pth_burgers_set_bit_exact": (_I, [_P, _I]),
     "pth_burgers_form": (_I, [_P, C.POINTER(_I), C.POINTER(_I)]),
     "pth_burgers_tuning": (_I, [_P, C.POINTER(_I)]),

@@ -86,7 +86,7 @@ class DeviceBurgersDriver:
     (use_graph) one captured hipGraph replayed per cycle."""
 
     def __init__(self, mesh, cfl=0.4, recon=burgers_package.WENO5, use_graph=True,
-                 overlap=None, sweep=None, bit_exact=False):
+                 overlap=None, sweep=None, bit_exact=False, split=None):
         import ctypes as C
         from . import _check, lib
         self._C, self._check, self._lib = C, _check, lib
@@ -107,6 +107,8 @@ class DeviceBurgersDriver:
             _check(lib().pth_burgers_set_overlap(mesh._h, int(overlap)), "set_overlap")
         if sweep is not None:
             _check(lib().pth_burgers_set_sweep(mesh._h, int(sweep)), "set_sweep")
+        if split is not None:  # the split sweep on refined 16^3 meshes (on by default)
+            _check(lib().pth_burgers_set_sweep_split(mesh._h, int(split)), "set_sweep_split")
 
     def initialize(self, pgen="sinusoid", seed=0):
         if pgen == "sinusoid":

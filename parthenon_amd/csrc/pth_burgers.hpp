@@ -49,12 +49,13 @@ bool BurgersSweepSupported(const Topology &t, int V);
 // compiled for (waves per SIMD); spread (1, 2 or 3) divides the resident workgroup count, so
 // each workgroup takes longer plane segments (fewer z-state rebuilds on small meshes); with
 // push (27 same-level neighbours per block, or -1) the output's boundary cells are also
-// written into those neighbours' ghost cells of out
+// written into those neighbours' ghost cells of out; with list (nlist block ids, device
+// memory; 16^3 blocks only) the launch covers those blocks alone
 pth_status BurgersSweep(const Topology &t, int V, const double *U, const double *base,
                         double *out, double *derived, double *partial, int max_partials,
                         const BlockCoefs *coef, double beta, const double *dtvar, int recon,
                         const FluxCorrection *fc, int waves, int spread, const int *push,
-                        int *groups, hipStream_t s);
+                        int *groups, hipStream_t s, const int *list = nullptr, int nlist = 0);
 // flux-correction send of the sweep: fine fluxes from the state, restricted into fc's store
 pth_status FlxcorDirect(const Topology &t, int V, const FluxCorrection &fc, const double *U,
                         int recon, hipStream_t s);

@@ -100,6 +100,7 @@ Mesh::~Mesh() {
   }
   if (arena_base) (void)hipFree(arena_base);
   if (same_level) (void)hipFree(same_level);
+  if (split_list) (void)hipFree(split_list);
   if (geom) (void)hipFree(geom);
   if (coef) (void)hipFree(coef);
   if (face) (void)hipFree(face);

@@ -23,6 +23,8 @@
 // trajectory of "base" is identical to the task-by-task form.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 #include <cfloat>
 #include <cstring>
 #include <vector>
@@ -87,7 +89,40 @@ struct Ctx {
   int fu, fd, V;
   const FluxCorrection *fc = nullptr;  // multilevel: flux-correction patches of U
   const int *push = nullptr;           // sweep: same-level neighbours it writes ghosts of
+  bool split = false;                  // sweep: split stages (Mesh::sweep_split)
 };
+
+// Lists A and B of the split sweep: A holds the blocks whose every neighbour is at their own
+// level and that touch no physical face (no cross-level channel, flux correction or boundary
+// condition writes or reads them), B the others, each in block order.
+pth_status SplitLists(Mesh &m) {
+  if (m.split_gen == m.generation) return PTH_COMPLETE;
+  const Topology &t = m.topo;
+  std::vector<int> a, b;
+  for (int q = 0; q < static_cast<int>(t.blocks.size()); ++q) {
+    const Block &B = t.blocks[q];
+    bool inner = true;
+    for (const Link &l : B.links) inner = inner && l.level == B.at.level;
+    for (int d = 0; d < 3; ++d) {
+      if (!t.active[d] || t.periodic[d]) continue;
+      const double tol = 1e-12 * (t.xmax[d] - t.xmin[d]);
+      inner = inner && std::fabs(B.xmin[d] - t.xmin[d]) > tol &&
+              std::fabs(B.xmax[d] - t.xmax[d]) > tol;
+    }
+    (inner ? a : b).push_back(q);
+  }
+  if (m.split_list) (void)hipFree(m.split_list);
+  m.split_list = nullptr;
+  m.split_nA = static_cast<int>(a.size());
+  m.split_nB = static_cast<int>(b.size());
+  a.insert(a.end(), b.begin(), b.end());
+  if (!a.empty()) {
+    PTH_HIP(hipMalloc(&m.split_list, sizeof(int) * a.size()));
+    PTH_HIP(hipMemcpy(m.split_list, a.data(), sizeof(int) * a.size(), hipMemcpyHostToDevice));
+  }
+  m.split_gen = m.generation;
+  return PTH_COMPLETE;
+}
 
 pth_status Setup(Mesh *m, Ctx &x) {
   if (!m->finalized) {
@@ -143,6 +178,11 @@ pth_status Setup(Mesh *m, Ctx &x) {
         m->SameLevelNeighbors() != PTH_COMPLETE)
       return PTH_FAIL;
     if (ok_base && ok_one && m->pack_mode == PTH_PACK_PERSISTENT) x.push = m->same_level;
+    if (x.push && m->sweep_split && m->topo.multilevel && m->topo.fine.n[0] == 16 &&
+        m->topo.fine.g[0] == 4) {
+      if (SplitLists(*m) != PTH_COMPLETE) return PTH_FAIL;
+      x.split = m->split_nA > 0 && m->split_nB > 0;
+    }
   }
   return PTH_COMPLETE;
 }
@@ -207,6 +247,32 @@ pth_status StageSweep(Ctx &x, int stage, hipStream_t s) {
   const double beta = stage == 1 ? 1.0 : 0.5;
   double *derived = x.fd >= 0 ? x.base->store[x.fd].data : nullptr;
   const double *U = mc0.store[x.fu].data;
+  if (x.split) {
+    // side stream: the cross-level part of the exchange of this stage's input (deferred from
+    // the end of the previous stage), beside the sweep of list A, whose blocks it neither
+    // reads nor writes; then flux correction and the sweep of list B, which need it
+    PTH_HIP(hipEventRecord(m.fork_ev[0], s));
+    PTH_HIP(hipStreamWaitEvent(m.side_stream, m.fork_ev[0], 0));
+    if (Exchange(m, mc0, m.side_stream, true) != PTH_COMPLETE) return PTH_FAIL;
+    PTH_HIP(hipEventRecord(m.fork_ev[1], m.side_stream));
+    int ga = 0, gb = 0;
+    double *part = stage == 2 ? m.scratch : nullptr;
+    if (BurgersSweep(m.topo, x.V, U, x.base->store[x.fu].data, mc1.store[x.fu].data, derived,
+                     part, static_cast<int>(m.scratch_n) / 2, m.coef, beta, m.dtvar,
+                     m.run_recon, x.fc, m.sweep_waves, m.sweep_spread, x.push, &ga, s,
+                     m.split_list, m.split_nA) != PTH_COMPLETE)
+      return PTH_FAIL;
+    PTH_HIP(hipStreamWaitEvent(s, m.fork_ev[1], 0));
+    if (x.fc && FlxcorDirect(m.topo, x.V, *x.fc, U, m.run_recon, s) != PTH_COMPLETE)
+      return PTH_FAIL;
+    if (BurgersSweep(m.topo, x.V, U, x.base->store[x.fu].data, mc1.store[x.fu].data, derived,
+                     part ? part + ga : nullptr, static_cast<int>(m.scratch_n) / 2, m.coef,
+                     beta, m.dtvar, m.run_recon, x.fc, m.sweep_waves, m.sweep_spread, x.push,
+                     &gb, s, m.split_list + m.split_nA, m.split_nB) != PTH_COMPLETE)
+      return PTH_FAIL;
+    if (stage == 2) return BurgersClock(m.scratch, ga + gb, m.run_cfl, m.dtvar, 1, m.run_lim, s);
+    return PTH_COMPLETE;
+  }
   if (x.fc && FlxcorDirect(m.topo, x.V, *x.fc, U, m.run_recon, s) != PTH_COMPLETE)
     return PTH_FAIL;
   int groups = 0;
@@ -620,6 +686,9 @@ pth_status pth_burgers_run_cycles(pth_mesh *mh, int ncycles, int use_graph) {
   if (RunCycles(*m, x, s, ncycles, use_graph != 0) != PTH_COMPLETE) return PTH_FAIL;
   // with the exchange at the start of each stage, the last output's ghosts are still stale
   if (ncycles > 0 && Overlapped(x) && Exchange(*m, *x.base, s) != PTH_COMPLETE) return PTH_FAIL;
+  // the split sweep defers the cross-level part of each stage's exchange to the next stage
+  if (ncycles > 0 && Swept(x) && x.split && Exchange(*m, *x.base, s, true) != PTH_COMPLETE)
+    return PTH_FAIL;
   PTH_HIP(hipEventRecord(m->order_ev[1], s));
   PTH_HIP(hipStreamWaitEvent(m->stream, m->order_ev[1], 0));
   return PTH_COMPLETE;
@@ -629,6 +698,16 @@ pth_status pth_burgers_set_sweep(pth_mesh *mh, int on) {
   Mesh *m = Impl(mh);
   m->run_sweep = on != 0;
   m->form_auto = false;
+  if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
+  if (m->graph) (void)hipGraphDestroy(m->graph);
+  m->graph_exec = nullptr;
+  m->graph = nullptr;
+  return PTH_COMPLETE;
+}
+
+pth_status pth_burgers_set_sweep_split(pth_mesh *mh, int on) {
+  Mesh *m = Impl(mh);
+  m->sweep_split = on != 0;
   if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
   if (m->graph) (void)hipGraphDestroy(m->graph);
   m->graph_exec = nullptr;

@@ -386,6 +386,12 @@ class Mesh {
   bool form_exact = false;                      // driver: timing picks among bit-exact forms only
   bool form_tuned = false;
   int sweep_waves = 4;                          // driver: sweep register budget (waves/SIMD)
+  // driver, sweep form on multilevel 16^3 meshes: each stage's sweep split into the blocks
+  // without channels to other levels or physical faces (list A) and the others (list B), the
+  // previous stage's cross-level exchange running beside sweep A
+  int sweep_split = 1;
+  int *split_list = nullptr;  // device: list A, then list B
+  int split_nA = 0, split_nB = 0, split_gen = -1;
   int sweep_spread = 1;                         // driver: resident sweep workgroups / spread
   bool rf_tuned = false;
   bool rf_auto = true;                          // driver: tuning chosen by timing (not set)

@@ -484,7 +484,8 @@ __global__ void __launch_bounds__(256, MINW)
                        double *__restrict__ derived, double *__restrict__ partial,
                        const BlockCoefs *__restrict__ coef, int nblocks, double beta,
                        const double *__restrict__ dtvar, const int *__restrict__ fover,
-                       const double *__restrict__ fcor, const int *__restrict__ push) {
+                       const double *__restrict__ fcor, const int *__restrict__ push,
+                       const int *__restrict__ list) {
   constexpr int N = 16, NG = 4, E = N + 2 * NG, NT = N * N;
   constexpr int H = 3;             // halo of the staged plane
   constexpr int PW = N + 2 * H;    // staged rows / columns
@@ -579,7 +580,8 @@ __global__ void __launch_bounds__(256, MINW)
     m = DMin(m, 1.0 / sp);
   };
   while (pnext < pend) {
-    const int b = static_cast<int>(pnext / N);
+    // the planes of the launch run over its blocks in order: all blocks, or the listed ones
+    const int b = list ? list[pnext / N] : static_cast<int>(pnext / N);
     const int k0 = static_cast<int>(pnext % N);
     const int k1 = static_cast<int>(pend - pnext < N - k0 ? k0 + (pend - pnext) : N);
     pnext += k1 - k0;
@@ -832,14 +834,18 @@ pth_status BurgersSweep(const Topology &t, int V, const double *U, const double 
                         double *out, double *derived, double *partial, int max_partials,
                         const BlockCoefs *coef, double beta, const double *dtvar, int recon,
                         const FluxCorrection *fc, int waves, int spread, const int *push,
-                        int *groups, hipStream_t s) {
+                        int *groups, hipStream_t s, const int *list, int nlist) {
   if (!BurgersSweepSupported(t, V) || waves < 2 || waves > 4 || spread < 1 || spread > 3) {
     SetError("block sweep: unsupported block shape or component count");
     return PTH_FAIL;
   }
-  ProfScope prof("block_sweep", s);
-  const int nb = static_cast<int>(t.blocks.size());
   const int N = t.fine.n[0];
+  if (list && !(N == 16 && t.fine.g[0] == 4 && nlist > 0)) {
+    SetError("block sweep: block lists need 16^3 blocks with 4 ghosts");
+    return PTH_FAIL;
+  }
+  ProfScope prof("block_sweep", s);
+  const int nb = list ? nlist : static_cast<int>(t.blocks.size());
   if (N == 16 && t.fine.g[0] == 4) {
     // register-ring kernel, compiled for 2 or 3 waves per SIMD
     const int minw = waves >= 3 ? 3 : 2;
@@ -858,7 +864,8 @@ pth_status BurgersSweep(const Topology &t, int V, const double *U, const double 
                                                               max_partials),                   \
                                                      static_cast<std::int64_t>(nb) * 16));     \
     hipLaunchKernelGGL((BlockSweep16Kernel<NV, W, RC>), dim3(grid16), dim3(256), 0, s, U, base, \
-                       out, derived, partial, coef, nb, beta, dtvar, fover16, fcor16, push);   \
+                       out, derived, partial, coef, nb, beta, dtvar, fover16, fcor16, push,   \
+                       list);                                                                  \
   }
 #define PTH_S16_W(NV, W)                                                                       \
   if (recon == 0)                                                                              \
