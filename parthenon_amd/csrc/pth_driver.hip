@@ -249,11 +249,15 @@ pth_status StageSweep(Ctx &x, int stage, hipStream_t s) {
   const double *U = mc0.store[x.fu].data;
   if (x.split) {
     // side stream: the cross-level part of the exchange of this stage's input (deferred from
-    // the end of the previous stage), beside the sweep of list A, whose blocks it neither
-    // reads nor writes; then flux correction and the sweep of list B, which need it
+    // the end of the previous stage) and the flux-correction send, beside the sweep of list
+    // A, whose blocks they neither read nor write; then the sweep of list B, which needs them
     PTH_HIP(hipEventRecord(m.fork_ev[0], s));
     PTH_HIP(hipStreamWaitEvent(m.side_stream, m.fork_ev[0], 0));
     if (Exchange(m, mc0, m.side_stream, true) != PTH_COMPLETE) return PTH_FAIL;
+    // the flux-correction send reads the interface blocks' input with its ghosts and writes
+    // the correction store, which only sweep B reads
+    if (x.fc && FlxcorDirect(m.topo, x.V, *x.fc, U, m.run_recon, m.side_stream) != PTH_COMPLETE)
+      return PTH_FAIL;
     PTH_HIP(hipEventRecord(m.fork_ev[1], m.side_stream));
     int ga = 0, gb = 0;
     double *part = stage == 2 ? m.scratch : nullptr;
@@ -263,8 +267,6 @@ pth_status StageSweep(Ctx &x, int stage, hipStream_t s) {
                      m.split_list, m.split_nA) != PTH_COMPLETE)
       return PTH_FAIL;
     PTH_HIP(hipStreamWaitEvent(s, m.fork_ev[1], 0));
-    if (x.fc && FlxcorDirect(m.topo, x.V, *x.fc, U, m.run_recon, s) != PTH_COMPLETE)
-      return PTH_FAIL;
     if (BurgersSweep(m.topo, x.V, U, x.base->store[x.fu].data, mc1.store[x.fu].data, derived,
                      part ? part + ga : nullptr, static_cast<int>(m.scratch_n) / 2, m.coef,
                      beta, m.dtvar, m.run_recon, x.fc, m.sweep_waves, m.sweep_spread, x.push,
