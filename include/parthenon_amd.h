@@ -450,6 +450,12 @@ pth_status pth_mg_result(const pth_mg *mg, int *iterations, double *rms_residual
 int pth_mg_levels(const pth_mg *mg);
 /* RMS residual before every V-cycle and after the last; returns the count */
 int pth_mg_history(const pth_mg *mg, double *rms, int max);
+/* Launch form of the block-mesh hierarchy (meshes other than the uniform cube; no counterpart
+ * in the reference, whose V-cycle is a task list run on the host): on (the default) the first
+ * V-cycle of the solver runs eagerly and every later one replays one captured hipGraph of
+ * the same launches, with the same results bit for bit; off: every V-cycle eagerly. The
+ * whole-level solver of the uniform cube is unaffected. */
+pth_status pth_mg_set_graph(pth_mg *mg, int on);
 
 /* Serving loop of independent Poisson solves (no counterpart in the reference, which holds
  * one Mesh per process, src/parthenon_manager.cpp:170-173). Requests arrive at given times;

@@ -186,6 +186,7 @@ _SIGNATURES = {
     "pth_mg_result": (_I, [_P, C.POINTER(_I), _DP]),
     "pth_mg_history": (_I, [_P, _DP, _I]),
     "pth_mg_levels": (_I, [_P]),
+    "pth_mg_set_graph": (_I, [_P, _I]),
     "pth_gmg_server_create": (_P, [_I, _D, _I, _I, _D, _I, _I64, _I, _I]),
     "pth_gmg_server_destroy": (None, [_P]),
     "pth_gmg_server_submit": (_I, [_P, _I, C.c_uint64, _D, C.POINTER(_I)]),

@@ -306,12 +306,25 @@ class MgTree {
   double D = 1.0, alpha = 0.0;
   std::vector<std::unique_ptr<TreeLevel>> lv;  // finest (a copy of the user's mesh) first
   std::vector<double> w_pre, w_post;           // Jacobi weight of each smoothing stage
+  // a V-cycle is ~10 launches per level, each a few microseconds of work on the coarse
+  // levels: after one eager V-cycle (which builds every level's exchange tables) the next is
+  // captured and later V-cycles replay it
+  bool use_graph = true, warmed = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  hipStream_t capture_stream = nullptr;
 
+  ~MgTree() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (capture_stream) (void)hipStreamDestroy(capture_stream);
+  }
   pth_status Build();
   pth_status Setup(MeshData &md);
   pth_status Solve(MeshData &md, int *iters, std::vector<double> *history);
 
  private:
+  pth_status VCycle();
   pth_status NewLevel(const Topology &t, bool coarse_buffers, int x_prolong);
   pth_status BuildTransfers(int l);
   pth_status Exchange(int l, int field);
@@ -586,6 +599,45 @@ pth_status MgTree::Cycle(int l) {
   return Smooth(l, w_post);
 }
 
+pth_status MgTree::VCycle() {
+  if (!use_graph) return Cycle(0);
+  if (!warmed) {
+    warmed = true;
+    return Cycle(0);
+  }
+  hipStream_t s = lv[0]->mesh->stream;
+  if (!graph_exec) {
+    // the caller's stream may be the null stream, which cannot be captured: the levels launch
+    // on a stream of the solver's own while the V-cycle is captured
+    if (!capture_stream)
+      PTH_HIP(hipStreamCreateWithFlags(&capture_stream, hipStreamNonBlocking));
+    for (auto &L : lv) L->mesh->stream = capture_stream;
+    const bool prof = Prof().on;
+    Prof().on = false;  // no event records inside the captured V-cycle
+    hipError_t e = hipStreamBeginCapture(capture_stream, hipStreamCaptureModeThreadLocal);
+    const bool began = e == hipSuccess;
+    pth_status st = PTH_FAIL;
+    hipGraph_t g = nullptr;
+    if (began) {
+      st = Cycle(0);
+      e = hipStreamEndCapture(capture_stream, &g);
+    }
+    Prof().on = prof;
+    for (auto &L : lv) L->mesh->stream = s;
+    if (e != hipSuccess && (!began || st == PTH_COMPLETE))
+      SetError(std::string("MGSolver: V-cycle capture: ") + hipGetErrorString(e));
+    if (st != PTH_COMPLETE || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return PTH_FAIL;
+    }
+    graph = g;
+    PTH_HIP(hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0));
+  }
+  ProfScope prof("mg_vcycle_graph", s);
+  PTH_HIP(hipGraphLaunch(graph_exec, s));
+  return PTH_COMPLETE;
+}
+
 pth_status MgTree::Rms(double *rms) {
   if (Exchange(0, kX) != PTH_COMPLETE) return PTH_FAIL;
   if (Cells(0, 1, kX, kR, 0.0) != PTH_COMPLETE) return PTH_FAIL;
@@ -640,7 +692,7 @@ pth_status MgTree::Solve(MeshData &md, int *iters, std::vector<double> *history)
   history->push_back(rms);
   int it = 0;
   do {
-    if (Cycle(0) != PTH_COMPLETE) return PTH_FAIL;
+    if (VCycle() != PTH_COMPLETE) return PTH_FAIL;
     ++it;
     if (Rms(&rms) != PTH_COMPLETE) return PTH_FAIL;
     history->push_back(rms);
@@ -669,6 +721,8 @@ MgTree *MgTreeCreate(Mesh *m, int fu, int frhs, int fD, const pth_mg_params &p, 
 void MgTreeDestroy(MgTree *t) { delete t; }
 
 int MgTreeLevels(const MgTree *t) { return static_cast<int>(t->lv.size()); }
+
+void MgTreeSetGraph(MgTree *t, bool on) { t->use_graph = on; }
 
 pth_status MgTreeSetup(MgTree *t, MeshData &md) { return t->Setup(md); }
 

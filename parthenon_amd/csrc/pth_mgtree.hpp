@@ -15,6 +15,8 @@ MgTree *MgTreeCreate(Mesh *m, int fu, int frhs, int fD, const pth_mg_params &p, 
                      double alpha);
 void MgTreeDestroy(MgTree *t);
 int MgTreeLevels(const MgTree *t);
+// on (the default): V-cycles after the first of a solver replay one captured hipGraph
+void MgTreeSetGraph(MgTree *t, bool on);
 // copies u, rhs (and the face field D, restricted down the levels) of md into the hierarchy
 pth_status MgTreeSetup(MgTree *t, MeshData &md);
 // V-cycles until the RMS residual is below the tolerance (at least one, at most max_iters);

@@ -613,6 +613,15 @@ int pth_mg_levels(const pth_mg *h) {
   return h->f ? static_cast<int>(h->f->p.sizes.size()) : static_cast<int>(h->d->p.sizes.size());
 }
 
+pth_status pth_mg_set_graph(pth_mg *h, int on) {
+  if (!h) {
+    SetError("MGSolver: null solver");
+    return PTH_FAIL;
+  }
+  if (h->tree) MgTreeSetGraph(h->tree, on != 0);
+  return PTH_COMPLETE;
+}
+
 int pth_mg_history(const pth_mg *h, double *out, int max) {
   if (!h) return 0;
   const int n = std::min(static_cast<int>(h->history.size()), std::max(max, 0));

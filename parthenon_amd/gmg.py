@@ -237,3 +237,8 @@ class MeshMGSolver:
     def levels(self):
         """Levels of the hierarchy, finest first."""
         return lib().pth_mg_levels(self._h)
+
+    def set_graph(self, on):
+        """Block-mesh hierarchy: replay V-cycles from one captured hipGraph (on, the default)
+        or launch every V-cycle eagerly (off); same results bit for bit."""
+        _check(lib().pth_mg_set_graph(self._h, int(on)), "set_graph")
