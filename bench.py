@@ -667,7 +667,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload",
                     choices=["burgers-amr", "burgers-uniform", "burgers-adaptive", "exchange",
-                             "copy", "gmg", "gmg-serve"],
+                             "copy", "gmg", "gmg-serve", "gmg-refined"],
                     default="burgers-amr")
     ap.add_argument("--pack", choices=["persistent", "per_buffer", "fused"], default="persistent")
     ap.add_argument("--steps", type=int, default=200)
@@ -707,6 +707,9 @@ def main():
         if "--warmup" not in sys.argv:
             args.warmup = 1
         return gmg_bench(args)
+    if args.workload == "gmg-refined":
+        print(json.dumps(gmg_refined_bench()), flush=True)
+        return
     if args.workload == "gmg-serve":
         if "--steps" not in sys.argv:
             args.steps = 2000
